@@ -1,0 +1,276 @@
+"""Pins the CPU oracle against the reference's own known-answer tests and
+against what the mathematics fixes (brute force, closed forms)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+
+def test_mt19937_standard_draw():
+    # mt19937.rs:95-103: the 10000th draw of mt19937(5489) is 4123659995
+    assert O.lib().orc_mt19937_nth(5489, 10000) == 4123659995
+
+
+def test_lcg_sample_properties():
+    # random.rs:77-95
+    out = np.zeros(10, np.int32)
+    assert O.lib().orc_random_sample(1, 5, 5, np.zeros(5, np.int32)) == 5
+    for n, k in [(1000, 10), (1000, 900), (50000, 20000)]:
+        buf = np.zeros(k, np.int32)
+        m = O.lib().orc_random_sample(7, n, k, buf)
+        s = buf[:m]
+        assert np.all(np.diff(s) > 0) and s.min() >= 0 and s.max() < n
+        if k <= 10:
+            assert m == k
+
+
+def test_to_i8_casts_like_x86():
+    # quant.rs:166-172
+    assert O.lib().orc_to_i8(2.7) == 2
+    assert O.lib().orc_to_i8(-2.7) == -2
+    assert O.lib().orc_to_i8(200.0) == -56
+    assert O.lib().orc_to_i8(float("nan")) == 0
+    assert O.lib().orc_to_i8(1e300) == 0
+
+
+def test_discretize_packing_known_answer():
+    # quant.rs:175-184: bins 4, seed 0, no stochastic rounding
+    gq, hq, gs, hs = O.discretize(np.array([1.0, -0.5], np.float32), np.array([0.5, 0.25], np.float32), 4, 0, False)
+    assert list(hq) == [4, 2] and list(gq) == [2, -1]
+    assert (gs, hs) == (0.5, 0.125)
+    packed = [(int(g) << 32) | (int(h) & 0xFF) for g, h in zip(gq, hq)]
+    assert packed[1] == (-1 << 32) | 2
+    assert sum(packed) == (1 << 32) | 6
+    sg = sum(float(g) * gs for g in gq)
+    sh = sum(float(h) * hs for h in hq)
+    assert (sg, sh) == (0.5, 0.75)
+
+
+def test_binning_midpoints():
+    # binning.rs:704-718
+    vals = np.repeat(np.arange(1, 11, dtype=np.float64), 5)
+    m = O.find_bin(vals, len(vals), max_bin=255, min_data_in_bin=3)
+    kz = float(np.float32(1e-35))
+    assert m["missing_type"] == 0
+    assert m["upper"][0] == kz
+    assert m["upper"][1] == np.nextafter(1.5, np.inf)
+    assert m["upper"][-1] == np.inf
+    vb = lambda v: O.lib().orc_value_to_bin_num(m["upper"], m["num_bin"], m["missing_type"], v)
+    assert vb(1.0) == 1 and vb(10.0) == m["num_bin"] - 1 and vb(-5.0) == 0
+
+
+def test_binning_nan_last_bin():
+    # binning.rs:720-729
+    vals = np.concatenate([np.arange(1, 51, dtype=np.float64), [np.nan, np.nan]])
+    m = O.find_bin(vals, len(vals), max_bin=255, min_data_in_bin=3)
+    assert m["missing_type"] == 2
+    assert math.isnan(m["upper"][-1])
+    assert O.lib().orc_value_to_bin_num(m["upper"], m["num_bin"], 2, float("nan")) == m["num_bin"] - 1
+
+
+def test_binning_greedy_respects_max_bin():
+    # binning.rs:731-739
+    vals = np.sin(np.arange(10000) * 0.37) + 2.0
+    m = O.find_bin(vals, len(vals), max_bin=16, min_data_in_bin=3)
+    assert m["num_bin"] <= 16
+    assert np.all(np.diff(m["upper"]) > 0)
+
+
+def test_binning_categorical_order():
+    # binning.rs:741-763: category 3 x30, 1 x20, 2 x10, -1 x5, zeros x15
+    vals = np.concatenate([np.full(30, 3.0), np.full(20, 1.0), np.full(10, 2.7), np.full(5, -1.0)])
+    m = O.find_bin(vals, len(vals) + 15, bin_type=1, max_bin=255, min_data_in_bin=3)
+    assert list(m["bin_2_categorical"]) == [-1, 3, 1, 0, 2]
+    assert m["missing_type"] == 2
+    assert m["default_bin"] == 3
+
+
+def test_binning_constant_is_trivial():
+    m = O.find_bin(np.zeros(0), 100, max_bin=255, min_data_in_bin=3)
+    assert m["is_trivial"]
+
+
+def test_value_to_bin_matches_bruteforce():
+    rng = np.random.default_rng(5)
+    vals = rng.normal(size=3000)
+    m = O.find_bin(vals, len(vals), max_bin=63, min_data_in_bin=3)
+    up = m["upper"]
+    for v in rng.normal(size=500):
+        b = O.lib().orc_value_to_bin_num(up, m["num_bin"], m["missing_type"], float(v))
+        # the first bin whose upper bound is >= v
+        assert b == int(np.argmax(up >= v))
+
+
+P1 = dict(lambda_l1=0.0, lambda_l2=0.0, max_delta_step=0.0, path_smooth=0.0, min_data_in_leaf=1,
+          min_sum_hessian_in_leaf=1e-3, min_gain_to_split=0.0, max_cat_to_onehot=4, max_cat_threshold=32,
+          cat_l2=10.0, cat_smooth=10.0, min_data_per_group=100)
+
+
+def test_finds_obvious_split():
+    # split.rs:1423-1443
+    hist = np.array([[-5.0, 5.0], [-5.0, 5.0], [5.0, 5.0], [5.0, 5.0]])
+    r = O.find_best_threshold(hist, 4, 0, 0, 0, 0, P1, 0.0, 20.0, 20)
+    assert r["splittable"] and r["threshold"] == 1 and r["left_count"] == 10
+    assert abs(r["left_output"] - 1.0) < 1e-12 and abs(r["right_output"] + 1.0) < 1e-12
+    assert abs(r["gain"] - 20.0) < 1e-9
+
+
+def test_categorical_one_hot_and_sorted():
+    # split.rs:1445-1470
+    hist = np.array([[0.0, 10.0], [-10.0, 10.0], [10.0, 10.0]])
+    r = O.find_best_threshold(hist, 3, 2, 1, 1, 1, P1, 0.0, 30.0, 30)
+    assert r["splittable"] and not r["default_left"] and r["left_count"] == 10
+
+
+def _bruteforce_best(hist, sg, sh, nd, p):
+    """All thresholds of a numerical feature without missing values (mfb = 0
+    so offset = 1 and bin 0 is the remainder), scored by G^2/H."""
+    eps = float(np.float32(1e-15))
+    shp = sh + 2 * eps
+    cf = nd / shp
+    best = (-np.inf, None)
+    B = len(hist)
+    for thr in range(0, B - 1):
+        rg = hist[thr + 1:, 0].sum()
+        rh = eps + hist[thr + 1:, 1].sum()
+        rc = sum(int(h * cf + 0.5) for h in hist[thr + 1:, 1])
+        lc = nd - rc
+        lh = shp - rh
+        lg = sg - rg
+        if rc < p["min_data_in_leaf"] or lc < p["min_data_in_leaf"]:
+            continue
+        gain = lg * lg / lh + rg * rg / rh - sg * sg / shp
+        if gain > best[0] + 1e-9:
+            best = (gain, thr)
+    return best
+
+
+def test_threshold_search_matches_bruteforce():
+    rng = np.random.default_rng(11)
+    p = dict(P1, min_data_in_leaf=3)
+    for _ in range(30):
+        B = int(rng.integers(3, 20))
+        cnt = rng.integers(1, 30, size=B).astype(np.float64)
+        g = rng.normal(size=B) * cnt
+        hist = np.stack([g, cnt], axis=1)
+        hist[0] = 0.0  # bin 0 is the most frequent bin (not stored)
+        tot_g = g.sum()
+        tot_h = cnt.sum()
+        nd = int(cnt.sum())
+        gain, thr = _bruteforce_best(np.stack([np.r_[tot_g - g[1:].sum(), g[1:]], cnt], 1), tot_g, tot_h, nd, p)
+        r = O.find_best_threshold(hist, B, 0, 0, 0, 0, p, tot_g, tot_h, nd)
+        if thr is None:
+            assert not r["splittable"] or r["gain"] <= 1e-9
+        else:
+            assert r["threshold"] == thr
+            assert abs(r["gain"] - gain) <= 1e-6 * max(1.0, abs(gain))
+
+
+def test_tie_break_prefers_smaller_feature_in_training():
+    # two identical columns: the split must use the smaller column index
+    rng = np.random.default_rng(1)
+    n = 500
+    x = rng.normal(size=n).astype(np.float32)
+    X = np.stack([x, x], axis=1)
+    y = (x > 0.1).astype(np.float32)
+    ds = O.Dataset(X, y, dict(max_bin=63, min_data_in_leaf=5))
+    b = O.Booster(ds, dict(objective="binary", num_leaves=4, min_data_in_leaf=5, max_bin=63))
+    b.update()
+    t = b.tree(0)
+    assert set(t.split_feature.tolist()) == {0}
+
+
+def _step_data(n):
+    a = np.arange(n) / n
+    X = np.stack([a, ((np.arange(n) * 37) % 11).astype(np.float64)], axis=1).astype(np.float32)
+    y = np.where(a < 0.5, -1.0, 1.0).astype(np.float32)
+    return X, y
+
+
+def test_learner_first_split_separates_step():
+    # learner/tests.rs:24-43
+    n = 200
+    X, y = _step_data(n)
+    ds = O.Dataset(X, y, dict(min_data_in_leaf=5))
+    b = O.Booster(ds, dict(objective="binary", num_leaves=2, min_data_in_leaf=5))
+    b.train_tree(-y, np.ones(n, np.float32))
+    t = b.tree(0)
+    assert t.num_leaves == 2 and t.split_feature[0] == 0
+    assert list(t.leaf_count) == [100, 100]
+    assert abs(t.leaf_value[0] + 1.0) < 1e-9 and abs(t.leaf_value[1] - 1.0) < 1e-9
+
+
+def test_learner_partition_stable_and_max_depth():
+    # learner/tests.rs:45-78
+    n = 300
+    X, y = _step_data(n)
+    ds = O.Dataset(X, y, dict(min_data_in_leaf=40))
+    b = O.Booster(ds, dict(objective="binary", num_leaves=31, max_depth=2, min_data_in_leaf=40))
+    b.train_tree(np.sin(np.arange(n, dtype=np.float32) * 0.37).astype(np.float32), np.ones(n, np.float32))
+    t = b.tree(0)
+    assert t.num_leaves <= 4
+    assert np.all(t.leaf_count >= 40)
+
+
+def test_learner_constant_gradient_single_leaf():
+    # learner/tests.rs:80-89
+    n = 100
+    X, y = _step_data(n)
+    ds = O.Dataset(X, y, {})
+    b = O.Booster(ds, dict(objective="binary"))
+    assert b.train_tree(np.full(n, 0.5, np.float32), np.ones(n, np.float32)) == 1
+
+
+def test_binary_gradients_closed_form():
+    rng = np.random.default_rng(2)
+    s = rng.normal(size=1000)
+    y = (rng.random(1000) > 0.5).astype(np.float32)
+    g, h = O.binary_gradients(s, y)
+    lab = np.where(y > 0, 1.0, -1.0)
+    resp = -lab / (1.0 + np.exp(lab * s))
+    np.testing.assert_array_equal(g, resp.astype(np.float32))
+    np.testing.assert_array_equal(h, (np.abs(resp) * (1.0 - np.abs(resp))).astype(np.float32))
+
+
+def test_multiclass_gradients_closed_form():
+    rng = np.random.default_rng(3)
+    K, n = 4, 500
+    s = rng.normal(size=K * n)
+    y = rng.integers(0, K, size=n).astype(np.float32)
+    g, h = O.multiclass_gradients(s, y, K)
+    S = s.reshape(K, n).T
+    P = np.exp(S - S.max(1, keepdims=True))
+    P = P / P.sum(1, keepdims=True)
+    onehot = np.eye(K)[y.astype(int)]
+    np.testing.assert_allclose(g.reshape(K, n).T, P - onehot, atol=1e-6)
+    np.testing.assert_allclose(h.reshape(K, n).T, K / (K - 1.0) * P * (1 - P), atol=1e-6)
+
+
+def test_auc_matches_sklearn():
+    from sklearn.metrics import roc_auc_score
+    rng = np.random.default_rng(4)
+    y = (rng.random(2000) > 0.6).astype(np.float32)
+    s = rng.normal(size=2000) + y
+    assert abs(O.auc(s, y) - roc_auc_score(y, s)) < 1e-12
+    s2 = np.round(s, 1)  # ties
+    assert abs(O.auc(s2, y) - roc_auc_score(y, s2)) < 1e-12
+
+
+def test_training_scores_equal_raw_prediction():
+    # end_to_end.rs:53-57
+    rng = np.random.default_rng(9)
+    n, p = 2000, 5
+    X = rng.uniform(-2, 2, size=(n, p)).astype(np.float32)
+    X[::7, 2] = np.nan
+    X[::3, 3] = 0.0
+    y = (X[:, 0] - 0.5 * X[:, 1] ** 2 + np.nan_to_num(X[:, 2], nan=1.0) > 0).astype(np.float32)
+    ds = O.Dataset(X, y, dict(max_bin=255))
+    b = O.Booster(ds, dict(objective="binary", num_leaves=15, learning_rate=0.1))
+    for _ in range(20):
+        b.update()
+    raw = b.predict_raw(X)[:, 0]
+    s = b.scores()
+    assert np.all(np.abs(raw - s) <= 1e-9 * (1 + np.abs(s)))
+    assert O.binary_logloss(s, y) < 0.4
