@@ -1,0 +1,191 @@
+/*
+ * lgbm_hip.h — C ABI of the MI355X GBDT engine (liblgbm_hip.so).
+ *
+ * The drop-in boundary for lightgbm-rust's training/prediction hot path.
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, borrows
+ * caller memory only for the duration of the call, catches every C++
+ * exception and GPU error, and returns an lgbmhip_status. A context owns its
+ * device memory and one HIP stream; it is not thread-safe (one caller at a
+ * time, as `&mut Gbdt` in the reference).
+ *
+ * Reference interfaces each entry point stands in for (paths relative to
+ * crates/lgbm-core/src/ of zrluety/lightgbm-rust):
+ *   lgbmhip_status codes      <- LgbmError variants            error.rs:6-18
+ *   lgbmhip_bin_dense         <- Dataset::from_matrix (binning) dataset.rs:360-422
+ *   lgbmhip_upload_bins       <- SerialTreeLearner::new         learner/mod.rs:220-310
+ *   lgbmhip_set_metadata      <- Objective::init / Metadata     objective/mod.rs:43-102, dataset.rs:104-130
+ *   lgbmhip_set_split_params  <- SplitParams                    learner/split.rs:33-46
+ *   lgbmhip_set_train_opts    <- Gbdt::new                      boosting.rs:206-337
+ *   lgbmhip_train_iteration   <- Gbdt::train_one_iter           boosting.rs:957-1139
+ *   lgbmhip_train_tree        <- SerialTreeLearner::train       learner/mod.rs:572-695
+ *   lgbmhip_tree_*            <- Tree arrays / SplitArgs        tree.rs:120-178, 254-297
+ *   lgbmhip_copy_scores       <- Gbdt::train_scores             boosting.rs:339-341
+ *   lgbmhip_gradients         <- RowObjective::gradients        objective/mod.rs:52, binary.rs:101-127,
+ *                                                               multiclass.rs:126-154, rank.rs:63-108
+ *   lgbmhip_discretize        <- GradientDiscretizer::discretize learner/quant.rs:92-129
+ *   lgbmhip_build_histogram*  <- MultiValBin::construct(_int)   multi_val_bin.rs:556-611, 710-753
+ *   lgbmhip_find_best_split*  <- find_best_threshold(_int)      learner/split.rs:246-271, 930-950
+ *   lgbmhip_partition         <- DataPartition::split           learner/partition.rs:107-174
+ *   lgbmhip_predict_raw       <- Gbdt::predict_matrix_early_stop predict.rs:231-339
+ *   lgbmhip_predict_binned    <- Tree::add_prediction_to_score  tree.rs:605-652
+ */
+#ifndef LGBM_HIP_H_
+#define LGBM_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t lgbmhip_status;
+#define LGBMHIP_OK 0
+#define LGBMHIP_INVALID_PARAMETER 1
+#define LGBMHIP_INVALID_DATA 2
+#define LGBMHIP_MODEL_FORMAT 3
+#define LGBMHIP_UNSUPPORTED 4
+#define LGBMHIP_INTERNAL 5
+
+typedef struct lgbmhip_ctx lgbmhip_ctx;
+typedef struct lgbmhip_bins lgbmhip_bins;
+
+/* Parameters of host binning (Config fields read by dataset.rs:860-910). */
+typedef struct {
+  int32_t max_bin;
+  int32_t min_data_in_bin;
+  int32_t min_data_in_leaf;
+  int32_t bin_construct_sample_cnt;
+  int32_t data_random_seed;
+  int32_t feature_pre_filter;
+  int32_t use_missing;
+  int32_t zero_as_missing;
+  int32_t num_threads; /* host threads for binning; <= 0: all */
+} lgbmhip_bin_params;
+
+/* One used feature as the learner sees it (split.rs:18-29 FeatureMeta). */
+typedef struct {
+  int32_t real_index;    /* column of the raw matrix */
+  int32_t num_bin;
+  int32_t missing_type;  /* 0 none, 1 zero, 2 NaN */
+  int32_t default_bin;
+  int32_t most_freq_bin;
+  int32_t bin_type;      /* 0 numerical, 1 categorical */
+} lgbmhip_feature_meta;
+
+/* split.rs:33-46 plus the tree-shape limits of the learner. */
+typedef struct {
+  double lambda_l1, lambda_l2, max_delta_step, path_smooth;
+  double min_sum_hessian_in_leaf, min_gain_to_split, cat_l2, cat_smooth;
+  int32_t min_data_in_leaf, max_cat_to_onehot, max_cat_threshold, min_data_per_group;
+  int32_t num_leaves, max_depth;
+} lgbmhip_split_params;
+
+/* Boosting options (the Config fields Gbdt::new reads for this path). */
+typedef struct {
+  int32_t objective;            /* 0 binary, 1 multiclass (softmax), 2 lambdarank */
+  int32_t num_class;
+  double sigmoid;
+  double learning_rate;
+  int32_t boost_from_average;
+  int32_t use_quantized_grad;
+  int32_t num_grad_quant_bins;
+  int32_t stochastic_rounding;
+  int32_t quant_train_renew_leaf;
+  int32_t seed;                 /* Config::seed (quantizer generators) */
+  int32_t quant_table_threads;  /* thread count the random tables are drawn for (reference num_threads) */
+  int32_t lambdarank_truncation_level;
+  int32_t lambdarank_norm;
+  int32_t use_graph;            /* capture each tree's launches in a hipGraph */
+} lgbmhip_train_opts;
+
+const char* lgbmhip_last_error(const lgbmhip_ctx* ctx);
+const char* lgbmhip_global_last_error(void);
+
+/* ---- host binning (BinMapper stays on the host) ---- */
+lgbmhip_status lgbmhip_bin_dense(const float* X, int64_t n, int32_t ncol, const int32_t* categorical, int32_t n_cat,
+                                 const lgbmhip_bin_params* params, lgbmhip_bins** out);
+void lgbmhip_bins_free(lgbmhip_bins* b);
+int32_t lgbmhip_bins_num_features(const lgbmhip_bins* b);
+/* metas: num_features entries; upper: per feature num_bin values concatenated (bin upper bounds, or the category of each bin). */
+lgbmhip_status lgbmhip_bins_metas(const lgbmhip_bins* b, lgbmhip_feature_meta* metas, double* upper);
+int64_t lgbmhip_bins_num_upper(const lgbmhip_bins* b);
+/* The binned matrix, n x num_features row-major u8 (num_bin <= 256 for every feature). */
+lgbmhip_status lgbmhip_bins_matrix(const lgbmhip_bins* b, uint8_t* out);
+/* Bin raw values with the mappers of `b` (validation / prediction data). */
+lgbmhip_status lgbmhip_bins_apply(const lgbmhip_bins* b, const float* X, int64_t n, int32_t ncol, uint8_t* out);
+
+/* ---- context ---- */
+lgbmhip_status lgbmhip_create(int32_t device, lgbmhip_ctx** out);
+void lgbmhip_destroy(lgbmhip_ctx* ctx);
+
+/* Uploads the binned matrix once (row-major and feature-major copies in HBM). */
+lgbmhip_status lgbmhip_upload_bins(lgbmhip_ctx* ctx, const lgbmhip_feature_meta* metas, int32_t num_features,
+                                   const double* upper, const uint8_t* bins, int64_t num_rows, int32_t num_total_features);
+lgbmhip_status lgbmhip_set_metadata(lgbmhip_ctx* ctx, const float* label, const float* weight,
+                                    const int32_t* query_boundaries, int32_t num_queries);
+lgbmhip_status lgbmhip_set_split_params(lgbmhip_ctx* ctx, const lgbmhip_split_params* p);
+lgbmhip_status lgbmhip_set_train_opts(lgbmhip_ctx* ctx, const lgbmhip_train_opts* o);
+
+/* One boosting iteration: gradients for all K classes, K trees, score update.
+ * *finished = 1 when no tree could split (boosting.rs:1118-1126). */
+lgbmhip_status lgbmhip_train_iteration(lgbmhip_ctx* ctx, int32_t* finished);
+/* One tree from caller gradients (length num_rows); no score update. */
+lgbmhip_status lgbmhip_train_tree(lgbmhip_ctx* ctx, const float* grad, const float* hess);
+lgbmhip_status lgbmhip_sync(lgbmhip_ctx* ctx);
+
+int32_t lgbmhip_num_trees(lgbmhip_ctx* ctx);
+int32_t lgbmhip_tree_num_leaves(lgbmhip_ctx* ctx, int32_t t);
+/* Tree t's arrays; node arrays hold num_leaves-1 entries, leaf arrays num_leaves. */
+lgbmhip_status lgbmhip_tree_arrays(lgbmhip_ctx* ctx, int32_t t, int32_t* split_feature, uint32_t* threshold_in_bin,
+                                   double* threshold, int8_t* decision_type, int32_t* left_child, int32_t* right_child,
+                                   float* split_gain, double* leaf_value, int32_t* leaf_count, double* leaf_weight,
+                                   double* internal_value);
+/* Leaf index of every training row for the last tree (partition state). */
+lgbmhip_status lgbmhip_leaf_of_row(lgbmhip_ctx* ctx, int32_t* out);
+lgbmhip_status lgbmhip_copy_scores(lgbmhip_ctx* ctx, double* out);
+lgbmhip_status lgbmhip_copy_gradients(lgbmhip_ctx* ctx, float* grad, float* hess);
+lgbmhip_status lgbmhip_quant_state(lgbmhip_ctx* ctx, int8_t* gq, int8_t* hq, double* scales);
+
+/* ---- stage entry points (parity checks) ---- */
+lgbmhip_status lgbmhip_gradients(lgbmhip_ctx* ctx, const double* scores, float* grad, float* hess);
+lgbmhip_status lgbmhip_discretize(lgbmhip_ctx* ctx, const float* grad, const float* hess, int32_t num_calls,
+                                  int8_t* gq, int8_t* hq, double* scales);
+/* Histogram over rows[0..m) of the uploaded matrix: out is [F][B][2] f64
+ * (B = max num_bin), most-frequent bins left at zero (filled later from totals). */
+lgbmhip_status lgbmhip_build_histogram(lgbmhip_ctx* ctx, const uint32_t* rows, int64_t m, const float* grad,
+                                       const float* hess, double* out, int32_t* out_num_bins);
+lgbmhip_status lgbmhip_build_histogram_int(lgbmhip_ctx* ctx, const uint32_t* rows, int64_t m, const int8_t* gq,
+                                           const int8_t* hq, int64_t* out, int32_t* out_num_bins);
+/* Threshold search on one feature's bin-indexed [B][2] histogram (mfb fixed from the sums first).
+ * out_d = {gain, left_output, right_output, lsg, lsh, rsg, rsh}; out_i = {threshold, default_left,
+ * left_count, right_count, splittable}. */
+lgbmhip_status lgbmhip_find_best_split(lgbmhip_ctx* ctx, int32_t feature, const double* hist, double sum_g,
+                                       double sum_h, int32_t num_data, double parent_output, double* out_d,
+                                       int32_t* out_i);
+lgbmhip_status lgbmhip_find_best_split_int(lgbmhip_ctx* ctx, int32_t feature, const int64_t* hist, int64_t int_sum,
+                                           double grad_scale, double hess_scale, int32_t num_data,
+                                           double parent_output, double* out_d, int32_t* out_i, int64_t* out_gh);
+/* Stable partition of rows[0..m) on inner feature `feature`: left rows (in order) then right rows. */
+lgbmhip_status lgbmhip_partition(lgbmhip_ctx* ctx, const uint32_t* rows, int64_t m, int32_t feature,
+                                 uint32_t threshold, int32_t default_left, uint32_t* out_rows, int64_t* out_left);
+/* Raw scores of the trained model on raw rows (rows x ncol f32): out [rows][K]. */
+lgbmhip_status lgbmhip_predict_raw(lgbmhip_ctx* ctx, const float* X, int64_t rows, int32_t ncol, double* out);
+/* Raw scores on binned rows (rows x F u8, the mappers of the training data): out [rows][K]. */
+lgbmhip_status lgbmhip_predict_binned(lgbmhip_ctx* ctx, const uint8_t* bins, int64_t rows, double* out);
+
+/* Profiling of training iterations: enable = 1 records HIP events around every
+ * histogram, split and partition launch (graphs off), 0 stops, -1 only reads.
+ * out8 (may be NULL) receives the totals since enabling: {histogram ms,
+ * histogram launches that summed rows, rows summed, algorithmic histogram
+ * bytes, split ms, partition ms, iteration ms, iterations}. No counterpart in
+ * the reference (it has no profiler hooks); for bench and rocprof cross-checks. */
+lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out8);
+
+/* Device-time of the last training iteration's kernels, by kind (ms): histogram, split, partition, other. */
+lgbmhip_status lgbmhip_last_timings(lgbmhip_ctx* ctx, double* out4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LGBM_HIP_H_ */

@@ -1,0 +1,293 @@
+"""MI355X GBDT engine: the lightgbm-rust training/prediction hot path on HIP.
+
+Host-side mirror of the reference's Python API (python/lightgbm_rust/:
+``Dataset`` basic.py:710, ``Booster`` basic.py:1364, ``train`` engine.py:50)
+for the hot path, over the C ABI of liblgbm_hip.so. The only API change is
+the device option: ``device_type`` (alias ``device``) = ``"gpu"`` / ``"hip"``
+selects this engine (config.rs:787-793 gates it in the reference).
+
+Binning (the BinMapper) runs on the host; the packed bin matrix is uploaded
+to HBM once; gradients, histograms, threshold search, partition, score
+update and prediction run as HIP kernels on gfx950.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from ._capi import LightGBMError, check, lib, ptr
+
+__all__ = ["Dataset", "Booster", "train", "LightGBMError", "DEFAULT_PARAMS"]
+
+# lgbm-core defaults of the parameters this path reads (config.rs:508-620).
+DEFAULT_PARAMS = dict(
+    objective="regression", num_iterations=100, learning_rate=0.1, num_leaves=31, max_depth=-1,
+    min_data_in_leaf=20, min_sum_hessian_in_leaf=1e-3, lambda_l1=0.0, lambda_l2=0.0, max_delta_step=0.0,
+    min_gain_to_split=0.0, path_smooth=0.0, max_bin=255, min_data_in_bin=3, bin_construct_sample_cnt=200000,
+    data_random_seed=1, seed=0, feature_pre_filter=True, use_missing=True, zero_as_missing=False, sigmoid=1.0,
+    boost_from_average=True, num_class=1, use_quantized_grad=False, num_grad_quant_bins=4,
+    stochastic_rounding=True, quant_train_renew_leaf=False, max_cat_to_onehot=4, max_cat_threshold=32,
+    cat_l2=10.0, cat_smooth=10.0, min_data_per_group=100, lambdarank_truncation_level=30, lambdarank_norm=True,
+    num_threads=0, device_type="gpu", gpu_device_id=0, use_graph=True, quant_table_threads=1,
+)
+
+ALIASES = {
+    "device": "device_type", "num_boost_round": "num_iterations", "n_estimators": "num_iterations",
+    "num_iteration": "num_iterations", "num_tree": "num_iterations", "num_trees": "num_iterations",
+    "eta": "learning_rate", "shrinkage_rate": "learning_rate", "num_leaf": "num_leaves", "max_leaves": "num_leaves",
+    "min_data_per_leaf": "min_data_in_leaf", "min_data": "min_data_in_leaf", "min_child_samples": "min_data_in_leaf",
+    "min_sum_hessian_per_leaf": "min_sum_hessian_in_leaf", "min_child_weight": "min_sum_hessian_in_leaf",
+    "reg_alpha": "lambda_l1", "l1_regularization": "lambda_l1", "reg_lambda": "lambda_l2", "lambda": "lambda_l2",
+    "l2_regularization": "lambda_l2", "min_split_gain": "min_gain_to_split", "max_bins": "max_bin",
+    "num_classes": "num_class", "application": "objective", "objective_type": "objective", "app": "objective",
+    "random_seed": "seed", "random_state": "seed", "data_seed": "data_random_seed", "subsample_for_bin":
+    "bin_construct_sample_cnt", "num_thread": "num_threads", "nthread": "num_threads", "n_jobs": "num_threads",
+    "cat_feature": "categorical_feature", "categorical_column": "categorical_feature",
+}
+
+OBJECTIVES = {"binary": 0, "multiclass": 1, "softmax": 1, "lambdarank": 2}
+DEVICES = {"gpu", "hip", "cuda"}
+
+
+def _params(params):
+    out = dict(DEFAULT_PARAMS)
+    for k, v in (params or {}).items():
+        k = ALIASES.get(k, k)
+        out[k] = v
+    if str(out["device_type"]).lower() not in DEVICES:
+        raise LightGBMError("device_type must be gpu/hip for this engine (the reference's CPU engine handles cpu)")
+    return out
+
+
+def _b(v):
+    if isinstance(v, str):
+        return v.lower() in ("true", "1", "+", "yes")
+    return bool(v)
+
+
+class Dataset:
+    """Binned training data (basic.py:710). Host binning; bins uploaded lazily by Booster."""
+
+    def __init__(self, data, label=None, weight=None, group=None, params=None, categorical_feature=None,
+                 reference=None, free_raw_data=True):
+        self.params = _params(params)
+        X = np.ascontiguousarray(data, dtype=np.float32)
+        if X.ndim != 2:
+            raise LightGBMError("data must be 2-D")
+        self.num_data, self.num_total_features = X.shape
+        self.label = None if label is None else np.ascontiguousarray(label, dtype=np.float32)
+        if self.label is not None and len(self.label) != self.num_data:
+            raise LightGBMError(f"Length of label ({len(self.label)}) is not same with #data ({self.num_data})")
+        self.weight = None if weight is None else np.ascontiguousarray(weight, dtype=np.float32)
+        self.group = None if group is None else np.asarray(group, dtype=np.int32)
+        cats = categorical_feature if categorical_feature is not None else self.params.get("categorical_feature")
+        if isinstance(cats, str):
+            cats = [int(c) for c in cats.split(",") if c.strip()] if cats else []
+        self.categorical = np.ascontiguousarray(cats or [], dtype=np.int32)
+        self.reference = reference
+        if reference is not None:
+            self._handle = reference._handle
+            self._owns = False
+            self.bins = np.zeros((self.num_data, reference.num_features), np.uint8)
+            check(lib().lgbmhip_bins_apply(self._handle, ptr(X), self.num_data, self.num_total_features,
+                                           ptr(self.bins)))
+            self.metas, self.upper = reference.metas, reference.upper
+            self.num_features = reference.num_features
+            return
+        p = self.params
+        bp = _capi.BinParams(int(p["max_bin"]), int(p["min_data_in_bin"]), int(p["min_data_in_leaf"]),
+                             int(p["bin_construct_sample_cnt"]), int(p["data_random_seed"]),
+                             int(_b(p["feature_pre_filter"])), int(_b(p["use_missing"])),
+                             int(_b(p["zero_as_missing"])), int(p["num_threads"]))
+        h = C.c_void_p()
+        check(lib().lgbmhip_bin_dense(ptr(X), self.num_data, self.num_total_features,
+                                      ptr(self.categorical) if len(self.categorical) else None,
+                                      len(self.categorical), C.byref(bp), C.byref(h)))
+        self._handle = h
+        self._owns = True
+        self.num_features = lib().lgbmhip_bins_num_features(h)
+        self.metas = (_capi.FeatureMeta * max(self.num_features, 1))()
+        self.upper = np.zeros(max(lib().lgbmhip_bins_num_upper(h), 1), np.float64)
+        check(lib().lgbmhip_bins_metas(h, C.cast(self.metas, C.c_void_p), ptr(self.upper)))
+        self.bins = np.zeros((self.num_data, self.num_features), np.uint8)
+        check(lib().lgbmhip_bins_matrix(h, ptr(self.bins)))
+
+    def feature_meta(self, f):
+        m = self.metas[f]
+        return dict(real_index=m.real_index, num_bin=m.num_bin, missing_type=m.missing_type,
+                    default_bin=m.default_bin, most_freq_bin=m.most_freq_bin, bin_type=m.bin_type)
+
+    def construct(self):
+        return self
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h and getattr(self, "_owns", False) and _capi._lib is not None:
+            _capi._lib.lgbmhip_bins_free(h)
+            self._handle = None
+
+
+class Tree:
+    """Tree arrays (tree.rs:120-159)."""
+
+
+class Booster:
+    """Training booster on the GPU engine (basic.py:1364 / py/lib.rs:435 RsBooster)."""
+
+    def __init__(self, params=None, train_set=None):
+        if train_set is None:
+            raise LightGBMError("Booster needs a train_set (model-file loading is outside the GPU path)")
+        self.params = _params({**(train_set.params or {}), **(params or {})})
+        p = self.params
+        obj = str(p["objective"])
+        if obj not in OBJECTIVES:
+            raise LightGBMError(f"objective {obj!r} is not on the GPU path (binary, multiclass, lambdarank)")
+        if train_set.label is None:
+            raise LightGBMError("train_set has no label")
+        self.train_set = train_set
+        self.K = int(p["num_class"]) if OBJECTIVES[obj] == 1 else 1
+        h = C.c_void_p()
+        check(lib().lgbmhip_create(int(p["gpu_device_id"]), C.byref(h)))
+        self._ctx = h
+        ds = train_set
+        check(lib().lgbmhip_upload_bins(h, C.cast(ds.metas, C.c_void_p), ds.num_features, ptr(ds.upper),
+                                        ptr(ds.bins), ds.num_data, ds.num_total_features), h)
+        qb = None
+        nq = 0
+        if ds.group is not None:
+            qb = np.ascontiguousarray(np.concatenate([[0], np.cumsum(ds.group)]), dtype=np.int32)
+            nq = len(ds.group)
+        check(lib().lgbmhip_set_metadata(h, ptr(ds.label), ptr(ds.weight), ptr(qb), nq), h)
+        sp = _capi.SplitParams(float(p["lambda_l1"]), float(p["lambda_l2"]), float(p["max_delta_step"]),
+                               float(p["path_smooth"]), float(p["min_sum_hessian_in_leaf"]),
+                               float(p["min_gain_to_split"]), float(p["cat_l2"]), float(p["cat_smooth"]),
+                               int(p["min_data_in_leaf"]), int(p["max_cat_to_onehot"]), int(p["max_cat_threshold"]),
+                               int(p["min_data_per_group"]), int(p["num_leaves"]), int(p["max_depth"]))
+        check(lib().lgbmhip_set_split_params(h, C.byref(sp)), h)
+        to = _capi.TrainOpts(OBJECTIVES[obj], self.K, float(p["sigmoid"]), float(p["learning_rate"]),
+                             int(_b(p["boost_from_average"])), int(_b(p["use_quantized_grad"])),
+                             int(p["num_grad_quant_bins"]), int(_b(p["stochastic_rounding"])),
+                             int(_b(p["quant_train_renew_leaf"])), int(p["seed"]), int(p["quant_table_threads"]),
+                             int(p["lambdarank_truncation_level"]), int(_b(p["lambdarank_norm"])),
+                             int(_b(p["use_graph"])))
+        check(lib().lgbmhip_set_train_opts(h, C.byref(to)), h)
+        self._finished = False
+
+    # --- training (basic.py:1519 update -> py/lib.rs:479-482 -> Gbdt::train_one_iter)
+    def update(self):
+        """One boosting iteration; returns True when training cannot continue."""
+        fin = C.c_int32(0)
+        check(lib().lgbmhip_train_iteration(self._ctx, C.byref(fin)), self._ctx)
+        self._finished = bool(fin.value)
+        return self._finished
+
+    def train_tree(self, grad, hess):
+        """One tree from caller gradients (SerialTreeLearner::train), no score update."""
+        g = np.ascontiguousarray(grad, np.float32)
+        h = np.ascontiguousarray(hess, np.float32)
+        check(lib().lgbmhip_train_tree(self._ctx, ptr(g), ptr(h)), self._ctx)
+
+    def num_trees(self):
+        return lib().lgbmhip_num_trees(self._ctx)
+
+    def current_iteration(self):
+        return self.num_trees() // self.K
+
+    def tree(self, t):
+        nl = lib().lgbmhip_tree_num_leaves(self._ctx, t)
+        if nl < 0:
+            raise LightGBMError("tree index out of range")
+        ni = max(nl - 1, 1)
+        T = Tree()
+        T.num_leaves = nl
+        T.split_feature = np.zeros(ni, np.int32)
+        T.threshold_in_bin = np.zeros(ni, np.uint32)
+        T.threshold = np.zeros(ni, np.float64)
+        T.decision_type = np.zeros(ni, np.int8)
+        T.left_child = np.zeros(ni, np.int32)
+        T.right_child = np.zeros(ni, np.int32)
+        T.split_gain = np.zeros(ni, np.float32)
+        T.leaf_value = np.zeros(nl, np.float64)
+        T.leaf_count = np.zeros(nl, np.int32)
+        T.leaf_weight = np.zeros(nl, np.float64)
+        T.internal_value = np.zeros(ni, np.float64)
+        check(lib().lgbmhip_tree_arrays(self._ctx, t, ptr(T.split_feature), ptr(T.threshold_in_bin),
+                                        ptr(T.threshold), ptr(T.decision_type), ptr(T.left_child),
+                                        ptr(T.right_child), ptr(T.split_gain), ptr(T.leaf_value),
+                                        ptr(T.leaf_count), ptr(T.leaf_weight), ptr(T.internal_value)), self._ctx)
+        for a in ("split_feature", "threshold_in_bin", "threshold", "decision_type", "left_child", "right_child",
+                  "split_gain", "internal_value"):
+            setattr(T, a, getattr(T, a)[: nl - 1])
+        return T
+
+    def train_scores(self):
+        out = np.zeros(self.K * self.train_set.num_data, np.float64)
+        check(lib().lgbmhip_copy_scores(self._ctx, ptr(out)), self._ctx)
+        return out
+
+    def last_gradients(self):
+        n = self.K * self.train_set.num_data
+        g = np.zeros(n, np.float32)
+        h = np.zeros(n, np.float32)
+        check(lib().lgbmhip_copy_gradients(self._ctx, ptr(g), ptr(h)), self._ctx)
+        return g, h
+
+    def leaf_of_row(self):
+        out = np.zeros(self.train_set.num_data, np.int32)
+        check(lib().lgbmhip_leaf_of_row(self._ctx, ptr(out)), self._ctx)
+        return out
+
+    # --- prediction (basic.py:1641 -> py/lib.rs:557-579 -> predict.rs:231)
+    def predict(self, data, raw_score=False):
+        X = np.ascontiguousarray(data, np.float32)
+        out = np.zeros(X.shape[0] * self.K, np.float64)
+        check(lib().lgbmhip_predict_raw(self._ctx, ptr(X), X.shape[0], X.shape[1], ptr(out)), self._ctx)
+        raw = out.reshape(X.shape[0], self.K)
+        if raw_score:
+            return raw[:, 0] if self.K == 1 else raw
+        obj = self.params["objective"]
+        if obj == "binary":
+            return 1.0 / (1.0 + np.exp(-float(self.params["sigmoid"]) * raw[:, 0]))
+        if self.K > 1:
+            e = np.exp(raw - raw.max(axis=1, keepdims=True))
+            return e / e.sum(axis=1, keepdims=True)
+        return raw[:, 0]
+
+    def predict_binned(self, bins):
+        b = np.ascontiguousarray(bins, np.uint8)
+        out = np.zeros(b.shape[0] * self.K, np.float64)
+        check(lib().lgbmhip_predict_binned(self._ctx, ptr(b), b.shape[0], ptr(out)), self._ctx)
+        return out.reshape(b.shape[0], self.K)
+
+    def profile(self, enable=-1):
+        """Kernel-kind timing totals (see lgbmhip_profile); enable=1 starts, 0 stops, -1 reads."""
+        out = np.zeros(8, np.float64)
+        check(lib().lgbmhip_profile(self._ctx, int(enable), ptr(out)), self._ctx)
+        keys = ("hist_ms", "hist_launches", "hist_rows", "hist_bytes", "split_ms", "part_ms", "iter_ms", "iters")
+        return dict(zip(keys, out.tolist()))
+
+    def last_iteration_ms(self):
+        out = np.zeros(4, np.float64)
+        check(lib().lgbmhip_last_timings(self._ctx, ptr(out)), self._ctx)
+        return float(out[0])
+
+    def free(self):
+        if getattr(self, "_ctx", None) and _capi._lib is not None:
+            _capi._lib.lgbmhip_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        self.free()
+
+
+def train(params, train_set, num_boost_round=100, callbacks=None):
+    """engine.py:50: boosting loop calling Booster.update until done."""
+    p = _params(params)
+    n = int(p.get("num_iterations", num_boost_round)) if "num_iterations" in (params or {}) else num_boost_round
+    b = Booster(params, train_set)
+    for i in range(n):
+        if b.update():
+            break
+        for cb in callbacks or []:
+            cb(b, i)
+    return b

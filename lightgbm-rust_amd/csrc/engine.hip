@@ -1,0 +1,1378 @@
+// Host driver of the MI355X GBDT engine and the C ABI (include/lgbm_hip.h).
+//
+// The driver mirrors the reference's boosting loop (boosting.rs:957-1139) on
+// the host: it launches the device-resident tree-growth sequence for each
+// class, copies the split log back once per iteration and replays it into
+// Tree arrays (tree.rs:254-334), which stay on the host for the model.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "kernels.hip"
+
+namespace lgbmhip {
+
+#define HIP_CHECK(x)                                                                             \
+  do {                                                                                           \
+    hipError_t e_ = (x);                                                                         \
+    if (e_ != hipSuccess)                                                                        \
+      throw Error(LGBMHIP_INTERNAL, std::string(#x) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+void launch_hist(const HistArgs& a, bool quant, int tpr, int grid, size_t lds, hipStream_t s);
+void set_hist_lds_limits(size_t bytes);
+void launch_split(const SplitArgs& a, hipStream_t s);
+void launch_part(const PartArgs& a, int grid, hipStream_t s);
+
+// mt19937.rs:14-88 (the quantizer's generators).
+struct Mt {
+  uint32_t st[624];
+  int i;
+  explicit Mt(uint32_t seed) {
+    st[0] = seed;
+    for (int k = 1; k < 624; ++k) st[k] = 1812433253u * (st[k - 1] ^ (st[k - 1] >> 30)) + (uint32_t)k;
+    i = 624;
+  }
+  uint32_t next() {
+    if (i >= 624) {
+      for (int k = 0; k < 624; ++k) {
+        uint32_t y = (st[k] & 0x80000000u) | (st[(k + 1) % 624] & 0x7fffffffu);
+        st[k] = st[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+      }
+      i = 0;
+    }
+    uint32_t z = st[i++];
+    z ^= z >> 11;
+    z ^= (z << 7) & 0x9d2c5680u;
+    z ^= (z << 15) & 0xefc60000u;
+    z ^= z >> 18;
+    return z;
+  }
+  double uniform01() {
+    const double R = 4294967296.0;
+    double s = (double)next();
+    s += (double)next() * R;
+    double r = s / (R * R);
+    return r >= 1.0 ? std::nextafter(1.0, 0.0) : r;
+  }
+  int uniform_int(int a, int b) {
+    uint32_t range = (uint32_t)((int64_t)b - (int64_t)a + 1);
+    uint64_t prod = (uint64_t)next() * range;
+    uint32_t low = (uint32_t)prod;
+    if (low < range) {
+      uint32_t thr = (uint32_t)(-range) % range;
+      while (low < thr) {
+        prod = (uint64_t)next() * range;
+        low = (uint32_t)prod;
+      }
+    }
+    return (int)((int64_t)(prod >> 32) + a);
+  }
+};
+
+// Tree arrays of the model (tree.rs:120-159).
+struct HostTree {
+  int num_leaves = 1;
+  std::vector<int> split_feature_inner, split_feature, left_child, right_child, leaf_parent{-1}, leaf_count{0},
+      leaf_depth{0}, internal_count;
+  std::vector<float> split_gain;
+  std::vector<uint32_t> threshold_in_bin;
+  std::vector<double> threshold, leaf_value{0.0}, leaf_weight{0.0}, internal_value, internal_weight;
+  std::vector<int8_t> decision_type;
+  int num_cat = 0;
+  std::vector<int> cat_boundaries{0}, cat_boundaries_inner{0};
+  std::vector<uint32_t> cat_threshold, cat_threshold_inner;
+  double shrinkage = 1.0;
+  static double r0(double v) { return (v >= -(double)1e-35f && v <= (double)1e-35f) ? 0.0 : v; }
+  int split(int leaf, int fi, int fr, uint32_t tb, double thr, double lv, double rv, int lc, int rc, double lw,
+            double rw, float gain, int mt, bool dl) {
+    int node = num_leaves - 1, nl = num_leaves;
+    int parent = leaf_parent[leaf];
+    if (parent >= 0) {
+      if (left_child[parent] == ~leaf) left_child[parent] = node; else right_child[parent] = node;
+    }
+    split_feature_inner.push_back(fi);
+    split_feature.push_back(fr);
+    split_gain.push_back(gain);
+    left_child.push_back(~leaf);
+    right_child.push_back(~nl);
+    leaf_parent[leaf] = node;
+    leaf_parent.push_back(node);
+    internal_weight.push_back(lw + rw);
+    internal_value.push_back(leaf_value[leaf]);
+    internal_count.push_back(lc + rc);
+    leaf_value[leaf] = std::isnan(lv) ? 0.0 : lv;
+    leaf_weight[leaf] = lw;
+    leaf_count[leaf] = lc;
+    leaf_value.push_back(std::isnan(rv) ? 0.0 : rv);
+    leaf_weight.push_back(rw);
+    leaf_count.push_back(rc);
+    int d = leaf_depth[leaf] + 1;
+    leaf_depth[leaf] = d;
+    leaf_depth.push_back(d);
+    int8_t dt = dl ? 2 : 0;
+    dt = (int8_t)((dt & 3) | (mt << 2));
+    decision_type.push_back(dt);
+    threshold_in_bin.push_back(tb);
+    threshold.push_back(thr);
+    num_leaves++;
+    return nl;
+  }
+  void shrink(double rate) {
+    for (int i = 0; i < num_leaves - 1; ++i) {
+      leaf_value[i] = r0(leaf_value[i] * rate);
+      internal_value[i] = r0(internal_value[i] * rate);
+    }
+    leaf_value[num_leaves - 1] = r0(leaf_value[num_leaves - 1] * rate);
+    shrinkage *= rate;
+  }
+  void add_bias(double v) {
+    for (int i = 0; i < num_leaves - 1; ++i) {
+      leaf_value[i] = r0(leaf_value[i] + v);
+      internal_value[i] = r0(internal_value[i] + v);
+    }
+    leaf_value[num_leaves - 1] = r0(leaf_value[num_leaves - 1] + v);
+    shrinkage = 1.0;
+  }
+  void as_constant(double v, int count) {
+    *this = HostTree();
+    leaf_value[0] = v;
+    leaf_count[0] = count;
+  }
+};
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    free();
+    if (count == 0) return;
+    HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void zero(hipStream_t s) {
+    if (p) HIP_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { free(); }
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int cus = 256;
+  size_t lds_max = 65536;
+  // data
+  int64_t N = 0;
+  int F = 0, S = 0, B = 0, ncol = 0;
+  std::vector<lgbmhip_feature_meta> metas;
+  std::vector<std::vector<double>> upper;
+  std::vector<FeatDev> feats;
+  DevBuf<uint8_t> bins_rm, bins_cm;
+  DevBuf<FeatDev> d_feats;
+  // metadata
+  std::vector<float> label;
+  DevBuf<float> d_label, d_weight;
+  bool has_weight = false;
+  std::vector<int32_t> qb;
+  // parameters
+  SplitParamsDev sp{};
+  bool sp_set = false;
+  lgbmhip_train_opts opts{};
+  bool opts_set = false;
+  int K = 1, L = 31;
+  // training state
+  bool state_ready = false;
+  DevBuf<double> score;
+  DevBuf<float> g, h;
+  DevBuf<unsigned> idx[2];
+  DevBuf<uint8_t> gh[2];
+  DevBuf<long long> acc;
+  DevBuf<double> pool;
+  DevBuf<long long> ipool;
+  DevBuf<SplitRec> feat_split, best, cur, log;
+  DevBuf<uint8_t> spl;
+  DevBuf<StepDev> st;
+  DevBuf<LeafDev> leaves;
+  DevBuf<IterDev> it;
+  DevBuf<int> tiles;
+  DevBuf<double> leafval;
+  DevBuf<long long> renew;
+  DevBuf<double> grv, hrv;
+  DevBuf<int> starts;
+  DevBuf<int> call_ctr;
+  DevBuf<int8_t> gq, hq;
+  std::unique_ptr<Mt> start_eng;
+  int n_calls = 0;
+  int hist_tpr = 8, hist_fb = 0, hist_ng = 1, hist_chunks = 512, hist_min_rows = 2048;
+  size_t hist_lds = 0;
+  int part_grid = 1024;
+  // pinned copies of the per-tree results
+  std::vector<SplitRec*> h_log;
+  std::vector<LeafDev*> h_leaves;
+  std::vector<StepDev*> h_st;
+  // graphs
+  std::vector<hipGraphExec_t> graphs;
+  // model
+  std::vector<HostTree> trees;
+  bool models_empty_init = true;
+  int iter = 0;
+  // prediction cache
+  bool pred_dirty = true;
+  DevBuf<PredNode> p_nodes;
+  DevBuf<int> p_node_off, p_leaf_off, p_nleaves;
+  DevBuf<double> p_leafval;
+  DevBuf<unsigned> p_cat;
+  int p_T = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  // profiling: HIP events around every histogram / split / partition launch
+  int profile = 0;
+  std::vector<hipEvent_t> pev;   // [K][L][3 kinds][2]
+  double prof[8] = {0};          // hist_ms, hist_launches, hist_rows, hist_bytes, split_ms, part_ms, iter_ms, iters
+
+  ~Ctx() {
+    for (auto e : pev) if (e) (void)hipEventDestroy(e);
+    for (auto g : graphs) if (g) (void)hipGraphExecDestroy(g);
+    for (auto p : h_log) if (p) (void)hipHostFree(p);
+    for (auto p : h_leaves) if (p) (void)hipHostFree(p);
+    for (auto p : h_st) if (p) (void)hipHostFree(p);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+static void set_device(Ctx* c) { HIP_CHECK(hipSetDevice(c->device)); }
+
+static int grid_for(int64_t n, int threads, int cap) {
+  int64_t g = (n + threads - 1) / threads;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+// Row-major stride: rows of <= 32 bytes padded to a power of two (one 32-byte
+// sector per row gather), wider rows to 16 bytes.
+static int row_stride(int F) {
+  if (F <= 32) {
+    int s = 4;
+    while (s < F) s <<= 1;
+    return s;
+  }
+  return (F + 15) / 16 * 16;
+}
+
+static void upload_bins(Ctx* c, const lgbmhip_feature_meta* metas, int F, const double* upper, const uint8_t* bins,
+                        int64_t N, int ncol) {
+  if (F <= 0 || N <= 0) throw Error(LGBMHIP_INVALID_DATA, "no rows or no used features");
+  if (N > INT32_MAX) throw Error(LGBMHIP_INVALID_DATA, "more than 2^31-1 rows");
+  set_device(c);
+  c->N = N;
+  c->F = F;
+  c->ncol = ncol;
+  c->metas.assign(metas, metas + F);
+  c->upper.clear();
+  c->feats.clear();
+  c->B = 1;
+  size_t off = 0;
+  for (int f = 0; f < F; ++f) {
+    const auto& m = metas[f];
+    if (m.num_bin < 1 || m.num_bin > kMaxBins) throw Error(LGBMHIP_INVALID_DATA, "num_bin out of range [1, 256]");
+    if (m.most_freq_bin < 0 || m.most_freq_bin >= m.num_bin || m.default_bin < 0 || m.default_bin >= m.num_bin)
+      throw Error(LGBMHIP_INVALID_DATA, "bin index out of range in feature meta");
+    c->upper.emplace_back(upper + off, upper + off + m.num_bin);
+    off += (size_t)m.num_bin;
+    FeatDev d{m.num_bin, m.missing_type, m.default_bin, m.most_freq_bin, m.bin_type, m.most_freq_bin == 0 ? 1 : 0};
+    c->feats.push_back(d);
+    c->B = std::max(c->B, m.num_bin);
+  }
+  for (int64_t i = 0; i < N * F; ++i)
+    if (bins[i] >= (uint8_t)c->metas[i % F].num_bin && c->metas[i % F].num_bin < 256)
+      throw Error(LGBMHIP_INVALID_DATA, "bin value >= num_bin");
+  c->S = row_stride(F);
+  std::vector<uint8_t> rm((size_t)N * c->S, 0), cm((size_t)N * F);
+  for (int64_t r = 0; r < N; ++r)
+    for (int f = 0; f < F; ++f) {
+      uint8_t v = bins[(size_t)r * F + f];
+      rm[(size_t)r * c->S + f] = v;
+      cm[(size_t)f * N + r] = v;
+    }
+  c->bins_rm.alloc(rm.size());
+  c->bins_cm.alloc(cm.size());
+  c->d_feats.alloc(F);
+  HIP_CHECK(hipMemcpy(c->bins_rm.p, rm.data(), rm.size(), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->bins_cm.p, cm.data(), cm.size(), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->d_feats.p, c->feats.data(), F * sizeof(FeatDev), hipMemcpyHostToDevice));
+  c->state_ready = false;
+  c->trees.clear();
+  c->pred_dirty = true;
+}
+
+// Histogram launch shape: features per block so the block's LDS histogram
+// fits the budget; threads per row = 4-byte chunks of the group, to a power of 2.
+static void plan_hist(Ctx* c, bool quant) {
+  const size_t per_feat = (size_t)c->B * 2 * (quant ? 4 : 8);
+  size_t budget = std::min<size_t>(c->lds_max, 160 * 1024);
+  int fb = (int)std::min<size_t>((size_t)c->F, budget / per_feat);
+  fb = std::max(4, fb / 4 * 4);
+  if (fb >= c->F) fb = (c->F + 3) / 4 * 4;
+  int ng = (c->F + fb - 1) / fb;
+  // balance the groups
+  fb = ((c->F + ng - 1) / ng + 3) / 4 * 4;
+  ng = (c->F + fb - 1) / fb;
+  int chunks4 = fb / 4;
+  int tpr = 1;
+  while (tpr < chunks4) tpr <<= 1;
+  if (tpr > 32) throw Error(LGBMHIP_UNSUPPORTED, "feature group wider than 128 bytes");
+  c->hist_fb = fb;
+  c->hist_ng = ng;
+  c->hist_tpr = tpr;
+  c->hist_lds = (size_t)std::min(fb, c->F) * per_feat;
+  if (c->hist_lds > c->lds_max) throw Error(LGBMHIP_UNSUPPORTED, "histogram does not fit in LDS");
+  set_hist_lds_limits(c->hist_lds);
+}
+
+static void setup_training(Ctx* c) {
+  if (!c->N) throw Error(LGBMHIP_INVALID_DATA, "no bins uploaded");
+  if (c->label.empty()) throw Error(LGBMHIP_INVALID_DATA, "no labels set");
+  if (!c->sp_set || !c->opts_set) throw Error(LGBMHIP_INVALID_PARAMETER, "split params / train opts not set");
+  set_device(c);
+  const auto& o = c->opts;
+  const bool quant = o.use_quantized_grad != 0;
+  c->K = o.objective == 1 ? o.num_class : 1;
+  if (c->K < 1 || c->K > 16) throw Error(LGBMHIP_UNSUPPORTED, "num_class must be in [1, 16]");
+  if (o.objective == 2) throw Error(LGBMHIP_UNSUPPORTED, "lambdarank is not available in this build");
+  c->L = std::max(c->sp.num_leaves, 2);
+  if (quant) {
+    for (auto& f : c->feats)
+      if (f.bin_type == 1) throw Error(LGBMHIP_UNSUPPORTED, "quantized gradients with categorical features");
+    if (o.num_grad_quant_bins < 2 || o.num_grad_quant_bins > 126)
+      throw Error(LGBMHIP_INVALID_PARAMETER, "num_grad_quant_bins out of range");
+  }
+  const int64_t N = c->N;
+  const int F = c->F, B = c->B, L = c->L, K = c->K;
+  c->score.alloc((size_t)K * N);
+  c->score.zero(c->stream);
+  c->g.alloc((size_t)K * N);
+  c->h.alloc((size_t)K * N);
+  for (int b = 0; b < 2; ++b) {
+    c->idx[b].alloc(N);
+    c->gh[b].alloc((size_t)N * (quant ? 2 : 8));
+  }
+  c->acc.alloc((size_t)F * B * (quant ? 1 : 2));
+  c->acc.zero(c->stream);
+  if (quant) { c->ipool.alloc((size_t)L * F * B); c->pool.free(); }
+  else { c->pool.alloc((size_t)L * F * B * 2); c->ipool.free(); }
+  c->feat_split.alloc(2 * F);
+  c->best.alloc(L);
+  c->cur.alloc(1);
+  c->log.alloc(L);
+  c->spl.alloc((size_t)L * F);
+  c->st.alloc(1);
+  c->st.zero(c->stream);
+  c->leaves.alloc(L);
+  c->leaves.zero(c->stream);
+  c->it.alloc(K);
+  c->it.zero(c->stream);
+  c->tiles.alloc((N + part_tile() - 1) / part_tile() + 1);
+  c->leafval.alloc((size_t)K * L);
+  c->renew.alloc((size_t)2 * L);
+  c->renew.zero(c->stream);
+  c->part_grid = (int)std::min<int64_t>((N + part_tile() - 1) / part_tile(), 2048);
+  if (quant) {
+    // quant.rs:45-79: per-row tables from block generators of `quant_table_threads` blocks
+    std::vector<double> grv(N), hrv(N);
+    int threads = std::max(1, o.quant_table_threads);
+    int64_t nblock = std::max<int64_t>(1, std::min<int64_t>(threads, (N + 511) / 512));
+    int64_t bsize = N;
+    if (nblock > 1) { bsize = (N + nblock - 1) / nblock; bsize = (bsize + 31) / 32 * 32; }
+    for (int64_t b = 0; b < nblock; ++b) {
+      Mt ge((uint32_t)(o.seed + (int)b)), he((uint32_t)(o.seed + (int)b + threads));
+      for (int64_t i = b * bsize; i < std::min(N, (b + 1) * bsize); ++i) {
+        grv[i] = ge.uniform01();
+        hrv[i] = he.uniform01();
+      }
+    }
+    c->grv.alloc(N);
+    c->hrv.alloc(N);
+    HIP_CHECK(hipMemcpy(c->grv.p, grv.data(), N * 8, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->hrv.p, hrv.data(), N * 8, hipMemcpyHostToDevice));
+    c->start_eng.reset(new Mt((uint32_t)o.seed));
+    const int cap = 1 << 16;
+    std::vector<int> s(cap);
+    for (int i = 0; i < cap; ++i) s[i] = c->start_eng->uniform_int(0, (int)N);
+    c->starts.alloc(cap);
+    HIP_CHECK(hipMemcpy(c->starts.p, s.data(), cap * 4, hipMemcpyHostToDevice));
+    c->call_ctr.alloc(1);
+    c->call_ctr.zero(c->stream);
+    c->gq.alloc(N);
+    c->hq.alloc(N);
+    c->n_calls = 0;
+  }
+  plan_hist(c, quant);
+  for (auto p : c->h_log) if (p) (void)hipHostFree(p);
+  for (auto p : c->h_leaves) if (p) (void)hipHostFree(p);
+  for (auto p : c->h_st) if (p) (void)hipHostFree(p);
+  c->h_log.assign(K, nullptr);
+  c->h_leaves.assign(K, nullptr);
+  c->h_st.assign(K, nullptr);
+  for (int k = 0; k < K; ++k) {
+    HIP_CHECK(hipHostMalloc((void**)&c->h_log[k], sizeof(SplitRec) * L));
+    HIP_CHECK(hipHostMalloc((void**)&c->h_leaves[k], sizeof(LeafDev) * L));
+    HIP_CHECK(hipHostMalloc((void**)&c->h_st[k], sizeof(StepDev)));
+  }
+  for (auto g : c->graphs) if (g) (void)hipGraphExecDestroy(g);
+  c->graphs.assign(K, nullptr);
+  c->trees.clear();
+  c->iter = 0;
+  c->pred_dirty = true;
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->state_ready = true;
+}
+
+static HistArgs hist_args(Ctx* c, bool quant) {
+  HistArgs a{};
+  a.bins = c->bins_rm.p;
+  a.S = c->S;
+  a.idx[0] = c->idx[0].p;
+  a.idx[1] = c->idx[1].p;
+  a.gh[0] = c->gh[0].p;
+  a.gh[1] = c->gh[1].p;
+  a.st = c->st.p;
+  a.leaves = c->leaves.p;
+  a.feats = c->d_feats.p;
+  a.F = c->F;
+  a.B = c->B;
+  a.FB = c->hist_fb;
+  a.NG = c->hist_ng;
+  a.max_chunks = c->hist_chunks;
+  a.min_rows_per_chunk = c->hist_min_rows;
+  a.acc = c->acc.p;
+  a.direct = 0;
+  (void)quant;
+  return a;
+}
+
+static SplitArgs split_args(Ctx* c, int k) {
+  SplitArgs a{};
+  a.st = c->st.p;
+  a.leaves = c->leaves.p;
+  a.feats = c->d_feats.p;
+  a.p = c->sp;
+  a.F = c->F;
+  a.B = c->B;
+  a.L = c->L;
+  a.acc = c->acc.p;
+  a.pool = c->pool.p;
+  a.ipool = c->ipool.p;
+  a.feat_split = c->feat_split.p;
+  a.best = c->best.p;
+  a.splittable = c->spl.p;
+  a.cur = c->cur.p;
+  a.log = c->log.p;
+  a.it = c->it.p + k;
+  a.quant = c->opts.use_quantized_grad != 0;
+  return a;
+}
+
+static PartArgs part_args(Ctx* c, int k) {
+  PartArgs a{};
+  a.st = c->st.p;
+  a.leaves = c->leaves.p;
+  a.feats = c->d_feats.p;
+  a.bins_cm = c->bins_cm.p;
+  a.N = c->N;
+  a.idx[0] = c->idx[0].p;
+  a.idx[1] = c->idx[1].p;
+  a.gh[0] = c->gh[0].p;
+  a.gh[1] = c->gh[1].p;
+  a.gh_bytes = c->opts.use_quantized_grad ? 2 : 8;
+  a.tile_left = c->tiles.p;
+  a.cur = c->cur.p;
+  a.best = c->best.p;
+  a.log = c->log.p;
+  a.p = c->sp;
+  a.L = c->L;
+  a.it = c->it.p + k;
+  a.quant = c->opts.use_quantized_grad != 0;
+  return a;
+}
+
+// The launch sequence of one tree for class k (gradients already in g/h and
+// max |g|, |h| in it[k]); `score_update` adds the shrunk leaf values.
+static hipEvent_t pevent(Ctx* c, int k, int step, int kind, int end) {
+  size_t i = ((((size_t)k * c->L + step) * 3 + kind) * 2 + end);
+  return c->pev[i];
+}
+static void prof_mark(Ctx* c, int k, int step, int kind, int end) {
+  if (c->profile) HIP_CHECK(hipEventRecord(pevent(c, k, step, kind, end), c->stream));
+}
+
+static void enqueue_tree(Ctx* c, int k, bool score_update, double lr) {
+  hipStream_t s = c->stream;
+  const int64_t N = c->N;
+  const bool quant = c->opts.use_quantized_grad != 0;
+  const float* gk = c->g.p + (size_t)k * N;
+  const float* hk = c->h.p + (size_t)k * N;
+  const int ig = grid_for(N, 256, c->cus * 8);
+  if (quant) {
+    hipLaunchKernelGGL(k_discretize, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, c->grv.p, c->hrv.p,
+                       c->opts.num_grad_quant_bins, c->opts.stochastic_rounding, 0, c->starts.p, c->call_ctr.p,
+                       c->it.p + k, c->idx[0].p, (char2*)c->gh[0].p, c->gq.p, c->hq.p, c->st.p, c->leaves.p,
+                       c->best.p, c->spl.p, c->F, c->L);
+    hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, c->call_ctr.p);
+  } else {
+    hipLaunchKernelGGL(k_tree_init, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, c->idx[0].p,
+                       (float2*)c->gh[0].p, c->it.p + k, c->st.p, c->leaves.p, c->best.p, c->spl.p, c->F, c->L);
+  }
+  HistArgs ha = hist_args(c, quant);
+  SplitArgs sa = split_args(c, k);
+  PartArgs pa = part_args(c, k);
+  const int hgrid = c->hist_chunks * c->hist_ng;
+  prof_mark(c, k, 0, 0, 0);
+  launch_hist(ha, quant, c->hist_tpr, hgrid, c->hist_lds, s);
+  prof_mark(c, k, 0, 0, 1);
+  launch_split(sa, s);
+  prof_mark(c, k, 0, 1, 1);
+  for (int step = 1; step < c->L; ++step) {
+    prof_mark(c, k, step, 2, 0);
+    launch_part(pa, c->part_grid, s);
+    prof_mark(c, k, step, 2, 1);
+    launch_hist(ha, quant, c->hist_tpr, hgrid, c->hist_lds, s);
+    prof_mark(c, k, step, 0, 1);
+    launch_split(sa, s);
+    prof_mark(c, k, step, 1, 1);
+  }
+  const bool renew = quant && c->opts.quant_train_renew_leaf;
+  if (renew)
+    hipLaunchKernelGGL(k_renew_sum, dim3(64, c->L), dim3(256), 0, s, c->st.p, c->leaves.p, c->idx[0].p, c->idx[1].p,
+                       gk, hk, c->it.p + k, c->renew.p);
+  hipLaunchKernelGGL(k_leaf_values, dim3((c->L + 255) / 256), dim3(256), 0, s, c->st.p, c->leaves.p, c->it.p + k,
+                     c->renew.p, renew ? 1 : 0, c->sp, lr, c->L, c->leafval.p + (size_t)k * c->L);
+  if (score_update)
+    hipLaunchKernelGGL(k_score, dim3(64, c->L), dim3(256), 0, s, c->st.p, c->leaves.p, c->idx[0].p, c->idx[1].p,
+                       c->leafval.p + (size_t)k * c->L, c->score.p + (size_t)k * N);
+}
+
+static void copy_tree_results(Ctx* c, int k) {
+  HIP_CHECK(hipMemcpyAsync(c->h_log[k], c->log.p, sizeof(SplitRec) * c->L, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->h_leaves[k], c->leaves.p, sizeof(LeafDev) * c->L, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->h_st[k], c->st.p, sizeof(StepDev), hipMemcpyDeviceToHost, c->stream));
+}
+
+// Replay the split log into Tree arrays (learner/mod.rs:1105-1225, tree.rs:254-297).
+static HostTree build_tree(Ctx* c, int k) {
+  HostTree t;
+  const StepDev& st = *c->h_st[k];
+  const LeafDev* lv = c->h_leaves[k];
+  if (st.num_splits == 0) {
+    t.leaf_value[0] = st.root_output;
+    t.leaf_count[0] = (int)c->N;
+    return t;
+  }
+  t.leaf_value[0] = st.root_output;
+  for (int i = 0; i < st.num_splits; ++i) {
+    const SplitRec& r = c->h_log[k][i];
+    const int f = r.feature;
+    const auto& m = c->metas[f];
+    const float gain = (float)(r.gain + c->sp.min_gain_to_split);
+    if (m.bin_type == 0) {
+      t.split(r.leaf, f, m.real_index, (uint32_t)r.threshold, c->upper[f][r.threshold], r.left_output,
+              r.right_output, r.left_count, r.right_count, r.lsh, r.rsh, gain, m.missing_type, r.default_left != 0);
+    } else {
+      std::vector<uint32_t> bits_in, bits;
+      int maxw = -1, maxc = -1;
+      for (int b = 0; b < kMaxBins; ++b)
+        if ((r.cat_bits[b >> 5] >> (b & 31)) & 1u) {
+          maxw = std::max(maxw, b / 32);
+          maxc = std::max(maxc, (int)c->upper[f][b] / 32);
+        }
+      bits_in.assign(maxw + 1, 0);
+      bits.assign(maxc + 1, 0);
+      for (int b = 0; b < kMaxBins; ++b)
+        if ((r.cat_bits[b >> 5] >> (b & 31)) & 1u) {
+          bits_in[b / 32] |= 1u << (b % 32);
+          int cat = (int)c->upper[f][b];
+          bits[cat / 32] |= 1u << (cat % 32);
+        }
+      t.split(r.leaf, f, m.real_index, (uint32_t)t.num_cat, (double)t.num_cat, r.left_output, r.right_output,
+              r.left_count, r.right_count, r.lsh, r.rsh, gain, m.missing_type, false);
+      t.decision_type.back() |= 1;
+      t.num_cat++;
+      t.cat_boundaries.push_back(t.cat_boundaries.back() + (int)bits.size());
+      t.cat_threshold.insert(t.cat_threshold.end(), bits.begin(), bits.end());
+      t.cat_boundaries_inner.push_back(t.cat_boundaries_inner.back() + (int)bits_in.size());
+      t.cat_threshold_inner.insert(t.cat_threshold_inner.end(), bits_in.begin(), bits_in.end());
+    }
+  }
+  if (c->opts.use_quantized_grad && c->opts.quant_train_renew_leaf)
+    for (int l = 0; l < t.num_leaves; ++l) t.leaf_value[l] = lv[l].output;
+  return t;
+}
+
+static void ensure_graph(Ctx* c, int k, bool score_update, double lr) {
+  if (!c->opts.use_graph || c->profile) {
+    enqueue_tree(c, k, score_update, lr);
+    return;
+  }
+  if (!c->graphs[k]) {
+    hipGraph_t g;
+    HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_tree(c, k, score_update, lr);
+    HIP_CHECK(hipStreamEndCapture(c->stream, &g));
+    HIP_CHECK(hipGraphInstantiate(&c->graphs[k], g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+  }
+  HIP_CHECK(hipGraphLaunch(c->graphs[k], c->stream));
+}
+
+// objective/binary.rs:129-154 and multiclass.rs:156-158 (sequential sums).
+static double boost_from_score(Ctx* c, int k) {
+  const auto& lab = c->label;
+  const int64_t n = (int64_t)lab.size();
+  if (c->opts.objective == 0) {
+    double suml = 0.0;
+    for (int64_t i = 0; i < n; ++i) suml += (double)(int)(lab[i] > 0.0f);
+    double pavg = suml / (double)n;
+    const double eps = (double)1e-15f;
+    pavg = std::min(pavg, 1.0 - eps);
+    pavg = std::max(pavg, eps);
+    return std::log(pavg / ((double)1.0f - pavg)) / c->opts.sigmoid;
+  }
+  if (c->opts.objective == 1) {
+    double cnt = 0.0;
+    for (int64_t i = 0; i < n; ++i) cnt += (int)lab[i] == k ? 1.0 : 0.0;
+    return std::log(std::max((double)1e-15f, cnt / (double)n));
+  }
+  return 0.0;
+}
+
+__global__ void k_add_const(double* s, long long n, double v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) s[i] += v;
+}
+
+__global__ void k_absmax(const float* g, const float* h, long long n, IterDev* it) {
+  unsigned mg = 0, mh = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    mg = max(mg, __float_as_uint(fabsf(g[i])));
+    mh = max(mh, __float_as_uint(fabsf(h[i])));
+  }
+  mg = wave_max_u(mg);
+  mh = wave_max_u(mh);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&it->max_abs_g_bits, mg);
+    atomicMax(&it->max_abs_h_bits, mh);
+  }
+}
+
+static void enqueue_gradients(Ctx* c) {
+  hipStream_t s = c->stream;
+  const int64_t N = c->N;
+  hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(64), 0, s, c->it.p, c->K);
+  const int grid = grid_for(N, 256, c->cus * 8);
+  const float* w = c->has_weight ? c->d_weight.p : nullptr;
+  if (c->opts.objective == 0) {
+    hipLaunchKernelGGL(k_binary_grad, dim3(grid), dim3(256), 0, s, c->score.p, c->d_label.p, w, (long long)N,
+                       c->opts.sigmoid, 1.0, 1.0, c->g.p, c->h.p, c->it.p);
+  } else {
+    hipLaunchKernelGGL(k_multiclass_grad, dim3(grid), dim3(256), 0, s, c->score.p, c->d_label.p, w, (long long)N,
+                       c->K, c->g.p, c->h.p, c->it.p);
+  }
+}
+
+// Algorithmic bytes of one histogram build over m rows (row-index read
+// skipped at the root, whose rows are contiguous): m * (F + idx + gh) + F*B*out.
+static double hist_bytes(Ctx* c, int64_t m, bool root) {
+  const bool quant = c->opts.use_quantized_grad != 0;
+  return (double)m * ((double)c->F + (root ? 0.0 : 4.0) + (quant ? 2.0 : 8.0)) +
+         (double)c->F * c->B * (quant ? 8.0 : 16.0);
+}
+
+// Per-kind device time of the tree just grown; histogram launches that did
+// work (the root and every non-skipped step) carry the rows they summed.
+static void account_profile(Ctx* c, int k, const HostTree& t) {
+  const StepDev& st = *c->h_st[k];
+  auto el = [&](hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    return (double)ms;
+  };
+  c->prof[0] += el(pevent(c, k, 0, 0, 0), pevent(c, k, 0, 0, 1));
+  c->prof[1] += 1;
+  c->prof[2] += (double)c->N;
+  c->prof[3] += hist_bytes(c, c->N, true);
+  c->prof[4] += el(pevent(c, k, 0, 0, 1), pevent(c, k, 0, 1, 1));
+  std::vector<int> depth(c->L, 0);
+  for (int i = 0; i < st.num_splits; ++i) {
+    const SplitRec& r = c->h_log[k][i];
+    const int step = i + 1;
+    c->prof[5] += el(pevent(c, k, step, 2, 0), pevent(c, k, step, 2, 1));
+    c->prof[4] += el(pevent(c, k, step, 0, 1), pevent(c, k, step, 1, 1));
+    const int d = depth[r.leaf] + 1;
+    depth[r.leaf] = d;
+    depth[i + 1] = d;
+    const bool skip = (c->sp.max_depth > 0 && d >= c->sp.max_depth) ||
+                      (r.right_count < 2LL * c->sp.min_data_in_leaf && r.left_count < 2LL * c->sp.min_data_in_leaf);
+    if (skip) continue;
+    const int64_t m = std::min(r.left_count, r.right_count);
+    c->prof[0] += el(pevent(c, k, step, 2, 1), pevent(c, k, step, 0, 1));
+    c->prof[1] += 1;
+    c->prof[2] += (double)m;
+    c->prof[3] += hist_bytes(c, m, false);
+  }
+  (void)t;
+}
+
+// boosting.rs:957-1139 for device=gpu.
+static int train_iteration(Ctx* c) {
+  if (!c->state_ready) setup_training(c);
+  set_device(c);
+  const int K = c->K;
+  const int64_t N = c->N;
+  std::vector<double> init(K, 0.0);
+  if (c->trees.empty() && c->opts.boost_from_average) {
+    for (int k = 0; k < K; ++k) {
+      double v = boost_from_score(c, k);
+      if (std::fabs(v) > (double)1e-15f) {
+        hipLaunchKernelGGL(k_add_const, dim3(grid_for(N, 256, 2048)), dim3(256), 0, c->stream,
+                           c->score.p + (size_t)k * N, (long long)N, v);
+        init[k] = v;
+      }
+    }
+  }
+  if (!c->ev0) {
+    HIP_CHECK(hipEventCreate(&c->ev0));
+    HIP_CHECK(hipEventCreate(&c->ev1));
+  }
+  if (c->opts.use_quantized_grad && c->n_calls + K > (int)c->starts.n)
+    throw Error(LGBMHIP_UNSUPPORTED, "more quantized trees than the start-offset table holds (65536)");
+  HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+  enqueue_gradients(c);
+  for (int k = 0; k < K; ++k) {
+    ensure_graph(c, k, true, c->opts.learning_rate);
+    HIP_CHECK(hipGetLastError());
+    copy_tree_results(c, k);
+    if (c->opts.use_quantized_grad) c->n_calls++;
+  }
+  HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->last_ms = ms;
+  bool cont = false;
+  std::vector<HostTree> made;
+  if (c->profile) {
+    c->prof[6] += ms;
+    c->prof[7] += 1;
+  }
+  for (int k = 0; k < K; ++k) {
+    HostTree t = build_tree(c, k);
+    if (c->profile) account_profile(c, k, t);
+    if (t.num_leaves > 1) {
+      cont = true;
+      t.shrink(c->opts.learning_rate);
+      if (std::fabs(init[k]) > (double)1e-15f) t.add_bias(init[k]);
+    } else if ((int)c->trees.size() < K) {
+      t.as_constant(init[k], (int)N);
+    } else {
+      t.as_constant(0.0, (int)N);
+    }
+    made.push_back(t);
+  }
+  for (auto& t : made) c->trees.push_back(t);
+  c->pred_dirty = true;
+  if (!cont) {
+    if ((int)c->trees.size() > K) c->trees.resize(c->trees.size() - K);
+    return 1;
+  }
+  c->iter++;
+  return 0;
+}
+
+// One tree from caller gradients (SerialTreeLearner::train), no score update.
+static void train_tree(Ctx* c, const float* g, const float* h) {
+  if (!c->state_ready) setup_training(c);
+  set_device(c);
+  const int64_t N = c->N;
+  HIP_CHECK(hipMemcpyAsync(c->g.p, g, N * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->h.p, h, N * 4, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(64), 0, c->stream, c->it.p, c->K);
+  hipLaunchKernelGGL(k_absmax, dim3(grid_for(N, 256, c->cus * 8)), dim3(256), 0, c->stream, c->g.p, c->h.p,
+                     (long long)N, c->it.p);
+  if (c->opts.use_quantized_grad && c->n_calls + 1 > (int)c->starts.n)
+    throw Error(LGBMHIP_UNSUPPORTED, "more quantized trees than the start-offset table holds (65536)");
+  enqueue_tree(c, 0, false, 1.0);
+  HIP_CHECK(hipGetLastError());
+  copy_tree_results(c, 0);
+  if (c->opts.use_quantized_grad) c->n_calls++;
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->trees.push_back(build_tree(c, 0));
+  c->pred_dirty = true;
+}
+
+// ---------------------------------------------------------------- prediction
+static void build_pred(Ctx* c, bool binned) {
+  (void)binned;
+  std::vector<PredNode> nodes;
+  std::vector<int> noff, loff, nl;
+  std::vector<double> lv;
+  std::vector<unsigned> cats;
+  for (const auto& t : c->trees) {
+    noff.push_back((int)nodes.size());
+    loff.push_back((int)lv.size());
+    nl.push_back(t.num_leaves);
+    for (int i = 0; i < t.num_leaves - 1; ++i) {
+      PredNode n{};
+      n.threshold = t.threshold[i];
+      n.feature = t.split_feature[i];
+      n.left = t.left_child[i];
+      n.right = t.right_child[i];
+      n.thr_bin = (int)t.threshold_in_bin[i];
+      n.missing_type = (int8_t)((t.decision_type[i] >> 2) & 3);
+      n.default_left = (int8_t)((t.decision_type[i] & 2) ? 1 : 0);
+      n.is_cat = (int8_t)(t.decision_type[i] & 1);
+      n.cat_off = -1;
+      n.cat_len = 0;
+      const int fi = t.split_feature_inner[i];
+      n.default_bin = c->metas[fi].default_bin;
+      n.nan_bin = c->metas[fi].num_bin - 1;
+      n.pad = 0;
+      nodes.push_back(n);
+    }
+    for (int l = 0; l < t.num_leaves; ++l) lv.push_back(t.leaf_value[l]);
+  }
+  c->p_T = (int)c->trees.size();
+  c->p_nodes.alloc(std::max<size_t>(nodes.size(), 1));
+  c->p_node_off.alloc(std::max<size_t>(noff.size(), 1));
+  c->p_leaf_off.alloc(std::max<size_t>(loff.size(), 1));
+  c->p_nleaves.alloc(std::max<size_t>(nl.size(), 1));
+  c->p_leafval.alloc(std::max<size_t>(lv.size(), 1));
+  if (!nodes.empty()) HIP_CHECK(hipMemcpy(c->p_nodes.p, nodes.data(), nodes.size() * sizeof(PredNode), hipMemcpyHostToDevice));
+  if (!noff.empty()) {
+    HIP_CHECK(hipMemcpy(c->p_node_off.p, noff.data(), noff.size() * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->p_leaf_off.p, loff.data(), loff.size() * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->p_nleaves.p, nl.data(), nl.size() * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->p_leafval.p, lv.data(), lv.size() * 8, hipMemcpyHostToDevice));
+  }
+  c->pred_dirty = false;
+}
+
+static void predict(Ctx* c, const float* X, const uint8_t* bins, int64_t rows, int ncol, double* out) {
+  set_device(c);
+  for (const auto& t : c->trees)
+    if (t.num_cat > 0) throw Error(LGBMHIP_UNSUPPORTED, "prediction of categorical splits");
+  if (c->pred_dirty) build_pred(c, bins != nullptr);
+  PredArgs a{};
+  a.nodes = c->p_nodes.p;
+  a.tree_node_off = c->p_node_off.p;
+  a.tree_leaf_off = c->p_leaf_off.p;
+  a.tree_num_leaves = c->p_nleaves.p;
+  a.leaf_val = c->p_leafval.p;
+  a.cat_words = nullptr;
+  a.T = c->p_T;
+  a.K = c->K;
+  a.rows = rows;
+  a.ncol = ncol;
+  a.F = c->F;
+  DevBuf<float> dx;
+  DevBuf<uint8_t> db;
+  DevBuf<double> dout;
+  dout.alloc((size_t)rows * c->K);
+  if (X) {
+    dx.alloc((size_t)rows * ncol);
+    HIP_CHECK(hipMemcpyAsync(dx.p, X, (size_t)rows * ncol * 4, hipMemcpyHostToDevice, c->stream));
+    a.X = dx.p;
+  } else {
+    db.alloc((size_t)rows * c->F);
+    HIP_CHECK(hipMemcpyAsync(db.p, bins, (size_t)rows * c->F, hipMemcpyHostToDevice, c->stream));
+    a.bins = db.p;
+  }
+  a.out = dout.p;
+  const int grid = (int)((rows + 255) / 256);
+  if (rows > 0) {
+    if (X) hipLaunchKernelGGL(k_predict_raw, dim3(grid), dim3(256), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_predict_binned, dim3(grid), dim3(256), 0, c->stream, a);
+  }
+  HIP_CHECK(hipMemcpyAsync(out, dout.p, (size_t)rows * c->K * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace lgbmhip
+
+// ==================================================================== C ABI
+using namespace lgbmhip;
+
+struct lgbmhip_ctx {
+  Ctx c;
+};
+struct lgbmhip_bins {
+  std::unique_ptr<Binned> b;
+};
+
+static thread_local std::string g_err;
+
+template <class F>
+static lgbmhip_status guarded(lgbmhip_ctx* ctx, F&& fn) {
+  try {
+    fn();
+    return LGBMHIP_OK;
+  } catch (const Error& e) {
+    if (ctx) ctx->c.err = e.what();
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->c.err = std::string("internal error: ") + e.what();
+    g_err = std::string("internal error: ") + e.what();
+    return LGBMHIP_INTERNAL;
+  } catch (...) {
+    if (ctx) ctx->c.err = "internal error";
+    g_err = "internal error";
+    return LGBMHIP_INTERNAL;
+  }
+}
+
+extern "C" {
+
+const char* lgbmhip_last_error(const lgbmhip_ctx* ctx) { return ctx ? ctx->c.err.c_str() : g_err.c_str(); }
+const char* lgbmhip_global_last_error(void) { return g_err.c_str(); }
+
+lgbmhip_status lgbmhip_bin_dense(const float* X, int64_t n, int32_t ncol, const int32_t* categorical, int32_t n_cat,
+                                 const lgbmhip_bin_params* params, lgbmhip_bins** out) {
+  return guarded(nullptr, [&] {
+    if (!X || !params || !out) throw Error(LGBMHIP_INVALID_PARAMETER, "null argument");
+    auto* b = new lgbmhip_bins();
+    try {
+      b->b.reset(bin_dense(X, n, ncol, categorical, n_cat, *params));
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+void lgbmhip_bins_free(lgbmhip_bins* b) { delete b; }
+int32_t lgbmhip_bins_num_features(const lgbmhip_bins* b) { return b ? (int32_t)b->b->used.size() : 0; }
+int64_t lgbmhip_bins_num_upper(const lgbmhip_bins* b) {
+  int64_t s = 0;
+  for (int c : b->b->used) s += b->b->mappers[c].num_bin;
+  return s;
+}
+lgbmhip_status lgbmhip_bins_metas(const lgbmhip_bins* b, lgbmhip_feature_meta* metas, double* upper) {
+  return guarded(nullptr, [&] {
+    size_t off = 0;
+    for (size_t f = 0; f < b->b->used.size(); ++f) {
+      const Mapper& m = b->b->mappers[b->b->used[f]];
+      metas[f] = lgbmhip_feature_meta{b->b->used[f], m.num_bin, m.missing_type, (int32_t)m.default_bin,
+                                      (int32_t)m.most_freq_bin, m.bin_type};
+      for (int i = 0; i < m.num_bin; ++i) upper[off + i] = m.bin_type == 1 ? (double)m.categories[i] : m.bounds[i];
+      off += (size_t)m.num_bin;
+    }
+  });
+}
+lgbmhip_status lgbmhip_bins_matrix(const lgbmhip_bins* b, uint8_t* out) {
+  return guarded(nullptr, [&] { std::memcpy(out, b->b->bins.data(), b->b->bins.size()); });
+}
+lgbmhip_status lgbmhip_bins_apply(const lgbmhip_bins* b, const float* X, int64_t n, int32_t ncol, uint8_t* out) {
+  return guarded(nullptr, [&] { apply_bins(*b->b, X, n, ncol, out); });
+}
+
+lgbmhip_status lgbmhip_create(int32_t device, lgbmhip_ctx** out) {
+  return guarded(nullptr, [&] {
+    int n = 0;
+    HIP_CHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw Error(LGBMHIP_INVALID_PARAMETER, "no such HIP device");
+    auto* x = new lgbmhip_ctx();
+    x->c.device = device;
+    try {
+      HIP_CHECK(hipSetDevice(device));
+      HIP_CHECK(hipStreamCreateWithFlags(&x->c.stream, hipStreamNonBlocking));
+      hipDeviceProp_t prop;
+      HIP_CHECK(hipGetDeviceProperties(&prop, device));
+      x->c.cus = prop.multiProcessorCount;
+      x->c.lds_max = prop.sharedMemPerBlock;
+      if (prop.maxSharedMemoryPerMultiProcessor > 0)
+        x->c.lds_max = std::min<size_t>(std::max<size_t>(x->c.lds_max, 65536), prop.maxSharedMemoryPerMultiProcessor);
+    } catch (...) {
+      delete x;
+      throw;
+    }
+    *out = x;
+  });
+}
+void lgbmhip_destroy(lgbmhip_ctx* ctx) { delete ctx; }
+
+lgbmhip_status lgbmhip_upload_bins(lgbmhip_ctx* ctx, const lgbmhip_feature_meta* metas, int32_t num_features,
+                                   const double* upper, const uint8_t* bins, int64_t num_rows,
+                                   int32_t num_total_features) {
+  return guarded(ctx, [&] { upload_bins(&ctx->c, metas, num_features, upper, bins, num_rows, num_total_features); });
+}
+
+lgbmhip_status lgbmhip_set_metadata(lgbmhip_ctx* ctx, const float* label, const float* weight,
+                                    const int32_t* query_boundaries, int32_t num_queries) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (!c->N) throw Error(LGBMHIP_INVALID_DATA, "upload bins first");
+    set_device(c);
+    c->label.assign(label, label + c->N);
+    c->d_label.alloc(c->N);
+    HIP_CHECK(hipMemcpy(c->d_label.p, label, c->N * 4, hipMemcpyHostToDevice));
+    c->has_weight = weight != nullptr;
+    if (weight) {
+      c->d_weight.alloc(c->N);
+      HIP_CHECK(hipMemcpy(c->d_weight.p, weight, c->N * 4, hipMemcpyHostToDevice));
+    }
+    if (query_boundaries) c->qb.assign(query_boundaries, query_boundaries + num_queries + 1);
+    c->state_ready = false;
+  });
+}
+
+lgbmhip_status lgbmhip_set_split_params(lgbmhip_ctx* ctx, const lgbmhip_split_params* p) {
+  return guarded(ctx, [&] {
+    if (p->num_leaves < 2 || p->num_leaves > 131072) throw Error(LGBMHIP_INVALID_PARAMETER, "num_leaves out of range");
+    if (p->min_data_in_leaf < 0) throw Error(LGBMHIP_INVALID_PARAMETER, "min_data_in_leaf < 0");
+    SplitParamsDev d{p->lambda_l1, p->lambda_l2, p->max_delta_step, p->path_smooth, p->min_sum_hessian_in_leaf,
+                     p->min_gain_to_split, p->cat_l2, p->cat_smooth, p->min_data_in_leaf, p->max_cat_to_onehot,
+                     p->max_cat_threshold, p->min_data_per_group, p->num_leaves, p->max_depth};
+    ctx->c.sp = d;
+    ctx->c.sp_set = true;
+    ctx->c.state_ready = false;
+  });
+}
+
+lgbmhip_status lgbmhip_set_train_opts(lgbmhip_ctx* ctx, const lgbmhip_train_opts* o) {
+  return guarded(ctx, [&] {
+    if (o->objective < 0 || o->objective > 2) throw Error(LGBMHIP_INVALID_PARAMETER, "unknown objective");
+    if (o->sigmoid <= 0.0) throw Error(LGBMHIP_INVALID_PARAMETER, "sigmoid must be > 0");
+    ctx->c.opts = *o;
+    ctx->c.opts_set = true;
+    ctx->c.state_ready = false;
+  });
+}
+
+lgbmhip_status lgbmhip_train_iteration(lgbmhip_ctx* ctx, int32_t* finished) {
+  return guarded(ctx, [&] { *finished = train_iteration(&ctx->c); });
+}
+lgbmhip_status lgbmhip_train_tree(lgbmhip_ctx* ctx, const float* grad, const float* hess) {
+  return guarded(ctx, [&] { train_tree(&ctx->c, grad, hess); });
+}
+lgbmhip_status lgbmhip_sync(lgbmhip_ctx* ctx) {
+  return guarded(ctx, [&] { HIP_CHECK(hipStreamSynchronize(ctx->c.stream)); });
+}
+int32_t lgbmhip_num_trees(lgbmhip_ctx* ctx) { return (int32_t)ctx->c.trees.size(); }
+int32_t lgbmhip_tree_num_leaves(lgbmhip_ctx* ctx, int32_t t) {
+  if (t < 0 || t >= (int32_t)ctx->c.trees.size()) return -1;
+  return ctx->c.trees[t].num_leaves;
+}
+lgbmhip_status lgbmhip_tree_arrays(lgbmhip_ctx* ctx, int32_t t, int32_t* split_feature, uint32_t* threshold_in_bin,
+                                   double* threshold, int8_t* decision_type, int32_t* left_child, int32_t* right_child,
+                                   float* split_gain, double* leaf_value, int32_t* leaf_count, double* leaf_weight,
+                                   double* internal_value) {
+  return guarded(ctx, [&] {
+    if (t < 0 || t >= (int32_t)ctx->c.trees.size()) throw Error(LGBMHIP_INVALID_PARAMETER, "tree index out of range");
+    const HostTree& tr = ctx->c.trees[t];
+    for (int i = 0; i < tr.num_leaves - 1; ++i) {
+      split_feature[i] = tr.split_feature[i];
+      threshold_in_bin[i] = tr.threshold_in_bin[i];
+      threshold[i] = tr.threshold[i];
+      decision_type[i] = tr.decision_type[i];
+      left_child[i] = tr.left_child[i];
+      right_child[i] = tr.right_child[i];
+      split_gain[i] = tr.split_gain[i];
+      internal_value[i] = tr.internal_value[i];
+    }
+    for (int i = 0; i < tr.num_leaves; ++i) {
+      leaf_value[i] = tr.leaf_value[i];
+      leaf_count[i] = tr.leaf_count[i];
+      leaf_weight[i] = tr.leaf_weight[i];
+    }
+  });
+}
+lgbmhip_status lgbmhip_leaf_of_row(lgbmhip_ctx* ctx, int32_t* out) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (!c->state_ready) throw Error(LGBMHIP_INVALID_PARAMETER, "no tree trained");
+    set_device(c);
+    DevBuf<int> d;
+    d.alloc(c->N);
+    hipLaunchKernelGGL(k_leaf_of_row, dim3(64, c->L), dim3(256), 0, c->stream, c->st.p, c->leaves.p, c->idx[0].p,
+                       c->idx[1].p, d.p);
+    HIP_CHECK(hipMemcpyAsync(out, d.p, c->N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+lgbmhip_status lgbmhip_copy_scores(lgbmhip_ctx* ctx, double* out) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (!c->state_ready) setup_training(c);
+    HIP_CHECK(hipMemcpyAsync(out, c->score.p, c->score.n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+lgbmhip_status lgbmhip_copy_gradients(lgbmhip_ctx* ctx, float* grad, float* hess) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (!c->state_ready) throw Error(LGBMHIP_INVALID_PARAMETER, "no training state");
+    HIP_CHECK(hipMemcpyAsync(grad, c->g.p, c->g.n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(hess, c->h.p, c->h.n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+lgbmhip_status lgbmhip_quant_state(lgbmhip_ctx* ctx, int8_t* gq, int8_t* hq, double* scales) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (!c->state_ready || !c->opts.use_quantized_grad) throw Error(LGBMHIP_INVALID_PARAMETER, "no quantized state");
+    HIP_CHECK(hipMemcpyAsync(gq, c->gq.p, c->N, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(hq, c->hq.p, c->N, hipMemcpyDeviceToHost, c->stream));
+    IterDev it;
+    HIP_CHECK(hipMemcpyAsync(&it, c->it.p, sizeof(IterDev), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    scales[0] = it.grad_scale;
+    scales[1] = it.hess_scale;
+  });
+}
+
+lgbmhip_status lgbmhip_gradients(lgbmhip_ctx* ctx, const double* scores, float* grad, float* hess) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (!c->state_ready) setup_training(c);
+    set_device(c);
+    HIP_CHECK(hipMemcpyAsync(c->score.p, scores, c->score.n * 8, hipMemcpyHostToDevice, c->stream));
+    enqueue_gradients(c);
+    HIP_CHECK(hipMemcpyAsync(grad, c->g.p, c->g.n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(hess, c->h.p, c->h.n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+lgbmhip_status lgbmhip_discretize(lgbmhip_ctx* ctx, const float* grad, const float* hess, int32_t num_calls,
+                                  int8_t* gq, int8_t* hq, double* scales) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (!c->opts.use_quantized_grad) throw Error(LGBMHIP_INVALID_PARAMETER, "use_quantized_grad is off");
+    if (!c->state_ready) setup_training(c);
+    set_device(c);
+    const int64_t N = c->N;
+    if (num_calls < 1 || c->n_calls + num_calls > (int)c->starts.n)
+      throw Error(LGBMHIP_INVALID_PARAMETER, "num_calls out of range of the start-offset table");
+    HIP_CHECK(hipMemcpyAsync(c->g.p, grad, N * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->h.p, hess, N * 4, hipMemcpyHostToDevice, c->stream));
+    for (int call = 0; call < num_calls; ++call) {
+      hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(64), 0, c->stream, c->it.p, 1);
+      hipLaunchKernelGGL(k_absmax, dim3(grid_for(N, 256, c->cus * 8)), dim3(256), 0, c->stream, c->g.p, c->h.p,
+                         (long long)N, c->it.p);
+      hipLaunchKernelGGL(k_discretize, dim3(grid_for(N, 256, c->cus * 8)), dim3(256), 0, c->stream, (long long)N,
+                         c->g.p, c->h.p, c->grv.p, c->hrv.p, c->opts.num_grad_quant_bins, c->opts.stochastic_rounding,
+                         0, c->starts.p, c->call_ctr.p, c->it.p, c->idx[0].p, (char2*)c->gh[0].p, c->gq.p, c->hq.p,
+                         c->st.p, c->leaves.p, c->best.p, c->spl.p, c->F, c->L);
+      hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, c->stream, c->call_ctr.p);
+      c->n_calls++;
+    }
+    HIP_CHECK(hipMemcpyAsync(gq, c->gq.p, N, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(hq, c->hq.p, N, hipMemcpyDeviceToHost, c->stream));
+    IterDev it;
+    HIP_CHECK(hipMemcpyAsync(&it, c->it.p, sizeof(IterDev), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    scales[0] = it.grad_scale;
+    scales[1] = it.hess_scale;
+  });
+}
+
+static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc, const void* hsrc, bool quant,
+                       void* out, int32_t* out_B) {
+  if (!c->state_ready) setup_training(c);
+  set_device(c);
+  if (quant != (c->opts.use_quantized_grad != 0)) throw Error(LGBMHIP_INVALID_PARAMETER, "histogram kind differs from train opts");
+  const int64_t N = c->N;
+  if (m < 0 || m > N) throw Error(LGBMHIP_INVALID_DATA, "row count out of range");
+  for (int64_t i = 0; i < m; ++i)
+    if (rows[i] >= (uint64_t)N) throw Error(LGBMHIP_INVALID_DATA, "row index out of range");
+  // pack (g, h) in the row order of `rows` into gh buffer 0 positions 0..m-1
+  std::vector<uint8_t> pk((size_t)std::max<int64_t>(m, 1) * (quant ? 2 : 8));
+  unsigned gbits = 0, hbits = 0;
+  for (int64_t i = 0; i < m; ++i) {
+    const uint32_t r = rows[i];
+    if (quant) {
+      pk[2 * i] = ((const uint8_t*)gsrc)[r];
+      pk[2 * i + 1] = ((const uint8_t*)hsrc)[r];
+    } else {
+      float gv = ((const float*)gsrc)[r], hv = ((const float*)hsrc)[r];
+      std::memcpy(&pk[8 * i], &gv, 4);
+      std::memcpy(&pk[8 * i + 4], &hv, 4);
+      unsigned a, b;
+      float ga = std::fabs(gv), ha = std::fabs(hv);
+      std::memcpy(&a, &ga, 4);
+      std::memcpy(&b, &ha, 4);
+      gbits = std::max(gbits, a);
+      hbits = std::max(hbits, b);
+    }
+  }
+  if (m > 0) {
+    HIP_CHECK(hipMemcpyAsync(c->idx[0].p, rows, m * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->gh[0].p, pk.data(), pk.size(), hipMemcpyHostToDevice, c->stream));
+  }
+  c->acc.zero(c->stream);
+  HistArgs a = hist_args(c, quant);
+  a.direct = 1;
+  a.d_begin = 0;
+  a.d_count = m;
+  a.d_buf = 0;
+  a.d_shift_g = fix_shift(m, gbits);
+  a.d_shift_h = fix_shift(m, hbits);
+  if (m > 0) launch_hist(a, quant, c->hist_tpr, c->hist_chunks * c->hist_ng, c->hist_lds, c->stream);
+  HIP_CHECK(hipGetLastError());
+  const size_t cnt = (size_t)c->F * c->B * (quant ? 1 : 2);
+  std::vector<long long> acc(cnt);
+  HIP_CHECK(hipMemcpyAsync(acc.data(), c->acc.p, cnt * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->acc.zero(c->stream);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (quant) std::memcpy(out, acc.data(), cnt * 8);
+  else
+    for (size_t i = 0; i < cnt; ++i)
+      ((double*)out)[i] = std::ldexp((double)acc[i], -((i & 1) ? a.d_shift_h : a.d_shift_g));
+  *out_B = c->B;
+}
+
+lgbmhip_status lgbmhip_build_histogram(lgbmhip_ctx* ctx, const uint32_t* rows, int64_t m, const float* grad,
+                                       const float* hess, double* out, int32_t* out_num_bins) {
+  return guarded(ctx, [&] { stage_hist(&ctx->c, rows, m, grad, hess, false, out, out_num_bins); });
+}
+lgbmhip_status lgbmhip_build_histogram_int(lgbmhip_ctx* ctx, const uint32_t* rows, int64_t m, const int8_t* gq,
+                                           const int8_t* hq, int64_t* out, int32_t* out_num_bins) {
+  return guarded(ctx, [&] { stage_hist(&ctx->c, rows, m, gq, hq, true, out, out_num_bins); });
+}
+
+static void stage_search(Ctx* c, int f, const double* hist, const int64_t* ihist, LeafSums ls, bool quant,
+                         double* od, int32_t* oi, int64_t* og) {
+  if (f < 0 || f >= c->F) throw Error(LGBMHIP_INVALID_PARAMETER, "feature out of range");
+  if (!c->sp_set) throw Error(LGBMHIP_INVALID_PARAMETER, "split params not set");
+  set_device(c);
+  const int nb = c->feats[f].num_bin;
+  DevBuf<double> dh;
+  DevBuf<long long> dih;
+  DevBuf<SplitRec> dr;
+  DevBuf<int> ds;
+  dr.alloc(1);
+  ds.alloc(1);
+  if (quant) {
+    dih.alloc(nb);
+    HIP_CHECK(hipMemcpy(dih.p, ihist, nb * 8, hipMemcpyHostToDevice));
+  } else {
+    dh.alloc(2 * nb);
+    HIP_CHECK(hipMemcpy(dh.p, hist, nb * 16, hipMemcpyHostToDevice));
+  }
+  hipLaunchKernelGGL(k_search_one, dim3(1), dim3(64), 0, c->stream, dh.p, dih.p, c->feats[f], c->sp, ls, quant ? 1 : 0,
+                     dr.p, ds.p);
+  SplitRec r;
+  int spl = 0;
+  HIP_CHECK(hipMemcpyAsync(&r, dr.p, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(&spl, ds.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  od[0] = r.gain; od[1] = r.left_output; od[2] = r.right_output;
+  od[3] = r.lsg; od[4] = r.lsh; od[5] = r.rsg; od[6] = r.rsh;
+  oi[0] = r.threshold; oi[1] = r.default_left; oi[2] = r.left_count; oi[3] = r.right_count; oi[4] = spl;
+  if (og) { og[0] = r.lgh; og[1] = r.rgh; }
+}
+
+lgbmhip_status lgbmhip_find_best_split(lgbmhip_ctx* ctx, int32_t feature, const double* hist, double sum_g,
+                                       double sum_h, int32_t num_data, double parent_output, double* out_d,
+                                       int32_t* out_i) {
+  return guarded(ctx, [&] {
+    LeafSums ls{};
+    ls.sg = sum_g;
+    ls.sh = sum_h + 2.0 * (double)1e-15f;
+    ls.nd = num_data;
+    ls.po = parent_output;
+    stage_search(&ctx->c, feature, hist, nullptr, ls, false, out_d, out_i, nullptr);
+  });
+}
+lgbmhip_status lgbmhip_find_best_split_int(lgbmhip_ctx* ctx, int32_t feature, const int64_t* hist, int64_t int_sum,
+                                           double grad_scale, double hess_scale, int32_t num_data,
+                                           double parent_output, double* out_d, int32_t* out_i, int64_t* out_gh) {
+  return guarded(ctx, [&] {
+    LeafSums ls{};
+    ls.isum = int_sum;
+    ls.gs = grad_scale;
+    ls.hs = hess_scale;
+    ls.nd = num_data;
+    ls.po = parent_output;
+    stage_search(&ctx->c, feature, nullptr, hist, ls, true, out_d, out_i, out_gh);
+  });
+}
+
+lgbmhip_status lgbmhip_partition(lgbmhip_ctx* ctx, const uint32_t* rows, int64_t m, int32_t feature,
+                                 uint32_t threshold, int32_t default_left, uint32_t* out_rows, int64_t* out_left) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (feature < 0 || feature >= c->F) throw Error(LGBMHIP_INVALID_PARAMETER, "feature out of range");
+    set_device(c);
+    for (int64_t i = 0; i < m; ++i)
+      if (rows[i] >= (uint64_t)c->N) throw Error(LGBMHIP_INVALID_DATA, "row index out of range");
+    DevBuf<unsigned> dr;
+    DevBuf<uint8_t> flag;
+    dr.alloc(std::max<int64_t>(m, 1));
+    flag.alloc(std::max<int64_t>(m, 1));
+    if (m > 0) HIP_CHECK(hipMemcpyAsync(dr.p, rows, m * 4, hipMemcpyHostToDevice, c->stream));
+    SplitRec s{};
+    s.threshold = (int)threshold;
+    s.default_left = default_left;
+    if (m > 0)
+      hipLaunchKernelGGL(k_partition_rows, dim3((int)((m + 255) / 256)), dim3(256), 0, c->stream, dr.p, (long long)m,
+                         c->bins_cm.p + (size_t)feature * c->N, c->feats[feature], s, flag.p);
+    std::vector<uint8_t> fl(std::max<int64_t>(m, 1));
+    HIP_CHECK(hipMemcpyAsync(fl.data(), flag.p, std::max<int64_t>(m, 1), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    int64_t nl = 0;
+    for (int64_t i = 0; i < m; ++i) if (fl[i]) out_rows[nl++] = rows[i];
+    int64_t k = nl;
+    for (int64_t i = 0; i < m; ++i) if (!fl[i]) out_rows[k++] = rows[i];
+    *out_left = nl;
+  });
+}
+
+lgbmhip_status lgbmhip_predict_raw(lgbmhip_ctx* ctx, const float* X, int64_t rows, int32_t ncol, double* out) {
+  return guarded(ctx, [&] { predict(&ctx->c, X, nullptr, rows, ncol, out); });
+}
+lgbmhip_status lgbmhip_predict_binned(lgbmhip_ctx* ctx, const uint8_t* bins, int64_t rows, double* out) {
+  return guarded(ctx, [&] { predict(&ctx->c, nullptr, bins, rows, 0, out); });
+}
+lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out8) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (out8) for (int i = 0; i < 8; ++i) out8[i] = c->prof[i];
+    if (enable >= 0) {
+      c->profile = enable;
+      for (int i = 0; i < 8; ++i) c->prof[i] = 0.0;
+      if (enable && c->pev.empty()) {
+        set_device(c);
+        if (!c->state_ready) setup_training(c);
+        c->pev.resize((size_t)c->K * c->L * 3 * 2);
+        for (auto& e : c->pev) HIP_CHECK(hipEventCreate(&e));
+      }
+    }
+  });
+}
+
+lgbmhip_status lgbmhip_last_timings(lgbmhip_ctx* ctx, double* out4) {
+  return guarded(ctx, [&] {
+    out4[0] = ctx->c.last_ms;
+    out4[1] = out4[2] = out4[3] = 0.0;
+  });
+}
+
+}  // extern "C"

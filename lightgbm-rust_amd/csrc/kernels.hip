@@ -1,0 +1,1626 @@
+// CDNA4 (gfx950) kernels of the GBDT hot path.
+//
+// Tree growth runs as a device-resident state machine (StepDev): every
+// kernel reads the leaf it works on, its row segment and the split to apply
+// from device memory, so one tree is a fixed sequence of launches with no
+// host round trip (and can be captured in a hipGraph):
+//
+//   tree_init -> hist -> split(root) -> { part_count -> part_scatter -> hist -> split } x (num_leaves-1)
+//             -> [renew] -> leaf_values -> score
+//
+// Determinism: FP64 histograms are accumulated as exact 64-bit fixed-point
+// integers (pre-rounding each f32 gradient to a power-of-two grid sized from
+// the leaf's row count and the iteration's max |g|, after Demmel & Nguyen's
+// reproducible-summation idea), so per-bin sums are independent of block
+// count, tiling and atomic order; the quantized form sums packed int8
+// gradients exactly. Threshold-search prefix sums run in the reference's
+// sequential order (FP64, one chain per wave), so identical histograms give
+// identical splits.
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "engine.h"
+
+namespace lgbmhip {
+
+#define KEPS ((double)1e-15f)
+#define KZERO ((double)1e-35f)
+
+__device__ __forceinline__ double round0(double v) { return (v >= -KZERO && v <= KZERO) ? 0.0 : v; }
+__device__ __forceinline__ int round_int(double x) { return (int)(x + (double)0.5f); }
+__device__ __forceinline__ double sgn(double x) { return (double)((x > 0.0) - (x < 0.0)); }
+__device__ __forceinline__ double thr_l1(double s, double l1) { return sgn(s) * fmax(fabs(s) - l1, 0.0); }
+
+// split.rs:137-161
+__device__ double leaf_output_t(double g, double h, const SplitParamsDev& p, bool use_l1, bool use_max,
+                                bool use_smooth, int nd, double po) {
+  double ret = use_l1 ? -thr_l1(g, p.lambda_l1) / (h + p.lambda_l2) : -g / (h + p.lambda_l2);
+  if (use_max && p.max_delta_step > 0.0 && fabs(ret) > p.max_delta_step) ret = sgn(ret) * p.max_delta_step;
+  if (use_smooth) {
+    double n = (double)nd;
+    ret = ret * (n / p.path_smooth) / (n / p.path_smooth + 1.0) + po / (n / p.path_smooth + 1.0);
+  }
+  return ret;
+}
+__device__ __forceinline__ double leaf_output(double g, double h, const SplitParamsDev& p, int nd, double po) {
+  return leaf_output_t(g, h, p, p.lambda_l1 > 0.0, p.max_delta_step > 0.0, p.path_smooth > KEPS, nd, po);
+}
+// split.rs:183-201
+__device__ double leaf_gain(double g, double h, const SplitParamsDev& p, int nd, double po) {
+  bool l1 = p.lambda_l1 > 0.0;
+  if (!(p.max_delta_step > 0.0) && !(p.path_smooth > KEPS)) {
+    if (l1) {
+      double s = thr_l1(g, p.lambda_l1);
+      return (s * s) / (h + p.lambda_l2);
+    }
+    return (g * g) / (h + p.lambda_l2);
+  }
+  double out = leaf_output(g, h, p, nd, po);
+  double s = l1 ? thr_l1(g, p.lambda_l1) : g;
+  return -(2.0 * s * out + (h + p.lambda_l2) * out * out);
+}
+__device__ __forceinline__ double split_gain(double lg, double lh, double rg, double rh, const SplitParamsDev& p,
+                                             int lc, int rc, double po) {
+  return leaf_gain(lg, lh, p, lc, po) + leaf_gain(rg, rh, p, rc, po);
+}
+
+// Fixed-point exponent for summing `count` values of magnitude < 2^e (e from
+// the max's float bits) into int64 without overflow: |sum * 2^s| < 2^62.
+__device__ __host__ __forceinline__ int fix_shift(long long count, unsigned max_bits) {
+  int r = 1;
+  while ((1LL << r) <= count && r < 62) ++r;   // 2^r > count
+  float m;
+  memcpy(&m, &max_bits, 4);
+  int e = 0;
+  if (m > 0.0f) {
+    int ex;
+    frexpf(m, &ex);  // m = f * 2^ex, f in [0.5, 1): m < 2^ex
+    e = ex;
+  }
+  return 62 - r - e;
+}
+__device__ __forceinline__ long long to_fixed(float v, int s) { return __double2ll_rn(ldexp((double)v, s)); }
+
+__device__ __forceinline__ int8_t to_i8(double x) {
+  if (isnan(x) || !(x >= -2147483648.0 && x < 2147483648.0)) return 0;
+  return (int8_t)(int32_t)x;
+}
+
+// A split is better: higher gain, then smaller feature index (split.rs:114-121).
+__device__ __forceinline__ bool better(double ga, int fa, double gb, int fb) {
+  if (ga != gb) return ga > gb;
+  int a = fa < 0 ? INT32_MAX : fa, b = fb < 0 ? INT32_MAX : fb;
+  return a < b;
+}
+
+__device__ __forceinline__ void reset_rec(SplitRec& r) {
+  r.gain = -INFINITY;
+  r.feature = -1;
+  r.threshold = 0;
+  r.default_left = 1;
+  r.left_output = r.right_output = 0.0;
+  r.lsg = r.lsh = r.rsg = r.rsh = 0.0;
+  r.lgh = r.rgh = 0;
+  r.left_count = r.right_count = 0;
+  r.num_cat = 0;
+  r.leaf = -1;
+  for (int i = 0; i < kCatWords; ++i) r.cat_bits[i] = 0;
+}
+
+// ---------------------------------------------------------------- block helpers
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ unsigned wave_max_u(unsigned v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+// Orders LDS traffic between the lanes of one wave (a wave-scope barrier;
+// the two waves of a search block never wait for each other).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Last block of the grid to arrive (release/acquire around a device counter).
+__device__ bool last_block(int* counter) {
+  __shared__ int ticket;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    ticket = atomicAdd(counter, 1);
+  }
+  __syncthreads();
+  bool last = ticket == (int)(gridDim.x * gridDim.y) - 1;
+  if (last) __threadfence();
+  return last;
+}
+
+// Advances the quantizer's call counter (one start offset per discretize call).
+__global__ void k_bump(int* p) { *p += 1; }
+
+// Clears the per-iteration reductions of K classes (a fill kernel, so the
+// iteration can be captured in a graph).
+__global__ void k_iter_reset(IterDev* it, int K) {
+  int k = threadIdx.x;
+  if (k < K) {
+    it[k].max_abs_g_bits = 0;
+    it[k].max_abs_h_bits = 0;
+    it[k].root_g_fix = 0;
+    it[k].root_h_fix = 0;
+    it[k].root_int_sum = 0;
+    it[k].root_gq_sum = 0;
+    it[k].root_hq_sum = 0;
+  }
+}
+
+// ---------------------------------------------------------------- objectives
+// objective/binary.rs:101-127. One row per thread; max |g|, |h| by block
+// reduction then one atomicMax per block (order-free).
+__global__ void k_binary_grad(const double* __restrict__ score, const float* __restrict__ label,
+                              const float* __restrict__ weight, long long n, double sigmoid, double lw_neg,
+                              double lw_pos, float* __restrict__ g, float* __restrict__ h, IterDev* it) {
+  unsigned mg = 0, mh = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    bool pos = label[i] > 0.0f;
+    double lab = pos ? 1.0 : -1.0;
+    double lw = pos ? lw_pos : lw_neg;
+    double resp = -lab * sigmoid / ((double)1.0f + exp(lab * sigmoid * score[i]));
+    double ar = fabs(resp);
+    float gv, hv;
+    if (weight) {
+      double w = (double)weight[i];
+      gv = (float)(resp * lw * w);
+      hv = (float)(ar * (sigmoid - ar) * lw * w);
+    } else {
+      gv = (float)(resp * lw);
+      hv = (float)(ar * (sigmoid - ar) * lw);
+    }
+    g[i] = gv;
+    h[i] = hv;
+    mg = max(mg, __float_as_uint(fabsf(gv)));
+    mh = max(mh, __float_as_uint(fabsf(hv)));
+  }
+  mg = wave_max_u(mg);
+  mh = wave_max_u(mh);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&it->max_abs_g_bits, mg);
+    atomicMax(&it->max_abs_h_bits, mh);
+  }
+}
+
+// objective/multiclass.rs:16-30, 126-154 (K <= 64 classes).
+__global__ void k_multiclass_grad(const double* __restrict__ score, const float* __restrict__ label,
+                                  const float* __restrict__ weight, long long n, int K, float* __restrict__ g,
+                                  float* __restrict__ h, IterDev* it) {
+  double factor = (double)K / (double)((float)K - 1.0f);
+  __shared__ unsigned smg[64], smh[64];
+  for (int c = threadIdx.x; c < K; c += blockDim.x) { smg[c] = 0; smh[c] = 0; }
+  __syncthreads();
+  double rec[64];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    double wmax = score[i];
+    for (int c = 1; c < K; ++c) {
+      double v = score[(long long)c * n + i];
+      wmax = v < wmax ? wmax : v;
+    }
+    double wsum = 0.0;
+    for (int c = 0; c < K; ++c) {
+      rec[c] = exp(score[(long long)c * n + i] - wmax);
+      wsum += rec[c];
+    }
+    int li = (int)label[i];
+    for (int c = 0; c < K; ++c) {
+      double p = rec[c] / wsum;
+      double gg = li == c ? p - (double)1.0f : p;
+      double hh = factor * p * ((double)1.0f - p);
+      if (weight) {
+        gg *= (double)weight[i];
+        hh *= (double)weight[i];
+      }
+      float gv = (float)gg, hv = (float)hh;
+      g[(long long)c * n + i] = gv;
+      h[(long long)c * n + i] = hv;
+      atomicMax(&smg[c], __float_as_uint(fabsf(gv)));
+      atomicMax(&smh[c], __float_as_uint(fabsf(hv)));
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < K; c += blockDim.x) {
+    atomicMax(&it[c].max_abs_g_bits, smg[c]);
+    atomicMax(&it[c].max_abs_h_bits, smh[c]);
+  }
+}
+
+// ---------------------------------------------------------------- tree init
+// Identity partition in buffer 0, gradients of class k copied beside the row
+// indices, fixed-point root sums, state machine reset (learner/mod.rs:580-628).
+__global__ void k_tree_init(long long n, const float* __restrict__ g, const float* __restrict__ h,
+                            unsigned* __restrict__ idx0, float2* __restrict__ gh0, IterDev* it, StepDev* st,
+                            LeafDev* leaves, SplitRec* best, uint8_t* splittable, int F, int L) {
+  int sg = fix_shift(n, it->max_abs_g_bits), sh = fix_shift(n, it->max_abs_h_bits);
+  long long ag = 0, ah = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    float gv = g[i], hv = h[i];
+    idx0[i] = (unsigned)i;
+    gh0[i] = make_float2(gv, hv);
+    ag += to_fixed(gv, sg);
+    ah += to_fixed(hv, sh);
+  }
+  ag = wave_sum(ag);
+  ah = wave_sum(ah);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd((unsigned long long*)&it->root_g_fix, (unsigned long long)ag);
+    atomicAdd((unsigned long long*)&it->root_h_fix, (unsigned long long)ah);
+  }
+  if (blockIdx.x == 0) {
+    for (int l = threadIdx.x; l < L; l += blockDim.x) reset_rec(best[l]);
+    for (int f = threadIdx.x; f < F; f += blockDim.x) splittable[f] = 1;
+    if (threadIdx.x == 0) {
+      st->num_leaves = 1;
+      st->done = 0;
+      st->cur_leaf = 0;
+      st->new_leaf = -1;
+      st->smaller = 0;
+      st->larger = -1;
+      st->skip = 0;
+      st->is_root = 1;
+      st->left_total = 0;
+      st->num_splits = 0;
+      st->counter = 0;
+      st->counter2 = 0;
+      st->hist_shift_g = sg;
+      st->hist_shift_h = sh;
+      LeafDev& r = leaves[0];
+      r.begin = 0;
+      r.count = n;
+      r.buf = 0;
+      r.depth = 0;
+      r.output = 0.0;
+      r.int_sum = 0;
+    }
+  }
+}
+
+// Quantized form: grad/hess -> int8 by stochastic rounding (learner/quant.rs:92-129),
+// packed (gq, hq) beside the identity partition; exact int root sums.
+__global__ void k_discretize(long long n, const float* __restrict__ g, const float* __restrict__ h,
+                             const double* __restrict__ grv, const double* __restrict__ hrv, int bins, int stochastic,
+                             int constant_hessian, const int* __restrict__ starts, const int* call_ctr, IterDev* it,
+                             unsigned* __restrict__ idx0, char2* __restrict__ q0, int8_t* __restrict__ gq_out,
+                             int8_t* __restrict__ hq_out, StepDev* st, LeafDev* leaves, SplitRec* best,
+                             uint8_t* splittable, int F, int L) {
+  double mg = (double)__uint_as_float(it->max_abs_g_bits);
+  double mh = (double)__uint_as_float(it->max_abs_h_bits);
+  double gs = mg / (double)(bins / 2);
+  double hs = constant_hessian ? mh : mh / (double)bins;
+  double inv_g = 1.0 / gs, inv_h = 1.0 / hs;
+  long long start = starts[*call_ctr];
+  long long ai = 0, agq = 0, ahq = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    double gv = (double)g[i];
+    double rg = 0.5, rh = 0.5;
+    if (stochastic) {
+      long long pos = (i + start) % n;
+      rg = grv[pos];
+      rh = hrv[pos];
+    }
+    int8_t a = gv >= 0.0 ? to_i8(gv * inv_g + rg) : to_i8(gv * inv_g - rg);
+    int8_t b = constant_hessian ? (int8_t)1 : to_i8((double)h[i] * inv_h + rh);
+    idx0[i] = (unsigned)i;
+    q0[i] = make_char2(a, b);
+    gq_out[i] = a;
+    hq_out[i] = b;
+    ai += (long long)(((unsigned long long)(long long)a << 32) | (unsigned long long)(uint8_t)b);
+    agq += a;
+    ahq += b;
+  }
+  ai = wave_sum(ai);
+  agq = wave_sum(agq);
+  ahq = wave_sum(ahq);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd((unsigned long long*)&it->root_int_sum, (unsigned long long)ai);
+    atomicAdd((unsigned long long*)&it->root_gq_sum, (unsigned long long)agq);
+    atomicAdd((unsigned long long*)&it->root_hq_sum, (unsigned long long)ahq);
+  }
+  if (blockIdx.x == 0) {
+    for (int l = threadIdx.x; l < L; l += blockDim.x) reset_rec(best[l]);
+    for (int f = threadIdx.x; f < F; f += blockDim.x) splittable[f] = 1;
+    if (threadIdx.x == 0) {
+      it->grad_scale = gs;
+      it->hess_scale = hs;
+      it->start = (int)start;
+      st->num_leaves = 1;
+      st->done = 0;
+      st->cur_leaf = 0;
+      st->new_leaf = -1;
+      st->smaller = 0;
+      st->larger = -1;
+      st->skip = 0;
+      st->is_root = 1;
+      st->left_total = 0;
+      st->num_splits = 0;
+      st->counter = 0;
+      st->counter2 = 0;
+      LeafDev& r = leaves[0];
+      r.begin = 0;
+      r.count = n;
+      r.buf = 0;
+      r.depth = 0;
+      r.output = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- histograms
+// Row-wise histogram of the smaller leaf (multi_val_bin.rs:556-611), FP64
+// values as exact fixed-point int64: grid = (row chunks) x (feature groups);
+// each block keeps its feature group's [FB][B][2] int64 histogram in LDS
+// (ds_add_u64), then adds its nonzero bins into the leaf accumulator with
+// global 64-bit atomics (exact, so the result is order-free).
+struct HistArgs {
+  const uint8_t* bins;     // row-major [N][S]
+  int S;                   // row stride in bytes
+  const unsigned* idx[2];
+  const void* gh[2];       // float2 (plain) or char2 (quantized)
+  const StepDev* st;
+  const LeafDev* leaves;
+  const FeatDev* feats;
+  int F, B, FB, NG;        // features, max bins, features per group, groups
+  int max_chunks;
+  int min_rows_per_chunk;
+  long long* acc;          // [F][B][2] (plain) or [F][B] (quantized)
+  // direct mode (stage entry point): explicit segment
+  int direct;
+  long long d_begin, d_count;
+  int d_buf, d_shift_g, d_shift_h;
+};
+
+template <int TPR>
+__global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
+  extern __shared__ unsigned long long lds[];
+  const StepDev* st = a.st;
+  long long begin, count;
+  int buf, sg, sh;
+  if (a.direct) {
+    begin = a.d_begin; count = a.d_count; buf = a.d_buf; sg = a.d_shift_g; sh = a.d_shift_h;
+  } else {
+    if (st->done || st->skip) return;
+    const LeafDev& L = a.leaves[st->smaller];
+    begin = L.begin; count = L.count; buf = L.buf; sg = st->hist_shift_g; sh = st->hist_shift_h;
+  }
+  const int fg = blockIdx.x % a.NG;
+  const int chunk = blockIdx.x / a.NG;
+  long long want = (count + a.min_rows_per_chunk - 1) / a.min_rows_per_chunk;
+  int nchunks = (int)min((long long)a.max_chunks, max(want, 1LL));
+  if (chunk >= nchunks) return;
+  long long per = (count + nchunks - 1) / nchunks;
+  long long r0 = (long long)chunk * per, r1 = min(count, r0 + per);
+  if (r0 >= r1) return;
+  const int f0 = fg * a.FB;
+  const int FBg = min(a.FB, a.F - f0);
+  const int B = a.B;
+  const int n_lds = FBg * B * 2;
+  for (int e = threadIdx.x; e < n_lds; e += blockDim.x) lds[e] = 0ull;
+  // most-frequent bin of each of this thread's 4 features (never accumulated)
+  const int sub = threadIdx.x / TPR, j = threadIdx.x % TPR;
+  int mfb[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int fl = j * 4 + k;
+    mfb[k] = fl < FBg ? a.feats[f0 + fl].mfb : 256;
+  }
+  __syncthreads();
+  const int rpp = 256 / TPR;
+  const unsigned* __restrict__ idx = a.idx[buf] + begin;
+  const float2* __restrict__ gh = (const float2*)a.gh[buf] + begin;
+  const bool active = j * 4 < FBg;
+  for (long long base = r0; base < r1; base += 4 * rpp) {
+    unsigned w[4];
+    long long gi[4], hi[4];
+    bool ok[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      long long p = base + u * rpp + sub;
+      ok[u] = active && p < r1;
+      if (ok[u]) {
+        unsigned r = idx[p];
+        float2 v = gh[p];
+        w[u] = *(const unsigned*)(a.bins + (size_t)r * a.S + f0 + j * 4);
+        gi[u] = to_fixed(v.x, sg);
+        hi[u] = to_fixed(v.y, sh);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!ok[u]) continue;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        int b = (w[u] >> (8 * k)) & 0xFF;
+        int fl = j * 4 + k;
+        if (fl < FBg && b != mfb[k]) {
+          unsigned long long* c = lds + ((size_t)fl * B + b) * 2;
+          atomicAdd(c, (unsigned long long)gi[u]);
+          atomicAdd(c + 1, (unsigned long long)hi[u]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  unsigned long long* out = (unsigned long long*)a.acc + (size_t)f0 * B * 2;
+  for (int e = threadIdx.x; e < n_lds; e += blockDim.x) {
+    unsigned long long v = lds[e];
+    if (v) atomicAdd(out + e, v);
+  }
+}
+
+// Quantized form (multi_val_bin.rs:710-791): int8 (g, h) pairs summed in LDS
+// as int32 g and uint32 h, flushed as packed (g << 32) + h into the int64
+// accumulator with wrapping global atomics. Equal to the reference's wrapping
+// sum of packed rows for any order.
+template <int TPR>
+__global__ void __launch_bounds__(256) k_hist_int(HistArgs a) {
+  extern __shared__ unsigned lds32[];
+  const StepDev* st = a.st;
+  long long begin, count;
+  int buf;
+  if (a.direct) {
+    begin = a.d_begin; count = a.d_count; buf = a.d_buf;
+  } else {
+    if (st->done || st->skip) return;
+    const LeafDev& L = a.leaves[st->smaller];
+    begin = L.begin; count = L.count; buf = L.buf;
+  }
+  const int fg = blockIdx.x % a.NG;
+  const int chunk = blockIdx.x / a.NG;
+  long long want = (count + a.min_rows_per_chunk - 1) / a.min_rows_per_chunk;
+  int nchunks = (int)min((long long)a.max_chunks, max(want, 1LL));
+  if (chunk >= nchunks) return;
+  long long per = (count + nchunks - 1) / nchunks;
+  long long r0 = (long long)chunk * per, r1 = min(count, r0 + per);
+  if (r0 >= r1) return;
+  const int f0 = fg * a.FB;
+  const int FBg = min(a.FB, a.F - f0);
+  const int B = a.B;
+  const int n_lds = FBg * B * 2;
+  for (int e = threadIdx.x; e < n_lds; e += blockDim.x) lds32[e] = 0u;
+  const int sub = threadIdx.x / TPR, j = threadIdx.x % TPR;
+  int mfb[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int fl = j * 4 + k;
+    mfb[k] = fl < FBg ? a.feats[f0 + fl].mfb : 256;
+  }
+  __syncthreads();
+  const int rpp = 256 / TPR;
+  const unsigned* __restrict__ idx = a.idx[buf] + begin;
+  const char2* __restrict__ q = (const char2*)a.gh[buf] + begin;
+  const bool active = j * 4 < FBg;
+  for (long long base = r0; base < r1; base += 4 * rpp) {
+    unsigned w[4];
+    int gq[4], hq[4];
+    bool ok[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      long long p = base + u * rpp + sub;
+      ok[u] = active && p < r1;
+      if (ok[u]) {
+        unsigned r = idx[p];
+        char2 v = q[p];
+        w[u] = *(const unsigned*)(a.bins + (size_t)r * a.S + f0 + j * 4);
+        gq[u] = v.x;
+        hq[u] = (int)(uint8_t)v.y;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!ok[u]) continue;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        int b = (w[u] >> (8 * k)) & 0xFF;
+        int fl = j * 4 + k;
+        if (fl < FBg && b != mfb[k]) {
+          unsigned* c = lds32 + ((size_t)fl * B + b) * 2;
+          atomicAdd(c, (unsigned)gq[u]);
+          atomicAdd(c + 1, (unsigned)hq[u]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  unsigned long long* out = (unsigned long long*)a.acc + (size_t)f0 * B;
+  for (int e = threadIdx.x; e < FBg * B; e += blockDim.x) {
+    unsigned gsum = lds32[2 * e], hsum = lds32[2 * e + 1];
+    if (gsum | hsum) {
+      unsigned long long packed = ((unsigned long long)(long long)(int)gsum << 32) + (unsigned long long)hsum;
+      atomicAdd(out + e, packed);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- threshold search
+// Shared scratch of one wave's search.
+struct ScanLds {
+  double val[3][kMaxBins + 1];   // g, h, count of each scan position (count as exact double)
+  double pre[3][kMaxBins + 1];   // running sums in scan order
+  int skipped[kMaxBins + 1];
+};
+
+struct LeafSums {
+  double sg, sh;     // sh already includes 2 * kEpsilon
+  int nd;
+  double po;
+  long long isum;    // quantized packed sum
+  double gs, hs;
+};
+
+// One sequential scan of split.rs:676-891 on a wave: the prefix chains run
+// in scan order on lanes 0-2 (g, h, count), every candidate is then scored
+// in parallel and the first-in-scan-order maximum is kept.
+__device__ void scan_f64(const double* hg, const double* hh, const FeatDev& fd, const SplitParamsDev& p,
+                         const LeafSums& ls, double min_gain_shift, bool reverse, bool skip_default, bool na,
+                         ScanLds& S, SplitRec& out, bool& splittable) {
+  const int lane = threadIdx.x & 63;
+  const int off = fd.offset;
+  const double cf = (double)ls.nd / ls.sh;
+  int t_hi, t_lo, K;
+  if (reverse) {
+    t_hi = fd.num_bin - 1 - off - (na ? 1 : 0);
+    t_lo = 1 - off;
+    K = t_hi - t_lo + 1;
+  } else {
+    t_lo = (na && off == 1) ? -1 : 0;
+    t_hi = fd.num_bin - 2 - off;
+    K = t_hi - t_lo + 1;
+  }
+  if (K <= 0) return;
+  // scan position k -> view index t -> bin t + off
+  for (int k = lane; k < K; k += 64) {
+    int t = reverse ? t_hi - k : t_lo + k;
+    int skip = skip_default && (t + off) == fd.default_bin;
+    S.skipped[k] = skip;
+    if (t >= 0) {
+      double hv = hh[t + off];
+      S.val[0][k] = hg[t + off];
+      S.val[1][k] = hv;
+      S.val[2][k] = (double)round_int(hv * cf);
+    } else {
+      S.val[0][k] = S.val[1][k] = S.val[2][k] = 0.0;
+    }
+  }
+  wave_sync();
+  if (lane < 3) {
+    double acc;
+    if (reverse) {
+      acc = lane == 1 ? KEPS : 0.0;
+    } else if (na && off == 1) {
+      // split.rs:795-806: everything but the NaN bin starts on the left
+      acc = lane == 0 ? ls.sg : lane == 1 ? ls.sh - KEPS : (double)ls.nd;
+      for (int i = 0; i < fd.num_bin - off; ++i) {
+        double v = lane == 2 ? (double)round_int(hh[i + off] * cf) : (lane == 0 ? hg[i + off] : hh[i + off]);
+        acc -= v;
+      }
+    } else {
+      acc = lane == 1 ? KEPS : 0.0;
+    }
+    const double* src = S.val[lane];
+    double* dst = S.pre[lane];
+    for (int k = 0; k < K; ++k) {
+      int t = reverse ? t_hi - k : t_lo + k;
+      if (!S.skipped[k] && t >= 0) acc += src[k];
+      dst[k] = acc;
+    }
+  }
+  wave_sync();
+  // candidates: kb = first break in scan order; best = max gain before it
+  int kb = INT32_MAX;
+  double bg = -INFINITY;
+  int bk = INT32_MAX;
+  bool spl = false;
+  for (int k = lane; k < K; k += 64) {
+    if (S.skipped[k]) continue;
+    double sa = S.pre[0][k], ha = S.pre[1][k];
+    int ca = (int)S.pre[2][k];
+    // reverse: a = right side; forward: a = left side
+    if (ca < p.min_data_in_leaf || ha < p.min_sum_hessian) continue;
+    int cb = ls.nd - ca;
+    double hb = ls.sh - ha;
+    if (cb < p.min_data_in_leaf || hb < p.min_sum_hessian) {
+      kb = min(kb, k);
+      continue;
+    }
+    double sb = ls.sg - sa;
+    double gain = reverse ? split_gain(sb, hb, sa, ha, p, cb, ca, ls.po) : split_gain(sa, ha, sb, hb, p, ca, cb, ls.po);
+    if (gain <= min_gain_shift) continue;
+    // candidate at k (valid only if k < first break)
+    S.skipped[k] = 2 + 0;  // marks a candidate; gains kept in val[0]
+    S.val[0][k] = gain;
+  }
+  // first break
+  for (int o = 32; o > 0; o >>= 1) kb = min(kb, __shfl_xor(kb, o, 64));
+  for (int k = lane; k < K; k += 64) {
+    if (S.skipped[k] == 2 && k < kb) {
+      spl = true;
+      double gain = S.val[0][k];
+      if (gain > bg || (gain == bg && k < bk)) { bg = gain; bk = k; }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    double og = __shfl_xor(bg, o, 64);
+    int ok = __shfl_xor(bk, o, 64);
+    if (og > bg || (og == bg && ok < bk)) { bg = og; bk = ok; }
+  }
+  int any = __any(spl);
+  wave_sync();
+  if (!any) return;
+  splittable = true;
+  if (bk != INT32_MAX && bg > out.gain + min_gain_shift) {
+    double sa = S.pre[0][bk], ha = S.pre[1][bk];
+    int ca = (int)S.pre[2][bk];
+    int t = reverse ? t_hi - bk : t_lo + bk;
+    double slg, slh;
+    int lc;
+    if (reverse) { slg = ls.sg - sa; slh = ls.sh - ha; lc = ls.nd - ca; }
+    else { slg = sa; slh = ha; lc = ca; }
+    out.threshold = reverse ? t - 1 + off : t + off;
+    out.left_output = leaf_output(slg, slh, p, lc, ls.po);
+    out.left_count = lc;
+    out.lsg = slg;
+    out.lsh = slh - KEPS;
+    out.right_output = leaf_output(ls.sg - slg, ls.sh - slh, p, ls.nd - lc, ls.po);
+    out.right_count = ls.nd - lc;
+    out.rsg = ls.sg - slg;
+    out.rsh = ls.sh - slh - KEPS;
+    out.gain = bg - min_gain_shift;
+    out.default_left = reverse ? 1 : 0;
+  }
+}
+
+// split.rs:402-441 on a wave.
+__device__ bool search_numerical(const double* hg, const double* hh, const FeatDev& fd, const SplitParamsDev& p,
+                                 const LeafSums& ls, ScanLds& S, SplitRec& out) {
+  double shift = leaf_gain(ls.sg, ls.sh, p, ls.nd, ls.po) + p.min_gain_to_split;
+  bool spl = false;
+  if (fd.num_bin > 2 && fd.missing_type != 0) {
+    if (fd.missing_type == 1) {
+      scan_f64(hg, hh, fd, p, ls, shift, true, true, false, S, out, spl);
+      scan_f64(hg, hh, fd, p, ls, shift, false, true, false, S, out, spl);
+    } else {
+      scan_f64(hg, hh, fd, p, ls, shift, true, false, true, S, out, spl);
+      scan_f64(hg, hh, fd, p, ls, shift, false, false, true, S, out, spl);
+    }
+  } else {
+    scan_f64(hg, hh, fd, p, ls, shift, true, false, false, S, out, spl);
+    if (fd.missing_type == 2) out.default_left = 0;
+  }
+  return spl;
+}
+
+// split.rs:446-670 (categorical, no monotone constraints) on lane 0; the
+// ctr order is a stable rank sort computed by all lanes.
+__device__ bool search_categorical(const double* hg, const double* hh, const FeatDev& fd, const SplitParamsDev& p0,
+                                   const LeafSums& ls, ScanLds& S, SplitRec& out) {
+  const int lane = threadIdx.x & 63;
+  const int off = fd.offset;
+  const int bin_start = 1 - off, bin_end = fd.num_bin - off;
+  const double cf = (double)ls.nd / ls.sh;
+  SplitParamsDev pc = p0;
+  const bool onehot = fd.num_bin <= p0.max_cat_to_onehot;
+  double gain_shift = p0.path_smooth > KEPS ? 0.0 : leaf_gain(ls.sg, ls.sh, p0, ls.nd, 0.0);
+  if (p0.path_smooth > KEPS) {
+    double s = p0.lambda_l1 > 0.0 ? thr_l1(ls.sg, p0.lambda_l1) : ls.sg;
+    gain_shift = -(2.0 * s * ls.po + (ls.sh + p0.lambda_l2) * ls.po * ls.po);
+  }
+  double min_gain_shift = gain_shift + p0.min_gain_to_split;
+  out.default_left = 0;
+  int used_bin = 0;
+  int* sorted = (int*)S.pre[2];   // reuse scratch as ints
+  if (!onehot) {
+    // bins kept for the sort: round_int(h * cf) >= cat_smooth, in bin order
+    for (int i = lane; i < bin_end - bin_start; i += 64) {
+      int t = bin_start + i;
+      S.skipped[i] = (double)round_int(hh[t + off] * cf) >= p0.cat_smooth;
+    }
+    wave_sync();
+    if (lane == 0) {
+      int c = 0;
+      for (int i = 0; i < bin_end - bin_start; ++i)
+        if (S.skipped[i]) sorted[c++] = bin_start + i;
+      S.skipped[kMaxBins] = c;
+    }
+    wave_sync();
+    used_bin = S.skipped[kMaxBins];
+    // stable sort by ctr = g / (h + cat_smooth): rank = #less + #equal-before
+    for (int i = lane; i < used_bin; i += 64) {
+      int ti = sorted[i];
+      double ci = hg[ti + off] / (hh[ti + off] + p0.cat_smooth);
+      int rank = 0;
+      for (int k = 0; k < used_bin; ++k) {
+        int tk = sorted[k];
+        double ck = hg[tk + off] / (hh[tk + off] + p0.cat_smooth);
+        if (ck < ci || (!(ci < ck) && !(ck < ci) && k < i)) rank++;
+      }
+      S.val[2][rank] = (double)ti;
+    }
+    wave_sync();
+  }
+  bool is_splittable = false;
+  if (lane == 0) {
+    double best_gain = -INFINITY, best_slg = 0, best_slh = 0;
+    int best_lc = 0, best_threshold = -1, best_dir = 1;
+    if (onehot) {
+      for (int t = bin_start; t < bin_end; ++t) {
+        double grad = hg[t + off], hess = hh[t + off];
+        int cnt = round_int(hess * cf);
+        if (cnt < p0.min_data_in_leaf || hess < p0.min_sum_hessian) continue;
+        int other = ls.nd - cnt;
+        if (other < p0.min_data_in_leaf) continue;
+        double soh = ls.sh - hess - KEPS;
+        if (soh < p0.min_sum_hessian) continue;
+        double sog = ls.sg - grad;
+        double cg = split_gain(sog, soh, grad, hess + KEPS, pc, other, cnt, ls.po);
+        if (cg <= min_gain_shift) continue;
+        is_splittable = true;
+        if (cg > best_gain) { best_threshold = t; best_slg = grad; best_slh = hess + KEPS; best_lc = cnt; best_gain = cg; }
+      }
+    } else {
+      pc.lambda_l2 += p0.cat_l2;
+      int max_num_cat = min(p0.max_cat_threshold, (used_bin + 1) / 2);
+      for (int d = 0; d < 2; ++d) {
+        int dir = d == 0 ? 1 : -1;
+        int pos = d == 0 ? 0 : used_bin - 1;
+        int cur_group = 0;
+        double slg = 0.0, slh = KEPS;
+        int lc = 0;
+        for (int i = 0; i < used_bin && i < max_num_cat; ++i) {
+          int t = (int)S.val[2][pos];
+          pos += dir;
+          double grad = hg[t + off], hess = hh[t + off];
+          int cnt = round_int(hess * cf);
+          slg += grad;
+          slh += hess;
+          lc += cnt;
+          cur_group += cnt;
+          if (lc < p0.min_data_in_leaf || slh < p0.min_sum_hessian) continue;
+          int rc = ls.nd - lc;
+          if (rc < p0.min_data_in_leaf || rc < p0.min_data_per_group) break;
+          double srh = ls.sh - slh;
+          if (srh < p0.min_sum_hessian) break;
+          if (cur_group < p0.min_data_per_group) continue;
+          cur_group = 0;
+          double srg = ls.sg - slg;
+          double cg = split_gain(slg, slh, srg, srh, pc, lc, rc, ls.po);
+          if (cg <= min_gain_shift) continue;
+          is_splittable = true;
+          if (cg > best_gain) {
+            best_lc = lc; best_slg = slg; best_slh = slh; best_threshold = i; best_gain = cg; best_dir = dir;
+          }
+        }
+      }
+    }
+    if (is_splittable) {
+      out.left_output = leaf_output(best_slg, best_slh, pc, best_lc, ls.po);
+      out.left_count = best_lc;
+      out.lsg = best_slg;
+      out.lsh = best_slh - KEPS;
+      out.right_output = leaf_output(ls.sg - best_slg, ls.sh - best_slh, pc, ls.nd - best_lc, ls.po);
+      out.right_count = ls.nd - best_lc;
+      out.rsg = ls.sg - best_slg;
+      out.rsh = ls.sh - best_slh - KEPS;
+      out.gain = best_gain - min_gain_shift;
+      for (int w = 0; w < kCatWords; ++w) out.cat_bits[w] = 0;
+      int nc = 0;
+      if (onehot) {
+        int b = best_threshold + off;
+        out.cat_bits[b / 32] |= 1u << (b % 32);
+        nc = 1;
+      } else {
+        for (int i = 0; i <= best_threshold; ++i) {
+          int t = best_dir == 1 ? (int)S.val[2][i] : (int)S.val[2][used_bin - 1 - i];
+          int b = t + off;
+          out.cat_bits[b / 32] |= 1u << (b % 32);
+          nc++;
+        }
+      }
+      out.num_cat = nc;
+    }
+    S.skipped[kMaxBins] = is_splittable;
+  }
+  wave_sync();
+  return S.skipped[kMaxBins] != 0;
+}
+
+// split.rs:993-1190 on a wave: packed int64 prefix sums are exact, so every
+// scan position is scored in parallel from a wave-parallel prefix.
+__device__ void scan_int(const long long* hv, const FeatDev& fd, const SplitParamsDev& p, const LeafSums& ls,
+                         double min_gain_shift, bool reverse, bool skip_default, bool na, ScanLds& S, SplitRec& out,
+                         bool& splittable) {
+  const int lane = threadIdx.x & 63;
+  const int off = fd.offset;
+  long long* pre = (long long*)S.pre[0];
+  int t_hi, t_lo, K;
+  if (reverse) { t_hi = fd.num_bin - 1 - off - (na ? 1 : 0); t_lo = 1 - off; }
+  else { t_lo = (na && off == 1) ? -1 : 0; t_hi = fd.num_bin - 2 - off; }
+  K = t_hi - t_lo + 1;
+  if (K <= 0) return;
+  const double cf = (double)ls.nd / (double)(unsigned)ls.isum;
+  if (lane == 0) {
+    unsigned long long acc = 0;
+    if (!reverse && na && off == 1) {
+      acc = (unsigned long long)ls.isum;
+      for (int i = 0; i < fd.num_bin - off; ++i) acc -= (unsigned long long)hv[i + off];
+    }
+    for (int k = 0; k < K; ++k) {
+      int t = reverse ? t_hi - k : t_lo + k;
+      bool skip = skip_default && (t + off) == fd.default_bin;
+      S.skipped[k] = skip;
+      if (!skip && t >= 0) acc += (unsigned long long)hv[t + off];
+      pre[k] = (long long)acc;
+    }
+  }
+  wave_sync();
+  int kb = INT32_MAX;
+  for (int k = lane; k < K; k += 64) {
+    if (S.skipped[k]) continue;
+    long long a = pre[k];
+    int ca = round_int((double)(unsigned)a * cf);
+    double ha = (double)(unsigned)a * ls.hs;
+    if (ca < p.min_data_in_leaf || ha < p.min_sum_hessian) continue;
+    int cb = ls.nd - ca;
+    if (cb < p.min_data_in_leaf) { kb = min(kb, k); continue; }
+    long long b = (long long)((unsigned long long)ls.isum - (unsigned long long)a);
+    double hb = (double)(unsigned)b * ls.hs;
+    if (hb < p.min_sum_hessian) { kb = min(kb, k); continue; }
+    double ga = (double)(int)(a >> 32) * ls.gs, gb = (double)(int)(b >> 32) * ls.gs;
+    double gain = reverse ? split_gain(gb, hb + KEPS, ga, ha + KEPS, p, cb, ca, ls.po)
+                          : split_gain(ga, ha + KEPS, gb, hb + KEPS, p, ca, cb, ls.po);
+    if (gain <= min_gain_shift) continue;
+    S.skipped[k] = 2;
+    S.val[0][k] = gain;
+  }
+  for (int o = 32; o > 0; o >>= 1) kb = min(kb, __shfl_xor(kb, o, 64));
+  double bg = -INFINITY;
+  int bk = INT32_MAX;
+  bool spl = false;
+  for (int k = lane; k < K; k += 64) {
+    if (S.skipped[k] == 2 && k < kb) {
+      spl = true;
+      double gain = S.val[0][k];
+      if (gain > bg || (gain == bg && k < bk)) { bg = gain; bk = k; }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    double og = __shfl_xor(bg, o, 64);
+    int ok = __shfl_xor(bk, o, 64);
+    if (og > bg || (og == bg && ok < bk)) { bg = og; bk = ok; }
+  }
+  int any = __any(spl);
+  wave_sync();
+  if (!any) return;
+  splittable = true;
+  if (bk != INT32_MAX && bg > out.gain + min_gain_shift) {
+    long long a = pre[bk];
+    int t = reverse ? t_hi - bk : t_lo + bk;
+    long long left = reverse ? (long long)((unsigned long long)ls.isum - (unsigned long long)a) : a;
+    long long right = (long long)((unsigned long long)ls.isum - (unsigned long long)left);
+    double lg = (double)(int)(left >> 32) * ls.gs, lh = (double)(unsigned)left * ls.hs;
+    double rg = (double)(int)(right >> 32) * ls.gs, rh = (double)(unsigned)right * ls.hs;
+    int lc = round_int((double)(unsigned)left * cf);
+    int rc = round_int((double)(unsigned)right * cf);
+    out.threshold = reverse ? t - 1 + off : t + off;
+    out.left_output = leaf_output(lg, lh, p, lc, ls.po);
+    out.left_count = lc;
+    out.lsg = lg;
+    out.lsh = lh;
+    out.lgh = left;
+    out.right_output = leaf_output(rg, rh, p, rc, ls.po);
+    out.right_count = rc;
+    out.rsg = rg;
+    out.rsh = rh;
+    out.rgh = right;
+    out.gain = bg - min_gain_shift;
+    out.default_left = reverse ? 1 : 0;
+  }
+}
+
+__device__ bool search_int(const long long* hv, const FeatDev& fd, const SplitParamsDev& p, const LeafSums& ls,
+                           ScanLds& S, SplitRec& out) {
+  double shift = leaf_gain((double)(int)(ls.isum >> 32) * ls.gs, (double)(unsigned)ls.isum * ls.hs, p, ls.nd, ls.po) +
+                 p.min_gain_to_split;
+  bool spl = false;
+  if (fd.num_bin > 2 && fd.missing_type != 0) {
+    if (fd.missing_type == 1) {
+      scan_int(hv, fd, p, ls, shift, true, true, false, S, out, spl);
+      scan_int(hv, fd, p, ls, shift, false, true, false, S, out, spl);
+    } else {
+      scan_int(hv, fd, p, ls, shift, true, false, true, S, out, spl);
+      scan_int(hv, fd, p, ls, shift, false, false, true, S, out, spl);
+    }
+  } else {
+    scan_int(hv, fd, p, ls, shift, true, false, false, S, out, spl);
+    if (fd.missing_type == 2) out.default_left = 0;
+  }
+  return spl;
+}
+
+struct SplitArgs {
+  StepDev* st;
+  LeafDev* leaves;
+  const FeatDev* feats;
+  SplitParamsDev p;
+  int F, B, L;
+  long long* acc;        // smaller-leaf accumulator (zeroed after reading)
+  double* pool;          // [L][F][B][2] FP64 histograms
+  long long* ipool;      // [L][F][B] packed histograms (quantized)
+  SplitRec* feat_split;  // [2][F]
+  SplitRec* best;        // [L]
+  uint8_t* splittable;   // [L][F]
+  SplitRec* cur;         // the split to apply next
+  SplitRec* log;         // [L-1] chosen splits in order
+  IterDev* it;
+  int quant;
+};
+
+// The leaf sums a search needs (leaf_splits.hpp; the root's come from the
+// fixed-point / packed root sums).
+__device__ LeafSums leaf_sums(const SplitArgs& a, int leaf, bool root) {
+  LeafSums s;
+  const LeafDev& L = a.leaves[leaf];
+  s.nd = (int)L.count;
+  s.isum = L.int_sum;
+  s.gs = a.it->grad_scale;
+  s.hs = a.it->hess_scale;
+  double sg = L.sum_g, sh = L.sum_h;
+  if (root) {
+    if (a.quant) {
+      s.isum = a.it->root_int_sum;
+      sg = (double)a.it->root_gq_sum * s.gs;
+      sh = (double)a.it->root_hq_sum * s.hs;
+    } else {
+      sg = ldexp((double)a.it->root_g_fix, -fix_shift(L.count, a.it->max_abs_g_bits));
+      sh = ldexp((double)a.it->root_h_fix, -fix_shift(L.count, a.it->max_abs_h_bits));
+    }
+  }
+  s.sg = sg;
+  s.sh = sh + 2.0 * KEPS;
+  // learner/mod.rs:1303-1309: the root's parent output is its own output
+  s.po = root ? leaf_output_t(sg, sh, a.p, true, true, false, s.nd, 0.0) : L.output;
+  return s;
+}
+
+// Threshold search for the smaller and larger leaf of this step, one block
+// (2 waves) per feature; the last block picks each leaf's best split and the
+// next leaf to split (learner/mod.rs:954-1101, 698-706).
+__global__ void __launch_bounds__(128) k_split(SplitArgs a) {
+  StepDev* st = a.st;
+  if (st->done) return;
+  __shared__ double hsm[2][kMaxBins];   // smaller: g, h
+  __shared__ double hlg[2][kMaxBins];   // larger: g, h
+  __shared__ long long ism[kMaxBins], ilg[kMaxBins];
+  __shared__ ScanLds scr[2];
+  __shared__ SplitRec res[2];
+  const int f = blockIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const bool root = st->is_root != 0;
+  const int sm = st->smaller, lg = root ? -1 : st->larger, cur = st->cur_leaf;
+  const int F = a.F, B = a.B;
+  if (!st->skip) {
+    const FeatDev fd = a.feats[f];
+    const bool parent_spl = root ? true : a.splittable[(size_t)cur * F + f] != 0;
+    LeafSums ls_s = leaf_sums(a, sm, root);
+    __syncthreads();
+    if (!a.quant) {
+      const int ssg = root ? fix_shift(a.leaves[sm].count, a.it->max_abs_g_bits) : st->hist_shift_g;
+      const int ssh = root ? fix_shift(a.leaves[sm].count, a.it->max_abs_h_bits) : st->hist_shift_h;
+      long long* acc = a.acc + (size_t)f * B * 2;
+      for (int b = tid; b < B; b += blockDim.x) {
+        long long gi = acc[2 * b], hi = acc[2 * b + 1];
+        acc[2 * b] = 0;
+        acc[2 * b + 1] = 0;
+        hsm[0][b] = ldexp((double)gi, -ssg);
+        hsm[1][b] = ldexp((double)hi, -ssh);
+      }
+      __syncthreads();
+      // histogram.rs:42-56: the most frequent bin from the leaf totals
+      if (fd.mfb > 0 && tid < 2) {
+        double v = tid == 0 ? ls_s.sg : ls_s.sh - 2.0 * KEPS;
+        for (int i = 0; i < fd.num_bin; ++i)
+          if (i != fd.mfb) v -= hsm[tid][i];
+        hsm[tid][fd.mfb] = v;
+      }
+      __syncthreads();
+      double* ps = a.pool + ((size_t)sm * F + f) * B * 2;
+      if (lg >= 0) {
+        const double* pp = a.pool + ((size_t)cur * F + f) * B * 2;   // parent (read before any write)
+        for (int b = tid; b < B; b += blockDim.x) {
+          hlg[0][b] = pp[2 * b] - hsm[0][b];
+          hlg[1][b] = pp[2 * b + 1] - hsm[1][b];
+        }
+        __syncthreads();
+        double* pl = a.pool + ((size_t)lg * F + f) * B * 2;
+        for (int b = tid; b < B; b += blockDim.x) {
+          pl[2 * b] = hlg[0][b];
+          pl[2 * b + 1] = hlg[1][b];
+        }
+      }
+      for (int b = tid; b < B; b += blockDim.x) {
+        ps[2 * b] = hsm[0][b];
+        ps[2 * b + 1] = hsm[1][b];
+      }
+    } else {
+      long long* acc = a.acc + (size_t)f * B;
+      for (int b = tid; b < B; b += blockDim.x) {
+        ism[b] = acc[b];
+        acc[b] = 0;
+      }
+      __syncthreads();
+      if (fd.mfb > 0 && tid == 0) {
+        unsigned long long s = (unsigned long long)ls_s.isum;
+        for (int i = 0; i < fd.num_bin; ++i)
+          if (i != fd.mfb) s -= (unsigned long long)ism[i];
+        ism[fd.mfb] = (long long)s;
+      }
+      __syncthreads();
+      long long* ps = a.ipool + ((size_t)sm * F + f) * B;
+      if (lg >= 0) {
+        const long long* pp = a.ipool + ((size_t)cur * F + f) * B;
+        for (int b = tid; b < B; b += blockDim.x)
+          ilg[b] = (long long)((unsigned long long)pp[b] - (unsigned long long)ism[b]);
+        __syncthreads();
+        long long* pl = a.ipool + ((size_t)lg * F + f) * B;
+        for (int b = tid; b < B; b += blockDim.x) pl[b] = ilg[b];
+      }
+      for (int b = tid; b < B; b += blockDim.x) ps[b] = ism[b];
+    }
+    __syncthreads();
+    // wave 0 searches the smaller leaf, wave 1 the larger
+    const int leaf = wave == 0 ? sm : lg;
+    if (leaf >= 0) {
+      LeafSums ls = wave == 0 ? ls_s : leaf_sums(a, leaf, false);
+      SplitRec& r = res[wave];
+      if (lane == 0) reset_rec(r);
+      bool spl = false;
+      wave_sync();
+      if (parent_spl) {
+        if (lane == 0) r.gain = -INFINITY;
+        if (a.quant) {
+          spl = search_int(wave == 0 ? ism : ilg, fd, a.p, ls, scr[wave], r);
+        } else if (fd.bin_type == 1) {
+          spl = search_categorical(wave == 0 ? hsm[0] : hlg[0], wave == 0 ? hsm[1] : hlg[1], fd, a.p, ls, scr[wave], r);
+        } else {
+          spl = search_numerical(wave == 0 ? hsm[0] : hlg[0], wave == 0 ? hsm[1] : hlg[1], fd, a.p, ls, scr[wave], r);
+        }
+      }
+      if (lane == 0) {
+        if (parent_spl) r.feature = f;
+        a.feat_split[wave * F + f] = r;
+        a.splittable[(size_t)leaf * F + f] = spl ? 1 : 0;
+      }
+    }
+  }
+  if (!last_block(&st->counter)) return;
+  // ---- last block: per-leaf best (over features), then the next leaf to split
+  __shared__ double sg_best[128];
+  __shared__ int sf_best[128];
+  if (!st->skip) {
+    for (int w = 0; w < 2; ++w) {
+      const int leaf = w == 0 ? sm : lg;
+      if (leaf < 0) continue;
+      double bgain = -INFINITY;
+      int bf = -1;
+      for (int k = tid; k < F; k += blockDim.x) {
+        const SplitRec& r = a.feat_split[w * F + k];
+        if (better(r.gain, r.feature, bgain, bf)) { bgain = r.gain; bf = k; }
+      }
+      sg_best[tid] = bgain;
+      sf_best[tid] = bf;
+      __syncthreads();
+      if (tid == 0) {
+        double g0 = -INFINITY;
+        int f0 = -1;
+        for (int k = 0; k < (int)blockDim.x; ++k)
+          if (better(sg_best[k], sf_best[k], g0, f0)) { g0 = sg_best[k]; f0 = sf_best[k]; }
+        if (f0 >= 0) a.best[leaf] = a.feat_split[w * F + f0];
+        else reset_rec(a.best[leaf]);
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    if (root) {
+      // learner/mod.rs:647: the root's output before any split
+      LeafSums r = leaf_sums(a, 0, true);
+      LeafDev& L0 = a.leaves[0];
+      L0.sum_g = r.sg;
+      L0.sum_h = r.sh - 2.0 * KEPS;
+      L0.int_sum = r.isum;
+      L0.output = round0(leaf_output_t(r.sg, r.sh - 2.0 * KEPS, a.p, true, true, false, r.nd, 0.0));
+      st->root_output = L0.output;
+    }
+    // ArgMax over every leaf slot (learner/mod.rs:698-706)
+    int bl = 0;
+    for (int l = 1; l < a.L; ++l)
+      if (better(a.best[l].gain, a.best[l].feature, a.best[bl].gain, a.best[bl].feature)) bl = l;
+    double gain = a.best[bl].gain;
+    if (!(gain > 0.0) || st->num_leaves >= a.L) {
+      st->done = 1;
+    } else {
+      *a.cur = a.best[bl];
+      st->cur_leaf = bl;
+      st->new_leaf = st->num_leaves;
+    }
+    st->is_root = 0;
+    st->counter = 0;
+  }
+}
+
+// ---------------------------------------------------------------- partition
+struct PartArgs {
+  StepDev* st;
+  LeafDev* leaves;
+  const FeatDev* feats;
+  const uint8_t* bins_cm;   // feature-major [F][N]
+  long long N;
+  unsigned* idx[2];
+  void* gh[2];
+  int gh_bytes;             // 8 (float2) or 2 (char2)
+  int* tile_left;           // per tile left count, then exclusive offsets
+  SplitRec* cur;
+  SplitRec* best;
+  SplitRec* log;
+  SplitParamsDev p;
+  int L;
+  const IterDev* it;
+  int quant;
+};
+
+constexpr int kPartThreads = 256;
+constexpr int kPartRowsPerThread = 8;
+constexpr int kPartTile = kPartThreads * kPartRowsPerThread;
+
+// learner/mod.rs:792-810 goes_left / goes_left_categorical on a bin.
+__device__ __forceinline__ bool goes_left(int b, const FeatDev& fd, const SplitRec& s) {
+  if (fd.bin_type == 1) {
+    int q = b;
+    if (b == fd.mfb) {
+      if (fd.mfb == 0) return false;
+      q = fd.mfb;
+    }
+    return (s.cat_bits[q >> 5] >> (q & 31)) & 1u;
+  }
+  bool miss = (fd.missing_type == 1 && b == fd.default_bin) || (fd.missing_type == 2 && b == fd.num_bin - 1);
+  return miss ? s.default_left != 0 : b <= s.threshold;
+}
+
+__device__ __forceinline__ void seg_of(const PartArgs& a, long long& begin, long long& count, int& buf) {
+  const LeafDev& L = a.leaves[a.st->cur_leaf];
+  begin = L.begin;
+  count = L.count;
+  buf = L.buf;
+}
+
+// Pass 1: left count per tile; the last block scans the tile counts and
+// applies the split to the leaf records (partition.rs:107-116, learner/mod.rs:1105-1225).
+__global__ void __launch_bounds__(kPartThreads) k_part_count(PartArgs a) {
+  StepDev* st = a.st;
+  if (st->done) return;
+  const SplitRec s = *a.cur;
+  const FeatDev fd = a.feats[s.feature];
+  long long begin, count;
+  int buf;
+  seg_of(a, begin, count, buf);
+  const unsigned* idx = a.idx[buf] + begin;
+  const uint8_t* col = a.bins_cm + (size_t)s.feature * a.N;
+  const int ntiles = (int)((count + kPartTile - 1) / kPartTile);
+  __shared__ int wsum[kPartThreads / 64];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    long long p0 = (long long)t * kPartTile + threadIdx.x * kPartRowsPerThread;
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < kPartRowsPerThread; ++k) {
+      long long p = p0 + k;
+      if (p < count) c += goes_left(col[idx[p]], fd, s) ? 1 : 0;
+    }
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int w = 0; w < kPartThreads / 64; ++w) tot += wsum[w];
+      a.tile_left[t] = tot;
+    }
+    __syncthreads();
+  }
+  if (!last_block(&st->counter2)) return;
+  // exclusive scan of tile counts (in place), block-wide
+  __shared__ long long carry;
+  __shared__ long long part[kPartThreads];
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < ntiles; base += kPartThreads) {
+    int t = base + threadIdx.x;
+    long long v = t < ntiles ? a.tile_left[t] : 0;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < kPartThreads; o <<= 1) {
+      long long x = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+      __syncthreads();
+      part[threadIdx.x] += x;
+      __syncthreads();
+    }
+    long long excl = part[threadIdx.x] - v + carry;
+    if (t < ntiles) a.tile_left[t] = (int)excl;
+    __syncthreads();
+    if (threadIdx.x == kPartThreads - 1) carry = excl + v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const long long lc = carry, rc = count - carry;
+    const int cur = st->cur_leaf, nl = st->new_leaf;
+    LeafDev& Lc = a.leaves[cur];
+    LeafDev& Ln = a.leaves[nl];
+    const int depth = Lc.depth + 1;
+    Ln.begin = begin + lc;
+    Ln.count = rc;
+    Ln.buf = 1 - buf;
+    Ln.depth = depth;
+    Ln.sum_g = s.rsg;
+    Ln.sum_h = s.rsh;
+    Ln.int_sum = s.rgh;
+    Ln.output = isnan(s.right_output) ? 0.0 : s.right_output;
+    Lc.begin = begin;
+    Lc.count = lc;
+    Lc.buf = 1 - buf;
+    Lc.depth = depth;
+    Lc.sum_g = s.lsg;
+    Lc.sum_h = s.lsh;
+    Lc.int_sum = s.lgh;
+    Lc.output = isnan(s.left_output) ? 0.0 : s.left_output;
+    SplitRec r = s;
+    r.left_count = (int)lc;
+    r.right_count = (int)rc;
+    r.leaf = cur;
+    a.log[st->num_splits] = r;
+    st->num_splits += 1;
+    st->num_leaves += 1;
+    st->left_total = lc;
+    if (lc < rc) { st->smaller = cur; st->larger = nl; }
+    else { st->smaller = nl; st->larger = cur; }
+    // learner/mod.rs:712-731 before_find_best_split
+    bool skip = (a.p.max_depth > 0 && depth >= a.p.max_depth) ||
+                (rc < 2LL * a.p.min_data_in_leaf && lc < 2LL * a.p.min_data_in_leaf);
+    st->skip = skip ? 1 : 0;
+    if (skip) {
+      a.best[cur].gain = -INFINITY;
+      a.best[nl].gain = -INFINITY;
+    }
+    // fixed-point exponents of the next histogram build (the smaller child)
+    if (!a.quant) {
+      const long long sc = lc < rc ? lc : rc;
+      st->hist_shift_g = fix_shift(sc, a.it->max_abs_g_bits);
+      st->hist_shift_h = fix_shift(sc, a.it->max_abs_h_bits);
+    }
+    st->counter2 = 0;
+  }
+}
+
+// Pass 2: stable scatter of (row, gradient) pairs to the other buffer.
+__global__ void __launch_bounds__(kPartThreads) k_part_scatter(PartArgs a) {
+  StepDev* st = a.st;
+  if (st->done) return;
+  const SplitRec s = *a.cur;
+  const FeatDev fd = a.feats[s.feature];
+  // the split leaf's records were already updated by pass 1: rebuild the parent segment
+  const int cur = st->cur_leaf;
+  const LeafDev& Lc = a.leaves[cur];
+  const long long begin = Lc.begin;
+  const long long lc = st->left_total;
+  const long long count = lc + a.leaves[st->new_leaf].count;
+  const int src = 1 - Lc.buf, dst = Lc.buf;
+  const unsigned* idx = a.idx[src] + begin;
+  unsigned* oidx = a.idx[dst] + begin;
+  const uint8_t* col = a.bins_cm + (size_t)s.feature * a.N;
+  const int ntiles = (int)((count + kPartTile - 1) / kPartTile);
+  __shared__ int wpre[kPartThreads / 64];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const long long t0 = (long long)t * kPartTile;
+    const long long p0 = t0 + threadIdx.x * kPartRowsPerThread;
+    unsigned rows[kPartRowsPerThread];
+    bool left[kPartRowsPerThread];
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < kPartRowsPerThread; ++k) {
+      long long p = p0 + k;
+      left[k] = false;
+      if (p < count) {
+        rows[k] = idx[p];
+        left[k] = goes_left(col[rows[k]], fd, s);
+        c += left[k] ? 1 : 0;
+      }
+    }
+    // exclusive scan of c over the block (thread order = row order)
+    int x = c;
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wpre[threadIdx.x >> 6] = x;
+    __syncthreads();
+    int wbase = 0;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) wbase += wpre[w];
+    const int excl_left = wbase + x - c;
+    __syncthreads();
+    const long long tile_off = a.tile_left[t];
+    long long lpos = tile_off + excl_left;
+    long long rpos = lc + (t0 - tile_off) + ((long long)threadIdx.x * kPartRowsPerThread - excl_left);
+#pragma unroll
+    for (int k = 0; k < kPartRowsPerThread; ++k) {
+      long long p = p0 + k;
+      if (p >= count) break;
+      long long d = left[k] ? lpos++ : rpos++;
+      oidx[d] = rows[k];
+      if (a.gh_bytes == 8) ((float2*)a.gh[dst] + begin)[d] = ((const float2*)a.gh[src] + begin)[p];
+      else ((char2*)a.gh[dst] + begin)[d] = ((const char2*)a.gh[src] + begin)[p];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- tree end
+// Quantized form, learner/mod.rs:736-747: each leaf's output from its float
+// gradient sums (exact fixed-point per leaf, then refit_leaf_output).
+__global__ void k_renew_sum(const StepDev* st, const LeafDev* leaves, const unsigned* idx0, const unsigned* idx1, const float* g,
+                            const float* h, const IterDev* it, long long* sums /* [L][2] */) {
+  const int l = blockIdx.y;
+  if (l >= st->num_leaves) return;
+  const LeafDev& L = leaves[l];
+  const unsigned* idx = (L.buf ? idx1 : idx0) + L.begin;
+  const int sg = fix_shift(L.count, it->max_abs_g_bits), sh = fix_shift(L.count, it->max_abs_h_bits);
+  long long ag = 0, ah = 0;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < L.count; p += (long long)gridDim.x * blockDim.x) {
+    unsigned r = idx[p];
+    ag += to_fixed(g[r], sg);
+    ah += to_fixed(h[r], sh);
+  }
+  ag = wave_sum(ag);
+  ah = wave_sum(ah);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd((unsigned long long*)&sums[2 * l], (unsigned long long)ag);
+    atomicAdd((unsigned long long*)&sums[2 * l + 1], (unsigned long long)ah);
+  }
+}
+
+// Shrunk leaf values (tree.rs:299-319): out[l] = round0(output * lr); with
+// `renew`, output = round0(refit_leaf_output(sum g, sum h)).
+__global__ void k_leaf_values(const StepDev* st, LeafDev* leaves, const IterDev* it, long long* renew_sums, int renew,
+                              SplitParamsDev p, double lr, int L, double* out) {
+  const int l = threadIdx.x + blockIdx.x * blockDim.x;
+  if (l >= L) return;
+  if (l >= st->num_leaves) { out[l] = 0.0; return; }
+  LeafDev& Lf = leaves[l];
+  if (renew) {
+    const int sg = fix_shift(Lf.count, it->max_abs_g_bits), sh = fix_shift(Lf.count, it->max_abs_h_bits);
+    double g = ldexp((double)renew_sums[2 * l], -sg), h = ldexp((double)renew_sums[2 * l + 1], -sh);
+    renew_sums[2 * l] = 0;
+    renew_sums[2 * l + 1] = 0;
+    Lf.output = round0(leaf_output_t(g, h, p, true, true, false, (int)Lf.count, 0.0));
+  }
+  out[l] = round0(Lf.output * lr);
+}
+
+// Score update (partition.rs:178-191): each row of leaf l gets out[l].
+__global__ void k_score(const StepDev* st, const LeafDev* leaves, const unsigned* idx0, const unsigned* idx1, const double* val,
+                        double* score) {
+  const int l = blockIdx.y;
+  if (st->num_leaves <= 1 || l >= st->num_leaves) return;
+  const LeafDev& L = leaves[l];
+  const unsigned* idx = (L.buf ? idx1 : idx0) + L.begin;
+  const double v = val[l];
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < L.count; p += (long long)gridDim.x * blockDim.x)
+    score[idx[p]] += v;
+}
+
+// Leaf of each row for the last tree (partition state).
+__global__ void k_leaf_of_row(const StepDev* st, const LeafDev* leaves, const unsigned* idx0, const unsigned* idx1, int* out) {
+  const int l = blockIdx.y;
+  if (l >= st->num_leaves) return;
+  const LeafDev& L = leaves[l];
+  const unsigned* idx = (L.buf ? idx1 : idx0) + L.begin;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < L.count; p += (long long)gridDim.x * blockDim.x)
+    out[idx[p]] = l;
+}
+
+// ---------------------------------------------------------------- prediction
+// Flattened trees: nodes of all trees concatenated; a node's children are
+// node indices (>= 0) or ~leaf; leaf values in a parallel array.
+struct PredNode {
+  double threshold;
+  int feature;      // raw column (raw predict) or inner feature (binned predict)
+  int left, right;
+  int thr_bin;
+  int8_t missing_type, default_left, is_cat, pad;
+  int cat_off;      // offset of the category bitset words, -1 none
+  int cat_len;
+  int default_bin, nan_bin;  // binned predict: missing bins of the node's feature
+};
+
+struct PredArgs {
+  const PredNode* nodes;
+  const int* tree_node_off;   // [T]
+  const int* tree_leaf_off;   // [T]
+  const int* tree_num_leaves; // [T]
+  const double* leaf_val;
+  const unsigned* cat_words;
+  int T, K;
+  long long rows;
+  int ncol;
+  const float* X;             // raw rows (row-major f32)
+  const uint8_t* bins;        // binned rows (row-major u8, F columns)
+  int F;
+  double* out;                // [rows][K]
+};
+
+// predict.rs:95-109 (Tree::decision semantics), one row per thread.
+__global__ void k_predict_raw(PredArgs a) {
+  long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.rows) return;
+  const float* x = a.X + (size_t)r * a.ncol;
+  double acc[16];
+  for (int k = 0; k < a.K; ++k) acc[k] = 0.0;
+  for (int t = 0; t < a.T; ++t) {
+    const int k = t % a.K;
+    int leaf = 0;
+    if (a.tree_num_leaves[t] > 1) {
+      const PredNode* nd = a.nodes + a.tree_node_off[t];
+      int node = 0;
+      while (node >= 0) {
+        const PredNode& n = nd[node];
+        double v = n.feature < a.ncol ? (double)x[n.feature] : 0.0;
+        if (!isnan(v) && fabs(v) <= KZERO) v = 0.0;
+        if (n.is_cat) {
+          bool lft = false;
+          if (!isnan(v)) {
+            int c = (int)v;
+            if (c >= 0 && (c >> 5) < n.cat_len) lft = (a.cat_words[n.cat_off + (c >> 5)] >> (c & 31)) & 1u;
+          }
+          node = lft ? n.left : n.right;
+          continue;
+        }
+        if (isnan(v) && n.missing_type != 2) v = 0.0;
+        bool zero = v >= -KZERO && v <= KZERO;
+        if ((n.missing_type == 1 && zero) || (n.missing_type == 2 && isnan(v))) node = n.default_left ? n.left : n.right;
+        else node = v <= n.threshold ? n.left : n.right;
+      }
+      leaf = ~node;
+    }
+    acc[k] += a.leaf_val[a.tree_leaf_off[t] + leaf];
+  }
+  for (int k = 0; k < a.K; ++k) a.out[(size_t)r * a.K + k] = acc[k];
+}
+
+// tree.rs:605-632 leaf_binned_with: bins compared with threshold_in_bin.
+__global__ void k_predict_binned(PredArgs a) {
+  long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.rows) return;
+  const uint8_t* x = a.bins + (size_t)r * a.F;
+  double acc[16];
+  for (int k = 0; k < a.K; ++k) acc[k] = 0.0;
+  for (int t = 0; t < a.T; ++t) {
+    const int k = t % a.K;
+    int leaf = 0;
+    if (a.tree_num_leaves[t] > 1) {
+      const PredNode* nd = a.nodes + a.tree_node_off[t];
+      int node = 0;
+      while (node >= 0) {
+        const PredNode& n = nd[node];
+        int b = x[n.feature];
+        if (n.is_cat) {
+          bool lft = (b >> 5) < n.cat_len && ((a.cat_words[n.cat_off + (b >> 5)] >> (b & 31)) & 1u);
+          node = lft ? n.left : n.right;
+          continue;
+        }
+        bool is_default = (n.missing_type == 1 && b == n.default_bin) || (n.missing_type == 2 && b == n.nan_bin);
+        if (is_default) node = n.default_left ? n.left : n.right;
+        else node = b <= n.thr_bin ? n.left : n.right;
+      }
+      leaf = ~node;
+    }
+    acc[k] += a.leaf_val[a.tree_leaf_off[t] + leaf];
+  }
+  for (int k = 0; k < a.K; ++k) a.out[(size_t)r * a.K + k] = acc[k];
+}
+
+// Stage entry point: stable partition of explicit rows (partition.rs:118-174).
+__global__ void k_partition_rows(const unsigned* rows, long long m, const uint8_t* col, FeatDev fd, SplitRec s,
+                                 uint8_t* flag) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) flag[i] = goes_left(col[rows[i]], fd, s) ? 1 : 0;
+}
+
+// Stage entry point: one feature's threshold search on a given histogram.
+__global__ void __launch_bounds__(64) k_search_one(const double* hist, const long long* ihist, FeatDev fd,
+                                                   SplitParamsDev p, LeafSums ls, int quant, SplitRec* out,
+                                                   int* splittable) {
+  __shared__ double hg[kMaxBins], hh[kMaxBins];
+  __shared__ long long ih[kMaxBins];
+  __shared__ ScanLds S;
+  __shared__ SplitRec r;
+  const int lane = threadIdx.x;
+  for (int b = lane; b < kMaxBins; b += 64) {
+    if (quant) ih[b] = b < fd.num_bin ? ihist[b] : 0;
+    else {
+      hg[b] = b < fd.num_bin ? hist[2 * b] : 0.0;
+      hh[b] = b < fd.num_bin ? hist[2 * b + 1] : 0.0;
+    }
+  }
+  if (lane == 0) reset_rec(r);
+  __syncthreads();
+  if (quant) {
+    if (fd.mfb > 0 && lane == 0) {
+      unsigned long long s = (unsigned long long)ls.isum;
+      for (int i = 0; i < fd.num_bin; ++i) if (i != fd.mfb) s -= (unsigned long long)ih[i];
+      ih[fd.mfb] = (long long)s;
+    }
+  } else if (fd.mfb > 0 && lane < 2) {
+    double v = lane == 0 ? ls.sg : ls.sh - 2.0 * KEPS;
+    double* arr = lane == 0 ? hg : hh;
+    for (int i = 0; i < fd.num_bin; ++i) if (i != fd.mfb) v -= arr[i];
+    arr[fd.mfb] = v;
+  }
+  __syncthreads();
+  bool spl;
+  if (quant) spl = search_int(ih, fd, p, ls, S, r);
+  else if (fd.bin_type == 1) spl = search_categorical(hg, hh, fd, p, ls, S, r);
+  else spl = search_numerical(hg, hh, fd, p, ls, S, r);
+  if (lane == 0) {
+    *out = r;
+    *splittable = spl ? 1 : 0;
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+extern "C++" {
+
+template <int TPR>
+static void launch_hist_t(const HistArgs& a, bool quant, int grid, size_t lds, hipStream_t s) {
+  if (quant) hipLaunchKernelGGL(k_hist_int<TPR>, dim3(grid), dim3(256), lds, s, a);
+  else hipLaunchKernelGGL(k_hist_f64<TPR>, dim3(grid), dim3(256), lds, s, a);
+}
+
+void launch_hist(const HistArgs& a, bool quant, int tpr, int grid, size_t lds, hipStream_t s) {
+  switch (tpr) {
+    case 1: launch_hist_t<1>(a, quant, grid, lds, s); break;
+    case 2: launch_hist_t<2>(a, quant, grid, lds, s); break;
+    case 4: launch_hist_t<4>(a, quant, grid, lds, s); break;
+    case 8: launch_hist_t<8>(a, quant, grid, lds, s); break;
+    case 16: launch_hist_t<16>(a, quant, grid, lds, s); break;
+    default: launch_hist_t<32>(a, quant, grid, lds, s); break;
+  }
+}
+
+void set_hist_lds_limits(size_t bytes) {
+  const void* ks[] = {(const void*)k_hist_f64<1>, (const void*)k_hist_f64<2>, (const void*)k_hist_f64<4>,
+                      (const void*)k_hist_f64<8>, (const void*)k_hist_f64<16>, (const void*)k_hist_f64<32>,
+                      (const void*)k_hist_int<1>, (const void*)k_hist_int<2>, (const void*)k_hist_int<4>,
+                      (const void*)k_hist_int<8>, (const void*)k_hist_int<16>, (const void*)k_hist_int<32>};
+  for (const void* k : ks) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+void launch_split(const SplitArgs& a, hipStream_t s) { hipLaunchKernelGGL(k_split, dim3(a.F), dim3(128), 0, s, a); }
+void launch_part(const PartArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_part_count, dim3(grid), dim3(kPartThreads), 0, s, a);
+  hipLaunchKernelGGL(k_part_scatter, dim3(grid), dim3(kPartThreads), 0, s, a);
+}
+int part_tile() { return kPartTile; }
+
+}  // extern "C++"
+
+}  // namespace lgbmhip

@@ -1,0 +1,98 @@
+"""Deterministic synthetic inputs of the benchmark workloads.
+
+One generator drives every workload: splitmix64 streams -> uniforms in [0, 1)
+(53-bit) -> Box-Muller normals. Seeds per workload c: X from 1000 + c, the
+weight vector from 2000 + c, labels from 3000 + c.
+
+Workloads (rows x features):
+  binary_100k   : 100,000 x 28 N(0,1); y ~ Bernoulli(sigmoid(w.x + 0.5 x0 x1))
+  binary_10p5m  : 10,500,000 x 28 (20 N(0,1) + 8 lognormal(0,1)); same label
+                  model, then 5% of labels flipped
+"""
+import numpy as np
+
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed, start, n):
+    """Outputs start..start+n of the splitmix64 stream seeded with `seed`."""
+    with np.errstate(over="ignore"):
+        i = np.arange(start + 1, start + n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + i * _GOLD
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+def uniforms(seed, start, n):
+    return (splitmix64(seed, start, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def normals(seed, start, n):
+    """Box-Muller on consecutive uniform pairs; output k uses pair k."""
+    u = uniforms(seed, 2 * start, 2 * n)
+    u1 = 1.0 - u[0::2]  # (0, 1]
+    u2 = u[1::2]
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def _features(c, n, F, lognormal_cols=(), chunk=1 << 20):
+    X = np.empty((n, F), np.float32)
+    for r0 in range(0, n, chunk):
+        r1 = min(n, r0 + chunk)
+        z = normals(1000 + c, r0 * F, (r1 - r0) * F).reshape(r1 - r0, F)
+        for j in lognormal_cols:
+            z[:, j] = np.exp(z[:, j])
+        X[r0:r1] = z.astype(np.float32)
+    return X
+
+
+def _binary_labels(c, X, flip_rate=0.0, chunk=1 << 20):
+    n, F = X.shape
+    w = normals(2000 + c, 0, F)
+    y = np.empty(n, np.float32)
+    for r0 in range(0, n, chunk):
+        r1 = min(n, r0 + chunk)
+        x = X[r0:r1].astype(np.float64)
+        z = x @ w + 0.5 * x[:, 0] * x[:, 1]
+        p = 1.0 / (1.0 + np.exp(-z))
+        u = uniforms(3000 + c, r0, r1 - r0)
+        lab = (u < p)
+        if flip_rate > 0:
+            f = uniforms(4000 + c, r0, r1 - r0) < flip_rate
+            lab = lab ^ f
+        y[r0:r1] = lab.astype(np.float32)
+    return y
+
+
+def binary_100k(n=100_000):
+    X = _features(1, n, 28)
+    return X, _binary_labels(1, X)
+
+
+def binary_10p5m(n=10_500_000):
+    X = _features(2, n, 28, lognormal_cols=range(20, 28))
+    return X, _binary_labels(2, X, flip_rate=0.05)
+
+
+def holdout(workload, n):
+    """A held-out sample of the same generator (different row range)."""
+    if workload == "binary_10p5m":
+        F, ln, flip, c = 28, range(20, 28), 0.05, 2
+    else:
+        F, ln, flip, c = 28, (), 0.0, 1
+    X = np.empty((n, F), np.float32)
+    off = 1 << 40
+    z = normals(1000 + c, off, n * F).reshape(n, F)
+    for j in ln:
+        z[:, j] = np.exp(z[:, j])
+    X[:] = z.astype(np.float32)
+    w = normals(2000 + c, 0, F)
+    x = X.astype(np.float64)
+    p = 1.0 / (1.0 + np.exp(-(x @ w + 0.5 * x[:, 0] * x[:, 1])))
+    lab = uniforms(3000 + c, off, n) < p
+    if flip:
+        lab = lab ^ (uniforms(4000 + c, off, n) < flip)
+    return X, lab.astype(np.float32)

@@ -1,0 +1,89 @@
+"""CPU-side checks of the product: the C ABI library loads and exports every
+declared entry point, and host binning equals the oracle's bit for bit."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests._pkg import capi, datagen, lgb
+from oracle import pyoracle as O
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "lgbm_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(lgbmhip_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = capi.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(capi.SIGNATURES), set(syms) ^ set(capi.SIGNATURES)
+
+
+def test_error_convention_without_gpu_call():
+    # binning with no rows is InvalidData (dataset.rs:555-592), reported through the ABI
+    with pytest.raises(lgb.LightGBMError, match="InvalidData"):
+        lgb.Dataset(np.zeros((0, 3), np.float32), label=np.zeros(0, np.float32))
+    with pytest.raises(lgb.LightGBMError, match="device_type"):
+        lgb.Dataset(np.zeros((3, 3), np.float32), params={"device": "cpu"})
+
+
+def _compare_bins(X, y, params):
+    ds = lgb.Dataset(X, label=y, params=params)
+    ods = O.Dataset(X, y, params)
+    assert ds.num_features == ods.num_features
+    off = 0
+    for f in range(ds.num_features):
+        m = ds.feature_meta(f)
+        o = ods.features[f]
+        assert (m["real_index"], m["num_bin"], m["missing_type"], m["default_bin"], m["most_freq_bin"],
+                m["bin_type"]) == (o.real, o.num_bin, o.missing_type, o.default_bin, o.most_freq_bin, o.bin_type)
+        up = ds.upper[off:off + m["num_bin"]]
+        np.testing.assert_array_equal(up, o.upper)
+        off += m["num_bin"]
+    np.testing.assert_array_equal(ds.bins.astype(np.uint16), ods.bins())
+    return ds
+
+
+@pytest.mark.parametrize("max_bin", [63, 255, 15])
+def test_host_binning_matches_oracle(max_bin):
+    X, y = datagen.binary_100k(20000)
+    _compare_bins(X, y, dict(max_bin=max_bin))
+
+
+def test_host_binning_missing_zero_skewed_and_sampled():
+    rng = np.random.default_rng(3)
+    n = 30000
+    X = np.stack([rng.normal(size=n), np.exp(rng.normal(size=n)), rng.integers(0, 5, n).astype(float),
+                  np.where(rng.random(n) < 0.7, 0.0, rng.normal(size=n)), rng.normal(size=n)], 1).astype(np.float32)
+    X[::11, 0] = np.nan
+    X[:, 4] = 1.0  # constant -> trivial, dropped
+    y = (rng.random(n) > 0.5).astype(np.float32)
+    ds = _compare_bins(X, y, dict(max_bin=255, bin_construct_sample_cnt=5000))
+    assert ds.num_features == 4
+
+
+def test_host_binning_categorical_matches_oracle():
+    rng = np.random.default_rng(5)
+    n = 20000
+    z = rng.zipf(1.2, size=n)
+    cat = np.where(z <= 50, z - 1, 49).astype(np.float32)
+    X = np.stack([rng.normal(size=n).astype(np.float32), cat], 1)
+    y = (rng.random(n) > 0.5).astype(np.float32)
+    _compare_bins(X, y, dict(max_bin=255, categorical_feature=[1]))
+
+
+def test_datagen_is_deterministic_and_shaped():
+    a = datagen.normals(7, 0, 1000)
+    b = datagen.normals(7, 0, 1000)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(datagen.normals(7, 500, 10), a[500:510])
+    assert abs(a.mean()) < 0.1 and abs(a.std() - 1) < 0.1
+    X, y = datagen.binary_100k(1000)
+    assert X.shape == (1000, 28) and set(np.unique(y)) <= {0.0, 1.0}

@@ -1,0 +1,386 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle.
+
+Tolerances:
+  * integer / index work (partition, quantized histograms, discretization,
+    split thresholds on identical histograms, prediction): bit-exact;
+  * FP64 histograms: the engine sums exact 64-bit fixed-point values, the
+    oracle sums sequentially in FP64; each bin may differ by at most
+    (count_bin + 1) * 2^-(s+1) + 64 ulp(|bin|), with 2^-s the fixed-point grid;
+  * gradients: libm exp vs the device exp may differ in the last ulp of the
+    f64 response, so at most 1e-5 of rows may differ, each by <= 1 f32 ulp.
+"""
+import numpy as np
+import pytest
+
+from tests._pkg import datagen, lgb, small_binary
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+P_BIN = dict(objective="binary", num_leaves=31, learning_rate=0.1, min_data_in_leaf=20, max_bin=63)
+
+
+def _booster(X, y, params):
+    ds = lgb.Dataset(X, label=y, params=params)
+    return ds, lgb.Booster(params, ds)
+
+
+def _fix_shift(count, m):
+    r = 1
+    while (1 << r) <= count and r < 62:
+        r += 1
+    e = 0
+    if m > 0:
+        e = int(np.frexp(np.float32(m))[1])
+    return 62 - r - e
+
+
+def test_histogram_f64_within_fixed_point_bound():
+    X, y = datagen.binary_100k(50000)
+    ds, b = _booster(X, y, P_BIN)
+    rng = np.random.default_rng(1)
+    g = rng.normal(size=len(y)).astype(np.float32)
+    h = rng.random(len(y)).astype(np.float32)
+    rows = np.sort(rng.choice(len(y), 20000, replace=False)).astype(np.uint32)
+    B = max(ds.feature_meta(f)["num_bin"] for f in range(ds.num_features))
+    out = np.zeros(ds.num_features * B * 2, np.float64)
+    nb = np.zeros(1, np.int32)
+    lib = lgb._capi.lib()
+    lgb._capi.check(lib.lgbmhip_build_histogram(b._ctx, lgb.ptr(rows), len(rows), lgb.ptr(g), lgb.ptr(h),
+                                                 lgb.ptr(out), lgb.ptr(nb)), b._ctx)
+    assert nb[0] == B
+    gpu = out.reshape(ds.num_features, B, 2)
+    mfb = np.array([ds.feature_meta(f)["most_freq_bin"] for f in range(ds.num_features)], np.int32)
+    cpu = O.hist_f64(ds.bins, mfb, B, rows, g, h)
+    sg = _fix_shift(len(rows), np.abs(g[rows]).max())
+    sh = _fix_shift(len(rows), np.abs(h[rows]).max())
+    cnt = np.zeros((ds.num_features, B))
+    for f in range(ds.num_features):
+        cnt[f] = np.bincount(ds.bins[rows, f], minlength=B)
+    for c, s in ((0, sg), (1, sh)):
+        bound = (cnt + 1) * 2.0 ** -(s + 1) + 64 * np.spacing(np.abs(cpu[:, :, c]))
+        assert np.all(np.abs(gpu[:, :, c] - cpu[:, :, c]) <= bound)
+    # the most frequent bins are never accumulated
+    for f in range(ds.num_features):
+        assert gpu[f, mfb[f]].tolist() == [0.0, 0.0]
+
+
+def test_histogram_is_deterministic_and_order_free():
+    X, y = datagen.binary_100k(30000)
+    ds, b = _booster(X, y, P_BIN)
+    rng = np.random.default_rng(2)
+    g = rng.normal(size=len(y)).astype(np.float32)
+    h = rng.random(len(y)).astype(np.float32)
+    rows = np.arange(len(y), dtype=np.uint32)
+    B = max(ds.feature_meta(f)["num_bin"] for f in range(ds.num_features))
+    lib = lgb._capi.lib()
+    outs = []
+    for perm in (rows, rows[::-1].copy(), rng.permutation(rows).astype(np.uint32)):
+        out = np.zeros(ds.num_features * B * 2, np.float64)
+        nb = np.zeros(1, np.int32)
+        lgb._capi.check(lib.lgbmhip_build_histogram(b._ctx, lgb.ptr(perm), len(perm), lgb.ptr(g), lgb.ptr(h),
+                                                     lgb.ptr(out), lgb.ptr(nb)), b._ctx)
+        outs.append(out)
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+def test_histogram_int_bit_exact():
+    X, y = datagen.binary_100k(40000)
+    params = dict(P_BIN, use_quantized_grad=True, num_grad_quant_bins=16)
+    ds, b = _booster(X, y, params)
+    rng = np.random.default_rng(3)
+    gq = rng.integers(-8, 9, len(y)).astype(np.int8)
+    hq = rng.integers(0, 17, len(y)).astype(np.int8)
+    rows = np.sort(rng.choice(len(y), 25000, replace=False)).astype(np.uint32)
+    B = max(ds.feature_meta(f)["num_bin"] for f in range(ds.num_features))
+    out = np.zeros(ds.num_features * B, np.int64)
+    nb = np.zeros(1, np.int32)
+    lib = lgb._capi.lib()
+    lgb._capi.check(lib.lgbmhip_build_histogram_int(b._ctx, lgb.ptr(rows), len(rows), lgb.ptr(gq), lgb.ptr(hq),
+                                                     lgb.ptr(out), lgb.ptr(nb)), b._ctx)
+    mfb = np.array([ds.feature_meta(f)["most_freq_bin"] for f in range(ds.num_features)], np.int32)
+    cpu = O.hist_int(ds.bins, mfb, B, rows, gq, hq)
+    np.testing.assert_array_equal(out.reshape(ds.num_features, B), cpu)
+
+
+def _random_hist(rng, nb, mfb, scale=1.0):
+    cnt = rng.integers(0, 60, nb).astype(np.float64)
+    g = rng.normal(size=nb) * cnt * scale
+    h = cnt * (0.05 + 0.2 * rng.random(nb))
+    hist = np.stack([g, h], 1)
+    tot_g, tot_h = g.sum(), h.sum()
+    hist[mfb] = 0.0
+    return hist, tot_g, tot_h, int(cnt.sum())
+
+
+@pytest.mark.parametrize("missing_type", [0, 2])
+def test_split_search_bit_exact_on_identical_histograms(missing_type):
+    X, y = small_binary(4000, 6, nan_col=0 if missing_type == 2 else None)
+    params = dict(P_BIN, min_data_in_leaf=5, max_bin=255)
+    ds, b = _booster(X, y, params)
+    rng = np.random.default_rng(4)
+    lib = lgb._capi.lib()
+    for f in range(ds.num_features):
+        m = ds.feature_meta(f)
+        if m["missing_type"] != missing_type:
+            continue
+        for trial in range(20):
+            hist, sg, sh, nd = _random_hist(rng, m["num_bin"], m["most_freq_bin"])
+            od = np.zeros(7)
+            oi = np.zeros(5, np.int32)
+            lgb._capi.check(lib.lgbmhip_find_best_split(b._ctx, f, lgb.ptr(np.ascontiguousarray(hist)), sg, sh, nd,
+                                                         0.0, lgb.ptr(od), lgb.ptr(oi)), b._ctx)
+            r = O.find_best_threshold(hist, m["num_bin"], m["missing_type"], m["default_bin"], m["most_freq_bin"],
+                                      m["bin_type"], params, sg, sh, nd)
+            assert bool(oi[4]) == r["splittable"]
+            if r["gain"] == -np.inf:
+                assert od[0] == -np.inf
+                continue
+            assert od[0] == r["gain"] and oi[0] == r["threshold"] and bool(oi[1]) == r["default_left"]
+            assert (oi[2], oi[3]) == (r["left_count"], r["right_count"])
+            assert od[1] == r["left_output"] and od[2] == r["right_output"]
+            assert (od[3], od[4], od[5], od[6]) == (r["left_sum_gradient"], r["left_sum_hessian"],
+                                                    r["right_sum_gradient"], r["right_sum_hessian"])
+
+
+def test_split_search_int_bit_exact():
+    X, y = small_binary(4000, 6)
+    params = dict(P_BIN, min_data_in_leaf=5, max_bin=63, use_quantized_grad=True, num_grad_quant_bins=16)
+    ds, b = _booster(X, y, params)
+    rng = np.random.default_rng(5)
+    lib = lgb._capi.lib()
+    for f in range(ds.num_features):
+        m = ds.feature_meta(f)
+        for trial in range(20):
+            nb = m["num_bin"]
+            cnt = rng.integers(0, 50, nb)
+            gs = rng.integers(-8, 9, nb) * cnt // 3
+            hs = cnt * 7
+            packed = (gs.astype(np.int64) << 32) + hs.astype(np.int64)
+            tot = int(packed.sum())
+            packed[m["most_freq_bin"]] = 0
+            gscale, hscale = 0.01, 0.002
+            od = np.zeros(7)
+            oi = np.zeros(5, np.int32)
+            og = np.zeros(2, np.int64)
+            lgb._capi.check(lib.lgbmhip_find_best_split_int(b._ctx, f, lgb.ptr(packed), tot, gscale, hscale,
+                                                             int(cnt.sum()), 0.0, lgb.ptr(od), lgb.ptr(oi),
+                                                             lgb.ptr(og)), b._ctx)
+            r = O.find_best_threshold_int(packed, nb, m["missing_type"], m["default_bin"], m["most_freq_bin"],
+                                          params, tot, gscale, hscale, int(cnt.sum()))
+            assert bool(oi[4]) == r["splittable"]
+            if r["gain"] == -np.inf:
+                continue
+            assert od[0] == r["gain"] and oi[0] == r["threshold"]
+            assert (oi[2], oi[3]) == (r["left_count"], r["right_count"])
+            assert (int(og[0]), int(og[1])) == (r["left_sum_gh"], r["right_sum_gh"])
+
+
+def test_partition_is_stable_and_exact():
+    X, y = small_binary(20000, 5, nan_col=2)
+    ds, b = _booster(X, y, dict(P_BIN, max_bin=255))
+    rng = np.random.default_rng(6)
+    lib = lgb._capi.lib()
+    for f in range(ds.num_features):
+        m = ds.feature_meta(f)
+        rows = np.sort(rng.choice(len(y), 12345, replace=False)).astype(np.uint32)
+        thr = int(rng.integers(0, m["num_bin"]))
+        for dl in (0, 1):
+            out = np.zeros_like(rows)
+            nl = np.zeros(1, np.int64)
+            lgb._capi.check(lib.lgbmhip_partition(b._ctx, lgb.ptr(rows), len(rows), f, thr, dl, lgb.ptr(out),
+                                                   lgb.ptr(nl)), b._ctx)
+            bins = ds.bins[rows, f].astype(int)
+            miss = (m["missing_type"] == 2) & (bins == m["num_bin"] - 1)
+            left = np.where(miss, bool(dl), bins <= thr)
+            np.testing.assert_array_equal(out, np.concatenate([rows[left], rows[~left]]))
+            assert nl[0] == left.sum()
+
+
+def test_binary_gradients_match_oracle():
+    X, y = datagen.binary_100k(200000)
+    ds, b = _booster(X, y, P_BIN)
+    rng = np.random.default_rng(7)
+    s = rng.normal(size=len(y)) * 3
+    g = np.zeros(len(y), np.float32)
+    h = np.zeros(len(y), np.float32)
+    lib = lgb._capi.lib()
+    lgb._capi.check(lib.lgbmhip_gradients(b._ctx, lgb.ptr(s), lgb.ptr(g), lgb.ptr(h)), b._ctx)
+    og, oh = O.binary_gradients(s, y)
+    for a, c in ((g, og), (h, oh)):
+        diff = a != c
+        assert diff.sum() <= max(1, int(1e-5 * len(y)))
+        if diff.any():
+            assert np.all(np.abs(a[diff].view(np.int32) - c[diff].view(np.int32)) <= 1)
+
+
+def test_multiclass_gradients_match_oracle():
+    rng = np.random.default_rng(8)
+    n, K = 50000, 5
+    X = rng.normal(size=(n, 6)).astype(np.float32)
+    y = rng.integers(0, K, n).astype(np.float32)
+    ds, b = _booster(X, y, dict(P_BIN, objective="multiclass", num_class=K))
+    s = rng.normal(size=n * K)
+    g = np.zeros(n * K, np.float32)
+    h = np.zeros(n * K, np.float32)
+    lib = lgb._capi.lib()
+    lgb._capi.check(lib.lgbmhip_gradients(b._ctx, lgb.ptr(s), lgb.ptr(g), lgb.ptr(h)), b._ctx)
+    og, oh = O.multiclass_gradients(s, y, K)
+    for a, c in ((g, og), (h, oh)):
+        diff = a != c
+        assert diff.sum() <= max(2, int(1e-4 * len(a)))
+        if diff.any():
+            assert np.all(np.abs(a[diff].view(np.int32) - c[diff].view(np.int32)) <= 1)
+
+
+def test_discretize_bit_exact():
+    X, y = datagen.binary_100k(30000)
+    params = dict(P_BIN, use_quantized_grad=True, num_grad_quant_bins=16, stochastic_rounding=True)
+    ds, b = _booster(X, y, params)
+    rng = np.random.default_rng(9)
+    g = rng.normal(size=len(y)).astype(np.float32) * 0.3
+    h = (rng.random(len(y)) * 0.25).astype(np.float32)
+    lib = lgb._capi.lib()
+    gq = np.zeros(len(y), np.int8)
+    hq = np.zeros(len(y), np.int8)
+    sc = np.zeros(2)
+    lgb._capi.check(lib.lgbmhip_discretize(b._ctx, lgb.ptr(g), lgb.ptr(h), 3, lgb.ptr(gq), lgb.ptr(hq),
+                                            lgb.ptr(sc)), b._ctx)
+    ogq, ohq, gs, hs = O.discretize(g, h, 16, seed=0, stochastic=True, num_calls=3)
+    assert (sc[0], sc[1]) == (gs, hs)
+    np.testing.assert_array_equal(gq, ogq)
+    np.testing.assert_array_equal(hq, ohq)
+
+
+def _compare_tree(gt, ot, value_rtol=1e-9):
+    """Structure must match exactly; values within FP64-summation noise."""
+    assert gt.num_leaves == ot.num_leaves
+    np.testing.assert_array_equal(gt.split_feature, ot.split_feature)
+    np.testing.assert_array_equal(gt.threshold_in_bin, ot.threshold_in_bin)
+    np.testing.assert_array_equal(gt.decision_type, ot.decision_type)
+    np.testing.assert_array_equal(gt.left_child, ot.left_child)
+    np.testing.assert_array_equal(gt.right_child, ot.right_child)
+    np.testing.assert_array_equal(gt.leaf_count, ot.leaf_count)
+    np.testing.assert_array_equal(gt.threshold, ot.threshold)
+    np.testing.assert_allclose(gt.leaf_value, ot.leaf_value, rtol=value_rtol, atol=1e-12)
+
+
+def test_learner_tree_matches_oracle_on_same_gradients():
+    X, y = datagen.binary_100k(100000)
+    params = dict(P_BIN, num_leaves=31, max_bin=255)
+    ds, b = _booster(X, y, params)
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    rng = np.random.default_rng(10)
+    trees_equal = 0
+    for t in range(4):
+        s = rng.normal(size=len(y)) * 0.5
+        g, h = O.binary_gradients(s, y)
+        b.train_tree(g, h)
+        ob.train_tree(g, h)
+        _compare_tree(b.tree(t), ob.tree(t))
+        trees_equal += 1
+        np.testing.assert_array_equal(b.leaf_of_row(), ob.leaf_of_row())
+    assert trees_equal == 4
+
+
+def test_quantized_learner_tree_bit_exact_structure():
+    X, y = datagen.binary_100k(60000)
+    params = dict(P_BIN, num_leaves=31, max_bin=63, use_quantized_grad=True, num_grad_quant_bins=16,
+                  quant_train_renew_leaf=True)
+    ds, b = _booster(X, y, params)
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    rng = np.random.default_rng(11)
+    for t in range(3):
+        s = rng.normal(size=len(y)) * 0.5
+        g, h = O.binary_gradients(s, y)
+        b.train_tree(g, h)
+        ob.train_tree(g, h)
+        # integer histograms are exact, so the structure and the split sums are too
+        _compare_tree(b.tree(t), ob.tree(t), value_rtol=1e-9)
+
+
+def test_boosting_matches_oracle_and_predicts():
+    X, y = datagen.binary_100k(100000)
+    params = dict(P_BIN, num_leaves=31, max_bin=255)
+    ds, b = _booster(X, y, params)
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    n_iter = 10
+    identical = 0
+    for it in range(n_iter):
+        b.update()
+        ob.update()
+        gt, ot = b.tree(it), ob.tree(it)
+        if (gt.num_leaves == ot.num_leaves and np.array_equal(gt.split_feature, ot.split_feature)
+                and np.array_equal(gt.threshold_in_bin, ot.threshold_in_bin)):
+            identical += 1
+    # tree structures are expected to coincide (near-ties aside)
+    assert identical >= n_iter - 1
+    s_gpu, s_cpu = b.train_scores(), ob.scores()
+    assert np.max(np.abs(s_gpu - s_cpu)) < 1e-6 or identical < n_iter
+    raw = b.predict(X, raw_score=True)
+    # training scores equal raw predictions (end_to_end.rs:53-57)
+    assert np.all(np.abs(raw - s_gpu) <= 1e-9 * (1 + np.abs(s_gpu)))
+    # binned prediction agrees with raw prediction on training rows
+    pb = b.predict_binned(ds.bins)[:, 0]
+    assert np.all(np.abs(pb - raw) <= 1e-9 * (1 + np.abs(raw)))
+    assert abs(O.auc(s_gpu, y) - O.auc(s_cpu, y)) < 1e-4
+
+
+def test_training_is_deterministic():
+    X, y = datagen.binary_100k(50000)
+    params = dict(P_BIN, num_leaves=63, max_bin=255)
+    runs = []
+    for _ in range(2):
+        ds, b = _booster(X, y, params)
+        for _ in range(5):
+            b.update()
+        runs.append(b.train_scores())
+        b.free()
+    np.testing.assert_array_equal(runs[0], runs[1])
+
+
+def test_graph_and_stream_paths_agree():
+    X, y = datagen.binary_100k(40000)
+    out = []
+    for use_graph in (False, True):
+        params = dict(P_BIN, num_leaves=31, use_graph=use_graph)
+        ds, b = _booster(X, y, params)
+        for _ in range(4):
+            b.update()
+        out.append(b.train_scores())
+        b.free()
+    np.testing.assert_array_equal(out[0], out[1])
+
+
+def test_quantized_boosting_quality_close_to_plain():
+    X, y = datagen.binary_100k(100000)
+    Xv, yv = datagen.holdout("binary_100k", 20000)
+    aucs = {}
+    for q in (False, True):
+        params = dict(P_BIN, num_leaves=31, max_bin=63, use_quantized_grad=q, num_grad_quant_bins=16,
+                      quant_train_renew_leaf=True)
+        ds, b = _booster(X, y, params)
+        for _ in range(20):
+            b.update()
+        aucs[q] = O.auc(b.predict(Xv, raw_score=True), yv)
+    assert abs(aucs[True] - aucs[False]) < 5e-3, aucs
+
+
+def test_multiclass_boosting_trains():
+    rng = np.random.default_rng(12)
+    n, K = 30000, 4
+    X = rng.normal(size=(n, 8)).astype(np.float32)
+    W = rng.normal(size=(8, K))
+    y = np.argmax(X @ W + rng.gumbel(size=(n, K)), 1).astype(np.float32)
+    params = dict(objective="multiclass", num_class=K, num_leaves=15, max_bin=63)
+    ds, b = _booster(X, y, params)
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    for it in range(3):
+        b.update()
+        ob.update()
+    same = sum(np.array_equal(b.tree(t).split_feature, ob.tree(t).split_feature) for t in range(3 * K))
+    assert same >= 3 * K - 1
+    p = b.predict(X)
+    assert p.shape == (n, K) and np.allclose(p.sum(1), 1.0)
+    acc = (p.argmax(1) == y).mean()
+    assert acc > 0.5
