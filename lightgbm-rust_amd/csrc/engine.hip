@@ -233,6 +233,11 @@ struct Ctx {
   std::vector<HostTree> trees;
   bool models_empty_init = true;
   int iter = 0;
+  // lambdarank state (rank.rs:129-341)
+  DevBuf<int> d_qb, r_rank, r_sorted;
+  DevBuf<double> r_imd, r_table, r_gain, r_disc;
+  DevBuf<float> r_lam, r_hes;
+  double r_min_in = 0, r_max_in = 0, r_factor = 0;
   // prediction cache
   bool pred_dirty = true;
   DevBuf<PredNode> p_nodes;
@@ -347,6 +352,67 @@ static void plan_hist(Ctx* c, bool quant) {
   set_hist_lds_limits(c->hist_lds);
 }
 
+// Host-side tables of the lambdarank objective, computed as the reference
+// does (rank.rs:313-328 inverse max DCG, :166-179 sigmoid table, dcg.rs:27-52).
+static void setup_rank(Ctx* c) {
+  const auto& qb = c->qb;
+  if (qb.size() < 2) throw Error(LGBMHIP_INVALID_DATA, "Ranking tasks require query information");
+  if (c->has_weight) throw Error(LGBMHIP_UNSUPPORTED, "lambdarank with row weights");
+  if (qb.front() != 0 || qb.back() != (int32_t)c->N) throw Error(LGBMHIP_INVALID_DATA, "query boundaries do not cover the data");
+  const int Q = (int)qb.size() - 1;
+  std::vector<double> gain(31), disc(10000);
+  gain[0] = 0.0;
+  for (int i = 1; i < 31; ++i) gain[i] = (double)((1 << i) - 1);
+  for (int i = 0; i < 10000; ++i) disc[i] = 1.0 / std::log2(2.0 + (double)i);
+  for (float l : c->label) {
+    if (std::fabs(l - (float)(int)l) > (double)1e-15f) throw Error(LGBMHIP_INVALID_DATA, "label should be int type for ranking task");
+    if (l < 0.0f) throw Error(LGBMHIP_INVALID_DATA, "Label should be non-negative for ranking task");
+    if ((int)l >= 31) throw Error(LGBMHIP_INVALID_DATA, "Label is not less than the number of label mappings (31)");
+  }
+  const int k = std::max(c->opts.lambdarank_truncation_level, 0);
+  std::vector<double> imd(Q);
+  for (int q = 0; q < Q; ++q) {
+    const int cnt = qb[q + 1] - qb[q];
+    if (cnt < 0) throw Error(LGBMHIP_INVALID_DATA, "query boundaries must ascend");
+    if (cnt > 10000) throw Error(LGBMHIP_INVALID_DATA, "Number of rows exceeds upper limit of 10000 for a query");
+    std::vector<int> cntl(31, 0);
+    for (int i = qb[q]; i < qb[q + 1]; ++i) cntl[(int)c->label[i]]++;
+    int top = 30;
+    double ret = 0.0;
+    for (int j = 0; j < std::min(k, cnt); ++j) {
+      while (top > 0 && cntl[top] <= 0) top--;
+      if (top < 0) break;
+      ret += disc[j] * gain[top];
+      cntl[top]--;
+    }
+    imd[q] = ret > 0.0 ? 1.0 / ret : ret;
+  }
+  const int bins = 1024 * 1024;
+  const double sig = c->opts.sigmoid;
+  c->r_min_in = -50.0 / sig / 2.0;
+  c->r_max_in = -c->r_min_in;
+  c->r_factor = (double)bins / (c->r_max_in - c->r_min_in);
+  std::vector<double> table(bins);
+  for (int i = 0; i < bins; ++i) {
+    const double sc = (double)i / c->r_factor + c->r_min_in;
+    table[i] = 1.0 / (1.0 + std::exp(sc * sig));
+  }
+  c->d_qb.alloc(qb.size());
+  c->r_imd.alloc(Q);
+  c->r_table.alloc(bins);
+  c->r_gain.alloc(31);
+  c->r_disc.alloc(10000);
+  c->r_rank.alloc(c->N);
+  c->r_sorted.alloc(c->N);
+  c->r_lam.alloc(c->N);
+  c->r_hes.alloc(c->N);
+  HIP_CHECK(hipMemcpy(c->d_qb.p, qb.data(), qb.size() * 4, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->r_imd.p, imd.data(), Q * 8, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->r_table.p, table.data(), bins * 8, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->r_gain.p, gain.data(), 31 * 8, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->r_disc.p, disc.data(), 10000 * 8, hipMemcpyHostToDevice));
+}
+
 static void setup_training(Ctx* c) {
   if (!c->N) throw Error(LGBMHIP_INVALID_DATA, "no bins uploaded");
   if (c->label.empty()) throw Error(LGBMHIP_INVALID_DATA, "no labels set");
@@ -356,7 +422,7 @@ static void setup_training(Ctx* c) {
   const bool quant = o.use_quantized_grad != 0;
   c->K = o.objective == 1 ? o.num_class : 1;
   if (c->K < 1 || c->K > 16) throw Error(LGBMHIP_UNSUPPORTED, "num_class must be in [1, 16]");
-  if (o.objective == 2) throw Error(LGBMHIP_UNSUPPORTED, "lambdarank is not available in this build");
+  if (o.objective == 2) setup_rank(c);
   c->L = std::max(c->sp.num_leaves, 2);
   if (quant) {
     for (auto& f : c->feats)
@@ -689,6 +755,30 @@ static void enqueue_gradients(Ctx* c) {
   if (c->opts.objective == 0) {
     hipLaunchKernelGGL(k_binary_grad, dim3(grid), dim3(256), 0, s, c->score.p, c->d_label.p, w, (long long)N,
                        c->opts.sigmoid, 1.0, 1.0, c->g.p, c->h.p, c->it.p);
+  } else if (c->opts.objective == 2) {
+    RankArgs a{};
+    a.score = c->score.p;
+    a.label = c->d_label.p;
+    a.qb = c->d_qb.p;
+    a.Q = (int)c->qb.size() - 1;
+    a.inv_max_dcg = c->r_imd.p;
+    a.table = c->r_table.p;
+    a.min_in = c->r_min_in;
+    a.max_in = c->r_max_in;
+    a.idx_factor = c->r_factor;
+    a.sigmoid = c->opts.sigmoid;
+    a.norm = c->opts.lambdarank_norm;
+    a.trunc = c->opts.lambdarank_truncation_level;
+    a.label_gain = c->r_gain.p;
+    a.discount = c->r_disc.p;
+    a.rank = c->r_rank.p;
+    a.sorted = c->r_sorted.p;
+    a.lam = c->r_lam.p;
+    a.hes = c->r_hes.p;
+    a.g = c->g.p;
+    a.h = c->h.p;
+    if (a.Q > 0) hipLaunchKernelGGL(k_lambdarank, dim3(a.Q), dim3(kRankThreads), 0, s, a);
+    hipLaunchKernelGGL(k_absmax_it, dim3(grid), dim3(256), 0, s, c->g.p, c->h.p, (long long)N, c->it.p);
   } else {
     hipLaunchKernelGGL(k_multiclass_grad, dim3(grid), dim3(256), 0, s, c->score.p, c->d_label.p, w, (long long)N,
                        c->K, c->g.p, c->h.p, c->it.p);

@@ -240,6 +240,166 @@ __global__ void k_multiclass_grad(const double* __restrict__ score, const float*
   }
 }
 
+// objective/rank.rs:192-257 LambdarankNdcg::one_query, one block per query.
+// The reference accumulates each document's f32 lambda/hessian in (i, j)
+// pair-loop order and the normalizer sum_lambdas as one f64 chain in the
+// same order. Outer iterations i run in sequence; within one, the partners
+// j > i each get their single contribution in parallel, while the anchor's
+// contributions and the f64 chain run in j order on one lane from LDS.
+struct RankArgs {
+  const double* score;
+  const float* label;
+  const int* qb;
+  int Q;
+  const double* inv_max_dcg;
+  const double* table;       // 1M-entry sigmoid table (rank.rs:166-179)
+  double min_in, max_in, idx_factor, sigmoid;
+  int norm, trunc;
+  const double* label_gain;  // 31 entries
+  const double* discount;    // 10000 entries
+  int* rank;                 // scratch [N]: rank of each row within its query
+  int* sorted;               // scratch [N]: row (query-local) at each rank
+  float* lam;                // scratch [N]: accumulators by rank
+  float* hes;
+  float* g;
+  float* h;
+};
+
+constexpr int kRankThreads = 128;
+constexpr int kRankChunk = 1024;
+
+__device__ __forceinline__ double rank_sigmoid(const RankArgs& a, double s) {
+  if (s <= a.min_in) return a.table[0];
+  if (s >= a.max_in) return a.table[1024 * 1024 - 1];
+  return a.table[(size_t)((s - a.min_in) * a.idx_factor)];
+}
+
+__global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
+  const int q = blockIdx.x;
+  if (q >= a.Q) return;
+  const int start = a.qb[q], cnt = a.qb[q + 1] - start;
+  const int tid = threadIdx.x;
+  const double* s = a.score + start;
+  const float* lab = a.label + start;
+  int* rank = a.rank + start;
+  int* sorted = a.sorted + start;
+  float* lam = a.lam + start;
+  float* hes = a.hes + start;
+  if (cnt <= 0) return;
+  // stable descending order: rank = #{j : s_j > s_i or (s_j == s_i and j < i)}
+  for (int i = tid; i < cnt; i += kRankThreads) {
+    const double si = s[i];
+    int r = 0;
+    for (int j = 0; j < cnt; ++j) {
+      const double sj = s[j];
+      r += (sj > si) || (!(sj < si) && !(sj > si) && j < i);
+    }
+    rank[i] = r;
+    sorted[r] = i;
+    lam[i] = 0.0f;
+    hes[i] = 0.0f;
+  }
+  __syncthreads();
+  const double best = s[sorted[0]];
+  int worst_idx = cnt - 1;
+  if (worst_idx > 0 && s[sorted[worst_idx]] == -INFINITY) worst_idx--;
+  const double worst = s[sorted[worst_idx]];
+  const double imd = a.inv_max_dcg[q];
+  const bool scale_by_delta = a.norm && best != worst;
+  __shared__ float st_l[kRankChunk], st_h[kRankChunk];
+  __shared__ double st_p[kRankChunk];
+  __shared__ int st_ok[kRankChunk];
+  double S = 0.0;
+  const int imax = min(cnt - 1, max(a.trunc, 0));
+  for (int i = 0; i < imax; ++i) {
+    const int ai = sorted[i];
+    if (s[ai] == -INFINITY) continue;
+    const float la = lab[ai];
+    float anc_l = lam[i], anc_h = hes[i];   // lane 0 only
+    for (int j0 = i + 1; j0 < cnt; j0 += kRankChunk) {
+      const int jn = min(cnt, j0 + kRankChunk);
+      for (int j = j0 + tid; j < jn; j += kRankThreads) {
+        const int bj = sorted[j];
+        const float lb = lab[bj];
+        int ok = s[bj] != -INFINITY && la != lb;
+        if (ok) {
+          const bool anchor_high = la > lb;
+          const int high = anchor_high ? ai : bj, low = anchor_high ? bj : ai;
+          const int hr = anchor_high ? i : j, lr = anchor_high ? j : i;
+          const double hs = s[high], ls = s[low];
+          const double hg = a.label_gain[(int)lab[high]], lg = a.label_gain[(int)lab[low]];
+          const double ds = hs - ls;
+          const double gap = hg - lg;
+          const double pd = fabs(a.discount[hr] - a.discount[lr]);
+          double dp = gap * pd * imd;
+          if (scale_by_delta) dp /= (double)0.01f + fabs(ds);
+          double pl = rank_sigmoid(a, ds);
+          double ph = pl * (1.0 - pl);
+          pl *= -a.sigmoid * dp;
+          ph *= a.sigmoid * a.sigmoid * dp;
+          const float fl = (float)pl, fh = (float)ph;
+          // the partner (rank j) gets its one contribution of this iteration
+          if (anchor_high) lam[j] -= fl;   // partner is low
+          else lam[j] += fl;
+          hes[j] += fh;
+          st_l[j - j0] = anchor_high ? fl : -fl;
+          st_h[j - j0] = fh;
+          st_p[j - j0] = pl;
+        }
+        st_ok[j - j0] = ok;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        for (int k = 0; k < jn - j0; ++k) {
+          if (!st_ok[k]) continue;
+          // anchor high: lambdas[high] += p; anchor low: lambdas[low] -= p
+          anc_l += st_l[k];
+          anc_h += st_h[k];
+          S -= 2.0 * st_p[k];
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      lam[i] = anc_l;
+      hes[i] = anc_h;
+    }
+    __syncthreads();
+  }
+  __shared__ double sfac;
+  __shared__ int sdo;
+  if (tid == 0) {
+    sdo = a.norm && S > 0.0;
+    sfac = sdo ? log2(1.0 + S) / S : 1.0;
+  }
+  __syncthreads();
+  for (int d = tid; d < cnt; d += kRankThreads) {
+    const int r = rank[d];
+    float l = lam[r], hh = hes[r];
+    if (sdo) {
+      l = (float)((double)l * sfac);
+      hh = (float)((double)hh * sfac);
+    }
+    a.g[start + d] = l;
+    a.h[start + d] = hh;
+  }
+}
+
+// max |g|, |h| after the ranking gradients (order-free).
+__global__ void k_absmax_it(const float* g, const float* h, long long n, IterDev* it) {
+  unsigned mg = 0, mh = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    mg = max(mg, __float_as_uint(fabsf(g[i])));
+    mh = max(mh, __float_as_uint(fabsf(h[i])));
+  }
+  mg = wave_max_u(mg);
+  mh = wave_max_u(mh);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&it->max_abs_g_bits, mg);
+    atomicMax(&it->max_abs_h_bits, mh);
+  }
+}
+
 // ---------------------------------------------------------------- tree init
 // Identity partition in buffer 0, gradients of class k copied beside the row
 // indices, fixed-point root sums, state machine reset (learner/mod.rs:580-628).

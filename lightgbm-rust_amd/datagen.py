@@ -77,6 +77,36 @@ def binary_10p5m(n=10_500_000):
     return X, _binary_labels(2, X, flip_rate=0.05)
 
 
+def rank_2m(n_queries=20_000, max_size=2000, F=136):
+    """Lambdarank workload: queries of Geometric(p=1/100) size capped at 2000,
+    half N(0,1) features, half N(0,1) x Bernoulli(0.3) (70% zeros); graded
+    relevance 0-4 = clip(floor(5 Phi(z)), 0, 4) of a noisy linear score."""
+    c = 5
+    u = uniforms(5000 + c, 0, n_queries)
+    sizes = np.minimum(np.floor(np.log1p(-u) / np.log1p(-0.01)).astype(np.int64) + 1, max_size)
+    qb = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    n = int(qb[-1])
+    X = _features(c, n, F)
+    half = F // 2
+    keep = uniforms(6000 + c, 0, n * (F - half)).reshape(n, F - half) < 0.3
+    X[:, half:] = np.where(keep, X[:, half:], 0.0).astype(np.float32)
+    v = normals(2000 + c, 0, F)
+    z = X.astype(np.float64) @ v + 0.5 * normals(3000 + c, 0, n) * np.linalg.norm(v)
+    z = z / np.sqrt(np.linalg.norm(v) ** 2 * 1.25)
+    from math import erf, sqrt
+    phi = 0.5 * (1.0 + np.vectorize(erf)(z / sqrt(2.0))) if n <= 200_000 else _phi(z)
+    rel = np.clip(np.floor(5.0 * phi), 0, 4).astype(np.float32)
+    return X, rel, qb
+
+
+def _phi(z):
+    # Abramowitz-Stegun 7.1.26 erf, vectorised (|error| < 1.5e-7)
+    t = 1.0 / (1.0 + 0.3275911 * np.abs(z) / np.sqrt(2.0))
+    y = 1.0 - (((((1.061405429 * t - 1.453152027) * t) + 1.421413741) * t - 0.284496736) * t + 0.254829592) * t * \
+        np.exp(-(z * z) / 2.0)
+    return 0.5 * (1.0 + np.sign(z) * y)
+
+
 def holdout(workload, n):
     """A held-out sample of the same generator (different row range)."""
     if workload == "binary_10p5m":

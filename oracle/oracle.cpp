@@ -1576,27 +1576,120 @@ static double multiclass_boost_from_score(const float* label, int64_t n, int K, 
 }
 
 // ---------------------------------------------------------------- lambdarank
-// dcg.rs:37-121 (default label gains 2^i - 1, discount 1/log2(2+i))
+// dcg.rs:27-121: default label gains 2^i - 1 (i < 31), discounts 1/log2(2+i)
 struct Dcg {
   std::vector<double> label_gain, discount;
   Dcg() {
-    for (int i = 0; i < 31; ++i) label_gain.push_back((double)((1LL << i) - 1));
+    label_gain.push_back(0.0);
+    for (int i = 1; i < 31; ++i) label_gain.push_back((double)((1 << i) - 1));
     for (int i = 0; i < 10000; ++i) discount.push_back(1.0 / std::log2(2.0 + (double)i));
   }
-  // dcg.rs: CalMaxDCGAtK
+  // dcg.rs:68-84 CalMaxDCGAtK
   double max_dcg_at_k(int k, const float* label, int cnt) const {
-    std::vector<int> counts(label_gain.size(), 0);
-    for (int i = 0; i < cnt; ++i) counts[(int)label[i]]++;
-    double ret = 0.0;
+    std::vector<int> c(label_gain.size(), 0);
+    for (int i = 0; i < cnt; ++i) c[(int)label[i]]++;
     int top = (int)label_gain.size() - 1;
-    if (k > cnt) k = cnt;
-    for (int j = 0; j < k; ++j) {
-      while (top > 0 && counts[top] <= 0) top--;
+    double ret = 0.0;
+    for (int j = 0; j < std::min(k, cnt); ++j) {
+      while (top > 0 && c[top] <= 0) top--;
       if (top < 0) break;
       ret += discount[j] * label_gain[top];
-      counts[top]--;
+      c[top]--;
     }
     return ret;
+  }
+  // dcg.rs:105-118 CalDCG at one k
+  double dcg_at_k(int k, const float* label, const double* score, int cnt) const {
+    std::vector<int> idx(cnt);
+    for (int i = 0; i < cnt; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return score[a] > score[b]; });
+    double cur = 0.0;
+    for (int j = 0; j < std::min(k, cnt); ++j) cur += label_gain[(int)label[idx[j]]] * discount[j];
+    return cur;
+  }
+};
+
+// objective/rank.rs:129-257 LambdarankNdcg (no position bias, no weights)
+struct Lambdarank {
+  double sigmoid;
+  bool norm;
+  int trunc;
+  Dcg dcg;
+  std::vector<double> inv_max_dcg, table;
+  double min_in = -50.0, max_in = 50.0, idx_factor = 0.0;
+  static const int kBins = 1024 * 1024;
+  Lambdarank(double sig, bool nrm, int tr, const float* label, const std::vector<int>& qb) : sigmoid(sig), norm(nrm), trunc(tr) {
+    int k = std::max(tr, 0);
+    for (size_t q = 0; q + 1 < qb.size(); ++q) {
+      double v = dcg.max_dcg_at_k(k, label + qb[q], qb[q + 1] - qb[q]);
+      inv_max_dcg.push_back(v > 0.0 ? 1.0 / v : v);
+    }
+    // rank.rs:166-179 ConstructSigmoidTable
+    min_in = min_in / sigmoid / 2.0;
+    max_in = -min_in;
+    idx_factor = (double)kBins / (max_in - min_in);
+    table.resize(kBins);
+    for (int i = 0; i < kBins; ++i) {
+      double sc = (double)i / idx_factor + min_in;
+      table[i] = 1.0 / (1.0 + std::exp(sc * sigmoid));
+    }
+  }
+  double get_sigmoid(double s) const {
+    if (s <= min_in) return table[0];
+    if (s >= max_in) return table[kBins - 1];
+    return table[(size_t)((s - min_in) * idx_factor)];
+  }
+  // rank.rs:192-257 one_query
+  void one_query(int q, const float* label, const double* score, int cnt, float* lambdas, float* hess) const {
+    double imd = inv_max_dcg[q];
+    for (int i = 0; i < cnt; ++i) { lambdas[i] = 0.0f; hess[i] = 0.0f; }
+    if (cnt == 0) return;
+    std::vector<int> sorted(cnt);
+    for (int i = 0; i < cnt; ++i) sorted[i] = i;
+    std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) { return score[a] > score[b]; });
+    double best = score[sorted[0]];
+    int worst_idx = cnt - 1;
+    if (worst_idx > 0 && score[sorted[worst_idx]] == K_MIN_SCORE) worst_idx--;
+    double worst = score[sorted[worst_idx]];
+    double sum_lambdas = 0.0;
+    for (int i = 0; i + 1 < cnt && (long)i < (long)trunc; ++i) {
+      if (score[sorted[i]] == K_MIN_SCORE) continue;
+      for (int j = i + 1; j < cnt; ++j) {
+        if (score[sorted[j]] == K_MIN_SCORE) continue;
+        if (label[sorted[i]] == label[sorted[j]]) continue;
+        int hr = label[sorted[i]] > label[sorted[j]] ? i : j;
+        int lr = label[sorted[i]] > label[sorted[j]] ? j : i;
+        int high = sorted[hr], low = sorted[lr];
+        double hs = score[high], ls = score[low];
+        double hg = dcg.label_gain[(int)label[high]], lg = dcg.label_gain[(int)label[low]];
+        double hd = dcg.discount[hr], ld = dcg.discount[lr];
+        double ds = hs - ls;
+        double gap = hg - lg;
+        double pd = std::fabs(hd - ld);
+        double dp = gap * pd * imd;
+        if (norm && best != worst) dp /= (double)0.01f + std::fabs(ds);
+        double pl = get_sigmoid(ds);
+        double ph = pl * (1.0 - pl);
+        pl *= -sigmoid * dp;
+        ph *= sigmoid * sigmoid * dp;
+        lambdas[low] -= (float)pl;
+        hess[low] += (float)ph;
+        lambdas[high] += (float)pl;
+        hess[high] += (float)ph;
+        sum_lambdas -= 2.0 * pl;
+      }
+    }
+    if (norm && sum_lambdas > 0.0) {
+      double f = std::log2(1.0 + sum_lambdas) / sum_lambdas;
+      for (int i = 0; i < cnt; ++i) {
+        lambdas[i] = (float)((double)lambdas[i] * f);
+        hess[i] = (float)((double)hess[i] * f);
+      }
+    }
+  }
+  void gradients(const double* score, const float* label, const std::vector<int>& qb, float* g, float* h) const {
+    for (size_t q = 0; q + 1 < qb.size(); ++q)
+      one_query((int)q, label + qb[q], score + qb[q], qb[q + 1] - qb[q], g + qb[q], h + qb[q]);
   }
 };
 
@@ -1616,6 +1709,7 @@ struct OrcBooster {
   std::vector<double> init_scores;
   int iter = 0;
   std::vector<std::vector<TraceEntry>> traces;
+  Lambdarank* rank = nullptr;
 };
 
 // boosting.rs:957-1139 (gbdt, no bagging, built-in objective)
@@ -1627,7 +1721,8 @@ static int booster_train_one_iter(OrcBooster* b) {
     // boosting.rs:484-511
     if (b->models.empty() && b->prm.boost_from_average) {
       double v = 0.0;
-      if (b->prm.objective == 0) v = binary_boost_from_score(b->ds->label.data(), n, b->prm.sigmoid);
+      if (b->prm.objective == 2) v = 0.0;
+      else if (b->prm.objective == 0) v = binary_boost_from_score(b->ds->label.data(), n, b->prm.sigmoid);
       else if (b->prm.objective == 1) v = multiclass_boost_from_score(b->ds->label.data(), n, K, k);
       if (std::fabs(v) > K_EPSILON) {
         for (int64_t i = 0; i < n; ++i) b->scores[(size_t)k * n + i] += v;
@@ -1637,6 +1732,7 @@ static int booster_train_one_iter(OrcBooster* b) {
   }
   if (b->prm.objective == 0) binary_gradients(b->scores.data(), b->ds->label.data(), n, b->prm.sigmoid, b->grad.data(), b->hess.data());
   else if (b->prm.objective == 1) multiclass_gradients(b->scores.data(), b->ds->label.data(), n, K, b->grad.data(), b->hess.data());
+  else b->rank->gradients(b->scores.data(), b->ds->label.data(), b->ds->query_boundaries, b->grad.data(), b->hess.data());
   bool cont = false;
   for (int k = 0; k < K; ++k) {
     Tree t = b->learner->train(b->grad.data() + (size_t)k * n, b->hess.data() + (size_t)k * n);
@@ -1705,6 +1801,11 @@ void* orc_booster_create(void* d, const char* params) {
   b->prm = p;
   b->K = p.objective == 1 ? p.num_class : 1;
   b->learner = new Learner(b->ds, p);
+  if (p.objective == 2) {
+    if (b->ds->query_boundaries.empty()) { delete b->learner; delete b; return nullptr; }
+    b->rank = new Lambdarank(p.sigmoid, p.lambdarank_norm, p.lambdarank_truncation_level, b->ds->label.data(),
+                             b->ds->query_boundaries);
+  }
   b->scores.assign((size_t)b->K * b->ds->n, 0.0);
   b->grad.assign((size_t)b->K * b->ds->n, 0.0f);
   b->hess.assign((size_t)b->K * b->ds->n, 0.0f);
@@ -1713,6 +1814,7 @@ void* orc_booster_create(void* d, const char* params) {
 void orc_booster_free(void* p) {
   OrcBooster* b = (OrcBooster*)p;
   delete b->learner;
+  delete b->rank;
   delete b;
 }
 int orc_booster_train_one_iter(void* p) { return booster_train_one_iter((OrcBooster*)p); }
@@ -1934,6 +2036,30 @@ int orc_value_to_bin_num(const double* upper, int num_bin, int missing_type, dou
   m.bin_type = BT_NUM;
   return (int)m.value_to_bin(v);
 }
+
+void orc_dataset_set_query(void* d, const int* qb, int nq) {
+  ((Dataset*)d)->query_boundaries.assign(qb, qb + nq + 1);
+}
+// Lambdarank gradients of every query (rank.rs:63-108 without weights / position bias).
+void orc_lambdarank_gradients(const double* score, const float* label, const int* qb, int nq, double sigmoid,
+                              int norm, int trunc, float* g, float* h) {
+  std::vector<int> b(qb, qb + nq + 1);
+  Lambdarank lr(sigmoid, norm != 0, trunc, label, b);
+  lr.gradients(score, label, b, g, h);
+}
+// metric.rs:551-600 NDCG@k averaged over queries (queries with max DCG 0 count as 1).
+double orc_ndcg(const double* score, const float* label, const int* qb, int nq, int k) {
+  Dcg d;
+  double sum = 0.0;
+  for (int q = 0; q < nq; ++q) {
+    int cnt = qb[q + 1] - qb[q];
+    double m = d.max_dcg_at_k(k, label + qb[q], cnt);
+    if (m <= 0.0) { sum += 1.0; continue; }
+    sum += d.dcg_at_k(k, label + qb[q], score + qb[q], cnt) / m;
+  }
+  return sum / (double)nq;
+}
+double orc_max_dcg_at_k(const float* label, int cnt, int k) { Dcg d; return d.max_dcg_at_k(k, label, cnt); }
 
 // metric.rs:701-729 (stable order for ties; ties are grouped by threshold)
 double orc_auc(const double* score, const float* label, int64_t n) {

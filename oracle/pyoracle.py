@@ -71,6 +71,10 @@ def lib():
             "orc_find_bin": (I, [PD, I, I64, I, I, I, I, I, PD, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I), PI]),
             "orc_value_to_bin_num": (I, [PD, I, I, D]),
             "orc_auc": (D, [PD, PF, I64]),
+            "orc_dataset_set_query": (None, [P, PI, I]),
+            "orc_lambdarank_gradients": (None, [PD, PF, PI, I, D, I, I, PF, PF]),
+            "orc_ndcg": (D, [PD, PF, PI, I, I]),
+            "orc_max_dcg_at_k": (D, [PF, I, I]),
             "orc_binary_logloss": (D, [PD, PF, I64, D]),
         }
         for name, (res, args) in sig.items():
@@ -117,6 +121,10 @@ class Dataset:
             fi.default_bin, fi.most_freq_bin, fi.bin_type = db.value, mfb.value, bt.value
             fi.upper = up[:nb].copy()
             self.features.append(fi)
+
+    def set_query_boundaries(self, qb):
+        qb = np.ascontiguousarray(qb, np.int32)
+        lib().orc_dataset_set_query(self.h, qb, len(qb) - 1)
 
     def bins(self):
         out = np.zeros((self.n, self.num_features), np.uint16)
@@ -293,6 +301,27 @@ def multiclass_gradients(scores, label, K):
     lib().orc_multiclass_gradients(np.ascontiguousarray(scores, np.float64), np.ascontiguousarray(label, np.float32),
                                    n, K, g, h)
     return g, h
+
+
+def lambdarank_gradients(scores, label, qb, sigmoid=1.0, norm=True, trunc=30):
+    n = len(label)
+    g = np.zeros(n, np.float32)
+    h = np.zeros(n, np.float32)
+    qb = np.ascontiguousarray(qb, np.int32)
+    lib().orc_lambdarank_gradients(np.ascontiguousarray(scores, np.float64), np.ascontiguousarray(label, np.float32),
+                                   qb, len(qb) - 1, sigmoid, int(norm), trunc, g, h)
+    return g, h
+
+
+def ndcg(scores, label, qb, k=10):
+    qb = np.ascontiguousarray(qb, np.int32)
+    return lib().orc_ndcg(np.ascontiguousarray(scores, np.float64), np.ascontiguousarray(label, np.float32), qb,
+                          len(qb) - 1, k)
+
+
+def max_dcg_at_k(label, k):
+    lab = np.ascontiguousarray(label, np.float32)
+    return lib().orc_max_dcg_at_k(lab, len(lab), k)
 
 
 def discretize(g, h, bins, seed=0, stochastic=True, num_calls=1):

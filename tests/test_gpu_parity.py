@@ -384,3 +384,43 @@ def test_multiclass_boosting_trains():
     assert p.shape == (n, K) and np.allclose(p.sum(1), 1.0)
     acc = (p.argmax(1) == y).mean()
     assert acc > 0.5
+
+
+def test_lambdarank_gradients_match_oracle():
+    X, rel, qb = datagen.rank_2m(400, 300)
+    params = dict(objective="lambdarank", num_leaves=31, max_bin=63)
+    sizes = np.diff(qb)
+    ds = lgb.Dataset(X, label=rel, group=sizes, params=params)
+    b = lgb.Booster(params, ds)
+    rng = np.random.default_rng(13)
+    s = np.round(rng.normal(size=len(rel)), 2)  # ties exercise the stable order
+    g = np.zeros(len(rel), np.float32)
+    h = np.zeros(len(rel), np.float32)
+    lib = lgb._capi.lib()
+    lgb._capi.check(lib.lgbmhip_gradients(b._ctx, lgb.ptr(s), lgb.ptr(g), lgb.ptr(h)), b._ctx)
+    og, oh = O.lambdarank_gradients(s, rel, qb)
+    # identical order of every f32 accumulation; only log2 of the normaliser may differ in its last ulp
+    for a, c in ((g, og), (h, oh)):
+        diff = a != c
+        assert diff.sum() <= max(2, int(1e-4 * len(a)))
+        if diff.any():
+            assert np.all(np.abs(a[diff].view(np.int32) - c[diff].view(np.int32)) <= 1)
+
+
+def test_lambdarank_boosting_improves_ndcg():
+    X, rel, qb = datagen.rank_2m(300, 300)
+    params = dict(objective="lambdarank", num_leaves=31, max_bin=63, min_data_in_leaf=20)
+    ds = lgb.Dataset(X, label=rel, group=np.diff(qb), params=params)
+    b = lgb.Booster(params, ds)
+    ods = O.Dataset(X, rel, params)
+    ods.set_query_boundaries(qb)
+    ob = O.Booster(ods, params)
+    n0 = O.ndcg(np.zeros(len(rel)), rel, qb, 10)
+    for it in range(5):
+        b.update()
+        ob.update()
+    same = sum(np.array_equal(b.tree(t).split_feature, ob.tree(t).split_feature) for t in range(5))
+    assert same >= 4
+    n1 = O.ndcg(b.train_scores(), rel, qb, 10)
+    n_cpu = O.ndcg(ob.scores(), rel, qb, 10)
+    assert n1 > n0 and abs(n1 - n_cpu) < 1e-3

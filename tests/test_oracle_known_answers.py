@@ -274,3 +274,106 @@ def test_training_scores_equal_raw_prediction():
     s = b.scores()
     assert np.all(np.abs(raw - s) <= 1e-9 * (1 + np.abs(s)))
     assert O.binary_logloss(s, y) < 0.4
+
+
+def test_dcg_known_answers():
+    # dcg.rs:195-209
+    label = np.array([0.0, 2.0, 1.0], np.float32)
+    ideal = 3.0 + 1.0 / np.log2(3.0)
+    assert O.max_dcg_at_k(label, 3) == ideal
+    assert O.max_dcg_at_k(label, 1) == 3.0
+
+
+def test_lambdarank_pair_known_answer():
+    # objective/rank.rs:491-509: one query, two documents, no normalisation
+    g, h = O.lambdarank_gradients(np.zeros(2), np.array([1.0, 0.0], np.float32), np.array([0, 2]), norm=False)
+    dndcg = 1.0 * (1.0 - 1.0 / np.log2(3.0))
+    # the table value at delta 0 is sigmoid of the bin's score, within 1e-4 of 0.5
+    p = 0.5
+    assert abs(float(-g[0]) / dndcg - p) < 1e-4
+    assert g[0] == -g[1] and g[0] < 0
+    assert h[0] == h[1] and abs(float(h[0]) / dndcg - 0.25) < 1e-4
+
+
+def test_lambdarank_bruteforce_small_queries():
+    """Pairwise LambdaRank restated in numpy (with the reference's sigmoid table
+    and f32 accumulation order) on tiny queries."""
+    rng = np.random.default_rng(21)
+    sizes = [1, 2, 5, 9, 40]
+    qb = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    n = qb[-1]
+    lab = rng.integers(0, 5, n).astype(np.float32)
+    s = np.round(rng.normal(size=n), 1)  # ties exercise the stable order
+    g, h = O.lambdarank_gradients(s, lab, qb, norm=True, trunc=30)
+    bins = 1024 * 1024
+    mn = -50.0 / 2.0
+    fac = bins / (2 * -mn)
+    gain = np.array([0.0] + [(1 << i) - 1 for i in range(1, 31)], np.float64)
+    disc = 1.0 / np.log2(2.0 + np.arange(10000))
+
+    def sig(x):
+        if x <= mn:
+            i = 0
+        elif x >= -mn:
+            i = bins - 1
+        else:
+            i = int((x - mn) * fac)
+        return 1.0 / (1.0 + np.exp((i / fac + mn)))
+
+    for q in range(len(sizes)):
+        a, b = qb[q], qb[q + 1]
+        cnt = b - a
+        if cnt == 0:
+            continue
+        ls, ll = s[a:b], lab[a:b]
+        order = sorted(range(cnt), key=lambda i: -ls[i])  # Python's sort is stable
+        srt = sorted(ll, reverse=True)
+        mx = sum(disc[j] * gain[int(srt[j])] for j in range(min(30, cnt)))
+        imd = 1.0 / mx if mx > 0 else mx
+        lam = np.zeros(cnt, np.float32)
+        hes = np.zeros(cnt, np.float32)
+        best, worst = ls[order[0]], ls[order[-1]]
+        S = 0.0
+        for i in range(min(cnt - 1, 30)):
+            for j in range(i + 1, cnt):
+                if ll[order[i]] == ll[order[j]]:
+                    continue
+                hr, lr = (i, j) if ll[order[i]] > ll[order[j]] else (j, i)
+                hi, lo = order[hr], order[lr]
+                ds = ls[hi] - ls[lo]
+                dp = (gain[int(ll[hi])] - gain[int(ll[lo])]) * abs(disc[hr] - disc[lr]) * imd
+                if best != worst:
+                    dp /= float(np.float32(0.01)) + abs(ds)
+                pl = sig(ds)
+                ph = pl * (1.0 - pl)
+                pl *= -1.0 * dp
+                ph *= 1.0 * 1.0 * dp
+                lam[lo] = np.float32(lam[lo] - np.float32(pl))
+                hes[lo] = np.float32(hes[lo] + np.float32(ph))
+                lam[hi] = np.float32(lam[hi] + np.float32(pl))
+                hes[hi] = np.float32(hes[hi] + np.float32(ph))
+                S -= 2.0 * pl
+        if S > 0:
+            f = np.log2(1.0 + S) / S
+            lam = (lam.astype(np.float64) * f).astype(np.float32)
+            hes = (hes.astype(np.float64) * f).astype(np.float32)
+        np.testing.assert_array_equal(g[a:b], lam)
+        np.testing.assert_array_equal(h[a:b], hes)
+
+
+def test_ndcg_matches_sklearn_on_tie_free_queries():
+    from sklearn.metrics import ndcg_score
+    rng = np.random.default_rng(22)
+    sizes = [5, 12, 30, 7]
+    qb = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    lab = rng.integers(0, 5, qb[-1]).astype(np.float32)
+    s = rng.normal(size=qb[-1])
+    ours = O.ndcg(s, lab, qb, 10)
+    ref = []
+    for q in range(len(sizes)):
+        a, b = qb[q], qb[q + 1]
+        if lab[a:b].max() == 0:
+            ref.append(1.0)
+            continue
+        ref.append(ndcg_score([2.0 ** lab[a:b] - 1], [s[a:b]], k=10))
+    assert abs(ours - np.mean(ref)) < 1e-12
