@@ -135,7 +135,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
-        cparams = dict(base)
+        cparams = {k: v for k, v in base.items() if k != "device_type"}
         if main_form == "quant":
             cparams.update(use_quantized_grad=True, num_grad_quant_bins=16, quant_train_renew_leaf=True)
         try:

@@ -98,6 +98,8 @@ struct StepDev {
   int counter;      // last-block-done counter
   int counter2;
   double root_output;  // the root's output before any split (learner/mod.rs:647)
+  long long seg_begin, seg_count;  // the segment of the leaf being split
+  int seg_buf;
 };
 
 // Per-iteration scalars written by device reductions.

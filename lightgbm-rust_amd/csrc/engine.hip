@@ -207,6 +207,8 @@ struct Ctx {
   DevBuf<double> pool;
   DevBuf<long long> ipool;
   DevBuf<SplitRec> feat_split, best, cur, log;
+  DevBuf<double> best_gain;
+  DevBuf<int> best_feat;
   DevBuf<uint8_t> spl;
   DevBuf<StepDev> st;
   DevBuf<LeafDev> leaves;
@@ -220,7 +222,7 @@ struct Ctx {
   DevBuf<int8_t> gq, hq;
   std::unique_ptr<Mt> start_eng;
   int n_calls = 0;
-  int hist_tpr = 8, hist_fb = 0, hist_ng = 1, hist_chunks = 512, hist_min_rows = 2048;
+  int hist_tpr = 8, hist_fb = 0, hist_ng = 1, hist_chunks = 512, hist_min_rows = 1024;
   size_t hist_lds = 0;
   int part_grid = 1024;
   // pinned copies of the per-tree results
@@ -333,7 +335,8 @@ static void upload_bins(Ctx* c, const lgbmhip_feature_meta* metas, int F, const 
 static void plan_hist(Ctx* c, bool quant) {
   const size_t per_feat = (size_t)c->B * 2 * (quant ? 4 : 8);
   size_t budget = std::min<size_t>(c->lds_max, 160 * 1024);
-  int fb = (int)std::min<size_t>((size_t)c->F, budget / per_feat);
+  // at most 128 bytes (32 threads of 4 bins) of a row per block
+  int fb = (int)std::min<size_t>(std::min<size_t>((size_t)c->F, 128), budget / per_feat);
   fb = std::max(4, fb / 4 * 4);
   if (fb >= c->F) fb = (c->F + 3) / 4 * 4;
   int ng = (c->F + fb - 1) / fb;
@@ -446,6 +449,8 @@ static void setup_training(Ctx* c) {
   else { c->pool.alloc((size_t)L * F * B * 2); c->ipool.free(); }
   c->feat_split.alloc(2 * F);
   c->best.alloc(L);
+  c->best_gain.alloc(L);
+  c->best_feat.alloc(L);
   c->cur.alloc(1);
   c->log.alloc(L);
   c->spl.alloc((size_t)L * F);
@@ -459,7 +464,12 @@ static void setup_training(Ctx* c) {
   c->leafval.alloc((size_t)K * L);
   c->renew.alloc((size_t)2 * L);
   c->renew.zero(c->stream);
-  c->part_grid = (int)std::min<int64_t>((N + part_tile() - 1) / part_tile(), 2048);
+  {
+    // each partition block owns at most kMaxOwnTiles tiles of the largest leaf
+    const int64_t tiles = (N + part_tile() - 1) / part_tile();
+    const int64_t need = (tiles + kMaxOwnTiles - 1) / kMaxOwnTiles;
+    c->part_grid = (int)std::min<int64_t>(tiles, std::max<int64_t>(2 * c->cus, need));
+  }
   if (quant) {
     // quant.rs:45-79: per-row tables from block generators of `quant_table_threads` blocks
     std::vector<double> grv(N), hrv(N);
@@ -527,6 +537,7 @@ static HistArgs hist_args(Ctx* c, bool quant) {
   a.FB = c->hist_fb;
   a.NG = c->hist_ng;
   a.max_chunks = c->hist_chunks;
+  a.L = c->L;
   a.min_rows_per_chunk = c->hist_min_rows;
   a.acc = c->acc.p;
   a.direct = 0;
@@ -569,10 +580,14 @@ static PartArgs part_args(Ctx* c, int k) {
   a.gh[1] = c->gh[1].p;
   a.gh_bytes = c->opts.use_quantized_grad ? 2 : 8;
   a.tile_left = c->tiles.p;
+  a.feat_split = c->feat_split.p;
   a.cur = c->cur.p;
   a.best = c->best.p;
+  a.best_gain = c->best_gain.p;
+  a.best_feat = c->best_feat.p;
   a.log = c->log.p;
   a.p = c->sp;
+  a.F = c->F;
   a.L = c->L;
   a.it = c->it.p + k;
   a.quant = c->opts.use_quantized_grad != 0;
@@ -600,11 +615,12 @@ static void enqueue_tree(Ctx* c, int k, bool score_update, double lr) {
     hipLaunchKernelGGL(k_discretize, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, c->grv.p, c->hrv.p,
                        c->opts.num_grad_quant_bins, c->opts.stochastic_rounding, 0, c->starts.p, c->call_ctr.p,
                        c->it.p + k, c->idx[0].p, (char2*)c->gh[0].p, c->gq.p, c->hq.p, c->st.p, c->leaves.p,
-                       c->best.p, c->spl.p, c->F, c->L);
+                       c->best.p, c->spl.p, c->F, c->L, c->best_gain.p, c->best_feat.p);
     hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, c->call_ctr.p);
   } else {
     hipLaunchKernelGGL(k_tree_init, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, c->idx[0].p,
-                       (float2*)c->gh[0].p, c->it.p + k, c->st.p, c->leaves.p, c->best.p, c->spl.p, c->F, c->L);
+                       (float2*)c->gh[0].p, c->it.p + k, c->st.p, c->leaves.p, c->best.p, c->spl.p, c->F, c->L,
+                       c->best_gain.p, c->best_feat.p);
   }
   HistArgs ha = hist_args(c, quant);
   SplitArgs sa = split_args(c, k);
@@ -1268,7 +1284,7 @@ lgbmhip_status lgbmhip_discretize(lgbmhip_ctx* ctx, const float* grad, const flo
       hipLaunchKernelGGL(k_discretize, dim3(grid_for(N, 256, c->cus * 8)), dim3(256), 0, c->stream, (long long)N,
                          c->g.p, c->h.p, c->grv.p, c->hrv.p, c->opts.num_grad_quant_bins, c->opts.stochastic_rounding,
                          0, c->starts.p, c->call_ctr.p, c->it.p, c->idx[0].p, (char2*)c->gh[0].p, c->gq.p, c->hq.p,
-                         c->st.p, c->leaves.p, c->best.p, c->spl.p, c->F, c->L);
+                         c->st.p, c->leaves.p, c->best.p, c->spl.p, c->F, c->L, c->best_gain.p, c->best_feat.p);
       hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, c->stream, c->call_ctr.p);
       c->n_calls++;
     }

@@ -405,7 +405,8 @@ __global__ void k_absmax_it(const float* g, const float* h, long long n, IterDev
 // indices, fixed-point root sums, state machine reset (learner/mod.rs:580-628).
 __global__ void k_tree_init(long long n, const float* __restrict__ g, const float* __restrict__ h,
                             unsigned* __restrict__ idx0, float2* __restrict__ gh0, IterDev* it, StepDev* st,
-                            LeafDev* leaves, SplitRec* best, uint8_t* splittable, int F, int L) {
+                            LeafDev* leaves, SplitRec* best, uint8_t* splittable, int F, int L, double* best_gain,
+                            int* best_feat) {
   int sg = fix_shift(n, it->max_abs_g_bits), sh = fix_shift(n, it->max_abs_h_bits);
   long long ag = 0, ah = 0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -422,7 +423,11 @@ __global__ void k_tree_init(long long n, const float* __restrict__ g, const floa
     atomicAdd((unsigned long long*)&it->root_h_fix, (unsigned long long)ah);
   }
   if (blockIdx.x == 0) {
-    for (int l = threadIdx.x; l < L; l += blockDim.x) reset_rec(best[l]);
+    for (int l = threadIdx.x; l < L; l += blockDim.x) {
+      reset_rec(best[l]);
+      best_gain[l] = -INFINITY;
+      best_feat[l] = -1;
+    }
     for (int f = threadIdx.x; f < F; f += blockDim.x) splittable[f] = 1;
     if (threadIdx.x == 0) {
       st->num_leaves = 1;
@@ -457,7 +462,7 @@ __global__ void k_discretize(long long n, const float* __restrict__ g, const flo
                              int constant_hessian, const int* __restrict__ starts, const int* call_ctr, IterDev* it,
                              unsigned* __restrict__ idx0, char2* __restrict__ q0, int8_t* __restrict__ gq_out,
                              int8_t* __restrict__ hq_out, StepDev* st, LeafDev* leaves, SplitRec* best,
-                             uint8_t* splittable, int F, int L) {
+                             uint8_t* splittable, int F, int L, double* best_gain, int* best_feat) {
   double mg = (double)__uint_as_float(it->max_abs_g_bits);
   double mh = (double)__uint_as_float(it->max_abs_h_bits);
   double gs = mg / (double)(bins / 2);
@@ -492,7 +497,11 @@ __global__ void k_discretize(long long n, const float* __restrict__ g, const flo
     atomicAdd((unsigned long long*)&it->root_hq_sum, (unsigned long long)ahq);
   }
   if (blockIdx.x == 0) {
-    for (int l = threadIdx.x; l < L; l += blockDim.x) reset_rec(best[l]);
+    for (int l = threadIdx.x; l < L; l += blockDim.x) {
+      reset_rec(best[l]);
+      best_gain[l] = -INFINITY;
+      best_feat[l] = -1;
+    }
     for (int f = threadIdx.x; f < F; f += blockDim.x) splittable[f] = 1;
     if (threadIdx.x == 0) {
       it->grad_scale = gs;
@@ -535,6 +544,7 @@ struct HistArgs {
   const LeafDev* leaves;
   const FeatDev* feats;
   int F, B, FB, NG;        // features, max bins, features per group, groups
+  int L;                   // leaves per tree
   int max_chunks;
   int min_rows_per_chunk;
   long long* acc;          // [F][B][2] (plain) or [F][B] (quantized)
@@ -543,6 +553,9 @@ struct HistArgs {
   long long d_begin, d_count;
   int d_buf, d_shift_g, d_shift_h;
 };
+
+// rows each thread has in flight per pass (all loads issued before the atomics)
+constexpr int kHistUnroll = 8;
 
 template <int TPR>
 __global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
@@ -553,7 +566,7 @@ __global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
   if (a.direct) {
     begin = a.d_begin; count = a.d_count; buf = a.d_buf; sg = a.d_shift_g; sh = a.d_shift_h;
   } else {
-    if (st->done || st->skip) return;
+    if (st->done || st->skip || st->num_leaves >= a.L) return;
     const LeafDev& L = a.leaves[st->smaller];
     begin = L.begin; count = L.count; buf = L.buf; sg = st->hist_shift_g; sh = st->hist_shift_h;
   }
@@ -583,12 +596,12 @@ __global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
   const unsigned* __restrict__ idx = a.idx[buf] + begin;
   const float2* __restrict__ gh = (const float2*)a.gh[buf] + begin;
   const bool active = j * 4 < FBg;
-  for (long long base = r0; base < r1; base += 4 * rpp) {
-    unsigned w[4];
-    long long gi[4], hi[4];
-    bool ok[4];
+  for (long long base = r0; base < r1; base += kHistUnroll * rpp) {
+    unsigned w[kHistUnroll];
+    long long gi[kHistUnroll], hi[kHistUnroll];
+    bool ok[kHistUnroll];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kHistUnroll; ++u) {
       long long p = base + u * rpp + sub;
       ok[u] = active && p < r1;
       if (ok[u]) {
@@ -600,7 +613,7 @@ __global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
       }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kHistUnroll; ++u) {
       if (!ok[u]) continue;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -635,7 +648,7 @@ __global__ void __launch_bounds__(256) k_hist_int(HistArgs a) {
   if (a.direct) {
     begin = a.d_begin; count = a.d_count; buf = a.d_buf;
   } else {
-    if (st->done || st->skip) return;
+    if (st->done || st->skip || st->num_leaves >= a.L) return;
     const LeafDev& L = a.leaves[st->smaller];
     begin = L.begin; count = L.count; buf = L.buf;
   }
@@ -664,12 +677,12 @@ __global__ void __launch_bounds__(256) k_hist_int(HistArgs a) {
   const unsigned* __restrict__ idx = a.idx[buf] + begin;
   const char2* __restrict__ q = (const char2*)a.gh[buf] + begin;
   const bool active = j * 4 < FBg;
-  for (long long base = r0; base < r1; base += 4 * rpp) {
-    unsigned w[4];
-    int gq[4], hq[4];
-    bool ok[4];
+  for (long long base = r0; base < r1; base += kHistUnroll * rpp) {
+    unsigned w[kHistUnroll];
+    int gq[kHistUnroll], hq[kHistUnroll];
+    bool ok[kHistUnroll];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kHistUnroll; ++u) {
       long long p = base + u * rpp + sub;
       ok[u] = active && p < r1;
       if (ok[u]) {
@@ -681,7 +694,7 @@ __global__ void __launch_bounds__(256) k_hist_int(HistArgs a) {
       }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kHistUnroll; ++u) {
       if (!ok[u]) continue;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -759,23 +772,51 @@ __device__ void scan_f64(const double* hg, const double* hh, const FeatDev& fd, 
   wave_sync();
   if (lane < 3) {
     double acc;
+    const double* __restrict__ src = S.val[lane];
+    double* __restrict__ dst = S.pre[lane];
+    const int* __restrict__ skp = S.skipped;
     if (reverse) {
       acc = lane == 1 ? KEPS : 0.0;
     } else if (na && off == 1) {
       // split.rs:795-806: everything but the NaN bin starts on the left
       acc = lane == 0 ? ls.sg : lane == 1 ? ls.sh - KEPS : (double)ls.nd;
-      for (int i = 0; i < fd.num_bin - off; ++i) {
-        double v = lane == 2 ? (double)round_int(hh[i + off] * cf) : (lane == 0 ? hg[i + off] : hh[i + off]);
-        acc -= v;
+      const double* arr = lane == 0 ? hg : hh;
+      const int n = fd.num_bin - off;
+      int i = 0;
+      for (; i + 8 <= n; i += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = lane == 2 ? (double)round_int(hh[i + u + off] * cf) : arr[i + u + off];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc -= v[u];
       }
+      for (; i < n; ++i) acc -= lane == 2 ? (double)round_int(hh[i + off] * cf) : arr[i + off];
     } else {
       acc = lane == 1 ? KEPS : 0.0;
     }
-    const double* src = S.val[lane];
-    double* dst = S.pre[lane];
-    for (int k = 0; k < K; ++k) {
-      int t = reverse ? t_hi - k : t_lo + k;
-      if (!S.skipped[k] && t >= 0) acc += src[k];
+    // the t >= 0 guard only matters at the forward scan's NaN position (t = -1, k = 0)
+    const int k_first_real = (!reverse && t_lo < 0) ? 1 : 0;
+    int k = 0;
+    for (; k < k_first_real; ++k) dst[k] = acc;
+    for (; k + 8 <= K; k += 8) {
+      double v[8];
+      int sk[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        v[u] = src[k + u];
+        sk[u] = skp[k + u];
+      }
+      double o[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        acc = sk[u] ? acc : acc + v[u];
+        o[u] = acc;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) dst[k + u] = o[u];
+    }
+    for (; k < K; ++k) {
+      acc = skp[k] ? acc : acc + src[k];
       dst[k] = acc;
     }
   }
@@ -1012,19 +1053,40 @@ __device__ void scan_int(const long long* hv, const FeatDev& fd, const SplitPara
   K = t_hi - t_lo + 1;
   if (K <= 0) return;
   const double cf = (double)ls.nd / (double)(unsigned)ls.isum;
-  if (lane == 0) {
-    unsigned long long acc = 0;
-    if (!reverse && na && off == 1) {
-      acc = (unsigned long long)ls.isum;
-      for (int i = 0; i < fd.num_bin - off; ++i) acc -= (unsigned long long)hv[i + off];
-    }
-    for (int k = 0; k < K; ++k) {
-      int t = reverse ? t_hi - k : t_lo + k;
-      bool skip = skip_default && (t + off) == fd.default_bin;
+  // packed sums wrap exactly, so the prefix is a wave scan: lane l owns
+  // positions 4l..4l+3
+  unsigned long long base = 0;
+  if (!reverse && na && off == 1) {
+    unsigned long long part = 0;
+    for (int i = lane; i < fd.num_bin - off; i += 64) part += (unsigned long long)hv[i + off];
+    for (int o = 32; o > 0; o >>= 1) part += (unsigned long long)__shfl_xor((long long)part, o, 64);
+    base = (unsigned long long)ls.isum - part;
+  }
+  unsigned long long v4[4], run = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = lane * 4 + j;
+    unsigned long long v = 0;
+    int skip = 1;
+    if (k < K) {
+      const int t = reverse ? t_hi - k : t_lo + k;
+      skip = skip_default && (t + off) == fd.default_bin;
+      if (!skip && t >= 0) v = (unsigned long long)hv[t + off];
       S.skipped[k] = skip;
-      if (!skip && t >= 0) acc += (unsigned long long)hv[t + off];
-      pre[k] = (long long)acc;
     }
+    run += v;
+    v4[j] = run;
+  }
+  unsigned long long incl = run;
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned long long y = (unsigned long long)__shfl_up((long long)incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  const unsigned long long excl = incl - run + base;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = lane * 4 + j;
+    if (k < K) pre[k] = (long long)(excl + v4[j]);
   }
   wave_sync();
   int kb = INT32_MAX;
@@ -1161,7 +1223,7 @@ __device__ LeafSums leaf_sums(const SplitArgs& a, int leaf, bool root) {
 // next leaf to split (learner/mod.rs:954-1101, 698-706).
 __global__ void __launch_bounds__(128) k_split(SplitArgs a) {
   StepDev* st = a.st;
-  if (st->done) return;
+  if (st->done || st->num_leaves >= a.L) return;
   __shared__ double hsm[2][kMaxBins];   // smaller: g, h
   __shared__ double hlg[2][kMaxBins];   // larger: g, h
   __shared__ long long ism[kMaxBins], ilg[kMaxBins];
@@ -1192,8 +1254,16 @@ __global__ void __launch_bounds__(128) k_split(SplitArgs a) {
       // histogram.rs:42-56: the most frequent bin from the leaf totals
       if (fd.mfb > 0 && tid < 2) {
         double v = tid == 0 ? ls_s.sg : ls_s.sh - 2.0 * KEPS;
-        for (int i = 0; i < fd.num_bin; ++i)
-          if (i != fd.mfb) v -= hsm[tid][i];
+        const double* __restrict__ arr = hsm[tid];
+        int i = 0;
+        for (; i + 8 <= fd.num_bin; i += 8) {
+          double x[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) x[u] = arr[i + u];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) v = (i + u == fd.mfb) ? v : v - x[u];
+        }
+        for (; i < fd.num_bin; ++i) v = (i == fd.mfb) ? v : v - arr[i];
         hsm[tid][fd.mfb] = v;
       }
       __syncthreads();
@@ -1266,59 +1336,10 @@ __global__ void __launch_bounds__(128) k_split(SplitArgs a) {
       }
     }
   }
-  if (!last_block(&st->counter)) return;
-  // ---- last block: per-leaf best (over features), then the next leaf to split
-  __shared__ double sg_best[128];
-  __shared__ int sf_best[128];
-  if (!st->skip) {
-    for (int w = 0; w < 2; ++w) {
-      const int leaf = w == 0 ? sm : lg;
-      if (leaf < 0) continue;
-      double bgain = -INFINITY;
-      int bf = -1;
-      for (int k = tid; k < F; k += blockDim.x) {
-        const SplitRec& r = a.feat_split[w * F + k];
-        if (better(r.gain, r.feature, bgain, bf)) { bgain = r.gain; bf = k; }
-      }
-      sg_best[tid] = bgain;
-      sf_best[tid] = bf;
-      __syncthreads();
-      if (tid == 0) {
-        double g0 = -INFINITY;
-        int f0 = -1;
-        for (int k = 0; k < (int)blockDim.x; ++k)
-          if (better(sg_best[k], sf_best[k], g0, f0)) { g0 = sg_best[k]; f0 = sf_best[k]; }
-        if (f0 >= 0) a.best[leaf] = a.feat_split[w * F + f0];
-        else reset_rec(a.best[leaf]);
-      }
-      __syncthreads();
-    }
-  }
-  if (tid == 0) {
-    if (root) {
-      // learner/mod.rs:647: the root's output before any split
-      LeafSums r = leaf_sums(a, 0, true);
-      LeafDev& L0 = a.leaves[0];
-      L0.sum_g = r.sg;
-      L0.sum_h = r.sh - 2.0 * KEPS;
-      L0.int_sum = r.isum;
-      L0.output = round0(leaf_output_t(r.sg, r.sh - 2.0 * KEPS, a.p, true, true, false, r.nd, 0.0));
-      st->root_output = L0.output;
-    }
-    // ArgMax over every leaf slot (learner/mod.rs:698-706)
-    int bl = 0;
-    for (int l = 1; l < a.L; ++l)
-      if (better(a.best[l].gain, a.best[l].feature, a.best[bl].gain, a.best[bl].feature)) bl = l;
-    double gain = a.best[bl].gain;
-    if (!(gain > 0.0) || st->num_leaves >= a.L) {
-      st->done = 1;
-    } else {
-      *a.cur = a.best[bl];
-      st->cur_leaf = bl;
-      st->new_leaf = st->num_leaves;
-    }
-    st->is_root = 0;
-    st->counter = 0;
+  if (root && blockIdx.x == 0 && tid == 0) {
+    // learner/mod.rs:647: the root's output before any split
+    LeafSums r = leaf_sums(a, 0, true);
+    st->root_output = round0(leaf_output_t(r.sg, r.sh - 2.0 * KEPS, a.p, true, true, false, r.nd, 0.0));
   }
 }
 
@@ -1332,12 +1353,15 @@ struct PartArgs {
   unsigned* idx[2];
   void* gh[2];
   int gh_bytes;             // 8 (float2) or 2 (char2)
-  int* tile_left;           // per tile left count, then exclusive offsets
+  int* tile_left;           // per-tile left counts
+  const SplitRec* feat_split;  // [2][F] searches of the last step
   SplitRec* cur;
   SplitRec* best;
+  double* best_gain;        // [L] compact copies of best[].gain / .feature
+  int* best_feat;
   SplitRec* log;
   SplitParamsDev p;
-  int L;
+  int F, L;
   const IterDev* it;
   int quant;
 };
@@ -1360,29 +1384,121 @@ __device__ __forceinline__ bool goes_left(int b, const FeatDev& fd, const SplitR
   return miss ? s.default_left != 0 : b <= s.threshold;
 }
 
-__device__ __forceinline__ void seg_of(const PartArgs& a, long long& begin, long long& count, int& buf) {
-  const LeafDev& L = a.leaves[a.st->cur_leaf];
-  begin = L.begin;
-  count = L.count;
-  buf = L.buf;
+// Best split of one searched leaf over its features (learner/mod.rs:1064-1101):
+// the feature index in `feat_split[w]`, or -1.
+__device__ int reduce_features(const PartArgs& a, int w, double* sg, int* sf) {
+  const int tid = threadIdx.x;
+  double bgain = -INFINITY;
+  int bf = -1;
+  for (int k = tid; k < a.F; k += blockDim.x) {
+    const SplitRec& r = a.feat_split[w * a.F + k];
+    if (better(r.gain, r.feature, bgain, bf)) { bgain = r.gain; bf = k; }
+  }
+  sg[tid] = bgain;
+  sf[tid] = bf;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (tid < o && better(sg[tid + o], sf[tid + o], sg[tid], sf[tid])) {
+      sg[tid] = sg[tid + o];
+      sf[tid] = sf[tid + o];
+    }
+    __syncthreads();
+  }
+  const int r = sf[0];
+  __syncthreads();
+  return r;
 }
 
-// Pass 1: left count per tile; the last block scans the tile counts and
-// applies the split to the leaf records (partition.rs:107-116, learner/mod.rs:1105-1225).
+// Pass 1. Every block first finishes the last step (each searched leaf's best
+// split, then ArgMax over all leaves, learner/mod.rs:698-706) - the same
+// result in every block, so no block waits for another - then counts the
+// left rows of its tiles. Block 0 publishes the decision for later kernels.
 __global__ void __launch_bounds__(kPartThreads) k_part_count(PartArgs a) {
   StepDev* st = a.st;
   if (st->done) return;
-  const SplitRec s = *a.cur;
+  __shared__ double sg[kPartThreads];
+  __shared__ int sf[kPartThreads], sl[kPartThreads];
+  __shared__ SplitRec chosen;
+  const int tid = threadIdx.x;
+  const bool root = st->is_root != 0;
+  const bool searched = st->skip == 0;
+  const int lv[2] = {st->smaller, root ? -1 : st->larger};
+  int fw[2] = {-1, -1};
+  if (searched)
+    for (int w = 0; w < 2; ++w)
+      if (lv[w] >= 0) fw[w] = reduce_features(a, w, sg, sf);
+  // ArgMax over leaf slots; the searched leaves use this step's results
+  double bg = -INFINITY;
+  int bf = -1, bl = INT32_MAX;
+  for (int l = tid; l < a.L; l += kPartThreads) {
+    double g;
+    int f;
+    int w = (searched && l == lv[0]) ? 0 : (searched && l == lv[1]) ? 1 : -1;
+    if (w >= 0) {
+      g = fw[w] >= 0 ? a.feat_split[w * a.F + fw[w]].gain : -INFINITY;
+      f = fw[w] >= 0 ? a.feat_split[w * a.F + fw[w]].feature : -1;
+    } else {
+      g = a.best_gain[l];
+      f = a.best_feat[l];
+    }
+    if (bl == INT32_MAX || better(g, f, bg, bf)) { bg = g; bf = f; bl = l; }
+  }
+  sg[tid] = bg;
+  sf[tid] = bf;
+  sl[tid] = bl;
+  __syncthreads();
+  for (int o = kPartThreads / 2; o > 0; o >>= 1) {
+    if (tid < o) {
+      const double g2 = sg[tid + o];
+      const int f2 = sf[tid + o], l2 = sl[tid + o];
+      const bool take = l2 != INT32_MAX &&
+                        (sl[tid] == INT32_MAX || better(g2, f2, sg[tid], sf[tid]) ||
+                         (!better(sg[tid], sf[tid], g2, f2) && l2 < sl[tid]));
+      if (take) { sg[tid] = g2; sf[tid] = f2; sl[tid] = l2; }
+    }
+    __syncthreads();
+  }
+  const int best_leaf = sl[0];
+  const double gain = sg[0];
+  const bool stop = !(gain > 0.0) || st->num_leaves >= a.L;
+  if (tid == 0) {
+    const int w = (searched && best_leaf == lv[0]) ? 0 : (searched && best_leaf == lv[1]) ? 1 : -1;
+    if (!stop) chosen = w >= 0 ? a.feat_split[w * a.F + fw[w]] : a.best[best_leaf];
+  }
+  if (blockIdx.x == 0 && tid == 0) {
+    for (int w = 0; w < 2; ++w) {
+      if (!searched || lv[w] < 0) continue;
+      SplitRec r;
+      if (fw[w] >= 0) r = a.feat_split[w * a.F + fw[w]];
+      else reset_rec(r);
+      a.best[lv[w]] = r;
+      a.best_gain[lv[w]] = r.gain;
+      a.best_feat[lv[w]] = r.feature;
+    }
+    if (stop) {
+      st->done = 1;
+    } else {
+      *a.cur = chosen;
+      st->cur_leaf = best_leaf;
+      st->new_leaf = st->num_leaves;
+      const LeafDev& Lc = a.leaves[best_leaf];
+      st->seg_begin = Lc.begin;
+      st->seg_count = Lc.count;
+      st->seg_buf = Lc.buf;
+    }
+  }
+  if (stop) return;
+  __syncthreads();
+  const SplitRec& s = chosen;
   const FeatDev fd = a.feats[s.feature];
-  long long begin, count;
-  int buf;
-  seg_of(a, begin, count, buf);
-  const unsigned* idx = a.idx[buf] + begin;
+  const LeafDev& Lc = a.leaves[best_leaf];
+  const long long begin = Lc.begin, count = Lc.count;
+  const unsigned* idx = a.idx[Lc.buf] + begin;
   const uint8_t* col = a.bins_cm + (size_t)s.feature * a.N;
   const int ntiles = (int)((count + kPartTile - 1) / kPartTile);
   __shared__ int wsum[kPartThreads / 64];
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    long long p0 = (long long)t * kPartTile + threadIdx.x * kPartRowsPerThread;
+    long long p0 = (long long)t * kPartTile + tid * kPartRowsPerThread;
     int c = 0;
 #pragma unroll
     for (int k = 0; k < kPartRowsPerThread; ++k) {
@@ -1390,47 +1506,112 @@ __global__ void __launch_bounds__(kPartThreads) k_part_count(PartArgs a) {
       if (p < count) c += goes_left(col[idx[p]], fd, s) ? 1 : 0;
     }
     c = wave_sum(c);
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    if ((tid & 63) == 0) wsum[tid >> 6] = c;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       int tot = 0;
       for (int w = 0; w < kPartThreads / 64; ++w) tot += wsum[w];
       a.tile_left[t] = tot;
     }
     __syncthreads();
   }
-  if (!last_block(&st->counter2)) return;
-  // exclusive scan of tile counts (in place), block-wide
-  __shared__ long long carry;
-  __shared__ long long part[kPartThreads];
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
+}
+
+// Pass 2: every block scans the tile counts itself (no grid handshake),
+// then scatters its tiles' (row, gradient) pairs stably to the other buffer
+// (partition.rs:107-174). Block 0 applies the split to the leaf records and
+// prepares the next step (learner/mod.rs:1105-1225, 712-731).
+constexpr int kMaxOwnTiles = 64;
+
+__global__ void __launch_bounds__(kPartThreads) k_part_scatter(PartArgs a) {
+  StepDev* st = a.st;
+  if (st->done) return;
+  const SplitRec s = *a.cur;
+  const FeatDev fd = a.feats[s.feature];
+  const long long begin = st->seg_begin, count = st->seg_count;
+  const int src = st->seg_buf, dst = 1 - src;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int ntiles = (int)((count + kPartTile - 1) / kPartTile);
+  // exclusive prefix of the tile counts at this block's tiles, and the total
+  __shared__ long long own_off[kMaxOwnTiles];
+  __shared__ long long wtot[kPartThreads / 64];
+  __shared__ long long carry_s;
+  long long carry = 0;
   for (int base = 0; base < ntiles; base += kPartThreads) {
-    int t = base + threadIdx.x;
-    long long v = t < ntiles ? a.tile_left[t] : 0;
-    part[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < kPartThreads; o <<= 1) {
-      long long x = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
-      __syncthreads();
-      part[threadIdx.x] += x;
-      __syncthreads();
+    const int t = base + tid;
+    const long long v = t < ntiles ? a.tile_left[t] : 0;
+    long long x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      long long y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
     }
-    long long excl = part[threadIdx.x] - v + carry;
-    if (t < ntiles) a.tile_left[t] = (int)excl;
+    if (lane == 63) wtot[tid >> 6] = x;
     __syncthreads();
-    if (threadIdx.x == kPartThreads - 1) carry = excl + v;
+    long long wb = 0;
+    for (int w = 0; w < (tid >> 6); ++w) wb += wtot[w];
+    const long long excl = carry + wb + x - v;
+    if (t < ntiles && (t % (int)gridDim.x) == (int)blockIdx.x) {
+      const int k = t / (int)gridDim.x;
+      if (k < kMaxOwnTiles) own_off[k] = excl;
+    }
+    if (tid == kPartThreads - 1) carry_s = excl + v;
+    __syncthreads();
+    carry = carry_s;
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    const long long lc = carry, rc = count - carry;
+  const long long lc = carry, rc = count - carry;
+  const unsigned* idx = a.idx[src] + begin;
+  unsigned* oidx = a.idx[dst] + begin;
+  const uint8_t* col = a.bins_cm + (size_t)s.feature * a.N;
+  __shared__ int wpre[kPartThreads / 64];
+  for (int t = blockIdx.x, k = 0; t < ntiles; t += gridDim.x, ++k) {
+    const long long t0 = (long long)t * kPartTile;
+    const long long p0 = t0 + tid * kPartRowsPerThread;
+    unsigned rows[kPartRowsPerThread];
+    bool left[kPartRowsPerThread];
+    int c = 0;
+#pragma unroll
+    for (int u = 0; u < kPartRowsPerThread; ++u) {
+      long long p = p0 + u;
+      left[u] = false;
+      if (p < count) {
+        rows[u] = idx[p];
+        left[u] = goes_left(col[rows[u]], fd, s);
+        c += left[u] ? 1 : 0;
+      }
+    }
+    int x = c;
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wpre[tid >> 6] = x;
+    __syncthreads();
+    int wbase = 0;
+    for (int w = 0; w < (tid >> 6); ++w) wbase += wpre[w];
+    const int excl_left = wbase + x - c;
+    __syncthreads();
+    const long long tile_off = own_off[k];
+    long long lpos = tile_off + excl_left;
+    long long rpos = lc + (t0 - tile_off) + ((long long)tid * kPartRowsPerThread - excl_left);
+#pragma unroll
+    for (int u = 0; u < kPartRowsPerThread; ++u) {
+      long long p = p0 + u;
+      if (p >= count) break;
+      long long d = left[u] ? lpos++ : rpos++;
+      oidx[d] = rows[u];
+      if (a.gh_bytes == 8) ((float2*)a.gh[dst] + begin)[d] = ((const float2*)a.gh[src] + begin)[p];
+      else ((char2*)a.gh[dst] + begin)[d] = ((const char2*)a.gh[src] + begin)[p];
+    }
+  }
+  if (blockIdx.x == 0 && tid == 0) {
     const int cur = st->cur_leaf, nl = st->new_leaf;
     LeafDev& Lc = a.leaves[cur];
     LeafDev& Ln = a.leaves[nl];
     const int depth = Lc.depth + 1;
     Ln.begin = begin + lc;
     Ln.count = rc;
-    Ln.buf = 1 - buf;
+    Ln.buf = dst;
     Ln.depth = depth;
     Ln.sum_g = s.rsg;
     Ln.sum_h = s.rsh;
@@ -1438,7 +1619,7 @@ __global__ void __launch_bounds__(kPartThreads) k_part_count(PartArgs a) {
     Ln.output = isnan(s.right_output) ? 0.0 : s.right_output;
     Lc.begin = begin;
     Lc.count = lc;
-    Lc.buf = 1 - buf;
+    Lc.buf = dst;
     Lc.depth = depth;
     Lc.sum_g = s.lsg;
     Lc.sum_h = s.lsh;
@@ -1452,84 +1633,25 @@ __global__ void __launch_bounds__(kPartThreads) k_part_count(PartArgs a) {
     st->num_splits += 1;
     st->num_leaves += 1;
     st->left_total = lc;
+    st->is_root = 0;
     if (lc < rc) { st->smaller = cur; st->larger = nl; }
     else { st->smaller = nl; st->larger = cur; }
     // learner/mod.rs:712-731 before_find_best_split
-    bool skip = (a.p.max_depth > 0 && depth >= a.p.max_depth) ||
-                (rc < 2LL * a.p.min_data_in_leaf && lc < 2LL * a.p.min_data_in_leaf);
+    const bool skip = (a.p.max_depth > 0 && depth >= a.p.max_depth) ||
+                      (rc < 2LL * a.p.min_data_in_leaf && lc < 2LL * a.p.min_data_in_leaf);
     st->skip = skip ? 1 : 0;
     if (skip) {
       a.best[cur].gain = -INFINITY;
       a.best[nl].gain = -INFINITY;
+      a.best_gain[cur] = -INFINITY;
+      a.best_gain[nl] = -INFINITY;
+      a.best_feat[nl] = a.best[nl].feature;
+      a.best_feat[cur] = a.best[cur].feature;
     }
-    // fixed-point exponents of the next histogram build (the smaller child)
     if (!a.quant) {
       const long long sc = lc < rc ? lc : rc;
       st->hist_shift_g = fix_shift(sc, a.it->max_abs_g_bits);
       st->hist_shift_h = fix_shift(sc, a.it->max_abs_h_bits);
-    }
-    st->counter2 = 0;
-  }
-}
-
-// Pass 2: stable scatter of (row, gradient) pairs to the other buffer.
-__global__ void __launch_bounds__(kPartThreads) k_part_scatter(PartArgs a) {
-  StepDev* st = a.st;
-  if (st->done) return;
-  const SplitRec s = *a.cur;
-  const FeatDev fd = a.feats[s.feature];
-  // the split leaf's records were already updated by pass 1: rebuild the parent segment
-  const int cur = st->cur_leaf;
-  const LeafDev& Lc = a.leaves[cur];
-  const long long begin = Lc.begin;
-  const long long lc = st->left_total;
-  const long long count = lc + a.leaves[st->new_leaf].count;
-  const int src = 1 - Lc.buf, dst = Lc.buf;
-  const unsigned* idx = a.idx[src] + begin;
-  unsigned* oidx = a.idx[dst] + begin;
-  const uint8_t* col = a.bins_cm + (size_t)s.feature * a.N;
-  const int ntiles = (int)((count + kPartTile - 1) / kPartTile);
-  __shared__ int wpre[kPartThreads / 64];
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const long long t0 = (long long)t * kPartTile;
-    const long long p0 = t0 + threadIdx.x * kPartRowsPerThread;
-    unsigned rows[kPartRowsPerThread];
-    bool left[kPartRowsPerThread];
-    int c = 0;
-#pragma unroll
-    for (int k = 0; k < kPartRowsPerThread; ++k) {
-      long long p = p0 + k;
-      left[k] = false;
-      if (p < count) {
-        rows[k] = idx[p];
-        left[k] = goes_left(col[rows[k]], fd, s);
-        c += left[k] ? 1 : 0;
-      }
-    }
-    // exclusive scan of c over the block (thread order = row order)
-    int x = c;
-    const int lane = threadIdx.x & 63;
-    for (int o = 1; o < 64; o <<= 1) {
-      int y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) wpre[threadIdx.x >> 6] = x;
-    __syncthreads();
-    int wbase = 0;
-    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) wbase += wpre[w];
-    const int excl_left = wbase + x - c;
-    __syncthreads();
-    const long long tile_off = a.tile_left[t];
-    long long lpos = tile_off + excl_left;
-    long long rpos = lc + (t0 - tile_off) + ((long long)threadIdx.x * kPartRowsPerThread - excl_left);
-#pragma unroll
-    for (int k = 0; k < kPartRowsPerThread; ++k) {
-      long long p = p0 + k;
-      if (p >= count) break;
-      long long d = left[k] ? lpos++ : rpos++;
-      oidx[d] = rows[k];
-      if (a.gh_bytes == 8) ((float2*)a.gh[dst] + begin)[d] = ((const float2*)a.gh[src] + begin)[p];
-      else ((char2*)a.gh[dst] + begin)[d] = ((const char2*)a.gh[src] + begin)[p];
     }
   }
 }
