@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_lib", "liblgbm_hip.so")
+LIB_PATH = os.environ.get("LGBMHIP_LIB") or os.path.join(_HERE, "_lib", "liblgbm_hip.so")
 
 OK = 0
 STATUS_NAMES = {1: "InvalidParameter", 2: "InvalidData", 3: "ModelFormat", 4: "Unsupported", 5: "Internal"}
@@ -97,6 +97,8 @@ SIGNATURES = {
     "lgbmhip_predict_binned": (I32, [P, P, I64, P]),
     "lgbmhip_last_timings": (I32, [P, P]),
     "lgbmhip_profile": (I32, [P, I32, P]),
+    "lgbmhip_set_tuning": (I32, [P, I32, I32, I32, I32]),
+    "lgbmhip_get_tuning": (I32, [P, P]),
 }
 
 _lib = None

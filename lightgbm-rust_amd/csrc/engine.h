@@ -99,7 +99,8 @@ struct StepDev {
   int counter2;
   double root_output;  // the root's output before any split (learner/mod.rs:647)
   long long seg_begin, seg_count;  // the segment of the leaf being split
-  int seg_buf;
+  int seg_buf, seg_depth;
+  unsigned long long lcur, rcur;   // rows placed left / right by the partition pass
 };
 
 // Per-iteration scalars written by device reductions.

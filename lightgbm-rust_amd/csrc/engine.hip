@@ -222,9 +222,10 @@ struct Ctx {
   DevBuf<int8_t> gq, hq;
   std::unique_ptr<Mt> start_eng;
   int n_calls = 0;
-  int hist_tpr = 8, hist_fb = 0, hist_ng = 1, hist_chunks = 512, hist_min_rows = 1024;
+  int hist_tpr = 8, hist_fb = 0, hist_ng = 1, hist_chunks = 256, hist_min_rows = 256;
   size_t hist_lds = 0;
   int part_grid = 1024;
+  int tune_part_grid = 0, tune_hist_chunks = 0, tune_hist_min_rows = 0, tune_hist_lds = 0;
   // pinned copies of the per-tree results
   std::vector<SplitRec*> h_log;
   std::vector<LeafDev*> h_leaves;
@@ -334,7 +335,11 @@ static void upload_bins(Ctx* c, const lgbmhip_feature_meta* metas, int F, const 
 // fits the budget; threads per row = 4-byte chunks of the group, to a power of 2.
 static void plan_hist(Ctx* c, bool quant) {
   const size_t per_feat = (size_t)c->B * 2 * (quant ? 4 : 8);
+  // 32 KiB by default: five blocks share a CU's LDS and each block zeroes and
+  // flushes a small histogram, which wins over wide feature groups at every
+  // leaf size measured on 10.5M x 28 (quantized and FP64 forms)
   size_t budget = std::min<size_t>(c->lds_max, 160 * 1024);
+  budget = std::min<size_t>(budget, c->tune_hist_lds > 0 ? (size_t)c->tune_hist_lds : (size_t)32 * 1024);
   // at most 128 bytes (32 threads of 4 bins) of a row per block
   int fb = (int)std::min<size_t>(std::min<size_t>((size_t)c->F, 128), budget / per_feat);
   fb = std::max(4, fb / 4 * 4);
@@ -464,12 +469,11 @@ static void setup_training(Ctx* c) {
   c->leafval.alloc((size_t)K * L);
   c->renew.alloc((size_t)2 * L);
   c->renew.zero(c->stream);
-  {
-    // each partition block owns at most kMaxOwnTiles tiles of the largest leaf
-    const int64_t tiles = (N + part_tile() - 1) / part_tile();
-    const int64_t need = (tiles + kMaxOwnTiles - 1) / kMaxOwnTiles;
-    c->part_grid = (int)std::min<int64_t>(tiles, std::max<int64_t>(2 * c->cus, need));
-  }
+  c->part_grid = (int)std::min<int64_t>((N + part_tile() - 1) / part_tile(),
+                                        c->tune_part_grid > 0 ? c->tune_part_grid : 2 * c->cus);
+  c->hist_chunks = c->tune_hist_chunks > 0 ? c->tune_hist_chunks : c->cus;
+  // small chunks keep small leaves spread over many CUs (one pass per block)
+  c->hist_min_rows = c->tune_hist_min_rows > 0 ? c->tune_hist_min_rows : 256;
   if (quant) {
     // quant.rs:45-79: per-row tables from block generators of `quant_table_threads` blocks
     std::vector<double> grv(N), hrv(N);
@@ -521,7 +525,7 @@ static void setup_training(Ctx* c) {
   c->state_ready = true;
 }
 
-static HistArgs hist_args(Ctx* c, bool quant) {
+static HistArgs hist_args(Ctx* c, bool quant, int k, int mode) {
   HistArgs a{};
   a.bins = c->bins_rm.p;
   a.S = c->S;
@@ -536,12 +540,20 @@ static HistArgs hist_args(Ctx* c, bool quant) {
   a.B = c->B;
   a.FB = c->hist_fb;
   a.NG = c->hist_ng;
-  a.max_chunks = c->hist_chunks;
   a.L = c->L;
+  a.N = c->N;
+  a.max_chunks = c->hist_chunks;
   a.min_rows_per_chunk = c->hist_min_rows;
   a.acc = c->acc.p;
-  a.direct = 0;
-  (void)quant;
+  a.it = c->it.p + k;
+  a.cur = c->cur.p;
+  a.log = c->log.p;
+  a.best = c->best.p;
+  a.best_gain = c->best_gain.p;
+  a.best_feat = c->best_feat.p;
+  a.p = c->sp;
+  a.quant = quant ? 1 : 0;
+  a.mode = mode;
   return a;
 }
 
@@ -622,12 +634,13 @@ static void enqueue_tree(Ctx* c, int k, bool score_update, double lr) {
                        (float2*)c->gh[0].p, c->it.p + k, c->st.p, c->leaves.p, c->best.p, c->spl.p, c->F, c->L,
                        c->best_gain.p, c->best_feat.p);
   }
-  HistArgs ha = hist_args(c, quant);
+  HistArgs ha_root = hist_args(c, quant, k, 2);
+  HistArgs ha = hist_args(c, quant, k, 0);
   SplitArgs sa = split_args(c, k);
   PartArgs pa = part_args(c, k);
   const int hgrid = c->hist_chunks * c->hist_ng;
   prof_mark(c, k, 0, 0, 0);
-  launch_hist(ha, quant, c->hist_tpr, hgrid, c->hist_lds, s);
+  launch_hist(ha_root, quant, c->hist_tpr, hgrid, c->hist_lds, s);
   prof_mark(c, k, 0, 0, 1);
   launch_split(sa, s);
   prof_mark(c, k, 0, 1, 1);
@@ -1332,8 +1345,7 @@ static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc
     HIP_CHECK(hipMemcpyAsync(c->gh[0].p, pk.data(), pk.size(), hipMemcpyHostToDevice, c->stream));
   }
   c->acc.zero(c->stream);
-  HistArgs a = hist_args(c, quant);
-  a.direct = 1;
+  HistArgs a = hist_args(c, quant, 0, 1);
   a.d_begin = 0;
   a.d_count = m;
   a.d_buf = 0;
@@ -1471,6 +1483,35 @@ lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out8) {
         for (auto& e : c->pev) HIP_CHECK(hipEventCreate(&e));
       }
     }
+  });
+}
+
+lgbmhip_status lgbmhip_set_tuning(lgbmhip_ctx* ctx, int32_t part_grid, int32_t hist_chunks, int32_t hist_min_rows,
+                                  int32_t hist_lds_budget) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (part_grid < 0 || hist_chunks < 0 || hist_min_rows < 0 || hist_lds_budget < 0)
+      throw Error(LGBMHIP_INVALID_PARAMETER, "tuning values must be >= 0");
+    if (hist_lds_budget > 0 && hist_lds_budget < 4096) throw Error(LGBMHIP_INVALID_PARAMETER, "LDS budget below 4 KiB");
+    if (part_grid) c->tune_part_grid = part_grid;
+    if (hist_chunks) c->tune_hist_chunks = hist_chunks;
+    if (hist_min_rows) c->tune_hist_min_rows = hist_min_rows;
+    if (hist_lds_budget) c->tune_hist_lds = hist_lds_budget;
+    c->state_ready = false;   // shapes are baked into the captured graphs
+  });
+}
+
+lgbmhip_status lgbmhip_get_tuning(lgbmhip_ctx* ctx, int32_t* out7) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (!c->state_ready) setup_training(c);
+    out7[0] = c->part_grid;
+    out7[1] = c->hist_chunks;
+    out7[2] = c->hist_min_rows;
+    out7[3] = (int32_t)c->hist_lds;
+    out7[4] = c->hist_fb;
+    out7[5] = c->hist_ng;
+    out7[6] = c->hist_tpr;
   });
 }
 

@@ -36,7 +36,7 @@ __device__ __forceinline__ double sgn(double x) { return (double)((x > 0.0) - (x
 __device__ __forceinline__ double thr_l1(double s, double l1) { return sgn(s) * fmax(fabs(s) - l1, 0.0); }
 
 // split.rs:137-161
-__device__ double leaf_output_t(double g, double h, const SplitParamsDev& p, bool use_l1, bool use_max,
+__device__ __forceinline__ double leaf_output_t(double g, double h, const SplitParamsDev& p, bool use_l1, bool use_max,
                                 bool use_smooth, int nd, double po) {
   double ret = use_l1 ? -thr_l1(g, p.lambda_l1) / (h + p.lambda_l2) : -g / (h + p.lambda_l2);
   if (use_max && p.max_delta_step > 0.0 && fabs(ret) > p.max_delta_step) ret = sgn(ret) * p.max_delta_step;
@@ -50,7 +50,7 @@ __device__ __forceinline__ double leaf_output(double g, double h, const SplitPar
   return leaf_output_t(g, h, p, p.lambda_l1 > 0.0, p.max_delta_step > 0.0, p.path_smooth > KEPS, nd, po);
 }
 // split.rs:183-201
-__device__ double leaf_gain(double g, double h, const SplitParamsDev& p, int nd, double po) {
+__device__ __forceinline__ double leaf_gain(double g, double h, const SplitParamsDev& p, int nd, double po) {
   bool l1 = p.lambda_l1 > 0.0;
   if (!(p.max_delta_step > 0.0) && !(p.path_smooth > KEPS)) {
     if (l1) {
@@ -442,6 +442,8 @@ __global__ void k_tree_init(long long n, const float* __restrict__ g, const floa
       st->num_splits = 0;
       st->counter = 0;
       st->counter2 = 0;
+      st->lcur = 0;
+      st->rcur = 0;
       st->hist_shift_g = sg;
       st->hist_shift_h = sh;
       LeafDev& r = leaves[0];
@@ -519,6 +521,8 @@ __global__ void k_discretize(long long n, const float* __restrict__ g, const flo
       st->num_splits = 0;
       st->counter = 0;
       st->counter2 = 0;
+      st->lcur = 0;
+      st->rcur = 0;
       LeafDev& r = leaves[0];
       r.begin = 0;
       r.count = n;
@@ -540,19 +544,106 @@ struct HistArgs {
   int S;                   // row stride in bytes
   const unsigned* idx[2];
   const void* gh[2];       // float2 (plain) or char2 (quantized)
-  const StepDev* st;
-  const LeafDev* leaves;
+  StepDev* st;
+  LeafDev* leaves;
   const FeatDev* feats;
   int F, B, FB, NG;        // features, max bins, features per group, groups
   int L;                   // leaves per tree
+  long long N;
   int max_chunks;
   int min_rows_per_chunk;
   long long* acc;          // [F][B][2] (plain) or [F][B] (quantized)
-  // direct mode (stage entry point): explicit segment
-  int direct;
+  const IterDev* it;
+  SplitRec* cur;           // the split the partition pass just applied
+  SplitRec* log;
+  SplitRec* best;
+  double* best_gain;
+  int* best_feat;
+  SplitParamsDev p;
+  int quant;
+  int mode;                // 0: after a partition pass, 1: explicit segment, 2: the root
   long long d_begin, d_count;
   int d_buf, d_shift_g, d_shift_h;
 };
+
+// The rows a histogram launch sums. After a partition pass (mode 0) this also
+// applies the split to the leaf records (block 0; learner/mod.rs:1105-1225)
+// and decides before_find_best_split (learner/mod.rs:712-731): every block
+// derives the same answer from the step record, which no block rewrites.
+__device__ __forceinline__ bool hist_segment(const HistArgs& a, long long& begin, long long& count, int& buf, int& sg, int& sh) {
+  StepDev* st = a.st;
+  if (a.mode == 1) {
+    begin = a.d_begin; count = a.d_count; buf = a.d_buf; sg = a.d_shift_g; sh = a.d_shift_h;
+    return true;
+  }
+  if (st->done) return false;
+  if (a.mode == 2) {
+    begin = 0; count = a.N; buf = 0; sg = st->hist_shift_g; sh = st->hist_shift_h;
+    return true;
+  }
+  const long long pb = st->seg_begin, pc = st->seg_count;
+  const long long lc = (long long)st->lcur, rc = pc - lc;
+  const int nb = 1 - st->seg_buf, depth = st->seg_depth + 1;
+  const int cur = st->cur_leaf, nl = st->new_leaf;
+  const bool skip = (a.p.max_depth > 0 && depth >= a.p.max_depth) ||
+                    (rc < 2LL * a.p.min_data_in_leaf && lc < 2LL * a.p.min_data_in_leaf);
+  const bool complete = nl + 1 >= a.L;
+  const long long sc = lc < rc ? lc : rc;
+  const int shg = a.quant ? 0 : fix_shift(sc, a.it->max_abs_g_bits);
+  const int shh = a.quant ? 0 : fix_shift(sc, a.it->max_abs_h_bits);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const SplitRec& s = *a.cur;
+    LeafDev& Lc = a.leaves[cur];
+    LeafDev& Ln = a.leaves[nl];
+    Ln.begin = pb + lc;
+    Ln.count = rc;
+    Ln.buf = nb;
+    Ln.depth = depth;
+    Ln.sum_g = s.rsg;
+    Ln.sum_h = s.rsh;
+    Ln.int_sum = s.rgh;
+    Ln.output = isnan(s.right_output) ? 0.0 : s.right_output;
+    Lc.begin = pb;
+    Lc.count = lc;
+    Lc.buf = nb;
+    Lc.depth = depth;
+    Lc.sum_g = s.lsg;
+    Lc.sum_h = s.lsh;
+    Lc.int_sum = s.lgh;
+    Lc.output = isnan(s.left_output) ? 0.0 : s.left_output;
+    SplitRec* lr = a.log + st->num_splits;
+    const unsigned* sw = (const unsigned*)a.cur;
+    unsigned* dw = (unsigned*)lr;
+    for (int w = 0; w < (int)(sizeof(SplitRec) / 4); ++w) dw[w] = sw[w];
+    lr->left_count = (int)lc;
+    lr->right_count = (int)rc;
+    lr->leaf = cur;
+    st->num_splits += 1;
+    st->num_leaves = nl + 1;
+    st->left_total = lc;
+    st->is_root = 0;
+    if (lc < rc) { st->smaller = cur; st->larger = nl; }
+    else { st->smaller = nl; st->larger = cur; }
+    st->skip = skip ? 1 : 0;
+    st->hist_shift_g = shg;
+    st->hist_shift_h = shh;
+    if (skip) {
+      a.best[cur].gain = -INFINITY;
+      a.best[nl].gain = -INFINITY;
+      a.best_gain[cur] = -INFINITY;
+      a.best_gain[nl] = -INFINITY;
+      a.best_feat[cur] = a.best[cur].feature;
+      a.best_feat[nl] = a.best[nl].feature;
+    }
+  }
+  if (skip || complete) return false;
+  begin = lc < rc ? pb : pb + lc;
+  count = sc;
+  buf = nb;
+  sg = shg;
+  sh = shh;
+  return true;
+}
 
 // rows each thread has in flight per pass (all loads issued before the atomics)
 constexpr int kHistUnroll = 8;
@@ -560,16 +651,9 @@ constexpr int kHistUnroll = 8;
 template <int TPR>
 __global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
   extern __shared__ unsigned long long lds[];
-  const StepDev* st = a.st;
   long long begin, count;
   int buf, sg, sh;
-  if (a.direct) {
-    begin = a.d_begin; count = a.d_count; buf = a.d_buf; sg = a.d_shift_g; sh = a.d_shift_h;
-  } else {
-    if (st->done || st->skip || st->num_leaves >= a.L) return;
-    const LeafDev& L = a.leaves[st->smaller];
-    begin = L.begin; count = L.count; buf = L.buf; sg = st->hist_shift_g; sh = st->hist_shift_h;
-  }
+  if (!hist_segment(a, begin, count, buf, sg, sh)) return;
   const int fg = blockIdx.x % a.NG;
   const int chunk = blockIdx.x / a.NG;
   long long want = (count + a.min_rows_per_chunk - 1) / a.min_rows_per_chunk;
@@ -593,9 +677,24 @@ __global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
   }
   __syncthreads();
   const int rpp = 256 / TPR;
-  const unsigned* __restrict__ idx = a.idx[buf] + begin;
-  const float2* __restrict__ gh = (const float2*)a.gh[buf] + begin;
+  // select, not index: a runtime index into a kernel-argument array makes
+  // the compiler copy the arguments to scratch memory
+  const unsigned* __restrict__ idx = (buf ? a.idx[1] : a.idx[0]) + begin;
+  const float2* __restrict__ gh = (const float2*)(buf ? a.gh[1] : a.gh[0]) + begin;
   const bool active = j * 4 < FBg;
+  // software pipeline: the row indices and gradients of the next pass are
+  // in flight while this pass gathers its bins and adds into LDS, so each
+  // pass waits on one memory round trip instead of two
+  unsigned rn[kHistUnroll];
+  float2 vn[kHistUnroll];
+#pragma unroll
+  for (int u = 0; u < kHistUnroll; ++u) {
+    long long p = r0 + u * rpp + sub;
+    if (active && p < r1) {
+      rn[u] = idx[p];
+      vn[u] = gh[p];
+    }
+  }
   for (long long base = r0; base < r1; base += kHistUnroll * rpp) {
     unsigned w[kHistUnroll];
     long long gi[kHistUnroll], hi[kHistUnroll];
@@ -605,11 +704,18 @@ __global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
       long long p = base + u * rpp + sub;
       ok[u] = active && p < r1;
       if (ok[u]) {
-        unsigned r = idx[p];
-        float2 v = gh[p];
-        w[u] = *(const unsigned*)(a.bins + (size_t)r * a.S + f0 + j * 4);
-        gi[u] = to_fixed(v.x, sg);
-        hi[u] = to_fixed(v.y, sh);
+        w[u] = *(const unsigned*)(a.bins + (size_t)rn[u] * a.S + f0 + j * 4);
+        gi[u] = to_fixed(vn[u].x, sg);
+        hi[u] = to_fixed(vn[u].y, sh);
+      }
+    }
+    const long long nbase = base + kHistUnroll * rpp;
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      long long p = nbase + u * rpp + sub;
+      if (active && p < r1) {
+        rn[u] = idx[p];
+        vn[u] = gh[p];
       }
     }
 #pragma unroll
@@ -642,16 +748,9 @@ __global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
 template <int TPR>
 __global__ void __launch_bounds__(256) k_hist_int(HistArgs a) {
   extern __shared__ unsigned lds32[];
-  const StepDev* st = a.st;
   long long begin, count;
-  int buf;
-  if (a.direct) {
-    begin = a.d_begin; count = a.d_count; buf = a.d_buf;
-  } else {
-    if (st->done || st->skip || st->num_leaves >= a.L) return;
-    const LeafDev& L = a.leaves[st->smaller];
-    begin = L.begin; count = L.count; buf = L.buf;
-  }
+  int buf, sg_unused, sh_unused;
+  if (!hist_segment(a, begin, count, buf, sg_unused, sh_unused)) return;
   const int fg = blockIdx.x % a.NG;
   const int chunk = blockIdx.x / a.NG;
   long long want = (count + a.min_rows_per_chunk - 1) / a.min_rows_per_chunk;
@@ -674,9 +773,19 @@ __global__ void __launch_bounds__(256) k_hist_int(HistArgs a) {
   }
   __syncthreads();
   const int rpp = 256 / TPR;
-  const unsigned* __restrict__ idx = a.idx[buf] + begin;
-  const char2* __restrict__ q = (const char2*)a.gh[buf] + begin;
+  const unsigned* __restrict__ idx = (buf ? a.idx[1] : a.idx[0]) + begin;
+  const char2* __restrict__ q = (const char2*)(buf ? a.gh[1] : a.gh[0]) + begin;
   const bool active = j * 4 < FBg;
+  unsigned rn[kHistUnroll];
+  char2 vn[kHistUnroll];
+#pragma unroll
+  for (int u = 0; u < kHistUnroll; ++u) {
+    long long p = r0 + u * rpp + sub;
+    if (active && p < r1) {
+      rn[u] = idx[p];
+      vn[u] = q[p];
+    }
+  }
   for (long long base = r0; base < r1; base += kHistUnroll * rpp) {
     unsigned w[kHistUnroll];
     int gq[kHistUnroll], hq[kHistUnroll];
@@ -686,11 +795,18 @@ __global__ void __launch_bounds__(256) k_hist_int(HistArgs a) {
       long long p = base + u * rpp + sub;
       ok[u] = active && p < r1;
       if (ok[u]) {
-        unsigned r = idx[p];
-        char2 v = q[p];
-        w[u] = *(const unsigned*)(a.bins + (size_t)r * a.S + f0 + j * 4);
-        gq[u] = v.x;
-        hq[u] = (int)(uint8_t)v.y;
+        w[u] = *(const unsigned*)(a.bins + (size_t)rn[u] * a.S + f0 + j * 4);
+        gq[u] = vn[u].x;
+        hq[u] = (int)(uint8_t)vn[u].y;
+      }
+    }
+    const long long nbase = base + kHistUnroll * rpp;
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      long long p = nbase + u * rpp + sub;
+      if (active && p < r1) {
+        rn[u] = idx[p];
+        vn[u] = q[p];
       }
     }
 #pragma unroll
@@ -738,7 +854,7 @@ struct LeafSums {
 // One sequential scan of split.rs:676-891 on a wave: the prefix chains run
 // in scan order on lanes 0-2 (g, h, count), every candidate is then scored
 // in parallel and the first-in-scan-order maximum is kept.
-__device__ void scan_f64(const double* hg, const double* hh, const FeatDev& fd, const SplitParamsDev& p,
+__device__ __forceinline__ void scan_f64(const double* hg, const double* hh, const FeatDev& fd, const SplitParamsDev& p,
                          const LeafSums& ls, double min_gain_shift, bool reverse, bool skip_default, bool na,
                          ScanLds& S, SplitRec& out, bool& splittable) {
   const int lane = threadIdx.x & 63;
@@ -886,7 +1002,7 @@ __device__ void scan_f64(const double* hg, const double* hh, const FeatDev& fd, 
 }
 
 // split.rs:402-441 on a wave.
-__device__ bool search_numerical(const double* hg, const double* hh, const FeatDev& fd, const SplitParamsDev& p,
+__device__ __forceinline__ bool search_numerical(const double* hg, const double* hh, const FeatDev& fd, const SplitParamsDev& p,
                                  const LeafSums& ls, ScanLds& S, SplitRec& out) {
   double shift = leaf_gain(ls.sg, ls.sh, p, ls.nd, ls.po) + p.min_gain_to_split;
   bool spl = false;
@@ -907,7 +1023,7 @@ __device__ bool search_numerical(const double* hg, const double* hh, const FeatD
 
 // split.rs:446-670 (categorical, no monotone constraints) on lane 0; the
 // ctr order is a stable rank sort computed by all lanes.
-__device__ bool search_categorical(const double* hg, const double* hh, const FeatDev& fd, const SplitParamsDev& p0,
+__device__ __forceinline__ bool search_categorical(const double* hg, const double* hh, const FeatDev& fd, const SplitParamsDev& p0,
                                    const LeafSums& ls, ScanLds& S, SplitRec& out) {
   const int lane = threadIdx.x & 63;
   const int off = fd.offset;
@@ -1041,7 +1157,7 @@ __device__ bool search_categorical(const double* hg, const double* hh, const Fea
 
 // split.rs:993-1190 on a wave: packed int64 prefix sums are exact, so every
 // scan position is scored in parallel from a wave-parallel prefix.
-__device__ void scan_int(const long long* hv, const FeatDev& fd, const SplitParamsDev& p, const LeafSums& ls,
+__device__ __forceinline__ void scan_int(const long long* hv, const FeatDev& fd, const SplitParamsDev& p, const LeafSums& ls,
                          double min_gain_shift, bool reverse, bool skip_default, bool na, ScanLds& S, SplitRec& out,
                          bool& splittable) {
   const int lane = threadIdx.x & 63;
@@ -1153,7 +1269,7 @@ __device__ void scan_int(const long long* hv, const FeatDev& fd, const SplitPara
   }
 }
 
-__device__ bool search_int(const long long* hv, const FeatDev& fd, const SplitParamsDev& p, const LeafSums& ls,
+__device__ __forceinline__ bool search_int(const long long* hv, const FeatDev& fd, const SplitParamsDev& p, const LeafSums& ls,
                            ScanLds& S, SplitRec& out) {
   double shift = leaf_gain((double)(int)(ls.isum >> 32) * ls.gs, (double)(unsigned)ls.isum * ls.hs, p, ls.nd, ls.po) +
                  p.min_gain_to_split;
@@ -1193,7 +1309,7 @@ struct SplitArgs {
 
 // The leaf sums a search needs (leaf_splits.hpp; the root's come from the
 // fixed-point / packed root sums).
-__device__ LeafSums leaf_sums(const SplitArgs& a, int leaf, bool root) {
+__device__ __forceinline__ LeafSums leaf_sums(const SplitArgs& a, int leaf, bool root) {
   LeafSums s;
   const LeafDev& L = a.leaves[leaf];
   s.nd = (int)L.count;
@@ -1223,7 +1339,13 @@ __device__ LeafSums leaf_sums(const SplitArgs& a, int leaf, bool root) {
 // next leaf to split (learner/mod.rs:954-1101, 698-706).
 __global__ void __launch_bounds__(128) k_split(SplitArgs a) {
   StepDev* st = a.st;
-  if (st->done || st->num_leaves >= a.L) return;
+  if (st->done) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // the next partition pass reserves slots from zero
+    st->lcur = 0;
+    st->rcur = 0;
+  }
+  if (st->num_leaves >= a.L) return;
   __shared__ double hsm[2][kMaxBins];   // smaller: g, h
   __shared__ double hlg[2][kMaxBins];   // larger: g, h
   __shared__ long long ism[kMaxBins], ilg[kMaxBins];
@@ -1235,23 +1357,48 @@ __global__ void __launch_bounds__(128) k_split(SplitArgs a) {
   const int sm = st->smaller, lg = root ? -1 : st->larger, cur = st->cur_leaf;
   const int F = a.F, B = a.B;
   if (!st->skip) {
+    // every load this block needs is independent once the step record is
+    // known: issue them together (each is a cache miss after the previous
+    // kernel's writes), then work from registers and LDS
     const FeatDev fd = a.feats[f];
     const bool parent_spl = root ? true : a.splittable[(size_t)cur * F + f] != 0;
-    LeafSums ls_s = leaf_sums(a, sm, root);
-    __syncthreads();
+    const LeafSums ls_s = leaf_sums(a, sm, root);
+    const LeafSums ls_l = lg >= 0 ? leaf_sums(a, lg, false) : ls_s;
+    constexpr int PB = kMaxBins / 128;   // bins per thread
     if (!a.quant) {
       const int ssg = root ? fix_shift(a.leaves[sm].count, a.it->max_abs_g_bits) : st->hist_shift_g;
       const int ssh = root ? fix_shift(a.leaves[sm].count, a.it->max_abs_h_bits) : st->hist_shift_h;
       long long* acc = a.acc + (size_t)f * B * 2;
-      for (int b = tid; b < B; b += blockDim.x) {
-        long long gi = acc[2 * b], hi = acc[2 * b + 1];
-        acc[2 * b] = 0;
-        acc[2 * b + 1] = 0;
-        hsm[0][b] = ldexp((double)gi, -ssg);
-        hsm[1][b] = ldexp((double)hi, -ssh);
+      const double* pp = lg >= 0 ? a.pool + ((size_t)cur * F + f) * B * 2 : nullptr;   // parent
+      long long gi[PB], hi[PB];
+      double pg[PB], ph[PB];
+#pragma unroll
+      for (int k = 0; k < PB; ++k) {
+        const int b = tid + 128 * k;
+        gi[k] = hi[k] = 0;
+        pg[k] = ph[k] = 0.0;
+        if (b < B) {
+          gi[k] = acc[2 * b];
+          hi[k] = acc[2 * b + 1];
+          if (pp) {
+            pg[k] = pp[2 * b];
+            ph[k] = pp[2 * b + 1];
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < PB; ++k) {
+        const int b = tid + 128 * k;
+        if (b < B) {
+          acc[2 * b] = 0;
+          acc[2 * b + 1] = 0;
+          hsm[0][b] = ldexp((double)gi[k], -ssg);
+          hsm[1][b] = ldexp((double)hi[k], -ssh);
+        }
       }
       __syncthreads();
-      // histogram.rs:42-56: the most frequent bin from the leaf totals
+      // histogram.rs:42-56: the most frequent bin from the leaf totals, in
+      // the reference's order (two sequential chains)
       if (fd.mfb > 0 && tid < 2) {
         double v = tid == 0 ? ls_s.sg : ls_s.sh - 2.0 * KEPS;
         const double* __restrict__ arr = hsm[tid];
@@ -1268,53 +1415,75 @@ __global__ void __launch_bounds__(128) k_split(SplitArgs a) {
       }
       __syncthreads();
       double* ps = a.pool + ((size_t)sm * F + f) * B * 2;
-      if (lg >= 0) {
-        const double* pp = a.pool + ((size_t)cur * F + f) * B * 2;   // parent (read before any write)
-        for (int b = tid; b < B; b += blockDim.x) {
-          hlg[0][b] = pp[2 * b] - hsm[0][b];
-          hlg[1][b] = pp[2 * b + 1] - hsm[1][b];
+      double* pl = lg >= 0 ? a.pool + ((size_t)lg * F + f) * B * 2 : nullptr;
+#pragma unroll
+      for (int k = 0; k < PB; ++k) {
+        const int b = tid + 128 * k;
+        if (b < B) {
+          const double sgv = hsm[0][b], shv = hsm[1][b];
+          ps[2 * b] = sgv;
+          ps[2 * b + 1] = shv;
+          if (pl) {
+            const double lgv = pg[k] - sgv, lhv = ph[k] - shv;
+            hlg[0][b] = lgv;
+            hlg[1][b] = lhv;
+            pl[2 * b] = lgv;
+            pl[2 * b + 1] = lhv;
+          }
         }
-        __syncthreads();
-        double* pl = a.pool + ((size_t)lg * F + f) * B * 2;
-        for (int b = tid; b < B; b += blockDim.x) {
-          pl[2 * b] = hlg[0][b];
-          pl[2 * b + 1] = hlg[1][b];
-        }
-      }
-      for (int b = tid; b < B; b += blockDim.x) {
-        ps[2 * b] = hsm[0][b];
-        ps[2 * b + 1] = hsm[1][b];
       }
     } else {
       long long* acc = a.acc + (size_t)f * B;
-      for (int b = tid; b < B; b += blockDim.x) {
-        ism[b] = acc[b];
-        acc[b] = 0;
+      const long long* pp = lg >= 0 ? a.ipool + ((size_t)cur * F + f) * B : nullptr;
+      long long v[PB], pv[PB];
+      unsigned long long part = 0;
+#pragma unroll
+      for (int k = 0; k < PB; ++k) {
+        const int b = tid + 128 * k;
+        v[k] = pv[k] = 0;
+        if (b < B) {
+          v[k] = acc[b];
+          if (pp) pv[k] = pp[b];
+        }
       }
+#pragma unroll
+      for (int k = 0; k < PB; ++k) {
+        const int b = tid + 128 * k;
+        if (b < B) {
+          acc[b] = 0;
+          ism[b] = v[k];
+          if (b != fd.mfb && b < fd.num_bin) part += (unsigned long long)v[k];
+        }
+      }
+      // histogram.rs:67-78: wrapping sums are exact in any order, so the
+      // most frequent bin is the leaf total minus a block reduction
+      for (int o = 32; o > 0; o >>= 1) part += (unsigned long long)__shfl_xor((long long)part, o, 64);
+      __shared__ unsigned long long wpart[2];
+      if (lane == 0) wpart[wave] = part;
       __syncthreads();
-      if (fd.mfb > 0 && tid == 0) {
-        unsigned long long s = (unsigned long long)ls_s.isum;
-        for (int i = 0; i < fd.num_bin; ++i)
-          if (i != fd.mfb) s -= (unsigned long long)ism[i];
-        ism[fd.mfb] = (long long)s;
-      }
+      if (fd.mfb > 0 && tid == 0) ism[fd.mfb] = (long long)((unsigned long long)ls_s.isum - wpart[0] - wpart[1]);
       __syncthreads();
       long long* ps = a.ipool + ((size_t)sm * F + f) * B;
-      if (lg >= 0) {
-        const long long* pp = a.ipool + ((size_t)cur * F + f) * B;
-        for (int b = tid; b < B; b += blockDim.x)
-          ilg[b] = (long long)((unsigned long long)pp[b] - (unsigned long long)ism[b]);
-        __syncthreads();
-        long long* pl = a.ipool + ((size_t)lg * F + f) * B;
-        for (int b = tid; b < B; b += blockDim.x) pl[b] = ilg[b];
+      long long* pl = lg >= 0 ? a.ipool + ((size_t)lg * F + f) * B : nullptr;
+#pragma unroll
+      for (int k = 0; k < PB; ++k) {
+        const int b = tid + 128 * k;
+        if (b < B) {
+          const long long sv = ism[b];
+          ps[b] = sv;
+          if (pl) {
+            const long long lv = (long long)((unsigned long long)pv[k] - (unsigned long long)sv);
+            ilg[b] = lv;
+            pl[b] = lv;
+          }
+        }
       }
-      for (int b = tid; b < B; b += blockDim.x) ps[b] = ism[b];
     }
     __syncthreads();
     // wave 0 searches the smaller leaf, wave 1 the larger
     const int leaf = wave == 0 ? sm : lg;
     if (leaf >= 0) {
-      LeafSums ls = wave == 0 ? ls_s : leaf_sums(a, leaf, false);
+      const LeafSums ls = wave == 0 ? ls_s : ls_l;
       SplitRec& r = res[wave];
       if (lane == 0) reset_rec(r);
       bool spl = false;
@@ -1384,195 +1553,180 @@ __device__ __forceinline__ bool goes_left(int b, const FeatDev& fd, const SplitR
   return miss ? s.default_left != 0 : b <= s.threshold;
 }
 
-// Best split of one searched leaf over its features (learner/mod.rs:1064-1101):
-// the feature index in `feat_split[w]`, or -1.
-__device__ int reduce_features(const PartArgs& a, int w, double* sg, int* sf) {
-  const int tid = threadIdx.x;
-  double bgain = -INFINITY;
-  int bf = -1;
-  for (int k = tid; k < a.F; k += blockDim.x) {
-    const SplitRec& r = a.feat_split[w * a.F + k];
-    if (better(r.gain, r.feature, bgain, bf)) { bgain = r.gain; bf = k; }
-  }
-  sg[tid] = bgain;
-  sf[tid] = bf;
-  __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if (tid < o && better(sg[tid + o], sf[tid + o], sg[tid], sf[tid])) {
-      sg[tid] = sg[tid + o];
-      sf[tid] = sf[tid + o];
-    }
-    __syncthreads();
-  }
-  const int r = sf[0];
-  __syncthreads();
+// A candidate of the split decision: gain, feature (-1 none) and position
+// (feature slot or leaf). pick() keeps the better one in the order the
+// reference's scans imply - higher gain, then smaller feature
+// (learner/mod.rs:1064-1101), then the earlier position (first max,
+// learner/mod.rs:698-706). The order is total, so any reduction tree gives
+// the same answer.
+struct Cand {
+  double g;
+  int f, l;
+};
+
+__device__ __forceinline__ Cand pick(Cand a, Cand b) {
+  const bool take = b.l != INT32_MAX &&
+                    (a.l == INT32_MAX || better(b.g, b.f, a.g, a.f) || (!better(a.g, a.f, b.g, b.f) && b.l < a.l));
+  // field-wise selects keep the candidates in registers
+  Cand r;
+  r.g = take ? b.g : a.g;
+  r.f = take ? b.f : a.f;
+  r.l = take ? b.l : a.l;
   return r;
 }
 
-// Pass 1. Every block first finishes the last step (each searched leaf's best
-// split, then ArgMax over all leaves, learner/mod.rs:698-706) - the same
-// result in every block, so no block waits for another - then counts the
-// left rows of its tiles. Block 0 publishes the decision for later kernels.
-__global__ void __launch_bounds__(kPartThreads) k_part_count(PartArgs a) {
-  StepDev* st = a.st;
-  if (st->done) return;
-  __shared__ double sg[kPartThreads];
-  __shared__ int sf[kPartThreads], sl[kPartThreads];
-  __shared__ SplitRec chosen;
-  const int tid = threadIdx.x;
-  const bool root = st->is_root != 0;
-  const bool searched = st->skip == 0;
-  const int lv[2] = {st->smaller, root ? -1 : st->larger};
-  int fw[2] = {-1, -1};
-  if (searched)
-    for (int w = 0; w < 2; ++w)
-      if (lv[w] >= 0) fw[w] = reduce_features(a, w, sg, sf);
-  // ArgMax over leaf slots; the searched leaves use this step's results
-  double bg = -INFINITY;
-  int bf = -1, bl = INT32_MAX;
-  for (int l = tid; l < a.L; l += kPartThreads) {
-    double g;
-    int f;
-    int w = (searched && l == lv[0]) ? 0 : (searched && l == lv[1]) ? 1 : -1;
-    if (w >= 0) {
-      g = fw[w] >= 0 ? a.feat_split[w * a.F + fw[w]].gain : -INFINITY;
-      f = fw[w] >= 0 ? a.feat_split[w * a.F + fw[w]].feature : -1;
-    } else {
-      g = a.best_gain[l];
-      f = a.best_feat[l];
-    }
-    if (bl == INT32_MAX || better(g, f, bg, bf)) { bg = g; bf = f; bl = l; }
+__device__ __forceinline__ Cand wave_pick(Cand c) {
+  for (int o = 32; o > 0; o >>= 1) {
+    Cand d;
+    d.g = __shfl_xor(c.g, o, 64);
+    d.f = __shfl_xor(c.f, o, 64);
+    d.l = __shfl_xor(c.l, o, 64);
+    c = pick(c, d);
   }
-  sg[tid] = bg;
-  sf[tid] = bf;
-  sl[tid] = bl;
-  __syncthreads();
-  for (int o = kPartThreads / 2; o > 0; o >>= 1) {
-    if (tid < o) {
-      const double g2 = sg[tid + o];
-      const int f2 = sf[tid + o], l2 = sl[tid + o];
-      const bool take = l2 != INT32_MAX &&
-                        (sl[tid] == INT32_MAX || better(g2, f2, sg[tid], sf[tid]) ||
-                         (!better(sg[tid], sf[tid], g2, f2) && l2 < sl[tid]));
-      if (take) { sg[tid] = g2; sf[tid] = f2; sl[tid] = l2; }
-    }
-    __syncthreads();
-  }
-  const int best_leaf = sl[0];
-  const double gain = sg[0];
-  const bool stop = !(gain > 0.0) || st->num_leaves >= a.L;
-  if (tid == 0) {
-    const int w = (searched && best_leaf == lv[0]) ? 0 : (searched && best_leaf == lv[1]) ? 1 : -1;
-    if (!stop) chosen = w >= 0 ? a.feat_split[w * a.F + fw[w]] : a.best[best_leaf];
-  }
-  if (blockIdx.x == 0 && tid == 0) {
-    for (int w = 0; w < 2; ++w) {
-      if (!searched || lv[w] < 0) continue;
-      SplitRec r;
-      if (fw[w] >= 0) r = a.feat_split[w * a.F + fw[w]];
-      else reset_rec(r);
-      a.best[lv[w]] = r;
-      a.best_gain[lv[w]] = r.gain;
-      a.best_feat[lv[w]] = r.feature;
-    }
-    if (stop) {
-      st->done = 1;
-    } else {
-      *a.cur = chosen;
-      st->cur_leaf = best_leaf;
-      st->new_leaf = st->num_leaves;
-      const LeafDev& Lc = a.leaves[best_leaf];
-      st->seg_begin = Lc.begin;
-      st->seg_count = Lc.count;
-      st->seg_buf = Lc.buf;
-    }
-  }
-  if (stop) return;
-  __syncthreads();
-  const SplitRec& s = chosen;
-  const FeatDev fd = a.feats[s.feature];
-  const LeafDev& Lc = a.leaves[best_leaf];
-  const long long begin = Lc.begin, count = Lc.count;
-  const unsigned* idx = a.idx[Lc.buf] + begin;
-  const uint8_t* col = a.bins_cm + (size_t)s.feature * a.N;
-  const int ntiles = (int)((count + kPartTile - 1) / kPartTile);
-  __shared__ int wsum[kPartThreads / 64];
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    long long p0 = (long long)t * kPartTile + tid * kPartRowsPerThread;
-    int c = 0;
-#pragma unroll
-    for (int k = 0; k < kPartRowsPerThread; ++k) {
-      long long p = p0 + k;
-      if (p < count) c += goes_left(col[idx[p]], fd, s) ? 1 : 0;
-    }
-    c = wave_sum(c);
-    if ((tid & 63) == 0) wsum[tid >> 6] = c;
-    __syncthreads();
-    if (tid == 0) {
-      int tot = 0;
-      for (int w = 0; w < kPartThreads / 64; ++w) tot += wsum[w];
-      a.tile_left[t] = tot;
-    }
-    __syncthreads();
-  }
+  return c;
 }
 
-// Pass 2: every block scans the tile counts itself (no grid handshake),
-// then scatters its tiles' (row, gradient) pairs stably to the other buffer
-// (partition.rs:107-174). Block 0 applies the split to the leaf records and
-// prepares the next step (learner/mod.rs:1105-1225, 712-731).
-constexpr int kMaxOwnTiles = 64;
-
-__global__ void __launch_bounds__(kPartThreads) k_part_scatter(PartArgs a) {
+// One pass per split. Every block first finishes the last step (each
+// searched leaf's best split, then ArgMax over all leaves) - the same result
+// in every block, so no block waits for another - then moves its tiles'
+// (row, gradient) pairs to the other buffer: left rows from the front of the
+// leaf's segment, right rows from its back, each tile reserving its slots
+// with one atomic per side (DataPartition::split, partition.rs:107-174). The
+// row order inside a child is unspecified: every consumer of a segment (exact
+// fixed-point or integer histograms, renewal sums, score update) is
+// order-independent. Block 0 publishes the decision for later kernels.
+__global__ void __launch_bounds__(kPartThreads) k_partition(PartArgs a) {
   StepDev* st = a.st;
-  if (st->done) return;
-  const SplitRec s = *a.cur;
-  const FeatDev fd = a.feats[s.feature];
-  const long long begin = st->seg_begin, count = st->seg_count;
-  const int src = st->seg_buf, dst = 1 - src;
+  constexpr int NW = kPartThreads / 64;
+  constexpr int RW = (int)(sizeof(SplitRec) / 4);
+  static_assert(RW <= kPartThreads, "a split record is copied one word per thread");
+  __shared__ Cand sc[3][NW];
+  __shared__ int sdone;
+  __shared__ SplitRec chosen;
   const int tid = threadIdx.x, lane = tid & 63;
-  const int ntiles = (int)((count + kPartTile - 1) / kPartTile);
-  // exclusive prefix of the tile counts at this block's tiles, and the total
-  __shared__ long long own_off[kMaxOwnTiles];
-  __shared__ long long wtot[kPartThreads / 64];
-  __shared__ long long carry_s;
-  long long carry = 0;
-  for (int base = 0; base < ntiles; base += kPartThreads) {
-    const int t = base + tid;
-    const long long v = t < ntiles ? a.tile_left[t] : 0;
-    long long x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-      long long y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) wtot[tid >> 6] = x;
-    __syncthreads();
-    long long wb = 0;
-    for (int w = 0; w < (tid >> 6); ++w) wb += wtot[w];
-    const long long excl = carry + wb + x - v;
-    if (t < ntiles && (t % (int)gridDim.x) == (int)blockIdx.x) {
-      const int k = t / (int)gridDim.x;
-      if (k < kMaxOwnTiles) own_off[k] = excl;
-    }
-    if (tid == kPartThreads - 1) carry_s = excl + v;
-    __syncthreads();
-    carry = carry_s;
-    __syncthreads();
+  // The step record, this step's per-feature searches and every leaf's best
+  // split are independent loads: issue them together.
+  const int done = st->done, root = st->is_root, skipped = st->skip;
+  const int lv0 = st->smaller, lv1 = root ? -1 : st->larger, nleaves = st->num_leaves;
+  Cand c0, c1, cl;
+  c0.g = c1.g = cl.g = -INFINITY;
+  c0.f = c1.f = cl.f = -1;
+  c0.l = c1.l = cl.l = INT32_MAX;
+  for (int k = tid; k < a.F; k += kPartThreads) {
+    const SplitRec& r0 = a.feat_split[k];
+    const SplitRec& r1 = a.feat_split[a.F + k];
+    c0 = pick(c0, Cand{r0.gain, r0.feature, k});
+    c1 = pick(c1, Cand{r1.gain, r1.feature, k});
   }
-  const long long lc = carry, rc = count - carry;
-  const unsigned* idx = a.idx[src] + begin;
-  unsigned* oidx = a.idx[dst] + begin;
+  const bool searched = skipped == 0;
+  for (int l = tid; l < a.L; l += kPartThreads) {
+    const double g = a.best_gain[l];
+    const int f = a.best_feat[l];
+    if (searched && (l == lv0 || l == lv1)) continue;   // replaced by this step's results
+    cl = pick(cl, Cand{g, f, l});
+  }
+  c0 = wave_pick(c0);
+  c1 = wave_pick(c1);
+  cl = wave_pick(cl);
+  if (lane == 0) {
+    sc[0][tid >> 6] = c0;
+    sc[1][tid >> 6] = c1;
+    sc[2][tid >> 6] = cl;
+  }
+  if (tid == 0) sdone = done;
+  __syncthreads();
+  if (sdone) return;
+  c0 = sc[0][0];
+  c1 = sc[1][0];
+  cl = sc[2][0];
+  for (int w = 1; w < NW; ++w) {
+    c0 = pick(c0, sc[0][w]);
+    c1 = pick(c1, sc[1][w]);
+    cl = pick(cl, sc[2][w]);
+  }
+  // a searched leaf without any split candidate carries (-inf, none)
+  const int fw0 = searched && better(c0.g, c0.f, -INFINITY, -1) ? c0.l : -1;
+  const int fw1 = searched && lv1 >= 0 && better(c1.g, c1.f, -INFINITY, -1) ? c1.l : -1;
+  Cand s0, s1;
+  s0.g = fw0 >= 0 ? c0.g : -INFINITY;
+  s0.f = fw0 >= 0 ? c0.f : -1;
+  s0.l = lv0;
+  s1.g = fw1 >= 0 ? c1.g : -INFINITY;
+  s1.f = fw1 >= 0 ? c1.f : -1;
+  s1.l = lv1;
+  Cand best = cl;
+  if (searched) {
+    best = pick(best, s0);
+    if (lv1 >= 0) best = pick(best, s1);
+  }
+  const int best_leaf = best.l;
+  const bool stop = !(best.g > 0.0) || nleaves >= a.L;
+  const int w = (searched && best_leaf == lv0) ? 0 : (searched && lv1 >= 0 && best_leaf == lv1) ? 1 : -1;
+  // second round of loads: the chosen record, its feature and its leaf
+  if (!stop && tid < RW) {
+    const SplitRec* src_rec = w == 0 ? &a.feat_split[fw0] : w == 1 ? &a.feat_split[a.F + fw1] : &a.best[best_leaf];
+    ((unsigned*)&chosen)[tid] = ((const unsigned*)src_rec)[tid];
+  }
+  if (blockIdx.x == 0 && searched) {
+    // the searched leaves' best splits become the leaves' records
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int leaf = v == 0 ? lv0 : lv1;
+      const int fw = v == 0 ? fw0 : fw1;
+      if (leaf < 0) continue;
+      if (fw >= 0) {
+        if (tid < RW) ((unsigned*)&a.best[leaf])[tid] = ((const unsigned*)&a.feat_split[v * a.F + fw])[tid];
+      } else if (tid == 0) {
+        SplitRec r;
+        reset_rec(r);
+        a.best[leaf] = r;
+      }
+      if (tid == 0) {
+        a.best_gain[leaf] = v == 0 ? s0.g : s1.g;
+        a.best_feat[leaf] = v == 0 ? s0.f : s1.f;
+      }
+    }
+  }
+  if (stop) {
+    if (blockIdx.x == 0 && tid == 0) st->done = 1;
+    return;
+  }
+  const FeatDev fd = a.feats[best.f];
+  const LeafDev& Lc = a.leaves[best_leaf];
+  const long long begin = Lc.begin, count = Lc.count;
+  const int lbuf = Lc.buf;
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    if (tid < RW) ((unsigned*)a.cur)[tid] = ((const unsigned*)&chosen)[tid];
+    if (tid == 0) {
+      st->cur_leaf = best_leaf;
+      st->new_leaf = nleaves;
+      st->seg_begin = begin;
+      st->seg_count = count;
+      st->seg_buf = lbuf;
+      st->seg_depth = Lc.depth;
+    }
+  }
+  const SplitRec& s = chosen;
+  const int src = lbuf;
+  const unsigned* idx = (src ? a.idx[1] : a.idx[0]) + begin;
+  unsigned* oidx = (src ? a.idx[0] : a.idx[1]) + begin;
+  const void* igh = src ? a.gh[1] : a.gh[0];
+  void* ogh = src ? a.gh[0] : a.gh[1];
   const uint8_t* col = a.bins_cm + (size_t)s.feature * a.N;
+  const int ntiles = (int)((count + kPartTile - 1) / kPartTile);
   __shared__ int wpre[kPartThreads / 64];
-  for (int t = blockIdx.x, k = 0; t < ntiles; t += gridDim.x, ++k) {
+  __shared__ unsigned long long slot[2];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const long long t0 = (long long)t * kPartTile;
     const long long p0 = t0 + tid * kPartRowsPerThread;
+    const int tile_rows = (int)min((long long)kPartTile, count - t0);
     unsigned rows[kPartRowsPerThread];
     bool left[kPartRowsPerThread];
     int c = 0;
 #pragma unroll
     for (int u = 0; u < kPartRowsPerThread; ++u) {
-      long long p = p0 + u;
+      const long long p = p0 + u;
       left[u] = false;
       if (p < count) {
         rows[u] = idx[p];
@@ -1582,77 +1736,35 @@ __global__ void __launch_bounds__(kPartThreads) k_part_scatter(PartArgs a) {
     }
     int x = c;
     for (int o = 1; o < 64; o <<= 1) {
-      int y = __shfl_up(x, o, 64);
+      const int y = __shfl_up(x, o, 64);
       if (lane >= o) x += y;
     }
     if (lane == 63) wpre[tid >> 6] = x;
     __syncthreads();
-    int wbase = 0;
-    for (int w = 0; w < (tid >> 6); ++w) wbase += wpre[w];
+    int wbase = 0, total = 0;
+    for (int w = 0; w < kPartThreads / 64; ++w) {
+      if (w < (tid >> 6)) wbase += wpre[w];
+      total += wpre[w];
+    }
     const int excl_left = wbase + x - c;
+    if (tid == 0) {
+      slot[0] = atomicAdd(&st->lcur, (unsigned long long)total);
+      slot[1] = atomicAdd(&st->rcur, (unsigned long long)(tile_rows - total));
+    }
     __syncthreads();
-    const long long tile_off = own_off[k];
-    long long lpos = tile_off + excl_left;
-    long long rpos = lc + (t0 - tile_off) + ((long long)tid * kPartRowsPerThread - excl_left);
+    const long long nr = tile_rows - total;
+    long long lpos = (long long)slot[0] + excl_left;
+    long long rpos = count - (long long)slot[1] - nr + ((long long)tid * kPartRowsPerThread - excl_left);
 #pragma unroll
     for (int u = 0; u < kPartRowsPerThread; ++u) {
-      long long p = p0 + u;
+      const long long p = p0 + u;
       if (p >= count) break;
-      long long d = left[u] ? lpos++ : rpos++;
+      const long long d = left[u] ? lpos++ : rpos++;
       oidx[d] = rows[u];
-      if (a.gh_bytes == 8) ((float2*)a.gh[dst] + begin)[d] = ((const float2*)a.gh[src] + begin)[p];
-      else ((char2*)a.gh[dst] + begin)[d] = ((const char2*)a.gh[src] + begin)[p];
+      if (a.gh_bytes == 8) ((float2*)ogh + begin)[d] = ((const float2*)igh + begin)[p];
+      else ((char2*)ogh + begin)[d] = ((const char2*)igh + begin)[p];
     }
-  }
-  if (blockIdx.x == 0 && tid == 0) {
-    const int cur = st->cur_leaf, nl = st->new_leaf;
-    LeafDev& Lc = a.leaves[cur];
-    LeafDev& Ln = a.leaves[nl];
-    const int depth = Lc.depth + 1;
-    Ln.begin = begin + lc;
-    Ln.count = rc;
-    Ln.buf = dst;
-    Ln.depth = depth;
-    Ln.sum_g = s.rsg;
-    Ln.sum_h = s.rsh;
-    Ln.int_sum = s.rgh;
-    Ln.output = isnan(s.right_output) ? 0.0 : s.right_output;
-    Lc.begin = begin;
-    Lc.count = lc;
-    Lc.buf = dst;
-    Lc.depth = depth;
-    Lc.sum_g = s.lsg;
-    Lc.sum_h = s.lsh;
-    Lc.int_sum = s.lgh;
-    Lc.output = isnan(s.left_output) ? 0.0 : s.left_output;
-    SplitRec r = s;
-    r.left_count = (int)lc;
-    r.right_count = (int)rc;
-    r.leaf = cur;
-    a.log[st->num_splits] = r;
-    st->num_splits += 1;
-    st->num_leaves += 1;
-    st->left_total = lc;
-    st->is_root = 0;
-    if (lc < rc) { st->smaller = cur; st->larger = nl; }
-    else { st->smaller = nl; st->larger = cur; }
-    // learner/mod.rs:712-731 before_find_best_split
-    const bool skip = (a.p.max_depth > 0 && depth >= a.p.max_depth) ||
-                      (rc < 2LL * a.p.min_data_in_leaf && lc < 2LL * a.p.min_data_in_leaf);
-    st->skip = skip ? 1 : 0;
-    if (skip) {
-      a.best[cur].gain = -INFINITY;
-      a.best[nl].gain = -INFINITY;
-      a.best_gain[cur] = -INFINITY;
-      a.best_gain[nl] = -INFINITY;
-      a.best_feat[nl] = a.best[nl].feature;
-      a.best_feat[cur] = a.best[cur].feature;
-    }
-    if (!a.quant) {
-      const long long sc = lc < rc ? lc : rc;
-      st->hist_shift_g = fix_shift(sc, a.it->max_abs_g_bits);
-      st->hist_shift_h = fix_shift(sc, a.it->max_abs_h_bits);
-    }
+    __syncthreads();
   }
 }
 
@@ -1898,8 +2010,7 @@ void set_hist_lds_limits(size_t bytes) {
 
 void launch_split(const SplitArgs& a, hipStream_t s) { hipLaunchKernelGGL(k_split, dim3(a.F), dim3(128), 0, s, a); }
 void launch_part(const PartArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_part_count, dim3(grid), dim3(kPartThreads), 0, s, a);
-  hipLaunchKernelGGL(k_part_scatter, dim3(grid), dim3(kPartThreads), 0, s, a);
+  hipLaunchKernelGGL(k_partition, dim3(grid), dim3(kPartThreads), 0, s, a);
 }
 int part_tile() { return kPartTile; }
 
