@@ -11,6 +11,14 @@ update) over the full matrix resident in HBM. Warmup iterations run first;
 then exactly --steps iterations are timed between device synchronizations.
 
 Prints one JSON line (the driver's contract) and exits 0.
+
+Kernel timing: the timed iterations run as captured HIP graphs with no
+instrumentation; --prof-iters further iterations then run with HIP events
+around every histogram / split / partition launch on the context's stream,
+which gives the histogram kernel's average launch duration and GB/s. HBM
+traffic per histogram launch comes from rocprofv3 FETCH_SIZE / WRITE_SIZE
+passes over this same command, scaled by the counters' reading on a copy of
+known size (--copy-calibrate), recorded in profiles/hist_traffic.json.
 """
 import argparse
 import importlib
@@ -25,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hist_traffic.json")
 
 
 def log(msg):
@@ -78,6 +87,34 @@ def cpu_baseline(rows, iters, params, full_rows):
                        f"timed iterations after 1 warm, {its:.4f} it/s on the sample")
 
 
+def copy_calibrate(mib):
+    """One device-to-device copy of `mib` MiB (a kernel of known traffic: the
+    counters should read mib MiB fetched and mib MiB written)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibrate_mib": mib, "copies": 3}), flush=True)
+
+
+def recorded_traffic(form, rows, max_bin, num_leaves):
+    """HBM bytes per histogram launch from the committed counter passes, when
+    they were taken on this exact workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = t.get("workload", {})
+    if (w.get("form"), w.get("rows"), w.get("max_bin"), w.get("num_leaves")) != (form, rows, max_bin, num_leaves):
+        return None
+    return t.get("hbm_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,7 +128,13 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="quant,plain")
+    ap.add_argument("--no-graph", action="store_true", help="stream launches instead of graphs (counter passes)")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (counter scaling) and exit")
     args = ap.parse_args()
+    if args.copy_calibrate:
+        copy_calibrate(args.copy_calibrate)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the multi-GPU row-sharded "
               "path is a design, not a build)", file=sys.stderr)
@@ -104,7 +147,7 @@ def main():
     Xv, yv = datagen.holdout("binary_10p5m", 200_000)
     log(f"data {X.shape} in {time.perf_counter() - t:.1f}s")
     base = dict(objective="binary", num_leaves=args.num_leaves, max_bin=args.max_bin, min_data_in_leaf=20,
-                learning_rate=0.1, device_type="gpu")
+                learning_rate=0.1, device_type="gpu", use_graph=not args.no_graph)
     t = time.perf_counter()
     ds = lgb.Dataset(X, label=y, params=base)
     log(f"host binning in {time.perf_counter() - t:.1f}s, {ds.num_features} features")
@@ -129,13 +172,14 @@ def main():
     m = forms[main_form]
     p = m["prof"]
     roof = dict(bound="hbm", achieved=m["hist_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=(m["hist_gbs"] / HBM_PEAK_GBS) if m["hist_gbs"] else None, traffic=None,
+                frac=(m["hist_gbs"] / HBM_PEAK_GBS) if m["hist_gbs"] else None,
+                traffic=recorded_traffic(main_form, args.rows, args.max_bin, args.num_leaves),
                 kernel="k_hist_int" if main_form == "quant" else "k_hist_f64",
                 bytes_per_launch=m["hist_bytes_per_launch"], avg_launch_us=m["hist_avg_launch_us"])
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
-        cparams = {k: v for k, v in base.items() if k != "device_type"}
+        cparams = {k: v for k, v in base.items() if k not in ("device_type", "use_graph")}
         if main_form == "quant":
             cparams.update(use_quantized_grad=True, num_grad_quant_bins=16, quant_train_renew_leaf=True)
         try:

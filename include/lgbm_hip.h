@@ -176,8 +176,9 @@ lgbmhip_status lgbmhip_predict_binned(lgbmhip_ctx* ctx, const uint8_t* bins, int
 /* Profiling of training iterations: enable = 1 records HIP events around every
  * histogram, split and partition launch (graphs off), 0 stops, -1 only reads.
  * out8 (may be NULL) receives the totals since enabling: {histogram ms,
- * histogram launches that summed rows, rows summed, algorithmic histogram
- * bytes, split ms, partition ms, iteration ms, iterations}. No counterpart in
+ * histogram launches (all of them, num_leaves per tree), rows summed,
+ * algorithmic histogram bytes, split ms, partition ms, iteration ms,
+ * iterations}. No counterpart in
  * the reference (it has no profiler hooks); for bench and rocprof cross-checks. */
 lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out8);
 

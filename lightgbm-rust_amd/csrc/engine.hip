@@ -822,8 +822,8 @@ static double hist_bytes(Ctx* c, int64_t m, bool root) {
          (double)c->F * c->B * (quant ? 8.0 : 16.0);
 }
 
-// Per-kind device time of the tree just grown; histogram launches that did
-// work (the root and every non-skipped step) carry the rows they summed.
+// Per-kind device time of the tree just grown (event to event on the
+// context's stream, so each figure includes that launch's dispatch gap).
 static void account_profile(Ctx* c, int k, const HostTree& t) {
   const StepDev& st = *c->h_st[k];
   auto el = [&](hipEvent_t a, hipEvent_t b) {
@@ -831,26 +831,30 @@ static void account_profile(Ctx* c, int k, const HostTree& t) {
     HIP_CHECK(hipEventElapsedTime(&ms, a, b));
     return (double)ms;
   };
+  // every launch counts (L histogram launches per tree, as a kernel trace
+  // counts them); rows and bytes only where the launch summed rows
   c->prof[0] += el(pevent(c, k, 0, 0, 0), pevent(c, k, 0, 0, 1));
   c->prof[1] += 1;
   c->prof[2] += (double)c->N;
   c->prof[3] += hist_bytes(c, c->N, true);
   c->prof[4] += el(pevent(c, k, 0, 0, 1), pevent(c, k, 0, 1, 1));
+  for (int step = 1; step < c->L; ++step) {
+    c->prof[0] += el(pevent(c, k, step, 2, 1), pevent(c, k, step, 0, 1));
+    c->prof[1] += 1;
+    c->prof[4] += el(pevent(c, k, step, 0, 1), pevent(c, k, step, 1, 1));
+    c->prof[5] += el(pevent(c, k, step, 2, 0), pevent(c, k, step, 2, 1));
+  }
   std::vector<int> depth(c->L, 0);
   for (int i = 0; i < st.num_splits; ++i) {
     const SplitRec& r = c->h_log[k][i];
-    const int step = i + 1;
-    c->prof[5] += el(pevent(c, k, step, 2, 0), pevent(c, k, step, 2, 1));
-    c->prof[4] += el(pevent(c, k, step, 0, 1), pevent(c, k, step, 1, 1));
     const int d = depth[r.leaf] + 1;
     depth[r.leaf] = d;
     depth[i + 1] = d;
     const bool skip = (c->sp.max_depth > 0 && d >= c->sp.max_depth) ||
                       (r.right_count < 2LL * c->sp.min_data_in_leaf && r.left_count < 2LL * c->sp.min_data_in_leaf);
-    if (skip) continue;
+    const bool complete = i + 2 >= c->L;   // the tree is full: no histogram after the last split
+    if (skip || complete) continue;
     const int64_t m = std::min(r.left_count, r.right_count);
-    c->prof[0] += el(pevent(c, k, step, 2, 1), pevent(c, k, step, 0, 1));
-    c->prof[1] += 1;
     c->prof[2] += (double)m;
     c->prof[3] += hist_bytes(c, m, false);
   }
