@@ -201,8 +201,11 @@ struct Ctx {
   bool state_ready = false;
   DevBuf<double> score;
   DevBuf<float> g, h;
-  DevBuf<unsigned> idx[2];
-  DevBuf<uint8_t> gh[2];
+  // three buffers of (row index, gradients, group-major bins): buffer 0
+  // holds the root in row order, leaves alternate between buffers 1 and 2
+  DevBuf<unsigned> idx[3];
+  DevBuf<uint8_t> gh[3];
+  DevBuf<uint8_t> grp[3];
   DevBuf<long long> acc;
   DevBuf<double> pool;
   DevBuf<long long> ipool;
@@ -444,7 +447,7 @@ static void setup_training(Ctx* c) {
   c->score.zero(c->stream);
   c->g.alloc((size_t)K * N);
   c->h.alloc((size_t)K * N);
-  for (int b = 0; b < 2; ++b) {
+  for (int b = 0; b < 3; ++b) {
     c->idx[b].alloc(N);
     c->gh[b].alloc((size_t)N * (quant ? 2 : 8));
   }
@@ -505,6 +508,15 @@ static void setup_training(Ctx* c) {
     c->n_calls = 0;
   }
   plan_hist(c, quant);
+  // ordered bins: the root's rows in row order, group-major
+  {
+    const size_t gbytes = (size_t)c->hist_ng * N * c->hist_fb;
+    for (int b = 0; b < 3; ++b) c->grp[b].alloc(gbytes);
+    hipLaunchKernelGGL(k_group_rows, dim3(grid_for(N * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
+                       c->stream, (const unsigned*)nullptr, (long long)N, c->bins_rm.p, c->S, c->F, c->hist_fb,
+                       c->hist_ng, (long long)N, c->grp[0].p);
+    HIP_CHECK(hipGetLastError());
+  }
   for (auto p : c->h_log) if (p) (void)hipHostFree(p);
   for (auto p : c->h_leaves) if (p) (void)hipHostFree(p);
   for (auto p : c->h_st) if (p) (void)hipHostFree(p);
@@ -527,12 +539,10 @@ static void setup_training(Ctx* c) {
 
 static HistArgs hist_args(Ctx* c, bool quant, int k, int mode) {
   HistArgs a{};
-  a.bins = c->bins_rm.p;
-  a.S = c->S;
-  a.idx[0] = c->idx[0].p;
-  a.idx[1] = c->idx[1].p;
-  a.gh[0] = c->gh[0].p;
-  a.gh[1] = c->gh[1].p;
+  for (int b = 0; b < 3; ++b) {
+    a.grp[b] = c->grp[b].p;
+    a.gh[b] = c->gh[b].p;
+  }
   a.st = c->st.p;
   a.leaves = c->leaves.p;
   a.feats = c->d_feats.p;
@@ -584,12 +594,14 @@ static PartArgs part_args(Ctx* c, int k) {
   a.st = c->st.p;
   a.leaves = c->leaves.p;
   a.feats = c->d_feats.p;
-  a.bins_cm = c->bins_cm.p;
+  for (int b = 0; b < 3; ++b) {
+    a.grp[b] = c->grp[b].p;
+    a.idx[b] = c->idx[b].p;
+    a.gh[b] = c->gh[b].p;
+  }
+  a.FB = c->hist_fb;
+  a.NG = c->hist_ng;
   a.N = c->N;
-  a.idx[0] = c->idx[0].p;
-  a.idx[1] = c->idx[1].p;
-  a.gh[0] = c->gh[0].p;
-  a.gh[1] = c->gh[1].p;
   a.gh_bytes = c->opts.use_quantized_grad ? 2 : 8;
   a.tile_left = c->tiles.p;
   a.feat_split = c->feat_split.p;
@@ -655,12 +667,12 @@ static void enqueue_tree(Ctx* c, int k, bool score_update, double lr) {
   }
   const bool renew = quant && c->opts.quant_train_renew_leaf;
   if (renew)
-    hipLaunchKernelGGL(k_renew_sum, dim3(64, c->L), dim3(256), 0, s, c->st.p, c->leaves.p, c->idx[0].p, c->idx[1].p,
+    hipLaunchKernelGGL(k_renew_sum, dim3(64, c->L), dim3(256), 0, s, c->st.p, c->leaves.p, c->idx[0].p, c->idx[1].p, c->idx[2].p,
                        gk, hk, c->it.p + k, c->renew.p);
   hipLaunchKernelGGL(k_leaf_values, dim3((c->L + 255) / 256), dim3(256), 0, s, c->st.p, c->leaves.p, c->it.p + k,
                      c->renew.p, renew ? 1 : 0, c->sp, lr, c->L, c->leafval.p + (size_t)k * c->L);
   if (score_update)
-    hipLaunchKernelGGL(k_score, dim3(64, c->L), dim3(256), 0, s, c->st.p, c->leaves.p, c->idx[0].p, c->idx[1].p,
+    hipLaunchKernelGGL(k_score, dim3(64, c->L), dim3(256), 0, s, c->st.p, c->leaves.p, c->idx[0].p, c->idx[1].p, c->idx[2].p,
                        c->leafval.p + (size_t)k * c->L, c->score.p + (size_t)k * N);
 }
 
@@ -1233,7 +1245,7 @@ lgbmhip_status lgbmhip_leaf_of_row(lgbmhip_ctx* ctx, int32_t* out) {
     DevBuf<int> d;
     d.alloc(c->N);
     hipLaunchKernelGGL(k_leaf_of_row, dim3(64, c->L), dim3(256), 0, c->stream, c->st.p, c->leaves.p, c->idx[0].p,
-                       c->idx[1].p, d.p);
+                       c->idx[1].p, c->idx[2].p, d.p);
     HIP_CHECK(hipMemcpyAsync(out, d.p, c->N * 4, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
@@ -1324,7 +1336,7 @@ static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc
   if (m < 0 || m > N) throw Error(LGBMHIP_INVALID_DATA, "row count out of range");
   for (int64_t i = 0; i < m; ++i)
     if (rows[i] >= (uint64_t)N) throw Error(LGBMHIP_INVALID_DATA, "row index out of range");
-  // pack (g, h) in the row order of `rows` into gh buffer 0 positions 0..m-1
+  // pack (g, h) in the row order of `rows` into gh buffer 1 positions 0..m-1
   std::vector<uint8_t> pk((size_t)std::max<int64_t>(m, 1) * (quant ? 2 : 8));
   unsigned gbits = 0, hbits = 0;
   for (int64_t i = 0; i < m; ++i) {
@@ -1345,14 +1357,18 @@ static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc
     }
   }
   if (m > 0) {
-    HIP_CHECK(hipMemcpyAsync(c->idx[0].p, rows, m * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(hipMemcpyAsync(c->gh[0].p, pk.data(), pk.size(), hipMemcpyHostToDevice, c->stream));
+    // the rows' bins gathered into positions 0..m-1 of ordered buffer 1
+    HIP_CHECK(hipMemcpyAsync(c->idx[1].p, rows, m * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->gh[1].p, pk.data(), pk.size(), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_group_rows, dim3(grid_for(m * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
+                       c->stream, (const unsigned*)c->idx[1].p, (long long)m, c->bins_rm.p, c->S, c->F, c->hist_fb,
+                       c->hist_ng, (long long)N, c->grp[1].p);
   }
   c->acc.zero(c->stream);
   HistArgs a = hist_args(c, quant, 0, 1);
   a.d_begin = 0;
   a.d_count = m;
-  a.d_buf = 0;
+  a.d_buf = 1;
   a.d_shift_g = fix_shift(m, gbits);
   a.d_shift_h = fix_shift(m, hbits);
   if (m > 0) launch_hist(a, quant, c->hist_tpr, c->hist_chunks * c->hist_ng, c->hist_lds, c->stream);

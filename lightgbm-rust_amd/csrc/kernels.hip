@@ -539,11 +539,19 @@ __global__ void k_discretize(long long n, const float* __restrict__ g, const flo
 // each block keeps its feature group's [FB][B][2] int64 histogram in LDS
 // (ds_add_u64), then adds its nonzero bins into the leaf accumulator with
 // global 64-bit atomics (exact, so the result is order-free).
+//
+// Ordered bins: every leaf's rows sit contiguously in one of three buffers
+// (0 holds the root in row order; a leaf in buffer 1 splits into buffer 2 and
+// vice versa), each feature group as its own [N][FB]-byte array, and the
+// partition pass moves each row's bins with its index and gradients. A
+// histogram pass therefore streams its segment with no index indirection and
+// reads exactly its group's bytes.
+__device__ __forceinline__ int child_buf(int buf) { return buf == 1 ? 2 : 1; }
+template <typename T>
+__device__ __forceinline__ T sel3(T a0, T a1, T a2, int b) { return b == 0 ? a0 : (b == 1 ? a1 : a2); }
 struct HistArgs {
-  const uint8_t* bins;     // row-major [N][S]
-  int S;                   // row stride in bytes
-  const unsigned* idx[2];
-  const void* gh[2];       // float2 (plain) or char2 (quantized)
+  const uint8_t* grp[3];   // ordered bins per buffer, group-major [NG][N][FB]
+  const void* gh[3];       // float2 (plain) or char2 (quantized), per buffer
   StepDev* st;
   LeafDev* leaves;
   const FeatDev* feats;
@@ -583,7 +591,7 @@ __device__ __forceinline__ bool hist_segment(const HistArgs& a, long long& begin
   }
   const long long pb = st->seg_begin, pc = st->seg_count;
   const long long lc = (long long)st->lcur, rc = pc - lc;
-  const int nb = 1 - st->seg_buf, depth = st->seg_depth + 1;
+  const int nb = child_buf(st->seg_buf), depth = st->seg_depth + 1;
   const int cur = st->cur_leaf, nl = st->new_leaf;
   const bool skip = (a.p.max_depth > 0 && depth >= a.p.max_depth) ||
                     (rc < 2LL * a.p.min_data_in_leaf && lc < 2LL * a.p.min_data_in_leaf);
@@ -679,22 +687,10 @@ __global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
   const int rpp = 256 / TPR;
   // select, not index: a runtime index into a kernel-argument array makes
   // the compiler copy the arguments to scratch memory
-  const unsigned* __restrict__ idx = (buf ? a.idx[1] : a.idx[0]) + begin;
-  const float2* __restrict__ gh = (const float2*)(buf ? a.gh[1] : a.gh[0]) + begin;
+  const uint8_t* __restrict__ gb =
+      sel3(a.grp[0], a.grp[1], a.grp[2], buf) + ((size_t)fg * a.N + begin) * a.FB + j * 4;
+  const float2* __restrict__ gh = (const float2*)sel3(a.gh[0], a.gh[1], a.gh[2], buf) + begin;
   const bool active = j * 4 < FBg;
-  // software pipeline: the row indices and gradients of the next pass are
-  // in flight while this pass gathers its bins and adds into LDS, so each
-  // pass waits on one memory round trip instead of two
-  unsigned rn[kHistUnroll];
-  float2 vn[kHistUnroll];
-#pragma unroll
-  for (int u = 0; u < kHistUnroll; ++u) {
-    long long p = r0 + u * rpp + sub;
-    if (active && p < r1) {
-      rn[u] = idx[p];
-      vn[u] = gh[p];
-    }
-  }
   for (long long base = r0; base < r1; base += kHistUnroll * rpp) {
     unsigned w[kHistUnroll];
     long long gi[kHistUnroll], hi[kHistUnroll];
@@ -704,18 +700,10 @@ __global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
       long long p = base + u * rpp + sub;
       ok[u] = active && p < r1;
       if (ok[u]) {
-        w[u] = *(const unsigned*)(a.bins + (size_t)rn[u] * a.S + f0 + j * 4);
-        gi[u] = to_fixed(vn[u].x, sg);
-        hi[u] = to_fixed(vn[u].y, sh);
-      }
-    }
-    const long long nbase = base + kHistUnroll * rpp;
-#pragma unroll
-    for (int u = 0; u < kHistUnroll; ++u) {
-      long long p = nbase + u * rpp + sub;
-      if (active && p < r1) {
-        rn[u] = idx[p];
-        vn[u] = gh[p];
+        const float2 v = gh[p];
+        w[u] = *(const unsigned*)(gb + (size_t)p * a.FB);
+        gi[u] = to_fixed(v.x, sg);
+        hi[u] = to_fixed(v.y, sh);
       }
     }
 #pragma unroll
@@ -773,19 +761,10 @@ __global__ void __launch_bounds__(256) k_hist_int(HistArgs a) {
   }
   __syncthreads();
   const int rpp = 256 / TPR;
-  const unsigned* __restrict__ idx = (buf ? a.idx[1] : a.idx[0]) + begin;
-  const char2* __restrict__ q = (const char2*)(buf ? a.gh[1] : a.gh[0]) + begin;
+  const uint8_t* __restrict__ gb =
+      sel3(a.grp[0], a.grp[1], a.grp[2], buf) + ((size_t)fg * a.N + begin) * a.FB + j * 4;
+  const char2* __restrict__ q = (const char2*)sel3(a.gh[0], a.gh[1], a.gh[2], buf) + begin;
   const bool active = j * 4 < FBg;
-  unsigned rn[kHistUnroll];
-  char2 vn[kHistUnroll];
-#pragma unroll
-  for (int u = 0; u < kHistUnroll; ++u) {
-    long long p = r0 + u * rpp + sub;
-    if (active && p < r1) {
-      rn[u] = idx[p];
-      vn[u] = q[p];
-    }
-  }
   for (long long base = r0; base < r1; base += kHistUnroll * rpp) {
     unsigned w[kHistUnroll];
     int gq[kHistUnroll], hq[kHistUnroll];
@@ -795,18 +774,10 @@ __global__ void __launch_bounds__(256) k_hist_int(HistArgs a) {
       long long p = base + u * rpp + sub;
       ok[u] = active && p < r1;
       if (ok[u]) {
-        w[u] = *(const unsigned*)(a.bins + (size_t)rn[u] * a.S + f0 + j * 4);
-        gq[u] = vn[u].x;
-        hq[u] = (int)(uint8_t)vn[u].y;
-      }
-    }
-    const long long nbase = base + kHistUnroll * rpp;
-#pragma unroll
-    for (int u = 0; u < kHistUnroll; ++u) {
-      long long p = nbase + u * rpp + sub;
-      if (active && p < r1) {
-        rn[u] = idx[p];
-        vn[u] = q[p];
+        const char2 v = q[p];
+        w[u] = *(const unsigned*)(gb + (size_t)p * a.FB);
+        gq[u] = v.x;
+        hq[u] = (int)(uint8_t)v.y;
       }
     }
 #pragma unroll
@@ -1517,10 +1488,11 @@ struct PartArgs {
   StepDev* st;
   LeafDev* leaves;
   const FeatDev* feats;
-  const uint8_t* bins_cm;   // feature-major [F][N]
+  uint8_t* grp[3];          // ordered bins per buffer, group-major [NG][N][FB]
+  int FB, NG;               // bytes (features) per group, groups
   long long N;
-  unsigned* idx[2];
-  void* gh[2];
+  unsigned* idx[3];
+  void* gh[3];
   int gh_bytes;             // 8 (float2) or 2 (char2)
   int* tile_left;           // per-tile left counts
   const SplitRec* feat_split;  // [2][F] searches of the last step
@@ -1536,7 +1508,7 @@ struct PartArgs {
 };
 
 constexpr int kPartThreads = 256;
-constexpr int kPartRowsPerThread = 8;
+constexpr int kPartRowsPerThread = 4;
 constexpr int kPartTile = kPartThreads * kPartRowsPerThread;
 
 // learner/mod.rs:792-810 goes_left / goes_left_categorical on a bin.
@@ -1708,75 +1680,157 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(PartArgs a) {
     }
   }
   const SplitRec& s = chosen;
-  const int src = lbuf;
-  const unsigned* idx = (src ? a.idx[1] : a.idx[0]) + begin;
-  unsigned* oidx = (src ? a.idx[0] : a.idx[1]) + begin;
-  const void* igh = src ? a.gh[1] : a.gh[0];
-  void* ogh = src ? a.gh[0] : a.gh[1];
-  const uint8_t* col = a.bins_cm + (size_t)s.feature * a.N;
+  const int src = lbuf, dst = child_buf(lbuf);   // buffer 0 is never a destination
+  const int FB = a.FB, gb = a.gh_bytes;
+  const size_t gstride = (size_t)a.N * FB;       // bytes of one feature group's array
+  const unsigned* idx = sel3(a.idx[0], a.idx[1], a.idx[2], src) + begin;
+  unsigned* oidx = sel3(a.idx[0], a.idx[1], a.idx[2], dst) + begin;
+  const uint8_t* igh = (const uint8_t*)sel3(a.gh[0], a.gh[1], a.gh[2], src) + (size_t)begin * gb;
+  uint8_t* ogh = (uint8_t*)sel3(a.gh[0], a.gh[1], a.gh[2], dst) + (size_t)begin * gb;
+  const uint8_t* gsrc = sel3(a.grp[0], a.grp[1], a.grp[2], src) + (size_t)begin * FB;
+  uint8_t* gdst = sel3(a.grp[0], a.grp[1], a.grp[2], dst) + (size_t)begin * FB;
+  const uint8_t* scol = gsrc + (size_t)(s.feature / FB) * gstride + (s.feature % FB);
+  const int words = FB / 4, tw = a.NG * words;   // 4-byte words of one row over all groups
   const int ntiles = (int)((count + kPartTile - 1) / kPartTile);
-  __shared__ int wpre[kPartThreads / 64];
-  __shared__ unsigned long long slot[2];
+  constexpr int R = kPartRowsPerThread;
+  constexpr int CW = 8;                          // row words moved per round
+  __shared__ int cnt[2][R][NW];
+  __shared__ long long slot[3];
+  const int wave = tid >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const long long t0 = (long long)t * kPartTile;
-    const long long p0 = t0 + tid * kPartRowsPerThread;
-    const int tile_rows = (int)min((long long)kPartTile, count - t0);
-    unsigned rows[kPartRowsPerThread];
-    bool left[kPartRowsPerThread];
-    int c = 0;
+    // one round of independent loads: index, gradients, the split feature's
+    // bin and the first words of every row (lanes take consecutive rows)
+    bool valid[R], left[R];
+    unsigned rid[R];
+    uint2 gv[R];
+    unsigned v[R][CW];
 #pragma unroll
-    for (int u = 0; u < kPartRowsPerThread; ++u) {
-      const long long p = p0 + u;
+    for (int u = 0; u < R; ++u) {
+      const long long p = t0 + u * kPartThreads + tid;
+      valid[u] = p < count;
       left[u] = false;
-      if (p < count) {
-        rows[u] = idx[p];
-        left[u] = goes_left(col[rows[u]], fd, s);
-        c += left[u] ? 1 : 0;
+      if (valid[u]) {
+        rid[u] = idx[p];
+        if (gb == 8) gv[u] = ((const uint2*)igh)[p];
+        else gv[u].x = ((const unsigned short*)igh)[p];
+        left[u] = goes_left(scol[(size_t)p * FB], fd, s);
+#pragma unroll
+        for (int i = 0; i < CW; ++i)
+          if (i < tw) v[u][i] = *(const unsigned*)(gsrc + (size_t)(i / words) * gstride + (size_t)p * FB + 4 * (i % words));
       }
     }
-    int x = c;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) wpre[tid >> 6] = x;
-    __syncthreads();
-    int wbase = 0, total = 0;
-    for (int w = 0; w < kPartThreads / 64; ++w) {
-      if (w < (tid >> 6)) wbase += wpre[w];
-      total += wpre[w];
-    }
-    const int excl_left = wbase + x - c;
-    if (tid == 0) {
-      slot[0] = atomicAdd(&st->lcur, (unsigned long long)total);
-      slot[1] = atomicAdd(&st->rcur, (unsigned long long)(tile_rows - total));
-    }
-    __syncthreads();
-    const long long nr = tile_rows - total;
-    long long lpos = (long long)slot[0] + excl_left;
-    long long rpos = count - (long long)slot[1] - nr + ((long long)tid * kPartRowsPerThread - excl_left);
+    // ranks inside each 64-row slice, slice counts to LDS
+    int rk[R];
 #pragma unroll
-    for (int u = 0; u < kPartRowsPerThread; ++u) {
-      const long long p = p0 + u;
-      if (p >= count) break;
-      const long long d = left[u] ? lpos++ : rpos++;
-      oidx[d] = rows[u];
-      if (a.gh_bytes == 8) ((float2*)ogh + begin)[d] = ((const float2*)igh + begin)[p];
-      else ((char2*)ogh + begin)[d] = ((const char2*)igh + begin)[p];
+    for (int u = 0; u < R; ++u) {
+      const unsigned long long lm = __ballot(left[u]);
+      const unsigned long long rm = __ballot(valid[u] && !left[u]);
+      rk[u] = (int)__popcll((left[u] ? lm : rm) & below);
+      if (lane == 0) {
+        cnt[0][u][wave] = (int)__popcll(lm);
+        cnt[1][u][wave] = (int)__popcll(rm);
+      }
     }
     __syncthreads();
+    if (tid == 0) {
+      int tl = 0, tr = 0;
+      for (int u = 0; u < R; ++u)
+        for (int w = 0; w < NW; ++w) {
+          tl += cnt[0][u][w];
+          tr += cnt[1][u][w];
+        }
+      slot[0] = (long long)atomicAdd(&st->lcur, (unsigned long long)tl);
+      slot[1] = (long long)atomicAdd(&st->rcur, (unsigned long long)tr);
+      slot[2] = tr;
+    }
+    __syncthreads();
+    // destinations: left rows fill from the front, right rows from the back;
+    // inside a tile the order is slice by slice, so stores of a slice are
+    // contiguous
+    long long d[R];
+    int pl = 0, pr = 0;
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      int bl = pl, br = pr;
+      for (int w = 0; w < NW; ++w) {
+        if (w < wave) {
+          bl += cnt[0][u][w];
+          br += cnt[1][u][w];
+        }
+        pl += cnt[0][u][w];
+        pr += cnt[1][u][w];
+      }
+      d[u] = left[u] ? slot[0] + bl + rk[u] : count - slot[1] - slot[2] + br + rk[u];
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      if (!valid[u]) continue;
+      oidx[d[u]] = rid[u];
+      if (gb == 8) ((uint2*)ogh)[d[u]] = gv[u];
+      else ((unsigned short*)ogh)[d[u]] = (unsigned short)gv[u].x;
+#pragma unroll
+      for (int i = 0; i < CW; ++i)
+        if (i < tw) *(unsigned*)(gdst + (size_t)(i / words) * gstride + (size_t)d[u] * FB + 4 * (i % words)) = v[u][i];
+    }
+    // rows wider than CW words: the rest, CW words per round
+    for (int k0 = CW; k0 < tw; k0 += CW) {
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        if (!valid[u]) continue;
+        const long long p = t0 + u * kPartThreads + tid;
+#pragma unroll
+        for (int i = 0; i < CW; ++i) {
+          const int k = k0 + i;
+          if (k < tw) v[u][i] = *(const unsigned*)(gsrc + (size_t)(k / words) * gstride + (size_t)p * FB + 4 * (k % words));
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        if (!valid[u]) continue;
+#pragma unroll
+        for (int i = 0; i < CW; ++i) {
+          const int k = k0 + i;
+          if (k < tw) *(unsigned*)(gdst + (size_t)(k / words) * gstride + (size_t)d[u] * FB + 4 * (k % words)) = v[u][i];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Rows of the row-major bins (stride S) into the group-major layout of the
+// ordered buffers: out[g][i][0..FB) = bins of features g*FB.. of row
+// rows[i] (or row i when rows is null), zero past the last feature.
+__global__ void k_group_rows(const unsigned* rows, long long m, const uint8_t* __restrict__ rm, int S, int F, int FB,
+                             int NG, long long N, uint8_t* __restrict__ out) {
+  const int words = FB / 4;
+  const long long total = m * NG * words;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
+    const long long i = e / (NG * words);
+    const int k = (int)(e % (NG * words));
+    const int g = k / words, w = k % words;
+    const long long r = rows ? (long long)rows[i] : i;
+    unsigned v = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int f = g * FB + w * 4 + b;
+      if (f < F) v |= (unsigned)rm[r * S + f] << (8 * b);
+    }
+    *(unsigned*)(out + (size_t)g * N * FB + (size_t)i * FB + 4 * w) = v;
   }
 }
 
 // ---------------------------------------------------------------- tree end
 // Quantized form, learner/mod.rs:736-747: each leaf's output from its float
 // gradient sums (exact fixed-point per leaf, then refit_leaf_output).
-__global__ void k_renew_sum(const StepDev* st, const LeafDev* leaves, const unsigned* idx0, const unsigned* idx1, const float* g,
+__global__ void k_renew_sum(const StepDev* st, const LeafDev* leaves, const unsigned* idx0, const unsigned* idx1, const unsigned* idx2, const float* g,
                             const float* h, const IterDev* it, long long* sums /* [L][2] */) {
   const int l = blockIdx.y;
   if (l >= st->num_leaves) return;
   const LeafDev& L = leaves[l];
-  const unsigned* idx = (L.buf ? idx1 : idx0) + L.begin;
+  const unsigned* idx = sel3(idx0, idx1, idx2, L.buf) + L.begin;
   const int sg = fix_shift(L.count, it->max_abs_g_bits), sh = fix_shift(L.count, it->max_abs_h_bits);
   long long ag = 0, ah = 0;
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < L.count; p += (long long)gridDim.x * blockDim.x) {
@@ -1811,23 +1865,23 @@ __global__ void k_leaf_values(const StepDev* st, LeafDev* leaves, const IterDev*
 }
 
 // Score update (partition.rs:178-191): each row of leaf l gets out[l].
-__global__ void k_score(const StepDev* st, const LeafDev* leaves, const unsigned* idx0, const unsigned* idx1, const double* val,
+__global__ void k_score(const StepDev* st, const LeafDev* leaves, const unsigned* idx0, const unsigned* idx1, const unsigned* idx2, const double* val,
                         double* score) {
   const int l = blockIdx.y;
   if (st->num_leaves <= 1 || l >= st->num_leaves) return;
   const LeafDev& L = leaves[l];
-  const unsigned* idx = (L.buf ? idx1 : idx0) + L.begin;
+  const unsigned* idx = sel3(idx0, idx1, idx2, L.buf) + L.begin;
   const double v = val[l];
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < L.count; p += (long long)gridDim.x * blockDim.x)
     score[idx[p]] += v;
 }
 
 // Leaf of each row for the last tree (partition state).
-__global__ void k_leaf_of_row(const StepDev* st, const LeafDev* leaves, const unsigned* idx0, const unsigned* idx1, int* out) {
+__global__ void k_leaf_of_row(const StepDev* st, const LeafDev* leaves, const unsigned* idx0, const unsigned* idx1, const unsigned* idx2, int* out) {
   const int l = blockIdx.y;
   if (l >= st->num_leaves) return;
   const LeafDev& L = leaves[l];
-  const unsigned* idx = (L.buf ? idx1 : idx0) + L.begin;
+  const unsigned* idx = sel3(idx0, idx1, idx2, L.buf) + L.begin;
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < L.count; p += (long long)gridDim.x * blockDim.x)
     out[idx[p]] = l;
 }
