@@ -83,24 +83,63 @@ struct LeafDev {
   long long int_sum;
 };
 
-// The tree-growth state machine, one record per context.
+// Leaves split per growth round (the tree grows in rounds; see grow.hip).
+constexpr int kMaxBatch = 32;
+
+// The tree-growth state, one record per context. Leaves live in slots; the
+// tree's own leaf numbering (the reference's, in split order) is the
+// position of a slot in the current position array.
 struct StepDev {
-  int num_leaves;   // leaves of the tree so far
-  int done;         // no further split this tree
-  int cur_leaf;     // leaf split in this step
-  int new_leaf;
-  int smaller, larger;  // leaves whose histograms / searches run next (larger = -1 at the root)
-  int skip;         // before_find_best_split said no
-  int is_root;
-  long long left_total;
-  int hist_shift_g, hist_shift_h;  // fixed-point exponents of the current histogram build
-  int num_splits;
-  int counter;      // last-block-done counter
-  int counter2;
+  int done;          // the tree is complete
+  int nseq;          // leaves of the tree (positions)
+  int npending;      // leaves split ahead of order, not yet replayed
+  int nslots;        // leaf slots in use
+  int num_splits;    // splits replayed so far (log entries)
+  int parity;        // current position array (0 / 1)
+  int search_base;   // first slot searched by the last split pass ...
+  int nsearched;     // ... and how many (feat_split rows 0..nsearched)
+  int rounds;        // growth rounds run on this tree
+  int hist_items;    // histograms built (segments), root included
+  long long hist_rows;   // rows summed into histograms, root included
   double root_output;  // the root's output before any split (learner/mod.rs:647)
-  long long seg_begin, seg_count;  // the segment of the leaf being split
-  int seg_buf, seg_depth;
-  unsigned long long lcur, rcur;   // rows placed left / right by the partition pass
+  unsigned long long lcur[kMaxBatch], rcur[kMaxBatch];   // rows placed left / right per item
+};
+
+// One leaf of the tree in sequential numbering: its slot and best split
+// (gain, feature); when the leaf was split ahead of order (spec >= 0), its
+// child slots are spec and spec + 1, with their best splits.
+struct PosRec {
+  double gain;
+  double cgain[2];
+  int feat, slot, spec;
+  int cfeat[2];
+  int pad;
+};
+
+// A split of the current round.
+struct PlanItem {
+  SplitRec rec;
+  long long begin, count;   // the parent's segment
+  int parent, lslot;        // slots: parent, left child (right = lslot + 1)
+  int buf, depth;           // the parent's buffer and depth
+  int tile0, pad;           // first partition tile
+};
+
+// A split replayed at the start of a round, in sequential order.
+struct PlanRep {
+  int pos, parent, lslot, pad;
+};
+
+// The plan of one round, written by the partition pass's block 0.
+struct PlanDev {
+  int done;         // the replay completed the tree
+  int nwork;        // splits this round
+  int total_tiles;
+  int nseq, npending, nslots, num_splits;   // state after this round
+  int nrep;         // replayed splits (PlanRep entries)
+  int complete;     // no child of this round can be split: skip their searches
+  int pad;
+  PlanItem item[kMaxBatch];
 };
 
 // Per-iteration scalars written by device reductions.

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -19,6 +20,7 @@
 
 #include "engine.h"
 #include "kernels.hip"
+#include "grow.hip"
 
 namespace lgbmhip {
 
@@ -29,10 +31,6 @@ namespace lgbmhip {
       throw Error(LGBMHIP_INTERNAL, std::string(#x) + ": " + hipGetErrorString(e_));             \
   } while (0)
 
-void launch_hist(const HistArgs& a, bool quant, int tpr, int grid, size_t lds, hipStream_t s);
-void set_hist_lds_limits(size_t bytes);
-void launch_split(const SplitArgs& a, hipStream_t s);
-void launch_part(const PartArgs& a, int grid, hipStream_t s);
 
 // mt19937.rs:14-88 (the quantizer's generators).
 struct Mt {
@@ -209,15 +207,17 @@ struct Ctx {
   DevBuf<long long> acc;
   DevBuf<double> pool;
   DevBuf<long long> ipool;
-  DevBuf<SplitRec> feat_split, best, cur, log;
-  DevBuf<double> best_gain;
-  DevBuf<int> best_feat;
+  // growth state: leaf slots (MS of them), positions, the round's plan
+  int MS = 0;
+  DevBuf<SplitRec> feat_split, best, log;
   DevBuf<uint8_t> spl;
   DevBuf<StepDev> st;
+  DevBuf<PlanDev> plan;
+  DevBuf<PlanRep> reps;
+  DevBuf<PosRec> pos;
   DevBuf<LeafDev> leaves;
   DevBuf<IterDev> it;
-  DevBuf<int> tiles;
-  DevBuf<double> leafval;
+  DevBuf<double> leafval, leafraw;
   DevBuf<long long> renew;
   DevBuf<double> grv, hrv;
   DevBuf<int> starts;
@@ -231,10 +231,13 @@ struct Ctx {
   int tune_part_grid = 0, tune_hist_chunks = 0, tune_hist_min_rows = 0, tune_hist_lds = 0;
   // pinned copies of the per-tree results
   std::vector<SplitRec*> h_log;
-  std::vector<LeafDev*> h_leaves;
+  std::vector<double*> h_raw;
   std::vector<StepDev*> h_st;
-  // graphs
-  std::vector<hipGraphExec_t> graphs;
+  // graphs: per class, one tree (init, root, rounds_main rounds, tail) and
+  // a continuation (rounds_cont rounds, tail) for trees that need more rounds
+  std::vector<hipGraphExec_t> graphs, graphs_cont;
+  int rounds_main = 24, rounds_cont = 8;
+  std::vector<int> last_rounds;   // rounds launched for each class's last tree
   // model
   std::vector<HostTree> trees;
   bool models_empty_init = true;
@@ -261,8 +264,9 @@ struct Ctx {
   ~Ctx() {
     for (auto e : pev) if (e) (void)hipEventDestroy(e);
     for (auto g : graphs) if (g) (void)hipGraphExecDestroy(g);
+    for (auto g : graphs_cont) if (g) (void)hipGraphExecDestroy(g);
     for (auto p : h_log) if (p) (void)hipHostFree(p);
-    for (auto p : h_leaves) if (p) (void)hipHostFree(p);
+    for (auto p : h_raw) if (p) (void)hipHostFree(p);
     for (auto p : h_st) if (p) (void)hipHostFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -435,6 +439,7 @@ static void setup_training(Ctx* c) {
   if (c->K < 1 || c->K > 16) throw Error(LGBMHIP_UNSUPPORTED, "num_class must be in [1, 16]");
   if (o.objective == 2) setup_rank(c);
   c->L = std::max(c->sp.num_leaves, 2);
+  if (c->L > 1024) throw Error(LGBMHIP_UNSUPPORTED, "num_leaves above 1024");
   if (quant) {
     for (auto& f : c->feats)
       if (f.bin_type == 1) throw Error(LGBMHIP_UNSUPPORTED, "quantized gradients with categorical features");
@@ -451,25 +456,31 @@ static void setup_training(Ctx* c) {
     c->idx[b].alloc(N);
     c->gh[b].alloc((size_t)N * (quant ? 2 : 8));
   }
-  c->acc.alloc((size_t)F * B * (quant ? 1 : 2));
+  // leaf slots: at most L - 1 replayed splits plus at most L splits made
+  // ahead of order and never replayed, two slots each, plus the root
+  c->MS = 4 * L;
+  const int MS = c->MS;
+  c->acc.alloc((size_t)kMaxBatch * F * B * (quant ? 1 : 2));
   c->acc.zero(c->stream);
-  if (quant) { c->ipool.alloc((size_t)L * F * B); c->pool.free(); }
-  else { c->pool.alloc((size_t)L * F * B * 2); c->ipool.free(); }
-  c->feat_split.alloc(2 * F);
-  c->best.alloc(L);
-  c->best_gain.alloc(L);
-  c->best_feat.alloc(L);
-  c->cur.alloc(1);
+  if (quant) { c->ipool.alloc((size_t)MS * F * B); c->pool.free(); }
+  else { c->pool.alloc((size_t)MS * F * B * 2); c->ipool.free(); }
+  c->feat_split.alloc((size_t)2 * kMaxBatch * F);
+  c->best.alloc(MS);
   c->log.alloc(L);
-  c->spl.alloc((size_t)L * F);
+  c->spl.alloc((size_t)MS * F);
   c->st.alloc(1);
   c->st.zero(c->stream);
-  c->leaves.alloc(L);
+  c->plan.alloc(1);
+  c->plan.zero(c->stream);
+  c->reps.alloc(L);
+  c->pos.alloc((size_t)2 * L);
+  c->leaves.alloc(MS);
   c->leaves.zero(c->stream);
   c->it.alloc(K);
   c->it.zero(c->stream);
-  c->tiles.alloc((N + part_tile() - 1) / part_tile() + 1);
   c->leafval.alloc((size_t)K * L);
+  c->leafraw.alloc((size_t)K * L);
+  set_part_lds_limit((size_t)L * sizeof(PosRec));
   c->renew.alloc((size_t)2 * L);
   c->renew.zero(c->stream);
   c->part_grid = (int)std::min<int64_t>((N + part_tile() - 1) / part_tile(),
@@ -518,18 +529,21 @@ static void setup_training(Ctx* c) {
     HIP_CHECK(hipGetLastError());
   }
   for (auto p : c->h_log) if (p) (void)hipHostFree(p);
-  for (auto p : c->h_leaves) if (p) (void)hipHostFree(p);
+  for (auto p : c->h_raw) if (p) (void)hipHostFree(p);
   for (auto p : c->h_st) if (p) (void)hipHostFree(p);
   c->h_log.assign(K, nullptr);
-  c->h_leaves.assign(K, nullptr);
+  c->h_raw.assign(K, nullptr);
   c->h_st.assign(K, nullptr);
   for (int k = 0; k < K; ++k) {
     HIP_CHECK(hipHostMalloc((void**)&c->h_log[k], sizeof(SplitRec) * L));
-    HIP_CHECK(hipHostMalloc((void**)&c->h_leaves[k], sizeof(LeafDev) * L));
+    HIP_CHECK(hipHostMalloc((void**)&c->h_raw[k], sizeof(double) * L));
     HIP_CHECK(hipHostMalloc((void**)&c->h_st[k], sizeof(StepDev)));
   }
   for (auto g : c->graphs) if (g) (void)hipGraphExecDestroy(g);
+  for (auto g : c->graphs_cont) if (g) (void)hipGraphExecDestroy(g);
   c->graphs.assign(K, nullptr);
+  c->graphs_cont.assign(K, nullptr);
+  c->last_rounds.assign(K, 0);
   c->trees.clear();
   c->iter = 0;
   c->pred_dirty = true;
@@ -544,7 +558,11 @@ static HistArgs hist_args(Ctx* c, bool quant, int k, int mode) {
     a.gh[b] = c->gh[b].p;
   }
   a.st = c->st.p;
+  a.plan = c->plan.p;
+  a.reps = c->reps.p;
   a.leaves = c->leaves.p;
+  a.log = c->log.p;
+  a.best = c->best.p;
   a.feats = c->d_feats.p;
   a.F = c->F;
   a.B = c->B;
@@ -556,20 +574,16 @@ static HistArgs hist_args(Ctx* c, bool quant, int k, int mode) {
   a.min_rows_per_chunk = c->hist_min_rows;
   a.acc = c->acc.p;
   a.it = c->it.p + k;
-  a.cur = c->cur.p;
-  a.log = c->log.p;
-  a.best = c->best.p;
-  a.best_gain = c->best_gain.p;
-  a.best_feat = c->best_feat.p;
   a.p = c->sp;
   a.quant = quant ? 1 : 0;
   a.mode = mode;
   return a;
 }
 
-static SplitArgs split_args(Ctx* c, int k) {
+static SplitArgs split_args(Ctx* c, int k, bool root) {
   SplitArgs a{};
   a.st = c->st.p;
+  a.plan = c->plan.p;
   a.leaves = c->leaves.p;
   a.feats = c->d_feats.p;
   a.p = c->sp;
@@ -580,20 +594,23 @@ static SplitArgs split_args(Ctx* c, int k) {
   a.pool = c->pool.p;
   a.ipool = c->ipool.p;
   a.feat_split = c->feat_split.p;
-  a.best = c->best.p;
   a.splittable = c->spl.p;
-  a.cur = c->cur.p;
-  a.log = c->log.p;
   a.it = c->it.p + k;
   a.quant = c->opts.use_quantized_grad != 0;
+  a.root = root ? 1 : 0;
   return a;
 }
 
-static PartArgs part_args(Ctx* c, int k) {
+static PartArgs part_args(Ctx* c) {
   PartArgs a{};
   a.st = c->st.p;
+  a.plan = c->plan.p;
+  a.reps = c->reps.p;
+  a.pos = c->pos.p;
   a.leaves = c->leaves.p;
   a.feats = c->d_feats.p;
+  a.best = c->best.p;
+  a.feat_split = c->feat_split.p;
   for (int b = 0; b < 3; ++b) {
     a.grp[b] = c->grp[b].p;
     a.idx[b] = c->idx[b].p;
@@ -603,32 +620,26 @@ static PartArgs part_args(Ctx* c, int k) {
   a.NG = c->hist_ng;
   a.N = c->N;
   a.gh_bytes = c->opts.use_quantized_grad ? 2 : 8;
-  a.tile_left = c->tiles.p;
-  a.feat_split = c->feat_split.p;
-  a.cur = c->cur.p;
-  a.best = c->best.p;
-  a.best_gain = c->best_gain.p;
-  a.best_feat = c->best_feat.p;
-  a.log = c->log.p;
-  a.p = c->sp;
   a.F = c->F;
   a.L = c->L;
-  a.it = c->it.p + k;
-  a.quant = c->opts.use_quantized_grad != 0;
   return a;
 }
 
-// The launch sequence of one tree for class k (gradients already in g/h and
-// max |g|, |h| in it[k]); `score_update` adds the shrunk leaf values.
+// Profiling events: per class, step 0 = the root's histogram and search,
+// step r + 1 = growth round r; kinds 0 histogram, 1 search, 2 partition.
+static int prof_steps(const Ctx* c) { return c->L + 2; }
 static hipEvent_t pevent(Ctx* c, int k, int step, int kind, int end) {
-  size_t i = ((((size_t)k * c->L + step) * 3 + kind) * 2 + end);
+  size_t i = ((((size_t)k * prof_steps(c) + step) * 3 + kind) * 2 + end);
   return c->pev[i];
 }
 static void prof_mark(Ctx* c, int k, int step, int kind, int end) {
-  if (c->profile) HIP_CHECK(hipEventRecord(pevent(c, k, step, kind, end), c->stream));
+  if (c->profile && step < prof_steps(c)) HIP_CHECK(hipEventRecord(pevent(c, k, step, kind, end), c->stream));
 }
 
-static void enqueue_tree(Ctx* c, int k, bool score_update, double lr) {
+// The first launches of a tree for class k (gradients already in g/h and
+// max |g|, |h| in it[k]): identity partition or quantization with the root
+// sums, the growth state, the root's histogram and search.
+static void enqueue_head(Ctx* c, int k) {
   hipStream_t s = c->stream;
   const int64_t N = c->N;
   const bool quant = c->opts.use_quantized_grad != 0;
@@ -638,25 +649,32 @@ static void enqueue_tree(Ctx* c, int k, bool score_update, double lr) {
   if (quant) {
     hipLaunchKernelGGL(k_discretize, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, c->grv.p, c->hrv.p,
                        c->opts.num_grad_quant_bins, c->opts.stochastic_rounding, 0, c->starts.p, c->call_ctr.p,
-                       c->it.p + k, c->idx[0].p, (char2*)c->gh[0].p, c->gq.p, c->hq.p, c->st.p, c->leaves.p,
-                       c->best.p, c->spl.p, c->F, c->L, c->best_gain.p, c->best_feat.p);
+                       c->it.p + k, c->idx[0].p, (char2*)c->gh[0].p, c->gq.p, c->hq.p);
     hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, c->call_ctr.p);
   } else {
     hipLaunchKernelGGL(k_tree_init, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, c->idx[0].p,
-                       (float2*)c->gh[0].p, c->it.p + k, c->st.p, c->leaves.p, c->best.p, c->spl.p, c->F, c->L,
-                       c->best_gain.p, c->best_feat.p);
+                       (float2*)c->gh[0].p, c->it.p + k);
   }
-  HistArgs ha_root = hist_args(c, quant, k, 2);
-  HistArgs ha = hist_args(c, quant, k, 0);
-  SplitArgs sa = split_args(c, k);
-  PartArgs pa = part_args(c, k);
+  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(256), 0, s, c->st.p, c->pos.p, c->leaves.p, c->spl.p, (long long)N,
+                     c->F);
   const int hgrid = c->hist_chunks * c->hist_ng;
   prof_mark(c, k, 0, 0, 0);
-  launch_hist(ha_root, quant, c->hist_tpr, hgrid, c->hist_lds, s);
+  launch_hist(hist_args(c, quant, k, 2), quant, c->hist_tpr, hgrid, c->hist_lds, s);
   prof_mark(c, k, 0, 0, 1);
-  launch_split(sa, s);
+  launch_split(split_args(c, k, true), s);
   prof_mark(c, k, 0, 1, 1);
-  for (int step = 1; step < c->L; ++step) {
+}
+
+// Growth rounds r0 .. r0 + n - 1 (a round after the tree is done returns at once).
+static void enqueue_rounds(Ctx* c, int k, int r0, int n) {
+  hipStream_t s = c->stream;
+  const bool quant = c->opts.use_quantized_grad != 0;
+  const HistArgs ha = hist_args(c, quant, k, 3);
+  const SplitArgs sa = split_args(c, k, false);
+  const PartArgs pa = part_args(c);
+  const int hgrid = c->hist_chunks * c->hist_ng;
+  for (int r = r0; r < r0 + n; ++r) {
+    const int step = r + 1;
     prof_mark(c, k, step, 2, 0);
     launch_part(pa, c->part_grid, s);
     prof_mark(c, k, step, 2, 1);
@@ -665,28 +683,88 @@ static void enqueue_tree(Ctx* c, int k, bool score_update, double lr) {
     launch_split(sa, s);
     prof_mark(c, k, step, 1, 1);
   }
+}
+
+// The tree's end: quantized leaf renewal, leaf values, score update. No-ops
+// while the tree is still growing.
+static void enqueue_tail(Ctx* c, int k, bool score_update, double lr) {
+  hipStream_t s = c->stream;
+  const int64_t N = c->N;
+  const bool quant = c->opts.use_quantized_grad != 0;
+  const float* gk = c->g.p + (size_t)k * N;
+  const float* hk = c->h.p + (size_t)k * N;
   const bool renew = quant && c->opts.quant_train_renew_leaf;
   if (renew)
-    hipLaunchKernelGGL(k_renew_sum, dim3(64, c->L), dim3(256), 0, s, c->st.p, c->leaves.p, c->idx[0].p, c->idx[1].p, c->idx[2].p,
-                       gk, hk, c->it.p + k, c->renew.p);
-  hipLaunchKernelGGL(k_leaf_values, dim3((c->L + 255) / 256), dim3(256), 0, s, c->st.p, c->leaves.p, c->it.p + k,
-                     c->renew.p, renew ? 1 : 0, c->sp, lr, c->L, c->leafval.p + (size_t)k * c->L);
+    hipLaunchKernelGGL(k_renew_sum, dim3(64, c->L), dim3(256), 0, s, c->st.p, c->pos.p, c->leaves.p, c->L,
+                       c->idx[0].p, c->idx[1].p, c->idx[2].p, gk, hk, c->it.p + k, c->renew.p);
+  hipLaunchKernelGGL(k_leaf_values, dim3((c->L + 255) / 256), dim3(256), 0, s, c->st.p, c->pos.p, c->leaves.p,
+                     c->it.p + k, c->renew.p, renew ? 1 : 0, c->sp, lr, c->L, c->leafval.p + (size_t)k * c->L,
+                     c->leafraw.p + (size_t)k * c->L);
   if (score_update)
-    hipLaunchKernelGGL(k_score, dim3(64, c->L), dim3(256), 0, s, c->st.p, c->leaves.p, c->idx[0].p, c->idx[1].p, c->idx[2].p,
-                       c->leafval.p + (size_t)k * c->L, c->score.p + (size_t)k * N);
+    hipLaunchKernelGGL(k_score, dim3(64, c->L), dim3(256), 0, s, c->st.p, c->pos.p, c->leaves.p, c->L, c->idx[0].p,
+                       c->idx[1].p, c->idx[2].p, c->leafval.p + (size_t)k * c->L, c->score.p + (size_t)k * N);
 }
 
-static void copy_tree_results(Ctx* c, int k) {
+static hipGraphExec_t capture(Ctx* c, const std::function<void()>& body) {
+  hipGraph_t g;
+  HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  body();
+  HIP_CHECK(hipStreamEndCapture(c->stream, &g));
+  hipGraphExec_t e = nullptr;
+  HIP_CHECK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(g);
+  return e;
+}
+
+// One tree for class k, to completion: the main launch (graph or stream),
+// then continuation rounds while the device says the tree still grows.
+// Leaves the tree's log, raw leaf outputs and state in the pinned copies.
+static void run_tree(Ctx* c, int k, bool score_update, double lr) {
+  const bool graph = c->opts.use_graph && !c->profile;
+  if (graph) {
+    if (!c->graphs[k])
+      c->graphs[k] = capture(c, [&] {
+        enqueue_head(c, k);
+        enqueue_rounds(c, k, 0, c->rounds_main);
+        enqueue_tail(c, k, score_update, lr);
+      });
+    HIP_CHECK(hipGraphLaunch(c->graphs[k], c->stream));
+  } else {
+    enqueue_head(c, k);
+    enqueue_rounds(c, k, 0, c->rounds_main);
+    enqueue_tail(c, k, score_update, lr);
+  }
+  HIP_CHECK(hipGetLastError());
+  int launched = c->rounds_main;
+  for (;;) {
+    HIP_CHECK(hipMemcpyAsync(c->h_st[k], c->st.p, sizeof(StepDev), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (c->h_st[k]->done) break;
+    // every round replays or splits at least one leaf: L + 1 rounds finish any tree
+    if (launched > c->L + 1) throw Error(LGBMHIP_INTERNAL, "tree growth did not finish");
+    if (graph) {
+      if (!c->graphs_cont[k])
+        c->graphs_cont[k] = capture(c, [&] {
+          enqueue_rounds(c, k, 0, c->rounds_cont);
+          enqueue_tail(c, k, score_update, lr);
+        });
+      HIP_CHECK(hipGraphLaunch(c->graphs_cont[k], c->stream));
+    } else {
+      enqueue_rounds(c, k, launched, c->rounds_cont);
+      enqueue_tail(c, k, score_update, lr);
+    }
+    HIP_CHECK(hipGetLastError());
+    launched += c->rounds_cont;
+  }
+  c->last_rounds[k] = launched;
   HIP_CHECK(hipMemcpyAsync(c->h_log[k], c->log.p, sizeof(SplitRec) * c->L, hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->h_leaves[k], c->leaves.p, sizeof(LeafDev) * c->L, hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->h_st[k], c->st.p, sizeof(StepDev), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->h_raw[k], c->leafraw.p + (size_t)k * c->L, sizeof(double) * c->L,
+                           hipMemcpyDeviceToHost, c->stream));
 }
 
-// Replay the split log into Tree arrays (learner/mod.rs:1105-1225, tree.rs:254-297).
 static HostTree build_tree(Ctx* c, int k) {
   HostTree t;
   const StepDev& st = *c->h_st[k];
-  const LeafDev* lv = c->h_leaves[k];
   if (st.num_splits == 0) {
     t.leaf_value[0] = st.root_output;
     t.leaf_count[0] = (int)c->N;
@@ -728,27 +806,10 @@ static HostTree build_tree(Ctx* c, int k) {
     }
   }
   if (c->opts.use_quantized_grad && c->opts.quant_train_renew_leaf)
-    for (int l = 0; l < t.num_leaves; ++l) t.leaf_value[l] = lv[l].output;
+    for (int l = 0; l < t.num_leaves; ++l) t.leaf_value[l] = c->h_raw[k][l];
   return t;
 }
 
-static void ensure_graph(Ctx* c, int k, bool score_update, double lr) {
-  if (!c->opts.use_graph || c->profile) {
-    enqueue_tree(c, k, score_update, lr);
-    return;
-  }
-  if (!c->graphs[k]) {
-    hipGraph_t g;
-    HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_tree(c, k, score_update, lr);
-    HIP_CHECK(hipStreamEndCapture(c->stream, &g));
-    HIP_CHECK(hipGraphInstantiate(&c->graphs[k], g, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(g);
-  }
-  HIP_CHECK(hipGraphLaunch(c->graphs[k], c->stream));
-}
-
-// objective/binary.rs:129-154 and multiclass.rs:156-158 (sequential sums).
 static double boost_from_score(Ctx* c, int k) {
   const auto& lab = c->label;
   const int64_t n = (int64_t)lab.size();
@@ -828,49 +889,37 @@ static void enqueue_gradients(Ctx* c) {
 
 // Algorithmic bytes of one histogram build over m rows (row-index read
 // skipped at the root, whose rows are contiguous): m * (F + idx + gh) + F*B*out.
-static double hist_bytes(Ctx* c, int64_t m, bool root) {
+// Algorithmic bytes of histogram builds over `rows` rows in `items`
+// segments: each row's bins and gradients once (ordered buffers: no index
+// read), each segment's histogram written once.
+static double hist_bytes(Ctx* c, double rows, double items) {
   const bool quant = c->opts.use_quantized_grad != 0;
-  return (double)m * ((double)c->F + (root ? 0.0 : 4.0) + (quant ? 2.0 : 8.0)) +
-         (double)c->F * c->B * (quant ? 8.0 : 16.0);
+  return rows * ((double)c->F + (quant ? 2.0 : 8.0)) + items * (double)c->F * c->B * (quant ? 8.0 : 16.0);
 }
 
 // Per-kind device time of the tree just grown (event to event on the
 // context's stream, so each figure includes that launch's dispatch gap).
-static void account_profile(Ctx* c, int k, const HostTree& t) {
+// Every histogram launch counts (one per round plus the root's, as a kernel
+// trace counts them).
+static void account_profile(Ctx* c, int k) {
   const StepDev& st = *c->h_st[k];
   auto el = [&](hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms, a, b));
     return (double)ms;
   };
-  // every launch counts (L histogram launches per tree, as a kernel trace
-  // counts them); rows and bytes only where the launch summed rows
   c->prof[0] += el(pevent(c, k, 0, 0, 0), pevent(c, k, 0, 0, 1));
   c->prof[1] += 1;
-  c->prof[2] += (double)c->N;
-  c->prof[3] += hist_bytes(c, c->N, true);
   c->prof[4] += el(pevent(c, k, 0, 0, 1), pevent(c, k, 0, 1, 1));
-  for (int step = 1; step < c->L; ++step) {
+  const int steps = std::min(c->last_rounds[k] + 1, prof_steps(c));
+  for (int step = 1; step < steps; ++step) {
     c->prof[0] += el(pevent(c, k, step, 2, 1), pevent(c, k, step, 0, 1));
     c->prof[1] += 1;
     c->prof[4] += el(pevent(c, k, step, 0, 1), pevent(c, k, step, 1, 1));
     c->prof[5] += el(pevent(c, k, step, 2, 0), pevent(c, k, step, 2, 1));
   }
-  std::vector<int> depth(c->L, 0);
-  for (int i = 0; i < st.num_splits; ++i) {
-    const SplitRec& r = c->h_log[k][i];
-    const int d = depth[r.leaf] + 1;
-    depth[r.leaf] = d;
-    depth[i + 1] = d;
-    const bool skip = (c->sp.max_depth > 0 && d >= c->sp.max_depth) ||
-                      (r.right_count < 2LL * c->sp.min_data_in_leaf && r.left_count < 2LL * c->sp.min_data_in_leaf);
-    const bool complete = i + 2 >= c->L;   // the tree is full: no histogram after the last split
-    if (skip || complete) continue;
-    const int64_t m = std::min(r.left_count, r.right_count);
-    c->prof[2] += (double)m;
-    c->prof[3] += hist_bytes(c, m, false);
-  }
-  (void)t;
+  c->prof[2] += (double)st.hist_rows;
+  c->prof[3] += hist_bytes(c, (double)st.hist_rows, (double)st.hist_items);
 }
 
 // boosting.rs:957-1139 for device=gpu.
@@ -899,9 +948,7 @@ static int train_iteration(Ctx* c) {
   HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   enqueue_gradients(c);
   for (int k = 0; k < K; ++k) {
-    ensure_graph(c, k, true, c->opts.learning_rate);
-    HIP_CHECK(hipGetLastError());
-    copy_tree_results(c, k);
+    run_tree(c, k, true, c->opts.learning_rate);
     if (c->opts.use_quantized_grad) c->n_calls++;
   }
   HIP_CHECK(hipEventRecord(c->ev1, c->stream));
@@ -917,7 +964,7 @@ static int train_iteration(Ctx* c) {
   }
   for (int k = 0; k < K; ++k) {
     HostTree t = build_tree(c, k);
-    if (c->profile) account_profile(c, k, t);
+    if (c->profile) account_profile(c, k);
     if (t.num_leaves > 1) {
       cont = true;
       t.shrink(c->opts.learning_rate);
@@ -951,9 +998,7 @@ static void train_tree(Ctx* c, const float* g, const float* h) {
                      (long long)N, c->it.p);
   if (c->opts.use_quantized_grad && c->n_calls + 1 > (int)c->starts.n)
     throw Error(LGBMHIP_UNSUPPORTED, "more quantized trees than the start-offset table holds (65536)");
-  enqueue_tree(c, 0, false, 1.0);
-  HIP_CHECK(hipGetLastError());
-  copy_tree_results(c, 0);
+  run_tree(c, 0, false, 1.0);
   if (c->opts.use_quantized_grad) c->n_calls++;
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->trees.push_back(build_tree(c, 0));
@@ -1244,8 +1289,8 @@ lgbmhip_status lgbmhip_leaf_of_row(lgbmhip_ctx* ctx, int32_t* out) {
     set_device(c);
     DevBuf<int> d;
     d.alloc(c->N);
-    hipLaunchKernelGGL(k_leaf_of_row, dim3(64, c->L), dim3(256), 0, c->stream, c->st.p, c->leaves.p, c->idx[0].p,
-                       c->idx[1].p, c->idx[2].p, d.p);
+    hipLaunchKernelGGL(k_leaf_of_row, dim3(64, c->L), dim3(256), 0, c->stream, c->st.p, c->pos.p, c->leaves.p, c->L,
+                       c->idx[0].p, c->idx[1].p, c->idx[2].p, d.p);
     HIP_CHECK(hipMemcpyAsync(out, d.p, c->N * 4, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
@@ -1312,8 +1357,7 @@ lgbmhip_status lgbmhip_discretize(lgbmhip_ctx* ctx, const float* grad, const flo
                          (long long)N, c->it.p);
       hipLaunchKernelGGL(k_discretize, dim3(grid_for(N, 256, c->cus * 8)), dim3(256), 0, c->stream, (long long)N,
                          c->g.p, c->h.p, c->grv.p, c->hrv.p, c->opts.num_grad_quant_bins, c->opts.stochastic_rounding,
-                         0, c->starts.p, c->call_ctr.p, c->it.p, c->idx[0].p, (char2*)c->gh[0].p, c->gq.p, c->hq.p,
-                         c->st.p, c->leaves.p, c->best.p, c->spl.p, c->F, c->L, c->best_gain.p, c->best_feat.p);
+                         0, c->starts.p, c->call_ctr.p, c->it.p, c->idx[0].p, (char2*)c->gh[0].p, c->gq.p, c->hq.p);
       hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, c->stream, c->call_ctr.p);
       c->n_calls++;
     }
@@ -1499,7 +1543,7 @@ lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out8) {
       if (enable && c->pev.empty()) {
         set_device(c);
         if (!c->state_ready) setup_training(c);
-        c->pev.resize((size_t)c->K * c->L * 3 * 2);
+        c->pev.resize((size_t)c->K * prof_steps(c) * 3 * 2);
         for (auto& e : c->pev) HIP_CHECK(hipEventCreate(&e));
       }
     }

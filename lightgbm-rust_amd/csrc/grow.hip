@@ -1,0 +1,1212 @@
+// Tree growth on the device: histograms, split search, partition and the
+// round driver (included by engine.hip after kernels.hip).
+//
+// The reference grows a tree one leaf at a time (SerialTreeLearner::train,
+// learner/mod.rs:572-695): ArgMax over the leaves' best splits (:698-706),
+// split that leaf, build the smaller child's histogram, derive the larger
+// one by subtraction, search both children. Only the ArgMax is sequential:
+// a leaf's histogram and best split depend on that leaf's rows alone. This
+// engine therefore grows in rounds. A round splits the k best leaves of the
+// frontier together (one partition, one histogram and one search launch for
+// all k), and the next round opens by replaying the reference's sequential
+// ArgMax over the results, in order, for as long as every split it picks has
+// been computed. The replayed tree - node numbering included - is the one the
+// sequential loop builds. Splits computed ahead of need that the tree never
+// reaches are dropped (their leaves keep their rows).
+//
+// Leaves live in slots (0 = root; a round allocates two child slots per
+// split). The tree's own leaf numbering is the position of a slot in the
+// current position array, double-buffered so that every block of a launch
+// reads the same version while block 0 writes the next.
+
+namespace lgbmhip {
+
+constexpr int kPartThreads = 256;
+constexpr int kPartRowsPerThread = 4;
+constexpr int kPartTile = kPartThreads * kPartRowsPerThread;
+constexpr int kRowThreads = kPartThreads / (2 * kMaxBatch);   // threads reducing one searched row
+static_assert(kRowThreads >= 1, "a searched row needs at least one thread");
+
+__device__ __forceinline__ int child_buf(int buf) { return buf == 1 ? 2 : 1; }
+template <typename T>
+__device__ __forceinline__ T sel3(T a0, T a1, T a2, int b) { return b == 0 ? a0 : (b == 1 ? a1 : a2); }
+
+// before_find_best_split (learner/mod.rs:712-731) for the two children of a
+// split: no search when the children reach max_depth or both are too small.
+__device__ __forceinline__ bool children_skip(const SplitParamsDev& p, int child_depth, long long lc, long long rc) {
+  return (p.max_depth > 0 && child_depth >= p.max_depth) ||
+         (rc < 2LL * p.min_data_in_leaf && lc < 2LL * p.min_data_in_leaf);
+}
+
+// ---------------------------------------------------------------- tree state
+// learner/mod.rs:580-628: one leaf (the root) holding every row, searched first.
+__global__ void k_state_init(StepDev* st, PosRec* pos, LeafDev* leaves, uint8_t* splittable, long long n, int F) {
+  const int t = threadIdx.x;
+  for (int f = t; f < F; f += blockDim.x) splittable[f] = 1;
+  if (t < kMaxBatch) {
+    st->lcur[t] = 0;
+    st->rcur[t] = 0;
+  }
+  if (t == 0) {
+    st->done = 0;
+    st->nseq = 1;
+    st->npending = 0;
+    st->nslots = 1;
+    st->num_splits = 0;
+    st->parity = 0;
+    st->search_base = 0;
+    st->nsearched = 1;
+    st->rounds = 0;
+    st->hist_items = 1;
+    st->hist_rows = n;
+    PosRec p;
+    p.gain = -INFINITY;
+    p.cgain[0] = p.cgain[1] = -INFINITY;
+    p.feat = -1;
+    p.slot = 0;
+    p.spec = -1;
+    p.cfeat[0] = p.cfeat[1] = -1;
+    p.pad = 0;
+    pos[0] = p;
+    LeafDev& r = leaves[0];
+    r.begin = 0;
+    r.count = n;
+    r.buf = 0;
+    r.depth = 0;
+    r.sum_g = r.sum_h = 0.0;
+    r.output = 0.0;
+    r.int_sum = 0;
+  }
+}
+
+// ---------------------------------------------------------------- histograms
+// Row-wise histograms (multi_val_bin.rs:556-611 FP64, :710-791 integer).
+// FP64 values are summed as exact fixed-point int64 (ds_add_u64 in LDS, then
+// 64-bit global atomics of the nonzero bins into the item's accumulator):
+// exact, so the result is independent of order and block count. The
+// quantized form sums int8 (g, h) as int32 / uint32 in LDS and flushes
+// packed (g << 32) + h, equal to the reference's wrapping sum of packed rows.
+//
+// Every leaf's rows are one contiguous segment of an ordered buffer (buffer
+// 0 holds the root in row order; a leaf in buffer 1 splits into buffer 2 and
+// vice versa), each feature group as its own [N][FB]-byte array: a block
+// streams its group's bytes of its rows with no index indirection.
+//
+// Work is a list of items (segments); each item is cut into row chunks, and
+// a unit = (item, chunk, feature group) is one LDS histogram. Blocks loop
+// over units.
+struct HistArgs {
+  const uint8_t* grp[3];   // ordered bins per buffer, group-major [NG][N][FB]
+  const void* gh[3];       // float2 (plain) or char2 (quantized), per buffer
+  StepDev* st;
+  const PlanDev* plan;
+  const PlanRep* reps;
+  LeafDev* leaves;
+  SplitRec* log;
+  const SplitRec* best;
+  const FeatDev* feats;
+  int F, B, FB, NG;        // features, max bins, features per group, groups
+  int L;
+  long long N;
+  int max_chunks;
+  int min_rows_per_chunk;
+  long long* acc;          // [kMaxBatch][F][B][2] (plain) or [kMaxBatch][F][B] (quantized)
+  const IterDev* it;
+  SplitParamsDev p;
+  int quant;
+  int mode;                // 1: explicit segment, 2: the root, 3: the round's items
+  long long d_begin, d_count;
+  int d_buf, d_shift_g, d_shift_h;
+};
+
+// rows each thread has in flight per pass (all loads issued before the atomics)
+constexpr int kHistUnroll = 8;
+
+// One unit: rows [r0, r1) of the segment's chunk, feature group fg, into LDS,
+// then into `out` (the item's accumulator).
+template <int TPR, bool Q>
+__device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long long begin, long long count, int buf,
+                                          int sg, int sh, int chunk, int nchunks, int fg, long long* out_acc) {
+  const long long per = (count + nchunks - 1) / nchunks;
+  const long long r0 = (long long)chunk * per, r1 = min(count, r0 + per);
+  if (r0 >= r1) return;   // uniform over the block
+  const int f0 = fg * a.FB;
+  const int FBg = min(a.FB, a.F - f0);
+  const int B = a.B;
+  const int n_lds = FBg * B * 2;
+  unsigned long long* lds64 = (unsigned long long*)ldsv;
+  unsigned* lds32 = (unsigned*)ldsv;
+  for (int e = threadIdx.x; e < n_lds; e += blockDim.x) {
+    if (Q) lds32[e] = 0u;
+    else lds64[e] = 0ull;
+  }
+  // most-frequent bin of each of this thread's 4 features (never accumulated)
+  const int sub = threadIdx.x / TPR, j = threadIdx.x % TPR;
+  int mfb[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int fl = j * 4 + k;
+    mfb[k] = fl < FBg ? a.feats[f0 + fl].mfb : 256;
+  }
+  __syncthreads();
+  const int rpp = 256 / TPR;
+  // select, not index: a runtime index into a kernel-argument array makes
+  // the compiler copy the arguments to scratch memory
+  const uint8_t* __restrict__ gb =
+      sel3(a.grp[0], a.grp[1], a.grp[2], buf) + ((size_t)fg * a.N + begin) * a.FB + j * 4;
+  const void* ghb = sel3(a.gh[0], a.gh[1], a.gh[2], buf);
+  const bool active = j * 4 < FBg;
+  for (long long base = r0; base < r1; base += kHistUnroll * rpp) {
+    unsigned w[kHistUnroll];
+    long long gv[kHistUnroll], hv[kHistUnroll];
+    bool ok[kHistUnroll];
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      const long long p = base + u * rpp + sub;
+      ok[u] = active && p < r1;
+      if (ok[u]) {
+        w[u] = *(const unsigned*)(gb + (size_t)p * a.FB);
+        if (Q) {
+          const char2 v = ((const char2*)ghb)[begin + p];
+          gv[u] = v.x;
+          hv[u] = (int)(uint8_t)v.y;
+        } else {
+          const float2 v = ((const float2*)ghb)[begin + p];
+          gv[u] = to_fixed(v.x, sg);
+          hv[u] = to_fixed(v.y, sh);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      if (!ok[u]) continue;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int b = (w[u] >> (8 * k)) & 0xFF;
+        const int fl = j * 4 + k;
+        if (fl < FBg && b != mfb[k]) {
+          const size_t c = ((size_t)fl * B + b) * 2;
+          if (Q) {
+            atomicAdd(lds32 + c, (unsigned)(int)gv[u]);
+            atomicAdd(lds32 + c + 1, (unsigned)hv[u]);
+          } else {
+            atomicAdd(lds64 + c, (unsigned long long)gv[u]);
+            atomicAdd(lds64 + c + 1, (unsigned long long)hv[u]);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (Q) {
+    unsigned long long* out = (unsigned long long*)out_acc + (size_t)f0 * B;
+    for (int e = threadIdx.x; e < FBg * B; e += blockDim.x) {
+      const unsigned gsum = lds32[2 * e], hsum = lds32[2 * e + 1];
+      if (gsum | hsum) {
+        const unsigned long long packed =
+            ((unsigned long long)(long long)(int)gsum << 32) + (unsigned long long)hsum;
+        atomicAdd(out + e, packed);
+      }
+    }
+  } else {
+    unsigned long long* out = (unsigned long long*)out_acc + (size_t)f0 * B * 2;
+    for (int e = threadIdx.x; e < n_lds; e += blockDim.x) {
+      const unsigned long long v = lds64[e];
+      if (v) atomicAdd(out + e, v);
+    }
+  }
+  __syncthreads();   // the next unit reuses the LDS histogram
+}
+
+// Applies the round's plan to the tree state (block 0 of the histogram pass,
+// the first launch after the partition): the children's leaf records
+// (learner/mod.rs:1105-1225), the replayed splits' log entries, and the
+// step record.
+__device__ void apply_plan(const HistArgs& a) {
+  StepDev* st = a.st;
+  const PlanDev* P = a.plan;
+  const int tid = threadIdx.x;
+  const int nw = P->nwork;
+  const int nspl0 = st->num_splits;
+  for (int w = tid; w < nw; w += blockDim.x) {
+    const PlanItem& it = P->item[w];
+    const long long lc = (long long)st->lcur[w], rc = it.count - lc;
+    const int nb = child_buf(it.buf);
+    LeafDev& Lf = a.leaves[it.lslot];
+    LeafDev& Rt = a.leaves[it.lslot + 1];
+    Lf.begin = it.begin;
+    Lf.count = lc;
+    Lf.buf = nb;
+    Lf.depth = it.depth + 1;
+    Lf.sum_g = it.rec.lsg;
+    Lf.sum_h = it.rec.lsh;
+    Lf.int_sum = it.rec.lgh;
+    Lf.output = isnan(it.rec.left_output) ? 0.0 : it.rec.left_output;
+    Rt.begin = it.begin + lc;
+    Rt.count = rc;
+    Rt.buf = nb;
+    Rt.depth = it.depth + 1;
+    Rt.sum_g = it.rec.rsg;
+    Rt.sum_h = it.rec.rsh;
+    Rt.int_sum = it.rec.rgh;
+    Rt.output = isnan(it.rec.right_output) ? 0.0 : it.rec.right_output;
+  }
+  // replayed splits: the parent's best split, applied to its position
+  constexpr int RW = (int)(sizeof(SplitRec) / 4);
+  for (int e = tid; e < P->nrep * RW; e += blockDim.x) {
+    const int i = e / RW, wd = e % RW;
+    const PlanRep rp = a.reps[i];
+    unsigned* dst = (unsigned*)(a.log + nspl0 + i);
+    dst[wd] = ((const unsigned*)(a.best + rp.parent))[wd];
+  }
+  __syncthreads();
+  for (int i = tid; i < P->nrep; i += blockDim.x) {
+    const PlanRep rp = a.reps[i];
+    SplitRec* d = a.log + nspl0 + i;
+    d->leaf = rp.pos;
+    d->left_count = (int)a.leaves[rp.lslot].count;
+    d->right_count = (int)a.leaves[rp.lslot + 1].count;
+  }
+  if (tid == 0) {
+    st->nseq = P->nseq;
+    st->npending = P->npending;
+    st->search_base = P->nslots - 2 * nw;
+    st->nsearched = 2 * nw;
+    st->nslots = P->nslots;
+    st->num_splits = P->num_splits;
+    st->parity ^= 1;
+    st->rounds += 1;
+    st->done = P->done;
+  }
+}
+
+template <int TPR, bool Q>
+__device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
+  __shared__ long long s_begin[kMaxBatch], s_count[kMaxBatch];
+  __shared__ int s_buf[kMaxBatch], s_sg[kMaxBatch], s_sh[kMaxBatch], s_nch[kMaxBatch], s_u0[kMaxBatch + 1];
+  const int tid = threadIdx.x;
+  const size_t stride = (size_t)a.F * a.B * (Q ? 1 : 2);   // one item's accumulator
+  int nitems;
+  if (a.mode != 3) {
+    if (tid == 0) {
+      const bool root = a.mode == 2;
+      const long long cnt = root ? a.N : a.d_count;
+      s_begin[0] = root ? 0 : a.d_begin;
+      s_count[0] = cnt;
+      s_buf[0] = root ? 0 : a.d_buf;
+      s_sg[0] = root ? (Q ? 0 : fix_shift(cnt, a.it->max_abs_g_bits)) : a.d_shift_g;
+      s_sh[0] = root ? (Q ? 0 : fix_shift(cnt, a.it->max_abs_h_bits)) : a.d_shift_h;
+      const long long want = (cnt + a.min_rows_per_chunk - 1) / a.min_rows_per_chunk;
+      s_nch[0] = (int)min((long long)a.max_chunks, max(want, 1LL));
+      s_u0[0] = 0;
+      s_u0[1] = s_nch[0] * a.NG;
+    }
+    nitems = 1;
+  } else {
+    StepDev* st = a.st;
+    const PlanDev* P = a.plan;
+    // block 0 applies the plan once: the round that finishes the tree still
+    // replays its last splits; later rounds find the tree done
+    if (blockIdx.x == 0) {
+      __shared__ int s_was_done;
+      if (tid == 0) s_was_done = st->done;
+      __syncthreads();
+      if (!s_was_done) apply_plan(a);
+    }
+    if (P->done) return;
+    nitems = P->nwork;
+    if (tid < nitems) {
+      const PlanItem& it = P->item[tid];
+      const long long lc = (long long)st->lcur[tid], rc = it.count - lc;
+      const bool run = !P->complete && !children_skip(a.p, it.depth + 1, lc, rc);
+      const long long sc = lc < rc ? lc : rc;
+      s_begin[tid] = lc < rc ? it.begin : it.begin + lc;
+      s_count[tid] = run ? sc : 0;
+      s_buf[tid] = child_buf(it.buf);
+      s_sg[tid] = Q ? 0 : fix_shift(sc, a.it->max_abs_g_bits);
+      s_sh[tid] = Q ? 0 : fix_shift(sc, a.it->max_abs_h_bits);
+      const long long want = (sc + a.min_rows_per_chunk - 1) / a.min_rows_per_chunk;
+      s_nch[tid] = run ? (int)min((long long)a.max_chunks, max(want, 1LL)) : 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int u = 0;
+      long long rows = 0;
+      int built = 0;
+      for (int w = 0; w < nitems; ++w) {
+        s_u0[w] = u;
+        u += s_nch[w] * a.NG;
+        rows += s_count[w];
+        built += s_nch[w] > 0 ? 1 : 0;
+      }
+      s_u0[nitems] = u;
+      if (blockIdx.x == 0) {
+        // work counters for profiles (read by the host after the tree)
+        st->hist_rows += rows;
+        st->hist_items += built;
+      }
+    }
+  }
+  __syncthreads();
+  const int units = s_u0[nitems];
+  int w = 0;
+  for (int u = blockIdx.x; u < units; u += gridDim.x) {
+    while (u >= s_u0[w + 1]) ++w;
+    const int r = u - s_u0[w];
+    hist_unit<TPR, Q>(a, lds, s_begin[w], s_count[w], s_buf[w], s_sg[w], s_sh[w], r / a.NG, s_nch[w], r % a.NG,
+                      a.acc + (size_t)w * stride);
+  }
+}
+
+template <int TPR>
+__global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
+  extern __shared__ unsigned long long lds_hist[];
+  hist_kernel<TPR, false>(a, lds_hist);
+}
+
+template <int TPR>
+__global__ void __launch_bounds__(256) k_hist_int(HistArgs a) {
+  extern __shared__ unsigned long long lds_hist[];
+  hist_kernel<TPR, true>(a, lds_hist);
+}
+
+// ---------------------------------------------------------------- split search
+struct SplitArgs {
+  StepDev* st;
+  const PlanDev* plan;
+  LeafDev* leaves;
+  const FeatDev* feats;
+  SplitParamsDev p;
+  int F, B, L;
+  long long* acc;        // per item: the smaller child's histogram (zeroed after reading)
+  double* pool;          // [slots][F][B][2] FP64 histograms
+  long long* ipool;      // [slots][F][B] packed histograms (quantized)
+  SplitRec* feat_split;  // [2 kMaxBatch][F] searches of this pass, row = slot - first child slot
+  uint8_t* splittable;   // [slots][F]
+  IterDev* it;
+  int quant;
+  int root;              // 1: search the root (slot 0, row 0)
+};
+
+// The leaf sums a search needs (leaf_splits.hpp; the root's come from the
+// fixed-point / packed root sums).
+__device__ __forceinline__ LeafSums leaf_sums(const SplitArgs& a, int slot, bool root) {
+  LeafSums s;
+  const LeafDev& L = a.leaves[slot];
+  s.nd = (int)L.count;
+  s.isum = L.int_sum;
+  s.gs = a.it->grad_scale;
+  s.hs = a.it->hess_scale;
+  double sg = L.sum_g, sh = L.sum_h;
+  if (root) {
+    if (a.quant) {
+      s.isum = a.it->root_int_sum;
+      sg = (double)a.it->root_gq_sum * s.gs;
+      sh = (double)a.it->root_hq_sum * s.hs;
+    } else {
+      sg = ldexp((double)a.it->root_g_fix, -fix_shift(L.count, a.it->max_abs_g_bits));
+      sh = ldexp((double)a.it->root_h_fix, -fix_shift(L.count, a.it->max_abs_h_bits));
+    }
+  }
+  s.sg = sg;
+  s.sh = sh + 2.0 * KEPS;
+  // learner/mod.rs:1303-1309: the root's parent output is its own output
+  s.po = root ? leaf_output_t(sg, sh, a.p, true, true, false, s.nd, 0.0) : L.output;
+  return s;
+}
+
+// One block (2 waves) per (feature, split of the round): the smaller child's
+// histogram from the accumulator (with the most-frequent-bin fix,
+// histogram.rs:42-78), the larger child's as parent - smaller
+// (histogram.rs:59-85), then wave 0 searches the smaller child and wave 1 the
+// larger (learner/mod.rs:954-1101, split.rs:246-271 / 930-950).
+__global__ void __launch_bounds__(128) k_grow_split(SplitArgs a) {
+  StepDev* st = a.st;
+  if (st->done) return;
+  __shared__ double hsm[2][kMaxBins];   // smaller: g, h
+  __shared__ double hlg[2][kMaxBins];   // larger: g, h
+  __shared__ long long ism[kMaxBins], ilg[kMaxBins];
+  __shared__ ScanLds scr[2];
+  __shared__ SplitRec res[2];
+  const int f = blockIdx.x, w = blockIdx.y;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int F = a.F, B = a.B;
+  const bool root = a.root != 0;
+  int sm, lg, parent, row_sm, row_lg;
+  long long sc;
+  if (root) {
+    sm = 0;
+    lg = -1;
+    parent = -1;
+    row_sm = 0;
+    row_lg = -1;
+    sc = a.leaves[0].count;
+  } else {
+    const PlanDev* P = a.plan;
+    if (w >= P->nwork) return;
+    const PlanItem& it = P->item[w];
+    const int ls = it.lslot, rs = it.lslot + 1;
+    const long long lc = a.leaves[ls].count, rc = a.leaves[rs].count;
+    if (P->complete || children_skip(a.p, it.depth + 1, lc, rc)) {
+      // no search: both children keep (-inf, none)
+      if (lane == 0 && wave < 2) {
+        SplitRec r;
+        reset_rec(r);
+        a.feat_split[(2 * w + wave) * F + f] = r;
+      }
+      if (f == 0 && w == 0 && tid < kMaxBatch) {
+        st->lcur[tid] = 0;
+        st->rcur[tid] = 0;
+      }
+      return;
+    }
+    const bool left_smaller = lc < rc;   // learner/mod.rs:1200-1208
+    sm = left_smaller ? ls : rs;
+    lg = left_smaller ? rs : ls;
+    row_sm = 2 * w + (left_smaller ? 0 : 1);
+    row_lg = 2 * w + (left_smaller ? 1 : 0);
+    parent = it.parent;
+    sc = left_smaller ? lc : rc;
+  }
+  // every load this block needs is independent once the item is known:
+  // issue them together, then work from registers and LDS
+  const FeatDev fd = a.feats[f];
+  const bool parent_spl = root ? true : a.splittable[(size_t)parent * F + f] != 0;
+  const LeafSums ls_s = leaf_sums(a, sm, root);
+  const LeafSums ls_l = lg >= 0 ? leaf_sums(a, lg, false) : ls_s;
+  constexpr int PB = kMaxBins / 128;   // bins per thread
+  if (!a.quant) {
+    const int ssg = fix_shift(sc, a.it->max_abs_g_bits);
+    const int ssh = fix_shift(sc, a.it->max_abs_h_bits);
+    long long* acc = a.acc + (size_t)w * F * B * 2 + (size_t)f * B * 2;
+    const double* pp = lg >= 0 ? a.pool + ((size_t)parent * F + f) * B * 2 : nullptr;
+    long long gi[PB], hi[PB];
+    double pg[PB], ph[PB];
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      const int b = tid + 128 * k;
+      gi[k] = hi[k] = 0;
+      pg[k] = ph[k] = 0.0;
+      if (b < B) {
+        gi[k] = acc[2 * b];
+        hi[k] = acc[2 * b + 1];
+        if (pp) {
+          pg[k] = pp[2 * b];
+          ph[k] = pp[2 * b + 1];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      const int b = tid + 128 * k;
+      if (b < B) {
+        acc[2 * b] = 0;
+        acc[2 * b + 1] = 0;
+        hsm[0][b] = ldexp((double)gi[k], -ssg);
+        hsm[1][b] = ldexp((double)hi[k], -ssh);
+      }
+    }
+    __syncthreads();
+    // histogram.rs:42-56: the most frequent bin from the leaf totals, in
+    // the reference's order (two sequential chains)
+    if (fd.mfb > 0 && tid < 2) {
+      double v = tid == 0 ? ls_s.sg : ls_s.sh - 2.0 * KEPS;
+      const double* __restrict__ arr = hsm[tid];
+      int i = 0;
+      for (; i + 8 <= fd.num_bin; i += 8) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = arr[i + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v = (i + u == fd.mfb) ? v : v - x[u];
+      }
+      for (; i < fd.num_bin; ++i) v = (i == fd.mfb) ? v : v - arr[i];
+      hsm[tid][fd.mfb] = v;
+    }
+    __syncthreads();
+    double* ps = a.pool + ((size_t)sm * F + f) * B * 2;
+    double* pl = lg >= 0 ? a.pool + ((size_t)lg * F + f) * B * 2 : nullptr;
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      const int b = tid + 128 * k;
+      if (b < B) {
+        const double sgv = hsm[0][b], shv = hsm[1][b];
+        ps[2 * b] = sgv;
+        ps[2 * b + 1] = shv;
+        if (pl) {
+          const double lgv = pg[k] - sgv, lhv = ph[k] - shv;
+          hlg[0][b] = lgv;
+          hlg[1][b] = lhv;
+          pl[2 * b] = lgv;
+          pl[2 * b + 1] = lhv;
+        }
+      }
+    }
+  } else {
+    long long* acc = a.acc + (size_t)w * F * B + (size_t)f * B;
+    const long long* pp = lg >= 0 ? a.ipool + ((size_t)parent * F + f) * B : nullptr;
+    long long v[PB], pv[PB];
+    unsigned long long part = 0;
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      const int b = tid + 128 * k;
+      v[k] = pv[k] = 0;
+      if (b < B) {
+        v[k] = acc[b];
+        if (pp) pv[k] = pp[b];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      const int b = tid + 128 * k;
+      if (b < B) {
+        acc[b] = 0;
+        ism[b] = v[k];
+        if (b != fd.mfb && b < fd.num_bin) part += (unsigned long long)v[k];
+      }
+    }
+    // histogram.rs:67-78: wrapping sums are exact in any order, so the
+    // most frequent bin is the leaf total minus a block reduction
+    for (int o = 32; o > 0; o >>= 1) part += (unsigned long long)__shfl_xor((long long)part, o, 64);
+    __shared__ unsigned long long wpart[2];
+    if (lane == 0) wpart[wave] = part;
+    __syncthreads();
+    if (fd.mfb > 0 && tid == 0) ism[fd.mfb] = (long long)((unsigned long long)ls_s.isum - wpart[0] - wpart[1]);
+    __syncthreads();
+    long long* ps = a.ipool + ((size_t)sm * F + f) * B;
+    long long* pl = lg >= 0 ? a.ipool + ((size_t)lg * F + f) * B : nullptr;
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      const int b = tid + 128 * k;
+      if (b < B) {
+        const long long sv = ism[b];
+        ps[b] = sv;
+        if (pl) {
+          const long long lv = (long long)((unsigned long long)pv[k] - (unsigned long long)sv);
+          ilg[b] = lv;
+          pl[b] = lv;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // wave 0 searches the smaller child, wave 1 the larger
+  const int slot = wave == 0 ? sm : lg;
+  if (slot >= 0) {
+    const LeafSums ls = wave == 0 ? ls_s : ls_l;
+    SplitRec& r = res[wave];
+    if (lane == 0) reset_rec(r);
+    bool spl = false;
+    wave_sync();
+    if (parent_spl) {
+      if (lane == 0) r.gain = -INFINITY;
+      if (a.quant) {
+        spl = search_int(wave == 0 ? ism : ilg, fd, a.p, ls, scr[wave], r);
+      } else if (fd.bin_type == 1) {
+        spl = search_categorical(wave == 0 ? hsm[0] : hlg[0], wave == 0 ? hsm[1] : hlg[1], fd, a.p, ls, scr[wave], r);
+      } else {
+        spl = search_numerical(wave == 0 ? hsm[0] : hlg[0], wave == 0 ? hsm[1] : hlg[1], fd, a.p, ls, scr[wave], r);
+      }
+    }
+    if (lane == 0) {
+      if (parent_spl) r.feature = f;
+      a.feat_split[(size_t)(wave == 0 ? row_sm : row_lg) * F + f] = r;
+      a.splittable[(size_t)slot * F + f] = spl ? 1 : 0;
+    }
+  }
+  if (root) {
+    if (f == 0 && tid == 0) {
+      // learner/mod.rs:647: the root's output before any split
+      const LeafSums r = leaf_sums(a, 0, true);
+      st->root_output = round0(leaf_output_t(r.sg, r.sh - 2.0 * KEPS, a.p, true, true, false, r.nd, 0.0));
+    }
+  } else if (f == 0 && w == 0 && tid < kMaxBatch) {
+    // the next partition pass reserves slots from zero (nothing in this
+    // launch reads the counters: child counts come from the leaf records)
+    st->lcur[tid] = 0;
+    st->rcur[tid] = 0;
+  }
+}
+
+// ---------------------------------------------------------------- partition
+struct PartArgs {
+  StepDev* st;
+  PlanDev* plan;
+  PlanRep* reps;            // [L] replayed splits of the round
+  PosRec* pos;              // [2][L] position arrays
+  LeafDev* leaves;
+  const FeatDev* feats;
+  SplitRec* best;           // [slots] best split of each searched slot
+  const SplitRec* feat_split;
+  uint8_t* grp[3];          // ordered bins per buffer, group-major [NG][N][FB]
+  int FB, NG;               // bytes (features) per group, groups
+  long long N;
+  unsigned* idx[3];
+  void* gh[3];
+  int gh_bytes;             // 8 (float2) or 2 (char2)
+  int F, L;
+};
+
+// A candidate of an ArgMax: gain, feature (-1 none) and position (feature
+// slot or leaf). pick() keeps the better one in the order the reference's
+// scans imply - higher gain, then smaller feature (learner/mod.rs:1064-1101),
+// then the earlier position (first max, learner/mod.rs:698-706). The order
+// is total, so any reduction tree gives the same answer.
+struct Cand {
+  double g;
+  int f, l;
+};
+
+__device__ __forceinline__ Cand pick(Cand a, Cand b) {
+  const bool take = b.l != INT32_MAX &&
+                    (a.l == INT32_MAX || better(b.g, b.f, a.g, a.f) || (!better(a.g, a.f, b.g, b.f) && b.l < a.l));
+  // field-wise selects keep the candidates in registers
+  Cand r;
+  r.g = take ? b.g : a.g;
+  r.f = take ? b.f : a.f;
+  r.l = take ? b.l : a.l;
+  return r;
+}
+
+__device__ __forceinline__ Cand wave_pick(Cand c) {
+  for (int o = 32; o > 0; o >>= 1) {
+    Cand d;
+    d.g = __shfl_xor(c.g, o, 64);
+    d.f = __shfl_xor(c.f, o, 64);
+    d.l = __shfl_xor(c.l, o, 64);
+    c = pick(c, d);
+  }
+  return c;
+}
+
+__device__ __forceinline__ Cand cand_none() {
+  Cand c;
+  c.g = -INFINITY;
+  c.f = -1;
+  c.l = INT32_MAX;
+  return c;
+}
+
+// One partition pass per round. Every block first derives the round's plan
+// from the same inputs - no block waits for another:
+//  1. each leaf searched by the last split pass gets its best split
+//     (learner/mod.rs:1064-1101);
+//  2. replay: the sequential ArgMax over the tree's leaves (:698-706), with
+//     the reference's stop rules (gain <= 0, num_leaves reached), applying
+//     each pick that was already split, until it picks a leaf that was not;
+//  3. the round splits that leaf and the next best unsplit leaves, at most
+//     kMaxBatch and no more than the tree can still take.
+// Block 0 publishes the plan, the next position array and the searched
+// leaves' best splits. Then every block moves its tiles' rows - index,
+// gradients and group-major bins - into the children's segments of the other
+// buffer: left rows from the front, right rows from the back, each tile
+// reserving its slots with one atomic per side (DataPartition::split,
+// partition.rs:107-174). The row order inside a child is unspecified: every
+// consumer of a segment (exact fixed-point or integer histograms, renewal
+// sums, score update) is order-independent.
+__global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
+  StepDev* st = a.st;
+  extern __shared__ PosRec P[];   // this block's copy of the position array
+  constexpr int NW = kPartThreads / 64;
+  constexpr int RW = (int)(sizeof(SplitRec) / 4);
+  __shared__ Cand rowbest[2 * kMaxBatch];
+  __shared__ SplitRec irec[kMaxBatch];
+  __shared__ FeatDev ifd[kMaxBatch];
+  __shared__ long long ibegin[kMaxBatch], icount[kMaxBatch];
+  __shared__ int ipos[kMaxBatch], iparent[kMaxBatch], ibuf[kMaxBatch], idepth[kMaxBatch], itile0[kMaxBatch + 1];
+  __shared__ int s_k, s_fin, s_nseq, s_nspl, s_nrep, s_npend, s_complete;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // st->done changes only in the histogram pass: uniform here
+  if (st->done) return;
+  const int nseq0 = st->nseq, npend0 = st->npending, nslots0 = st->nslots, nspl0 = st->num_splits;
+  const int par = st->parity, sbase = st->search_base, nsearched = st->nsearched;
+  const int F = a.F, L = a.L;
+  // ---- 1. positions and searched rows (independent loads)
+  for (int i = tid; i < nseq0; i += kPartThreads) P[i] = a.pos[(size_t)par * L + i];
+  {
+    const int r = tid / kRowThreads, q = tid % kRowThreads;
+    Cand c = cand_none();
+    if (r < nsearched)
+      for (int f = q; f < F; f += kRowThreads) {
+        const SplitRec& rec = a.feat_split[(size_t)r * F + f];
+        Cand d;
+        d.g = rec.gain;
+        d.f = rec.feature;
+        d.l = f;
+        c = pick(c, d);
+      }
+    for (int o = 1; o < kRowThreads; o <<= 1) {
+      Cand d;
+      d.g = __shfl_xor(c.g, o, 64);
+      d.f = __shfl_xor(c.f, o, 64);
+      d.l = __shfl_xor(c.l, o, 64);
+      c = pick(c, d);
+    }
+    if (q == 0 && r < 2 * kMaxBatch) {
+      // a leaf without any candidate carries (-inf, none)
+      if (!better(c.g, c.f, -INFINITY, -1)) {
+        c.g = -INFINITY;
+        c.f = -1;
+        c.l = -1;
+      }
+      rowbest[r] = c;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < nseq0; i += kPartThreads) {
+    PosRec& q = P[i];
+    const int rs = q.slot - sbase;
+    if (rs >= 0 && rs < nsearched) {
+      q.gain = rowbest[rs].g;
+      q.feat = rowbest[rs].f;
+    }
+    if (q.spec >= 0)
+      for (int c = 0; c < 2; ++c) {
+        const int rc = q.spec + c - sbase;
+        if (rc >= 0 && rc < nsearched) {
+          q.cgain[c] = rowbest[rc].g;
+          q.cfeat[c] = rowbest[rc].f;
+        }
+      }
+  }
+  __syncthreads();
+  // ---- 2 + 3. replay and selection, wave 0
+  if (wave == 0) {
+    int nseq = nseq0, nspl = nspl0, nrep = 0, npend = npend0;
+    bool fin = false;
+    for (;;) {
+      if (nseq >= L) {
+        fin = true;
+        break;
+      }
+      Cand c = cand_none();
+      for (int i = lane; i < nseq; i += 64) {
+        Cand d;
+        d.g = P[i].gain;
+        d.f = P[i].feat;
+        d.l = i;
+        c = pick(c, d);
+      }
+      c = wave_pick(c);
+      if (!(c.g > 0.0)) {
+        fin = true;
+        break;
+      }
+      const int m = c.l;
+      const int sp = P[m].spec;
+      if (sp < 0) break;   // not split yet: this round splits it
+      if (lane == 0) {
+        if (blockIdx.x == 0) {
+          PlanRep rp;
+          rp.pos = m;
+          rp.parent = P[m].slot;
+          rp.lslot = sp;
+          rp.pad = 0;
+          a.reps[nrep] = rp;
+        }
+        const PosRec q = P[m];
+        PosRec lp, rp;
+        lp.gain = q.cgain[0];
+        lp.feat = q.cfeat[0];
+        lp.slot = sp;
+        rp.gain = q.cgain[1];
+        rp.feat = q.cfeat[1];
+        rp.slot = sp + 1;
+        lp.spec = rp.spec = -1;
+        lp.cgain[0] = lp.cgain[1] = rp.cgain[0] = rp.cgain[1] = -INFINITY;
+        lp.cfeat[0] = lp.cfeat[1] = rp.cfeat[0] = rp.cfeat[1] = -1;
+        lp.pad = rp.pad = 0;
+        P[m] = lp;        // learner/mod.rs:1105-1225: the left child keeps the leaf's number,
+        P[nseq] = rp;     // the right child takes the next one
+      }
+      wave_sync();
+      ++nseq;
+      ++nspl;
+      ++nrep;
+      --npend;
+    }
+    int k = 0;
+    if (!fin) {
+      const int kmax = min(kMaxBatch, L - nseq);
+      while (k < kmax) {
+        Cand c = cand_none();
+        for (int i = lane; i < nseq; i += 64) {
+          if (P[i].spec >= 0) continue;
+          Cand d;
+          d.g = P[i].gain;
+          d.f = P[i].feat;
+          d.l = i;
+          c = pick(c, d);
+        }
+        c = wave_pick(c);
+        if (c.l == INT32_MAX || !(c.g > 0.0)) break;
+        if (lane == 0) {
+          ipos[k] = c.l;
+          P[c.l].spec = nslots0 + 2 * k;   // split this round: its children's slots
+          P[c.l].cgain[0] = P[c.l].cgain[1] = -INFINITY;
+          P[c.l].cfeat[0] = P[c.l].cfeat[1] = -1;
+        }
+        wave_sync();
+        ++k;
+      }
+    }
+    if (lane == 0) {
+      s_k = k;
+      s_fin = fin ? 1 : 0;
+      s_nseq = nseq;
+      s_nspl = nspl;
+      s_nrep = nrep;
+      s_npend = npend + k;
+      // a child of this round can be split only after its parent is
+      // replayed: two more leaves
+      s_complete = nseq + 2 > L ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  const int k = s_k;
+  // ---- the round's splits: parent segment, chosen record, feature
+  if (tid < k) {
+    const int pslot = P[ipos[tid]].slot;
+    iparent[tid] = pslot;
+    const LeafDev& lf = a.leaves[pslot];
+    ibegin[tid] = lf.begin;
+    icount[tid] = lf.count;
+    ibuf[tid] = lf.buf;
+    idepth[tid] = lf.depth;
+    const int rs = pslot - sbase;
+    const int feat = (rs >= 0 && rs < nsearched) ? rowbest[rs].f : P[ipos[tid]].feat;
+    ifd[tid] = a.feats[feat];
+  }
+  for (int e = tid; e < k * RW; e += kPartThreads) {
+    const int w = e / RW, wd = e % RW;
+    const int pslot = P[ipos[w]].slot;
+    const int rs = pslot - sbase;
+    const SplitRec* src = (rs >= 0 && rs < nsearched) ? a.feat_split + (size_t)rs * F + rowbest[rs].l : a.best + pslot;
+    ((unsigned*)&irec[w])[wd] = ((const unsigned*)src)[wd];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int t = 0;
+    for (int w = 0; w < k; ++w) {
+      itile0[w] = t;
+      t += (int)((icount[w] + kPartTile - 1) / kPartTile);
+    }
+    itile0[k] = t;
+  }
+  __syncthreads();
+  const int total_tiles = itile0[k];
+  if (blockIdx.x == 0) {
+    // publish: the plan, the next position array, the searched leaves' best splits
+    PlanDev* pl = a.plan;
+    for (int e = tid; e < k * RW; e += kPartThreads) {
+      const int w = e / RW, wd = e % RW;
+      ((unsigned*)&pl->item[w].rec)[wd] = ((const unsigned*)&irec[w])[wd];
+    }
+    if (tid < k) {
+      PlanItem& it = pl->item[tid];
+      it.begin = ibegin[tid];
+      it.count = icount[tid];
+      it.parent = iparent[tid];
+      it.lslot = nslots0 + 2 * tid;
+      it.buf = ibuf[tid];
+      it.depth = idepth[tid];
+      it.tile0 = itile0[tid];
+      it.pad = 0;
+    }
+    if (tid == 0) {
+      pl->done = s_fin;
+      pl->nwork = k;
+      pl->total_tiles = total_tiles;
+      pl->nseq = s_nseq;
+      pl->npending = s_npend;
+      pl->nslots = nslots0 + 2 * k;
+      pl->num_splits = s_nspl;
+      pl->nrep = s_nrep;
+      pl->complete = s_complete;
+      pl->pad = 0;
+    }
+    PosRec* nxt = a.pos + (size_t)(par ^ 1) * L;
+    for (int i = tid; i < s_nseq; i += kPartThreads) nxt[i] = P[i];
+    for (int e = tid; e < nsearched * RW; e += kPartThreads) {
+      const int r = e / RW, wd = e % RW;
+      if (rowbest[r].l >= 0)
+        ((unsigned*)(a.best + sbase + r))[wd] = ((const unsigned*)(a.feat_split + (size_t)r * F + rowbest[r].l))[wd];
+    }
+    for (int r = tid; r < nsearched; r += kPartThreads)
+      if (rowbest[r].l < 0) {
+        SplitRec z;
+        reset_rec(z);
+        a.best[sbase + r] = z;
+      }
+  }
+  if (k == 0) return;
+  // ---- tiles
+  const int FB = a.FB, gbytes = a.gh_bytes;
+  const size_t gstride = (size_t)a.N * FB;   // bytes of one feature group's array
+  const int words = FB / 4, tw = a.NG * words;   // 4-byte words of one row over all groups
+  constexpr int R = kPartRowsPerThread;
+  constexpr int CW = 8;                          // row words moved per round of loads
+  __shared__ int cnt[2][R][NW];
+  __shared__ long long slot[3];
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int w = 0;
+  for (int t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+    while (t >= itile0[w + 1]) ++w;
+    const SplitRec& s = irec[w];
+    const FeatDev& fd = ifd[w];
+    const long long begin = ibegin[w], count = icount[w];
+    const int src = ibuf[w], dst = child_buf(src);   // buffer 0 is never a destination
+    const unsigned* idx = sel3(a.idx[0], a.idx[1], a.idx[2], src) + begin;
+    unsigned* oidx = sel3(a.idx[0], a.idx[1], a.idx[2], dst) + begin;
+    const uint8_t* igh = (const uint8_t*)sel3(a.gh[0], a.gh[1], a.gh[2], src) + (size_t)begin * gbytes;
+    uint8_t* ogh = (uint8_t*)sel3(a.gh[0], a.gh[1], a.gh[2], dst) + (size_t)begin * gbytes;
+    const uint8_t* gsrc = sel3(a.grp[0], a.grp[1], a.grp[2], src) + (size_t)begin * FB;
+    uint8_t* gdst = sel3(a.grp[0], a.grp[1], a.grp[2], dst) + (size_t)begin * FB;
+    const uint8_t* scol = gsrc + (size_t)(s.feature / FB) * gstride + (s.feature % FB);
+    const long long t0 = (long long)(t - itile0[w]) * kPartTile;
+    // one round of independent loads: index, gradients, the split feature's
+    // bin and the first words of every row (lanes take consecutive rows)
+    bool valid[R], left[R];
+    unsigned rid[R];
+    uint2 gv[R];
+    unsigned v[R][CW];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const long long p = t0 + u * kPartThreads + tid;
+      valid[u] = p < count;
+      left[u] = false;
+      if (valid[u]) {
+        rid[u] = idx[p];
+        if (gbytes == 8) gv[u] = ((const uint2*)igh)[p];
+        else gv[u].x = ((const unsigned short*)igh)[p];
+        left[u] = goes_left(scol[(size_t)p * FB], fd, s);
+#pragma unroll
+        for (int i = 0; i < CW; ++i)
+          if (i < tw) v[u][i] = *(const unsigned*)(gsrc + (size_t)(i / words) * gstride + (size_t)p * FB + 4 * (i % words));
+      }
+    }
+    // ranks inside each 64-row slice, slice counts to LDS
+    int rk[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const unsigned long long lm = __ballot(left[u]);
+      const unsigned long long rm = __ballot(valid[u] && !left[u]);
+      rk[u] = (int)__popcll((left[u] ? lm : rm) & below);
+      if (lane == 0) {
+        cnt[0][u][wave] = (int)__popcll(lm);
+        cnt[1][u][wave] = (int)__popcll(rm);
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int tl = 0, tr = 0;
+      for (int u = 0; u < R; ++u)
+        for (int q = 0; q < NW; ++q) {
+          tl += cnt[0][u][q];
+          tr += cnt[1][u][q];
+        }
+      slot[0] = (long long)atomicAdd(&st->lcur[w], (unsigned long long)tl);
+      slot[1] = (long long)atomicAdd(&st->rcur[w], (unsigned long long)tr);
+      slot[2] = tr;
+    }
+    __syncthreads();
+    // destinations: left rows fill from the front, right rows from the back;
+    // inside a tile the order is slice by slice, so a slice's stores are
+    // contiguous
+    long long d[R];
+    int pl = 0, pr = 0;
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      int bl = pl, br = pr;
+      for (int q = 0; q < NW; ++q) {
+        if (q < wave) {
+          bl += cnt[0][u][q];
+          br += cnt[1][u][q];
+        }
+        pl += cnt[0][u][q];
+        pr += cnt[1][u][q];
+      }
+      d[u] = left[u] ? slot[0] + bl + rk[u] : count - slot[1] - slot[2] + br + rk[u];
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      if (!valid[u]) continue;
+      oidx[d[u]] = rid[u];
+      if (gbytes == 8) ((uint2*)ogh)[d[u]] = gv[u];
+      else ((unsigned short*)ogh)[d[u]] = (unsigned short)gv[u].x;
+#pragma unroll
+      for (int i = 0; i < CW; ++i)
+        if (i < tw) *(unsigned*)(gdst + (size_t)(i / words) * gstride + (size_t)d[u] * FB + 4 * (i % words)) = v[u][i];
+    }
+    // rows wider than CW words: the rest, CW words per round of loads
+    for (int k0 = CW; k0 < tw; k0 += CW) {
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        if (!valid[u]) continue;
+        const long long p = t0 + u * kPartThreads + tid;
+#pragma unroll
+        for (int i = 0; i < CW; ++i) {
+          const int kk = k0 + i;
+          if (kk < tw) v[u][i] = *(const unsigned*)(gsrc + (size_t)(kk / words) * gstride + (size_t)p * FB + 4 * (kk % words));
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        if (!valid[u]) continue;
+#pragma unroll
+        for (int i = 0; i < CW; ++i) {
+          const int kk = k0 + i;
+          if (kk < tw) *(unsigned*)(gdst + (size_t)(kk / words) * gstride + (size_t)d[u] * FB + 4 * (kk % words)) = v[u][i];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Rows of the row-major bins (stride S) into the group-major layout of the
+// ordered buffers: out[g][i][0..FB) = bins of features g*FB.. of row
+// rows[i] (or row i when rows is null), zero past the last feature.
+__global__ void k_group_rows(const unsigned* rows, long long m, const uint8_t* __restrict__ rm, int S, int F, int FB,
+                             int NG, long long N, uint8_t* __restrict__ out) {
+  const int words = FB / 4;
+  const long long total = m * NG * words;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
+    const long long i = e / (NG * words);
+    const int k = (int)(e % (NG * words));
+    const int g = k / words, w = k % words;
+    const long long r = rows ? (long long)rows[i] : i;
+    unsigned v = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int f = g * FB + w * 4 + b;
+      if (f < F) v |= (unsigned)rm[r * S + f] << (8 * b);
+    }
+    *(unsigned*)(out + (size_t)g * N * FB + (size_t)i * FB + 4 * w) = v;
+  }
+}
+
+// ---------------------------------------------------------------- tree end
+// The tree's leaves are the positions 0..nseq of the current position array.
+// These kernels run after the last round; they do nothing while the tree is
+// still growing (a continuation launch runs them again).
+__device__ __forceinline__ const LeafDev& leaf_at(const StepDev* st, const PosRec* pos, const LeafDev* leaves, int L,
+                                                   int l) {
+  return leaves[pos[(size_t)st->parity * L + l].slot];
+}
+
+// Quantized form, learner/mod.rs:736-747: each leaf's output from its float
+// gradient sums (exact fixed-point per leaf, then refit_leaf_output).
+__global__ void k_renew_sum(const StepDev* st, const PosRec* pos, const LeafDev* leaves, int L, const unsigned* idx0,
+                            const unsigned* idx1, const unsigned* idx2, const float* g, const float* h,
+                            const IterDev* it, long long* sums /* [L][2] */) {
+  const int l = blockIdx.y;
+  if (!st->done || l >= st->nseq) return;
+  const LeafDev& Lf = leaf_at(st, pos, leaves, L, l);
+  const unsigned* idx = sel3(idx0, idx1, idx2, Lf.buf) + Lf.begin;
+  const int sg = fix_shift(Lf.count, it->max_abs_g_bits), sh = fix_shift(Lf.count, it->max_abs_h_bits);
+  long long ag = 0, ah = 0;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < Lf.count; p += (long long)gridDim.x * blockDim.x) {
+    const unsigned r = idx[p];
+    ag += to_fixed(g[r], sg);
+    ah += to_fixed(h[r], sh);
+  }
+  ag = wave_sum(ag);
+  ah = wave_sum(ah);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd((unsigned long long*)&sums[2 * l], (unsigned long long)ag);
+    atomicAdd((unsigned long long*)&sums[2 * l + 1], (unsigned long long)ah);
+  }
+}
+
+// Shrunk leaf values (tree.rs:299-319): out[l] = round0(output * lr); with
+// `renew`, output = round0(refit_leaf_output(sum g, sum h)). raw[l] keeps the
+// unshrunk output for the host's tree arrays.
+__global__ void k_leaf_values(const StepDev* st, const PosRec* pos, const LeafDev* leaves, const IterDev* it,
+                              long long* renew_sums, int renew, SplitParamsDev p, double lr, int L, double* out,
+                              double* raw) {
+  const int l = threadIdx.x + blockIdx.x * blockDim.x;
+  if (l >= L || !st->done) return;
+  if (l >= st->nseq) {
+    out[l] = 0.0;
+    raw[l] = 0.0;
+    return;
+  }
+  const LeafDev& Lf = leaf_at(st, pos, leaves, L, l);
+  double o = Lf.output;
+  if (renew) {
+    const int sg = fix_shift(Lf.count, it->max_abs_g_bits), sh = fix_shift(Lf.count, it->max_abs_h_bits);
+    const double sgv = ldexp((double)renew_sums[2 * l], -sg);
+    const double shv = ldexp((double)renew_sums[2 * l + 1], -sh);
+    o = round0(leaf_output_t(sgv, shv, p, true, true, false, (int)Lf.count, 0.0));
+    renew_sums[2 * l] = 0;
+    renew_sums[2 * l + 1] = 0;
+  }
+  raw[l] = o;
+  out[l] = round0(o * lr);
+}
+
+// tree.rs:635-652 / partition.rs:178-191: score[row] += value of its leaf.
+__global__ void k_score(const StepDev* st, const PosRec* pos, const LeafDev* leaves, int L, const unsigned* idx0,
+                        const unsigned* idx1, const unsigned* idx2, const double* val, double* score) {
+  const int l = blockIdx.y;
+  if (!st->done || l >= st->nseq) return;
+  const LeafDev& Lf = leaf_at(st, pos, leaves, L, l);
+  const unsigned* idx = sel3(idx0, idx1, idx2, Lf.buf) + Lf.begin;
+  const double v = val[l];
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < Lf.count; p += (long long)gridDim.x * blockDim.x)
+    score[idx[p]] += v;
+}
+
+// Leaf index of every training row (the tree's numbering).
+__global__ void k_leaf_of_row(const StepDev* st, const PosRec* pos, const LeafDev* leaves, int L, const unsigned* idx0,
+                              const unsigned* idx1, const unsigned* idx2, int* out) {
+  const int l = blockIdx.y;
+  if (l >= st->nseq) return;
+  const LeafDev& Lf = leaf_at(st, pos, leaves, L, l);
+  const unsigned* idx = sel3(idx0, idx1, idx2, Lf.buf) + Lf.begin;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < Lf.count; p += (long long)gridDim.x * blockDim.x)
+    out[idx[p]] = l;
+}
+
+// ---------------------------------------------------------------- launchers
+template <int TPR>
+static void launch_hist_t(const HistArgs& a, bool quant, int grid, size_t lds, hipStream_t s) {
+  if (quant) hipLaunchKernelGGL(k_hist_int<TPR>, dim3(grid), dim3(256), lds, s, a);
+  else hipLaunchKernelGGL(k_hist_f64<TPR>, dim3(grid), dim3(256), lds, s, a);
+}
+
+static void launch_hist(const HistArgs& a, bool quant, int tpr, int grid, size_t lds, hipStream_t s) {
+  switch (tpr) {
+    case 1: launch_hist_t<1>(a, quant, grid, lds, s); break;
+    case 2: launch_hist_t<2>(a, quant, grid, lds, s); break;
+    case 4: launch_hist_t<4>(a, quant, grid, lds, s); break;
+    case 8: launch_hist_t<8>(a, quant, grid, lds, s); break;
+    case 16: launch_hist_t<16>(a, quant, grid, lds, s); break;
+    default: launch_hist_t<32>(a, quant, grid, lds, s); break;
+  }
+}
+
+static void set_hist_lds_limits(size_t bytes) {
+  const void* ks[] = {(const void*)k_hist_f64<1>, (const void*)k_hist_f64<2>, (const void*)k_hist_f64<4>,
+                      (const void*)k_hist_f64<8>, (const void*)k_hist_f64<16>, (const void*)k_hist_f64<32>,
+                      (const void*)k_hist_int<1>, (const void*)k_hist_int<2>, (const void*)k_hist_int<4>,
+                      (const void*)k_hist_int<8>, (const void*)k_hist_int<16>, (const void*)k_hist_int<32>};
+  for (const void* k : ks) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+static void set_part_lds_limit(size_t bytes) {
+  (void)hipFuncSetAttribute((const void*)k_grow_part, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+static void launch_split(const SplitArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_grow_split, dim3(a.F, a.root ? 1 : kMaxBatch), dim3(128), 0, s, a);
+}
+
+static void launch_part(const PartArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_grow_part, dim3(grid), dim3(kPartThreads), (size_t)a.L * sizeof(PosRec), s, a);
+}
+
+static int part_tile() { return kPartTile; }
+
+}  // namespace lgbmhip

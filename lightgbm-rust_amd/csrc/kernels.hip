@@ -402,15 +402,13 @@ __global__ void k_absmax_it(const float* g, const float* h, long long n, IterDev
 
 // ---------------------------------------------------------------- tree init
 // Identity partition in buffer 0, gradients of class k copied beside the row
-// indices, fixed-point root sums, state machine reset (learner/mod.rs:580-628).
+// indices, fixed-point root sums (learner/mod.rs:580-628).
 __global__ void k_tree_init(long long n, const float* __restrict__ g, const float* __restrict__ h,
-                            unsigned* __restrict__ idx0, float2* __restrict__ gh0, IterDev* it, StepDev* st,
-                            LeafDev* leaves, SplitRec* best, uint8_t* splittable, int F, int L, double* best_gain,
-                            int* best_feat) {
-  int sg = fix_shift(n, it->max_abs_g_bits), sh = fix_shift(n, it->max_abs_h_bits);
+                            unsigned* __restrict__ idx0, float2* __restrict__ gh0, IterDev* it) {
+  const int sg = fix_shift(n, it->max_abs_g_bits), sh = fix_shift(n, it->max_abs_h_bits);
   long long ag = 0, ah = 0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    float gv = g[i], hv = h[i];
+    const float gv = g[i], hv = h[i];
     idx0[i] = (unsigned)i;
     gh0[i] = make_float2(gv, hv);
     ag += to_fixed(gv, sg);
@@ -422,39 +420,6 @@ __global__ void k_tree_init(long long n, const float* __restrict__ g, const floa
     atomicAdd((unsigned long long*)&it->root_g_fix, (unsigned long long)ag);
     atomicAdd((unsigned long long*)&it->root_h_fix, (unsigned long long)ah);
   }
-  if (blockIdx.x == 0) {
-    for (int l = threadIdx.x; l < L; l += blockDim.x) {
-      reset_rec(best[l]);
-      best_gain[l] = -INFINITY;
-      best_feat[l] = -1;
-    }
-    for (int f = threadIdx.x; f < F; f += blockDim.x) splittable[f] = 1;
-    if (threadIdx.x == 0) {
-      st->num_leaves = 1;
-      st->done = 0;
-      st->cur_leaf = 0;
-      st->new_leaf = -1;
-      st->smaller = 0;
-      st->larger = -1;
-      st->skip = 0;
-      st->is_root = 1;
-      st->left_total = 0;
-      st->num_splits = 0;
-      st->counter = 0;
-      st->counter2 = 0;
-      st->lcur = 0;
-      st->rcur = 0;
-      st->hist_shift_g = sg;
-      st->hist_shift_h = sh;
-      LeafDev& r = leaves[0];
-      r.begin = 0;
-      r.count = n;
-      r.buf = 0;
-      r.depth = 0;
-      r.output = 0.0;
-      r.int_sum = 0;
-    }
-  }
 }
 
 // Quantized form: grad/hess -> int8 by stochastic rounding (learner/quant.rs:92-129),
@@ -463,25 +428,24 @@ __global__ void k_discretize(long long n, const float* __restrict__ g, const flo
                              const double* __restrict__ grv, const double* __restrict__ hrv, int bins, int stochastic,
                              int constant_hessian, const int* __restrict__ starts, const int* call_ctr, IterDev* it,
                              unsigned* __restrict__ idx0, char2* __restrict__ q0, int8_t* __restrict__ gq_out,
-                             int8_t* __restrict__ hq_out, StepDev* st, LeafDev* leaves, SplitRec* best,
-                             uint8_t* splittable, int F, int L, double* best_gain, int* best_feat) {
-  double mg = (double)__uint_as_float(it->max_abs_g_bits);
-  double mh = (double)__uint_as_float(it->max_abs_h_bits);
-  double gs = mg / (double)(bins / 2);
-  double hs = constant_hessian ? mh : mh / (double)bins;
-  double inv_g = 1.0 / gs, inv_h = 1.0 / hs;
-  long long start = starts[*call_ctr];
+                             int8_t* __restrict__ hq_out) {
+  const double mg = (double)__uint_as_float(it->max_abs_g_bits);
+  const double mh = (double)__uint_as_float(it->max_abs_h_bits);
+  const double gs = mg / (double)(bins / 2);
+  const double hs = constant_hessian ? mh : mh / (double)bins;
+  const double inv_g = 1.0 / gs, inv_h = 1.0 / hs;
+  const long long start = starts[*call_ctr];
   long long ai = 0, agq = 0, ahq = 0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    double gv = (double)g[i];
+    const double gv = (double)g[i];
     double rg = 0.5, rh = 0.5;
     if (stochastic) {
-      long long pos = (i + start) % n;
+      const long long pos = (i + start) % n;
       rg = grv[pos];
       rh = hrv[pos];
     }
-    int8_t a = gv >= 0.0 ? to_i8(gv * inv_g + rg) : to_i8(gv * inv_g - rg);
-    int8_t b = constant_hessian ? (int8_t)1 : to_i8((double)h[i] * inv_h + rh);
+    const int8_t a = gv >= 0.0 ? to_i8(gv * inv_g + rg) : to_i8(gv * inv_g - rg);
+    const int8_t b = constant_hessian ? (int8_t)1 : to_i8((double)h[i] * inv_h + rh);
     idx0[i] = (unsigned)i;
     q0[i] = make_char2(a, b);
     gq_out[i] = a;
@@ -498,311 +462,10 @@ __global__ void k_discretize(long long n, const float* __restrict__ g, const flo
     atomicAdd((unsigned long long*)&it->root_gq_sum, (unsigned long long)agq);
     atomicAdd((unsigned long long*)&it->root_hq_sum, (unsigned long long)ahq);
   }
-  if (blockIdx.x == 0) {
-    for (int l = threadIdx.x; l < L; l += blockDim.x) {
-      reset_rec(best[l]);
-      best_gain[l] = -INFINITY;
-      best_feat[l] = -1;
-    }
-    for (int f = threadIdx.x; f < F; f += blockDim.x) splittable[f] = 1;
-    if (threadIdx.x == 0) {
-      it->grad_scale = gs;
-      it->hess_scale = hs;
-      it->start = (int)start;
-      st->num_leaves = 1;
-      st->done = 0;
-      st->cur_leaf = 0;
-      st->new_leaf = -1;
-      st->smaller = 0;
-      st->larger = -1;
-      st->skip = 0;
-      st->is_root = 1;
-      st->left_total = 0;
-      st->num_splits = 0;
-      st->counter = 0;
-      st->counter2 = 0;
-      st->lcur = 0;
-      st->rcur = 0;
-      LeafDev& r = leaves[0];
-      r.begin = 0;
-      r.count = n;
-      r.buf = 0;
-      r.depth = 0;
-      r.output = 0.0;
-    }
-  }
-}
-
-// ---------------------------------------------------------------- histograms
-// Row-wise histogram of the smaller leaf (multi_val_bin.rs:556-611), FP64
-// values as exact fixed-point int64: grid = (row chunks) x (feature groups);
-// each block keeps its feature group's [FB][B][2] int64 histogram in LDS
-// (ds_add_u64), then adds its nonzero bins into the leaf accumulator with
-// global 64-bit atomics (exact, so the result is order-free).
-//
-// Ordered bins: every leaf's rows sit contiguously in one of three buffers
-// (0 holds the root in row order; a leaf in buffer 1 splits into buffer 2 and
-// vice versa), each feature group as its own [N][FB]-byte array, and the
-// partition pass moves each row's bins with its index and gradients. A
-// histogram pass therefore streams its segment with no index indirection and
-// reads exactly its group's bytes.
-__device__ __forceinline__ int child_buf(int buf) { return buf == 1 ? 2 : 1; }
-template <typename T>
-__device__ __forceinline__ T sel3(T a0, T a1, T a2, int b) { return b == 0 ? a0 : (b == 1 ? a1 : a2); }
-struct HistArgs {
-  const uint8_t* grp[3];   // ordered bins per buffer, group-major [NG][N][FB]
-  const void* gh[3];       // float2 (plain) or char2 (quantized), per buffer
-  StepDev* st;
-  LeafDev* leaves;
-  const FeatDev* feats;
-  int F, B, FB, NG;        // features, max bins, features per group, groups
-  int L;                   // leaves per tree
-  long long N;
-  int max_chunks;
-  int min_rows_per_chunk;
-  long long* acc;          // [F][B][2] (plain) or [F][B] (quantized)
-  const IterDev* it;
-  SplitRec* cur;           // the split the partition pass just applied
-  SplitRec* log;
-  SplitRec* best;
-  double* best_gain;
-  int* best_feat;
-  SplitParamsDev p;
-  int quant;
-  int mode;                // 0: after a partition pass, 1: explicit segment, 2: the root
-  long long d_begin, d_count;
-  int d_buf, d_shift_g, d_shift_h;
-};
-
-// The rows a histogram launch sums. After a partition pass (mode 0) this also
-// applies the split to the leaf records (block 0; learner/mod.rs:1105-1225)
-// and decides before_find_best_split (learner/mod.rs:712-731): every block
-// derives the same answer from the step record, which no block rewrites.
-__device__ __forceinline__ bool hist_segment(const HistArgs& a, long long& begin, long long& count, int& buf, int& sg, int& sh) {
-  StepDev* st = a.st;
-  if (a.mode == 1) {
-    begin = a.d_begin; count = a.d_count; buf = a.d_buf; sg = a.d_shift_g; sh = a.d_shift_h;
-    return true;
-  }
-  if (st->done) return false;
-  if (a.mode == 2) {
-    begin = 0; count = a.N; buf = 0; sg = st->hist_shift_g; sh = st->hist_shift_h;
-    return true;
-  }
-  const long long pb = st->seg_begin, pc = st->seg_count;
-  const long long lc = (long long)st->lcur, rc = pc - lc;
-  const int nb = child_buf(st->seg_buf), depth = st->seg_depth + 1;
-  const int cur = st->cur_leaf, nl = st->new_leaf;
-  const bool skip = (a.p.max_depth > 0 && depth >= a.p.max_depth) ||
-                    (rc < 2LL * a.p.min_data_in_leaf && lc < 2LL * a.p.min_data_in_leaf);
-  const bool complete = nl + 1 >= a.L;
-  const long long sc = lc < rc ? lc : rc;
-  const int shg = a.quant ? 0 : fix_shift(sc, a.it->max_abs_g_bits);
-  const int shh = a.quant ? 0 : fix_shift(sc, a.it->max_abs_h_bits);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const SplitRec& s = *a.cur;
-    LeafDev& Lc = a.leaves[cur];
-    LeafDev& Ln = a.leaves[nl];
-    Ln.begin = pb + lc;
-    Ln.count = rc;
-    Ln.buf = nb;
-    Ln.depth = depth;
-    Ln.sum_g = s.rsg;
-    Ln.sum_h = s.rsh;
-    Ln.int_sum = s.rgh;
-    Ln.output = isnan(s.right_output) ? 0.0 : s.right_output;
-    Lc.begin = pb;
-    Lc.count = lc;
-    Lc.buf = nb;
-    Lc.depth = depth;
-    Lc.sum_g = s.lsg;
-    Lc.sum_h = s.lsh;
-    Lc.int_sum = s.lgh;
-    Lc.output = isnan(s.left_output) ? 0.0 : s.left_output;
-    SplitRec* lr = a.log + st->num_splits;
-    const unsigned* sw = (const unsigned*)a.cur;
-    unsigned* dw = (unsigned*)lr;
-    for (int w = 0; w < (int)(sizeof(SplitRec) / 4); ++w) dw[w] = sw[w];
-    lr->left_count = (int)lc;
-    lr->right_count = (int)rc;
-    lr->leaf = cur;
-    st->num_splits += 1;
-    st->num_leaves = nl + 1;
-    st->left_total = lc;
-    st->is_root = 0;
-    if (lc < rc) { st->smaller = cur; st->larger = nl; }
-    else { st->smaller = nl; st->larger = cur; }
-    st->skip = skip ? 1 : 0;
-    st->hist_shift_g = shg;
-    st->hist_shift_h = shh;
-    if (skip) {
-      a.best[cur].gain = -INFINITY;
-      a.best[nl].gain = -INFINITY;
-      a.best_gain[cur] = -INFINITY;
-      a.best_gain[nl] = -INFINITY;
-      a.best_feat[cur] = a.best[cur].feature;
-      a.best_feat[nl] = a.best[nl].feature;
-    }
-  }
-  if (skip || complete) return false;
-  begin = lc < rc ? pb : pb + lc;
-  count = sc;
-  buf = nb;
-  sg = shg;
-  sh = shh;
-  return true;
-}
-
-// rows each thread has in flight per pass (all loads issued before the atomics)
-constexpr int kHistUnroll = 8;
-
-template <int TPR>
-__global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
-  extern __shared__ unsigned long long lds[];
-  long long begin, count;
-  int buf, sg, sh;
-  if (!hist_segment(a, begin, count, buf, sg, sh)) return;
-  const int fg = blockIdx.x % a.NG;
-  const int chunk = blockIdx.x / a.NG;
-  long long want = (count + a.min_rows_per_chunk - 1) / a.min_rows_per_chunk;
-  int nchunks = (int)min((long long)a.max_chunks, max(want, 1LL));
-  if (chunk >= nchunks) return;
-  long long per = (count + nchunks - 1) / nchunks;
-  long long r0 = (long long)chunk * per, r1 = min(count, r0 + per);
-  if (r0 >= r1) return;
-  const int f0 = fg * a.FB;
-  const int FBg = min(a.FB, a.F - f0);
-  const int B = a.B;
-  const int n_lds = FBg * B * 2;
-  for (int e = threadIdx.x; e < n_lds; e += blockDim.x) lds[e] = 0ull;
-  // most-frequent bin of each of this thread's 4 features (never accumulated)
-  const int sub = threadIdx.x / TPR, j = threadIdx.x % TPR;
-  int mfb[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int fl = j * 4 + k;
-    mfb[k] = fl < FBg ? a.feats[f0 + fl].mfb : 256;
-  }
-  __syncthreads();
-  const int rpp = 256 / TPR;
-  // select, not index: a runtime index into a kernel-argument array makes
-  // the compiler copy the arguments to scratch memory
-  const uint8_t* __restrict__ gb =
-      sel3(a.grp[0], a.grp[1], a.grp[2], buf) + ((size_t)fg * a.N + begin) * a.FB + j * 4;
-  const float2* __restrict__ gh = (const float2*)sel3(a.gh[0], a.gh[1], a.gh[2], buf) + begin;
-  const bool active = j * 4 < FBg;
-  for (long long base = r0; base < r1; base += kHistUnroll * rpp) {
-    unsigned w[kHistUnroll];
-    long long gi[kHistUnroll], hi[kHistUnroll];
-    bool ok[kHistUnroll];
-#pragma unroll
-    for (int u = 0; u < kHistUnroll; ++u) {
-      long long p = base + u * rpp + sub;
-      ok[u] = active && p < r1;
-      if (ok[u]) {
-        const float2 v = gh[p];
-        w[u] = *(const unsigned*)(gb + (size_t)p * a.FB);
-        gi[u] = to_fixed(v.x, sg);
-        hi[u] = to_fixed(v.y, sh);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kHistUnroll; ++u) {
-      if (!ok[u]) continue;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        int b = (w[u] >> (8 * k)) & 0xFF;
-        int fl = j * 4 + k;
-        if (fl < FBg && b != mfb[k]) {
-          unsigned long long* c = lds + ((size_t)fl * B + b) * 2;
-          atomicAdd(c, (unsigned long long)gi[u]);
-          atomicAdd(c + 1, (unsigned long long)hi[u]);
-        }
-      }
-    }
-  }
-  __syncthreads();
-  unsigned long long* out = (unsigned long long*)a.acc + (size_t)f0 * B * 2;
-  for (int e = threadIdx.x; e < n_lds; e += blockDim.x) {
-    unsigned long long v = lds[e];
-    if (v) atomicAdd(out + e, v);
-  }
-}
-
-// Quantized form (multi_val_bin.rs:710-791): int8 (g, h) pairs summed in LDS
-// as int32 g and uint32 h, flushed as packed (g << 32) + h into the int64
-// accumulator with wrapping global atomics. Equal to the reference's wrapping
-// sum of packed rows for any order.
-template <int TPR>
-__global__ void __launch_bounds__(256) k_hist_int(HistArgs a) {
-  extern __shared__ unsigned lds32[];
-  long long begin, count;
-  int buf, sg_unused, sh_unused;
-  if (!hist_segment(a, begin, count, buf, sg_unused, sh_unused)) return;
-  const int fg = blockIdx.x % a.NG;
-  const int chunk = blockIdx.x / a.NG;
-  long long want = (count + a.min_rows_per_chunk - 1) / a.min_rows_per_chunk;
-  int nchunks = (int)min((long long)a.max_chunks, max(want, 1LL));
-  if (chunk >= nchunks) return;
-  long long per = (count + nchunks - 1) / nchunks;
-  long long r0 = (long long)chunk * per, r1 = min(count, r0 + per);
-  if (r0 >= r1) return;
-  const int f0 = fg * a.FB;
-  const int FBg = min(a.FB, a.F - f0);
-  const int B = a.B;
-  const int n_lds = FBg * B * 2;
-  for (int e = threadIdx.x; e < n_lds; e += blockDim.x) lds32[e] = 0u;
-  const int sub = threadIdx.x / TPR, j = threadIdx.x % TPR;
-  int mfb[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int fl = j * 4 + k;
-    mfb[k] = fl < FBg ? a.feats[f0 + fl].mfb : 256;
-  }
-  __syncthreads();
-  const int rpp = 256 / TPR;
-  const uint8_t* __restrict__ gb =
-      sel3(a.grp[0], a.grp[1], a.grp[2], buf) + ((size_t)fg * a.N + begin) * a.FB + j * 4;
-  const char2* __restrict__ q = (const char2*)sel3(a.gh[0], a.gh[1], a.gh[2], buf) + begin;
-  const bool active = j * 4 < FBg;
-  for (long long base = r0; base < r1; base += kHistUnroll * rpp) {
-    unsigned w[kHistUnroll];
-    int gq[kHistUnroll], hq[kHistUnroll];
-    bool ok[kHistUnroll];
-#pragma unroll
-    for (int u = 0; u < kHistUnroll; ++u) {
-      long long p = base + u * rpp + sub;
-      ok[u] = active && p < r1;
-      if (ok[u]) {
-        const char2 v = q[p];
-        w[u] = *(const unsigned*)(gb + (size_t)p * a.FB);
-        gq[u] = v.x;
-        hq[u] = (int)(uint8_t)v.y;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kHistUnroll; ++u) {
-      if (!ok[u]) continue;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        int b = (w[u] >> (8 * k)) & 0xFF;
-        int fl = j * 4 + k;
-        if (fl < FBg && b != mfb[k]) {
-          unsigned* c = lds32 + ((size_t)fl * B + b) * 2;
-          atomicAdd(c, (unsigned)gq[u]);
-          atomicAdd(c + 1, (unsigned)hq[u]);
-        }
-      }
-    }
-  }
-  __syncthreads();
-  unsigned long long* out = (unsigned long long*)a.acc + (size_t)f0 * B;
-  for (int e = threadIdx.x; e < FBg * B; e += blockDim.x) {
-    unsigned gsum = lds32[2 * e], hsum = lds32[2 * e + 1];
-    if (gsum | hsum) {
-      unsigned long long packed = ((unsigned long long)(long long)(int)gsum << 32) + (unsigned long long)hsum;
-      atomicAdd(out + e, packed);
-    }
+    it->grad_scale = gs;
+    it->hess_scale = hs;
+    it->start = (int)start;
   }
 }
 
@@ -1260,257 +923,7 @@ __device__ __forceinline__ bool search_int(const long long* hv, const FeatDev& f
   return spl;
 }
 
-struct SplitArgs {
-  StepDev* st;
-  LeafDev* leaves;
-  const FeatDev* feats;
-  SplitParamsDev p;
-  int F, B, L;
-  long long* acc;        // smaller-leaf accumulator (zeroed after reading)
-  double* pool;          // [L][F][B][2] FP64 histograms
-  long long* ipool;      // [L][F][B] packed histograms (quantized)
-  SplitRec* feat_split;  // [2][F]
-  SplitRec* best;        // [L]
-  uint8_t* splittable;   // [L][F]
-  SplitRec* cur;         // the split to apply next
-  SplitRec* log;         // [L-1] chosen splits in order
-  IterDev* it;
-  int quant;
-};
-
-// The leaf sums a search needs (leaf_splits.hpp; the root's come from the
-// fixed-point / packed root sums).
-__device__ __forceinline__ LeafSums leaf_sums(const SplitArgs& a, int leaf, bool root) {
-  LeafSums s;
-  const LeafDev& L = a.leaves[leaf];
-  s.nd = (int)L.count;
-  s.isum = L.int_sum;
-  s.gs = a.it->grad_scale;
-  s.hs = a.it->hess_scale;
-  double sg = L.sum_g, sh = L.sum_h;
-  if (root) {
-    if (a.quant) {
-      s.isum = a.it->root_int_sum;
-      sg = (double)a.it->root_gq_sum * s.gs;
-      sh = (double)a.it->root_hq_sum * s.hs;
-    } else {
-      sg = ldexp((double)a.it->root_g_fix, -fix_shift(L.count, a.it->max_abs_g_bits));
-      sh = ldexp((double)a.it->root_h_fix, -fix_shift(L.count, a.it->max_abs_h_bits));
-    }
-  }
-  s.sg = sg;
-  s.sh = sh + 2.0 * KEPS;
-  // learner/mod.rs:1303-1309: the root's parent output is its own output
-  s.po = root ? leaf_output_t(sg, sh, a.p, true, true, false, s.nd, 0.0) : L.output;
-  return s;
-}
-
-// Threshold search for the smaller and larger leaf of this step, one block
-// (2 waves) per feature; the last block picks each leaf's best split and the
-// next leaf to split (learner/mod.rs:954-1101, 698-706).
-__global__ void __launch_bounds__(128) k_split(SplitArgs a) {
-  StepDev* st = a.st;
-  if (st->done) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // the next partition pass reserves slots from zero
-    st->lcur = 0;
-    st->rcur = 0;
-  }
-  if (st->num_leaves >= a.L) return;
-  __shared__ double hsm[2][kMaxBins];   // smaller: g, h
-  __shared__ double hlg[2][kMaxBins];   // larger: g, h
-  __shared__ long long ism[kMaxBins], ilg[kMaxBins];
-  __shared__ ScanLds scr[2];
-  __shared__ SplitRec res[2];
-  const int f = blockIdx.x;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const bool root = st->is_root != 0;
-  const int sm = st->smaller, lg = root ? -1 : st->larger, cur = st->cur_leaf;
-  const int F = a.F, B = a.B;
-  if (!st->skip) {
-    // every load this block needs is independent once the step record is
-    // known: issue them together (each is a cache miss after the previous
-    // kernel's writes), then work from registers and LDS
-    const FeatDev fd = a.feats[f];
-    const bool parent_spl = root ? true : a.splittable[(size_t)cur * F + f] != 0;
-    const LeafSums ls_s = leaf_sums(a, sm, root);
-    const LeafSums ls_l = lg >= 0 ? leaf_sums(a, lg, false) : ls_s;
-    constexpr int PB = kMaxBins / 128;   // bins per thread
-    if (!a.quant) {
-      const int ssg = root ? fix_shift(a.leaves[sm].count, a.it->max_abs_g_bits) : st->hist_shift_g;
-      const int ssh = root ? fix_shift(a.leaves[sm].count, a.it->max_abs_h_bits) : st->hist_shift_h;
-      long long* acc = a.acc + (size_t)f * B * 2;
-      const double* pp = lg >= 0 ? a.pool + ((size_t)cur * F + f) * B * 2 : nullptr;   // parent
-      long long gi[PB], hi[PB];
-      double pg[PB], ph[PB];
-#pragma unroll
-      for (int k = 0; k < PB; ++k) {
-        const int b = tid + 128 * k;
-        gi[k] = hi[k] = 0;
-        pg[k] = ph[k] = 0.0;
-        if (b < B) {
-          gi[k] = acc[2 * b];
-          hi[k] = acc[2 * b + 1];
-          if (pp) {
-            pg[k] = pp[2 * b];
-            ph[k] = pp[2 * b + 1];
-          }
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < PB; ++k) {
-        const int b = tid + 128 * k;
-        if (b < B) {
-          acc[2 * b] = 0;
-          acc[2 * b + 1] = 0;
-          hsm[0][b] = ldexp((double)gi[k], -ssg);
-          hsm[1][b] = ldexp((double)hi[k], -ssh);
-        }
-      }
-      __syncthreads();
-      // histogram.rs:42-56: the most frequent bin from the leaf totals, in
-      // the reference's order (two sequential chains)
-      if (fd.mfb > 0 && tid < 2) {
-        double v = tid == 0 ? ls_s.sg : ls_s.sh - 2.0 * KEPS;
-        const double* __restrict__ arr = hsm[tid];
-        int i = 0;
-        for (; i + 8 <= fd.num_bin; i += 8) {
-          double x[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) x[u] = arr[i + u];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) v = (i + u == fd.mfb) ? v : v - x[u];
-        }
-        for (; i < fd.num_bin; ++i) v = (i == fd.mfb) ? v : v - arr[i];
-        hsm[tid][fd.mfb] = v;
-      }
-      __syncthreads();
-      double* ps = a.pool + ((size_t)sm * F + f) * B * 2;
-      double* pl = lg >= 0 ? a.pool + ((size_t)lg * F + f) * B * 2 : nullptr;
-#pragma unroll
-      for (int k = 0; k < PB; ++k) {
-        const int b = tid + 128 * k;
-        if (b < B) {
-          const double sgv = hsm[0][b], shv = hsm[1][b];
-          ps[2 * b] = sgv;
-          ps[2 * b + 1] = shv;
-          if (pl) {
-            const double lgv = pg[k] - sgv, lhv = ph[k] - shv;
-            hlg[0][b] = lgv;
-            hlg[1][b] = lhv;
-            pl[2 * b] = lgv;
-            pl[2 * b + 1] = lhv;
-          }
-        }
-      }
-    } else {
-      long long* acc = a.acc + (size_t)f * B;
-      const long long* pp = lg >= 0 ? a.ipool + ((size_t)cur * F + f) * B : nullptr;
-      long long v[PB], pv[PB];
-      unsigned long long part = 0;
-#pragma unroll
-      for (int k = 0; k < PB; ++k) {
-        const int b = tid + 128 * k;
-        v[k] = pv[k] = 0;
-        if (b < B) {
-          v[k] = acc[b];
-          if (pp) pv[k] = pp[b];
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < PB; ++k) {
-        const int b = tid + 128 * k;
-        if (b < B) {
-          acc[b] = 0;
-          ism[b] = v[k];
-          if (b != fd.mfb && b < fd.num_bin) part += (unsigned long long)v[k];
-        }
-      }
-      // histogram.rs:67-78: wrapping sums are exact in any order, so the
-      // most frequent bin is the leaf total minus a block reduction
-      for (int o = 32; o > 0; o >>= 1) part += (unsigned long long)__shfl_xor((long long)part, o, 64);
-      __shared__ unsigned long long wpart[2];
-      if (lane == 0) wpart[wave] = part;
-      __syncthreads();
-      if (fd.mfb > 0 && tid == 0) ism[fd.mfb] = (long long)((unsigned long long)ls_s.isum - wpart[0] - wpart[1]);
-      __syncthreads();
-      long long* ps = a.ipool + ((size_t)sm * F + f) * B;
-      long long* pl = lg >= 0 ? a.ipool + ((size_t)lg * F + f) * B : nullptr;
-#pragma unroll
-      for (int k = 0; k < PB; ++k) {
-        const int b = tid + 128 * k;
-        if (b < B) {
-          const long long sv = ism[b];
-          ps[b] = sv;
-          if (pl) {
-            const long long lv = (long long)((unsigned long long)pv[k] - (unsigned long long)sv);
-            ilg[b] = lv;
-            pl[b] = lv;
-          }
-        }
-      }
-    }
-    __syncthreads();
-    // wave 0 searches the smaller leaf, wave 1 the larger
-    const int leaf = wave == 0 ? sm : lg;
-    if (leaf >= 0) {
-      const LeafSums ls = wave == 0 ? ls_s : ls_l;
-      SplitRec& r = res[wave];
-      if (lane == 0) reset_rec(r);
-      bool spl = false;
-      wave_sync();
-      if (parent_spl) {
-        if (lane == 0) r.gain = -INFINITY;
-        if (a.quant) {
-          spl = search_int(wave == 0 ? ism : ilg, fd, a.p, ls, scr[wave], r);
-        } else if (fd.bin_type == 1) {
-          spl = search_categorical(wave == 0 ? hsm[0] : hlg[0], wave == 0 ? hsm[1] : hlg[1], fd, a.p, ls, scr[wave], r);
-        } else {
-          spl = search_numerical(wave == 0 ? hsm[0] : hlg[0], wave == 0 ? hsm[1] : hlg[1], fd, a.p, ls, scr[wave], r);
-        }
-      }
-      if (lane == 0) {
-        if (parent_spl) r.feature = f;
-        a.feat_split[wave * F + f] = r;
-        a.splittable[(size_t)leaf * F + f] = spl ? 1 : 0;
-      }
-    }
-  }
-  if (root && blockIdx.x == 0 && tid == 0) {
-    // learner/mod.rs:647: the root's output before any split
-    LeafSums r = leaf_sums(a, 0, true);
-    st->root_output = round0(leaf_output_t(r.sg, r.sh - 2.0 * KEPS, a.p, true, true, false, r.nd, 0.0));
-  }
-}
-
 // ---------------------------------------------------------------- partition
-struct PartArgs {
-  StepDev* st;
-  LeafDev* leaves;
-  const FeatDev* feats;
-  uint8_t* grp[3];          // ordered bins per buffer, group-major [NG][N][FB]
-  int FB, NG;               // bytes (features) per group, groups
-  long long N;
-  unsigned* idx[3];
-  void* gh[3];
-  int gh_bytes;             // 8 (float2) or 2 (char2)
-  int* tile_left;           // per-tile left counts
-  const SplitRec* feat_split;  // [2][F] searches of the last step
-  SplitRec* cur;
-  SplitRec* best;
-  double* best_gain;        // [L] compact copies of best[].gain / .feature
-  int* best_feat;
-  SplitRec* log;
-  SplitParamsDev p;
-  int F, L;
-  const IterDev* it;
-  int quant;
-};
-
-constexpr int kPartThreads = 256;
-constexpr int kPartRowsPerThread = 4;
-constexpr int kPartTile = kPartThreads * kPartRowsPerThread;
-
 // learner/mod.rs:792-810 goes_left / goes_left_categorical on a bin.
 __device__ __forceinline__ bool goes_left(int b, const FeatDev& fd, const SplitRec& s) {
   if (fd.bin_type == 1) {
@@ -1521,369 +934,8 @@ __device__ __forceinline__ bool goes_left(int b, const FeatDev& fd, const SplitR
     }
     return (s.cat_bits[q >> 5] >> (q & 31)) & 1u;
   }
-  bool miss = (fd.missing_type == 1 && b == fd.default_bin) || (fd.missing_type == 2 && b == fd.num_bin - 1);
+  const bool miss = (fd.missing_type == 1 && b == fd.default_bin) || (fd.missing_type == 2 && b == fd.num_bin - 1);
   return miss ? s.default_left != 0 : b <= s.threshold;
-}
-
-// A candidate of the split decision: gain, feature (-1 none) and position
-// (feature slot or leaf). pick() keeps the better one in the order the
-// reference's scans imply - higher gain, then smaller feature
-// (learner/mod.rs:1064-1101), then the earlier position (first max,
-// learner/mod.rs:698-706). The order is total, so any reduction tree gives
-// the same answer.
-struct Cand {
-  double g;
-  int f, l;
-};
-
-__device__ __forceinline__ Cand pick(Cand a, Cand b) {
-  const bool take = b.l != INT32_MAX &&
-                    (a.l == INT32_MAX || better(b.g, b.f, a.g, a.f) || (!better(a.g, a.f, b.g, b.f) && b.l < a.l));
-  // field-wise selects keep the candidates in registers
-  Cand r;
-  r.g = take ? b.g : a.g;
-  r.f = take ? b.f : a.f;
-  r.l = take ? b.l : a.l;
-  return r;
-}
-
-__device__ __forceinline__ Cand wave_pick(Cand c) {
-  for (int o = 32; o > 0; o >>= 1) {
-    Cand d;
-    d.g = __shfl_xor(c.g, o, 64);
-    d.f = __shfl_xor(c.f, o, 64);
-    d.l = __shfl_xor(c.l, o, 64);
-    c = pick(c, d);
-  }
-  return c;
-}
-
-// One pass per split. Every block first finishes the last step (each
-// searched leaf's best split, then ArgMax over all leaves) - the same result
-// in every block, so no block waits for another - then moves its tiles'
-// (row, gradient) pairs to the other buffer: left rows from the front of the
-// leaf's segment, right rows from its back, each tile reserving its slots
-// with one atomic per side (DataPartition::split, partition.rs:107-174). The
-// row order inside a child is unspecified: every consumer of a segment (exact
-// fixed-point or integer histograms, renewal sums, score update) is
-// order-independent. Block 0 publishes the decision for later kernels.
-__global__ void __launch_bounds__(kPartThreads) k_partition(PartArgs a) {
-  StepDev* st = a.st;
-  constexpr int NW = kPartThreads / 64;
-  constexpr int RW = (int)(sizeof(SplitRec) / 4);
-  static_assert(RW <= kPartThreads, "a split record is copied one word per thread");
-  __shared__ Cand sc[3][NW];
-  __shared__ int sdone;
-  __shared__ SplitRec chosen;
-  const int tid = threadIdx.x, lane = tid & 63;
-  // The step record, this step's per-feature searches and every leaf's best
-  // split are independent loads: issue them together.
-  const int done = st->done, root = st->is_root, skipped = st->skip;
-  const int lv0 = st->smaller, lv1 = root ? -1 : st->larger, nleaves = st->num_leaves;
-  Cand c0, c1, cl;
-  c0.g = c1.g = cl.g = -INFINITY;
-  c0.f = c1.f = cl.f = -1;
-  c0.l = c1.l = cl.l = INT32_MAX;
-  for (int k = tid; k < a.F; k += kPartThreads) {
-    const SplitRec& r0 = a.feat_split[k];
-    const SplitRec& r1 = a.feat_split[a.F + k];
-    c0 = pick(c0, Cand{r0.gain, r0.feature, k});
-    c1 = pick(c1, Cand{r1.gain, r1.feature, k});
-  }
-  const bool searched = skipped == 0;
-  for (int l = tid; l < a.L; l += kPartThreads) {
-    const double g = a.best_gain[l];
-    const int f = a.best_feat[l];
-    if (searched && (l == lv0 || l == lv1)) continue;   // replaced by this step's results
-    cl = pick(cl, Cand{g, f, l});
-  }
-  c0 = wave_pick(c0);
-  c1 = wave_pick(c1);
-  cl = wave_pick(cl);
-  if (lane == 0) {
-    sc[0][tid >> 6] = c0;
-    sc[1][tid >> 6] = c1;
-    sc[2][tid >> 6] = cl;
-  }
-  if (tid == 0) sdone = done;
-  __syncthreads();
-  if (sdone) return;
-  c0 = sc[0][0];
-  c1 = sc[1][0];
-  cl = sc[2][0];
-  for (int w = 1; w < NW; ++w) {
-    c0 = pick(c0, sc[0][w]);
-    c1 = pick(c1, sc[1][w]);
-    cl = pick(cl, sc[2][w]);
-  }
-  // a searched leaf without any split candidate carries (-inf, none)
-  const int fw0 = searched && better(c0.g, c0.f, -INFINITY, -1) ? c0.l : -1;
-  const int fw1 = searched && lv1 >= 0 && better(c1.g, c1.f, -INFINITY, -1) ? c1.l : -1;
-  Cand s0, s1;
-  s0.g = fw0 >= 0 ? c0.g : -INFINITY;
-  s0.f = fw0 >= 0 ? c0.f : -1;
-  s0.l = lv0;
-  s1.g = fw1 >= 0 ? c1.g : -INFINITY;
-  s1.f = fw1 >= 0 ? c1.f : -1;
-  s1.l = lv1;
-  Cand best = cl;
-  if (searched) {
-    best = pick(best, s0);
-    if (lv1 >= 0) best = pick(best, s1);
-  }
-  const int best_leaf = best.l;
-  const bool stop = !(best.g > 0.0) || nleaves >= a.L;
-  const int w = (searched && best_leaf == lv0) ? 0 : (searched && lv1 >= 0 && best_leaf == lv1) ? 1 : -1;
-  // second round of loads: the chosen record, its feature and its leaf
-  if (!stop && tid < RW) {
-    const SplitRec* src_rec = w == 0 ? &a.feat_split[fw0] : w == 1 ? &a.feat_split[a.F + fw1] : &a.best[best_leaf];
-    ((unsigned*)&chosen)[tid] = ((const unsigned*)src_rec)[tid];
-  }
-  if (blockIdx.x == 0 && searched) {
-    // the searched leaves' best splits become the leaves' records
-#pragma unroll
-    for (int v = 0; v < 2; ++v) {
-      const int leaf = v == 0 ? lv0 : lv1;
-      const int fw = v == 0 ? fw0 : fw1;
-      if (leaf < 0) continue;
-      if (fw >= 0) {
-        if (tid < RW) ((unsigned*)&a.best[leaf])[tid] = ((const unsigned*)&a.feat_split[v * a.F + fw])[tid];
-      } else if (tid == 0) {
-        SplitRec r;
-        reset_rec(r);
-        a.best[leaf] = r;
-      }
-      if (tid == 0) {
-        a.best_gain[leaf] = v == 0 ? s0.g : s1.g;
-        a.best_feat[leaf] = v == 0 ? s0.f : s1.f;
-      }
-    }
-  }
-  if (stop) {
-    if (blockIdx.x == 0 && tid == 0) st->done = 1;
-    return;
-  }
-  const FeatDev fd = a.feats[best.f];
-  const LeafDev& Lc = a.leaves[best_leaf];
-  const long long begin = Lc.begin, count = Lc.count;
-  const int lbuf = Lc.buf;
-  __syncthreads();
-  if (blockIdx.x == 0) {
-    if (tid < RW) ((unsigned*)a.cur)[tid] = ((const unsigned*)&chosen)[tid];
-    if (tid == 0) {
-      st->cur_leaf = best_leaf;
-      st->new_leaf = nleaves;
-      st->seg_begin = begin;
-      st->seg_count = count;
-      st->seg_buf = lbuf;
-      st->seg_depth = Lc.depth;
-    }
-  }
-  const SplitRec& s = chosen;
-  const int src = lbuf, dst = child_buf(lbuf);   // buffer 0 is never a destination
-  const int FB = a.FB, gb = a.gh_bytes;
-  const size_t gstride = (size_t)a.N * FB;       // bytes of one feature group's array
-  const unsigned* idx = sel3(a.idx[0], a.idx[1], a.idx[2], src) + begin;
-  unsigned* oidx = sel3(a.idx[0], a.idx[1], a.idx[2], dst) + begin;
-  const uint8_t* igh = (const uint8_t*)sel3(a.gh[0], a.gh[1], a.gh[2], src) + (size_t)begin * gb;
-  uint8_t* ogh = (uint8_t*)sel3(a.gh[0], a.gh[1], a.gh[2], dst) + (size_t)begin * gb;
-  const uint8_t* gsrc = sel3(a.grp[0], a.grp[1], a.grp[2], src) + (size_t)begin * FB;
-  uint8_t* gdst = sel3(a.grp[0], a.grp[1], a.grp[2], dst) + (size_t)begin * FB;
-  const uint8_t* scol = gsrc + (size_t)(s.feature / FB) * gstride + (s.feature % FB);
-  const int words = FB / 4, tw = a.NG * words;   // 4-byte words of one row over all groups
-  const int ntiles = (int)((count + kPartTile - 1) / kPartTile);
-  constexpr int R = kPartRowsPerThread;
-  constexpr int CW = 8;                          // row words moved per round
-  __shared__ int cnt[2][R][NW];
-  __shared__ long long slot[3];
-  const int wave = tid >> 6;
-  const unsigned long long below = (1ull << lane) - 1ull;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const long long t0 = (long long)t * kPartTile;
-    // one round of independent loads: index, gradients, the split feature's
-    // bin and the first words of every row (lanes take consecutive rows)
-    bool valid[R], left[R];
-    unsigned rid[R];
-    uint2 gv[R];
-    unsigned v[R][CW];
-#pragma unroll
-    for (int u = 0; u < R; ++u) {
-      const long long p = t0 + u * kPartThreads + tid;
-      valid[u] = p < count;
-      left[u] = false;
-      if (valid[u]) {
-        rid[u] = idx[p];
-        if (gb == 8) gv[u] = ((const uint2*)igh)[p];
-        else gv[u].x = ((const unsigned short*)igh)[p];
-        left[u] = goes_left(scol[(size_t)p * FB], fd, s);
-#pragma unroll
-        for (int i = 0; i < CW; ++i)
-          if (i < tw) v[u][i] = *(const unsigned*)(gsrc + (size_t)(i / words) * gstride + (size_t)p * FB + 4 * (i % words));
-      }
-    }
-    // ranks inside each 64-row slice, slice counts to LDS
-    int rk[R];
-#pragma unroll
-    for (int u = 0; u < R; ++u) {
-      const unsigned long long lm = __ballot(left[u]);
-      const unsigned long long rm = __ballot(valid[u] && !left[u]);
-      rk[u] = (int)__popcll((left[u] ? lm : rm) & below);
-      if (lane == 0) {
-        cnt[0][u][wave] = (int)__popcll(lm);
-        cnt[1][u][wave] = (int)__popcll(rm);
-      }
-    }
-    __syncthreads();
-    if (tid == 0) {
-      int tl = 0, tr = 0;
-      for (int u = 0; u < R; ++u)
-        for (int w = 0; w < NW; ++w) {
-          tl += cnt[0][u][w];
-          tr += cnt[1][u][w];
-        }
-      slot[0] = (long long)atomicAdd(&st->lcur, (unsigned long long)tl);
-      slot[1] = (long long)atomicAdd(&st->rcur, (unsigned long long)tr);
-      slot[2] = tr;
-    }
-    __syncthreads();
-    // destinations: left rows fill from the front, right rows from the back;
-    // inside a tile the order is slice by slice, so stores of a slice are
-    // contiguous
-    long long d[R];
-    int pl = 0, pr = 0;
-#pragma unroll
-    for (int u = 0; u < R; ++u) {
-      int bl = pl, br = pr;
-      for (int w = 0; w < NW; ++w) {
-        if (w < wave) {
-          bl += cnt[0][u][w];
-          br += cnt[1][u][w];
-        }
-        pl += cnt[0][u][w];
-        pr += cnt[1][u][w];
-      }
-      d[u] = left[u] ? slot[0] + bl + rk[u] : count - slot[1] - slot[2] + br + rk[u];
-    }
-#pragma unroll
-    for (int u = 0; u < R; ++u) {
-      if (!valid[u]) continue;
-      oidx[d[u]] = rid[u];
-      if (gb == 8) ((uint2*)ogh)[d[u]] = gv[u];
-      else ((unsigned short*)ogh)[d[u]] = (unsigned short)gv[u].x;
-#pragma unroll
-      for (int i = 0; i < CW; ++i)
-        if (i < tw) *(unsigned*)(gdst + (size_t)(i / words) * gstride + (size_t)d[u] * FB + 4 * (i % words)) = v[u][i];
-    }
-    // rows wider than CW words: the rest, CW words per round
-    for (int k0 = CW; k0 < tw; k0 += CW) {
-#pragma unroll
-      for (int u = 0; u < R; ++u) {
-        if (!valid[u]) continue;
-        const long long p = t0 + u * kPartThreads + tid;
-#pragma unroll
-        for (int i = 0; i < CW; ++i) {
-          const int k = k0 + i;
-          if (k < tw) v[u][i] = *(const unsigned*)(gsrc + (size_t)(k / words) * gstride + (size_t)p * FB + 4 * (k % words));
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < R; ++u) {
-        if (!valid[u]) continue;
-#pragma unroll
-        for (int i = 0; i < CW; ++i) {
-          const int k = k0 + i;
-          if (k < tw) *(unsigned*)(gdst + (size_t)(k / words) * gstride + (size_t)d[u] * FB + 4 * (k % words)) = v[u][i];
-        }
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// Rows of the row-major bins (stride S) into the group-major layout of the
-// ordered buffers: out[g][i][0..FB) = bins of features g*FB.. of row
-// rows[i] (or row i when rows is null), zero past the last feature.
-__global__ void k_group_rows(const unsigned* rows, long long m, const uint8_t* __restrict__ rm, int S, int F, int FB,
-                             int NG, long long N, uint8_t* __restrict__ out) {
-  const int words = FB / 4;
-  const long long total = m * NG * words;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
-    const long long i = e / (NG * words);
-    const int k = (int)(e % (NG * words));
-    const int g = k / words, w = k % words;
-    const long long r = rows ? (long long)rows[i] : i;
-    unsigned v = 0;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int f = g * FB + w * 4 + b;
-      if (f < F) v |= (unsigned)rm[r * S + f] << (8 * b);
-    }
-    *(unsigned*)(out + (size_t)g * N * FB + (size_t)i * FB + 4 * w) = v;
-  }
-}
-
-// ---------------------------------------------------------------- tree end
-// Quantized form, learner/mod.rs:736-747: each leaf's output from its float
-// gradient sums (exact fixed-point per leaf, then refit_leaf_output).
-__global__ void k_renew_sum(const StepDev* st, const LeafDev* leaves, const unsigned* idx0, const unsigned* idx1, const unsigned* idx2, const float* g,
-                            const float* h, const IterDev* it, long long* sums /* [L][2] */) {
-  const int l = blockIdx.y;
-  if (l >= st->num_leaves) return;
-  const LeafDev& L = leaves[l];
-  const unsigned* idx = sel3(idx0, idx1, idx2, L.buf) + L.begin;
-  const int sg = fix_shift(L.count, it->max_abs_g_bits), sh = fix_shift(L.count, it->max_abs_h_bits);
-  long long ag = 0, ah = 0;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < L.count; p += (long long)gridDim.x * blockDim.x) {
-    unsigned r = idx[p];
-    ag += to_fixed(g[r], sg);
-    ah += to_fixed(h[r], sh);
-  }
-  ag = wave_sum(ag);
-  ah = wave_sum(ah);
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd((unsigned long long*)&sums[2 * l], (unsigned long long)ag);
-    atomicAdd((unsigned long long*)&sums[2 * l + 1], (unsigned long long)ah);
-  }
-}
-
-// Shrunk leaf values (tree.rs:299-319): out[l] = round0(output * lr); with
-// `renew`, output = round0(refit_leaf_output(sum g, sum h)).
-__global__ void k_leaf_values(const StepDev* st, LeafDev* leaves, const IterDev* it, long long* renew_sums, int renew,
-                              SplitParamsDev p, double lr, int L, double* out) {
-  const int l = threadIdx.x + blockIdx.x * blockDim.x;
-  if (l >= L) return;
-  if (l >= st->num_leaves) { out[l] = 0.0; return; }
-  LeafDev& Lf = leaves[l];
-  if (renew) {
-    const int sg = fix_shift(Lf.count, it->max_abs_g_bits), sh = fix_shift(Lf.count, it->max_abs_h_bits);
-    double g = ldexp((double)renew_sums[2 * l], -sg), h = ldexp((double)renew_sums[2 * l + 1], -sh);
-    renew_sums[2 * l] = 0;
-    renew_sums[2 * l + 1] = 0;
-    Lf.output = round0(leaf_output_t(g, h, p, true, true, false, (int)Lf.count, 0.0));
-  }
-  out[l] = round0(Lf.output * lr);
-}
-
-// Score update (partition.rs:178-191): each row of leaf l gets out[l].
-__global__ void k_score(const StepDev* st, const LeafDev* leaves, const unsigned* idx0, const unsigned* idx1, const unsigned* idx2, const double* val,
-                        double* score) {
-  const int l = blockIdx.y;
-  if (st->num_leaves <= 1 || l >= st->num_leaves) return;
-  const LeafDev& L = leaves[l];
-  const unsigned* idx = sel3(idx0, idx1, idx2, L.buf) + L.begin;
-  const double v = val[l];
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < L.count; p += (long long)gridDim.x * blockDim.x)
-    score[idx[p]] += v;
-}
-
-// Leaf of each row for the last tree (partition state).
-__global__ void k_leaf_of_row(const StepDev* st, const LeafDev* leaves, const unsigned* idx0, const unsigned* idx1, const unsigned* idx2, int* out) {
-  const int l = blockIdx.y;
-  if (l >= st->num_leaves) return;
-  const LeafDev& L = leaves[l];
-  const unsigned* idx = sel3(idx0, idx1, idx2, L.buf) + L.begin;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < L.count; p += (long long)gridDim.x * blockDim.x)
-    out[idx[p]] = l;
 }
 
 // ---------------------------------------------------------------- prediction
@@ -2033,41 +1085,5 @@ __global__ void __launch_bounds__(64) k_search_one(const double* hist, const lon
     *splittable = spl ? 1 : 0;
   }
 }
-
-// ---------------------------------------------------------------- launchers
-extern "C++" {
-
-template <int TPR>
-static void launch_hist_t(const HistArgs& a, bool quant, int grid, size_t lds, hipStream_t s) {
-  if (quant) hipLaunchKernelGGL(k_hist_int<TPR>, dim3(grid), dim3(256), lds, s, a);
-  else hipLaunchKernelGGL(k_hist_f64<TPR>, dim3(grid), dim3(256), lds, s, a);
-}
-
-void launch_hist(const HistArgs& a, bool quant, int tpr, int grid, size_t lds, hipStream_t s) {
-  switch (tpr) {
-    case 1: launch_hist_t<1>(a, quant, grid, lds, s); break;
-    case 2: launch_hist_t<2>(a, quant, grid, lds, s); break;
-    case 4: launch_hist_t<4>(a, quant, grid, lds, s); break;
-    case 8: launch_hist_t<8>(a, quant, grid, lds, s); break;
-    case 16: launch_hist_t<16>(a, quant, grid, lds, s); break;
-    default: launch_hist_t<32>(a, quant, grid, lds, s); break;
-  }
-}
-
-void set_hist_lds_limits(size_t bytes) {
-  const void* ks[] = {(const void*)k_hist_f64<1>, (const void*)k_hist_f64<2>, (const void*)k_hist_f64<4>,
-                      (const void*)k_hist_f64<8>, (const void*)k_hist_f64<16>, (const void*)k_hist_f64<32>,
-                      (const void*)k_hist_int<1>, (const void*)k_hist_int<2>, (const void*)k_hist_int<4>,
-                      (const void*)k_hist_int<8>, (const void*)k_hist_int<16>, (const void*)k_hist_int<32>};
-  for (const void* k : ks) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-}
-
-void launch_split(const SplitArgs& a, hipStream_t s) { hipLaunchKernelGGL(k_split, dim3(a.F), dim3(128), 0, s, a); }
-void launch_part(const PartArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_partition, dim3(grid), dim3(kPartThreads), 0, s, a);
-}
-int part_tile() { return kPartTile; }
-
-}  // extern "C++"
 
 }  // namespace lgbmhip
