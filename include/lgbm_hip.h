@@ -184,15 +184,17 @@ lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out8);
 
 /* Launch shapes of the tree-growth kernels (0 keeps the current value):
  * partition blocks, histogram row chunks, minimum rows per histogram chunk,
- * LDS budget in bytes of one histogram block (sets features per block).
- * Defaults: part_grid = min(tiles, 2 x CUs), hist_chunks = CUs, 256 rows, 32 KiB.
+ * LDS budget in bytes of one histogram block (sets features per block),
+ * growth rounds in a tree's first launch and in each continuation launch.
+ * Defaults: part_grid = min(tiles, 2 x CUs), hist_chunks = CUs, 256 rows,
+ * 32 KiB, 24 and 8 rounds. Trees do not depend on any of them.
  * No counterpart in the reference (its thread count is its only knob); for
  * launch-shape sweeps on other row/feature shapes (Booster.set_tuning). */
 lgbmhip_status lgbmhip_set_tuning(lgbmhip_ctx* ctx, int32_t part_grid, int32_t hist_chunks, int32_t hist_min_rows,
-                                  int32_t hist_lds_budget);
+                                  int32_t hist_lds_budget, int32_t rounds_main, int32_t rounds_cont);
 /* Current launch shapes: {part_grid, hist_chunks, hist_min_rows, hist_lds_budget, features per hist block,
- * hist feature groups, threads per row}. */
-lgbmhip_status lgbmhip_get_tuning(lgbmhip_ctx* ctx, int32_t* out7);
+ * hist feature groups, threads per row, rounds_main, rounds_cont}. */
+lgbmhip_status lgbmhip_get_tuning(lgbmhip_ctx* ctx, int32_t* out9);
 
 /* Device-time of the last training iteration's kernels, by kind (ms): histogram, split, partition, other. */
 lgbmhip_status lgbmhip_last_timings(lgbmhip_ctx* ctx, double* out4);

@@ -266,16 +266,18 @@ class Booster:
         keys = ("hist_ms", "hist_launches", "hist_rows", "hist_bytes", "split_ms", "part_ms", "iter_ms", "iters")
         return dict(zip(keys, out.tolist()))
 
-    def set_tuning(self, part_grid=0, hist_chunks=0, hist_min_rows=0, hist_lds_budget=0):
-        """Launch shapes of the tree-growth kernels (before the first update)."""
+    def set_tuning(self, part_grid=0, hist_chunks=0, hist_min_rows=0, hist_lds_budget=0, rounds_main=0,
+                   rounds_cont=0):
+        """Launch shapes of the tree-growth kernels (before the first update;
+        0 keeps a value). Trees do not depend on them."""
         check(lib().lgbmhip_set_tuning(self._ctx, int(part_grid), int(hist_chunks), int(hist_min_rows),
-                                       int(hist_lds_budget)), self._ctx)
+                                       int(hist_lds_budget), int(rounds_main), int(rounds_cont)), self._ctx)
 
     def get_tuning(self):
-        out = np.zeros(7, np.int32)
+        out = np.zeros(9, np.int32)
         check(lib().lgbmhip_get_tuning(self._ctx, ptr(out)), self._ctx)
         keys = ("part_grid", "hist_chunks", "hist_min_rows", "hist_lds", "hist_features_per_block",
-                "hist_groups", "hist_threads_per_row")
+                "hist_groups", "hist_threads_per_row", "rounds_main", "rounds_cont")
         return dict(zip(keys, out.tolist()))
 
     def last_iteration_ms(self):

@@ -97,7 +97,7 @@ SIGNATURES = {
     "lgbmhip_predict_binned": (I32, [P, P, I64, P]),
     "lgbmhip_last_timings": (I32, [P, P]),
     "lgbmhip_profile": (I32, [P, I32, P]),
-    "lgbmhip_set_tuning": (I32, [P, I32, I32, I32, I32]),
+    "lgbmhip_set_tuning": (I32, [P, I32, I32, I32, I32, I32, I32]),
     "lgbmhip_get_tuning": (I32, [P, P]),
 }
 

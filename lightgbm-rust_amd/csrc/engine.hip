@@ -1551,31 +1551,37 @@ lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out8) {
 }
 
 lgbmhip_status lgbmhip_set_tuning(lgbmhip_ctx* ctx, int32_t part_grid, int32_t hist_chunks, int32_t hist_min_rows,
-                                  int32_t hist_lds_budget) {
+                                  int32_t hist_lds_budget, int32_t rounds_main, int32_t rounds_cont) {
   return guarded(ctx, [&] {
     Ctx* c = &ctx->c;
-    if (part_grid < 0 || hist_chunks < 0 || hist_min_rows < 0 || hist_lds_budget < 0)
+    if (part_grid < 0 || hist_chunks < 0 || hist_min_rows < 0 || hist_lds_budget < 0 || rounds_main < 0 ||
+        rounds_cont < 0)
       throw Error(LGBMHIP_INVALID_PARAMETER, "tuning values must be >= 0");
     if (hist_lds_budget > 0 && hist_lds_budget < 4096) throw Error(LGBMHIP_INVALID_PARAMETER, "LDS budget below 4 KiB");
+    if (rounds_main > 4096 || rounds_cont > 4096) throw Error(LGBMHIP_INVALID_PARAMETER, "growth rounds above 4096");
     if (part_grid) c->tune_part_grid = part_grid;
     if (hist_chunks) c->tune_hist_chunks = hist_chunks;
     if (hist_min_rows) c->tune_hist_min_rows = hist_min_rows;
     if (hist_lds_budget) c->tune_hist_lds = hist_lds_budget;
+    if (rounds_main) c->rounds_main = rounds_main;
+    if (rounds_cont) c->rounds_cont = rounds_cont;
     c->state_ready = false;   // shapes are baked into the captured graphs
   });
 }
 
-lgbmhip_status lgbmhip_get_tuning(lgbmhip_ctx* ctx, int32_t* out7) {
+lgbmhip_status lgbmhip_get_tuning(lgbmhip_ctx* ctx, int32_t* out9) {
   return guarded(ctx, [&] {
     Ctx* c = &ctx->c;
     if (!c->state_ready) setup_training(c);
-    out7[0] = c->part_grid;
-    out7[1] = c->hist_chunks;
-    out7[2] = c->hist_min_rows;
-    out7[3] = (int32_t)c->hist_lds;
-    out7[4] = c->hist_fb;
-    out7[5] = c->hist_ng;
-    out7[6] = c->hist_tpr;
+    out9[0] = c->part_grid;
+    out9[1] = c->hist_chunks;
+    out9[2] = c->hist_min_rows;
+    out9[3] = (int32_t)c->hist_lds;
+    out9[4] = c->hist_fb;
+    out9[5] = c->hist_ng;
+    out9[6] = c->hist_tpr;
+    out9[7] = c->rounds_main;
+    out9[8] = c->rounds_cont;
   });
 }
 

@@ -299,6 +299,66 @@ def test_quantized_learner_tree_bit_exact_structure():
         _compare_tree(b.tree(t), ob.tree(t), value_rtol=1e-9)
 
 
+def _same_trees_as_oracle(params, n=100000, trees=2, seed=20, tuning=None, value_rtol=1e-9):
+    """Trains `trees` trees on the same gradients on both sides; every tree
+    must equal the oracle's and every row must land in the same leaf."""
+    X, y = datagen.binary_100k(n)
+    ds, b = _booster(X, y, params)
+    if tuning:
+        b.set_tuning(**tuning)
+    op = {k: v for k, v in params.items() if k != "use_graph"}   # an engine option, not a model parameter
+    ob = O.Booster(O.Dataset(X, y, op), op)
+    rng = np.random.default_rng(seed)
+    for t in range(trees):
+        s = rng.normal(size=len(y)) * 0.5
+        g, h = O.binary_gradients(s, y)
+        b.train_tree(g, h)
+        ob.train_tree(g, h)
+        _compare_tree(b.tree(t), ob.tree(t), value_rtol=value_rtol)
+        np.testing.assert_array_equal(b.leaf_of_row(), ob.leaf_of_row())
+    return b
+
+
+@pytest.mark.parametrize("quant", [False, True])
+def test_batched_growth_255_leaves_matches_oracle(quant):
+    # many rounds, up to 32 splits each, splits made ahead of the sequential
+    # order and a tree that fills up with splits still pending
+    params = dict(P_BIN, num_leaves=255, max_bin=63, min_data_in_leaf=20)
+    if quant:
+        params.update(use_quantized_grad=True, num_grad_quant_bins=16, quant_train_renew_leaf=True)
+    b = _same_trees_as_oracle(params)
+    assert b.tree(0).num_leaves == 255
+
+
+def test_growth_rounds_per_launch_do_not_change_trees():
+    # one round per launch: every round after the first runs as a continuation
+    params = dict(P_BIN, num_leaves=127, max_bin=63)
+    _same_trees_as_oracle(params, trees=2, tuning=dict(rounds_main=1, rounds_cont=1))
+    # stream launches, uneven batches of rounds
+    _same_trees_as_oracle(dict(params, use_graph=False), trees=1, tuning=dict(rounds_main=2, rounds_cont=3))
+
+
+def test_batched_growth_with_max_depth_and_min_data():
+    # children that may not be searched (depth, min_data) inside batched rounds
+    params = dict(P_BIN, num_leaves=255, max_bin=63, max_depth=6, min_data_in_leaf=300)
+    b = _same_trees_as_oracle(params, trees=2)
+    assert b.tree(0).num_leaves < 255
+
+
+def test_quantized_boosting_255_leaves_matches_oracle():
+    X, y = datagen.binary_100k(100000)
+    params = dict(P_BIN, num_leaves=255, max_bin=63, use_quantized_grad=True, num_grad_quant_bins=16,
+                  stochastic_rounding=True, quant_train_renew_leaf=True)
+    ds, b = _booster(X, y, params)
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    for _ in range(3):
+        b.update()
+        ob.update()
+    for t in range(3):
+        _compare_tree(b.tree(t), ob.tree(t), value_rtol=1e-9)
+    np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
+
+
 def test_boosting_matches_oracle_and_predicts():
     X, y = datagen.binary_100k(100000)
     params = dict(P_BIN, num_leaves=31, max_bin=255)
