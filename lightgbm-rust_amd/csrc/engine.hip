@@ -225,7 +225,7 @@ struct Ctx {
   DevBuf<int8_t> gq, hq;
   std::unique_ptr<Mt> start_eng;
   int n_calls = 0;
-  int hist_tpr = 8, hist_fb = 0, hist_ng = 1, hist_chunks = 256, hist_min_rows = 256;
+  int hist_tpr = 8, hist_fb = 0, hist_ng = 1, hist_chunks = 256, hist_min_rows = 2048;
   size_t hist_lds = 0;
   int part_grid = 1024;
   int tune_part_grid = 0, tune_hist_chunks = 0, tune_hist_min_rows = 0, tune_hist_lds = 0;
@@ -483,11 +483,11 @@ static void setup_training(Ctx* c) {
   set_part_lds_limit((size_t)L * sizeof(PosRec));
   c->renew.alloc((size_t)2 * L);
   c->renew.zero(c->stream);
+  // defaults from sweeps on 10.5M x 28 (quantized and FP64 forms)
   c->part_grid = (int)std::min<int64_t>((N + part_tile() - 1) / part_tile(),
-                                        c->tune_part_grid > 0 ? c->tune_part_grid : 2 * c->cus);
+                                        c->tune_part_grid > 0 ? c->tune_part_grid : 4 * c->cus);
   c->hist_chunks = c->tune_hist_chunks > 0 ? c->tune_hist_chunks : c->cus;
-  // small chunks keep small leaves spread over many CUs (one pass per block)
-  c->hist_min_rows = c->tune_hist_min_rows > 0 ? c->tune_hist_min_rows : 256;
+  c->hist_min_rows = c->tune_hist_min_rows > 0 ? c->tune_hist_min_rows : 2048;
   if (quant) {
     // quant.rs:45-79: per-row tables from block generators of `quant_table_threads` blocks
     std::vector<double> grv(N), hrv(N);
@@ -519,8 +519,11 @@ static void setup_training(Ctx* c) {
     c->n_calls = 0;
   }
   plan_hist(c, quant);
-  // ordered bins: the root's rows in row order, group-major
+  // buffer 0: the root's rows in row order (identity index), bins group-major
   {
+    std::vector<unsigned> iota(N);
+    for (int64_t i = 0; i < N; ++i) iota[i] = (unsigned)i;
+    HIP_CHECK(hipMemcpy(c->idx[0].p, iota.data(), N * 4, hipMemcpyHostToDevice));
     const size_t gbytes = (size_t)c->hist_ng * N * c->hist_fb;
     for (int b = 0; b < 3; ++b) c->grp[b].alloc(gbytes);
     hipLaunchKernelGGL(k_group_rows, dim3(grid_for(N * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
@@ -649,10 +652,10 @@ static void enqueue_head(Ctx* c, int k) {
   if (quant) {
     hipLaunchKernelGGL(k_discretize, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, c->grv.p, c->hrv.p,
                        c->opts.num_grad_quant_bins, c->opts.stochastic_rounding, 0, c->starts.p, c->call_ctr.p,
-                       c->it.p + k, c->idx[0].p, (char2*)c->gh[0].p, c->gq.p, c->hq.p);
+                       c->it.p + k, (unsigned*)nullptr, (char2*)c->gh[0].p, (int8_t*)nullptr, (int8_t*)nullptr);
     hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, c->call_ctr.p);
   } else {
-    hipLaunchKernelGGL(k_tree_init, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, c->idx[0].p,
+    hipLaunchKernelGGL(k_tree_init, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, (unsigned*)nullptr,
                        (float2*)c->gh[0].p, c->it.p + k);
   }
   hipLaunchKernelGGL(k_state_init, dim3(1), dim3(256), 0, s, c->st.p, c->pos.p, c->leaves.p, c->spl.p, (long long)N,
@@ -1316,8 +1319,14 @@ lgbmhip_status lgbmhip_quant_state(lgbmhip_ctx* ctx, int8_t* gq, int8_t* hq, dou
   return guarded(ctx, [&] {
     Ctx* c = &ctx->c;
     if (!c->state_ready || !c->opts.use_quantized_grad) throw Error(LGBMHIP_INVALID_PARAMETER, "no quantized state");
-    HIP_CHECK(hipMemcpyAsync(gq, c->gq.p, c->N, hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(hipMemcpyAsync(hq, c->hq.p, c->N, hipMemcpyDeviceToHost, c->stream));
+    // buffer 0 holds the last tree's (gq, hq) pairs in row order
+    std::vector<int8_t> q((size_t)2 * c->N);
+    HIP_CHECK(hipMemcpyAsync(q.data(), c->gh[0].p, q.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < c->N; ++i) {
+      gq[i] = q[2 * i];
+      hq[i] = q[2 * i + 1];
+    }
     IterDev it;
     HIP_CHECK(hipMemcpyAsync(&it, c->it.p, sizeof(IterDev), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));

@@ -704,7 +704,21 @@ __device__ __forceinline__ Cand cand_none() {
 // partition.rs:107-174). The row order inside a child is unspecified: every
 // consumer of a segment (exact fixed-point or integer histograms, renewal
 // sums, score update) is order-independent.
+template <int VW>
+struct RowVec;   // the vector type that moves VW bytes of a row
+template <>
+struct RowVec<4> { using T = unsigned; };
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+template <>
+struct RowVec<8> { using T = u32x2; };
+template <>
+struct RowVec<16> { using T = u32x4; };
+
+template <int VW>
 __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
+  using VT = typename RowVec<VW>::T;
+  constexpr int VWW = VW / 4;   // words per vector
   StepDev* st = a.st;
   extern __shared__ PosRec P[];   // this block's copy of the position array
   constexpr int NW = kPartThreads / 64;
@@ -969,7 +983,7 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
     bool valid[R], left[R];
     unsigned rid[R];
     uint2 gv[R];
-    unsigned v[R][CW];
+    VT v[R][CW / VWW];
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const long long p = t0 + u * kPartThreads + tid;
@@ -981,8 +995,9 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
         else gv[u].x = ((const unsigned short*)igh)[p];
         left[u] = goes_left(scol[(size_t)p * FB], fd, s);
 #pragma unroll
-        for (int i = 0; i < CW; ++i)
-          if (i < tw) v[u][i] = *(const unsigned*)(gsrc + (size_t)(i / words) * gstride + (size_t)p * FB + 4 * (i % words));
+        for (int i = 0; i < CW; i += VWW)
+          if (i < tw)
+            v[u][i / VWW] = *(const VT*)(gsrc + (size_t)(i / words) * gstride + (size_t)p * FB + 4 * (i % words));
       }
     }
     // ranks inside each 64-row slice, slice counts to LDS
@@ -1035,8 +1050,9 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
       if (gbytes == 8) ((uint2*)ogh)[d[u]] = gv[u];
       else ((unsigned short*)ogh)[d[u]] = (unsigned short)gv[u].x;
 #pragma unroll
-      for (int i = 0; i < CW; ++i)
-        if (i < tw) *(unsigned*)(gdst + (size_t)(i / words) * gstride + (size_t)d[u] * FB + 4 * (i % words)) = v[u][i];
+      for (int i = 0; i < CW; i += VWW)
+        if (i < tw)
+          *(VT*)(gdst + (size_t)(i / words) * gstride + (size_t)d[u] * FB + 4 * (i % words)) = v[u][i / VWW];
     }
     // rows wider than CW words: the rest, CW words per round of loads
     for (int k0 = CW; k0 < tw; k0 += CW) {
@@ -1045,18 +1061,20 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
         if (!valid[u]) continue;
         const long long p = t0 + u * kPartThreads + tid;
 #pragma unroll
-        for (int i = 0; i < CW; ++i) {
+        for (int i = 0; i < CW; i += VWW) {
           const int kk = k0 + i;
-          if (kk < tw) v[u][i] = *(const unsigned*)(gsrc + (size_t)(kk / words) * gstride + (size_t)p * FB + 4 * (kk % words));
+          if (kk < tw)
+            v[u][i / VWW] = *(const VT*)(gsrc + (size_t)(kk / words) * gstride + (size_t)p * FB + 4 * (kk % words));
         }
       }
 #pragma unroll
       for (int u = 0; u < R; ++u) {
         if (!valid[u]) continue;
 #pragma unroll
-        for (int i = 0; i < CW; ++i) {
+        for (int i = 0; i < CW; i += VWW) {
           const int kk = k0 + i;
-          if (kk < tw) *(unsigned*)(gdst + (size_t)(kk / words) * gstride + (size_t)d[u] * FB + 4 * (kk % words)) = v[u][i];
+          if (kk < tw)
+            *(VT*)(gdst + (size_t)(kk / words) * gstride + (size_t)d[u] * FB + 4 * (kk % words)) = v[u][i / VWW];
         }
       }
     }
@@ -1196,15 +1214,21 @@ static void set_hist_lds_limits(size_t bytes) {
 }
 
 static void set_part_lds_limit(size_t bytes) {
-  (void)hipFuncSetAttribute((const void*)k_grow_part, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipFuncSetAttribute((const void*)k_grow_part<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipFuncSetAttribute((const void*)k_grow_part<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipFuncSetAttribute((const void*)k_grow_part<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 static void launch_split(const SplitArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_grow_split, dim3(a.F, a.root ? 1 : kMaxBatch), dim3(128), 0, s, a);
 }
 
+// Rows move in the widest vectors that divide a group's bytes.
 static void launch_part(const PartArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_grow_part, dim3(grid), dim3(kPartThreads), (size_t)a.L * sizeof(PosRec), s, a);
+  const size_t lds = (size_t)a.L * sizeof(PosRec);
+  if (a.FB % 16 == 0) hipLaunchKernelGGL(k_grow_part<16>, dim3(grid), dim3(kPartThreads), lds, s, a);
+  else if (a.FB % 8 == 0) hipLaunchKernelGGL(k_grow_part<8>, dim3(grid), dim3(kPartThreads), lds, s, a);
+  else hipLaunchKernelGGL(k_grow_part<4>, dim3(grid), dim3(kPartThreads), lds, s, a);
 }
 
 static int part_tile() { return kPartTile; }

@@ -401,15 +401,16 @@ __global__ void k_absmax_it(const float* g, const float* h, long long n, IterDev
 }
 
 // ---------------------------------------------------------------- tree init
-// Identity partition in buffer 0, gradients of class k copied beside the row
-// indices, fixed-point root sums (learner/mod.rs:580-628).
+// Gradients of class k in buffer 0 (the root, rows in order; the identity
+// row index of buffer 0 is written once per training run), fixed-point root
+// sums (learner/mod.rs:580-628).
 __global__ void k_tree_init(long long n, const float* __restrict__ g, const float* __restrict__ h,
                             unsigned* __restrict__ idx0, float2* __restrict__ gh0, IterDev* it) {
   const int sg = fix_shift(n, it->max_abs_g_bits), sh = fix_shift(n, it->max_abs_h_bits);
   long long ag = 0, ah = 0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const float gv = g[i], hv = h[i];
-    idx0[i] = (unsigned)i;
+    if (idx0) idx0[i] = (unsigned)i;
     gh0[i] = make_float2(gv, hv);
     ag += to_fixed(gv, sg);
     ah += to_fixed(hv, sh);
@@ -423,7 +424,8 @@ __global__ void k_tree_init(long long n, const float* __restrict__ g, const floa
 }
 
 // Quantized form: grad/hess -> int8 by stochastic rounding (learner/quant.rs:92-129),
-// packed (gq, hq) beside the identity partition; exact int root sums.
+// packed (gq, hq) into buffer 0; exact int root sums. idx0 / gq_out / hq_out
+// are written when given (the stage entry point).
 __global__ void k_discretize(long long n, const float* __restrict__ g, const float* __restrict__ h,
                              const double* __restrict__ grv, const double* __restrict__ hrv, int bins, int stochastic,
                              int constant_hessian, const int* __restrict__ starts, const int* call_ctr, IterDev* it,
@@ -440,16 +442,19 @@ __global__ void k_discretize(long long n, const float* __restrict__ g, const flo
     const double gv = (double)g[i];
     double rg = 0.5, rh = 0.5;
     if (stochastic) {
-      const long long pos = (i + start) % n;
+      long long pos = i + start;   // (i + start) mod n, with i, start < n
+      if (pos >= n) pos -= n;
       rg = grv[pos];
       rh = hrv[pos];
     }
     const int8_t a = gv >= 0.0 ? to_i8(gv * inv_g + rg) : to_i8(gv * inv_g - rg);
     const int8_t b = constant_hessian ? (int8_t)1 : to_i8((double)h[i] * inv_h + rh);
-    idx0[i] = (unsigned)i;
+    if (idx0) idx0[i] = (unsigned)i;
     q0[i] = make_char2(a, b);
-    gq_out[i] = a;
-    hq_out[i] = b;
+    if (gq_out) {
+      gq_out[i] = a;
+      hq_out[i] = b;
+    }
     ai += (long long)(((unsigned long long)(long long)a << 32) | (unsigned long long)(uint8_t)b);
     agq += a;
     ahq += b;
