@@ -204,7 +204,10 @@ struct Ctx {
   DevBuf<unsigned> idx[3];
   DevBuf<uint8_t> gh[3];
   DevBuf<uint8_t> grp[3];
-  DevBuf<long long> acc;
+  DevBuf<long long> acc;     // each item's histogram summed over its chunks, one unit per group
+  DevBuf<int> acc_units;     // {0, G, 2G, ...}: acc read as one unit per (item, group)
+  DevBuf<long long> slab;    // histogram units of one histogram pass
+  DevBuf<int> units;         // [kMaxBatch + 1] first unit of each item
   DevBuf<double> pool;
   DevBuf<long long> ipool;
   // growth state: leaf slots (MS of them), positions, the round's plan
@@ -460,8 +463,9 @@ static void setup_training(Ctx* c) {
   // ahead of order and never replayed, two slots each, plus the root
   c->MS = 4 * L;
   const int MS = c->MS;
-  c->acc.alloc((size_t)kMaxBatch * F * B * (quant ? 1 : 2));
-  c->acc.zero(c->stream);
+  c->units.alloc(kMaxBatch + 2);
+  c->units.zero(c->stream);
+  c->acc_units.alloc(kMaxBatch + 1);
   if (quant) { c->ipool.alloc((size_t)MS * F * B); c->pool.free(); }
   else { c->pool.alloc((size_t)MS * F * B * 2); c->ipool.free(); }
   c->feat_split.alloc((size_t)2 * kMaxBatch * F);
@@ -486,8 +490,8 @@ static void setup_training(Ctx* c) {
   // defaults from sweeps on 10.5M x 28 (quantized and FP64 forms)
   c->part_grid = (int)std::min<int64_t>((N + part_tile() - 1) / part_tile(),
                                         c->tune_part_grid > 0 ? c->tune_part_grid : 4 * c->cus);
-  c->hist_chunks = c->tune_hist_chunks > 0 ? c->tune_hist_chunks : c->cus;
-  c->hist_min_rows = c->tune_hist_min_rows > 0 ? c->tune_hist_min_rows : 2048;
+  c->hist_chunks = c->tune_hist_chunks > 0 ? c->tune_hist_chunks : (quant ? 2 : 1) * c->cus;
+  c->hist_min_rows = c->tune_hist_min_rows > 0 ? c->tune_hist_min_rows : (quant ? 2048 : 4096);
   if (quant) {
     // quant.rs:45-79: per-row tables from block generators of `quant_table_threads` blocks
     std::vector<double> grv(N), hrv(N);
@@ -519,6 +523,17 @@ static void setup_training(Ctx* c) {
     c->n_calls = 0;
   }
   plan_hist(c, quant);
+  c->acc.alloc((size_t)kMaxBatch * c->hist_ng * c->hist_fb * B * (quant ? 1 : 2));
+  {
+    std::vector<int> au(kMaxBatch + 1);
+    for (int w = 0; w <= kMaxBatch; ++w) au[w] = w * c->hist_ng;
+    HIP_CHECK(hipMemcpy(c->acc_units.p, au.data(), au.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  {
+    // histogram units of one pass: at most (rows / chunk + splits) per group
+    const int64_t max_units = ((N + c->hist_min_rows - 1) / c->hist_min_rows + kMaxBatch + 1) * c->hist_ng;
+    c->slab.alloc((size_t)max_units * c->hist_fb * B * (quant ? 1 : 2));
+  }
   // buffer 0: the root's rows in row order (identity index), bins group-major
   {
     std::vector<unsigned> iota(N);
@@ -575,7 +590,8 @@ static HistArgs hist_args(Ctx* c, bool quant, int k, int mode) {
   a.N = c->N;
   a.max_chunks = c->hist_chunks;
   a.min_rows_per_chunk = c->hist_min_rows;
-  a.acc = c->acc.p;
+  a.slab = c->slab.p;
+  a.units = c->units.p;
   a.it = c->it.p + k;
   a.p = c->sp;
   a.quant = quant ? 1 : 0;
@@ -593,7 +609,10 @@ static SplitArgs split_args(Ctx* c, int k, bool root) {
   a.F = c->F;
   a.B = c->B;
   a.L = c->L;
-  a.acc = c->acc.p;
+  a.slab = c->acc.p;          // summed by the reduction after each histogram pass
+  a.units = c->acc_units.p;
+  a.FB = c->hist_fb;
+  a.NG = c->hist_ng;
   a.pool = c->pool.p;
   a.ipool = c->ipool.p;
   a.feat_split = c->feat_split.p;
@@ -639,6 +658,16 @@ static void prof_mark(Ctx* c, int k, int step, int kind, int end) {
   if (c->profile && step < prof_steps(c)) HIP_CHECK(hipEventRecord(pevent(c, k, step, kind, end), c->stream));
 }
 
+// Every item's histogram units of the last histogram pass summed into acc
+// (one unit per item and feature group).
+static void slab_total(Ctx* c, bool quant, hipStream_t s) {
+  const int per_unit = c->hist_fb * c->B * (quant ? 1 : 2);
+  const int64_t eblocks = ((int64_t)c->hist_ng * per_unit + 31) / 32;
+  const int grid = (int)std::min<int64_t>(eblocks * kMaxBatch, 4LL * c->cus);
+  hipLaunchKernelGGL(k_slab_reduce, dim3(grid), dim3(256), 0, s, c->slab.p, c->units.p, c->hist_ng, per_unit,
+                     c->acc.p);
+}
+
 // The first launches of a tree for class k (gradients already in g/h and
 // max |g|, |h| in it[k]): identity partition or quantization with the root
 // sums, the growth state, the root's histogram and search.
@@ -663,6 +692,7 @@ static void enqueue_head(Ctx* c, int k) {
   const int hgrid = c->hist_chunks * c->hist_ng;
   prof_mark(c, k, 0, 0, 0);
   launch_hist(hist_args(c, quant, k, 2), quant, c->hist_tpr, hgrid, c->hist_lds, s);
+  slab_total(c, quant, s);
   prof_mark(c, k, 0, 0, 1);
   launch_split(split_args(c, k, true), s);
   prof_mark(c, k, 0, 1, 1);
@@ -682,6 +712,7 @@ static void enqueue_rounds(Ctx* c, int k, int r0, int n) {
     launch_part(pa, c->part_grid, s);
     prof_mark(c, k, step, 2, 1);
     launch_hist(ha, quant, c->hist_tpr, hgrid, c->hist_lds, s);
+    slab_total(c, quant, s);
     prof_mark(c, k, step, 0, 1);
     launch_split(sa, s);
     prof_mark(c, k, step, 1, 1);
@@ -1417,25 +1448,29 @@ static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc
                        c->stream, (const unsigned*)c->idx[1].p, (long long)m, c->bins_rm.p, c->S, c->F, c->hist_fb,
                        c->hist_ng, (long long)N, c->grp[1].p);
   }
-  c->acc.zero(c->stream);
   HistArgs a = hist_args(c, quant, 0, 1);
   a.d_begin = 0;
   a.d_count = m;
   a.d_buf = 1;
   a.d_shift_g = fix_shift(m, gbits);
   a.d_shift_h = fix_shift(m, hbits);
-  if (m > 0) launch_hist(a, quant, c->hist_tpr, c->hist_chunks * c->hist_ng, c->hist_lds, c->stream);
-  HIP_CHECK(hipGetLastError());
-  const size_t cnt = (size_t)c->F * c->B * (quant ? 1 : 2);
-  std::vector<long long> acc(cnt);
-  HIP_CHECK(hipMemcpyAsync(acc.data(), c->acc.p, cnt * 8, hipMemcpyDeviceToHost, c->stream));
+  const int C = quant ? 1 : 2;
+  const size_t cnt = (size_t)c->F * c->B * C;
+  std::vector<long long> acc(c->acc.n, 0);
+  if (m > 0) {
+    launch_hist(a, quant, c->hist_tpr, c->hist_chunks * c->hist_ng, c->hist_lds, c->stream);
+    slab_total(c, quant, c->stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(acc.data(), c->acc.p, c->acc.n * 8, hipMemcpyDeviceToHost, c->stream));
+  }
   HIP_CHECK(hipStreamSynchronize(c->stream));
-  c->acc.zero(c->stream);
-  HIP_CHECK(hipStreamSynchronize(c->stream));
-  if (quant) std::memcpy(out, acc.data(), cnt * 8);
-  else
-    for (size_t i = 0; i < cnt; ++i)
-      ((double*)out)[i] = std::ldexp((double)acc[i], -((i & 1) ? a.d_shift_h : a.d_shift_g));
+  // acc is [group][feature in group][bin][C]: out is [feature][bin][C]
+  for (size_t i = 0; i < cnt; ++i) {
+    const size_t f = i / ((size_t)c->B * C), r = i % ((size_t)c->B * C);
+    const size_t src = (f / c->hist_fb) * ((size_t)c->hist_fb * c->B * C) + (f % c->hist_fb) * ((size_t)c->B * C) + r;
+    if (quant) ((long long*)out)[i] = acc[src];
+    else ((double*)out)[i] = std::ldexp((double)acc[src], -((i & 1) ? a.d_shift_h : a.d_shift_g));
+  }
   *out_B = c->B;
 }
 

@@ -81,11 +81,10 @@ __global__ void k_state_init(StepDev* st, PosRec* pos, LeafDev* leaves, uint8_t*
 
 // ---------------------------------------------------------------- histograms
 // Row-wise histograms (multi_val_bin.rs:556-611 FP64, :710-791 integer).
-// FP64 values are summed as exact fixed-point int64 (ds_add_u64 in LDS, then
-// 64-bit global atomics of the nonzero bins into the item's accumulator):
+// FP64 values are summed as exact fixed-point int64 (ds_add_u64 in LDS):
 // exact, so the result is independent of order and block count. The
-// quantized form sums int8 (g, h) as int32 / uint32 in LDS and flushes
-// packed (g << 32) + h, equal to the reference's wrapping sum of packed rows.
+// quantized form sums int8 (g, h) as int32 / uint32 in LDS and stores packed
+// (g << 32) + h, equal to the reference's wrapping sum of packed rows.
 //
 // Every leaf's rows are one contiguous segment of an ordered buffer (buffer
 // 0 holds the root in row order; a leaf in buffer 1 splits into buffer 2 and
@@ -94,7 +93,9 @@ __global__ void k_state_init(StepDev* st, PosRec* pos, LeafDev* leaves, uint8_t*
 //
 // Work is a list of items (segments); each item is cut into row chunks, and
 // a unit = (item, chunk, feature group) is one LDS histogram. Blocks loop
-// over units.
+// over units; each unit stores its whole LDS histogram to its own slab with
+// plain coalesced stores (no global atomics), and the split pass sums an
+// item's slabs - integer sums, exact in any order.
 struct HistArgs {
   const uint8_t* grp[3];   // ordered bins per buffer, group-major [NG][N][FB]
   const void* gh[3];       // float2 (plain) or char2 (quantized), per buffer
@@ -110,7 +111,8 @@ struct HistArgs {
   long long N;
   int max_chunks;
   int min_rows_per_chunk;
-  long long* acc;          // [kMaxBatch][F][B][2] (plain) or [kMaxBatch][F][B] (quantized)
+  long long* slab;         // per unit: its group's [FB][B][C] sums, C = 2 (plain) or 1 (quantized)
+  int* units;              // [kMaxBatch + 2] first unit of each item, the total, the item count
   const IterDev* it;
   SplitParamsDev p;
   int quant;
@@ -123,13 +125,12 @@ struct HistArgs {
 constexpr int kHistUnroll = 8;
 
 // One unit: rows [r0, r1) of the segment's chunk, feature group fg, into LDS,
-// then into `out` (the item's accumulator).
+// then stored to the unit's slab (an empty chunk stores zeros).
 template <int TPR, bool Q>
 __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long long begin, long long count, int buf,
-                                          int sg, int sh, int chunk, int nchunks, int fg, long long* out_acc) {
+                                          int sg, int sh, int chunk, int nchunks, int fg, long long* out_slab) {
   const long long per = (count + nchunks - 1) / nchunks;
   const long long r0 = (long long)chunk * per, r1 = min(count, r0 + per);
-  if (r0 >= r1) return;   // uniform over the block
   const int f0 = fg * a.FB;
   const int FBg = min(a.FB, a.F - f0);
   const int B = a.B;
@@ -199,21 +200,13 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
   }
   __syncthreads();
   if (Q) {
-    unsigned long long* out = (unsigned long long*)out_acc + (size_t)f0 * B;
+    // packed (g << 32) + h, as multi_val_bin.rs:710-791 sums rows
     for (int e = threadIdx.x; e < FBg * B; e += blockDim.x) {
       const unsigned gsum = lds32[2 * e], hsum = lds32[2 * e + 1];
-      if (gsum | hsum) {
-        const unsigned long long packed =
-            ((unsigned long long)(long long)(int)gsum << 32) + (unsigned long long)hsum;
-        atomicAdd(out + e, packed);
-      }
+      out_slab[e] = (long long)(((unsigned long long)(long long)(int)gsum << 32) + (unsigned long long)hsum);
     }
   } else {
-    unsigned long long* out = (unsigned long long*)out_acc + (size_t)f0 * B * 2;
-    for (int e = threadIdx.x; e < n_lds; e += blockDim.x) {
-      const unsigned long long v = lds64[e];
-      if (v) atomicAdd(out + e, v);
-    }
+    for (int e = threadIdx.x; e < n_lds; e += blockDim.x) out_slab[e] = (long long)lds64[e];
   }
   __syncthreads();   // the next unit reuses the LDS histogram
 }
@@ -285,7 +278,7 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
   __shared__ long long s_begin[kMaxBatch], s_count[kMaxBatch];
   __shared__ int s_buf[kMaxBatch], s_sg[kMaxBatch], s_sh[kMaxBatch], s_nch[kMaxBatch], s_u0[kMaxBatch + 1];
   const int tid = threadIdx.x;
-  const size_t stride = (size_t)a.F * a.B * (Q ? 1 : 2);   // one item's accumulator
+  const size_t ustride = (size_t)a.FB * a.B * (Q ? 1 : 2);   // one unit's slab
   int nitems;
   if (a.mode != 3) {
     if (tid == 0) {
@@ -300,6 +293,11 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
       s_nch[0] = (int)min((long long)a.max_chunks, max(want, 1LL));
       s_u0[0] = 0;
       s_u0[1] = s_nch[0] * a.NG;
+      if (blockIdx.x == 0) {
+        a.units[0] = 0;
+        a.units[1] = s_u0[1];
+        a.units[kMaxBatch + 1] = 1;
+      }
     }
     nitems = 1;
   } else {
@@ -313,7 +311,10 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
       __syncthreads();
       if (!s_was_done) apply_plan(a);
     }
-    if (P->done) return;
+    if (P->done) {
+      if (blockIdx.x == 0 && tid == 0) a.units[kMaxBatch + 1] = 0;   // nothing for the reduction
+      return;
+    }
     nitems = P->nwork;
     if (tid < nitems) {
       const PlanItem& it = P->item[tid];
@@ -341,6 +342,8 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
       }
       s_u0[nitems] = u;
       if (blockIdx.x == 0) {
+        for (int w = 0; w <= nitems; ++w) a.units[w] = s_u0[w];
+        a.units[kMaxBatch + 1] = nitems;
         // work counters for profiles (read by the host after the tree)
         st->hist_rows += rows;
         st->hist_items += built;
@@ -354,7 +357,7 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
     while (u >= s_u0[w + 1]) ++w;
     const int r = u - s_u0[w];
     hist_unit<TPR, Q>(a, lds, s_begin[w], s_count[w], s_buf[w], s_sg[w], s_sh[w], r / a.NG, s_nch[w], r % a.NG,
-                      a.acc + (size_t)w * stride);
+                      a.slab + (size_t)u * ustride);
   }
 }
 
@@ -378,7 +381,9 @@ struct SplitArgs {
   const FeatDev* feats;
   SplitParamsDev p;
   int F, B, L;
-  long long* acc;        // per item: the smaller child's histogram (zeroed after reading)
+  const long long* slab;  // histogram units of the last histogram pass
+  const int* units;       // [kMaxBatch + 1] first unit of each item
+  int FB, NG;             // features per histogram group, groups
   double* pool;          // [slots][F][B][2] FP64 histograms
   long long* ipool;      // [slots][F][B] packed histograms (quantized)
   SplitRec* feat_split;  // [2 kMaxBatch][F] searches of this pass, row = slot - first child slot
@@ -415,8 +420,62 @@ __device__ __forceinline__ LeafSums leaf_sums(const SplitArgs& a, int slot, bool
   return s;
 }
 
+// Sum of feature f's bins over the histogram units of item w (all chunks of
+// f's group): out0 (and out1 for C = 2, the hessian sums) per bin b = tid +
+// 128 k. Wrapping int64 sums: exact in any order.
+template <int C>
+__device__ __forceinline__ void slab_sum(const SplitArgs& a, int w, int f, int tid, long long* out0, long long* out1) {
+  constexpr int PB = kMaxBins / 128;
+  const int g = f / a.FB, fl = f % a.FB;
+  const int u0 = a.units[w], nch = (a.units[w + 1] - u0) / a.NG;
+  const size_t ustride = (size_t)a.FB * a.B * C;
+  const long long* base = a.slab + (size_t)(u0 + g) * ustride + (size_t)fl * a.B * C;
+  const size_t cstride = (size_t)a.NG * ustride;   // the next chunk of the same group
+  unsigned long long s0[PB], s1[PB];
+#pragma unroll
+  for (int k = 0; k < PB; ++k) s0[k] = s1[k] = 0;
+  int c = 0;
+  for (; c + 8 <= nch; c += 8) {
+    long long x0[8][PB], x1[8][PB];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int k = 0; k < PB; ++k) {
+        const int b = tid + 128 * k;
+        x0[j][k] = x1[j][k] = 0;
+        if (b < a.B) {
+          const long long* q = base + (size_t)(c + j) * cstride + (size_t)b * C;
+          x0[j][k] = q[0];
+          if (C == 2) x1[j][k] = q[1];
+        }
+      }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int k = 0; k < PB; ++k) {
+        s0[k] += (unsigned long long)x0[j][k];
+        s1[k] += (unsigned long long)x1[j][k];
+      }
+  }
+  for (; c < nch; ++c)
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      const int b = tid + 128 * k;
+      if (b < a.B) {
+        const long long* q = base + (size_t)c * cstride + (size_t)b * C;
+        s0[k] += (unsigned long long)q[0];
+        if (C == 2) s1[k] += (unsigned long long)q[1];
+      }
+    }
+#pragma unroll
+  for (int k = 0; k < PB; ++k) {
+    out0[k] = (long long)s0[k];
+    out1[k] = (long long)s1[k];
+  }
+}
+
 // One block (2 waves) per (feature, split of the round): the smaller child's
-// histogram from the accumulator (with the most-frequent-bin fix,
+// histogram from its histogram units (with the most-frequent-bin fix,
 // histogram.rs:42-78), the larger child's as parent - smaller
 // (histogram.rs:59-85), then wave 0 searches the smaller child and wave 1 the
 // larger (learner/mod.rs:954-1101, split.rs:246-271 / 930-950).
@@ -478,30 +537,23 @@ __global__ void __launch_bounds__(128) k_grow_split(SplitArgs a) {
   if (!a.quant) {
     const int ssg = fix_shift(sc, a.it->max_abs_g_bits);
     const int ssh = fix_shift(sc, a.it->max_abs_h_bits);
-    long long* acc = a.acc + (size_t)w * F * B * 2 + (size_t)f * B * 2;
     const double* pp = lg >= 0 ? a.pool + ((size_t)parent * F + f) * B * 2 : nullptr;
     long long gi[PB], hi[PB];
     double pg[PB], ph[PB];
 #pragma unroll
     for (int k = 0; k < PB; ++k) {
       const int b = tid + 128 * k;
-      gi[k] = hi[k] = 0;
       pg[k] = ph[k] = 0.0;
-      if (b < B) {
-        gi[k] = acc[2 * b];
-        hi[k] = acc[2 * b + 1];
-        if (pp) {
-          pg[k] = pp[2 * b];
-          ph[k] = pp[2 * b + 1];
-        }
+      if (b < B && pp) {
+        pg[k] = pp[2 * b];
+        ph[k] = pp[2 * b + 1];
       }
     }
+    slab_sum<2>(a, w, f, tid, gi, hi);
 #pragma unroll
     for (int k = 0; k < PB; ++k) {
       const int b = tid + 128 * k;
       if (b < B) {
-        acc[2 * b] = 0;
-        acc[2 * b + 1] = 0;
         hsm[0][b] = ldexp((double)gi[k], -ssg);
         hsm[1][b] = ldexp((double)hi[k], -ssh);
       }
@@ -543,24 +595,20 @@ __global__ void __launch_bounds__(128) k_grow_split(SplitArgs a) {
       }
     }
   } else {
-    long long* acc = a.acc + (size_t)w * F * B + (size_t)f * B;
     const long long* pp = lg >= 0 ? a.ipool + ((size_t)parent * F + f) * B : nullptr;
-    long long v[PB], pv[PB];
+    long long v[PB], pv[PB], unused[PB];
     unsigned long long part = 0;
 #pragma unroll
     for (int k = 0; k < PB; ++k) {
       const int b = tid + 128 * k;
-      v[k] = pv[k] = 0;
-      if (b < B) {
-        v[k] = acc[b];
-        if (pp) pv[k] = pp[b];
-      }
+      pv[k] = 0;
+      if (b < B && pp) pv[k] = pp[b];
     }
+    slab_sum<1>(a, w, f, tid, v, unused);
 #pragma unroll
     for (int k = 0; k < PB; ++k) {
       const int b = tid + 128 * k;
       if (b < B) {
-        acc[b] = 0;
         ism[b] = v[k];
         if (b != fd.mfb && b < fd.num_bin) part += (unsigned long long)v[k];
       }
@@ -1185,6 +1233,51 @@ __global__ void k_leaf_of_row(const StepDev* st, const PosRec* pos, const LeafDe
   const unsigned* idx = sel3(idx0, idx1, idx2, Lf.buf) + Lf.begin;
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < Lf.count; p += (long long)gridDim.x * blockDim.x)
     out[idx[p]] = l;
+}
+
+// Each item's histogram summed over its chunks: out[w][g][entry], one unit
+// per feature group (the search then reads one unit per group). units holds
+// the first unit of each item and, at [kMaxBatch + 1], the item count of the
+// last histogram pass (0 when it had nothing to build). A block sums 32
+// consecutive entries of one item, each over the chunks c = s, s + 8, ... on
+// slice s of its 8 thread slices; the slices meet in LDS. Wrapping int64
+// sums: exact in any order.
+__global__ void __launch_bounds__(256) k_slab_reduce(const long long* __restrict__ slab, const int* units, int NG,
+                                                     int per_unit, long long* __restrict__ out) {
+  __shared__ unsigned long long part[8][32];
+  const int t = threadIdx.x, bo = t & 31, sl = t >> 5;
+  const int nitems = units[kMaxBatch + 1];
+  const long long per_item = (long long)NG * per_unit;
+  const long long eblocks = (per_item + 31) / 32;
+  const size_t cstride = (size_t)NG * per_unit;   // the next chunk of the same group
+  for (long long bi = blockIdx.x; bi < eblocks * nitems; bi += gridDim.x) {
+    const int w = (int)(bi / eblocks);
+    const long long e = (bi % eblocks) * 32 + bo;   // entry of item w: [NG][per_unit]
+    const int u0 = units[w], nch = (units[w + 1] - u0) / NG;
+    unsigned long long acc = 0;
+    if (e < per_item) {
+      const int g = (int)(e / per_unit), r = (int)(e % per_unit);
+      const long long* q = slab + (size_t)(u0 + g) * per_unit + r;
+      int c = sl;
+      for (; c + 8 * 7 < nch; c += 8 * 8) {
+        long long x[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = q[(size_t)(c + 8 * j) * cstride];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc += (unsigned long long)x[j];
+      }
+      for (; c < nch; c += 8) acc += (unsigned long long)q[(size_t)c * cstride];
+    }
+    part[sl][bo] = acc;
+    __syncthreads();
+    if (sl == 0 && e < per_item) {
+      unsigned long long s = 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s += part[j][bo];
+      out[(size_t)w * per_item + e] = (long long)s;
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------- launchers
