@@ -175,12 +175,13 @@ lgbmhip_status lgbmhip_predict_binned(lgbmhip_ctx* ctx, const uint8_t* bins, int
 
 /* Profiling of training iterations: enable = 1 records HIP events around every
  * histogram, split and partition launch (graphs off), 0 stops, -1 only reads.
- * out8 (may be NULL) receives the totals since enabling: {histogram ms,
- * histogram launches (all of them, num_leaves per tree), rows summed,
- * algorithmic histogram bytes, split ms, partition ms, iteration ms,
- * iterations}. No counterpart in
- * the reference (it has no profiler hooks); for bench and rocprof cross-checks. */
-lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out8);
+ * out11 (may be NULL) receives the totals since enabling: {histogram ms,
+ * histogram launches (one per growth round plus the root's), rows summed into
+ * histograms, algorithmic histogram bytes, split ms, partition ms, iteration
+ * ms, iterations, rows moved by partitions, splits computed, splits in the
+ * trees}. No counterpart in the reference (it has no profiler hooks); for
+ * bench and rocprof cross-checks. */
+lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out11);
 
 /* Launch shapes of the tree-growth kernels (0 keeps the current value):
  * partition blocks, histogram row chunks, minimum rows per histogram chunk,

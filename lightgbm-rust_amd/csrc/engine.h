@@ -101,6 +101,9 @@ struct StepDev {
   int rounds;        // growth rounds run on this tree
   int hist_items;    // histograms built (segments), root included
   long long hist_rows;   // rows summed into histograms, root included
+  long long part_rows;   // rows moved by partition passes
+  int computed;      // splits computed (replayed or not)
+  int pad2;
   double root_output;  // the root's output before any split (learner/mod.rs:647)
   unsigned long long lcur[kMaxBatch], rcur[kMaxBatch];   // rows placed left / right per item
 };

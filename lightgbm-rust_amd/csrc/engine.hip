@@ -262,7 +262,8 @@ struct Ctx {
   // profiling: HIP events around every histogram / split / partition launch
   int profile = 0;
   std::vector<hipEvent_t> pev;   // [K][L][3 kinds][2]
-  double prof[8] = {0};          // hist_ms, hist_launches, hist_rows, hist_bytes, split_ms, part_ms, iter_ms, iters
+  double prof[11] = {0};         // hist_ms, hist_launches, hist_rows, hist_bytes, split_ms, part_ms, iter_ms, iters,
+                                 // rows partitioned, splits computed, splits used
 
   ~Ctx() {
     for (auto e : pev) if (e) (void)hipEventDestroy(e);
@@ -954,6 +955,9 @@ static void account_profile(Ctx* c, int k) {
   }
   c->prof[2] += (double)st.hist_rows;
   c->prof[3] += hist_bytes(c, (double)st.hist_rows, (double)st.hist_items);
+  c->prof[8] += (double)st.part_rows;
+  c->prof[9] += (double)st.computed;
+  c->prof[10] += (double)st.num_splits;
 }
 
 // boosting.rs:957-1139 for device=gpu.
@@ -1577,13 +1581,13 @@ lgbmhip_status lgbmhip_predict_raw(lgbmhip_ctx* ctx, const float* X, int64_t row
 lgbmhip_status lgbmhip_predict_binned(lgbmhip_ctx* ctx, const uint8_t* bins, int64_t rows, double* out) {
   return guarded(ctx, [&] { predict(&ctx->c, nullptr, bins, rows, 0, out); });
 }
-lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out8) {
+lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out11) {
   return guarded(ctx, [&] {
     Ctx* c = &ctx->c;
-    if (out8) for (int i = 0; i < 8; ++i) out8[i] = c->prof[i];
+    if (out11) for (int i = 0; i < 11; ++i) out11[i] = c->prof[i];
     if (enable >= 0) {
       c->profile = enable;
-      for (int i = 0; i < 8; ++i) c->prof[i] = 0.0;
+      for (int i = 0; i < 11; ++i) c->prof[i] = 0.0;
       if (enable && c->pev.empty()) {
         set_device(c);
         if (!c->state_ready) setup_training(c);

@@ -59,6 +59,8 @@ __global__ void k_state_init(StepDev* st, PosRec* pos, LeafDev* leaves, uint8_t*
     st->rounds = 0;
     st->hist_items = 1;
     st->hist_rows = n;
+    st->part_rows = 0;
+    st->computed = 0;
     PosRec p;
     p.gain = -INFINITY;
     p.cgain[0] = p.cgain[1] = -INFINITY;
@@ -261,6 +263,10 @@ __device__ void apply_plan(const HistArgs& a) {
     d->right_count = (int)a.leaves[rp.lslot + 1].count;
   }
   if (tid == 0) {
+    long long moved = 0;
+    for (int w = 0; w < nw; ++w) moved += P->item[w].count;
+    st->part_rows += moved;
+    st->computed += nw;
     st->nseq = P->nseq;
     st->npending = P->npending;
     st->search_base = P->nslots - 2 * nw;
@@ -832,93 +838,132 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
       }
   }
   __syncthreads();
-  // ---- 2 + 3. replay and selection, wave 0
+  // ---- 2. replay, wave 0. Each lane keeps the best of its positions
+  // (i = lane mod 64) in registers; a replayed split changes two positions,
+  // so only their two lanes rescan before the next wave ArgMax.
   if (wave == 0) {
     int nseq = nseq0, nspl = nspl0, nrep = 0, npend = npend0;
     bool fin = false;
-    for (;;) {
-      if (nseq >= L) {
-        fin = true;
-        break;
-      }
+    auto lane_best = [&](int n) {
       Cand c = cand_none();
-      for (int i = lane; i < nseq; i += 64) {
+      for (int i = lane; i < n; i += 64) {
         Cand d;
         d.g = P[i].gain;
         d.f = P[i].feat;
         d.l = i;
         c = pick(c, d);
       }
-      c = wave_pick(c);
+      return c;
+    };
+    Cand mine = lane_best(nseq);
+    for (;;) {
+      if (nseq >= L) {
+        fin = true;
+        break;
+      }
+      const Cand c = wave_pick(mine);
       if (!(c.g > 0.0)) {
         fin = true;
         break;
       }
       const int m = c.l;
-      const int sp = P[m].spec;
-      if (sp < 0) break;   // not split yet: this round splits it
-      if (lane == 0) {
-        if (blockIdx.x == 0) {
-          PlanRep rp;
-          rp.pos = m;
-          rp.parent = P[m].slot;
-          rp.lslot = sp;
-          rp.pad = 0;
-          a.reps[nrep] = rp;
-        }
-        const PosRec q = P[m];
-        PosRec lp, rp;
+      const PosRec q = P[m];   // every lane reads the picked position
+      if (q.spec < 0) break;   // not split yet: this round splits it
+      if (lane == 0 && blockIdx.x == 0) {
+        PlanRep rp;
+        rp.pos = m;
+        rp.parent = q.slot;
+        rp.lslot = q.spec;
+        rp.pad = 0;
+        a.reps[nrep] = rp;
+      }
+      // learner/mod.rs:1105-1225: the left child keeps the leaf's number,
+      // the right child takes the next one
+      const int n = nseq;
+      if (lane == (m & 63)) {
+        PosRec lp;
         lp.gain = q.cgain[0];
         lp.feat = q.cfeat[0];
-        lp.slot = sp;
+        lp.slot = q.spec;
+        lp.spec = -1;
+        lp.cgain[0] = lp.cgain[1] = -INFINITY;
+        lp.cfeat[0] = lp.cfeat[1] = -1;
+        lp.pad = 0;
+        P[m] = lp;
+      }
+      if (lane == (n & 63)) {
+        PosRec rp;
         rp.gain = q.cgain[1];
         rp.feat = q.cfeat[1];
-        rp.slot = sp + 1;
-        lp.spec = rp.spec = -1;
-        lp.cgain[0] = lp.cgain[1] = rp.cgain[0] = rp.cgain[1] = -INFINITY;
-        lp.cfeat[0] = lp.cfeat[1] = rp.cfeat[0] = rp.cfeat[1] = -1;
-        lp.pad = rp.pad = 0;
-        P[m] = lp;        // learner/mod.rs:1105-1225: the left child keeps the leaf's number,
-        P[nseq] = rp;     // the right child takes the next one
+        rp.slot = q.spec + 1;
+        rp.spec = -1;
+        rp.cgain[0] = rp.cgain[1] = -INFINITY;
+        rp.cfeat[0] = rp.cfeat[1] = -1;
+        rp.pad = 0;
+        P[n] = rp;
       }
-      wave_sync();
       ++nseq;
       ++nspl;
       ++nrep;
       --npend;
-    }
-    int k = 0;
-    if (!fin) {
-      const int kmax = min(kMaxBatch, L - nseq);
-      while (k < kmax) {
-        Cand c = cand_none();
-        for (int i = lane; i < nseq; i += 64) {
-          if (P[i].spec >= 0) continue;
-          Cand d;
-          d.g = P[i].gain;
-          d.f = P[i].feat;
-          d.l = i;
-          c = pick(c, d);
-        }
-        c = wave_pick(c);
-        if (c.l == INT32_MAX || !(c.g > 0.0)) break;
-        if (lane == 0) {
-          ipos[k] = c.l;
-          P[c.l].spec = nslots0 + 2 * k;   // split this round: its children's slots
-          P[c.l].cgain[0] = P[c.l].cgain[1] = -INFINITY;
-          P[c.l].cfeat[0] = P[c.l].cfeat[1] = -1;
-        }
-        wave_sync();
-        ++k;
-      }
+      wave_sync();   // the two writes are visible to the next pick's broadcast read
+      if (lane == (m & 63) || lane == (n & 63)) mine = lane_best(nseq);
     }
     if (lane == 0) {
-      s_k = k;
       s_fin = fin ? 1 : 0;
       s_nseq = nseq;
       s_nspl = nspl;
       s_nrep = nrep;
-      s_npend = npend + k;
+      s_npend = npend;
+    }
+  }
+  __syncthreads();
+  // ---- 3. selection, all threads: the round splits the unsplit leaves in
+  // ArgMax order, at most kMaxBatch, and only leaves the tree can still
+  // reach (every leaf split ahead of order that ranks above a leaf is taken
+  // before it, and the tree has L - nseq leaves of room). A leaf's rank among
+  // the unsplit leaves is its item index; the chosen set is a prefix of that
+  // order.
+  {
+    const int nseq = s_nseq;
+    if (tid == 0) s_k = 0;
+    __syncthreads();
+    if (!s_fin) {
+      for (int i = tid; i < nseq; i += kPartThreads) {
+        const PosRec& q = P[i];
+        if (q.spec >= 0 || !(q.gain > 0.0)) continue;
+        Cand ci;
+        ci.g = q.gain;
+        ci.f = q.feat;
+        ci.l = i;
+        int rank = 0, above = 0;
+        for (int j = 0; j < nseq; ++j) {
+          if (j == i) continue;
+          Cand cj;
+          cj.g = P[j].gain;
+          cj.f = P[j].feat;
+          cj.l = j;
+          if (pick(ci, cj).l == j) {
+            if (P[j].spec >= 0) ++above;
+            else if (cj.g > 0.0) ++rank;
+          }
+        }
+        if (rank < kMaxBatch && rank + above + 1 <= L - nseq) {
+          ipos[rank] = i;
+          atomicAdd(&s_k, 1);
+        }
+      }
+    }
+    __syncthreads();
+    const int k = s_k;
+    if (tid < k) {
+      PosRec& q = P[ipos[tid]];
+      q.spec = nslots0 + 2 * tid;   // split this round: its children's slots
+      q.cgain[0] = q.cgain[1] = -INFINITY;
+      q.cfeat[0] = q.cfeat[1] = -1;
+    }
+    if (tid == 0) {
+      s_npend += k;
       // a child of this round can be split only after its parent is
       // replayed: two more leaves
       s_complete = nseq + 2 > L ? 1 : 0;
