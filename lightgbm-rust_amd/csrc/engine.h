@@ -122,6 +122,7 @@ struct PosRec {
 // A split of the current round.
 struct PlanItem {
   SplitRec rec;
+  FeatDev fd;               // the split feature
   long long begin, count;   // the parent's segment
   int parent, lslot;        // slots: parent, left child (right = lslot + 1)
   int buf, depth;           // the parent's buffer and depth

@@ -444,6 +444,7 @@ static void setup_training(Ctx* c) {
   if (o.objective == 2) setup_rank(c);
   c->L = std::max(c->sp.num_leaves, 2);
   if (c->L > 1024) throw Error(LGBMHIP_UNSUPPORTED, "num_leaves above 1024");
+  if (c->F >= 65535) throw Error(LGBMHIP_UNSUPPORTED, "65535 or more used features");
   if (quant) {
     for (auto& f : c->feats)
       if (f.bin_type == 1) throw Error(LGBMHIP_UNSUPPORTED, "quantized gradients with categorical features");
@@ -485,7 +486,7 @@ static void setup_training(Ctx* c) {
   c->it.zero(c->stream);
   c->leafval.alloc((size_t)K * L);
   c->leafraw.alloc((size_t)K * L);
-  set_part_lds_limit((size_t)L * sizeof(PosRec));
+  set_part_lds_limit(part_lds(L));
   c->renew.alloc((size_t)2 * L);
   c->renew.zero(c->stream);
   // defaults from sweeps on 10.5M x 28 (quantized and FP64 forms)
@@ -678,7 +679,7 @@ static void enqueue_head(Ctx* c, int k) {
   const bool quant = c->opts.use_quantized_grad != 0;
   const float* gk = c->g.p + (size_t)k * N;
   const float* hk = c->h.p + (size_t)k * N;
-  const int ig = grid_for(N, 256, c->cus * 8);
+  const int ig = grid_for(N, 256, c->cus * 4);
   if (quant) {
     hipLaunchKernelGGL(k_discretize, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, c->grv.p, c->hrv.p,
                        c->opts.num_grad_quant_bins, c->opts.stochastic_rounding, 0, c->starts.p, c->call_ctr.p,
@@ -869,25 +870,12 @@ __global__ void k_add_const(double* s, long long n, double v) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) s[i] += v;
 }
 
-__global__ void k_absmax(const float* g, const float* h, long long n, IterDev* it) {
-  unsigned mg = 0, mh = 0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    mg = max(mg, __float_as_uint(fabsf(g[i])));
-    mh = max(mh, __float_as_uint(fabsf(h[i])));
-  }
-  mg = wave_max_u(mg);
-  mh = wave_max_u(mh);
-  if ((threadIdx.x & 63) == 0) {
-    atomicMax(&it->max_abs_g_bits, mg);
-    atomicMax(&it->max_abs_h_bits, mh);
-  }
-}
 
 static void enqueue_gradients(Ctx* c) {
   hipStream_t s = c->stream;
   const int64_t N = c->N;
   hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(64), 0, s, c->it.p, c->K);
-  const int grid = grid_for(N, 256, c->cus * 8);
+  const int grid = grid_for(N, 256, c->cus * 4);
   const float* w = c->has_weight ? c->d_weight.p : nullptr;
   if (c->opts.objective == 0) {
     hipLaunchKernelGGL(k_binary_grad, dim3(grid), dim3(256), 0, s, c->score.p, c->d_label.p, w, (long long)N,
@@ -1032,7 +1020,7 @@ static void train_tree(Ctx* c, const float* g, const float* h) {
   HIP_CHECK(hipMemcpyAsync(c->g.p, g, N * 4, hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(hipMemcpyAsync(c->h.p, h, N * 4, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(64), 0, c->stream, c->it.p, c->K);
-  hipLaunchKernelGGL(k_absmax, dim3(grid_for(N, 256, c->cus * 8)), dim3(256), 0, c->stream, c->g.p, c->h.p,
+  hipLaunchKernelGGL(k_absmax_it, dim3(grid_for(N, 256, c->cus * 4)), dim3(256), 0, c->stream, c->g.p, c->h.p,
                      (long long)N, c->it.p);
   if (c->opts.use_quantized_grad && c->n_calls + 1 > (int)c->starts.n)
     throw Error(LGBMHIP_UNSUPPORTED, "more quantized trees than the start-offset table holds (65536)");
@@ -1397,9 +1385,9 @@ lgbmhip_status lgbmhip_discretize(lgbmhip_ctx* ctx, const float* grad, const flo
     HIP_CHECK(hipMemcpyAsync(c->h.p, hess, N * 4, hipMemcpyHostToDevice, c->stream));
     for (int call = 0; call < num_calls; ++call) {
       hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(64), 0, c->stream, c->it.p, 1);
-      hipLaunchKernelGGL(k_absmax, dim3(grid_for(N, 256, c->cus * 8)), dim3(256), 0, c->stream, c->g.p, c->h.p,
+      hipLaunchKernelGGL(k_absmax_it, dim3(grid_for(N, 256, c->cus * 4)), dim3(256), 0, c->stream, c->g.p, c->h.p,
                          (long long)N, c->it.p);
-      hipLaunchKernelGGL(k_discretize, dim3(grid_for(N, 256, c->cus * 8)), dim3(256), 0, c->stream, (long long)N,
+      hipLaunchKernelGGL(k_discretize, dim3(grid_for(N, 256, c->cus * 4)), dim3(256), 0, c->stream, (long long)N,
                          c->g.p, c->h.p, c->grv.p, c->hrv.p, c->opts.num_grad_quant_bins, c->opts.stochastic_rounding,
                          0, c->starts.p, c->call_ctr.p, c->it.p, c->idx[0].p, (char2*)c->gh[0].p, c->gq.p, c->hq.p);
       hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, c->stream, c->call_ctr.p);

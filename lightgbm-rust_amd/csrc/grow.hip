@@ -741,8 +741,66 @@ __device__ __forceinline__ Cand cand_none() {
   return c;
 }
 
-// One partition pass per round. Every block first derives the round's plan
-// from the same inputs - no block waits for another:
+// The same order as packed keys: k1 = the gain as an order-preserving 64-bit
+// integer (larger is better), k2 = (feature, position) with a missing
+// feature last (smaller is better). Positions < 65536, features < 65535.
+struct KeyRec {
+  unsigned long long k1;
+  unsigned k2;
+  int spec;   // >= 0: split ahead of order
+};
+
+__device__ __forceinline__ unsigned long long gain_key(double g) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(g);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double key_gain(unsigned long long k) {
+  const unsigned long long b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+__device__ __forceinline__ unsigned tie_key(int feat, int pos) {
+  return ((unsigned)(feat < 0 ? 0xFFFF : feat) << 16) | (unsigned)pos;
+}
+__device__ __forceinline__ bool key_better(unsigned long long a1, unsigned a2, unsigned long long b1, unsigned b2) {
+  return a1 > b1 || (a1 == b1 && a2 < b2);
+}
+
+// DPP step of a wave reduction: this lane's value and the one DPP control
+// CTRL brings from another lane (rows outside ROWMASK keep their own).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ unsigned dpp_from(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROWMASK, 0xf, false);
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void dpp_best_step(unsigned& h, unsigned& l, unsigned& t) {
+  const unsigned oh = dpp_from<CTRL, ROWMASK>(h), ol = dpp_from<CTRL, ROWMASK>(l), ot = dpp_from<CTRL, ROWMASK>(t);
+  const unsigned long long a = ((unsigned long long)h << 32) | l, b = ((unsigned long long)oh << 32) | ol;
+  const bool take = key_better(b, ot, a, t);
+  h = take ? oh : h;
+  l = take ? ol : l;
+  t = take ? ot : t;
+}
+
+// Wave ArgMax of (k1, k2) pairs with DPP row operations (quad swaps, row
+// mirrors, row broadcasts): lane 63 ends with the best pair, read back into
+// every lane.
+__device__ __forceinline__ void wave_best_key(unsigned long long& k1, unsigned& k2) {
+  unsigned h = (unsigned)(k1 >> 32), l = (unsigned)k1, t = k2;
+  dpp_best_step<0xB1, 0xF>(h, l, t);    // quad_perm [1,0,3,2]
+  dpp_best_step<0x4E, 0xF>(h, l, t);    // quad_perm [2,3,0,1]
+  dpp_best_step<0x141, 0xF>(h, l, t);   // row_half_mirror
+  dpp_best_step<0x140, 0xF>(h, l, t);   // row_mirror
+  dpp_best_step<0x142, 0xA>(h, l, t);   // row_bcast:15 into rows 1, 3
+  dpp_best_step<0x143, 0xC>(h, l, t);   // row_bcast:31 into rows 2, 3
+  h = __builtin_amdgcn_readlane(h, 63);
+  l = __builtin_amdgcn_readlane(l, 63);
+  t = __builtin_amdgcn_readlane(t, 63);
+  k1 = ((unsigned long long)h << 32) | l;
+  k2 = t;
+}
+
+// The plan of a round, one block:
 //  1. each leaf searched by the last split pass gets its best split
 //     (learner/mod.rs:1064-1101);
 //  2. replay: the sequential ArgMax over the tree's leaves (:698-706), with
@@ -750,14 +808,8 @@ __device__ __forceinline__ Cand cand_none() {
 //     each pick that was already split, until it picks a leaf that was not;
 //  3. the round splits that leaf and the next best unsplit leaves, at most
 //     kMaxBatch and no more than the tree can still take.
-// Block 0 publishes the plan, the next position array and the searched
-// leaves' best splits. Then every block moves its tiles' rows - index,
-// gradients and group-major bins - into the children's segments of the other
-// buffer: left rows from the front, right rows from the back, each tile
-// reserving its slots with one atomic per side (DataPartition::split,
-// partition.rs:107-174). The row order inside a child is unspecified: every
-// consumer of a segment (exact fixed-point or integer histograms, renewal
-// sums, score update) is order-independent.
+// It publishes the plan, the next position array and the searched leaves'
+// best splits.
 template <int VW>
 struct RowVec;   // the vector type that moves VW bytes of a row
 template <>
@@ -769,12 +821,11 @@ struct RowVec<8> { using T = u32x2; };
 template <>
 struct RowVec<16> { using T = u32x4; };
 
-template <int VW>
-__global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
-  using VT = typename RowVec<VW>::T;
-  constexpr int VWW = VW / 4;   // words per vector
+template <int RP>   // positions per lane: num_leaves <= 64 RP
+__global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
   StepDev* st = a.st;
-  extern __shared__ PosRec P[];   // this block's copy of the position array
+  extern __shared__ PosRec P[];   // this block's copy of the position array, then its keys
+  KeyRec* Q = (KeyRec*)(P + a.L);
   constexpr int NW = kPartThreads / 64;
   constexpr int RW = (int)(sizeof(SplitRec) / 4);
   __shared__ Cand rowbest[2 * kMaxBatch];
@@ -784,7 +835,6 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
   __shared__ int ipos[kMaxBatch], iparent[kMaxBatch], ibuf[kMaxBatch], idepth[kMaxBatch], itile0[kMaxBatch + 1];
   __shared__ int s_k, s_fin, s_nseq, s_nspl, s_nrep, s_npend, s_complete;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // st->done changes only in the histogram pass: uniform here
   if (st->done) return;
   const int nseq0 = st->nseq, npend0 = st->npending, nslots0 = st->nslots, nspl0 = st->num_splits;
   const int par = st->parity, sbase = st->search_base, nsearched = st->nsearched;
@@ -828,6 +878,9 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
       q.gain = rowbest[rs].g;
       q.feat = rowbest[rs].f;
     }
+    Q[i].k1 = gain_key(q.gain);
+    Q[i].k2 = tie_key(q.feat, i);
+    Q[i].spec = q.spec;
     if (q.spec >= 0)
       for (int c = 0; c < 2; ++c) {
         const int rc = q.spec + c - sbase;
@@ -838,76 +891,141 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
       }
   }
   __syncthreads();
-  // ---- 2. replay, wave 0. Each lane keeps the best of its positions
-  // (i = lane mod 64) in registers; a replayed split changes two positions,
-  // so only their two lanes rescan before the next wave ArgMax.
+  // ---- 2. replay, wave 0. Each lane keeps its positions i = lane + 64 r
+  // in registers (keys, slot, and for a leaf split ahead of order its child
+  // slots and their keys); a picked position's data reaches every lane by
+  // readlane, and a replayed split changes two positions, so only their two
+  // lanes rescan before the next wave ArgMax. LDS copies are written for the
+  // selection and the next position array.
   if (wave == 0) {
     int nseq = nseq0, nspl = nspl0, nrep = 0, npend = npend0;
     bool fin = false;
-    auto lane_best = [&](int n) {
-      Cand c = cand_none();
-      for (int i = lane; i < n; i += 64) {
-        Cand d;
-        d.g = P[i].gain;
-        d.f = P[i].feat;
-        d.l = i;
-        c = pick(c, d);
+    unsigned long long r1[RP], c1a[RP], c1b[RP];
+    unsigned r2[RP];
+    int rsl[RP], rsp[RP], cfa[RP], cfb[RP];
+#pragma unroll
+    for (int r = 0; r < RP; ++r) {
+      const int i = lane + 64 * r;
+      r1[r] = 0;
+      r2[r] = 0xFFFFFFFFu;
+      rsl[r] = rsp[r] = -1;
+      c1a[r] = c1b[r] = 0;
+      cfa[r] = cfb[r] = -1;
+      if (i < nseq) {
+        const PosRec& q = P[i];
+        r1[r] = Q[i].k1;
+        r2[r] = Q[i].k2;
+        rsl[r] = q.slot;
+        rsp[r] = q.spec;
+        c1a[r] = gain_key(q.cgain[0]);
+        c1b[r] = gain_key(q.cgain[1]);
+        cfa[r] = q.cfeat[0];
+        cfb[r] = q.cfeat[1];
       }
-      return c;
+    }
+    auto lane_best = [&](unsigned long long& b1, unsigned& b2) {
+      b1 = r1[0];
+      b2 = r2[0];
+#pragma unroll
+      for (int r = 1; r < RP; ++r)
+        if (key_better(r1[r], r2[r], b1, b2)) {
+          b1 = r1[r];
+          b2 = r2[r];
+        }
     };
-    Cand mine = lane_best(nseq);
+    auto bcast_u32 = [&](unsigned v, int src) { return (unsigned)__builtin_amdgcn_readlane((int)v, src); };
+    unsigned long long m1;
+    unsigned m2;
+    lane_best(m1, m2);
     for (;;) {
       if (nseq >= L) {
         fin = true;
         break;
       }
-      const Cand c = wave_pick(mine);
-      if (!(c.g > 0.0)) {
+      unsigned long long c1 = m1;
+      unsigned c2 = m2;
+      wave_best_key(c1, c2);
+      if (!(c1 > gain_key(0.0))) {   // the best gain is not positive (or there is no leaf)
         fin = true;
         break;
       }
-      const int m = c.l;
-      const PosRec q = P[m];   // every lane reads the picked position
-      if (q.spec < 0) break;   // not split yet: this round splits it
+      const int m = (int)(c2 & 0xFFFF);
+      const int om = m & 63, rm = m >> 6;
+      // the picked position's slot, split state and child keys, from its lane
+      int xsl = 0, xsp = 0, xfa = 0, xfb = 0;
+      unsigned long long xa = 0, xb = 0;
+#pragma unroll
+      for (int r = 0; r < RP; ++r)
+        if (r == rm) {
+          xsl = rsl[r];
+          xsp = rsp[r];
+          xfa = cfa[r];
+          xfb = cfb[r];
+          xa = c1a[r];
+          xb = c1b[r];
+        }
+      const int spec = (int)bcast_u32((unsigned)xsp, om);
+      if (spec < 0) break;   // not split yet: this round splits it
+      const int pslot = (int)bcast_u32((unsigned)xsl, om);
+      const int fa = (int)bcast_u32((unsigned)xfa, om), fb = (int)bcast_u32((unsigned)xfb, om);
+      const unsigned long long ka = ((unsigned long long)bcast_u32((unsigned)(xa >> 32), om) << 32) |
+                                    bcast_u32((unsigned)xa, om);
+      const unsigned long long kb = ((unsigned long long)bcast_u32((unsigned)(xb >> 32), om) << 32) |
+                                    bcast_u32((unsigned)xb, om);
       if (lane == 0 && blockIdx.x == 0) {
         PlanRep rp;
         rp.pos = m;
-        rp.parent = q.slot;
-        rp.lslot = q.spec;
+        rp.parent = pslot;
+        rp.lslot = spec;
         rp.pad = 0;
         a.reps[nrep] = rp;
       }
       // learner/mod.rs:1105-1225: the left child keeps the leaf's number,
       // the right child takes the next one
       const int n = nseq;
-      if (lane == (m & 63)) {
-        PosRec lp;
-        lp.gain = q.cgain[0];
-        lp.feat = q.cfeat[0];
-        lp.slot = q.spec;
-        lp.spec = -1;
-        lp.cgain[0] = lp.cgain[1] = -INFINITY;
-        lp.cfeat[0] = lp.cfeat[1] = -1;
-        lp.pad = 0;
-        P[m] = lp;
-      }
-      if (lane == (n & 63)) {
-        PosRec rp;
-        rp.gain = q.cgain[1];
-        rp.feat = q.cfeat[1];
-        rp.slot = q.spec + 1;
-        rp.spec = -1;
-        rp.cgain[0] = rp.cgain[1] = -INFINITY;
-        rp.cfeat[0] = rp.cfeat[1] = -1;
-        rp.pad = 0;
-        P[n] = rp;
+      const int on = n & 63, rn = n >> 6;
+      const unsigned k2a = tie_key(fa, m), k2b = tie_key(fb, n);
+#pragma unroll
+      for (int r = 0; r < RP; ++r) {
+        if (lane == om && r == rm) {
+          r1[r] = ka;
+          r2[r] = k2a;
+          rsl[r] = spec;
+          rsp[r] = -1;
+        }
+        if (lane == on && r == rn) {
+          r1[r] = kb;
+          r2[r] = k2b;
+          rsl[r] = spec + 1;
+          rsp[r] = -1;
+        }
       }
       ++nseq;
       ++nspl;
       ++nrep;
       --npend;
-      wave_sync();   // the two writes are visible to the next pick's broadcast read
-      if (lane == (m & 63) || lane == (n & 63)) mine = lane_best(nseq);
+      if (lane == om || lane == on) lane_best(m1, m2);
+    }
+    // LDS copies of this wave's final positions
+#pragma unroll
+    for (int r = 0; r < RP; ++r) {
+      const int i = lane + 64 * r;
+      if (i < nseq) {
+        PosRec q;
+        q.gain = key_gain(r1[r]);
+        q.feat = (int)(r2[r] >> 16) == 0xFFFF ? -1 : (int)(r2[r] >> 16);
+        q.slot = rsl[r];
+        q.spec = rsp[r];
+        q.cgain[0] = rsp[r] >= 0 ? key_gain(c1a[r]) : -INFINITY;
+        q.cgain[1] = rsp[r] >= 0 ? key_gain(c1b[r]) : -INFINITY;
+        q.cfeat[0] = rsp[r] >= 0 ? cfa[r] : -1;
+        q.cfeat[1] = rsp[r] >= 0 ? cfb[r] : -1;
+        q.pad = 0;
+        P[i] = q;
+        Q[i].k1 = r1[r];
+        Q[i].k2 = r2[r];
+        Q[i].spec = rsp[r];
+      }
     }
     if (lane == 0) {
       s_fin = fin ? 1 : 0;
@@ -929,24 +1047,28 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
     if (tid == 0) s_k = 0;
     __syncthreads();
     if (!s_fin) {
+      const unsigned long long zero = gain_key(0.0);
       for (int i = tid; i < nseq; i += kPartThreads) {
-        const PosRec& q = P[i];
-        if (q.spec >= 0 || !(q.gain > 0.0)) continue;
-        Cand ci;
-        ci.g = q.gain;
-        ci.f = q.feat;
-        ci.l = i;
+        const KeyRec qi = Q[i];
+        if (qi.spec >= 0 || !(qi.k1 > zero)) continue;
         int rank = 0, above = 0;
-        for (int j = 0; j < nseq; ++j) {
-          if (j == i) continue;
-          Cand cj;
-          cj.g = P[j].gain;
-          cj.f = P[j].feat;
-          cj.l = j;
-          if (pick(ci, cj).l == j) {
-            if (P[j].spec >= 0) ++above;
-            else if (cj.g > 0.0) ++rank;
+        int j = 0;
+        for (; j + 8 <= nseq; j += 8) {
+          KeyRec qj[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) qj[u] = Q[j + u];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const bool bt = key_better(qj[u].k1, qj[u].k2, qi.k1, qi.k2);
+            above += (bt && qj[u].spec >= 0) ? 1 : 0;
+            rank += (bt && qj[u].spec < 0) ? 1 : 0;
           }
+        }
+        for (; j < nseq; ++j) {
+          const KeyRec qj = Q[j];
+          const bool bt = key_better(qj.k1, qj.k2, qi.k1, qi.k2);
+          above += (bt && qj.spec >= 0) ? 1 : 0;
+          rank += (bt && qj.spec < 0) ? 1 : 0;
         }
         if (rank < kMaxBatch && rank + above + 1 <= L - nseq) {
           ipos[rank] = i;
@@ -1002,7 +1124,7 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
   }
   __syncthreads();
   const int total_tiles = itile0[k];
-  if (blockIdx.x == 0) {
+  {
     // publish: the plan, the next position array, the searched leaves' best splits
     PlanDev* pl = a.plan;
     for (int e = tid; e < k * RW; e += kPartThreads) {
@@ -1019,6 +1141,7 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
       it.depth = idepth[tid];
       it.tile0 = itile0[tid];
       it.pad = 0;
+      it.fd = ifd[tid];
     }
     if (tid == 0) {
       pl->done = s_fin;
@@ -1046,6 +1169,37 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
         a.best[sbase + r] = z;
       }
   }
+}
+
+// One partition pass per round: every block moves its tiles' rows - index,
+// gradients and group-major bins - into the children's segments of the other
+// buffer: left rows from the front, right rows from the back, each tile
+// reserving its slots with one atomic per side (DataPartition::split,
+// partition.rs:107-174). The row order inside a child is unspecified: every
+// consumer of a segment (exact fixed-point or integer histograms, renewal
+// sums, score update) is order-independent.
+template <int VW>
+__global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
+  using VT = typename RowVec<VW>::T;
+  constexpr int VWW = VW / 4;   // words per vector
+  StepDev* st = a.st;
+  constexpr int NW = kPartThreads / 64;
+  constexpr int IW = (int)(sizeof(PlanItem) / 4);
+  __shared__ PlanItem items[kMaxBatch];
+  __shared__ int s_hdr[3];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (st->done) return;   // changes only in the histogram pass: uniform here
+  // one round of loads: the plan's header and every item slot
+  const PlanDev* pd = a.plan;
+  for (int e = tid; e < kMaxBatch * IW; e += kPartThreads) ((unsigned*)items)[e] = ((const unsigned*)pd->item)[e];
+  if (tid == 0) {
+    s_hdr[0] = pd->done;
+    s_hdr[1] = pd->nwork;
+    s_hdr[2] = pd->total_tiles;
+  }
+  __syncthreads();
+  if (s_hdr[0]) return;
+  const int k = s_hdr[1], total_tiles = s_hdr[2];
   if (k == 0) return;
   // ---- tiles
   const int FB = a.FB, gbytes = a.gh_bytes;
@@ -1058,11 +1212,11 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
   const unsigned long long below = (1ull << lane) - 1ull;
   int w = 0;
   for (int t = blockIdx.x; t < total_tiles; t += gridDim.x) {
-    while (t >= itile0[w + 1]) ++w;
-    const SplitRec& s = irec[w];
-    const FeatDev& fd = ifd[w];
-    const long long begin = ibegin[w], count = icount[w];
-    const int src = ibuf[w], dst = child_buf(src);   // buffer 0 is never a destination
+    while (w + 1 < k && t >= items[w + 1].tile0) ++w;
+    const SplitRec& s = items[w].rec;
+    const FeatDev& fd = items[w].fd;
+    const long long begin = items[w].begin, count = items[w].count;
+    const int src = items[w].buf, dst = child_buf(src);   // buffer 0 is never a destination
     const unsigned* idx = sel3(a.idx[0], a.idx[1], a.idx[2], src) + begin;
     unsigned* oidx = sel3(a.idx[0], a.idx[1], a.idx[2], dst) + begin;
     const uint8_t* igh = (const uint8_t*)sel3(a.gh[0], a.gh[1], a.gh[2], src) + (size_t)begin * gbytes;
@@ -1070,7 +1224,7 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
     const uint8_t* gsrc = sel3(a.grp[0], a.grp[1], a.grp[2], src) + (size_t)begin * FB;
     uint8_t* gdst = sel3(a.grp[0], a.grp[1], a.grp[2], dst) + (size_t)begin * FB;
     const uint8_t* scol = gsrc + (size_t)(s.feature / FB) * gstride + (s.feature % FB);
-    const long long t0 = (long long)(t - itile0[w]) * kPartTile;
+    const long long t0 = (long long)(t - items[w].tile0) * kPartTile;
     // one round of independent loads: index, gradients, the split feature's
     // bin and the first words of every row (lanes take consecutive rows)
     bool valid[R], left[R];
@@ -1208,7 +1362,7 @@ __device__ __forceinline__ const LeafDev& leaf_at(const StepDev* st, const PosRe
 
 // Quantized form, learner/mod.rs:736-747: each leaf's output from its float
 // gradient sums (exact fixed-point per leaf, then refit_leaf_output).
-__global__ void k_renew_sum(const StepDev* st, const PosRec* pos, const LeafDev* leaves, int L, const unsigned* idx0,
+__global__ void __launch_bounds__(256) k_renew_sum(const StepDev* st, const PosRec* pos, const LeafDev* leaves, int L, const unsigned* idx0,
                             const unsigned* idx1, const unsigned* idx2, const float* g, const float* h,
                             const IterDev* it, long long* sums /* [L][2] */) {
   const int l = blockIdx.y;
@@ -1222,9 +1376,9 @@ __global__ void k_renew_sum(const StepDev* st, const PosRec* pos, const LeafDev*
     ag += to_fixed(g[r], sg);
     ah += to_fixed(h[r], sh);
   }
-  ag = wave_sum(ag);
-  ah = wave_sum(ah);
-  if ((threadIdx.x & 63) == 0) {
+  ag = block_sum_ll(ag);
+  ah = block_sum_ll(ah);
+  if (threadIdx.x == 0) {
     atomicAdd((unsigned long long*)&sums[2 * l], (unsigned long long)ag);
     atomicAdd((unsigned long long*)&sums[2 * l + 1], (unsigned long long)ah);
   }
@@ -1351,22 +1505,26 @@ static void set_hist_lds_limits(size_t bytes) {
   for (const void* k : ks) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+static size_t part_lds(int L) { return (size_t)L * (sizeof(PosRec) + sizeof(KeyRec)); }
 static void set_part_lds_limit(size_t bytes) {
-  (void)hipFuncSetAttribute((const void*)k_grow_part<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  (void)hipFuncSetAttribute((const void*)k_grow_part<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  (void)hipFuncSetAttribute((const void*)k_grow_part<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipFuncSetAttribute((const void*)k_grow_plan<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipFuncSetAttribute((const void*)k_grow_plan<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipFuncSetAttribute((const void*)k_grow_plan<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 static void launch_split(const SplitArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_grow_split, dim3(a.F, a.root ? 1 : kMaxBatch), dim3(128), 0, s, a);
 }
 
-// Rows move in the widest vectors that divide a group's bytes.
+// The round's plan (one block), then the partition pass; rows move in the
+// widest vectors that divide a group's bytes.
 static void launch_part(const PartArgs& a, int grid, hipStream_t s) {
-  const size_t lds = (size_t)a.L * sizeof(PosRec);
-  if (a.FB % 16 == 0) hipLaunchKernelGGL(k_grow_part<16>, dim3(grid), dim3(kPartThreads), lds, s, a);
-  else if (a.FB % 8 == 0) hipLaunchKernelGGL(k_grow_part<8>, dim3(grid), dim3(kPartThreads), lds, s, a);
-  else hipLaunchKernelGGL(k_grow_part<4>, dim3(grid), dim3(kPartThreads), lds, s, a);
+  if (a.L <= 256) hipLaunchKernelGGL(k_grow_plan<4>, dim3(1), dim3(kPartThreads), part_lds(a.L), s, a);
+  else if (a.L <= 512) hipLaunchKernelGGL(k_grow_plan<8>, dim3(1), dim3(kPartThreads), part_lds(a.L), s, a);
+  else hipLaunchKernelGGL(k_grow_plan<16>, dim3(1), dim3(kPartThreads), part_lds(a.L), s, a);
+  if (a.FB % 16 == 0) hipLaunchKernelGGL(k_grow_part<16>, dim3(grid), dim3(kPartThreads), 0, s, a);
+  else if (a.FB % 8 == 0) hipLaunchKernelGGL(k_grow_part<8>, dim3(grid), dim3(kPartThreads), 0, s, a);
+  else hipLaunchKernelGGL(k_grow_part<4>, dim3(grid), dim3(kPartThreads), 0, s, a);
 }
 
 static int part_tile() { return kPartTile; }

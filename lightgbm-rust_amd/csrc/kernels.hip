@@ -122,6 +122,28 @@ __device__ __forceinline__ unsigned wave_max_u(unsigned v) {
   return v;
 }
 
+// Block-wide reductions for 256-thread blocks, result in thread 0: one
+// global atomic per block instead of one per wave (same-address atomics
+// serialize in the memory system).
+__device__ __forceinline__ unsigned block_max_u(unsigned v) {
+  __shared__ unsigned sm_max[4];
+  v = wave_max_u(v);
+  if ((threadIdx.x & 63) == 0) sm_max[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) v = max(max(sm_max[0], sm_max[1]), max(sm_max[2], sm_max[3]));
+  __syncthreads();
+  return v;
+}
+__device__ __forceinline__ long long block_sum_ll(long long v) {
+  __shared__ long long sm_sum[4];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sm_sum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) v = sm_sum[0] + sm_sum[1] + sm_sum[2] + sm_sum[3];
+  __syncthreads();
+  return v;
+}
+
 // Orders LDS traffic between the lanes of one wave (a wave-scope barrier;
 // the two waves of a search block never wait for each other).
 __device__ __forceinline__ void wave_sync() {
@@ -165,33 +187,55 @@ __global__ void k_iter_reset(IterDev* it, int K) {
 // ---------------------------------------------------------------- objectives
 // objective/binary.rs:101-127. One row per thread; max |g|, |h| by block
 // reduction then one atomicMax per block (order-free).
-__global__ void k_binary_grad(const double* __restrict__ score, const float* __restrict__ label,
-                              const float* __restrict__ weight, long long n, double sigmoid, double lw_neg,
-                              double lw_pos, float* __restrict__ g, float* __restrict__ h, IterDev* it) {
+__global__ void __launch_bounds__(256) k_binary_grad(const double* __restrict__ score, const float* __restrict__ label,
+                                                     const float* __restrict__ weight, long long n, double sigmoid,
+                                                     double lw_neg, double lw_pos, float* __restrict__ g,
+                                                     float* __restrict__ h, IterDev* it) {
+  constexpr int U = 4;   // rows in flight per thread
   unsigned mg = 0, mh = 0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    bool pos = label[i] > 0.0f;
-    double lab = pos ? 1.0 : -1.0;
-    double lw = pos ? lw_pos : lw_neg;
-    double resp = -lab * sigmoid / ((double)1.0f + exp(lab * sigmoid * score[i]));
-    double ar = fabs(resp);
-    float gv, hv;
-    if (weight) {
-      double w = (double)weight[i];
-      gv = (float)(resp * lw * w);
-      hv = (float)(ar * (sigmoid - ar) * lw * w);
-    } else {
-      gv = (float)(resp * lw);
-      hv = (float)(ar * (sigmoid - ar) * lw);
+  const long long step = (long long)gridDim.x * blockDim.x * U;
+  for (long long base = (long long)blockIdx.x * blockDim.x * U + threadIdx.x; base < n; base += step) {
+    float lv[U], wv[U];
+    double sv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = base + (long long)u * blockDim.x;
+      lv[u] = 0.f;
+      sv[u] = 0.0;
+      wv[u] = 1.f;
+      if (i < n) {
+        lv[u] = label[i];
+        sv[u] = score[i];
+        if (weight) wv[u] = weight[i];
+      }
     }
-    g[i] = gv;
-    h[i] = hv;
-    mg = max(mg, __float_as_uint(fabsf(gv)));
-    mh = max(mh, __float_as_uint(fabsf(hv)));
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = base + (long long)u * blockDim.x;
+      if (i >= n) continue;
+      const bool pos = lv[u] > 0.0f;
+      const double lab = pos ? 1.0 : -1.0;
+      const double lw = pos ? lw_pos : lw_neg;
+      const double resp = -lab * sigmoid / ((double)1.0f + exp(lab * sigmoid * sv[u]));
+      const double ar = fabs(resp);
+      float gv, hv;
+      if (weight) {
+        const double w = (double)wv[u];
+        gv = (float)(resp * lw * w);
+        hv = (float)(ar * (sigmoid - ar) * lw * w);
+      } else {
+        gv = (float)(resp * lw);
+        hv = (float)(ar * (sigmoid - ar) * lw);
+      }
+      g[i] = gv;
+      h[i] = hv;
+      mg = max(mg, __float_as_uint(fabsf(gv)));
+      mh = max(mh, __float_as_uint(fabsf(hv)));
+    }
   }
-  mg = wave_max_u(mg);
-  mh = wave_max_u(mh);
-  if ((threadIdx.x & 63) == 0) {
+  mg = block_max_u(mg);
+  mh = block_max_u(mh);
+  if (threadIdx.x == 0) {
     atomicMax(&it->max_abs_g_bits, mg);
     atomicMax(&it->max_abs_h_bits, mh);
   }
@@ -386,15 +430,15 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
 }
 
 // max |g|, |h| after the ranking gradients (order-free).
-__global__ void k_absmax_it(const float* g, const float* h, long long n, IterDev* it) {
+__global__ void __launch_bounds__(256) k_absmax_it(const float* g, const float* h, long long n, IterDev* it) {
   unsigned mg = 0, mh = 0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     mg = max(mg, __float_as_uint(fabsf(g[i])));
     mh = max(mh, __float_as_uint(fabsf(h[i])));
   }
-  mg = wave_max_u(mg);
-  mh = wave_max_u(mh);
-  if ((threadIdx.x & 63) == 0) {
+  mg = block_max_u(mg);
+  mh = block_max_u(mh);
+  if (threadIdx.x == 0) {
     atomicMax(&it->max_abs_g_bits, mg);
     atomicMax(&it->max_abs_h_bits, mh);
   }
@@ -404,7 +448,7 @@ __global__ void k_absmax_it(const float* g, const float* h, long long n, IterDev
 // Gradients of class k in buffer 0 (the root, rows in order; the identity
 // row index of buffer 0 is written once per training run), fixed-point root
 // sums (learner/mod.rs:580-628).
-__global__ void k_tree_init(long long n, const float* __restrict__ g, const float* __restrict__ h,
+__global__ void __launch_bounds__(256) k_tree_init(long long n, const float* __restrict__ g, const float* __restrict__ h,
                             unsigned* __restrict__ idx0, float2* __restrict__ gh0, IterDev* it) {
   const int sg = fix_shift(n, it->max_abs_g_bits), sh = fix_shift(n, it->max_abs_h_bits);
   long long ag = 0, ah = 0;
@@ -415,9 +459,9 @@ __global__ void k_tree_init(long long n, const float* __restrict__ g, const floa
     ag += to_fixed(gv, sg);
     ah += to_fixed(hv, sh);
   }
-  ag = wave_sum(ag);
-  ah = wave_sum(ah);
-  if ((threadIdx.x & 63) == 0) {
+  ag = block_sum_ll(ag);
+  ah = block_sum_ll(ah);
+  if (threadIdx.x == 0) {
     atomicAdd((unsigned long long*)&it->root_g_fix, (unsigned long long)ag);
     atomicAdd((unsigned long long*)&it->root_h_fix, (unsigned long long)ah);
   }
@@ -426,7 +470,7 @@ __global__ void k_tree_init(long long n, const float* __restrict__ g, const floa
 // Quantized form: grad/hess -> int8 by stochastic rounding (learner/quant.rs:92-129),
 // packed (gq, hq) into buffer 0; exact int root sums. idx0 / gq_out / hq_out
 // are written when given (the stage entry point).
-__global__ void k_discretize(long long n, const float* __restrict__ g, const float* __restrict__ h,
+__global__ void __launch_bounds__(256) k_discretize(long long n, const float* __restrict__ g, const float* __restrict__ h,
                              const double* __restrict__ grv, const double* __restrict__ hrv, int bins, int stochastic,
                              int constant_hessian, const int* __restrict__ starts, const int* call_ctr, IterDev* it,
                              unsigned* __restrict__ idx0, char2* __restrict__ q0, int8_t* __restrict__ gq_out,
@@ -459,10 +503,10 @@ __global__ void k_discretize(long long n, const float* __restrict__ g, const flo
     agq += a;
     ahq += b;
   }
-  ai = wave_sum(ai);
-  agq = wave_sum(agq);
-  ahq = wave_sum(ahq);
-  if ((threadIdx.x & 63) == 0) {
+  ai = block_sum_ll(ai);
+  agq = block_sum_ll(agq);
+  ahq = block_sum_ll(ahq);
+  if (threadIdx.x == 0) {
     atomicAdd((unsigned long long*)&it->root_int_sum, (unsigned long long)ai);
     atomicAdd((unsigned long long*)&it->root_gq_sum, (unsigned long long)agq);
     atomicAdd((unsigned long long*)&it->root_hq_sum, (unsigned long long)ahq);
