@@ -199,10 +199,12 @@ struct Ctx {
   bool state_ready = false;
   DevBuf<double> score;
   DevBuf<float> g, h;
-  // three buffers of (row index, gradients, group-major bins): buffer 0
+  // three buffers of (gradients, group-major bins) in leaf order: buffer 0
   // holds the root in row order, leaves alternate between buffers 1 and 2
-  DevBuf<unsigned> idx[3];
   DevBuf<uint8_t> gh[3];
+  DevBuf<unsigned> rows_buf;   // stage entry points: caller row lists
+  DevBuf<NodeDev> nodes;       // the last tree's nodes
+  DevBuf<int> leaf_shift;      // [L][2] renewal fixed-point shifts
   DevBuf<uint8_t> grp[3];
   DevBuf<long long> acc;     // each item's histogram summed over its chunks, one unit per group
   DevBuf<int> acc_units;     // {0, G, 2G, ...}: acc read as one unit per (item, group)
@@ -457,10 +459,10 @@ static void setup_training(Ctx* c) {
   c->score.zero(c->stream);
   c->g.alloc((size_t)K * N);
   c->h.alloc((size_t)K * N);
-  for (int b = 0; b < 3; ++b) {
-    c->idx[b].alloc(N);
-    c->gh[b].alloc((size_t)N * (quant ? 2 : 8));
-  }
+  for (int b = 0; b < 3; ++b) c->gh[b].alloc((size_t)N * (quant ? 2 : 8));
+  c->nodes.alloc(L);
+  c->leaf_shift.alloc((size_t)2 * L);
+  set_walk_lds_limit(L);
   // leaf slots: at most L - 1 replayed splits plus at most L splits made
   // ahead of order and never replayed, two slots each, plus the root
   c->MS = 4 * L;
@@ -536,11 +538,8 @@ static void setup_training(Ctx* c) {
     const int64_t max_units = ((N + c->hist_min_rows - 1) / c->hist_min_rows + kMaxBatch + 1) * c->hist_ng;
     c->slab.alloc((size_t)max_units * c->hist_fb * B * (quant ? 1 : 2));
   }
-  // buffer 0: the root's rows in row order (identity index), bins group-major
+  // buffer 0: the root's rows in row order, bins group-major
   {
-    std::vector<unsigned> iota(N);
-    for (int64_t i = 0; i < N; ++i) iota[i] = (unsigned)i;
-    HIP_CHECK(hipMemcpy(c->idx[0].p, iota.data(), N * 4, hipMemcpyHostToDevice));
     const size_t gbytes = (size_t)c->hist_ng * N * c->hist_fb;
     for (int b = 0; b < 3; ++b) c->grp[b].alloc(gbytes);
     hipLaunchKernelGGL(k_group_rows, dim3(grid_for(N * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
@@ -637,7 +636,6 @@ static PartArgs part_args(Ctx* c) {
   a.feat_split = c->feat_split.p;
   for (int b = 0; b < 3; ++b) {
     a.grp[b] = c->grp[b].p;
-    a.idx[b] = c->idx[b].p;
     a.gh[b] = c->gh[b].p;
   }
   a.FB = c->hist_fb;
@@ -723,22 +721,47 @@ static void enqueue_rounds(Ctx* c, int k, int r0, int n) {
 
 // The tree's end: quantized leaf renewal, leaf values, score update. No-ops
 // while the tree is still growing.
+static WalkArgs walk_args(Ctx* c, int mode) {
+  WalkArgs w{};
+  w.st = c->st.p;
+  w.nodes = c->nodes.p;
+  w.feats = c->d_feats.p;
+  w.bins = c->bins_rm.p;
+  w.S = c->S;
+  w.L = c->L;
+  w.N = c->N;
+  w.shifts = c->leaf_shift.p;
+  w.sums = c->renew.p;
+  w.mode = mode;
+  return w;
+}
+
+// The tree's end: its nodes, quantized leaf renewal, leaf values, score
+// update (the last two walk the tree over every row's bins in row order).
+// No-ops while the tree is still growing.
 static void enqueue_tail(Ctx* c, int k, bool score_update, double lr) {
   hipStream_t s = c->stream;
   const int64_t N = c->N;
   const bool quant = c->opts.use_quantized_grad != 0;
-  const float* gk = c->g.p + (size_t)k * N;
-  const float* hk = c->h.p + (size_t)k * N;
   const bool renew = quant && c->opts.quant_train_renew_leaf;
-  if (renew)
-    hipLaunchKernelGGL(k_renew_sum, dim3(64, c->L), dim3(256), 0, s, c->st.p, c->pos.p, c->leaves.p, c->L,
-                       c->idx[0].p, c->idx[1].p, c->idx[2].p, gk, hk, c->it.p + k, c->renew.p);
+  const int wgrid = grid_for(N, 256, c->cus * 4);
+  hipLaunchKernelGGL(k_tree_nodes, dim3(1), dim3(256), 0, s, c->st.p, c->log.p, c->pos.p, c->leaves.p, c->it.p + k,
+                     c->L, c->nodes.p, c->leaf_shift.p);
+  if (renew) {
+    WalkArgs w = walk_args(c, 0);
+    w.g = c->g.p + (size_t)k * N;
+    w.h = c->h.p + (size_t)k * N;
+    launch_walk(w, wgrid, s);
+  }
   hipLaunchKernelGGL(k_leaf_values, dim3((c->L + 255) / 256), dim3(256), 0, s, c->st.p, c->pos.p, c->leaves.p,
                      c->it.p + k, c->renew.p, renew ? 1 : 0, c->sp, lr, c->L, c->leafval.p + (size_t)k * c->L,
                      c->leafraw.p + (size_t)k * c->L);
-  if (score_update)
-    hipLaunchKernelGGL(k_score, dim3(64, c->L), dim3(256), 0, s, c->st.p, c->pos.p, c->leaves.p, c->L, c->idx[0].p,
-                       c->idx[1].p, c->idx[2].p, c->leafval.p + (size_t)k * c->L, c->score.p + (size_t)k * N);
+  if (score_update) {
+    WalkArgs w = walk_args(c, 1);
+    w.val = c->leafval.p + (size_t)k * c->L;
+    w.score = c->score.p + (size_t)k * N;
+    launch_walk(w, wgrid, s);
+  }
 }
 
 static hipGraphExec_t capture(Ctx* c, const std::function<void()>& body) {
@@ -1315,8 +1338,9 @@ lgbmhip_status lgbmhip_leaf_of_row(lgbmhip_ctx* ctx, int32_t* out) {
     set_device(c);
     DevBuf<int> d;
     d.alloc(c->N);
-    hipLaunchKernelGGL(k_leaf_of_row, dim3(64, c->L), dim3(256), 0, c->stream, c->st.p, c->pos.p, c->leaves.p, c->L,
-                       c->idx[0].p, c->idx[1].p, c->idx[2].p, d.p);
+    WalkArgs w = walk_args(c, 2);
+    w.out = d.p;
+    launch_walk(w, grid_for(c->N, 256, c->cus * 4), c->stream);
     HIP_CHECK(hipMemcpyAsync(out, d.p, c->N * 4, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
@@ -1389,7 +1413,8 @@ lgbmhip_status lgbmhip_discretize(lgbmhip_ctx* ctx, const float* grad, const flo
                          (long long)N, c->it.p);
       hipLaunchKernelGGL(k_discretize, dim3(grid_for(N, 256, c->cus * 4)), dim3(256), 0, c->stream, (long long)N,
                          c->g.p, c->h.p, c->grv.p, c->hrv.p, c->opts.num_grad_quant_bins, c->opts.stochastic_rounding,
-                         0, c->starts.p, c->call_ctr.p, c->it.p, c->idx[0].p, (char2*)c->gh[0].p, c->gq.p, c->hq.p);
+                         0, c->starts.p, c->call_ctr.p, c->it.p, (unsigned*)nullptr, (char2*)c->gh[0].p, c->gq.p,
+                         c->hq.p);
       hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, c->stream, c->call_ctr.p);
       c->n_calls++;
     }
@@ -1434,10 +1459,11 @@ static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc
   }
   if (m > 0) {
     // the rows' bins gathered into positions 0..m-1 of ordered buffer 1
-    HIP_CHECK(hipMemcpyAsync(c->idx[1].p, rows, m * 4, hipMemcpyHostToDevice, c->stream));
+    c->rows_buf.alloc(m);
+    HIP_CHECK(hipMemcpyAsync(c->rows_buf.p, rows, m * 4, hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(c->gh[1].p, pk.data(), pk.size(), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_group_rows, dim3(grid_for(m * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
-                       c->stream, (const unsigned*)c->idx[1].p, (long long)m, c->bins_rm.p, c->S, c->F, c->hist_fb,
+                       c->stream, (const unsigned*)c->rows_buf.p, (long long)m, c->bins_rm.p, c->S, c->F, c->hist_fb,
                        c->hist_ng, (long long)N, c->grp[1].p);
   }
   HistArgs a = hist_args(c, quant, 0, 1);

@@ -85,8 +85,8 @@ __global__ void k_state_init(StepDev* st, PosRec* pos, LeafDev* leaves, uint8_t*
 // Row-wise histograms (multi_val_bin.rs:556-611 FP64, :710-791 integer).
 // FP64 values are summed as exact fixed-point int64 (ds_add_u64 in LDS):
 // exact, so the result is independent of order and block count. The
-// quantized form sums int8 (g, h) as int32 / uint32 in LDS and stores packed
-// (g << 32) + h, equal to the reference's wrapping sum of packed rows.
+// quantized form adds each row's packed (g << 32) | u8(h) with one wrapping
+// ds_add_u64 per bin, the reference's own sum of packed rows.
 //
 // Every leaf's rows are one contiguous segment of an ordered buffer (buffer
 // 0 holds the root in row order; a leaf in buffer 1 splits into buffer 2 and
@@ -136,13 +136,10 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
   const int f0 = fg * a.FB;
   const int FBg = min(a.FB, a.F - f0);
   const int B = a.B;
-  const int n_lds = FBg * B * 2;
+  // quantized: one packed (g << 32) + h int64 per bin; FP64: g and h int64
+  const int n_lds = FBg * B * (Q ? 1 : 2);
   unsigned long long* lds64 = (unsigned long long*)ldsv;
-  unsigned* lds32 = (unsigned*)ldsv;
-  for (int e = threadIdx.x; e < n_lds; e += blockDim.x) {
-    if (Q) lds32[e] = 0u;
-    else lds64[e] = 0ull;
-  }
+  for (int e = threadIdx.x; e < n_lds; e += blockDim.x) lds64[e] = 0ull;
   // most-frequent bin of each of this thread's 4 features (never accumulated)
   const int sub = threadIdx.x / TPR, j = threadIdx.x % TPR;
   int mfb[4];
@@ -170,9 +167,10 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
       if (ok[u]) {
         w[u] = *(const unsigned*)(gb + (size_t)p * a.FB);
         if (Q) {
+          // multi_val_bin.rs:710-791: the row's packed (g << 32) | u8(h)
           const char2 v = ((const char2*)ghb)[begin + p];
-          gv[u] = v.x;
-          hv[u] = (int)(uint8_t)v.y;
+          gv[u] = (long long)(((unsigned long long)(long long)v.x << 32) | (unsigned long long)(uint8_t)v.y);
+          hv[u] = 0;
         } else {
           const float2 v = ((const float2*)ghb)[begin + p];
           gv[u] = to_fixed(v.x, sg);
@@ -188,11 +186,10 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
         const int b = (w[u] >> (8 * k)) & 0xFF;
         const int fl = j * 4 + k;
         if (fl < FBg && b != mfb[k]) {
-          const size_t c = ((size_t)fl * B + b) * 2;
           if (Q) {
-            atomicAdd(lds32 + c, (unsigned)(int)gv[u]);
-            atomicAdd(lds32 + c + 1, (unsigned)hv[u]);
+            atomicAdd(lds64 + (size_t)fl * B + b, (unsigned long long)gv[u]);   // wrapping, as the reference
           } else {
+            const size_t c = ((size_t)fl * B + b) * 2;
             atomicAdd(lds64 + c, (unsigned long long)gv[u]);
             atomicAdd(lds64 + c + 1, (unsigned long long)hv[u]);
           }
@@ -201,15 +198,7 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
     }
   }
   __syncthreads();
-  if (Q) {
-    // packed (g << 32) + h, as multi_val_bin.rs:710-791 sums rows
-    for (int e = threadIdx.x; e < FBg * B; e += blockDim.x) {
-      const unsigned gsum = lds32[2 * e], hsum = lds32[2 * e + 1];
-      out_slab[e] = (long long)(((unsigned long long)(long long)(int)gsum << 32) + (unsigned long long)hsum);
-    }
-  } else {
-    for (int e = threadIdx.x; e < n_lds; e += blockDim.x) out_slab[e] = (long long)lds64[e];
-  }
+  for (int e = threadIdx.x; e < n_lds; e += blockDim.x) out_slab[e] = (long long)lds64[e];
   __syncthreads();   // the next unit reuses the LDS histogram
 }
 
@@ -695,7 +684,6 @@ struct PartArgs {
   uint8_t* grp[3];          // ordered bins per buffer, group-major [NG][N][FB]
   int FB, NG;               // bytes (features) per group, groups
   long long N;
-  unsigned* idx[3];
   void* gh[3];
   int gh_bytes;             // 8 (float2) or 2 (char2)
   int F, L;
@@ -1171,7 +1159,7 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
   }
 }
 
-// One partition pass per round: every block moves its tiles' rows - index,
+// One partition pass per round: every block moves its tiles' rows -
 // gradients and group-major bins - into the children's segments of the other
 // buffer: left rows from the front, right rows from the back, each tile
 // reserving its slots with one atomic per side (DataPartition::split,
@@ -1217,18 +1205,15 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
     const FeatDev& fd = items[w].fd;
     const long long begin = items[w].begin, count = items[w].count;
     const int src = items[w].buf, dst = child_buf(src);   // buffer 0 is never a destination
-    const unsigned* idx = sel3(a.idx[0], a.idx[1], a.idx[2], src) + begin;
-    unsigned* oidx = sel3(a.idx[0], a.idx[1], a.idx[2], dst) + begin;
     const uint8_t* igh = (const uint8_t*)sel3(a.gh[0], a.gh[1], a.gh[2], src) + (size_t)begin * gbytes;
     uint8_t* ogh = (uint8_t*)sel3(a.gh[0], a.gh[1], a.gh[2], dst) + (size_t)begin * gbytes;
     const uint8_t* gsrc = sel3(a.grp[0], a.grp[1], a.grp[2], src) + (size_t)begin * FB;
     uint8_t* gdst = sel3(a.grp[0], a.grp[1], a.grp[2], dst) + (size_t)begin * FB;
     const uint8_t* scol = gsrc + (size_t)(s.feature / FB) * gstride + (s.feature % FB);
     const long long t0 = (long long)(t - items[w].tile0) * kPartTile;
-    // one round of independent loads: index, gradients, the split feature's
-    // bin and the first words of every row (lanes take consecutive rows)
+    // one round of independent loads: gradients, the split feature's bin and
+    // the first words of every row (lanes take consecutive rows)
     bool valid[R], left[R];
-    unsigned rid[R];
     uint2 gv[R];
     VT v[R][CW / VWW];
 #pragma unroll
@@ -1237,7 +1222,6 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
       valid[u] = p < count;
       left[u] = false;
       if (valid[u]) {
-        rid[u] = idx[p];
         if (gbytes == 8) gv[u] = ((const uint2*)igh)[p];
         else gv[u].x = ((const unsigned short*)igh)[p];
         left[u] = goes_left(scol[(size_t)p * FB], fd, s);
@@ -1293,7 +1277,6 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       if (!valid[u]) continue;
-      oidx[d[u]] = rid[u];
       if (gbytes == 8) ((uint2*)ogh)[d[u]] = gv[u];
       else ((unsigned short*)ogh)[d[u]] = (unsigned short)gv[u].x;
 #pragma unroll
@@ -1352,35 +1335,244 @@ __global__ void k_group_rows(const unsigned* rows, long long m, const uint8_t* _
 }
 
 // ---------------------------------------------------------------- tree end
-// The tree's leaves are the positions 0..nseq of the current position array.
-// These kernels run after the last round; they do nothing while the tree is
-// still growing (a continuation launch runs them again).
+// After the last round the tree's leaves are the positions 0..nseq of the
+// current position array. The kernels below run after every launch of
+// rounds and do nothing while the tree still grows (a continuation launch
+// runs them again).
 __device__ __forceinline__ const LeafDev& leaf_at(const StepDev* st, const PosRec* pos, const LeafDev* leaves, int L,
                                                    int l) {
   return leaves[pos[(size_t)st->parity * L + l].slot];
 }
 
-// Quantized form, learner/mod.rs:736-747: each leaf's output from its float
-// gradient sums (exact fixed-point per leaf, then refit_leaf_output).
-__global__ void __launch_bounds__(256) k_renew_sum(const StepDev* st, const PosRec* pos, const LeafDev* leaves, int L, const unsigned* idx0,
-                            const unsigned* idx1, const unsigned* idx2, const float* g, const float* h,
-                            const IterDev* it, long long* sums /* [L][2] */) {
-  const int l = blockIdx.y;
-  if (!st->done || l >= st->nseq) return;
-  const LeafDev& Lf = leaf_at(st, pos, leaves, L, l);
-  const unsigned* idx = sel3(idx0, idx1, idx2, Lf.buf) + Lf.begin;
-  const int sg = fix_shift(Lf.count, it->max_abs_g_bits), sh = fix_shift(Lf.count, it->max_abs_h_bits);
-  long long ag = 0, ah = 0;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < Lf.count; p += (long long)gridDim.x * blockDim.x) {
-    const unsigned r = idx[p];
-    ag += to_fixed(g[r], sg);
-    ah += to_fixed(h[r], sh);
+// One internal node of the finished tree: its split (feature, threshold,
+// default direction, categorical bins) and children (>= 0 node, < 0 ~leaf),
+// numbered as tree.rs:254-297 numbers them.
+struct NodeDev {
+  int feature, threshold, default_left;
+  int left, right;
+  int pad;
+  unsigned cat[kCatWords];
+};
+
+// The tree's nodes from the replayed split log: split i applied to leaf l
+// makes node i with children ~l (left keeps the leaf's number) and ~(i + 1),
+// and takes l's place under l's parent (tree.rs:254-297). Also each leaf's
+// fixed-point shifts for renewal sums (by leaf row count).
+__global__ void __launch_bounds__(256) k_tree_nodes(const StepDev* st, const SplitRec* log, const PosRec* pos,
+                                                    const LeafDev* leaves, const IterDev* it, int L, NodeDev* nodes,
+                                                    int* shifts /* [L][2] */) {
+  __shared__ int lparent[1024];   // node whose child is leaf l (-1: the root)
+  __shared__ int lside[1024];
+  const int ns = st->num_splits;
+  if (!st->done) return;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < ns; i += blockDim.x) {
+    const SplitRec& r = log[i];
+    NodeDev& n = nodes[i];
+    n.feature = r.feature;
+    n.threshold = r.threshold;
+    n.default_left = r.default_left;
+    n.pad = 0;
+#pragma unroll
+    for (int w = 0; w < kCatWords; ++w) n.cat[w] = r.cat_bits[w];
   }
-  ag = block_sum_ll(ag);
-  ah = block_sum_ll(ah);
-  if (threadIdx.x == 0) {
-    atomicAdd((unsigned long long*)&sums[2 * l], (unsigned long long)ag);
-    atomicAdd((unsigned long long*)&sums[2 * l + 1], (unsigned long long)ah);
+  for (int l = tid; l < st->nseq; l += blockDim.x) {
+    const LeafDev& lf = leaf_at(st, pos, leaves, L, l);
+    shifts[2 * l] = fix_shift(lf.count, it->max_abs_g_bits);
+    shifts[2 * l + 1] = fix_shift(lf.count, it->max_abs_h_bits);
+  }
+  __shared__ int sleaf[1024], sl[1024], sr[1024];
+  for (int i = tid; i < ns; i += blockDim.x) sleaf[i] = log[i].leaf;
+  __syncthreads();
+  if (tid == 0) {
+    lparent[0] = -1;
+    lside[0] = 0;
+    for (int i = 0; i < ns; ++i) {
+      const int l = sleaf[i], nl = i + 1;
+      sl[i] = ~l;
+      sr[i] = ~nl;
+      if (lparent[l] >= 0) {
+        if (lside[l] == 0) sl[lparent[l]] = i;
+        else sr[lparent[l]] = i;
+      }
+      lparent[l] = i;
+      lside[l] = 0;
+      lparent[nl] = i;
+      lside[nl] = 1;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < ns; i += blockDim.x) {
+    nodes[i].left = sl[i];
+    nodes[i].right = sr[i];
+  }
+}
+
+// A node packed for walking: x = feature | threshold << 16 | missing bin
+// << 24, y = flags (1 default left, 2 missing applies, 4 categorical),
+// z = left child, w = right child (>= 0 node, < 0 ~leaf).
+__device__ __forceinline__ uint4 pack_node(const NodeDev& n, const FeatDev& fd) {
+  uint4 q;
+  const bool cat = fd.bin_type == 1;
+  int mbin = 0, flags = n.default_left ? 1 : 0;
+  if (cat) {
+    flags |= 4;
+    mbin = fd.mfb;
+  } else if (fd.missing_type == 1) {
+    flags |= 2;
+    mbin = fd.default_bin;
+  } else if (fd.missing_type == 2) {
+    flags |= 2;
+    mbin = fd.num_bin - 1;
+  }
+  q.x = (unsigned)n.feature | ((unsigned)n.threshold & 0xFF) << 16 | ((unsigned)mbin & 0xFF) << 24;
+  q.y = (unsigned)flags;
+  q.z = (unsigned)n.left;
+  q.w = (unsigned)n.right;
+  return q;
+}
+
+// One step of a walk with the partition's decisions (goes_left): the child
+// of packed node q for bin b; cat: the node's categorical bin set.
+__device__ __forceinline__ int walk_step(const uint4& q, int b, const unsigned* cat) {
+  bool left;
+  const int mbin = (int)(q.x >> 24);
+  if (q.y & 4u) {
+    // goes_left_categorical: the most frequent bin 0 goes right
+    left = !(b == mbin && mbin == 0) && ((cat[b >> 5] >> (b & 31)) & 1u);
+  } else {
+    const bool miss = (q.y & 2u) && b == mbin;
+    left = miss ? (q.y & 1u) != 0 : b <= (int)((q.x >> 16) & 0xFF);
+  }
+  return (int)(left ? q.z : q.w);
+}
+
+// Rows in row order (blocks stream contiguous rows), every row walked down
+// the tree held in LDS, four rows per thread in lockstep. mode 0: renewal
+// sums (quantized form, learner/mod.rs:736-747: each leaf's float gradient
+// sums as exact fixed-point, LDS per leaf, one atomic per leaf and block);
+// mode 1: score update (tree.rs:635-652: score[row] += value of its leaf);
+// mode 2: leaf index of every row.
+struct WalkArgs {
+  const StepDev* st;
+  const NodeDev* nodes;
+  const FeatDev* feats;
+  const uint8_t* bins;   // row-major [N][S]
+  int S, L;
+  long long N;
+  const float* g;
+  const float* h;
+  const int* shifts;
+  long long* sums;       // [L][2]
+  const double* val;     // [L]
+  double* score;
+  int* out;
+  int mode;
+};
+
+// Dynamic LDS of k_walk: packed nodes, categorical bin sets, per-leaf sums,
+// values and shifts.
+__host__ __device__ constexpr size_t walk_lds(int L) {
+  return (size_t)L * (16 + 4 * kCatWords) + (size_t)L * (2 * 8 + 8 + 2 * 4);
+}
+
+template <int RW>
+__global__ void __launch_bounds__(256) k_walk(WalkArgs a) {
+  extern __shared__ unsigned long long walk_sm[];
+  uint4* nq = (uint4*)walk_sm;                                      // [L]
+  unsigned long long* lsum = (unsigned long long*)(nq + a.L);      // [2 L]
+  double* lval = (double*)(lsum + 2 * a.L);                        // [L]
+  unsigned* ncat = (unsigned*)(lval + a.L);                        // [L][kCatWords]
+  int* lsh = (int*)(ncat + (size_t)a.L * kCatWords);               // [2 L]
+  const StepDev* st = a.st;
+  if (a.mode != 2 && !st->done) return;
+  const int ns = st->num_splits, nl = ns + 1;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < ns; i += blockDim.x) {
+    const NodeDev& n = a.nodes[i];
+    nq[i] = pack_node(n, a.feats[n.feature]);
+#pragma unroll
+    for (int w = 0; w < kCatWords; ++w) ncat[(size_t)i * kCatWords + w] = n.cat[w];
+  }
+  if (a.mode == 0)
+    for (int l = tid; l < nl; l += blockDim.x) {
+      lsum[2 * l] = lsum[2 * l + 1] = 0ull;
+      lsh[2 * l] = a.shifts[2 * l];
+      lsh[2 * l + 1] = a.shifts[2 * l + 1];
+    }
+  if (a.mode == 1)
+    for (int l = tid; l < nl; l += blockDim.x) lval[l] = a.val[l];
+  __syncthreads();
+  constexpr int U = 4;   // rows walked together per thread
+  const long long step = (long long)gridDim.x * blockDim.x * U;
+  for (long long base = (long long)blockIdx.x * blockDim.x * U + tid; base < a.N; base += step) {
+    unsigned words[U][RW > 0 ? RW : 1];
+    int node[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long r = base + (long long)u * blockDim.x;
+      node[u] = (ns == 0 || r >= a.N) ? -1 : 0;   // ~0: leaf 0
+      if (RW > 0 && r < a.N) {
+        const uint8_t* row = a.bins + (size_t)r * a.S;
+#pragma unroll
+        for (int i = 0; i < RW; i += 4) {
+          if (i + 4 <= RW) {
+            const uint4 v = *(const uint4*)(row + 4 * i);
+            words[u][i] = v.x;
+            words[u][i + 1] = v.y;
+            words[u][i + 2] = v.z;
+            words[u][i + 3] = v.w;
+          } else {
+            for (int j = i; j < RW; ++j) words[u][j] = ((const unsigned*)row)[j];
+          }
+        }
+      }
+    }
+    for (;;) {
+      bool any = false;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (node[u] >= 0) {
+          const uint4 q = nq[node[u]];
+          const int f = (int)(q.x & 0xFFFF);
+          int b;
+          if (RW > 0) {
+            const int wi = f >> 2;
+            unsigned w = words[u][0];
+#pragma unroll
+            for (int i = 1; i < RW; ++i) w = wi == i ? words[u][i] : w;
+            b = (int)((w >> (8 * (f & 3))) & 0xFF);
+          } else {
+            b = a.bins[(size_t)(base + (long long)u * blockDim.x) * a.S + f];
+          }
+          node[u] = walk_step(q, b, ncat + (size_t)node[u] * kCatWords);
+          any = true;
+        }
+      }
+      if (!any) break;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long r = base + (long long)u * blockDim.x;
+      if (r >= a.N) continue;
+      const int l = ~node[u];
+      if (a.mode == 0) {
+        atomicAdd(&lsum[2 * l], (unsigned long long)to_fixed(a.g[r], lsh[2 * l]));
+        atomicAdd(&lsum[2 * l + 1], (unsigned long long)to_fixed(a.h[r], lsh[2 * l + 1]));
+      } else if (a.mode == 1) {
+        a.score[r] += lval[l];
+      } else {
+        a.out[r] = l;
+      }
+    }
+  }
+  if (a.mode == 0) {
+    __syncthreads();
+    for (int l = tid; l < nl; l += blockDim.x) {
+      const unsigned long long vg = lsum[2 * l], vh = lsum[2 * l + 1];
+      if (vg) atomicAdd((unsigned long long*)&a.sums[2 * l], vg);
+      if (vh) atomicAdd((unsigned long long*)&a.sums[2 * l + 1], vh);
+    }
   }
 }
 
@@ -1409,29 +1601,6 @@ __global__ void k_leaf_values(const StepDev* st, const PosRec* pos, const LeafDe
   }
   raw[l] = o;
   out[l] = round0(o * lr);
-}
-
-// tree.rs:635-652 / partition.rs:178-191: score[row] += value of its leaf.
-__global__ void k_score(const StepDev* st, const PosRec* pos, const LeafDev* leaves, int L, const unsigned* idx0,
-                        const unsigned* idx1, const unsigned* idx2, const double* val, double* score) {
-  const int l = blockIdx.y;
-  if (!st->done || l >= st->nseq) return;
-  const LeafDev& Lf = leaf_at(st, pos, leaves, L, l);
-  const unsigned* idx = sel3(idx0, idx1, idx2, Lf.buf) + Lf.begin;
-  const double v = val[l];
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < Lf.count; p += (long long)gridDim.x * blockDim.x)
-    score[idx[p]] += v;
-}
-
-// Leaf index of every training row (the tree's numbering).
-__global__ void k_leaf_of_row(const StepDev* st, const PosRec* pos, const LeafDev* leaves, int L, const unsigned* idx0,
-                              const unsigned* idx1, const unsigned* idx2, int* out) {
-  const int l = blockIdx.y;
-  if (l >= st->nseq) return;
-  const LeafDev& Lf = leaf_at(st, pos, leaves, L, l);
-  const unsigned* idx = sel3(idx0, idx1, idx2, Lf.buf) + Lf.begin;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < Lf.count; p += (long long)gridDim.x * blockDim.x)
-    out[idx[p]] = l;
 }
 
 // Each item's histogram summed over its chunks: out[w][g][entry], one unit
@@ -1480,6 +1649,23 @@ __global__ void __launch_bounds__(256) k_slab_reduce(const long long* __restrict
 }
 
 // ---------------------------------------------------------------- launchers
+// Rows of at most 32 bytes are walked from registers.
+static void launch_walk(const WalkArgs& a, int grid, hipStream_t s) {
+  const size_t lds = walk_lds(a.L);
+  switch (a.S) {
+    case 4: hipLaunchKernelGGL(k_walk<1>, dim3(grid), dim3(256), lds, s, a); break;
+    case 8: hipLaunchKernelGGL(k_walk<2>, dim3(grid), dim3(256), lds, s, a); break;
+    case 16: hipLaunchKernelGGL(k_walk<4>, dim3(grid), dim3(256), lds, s, a); break;
+    case 32: hipLaunchKernelGGL(k_walk<8>, dim3(grid), dim3(256), lds, s, a); break;
+    default: hipLaunchKernelGGL(k_walk<0>, dim3(grid), dim3(256), lds, s, a); break;
+  }
+}
+static void set_walk_lds_limit(int L) {
+  const void* ks[] = {(const void*)k_walk<0>, (const void*)k_walk<1>, (const void*)k_walk<2>, (const void*)k_walk<4>,
+                      (const void*)k_walk<8>};
+  for (const void* k : ks) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)walk_lds(L));
+}
+
 template <int TPR>
 static void launch_hist_t(const HistArgs& a, bool quant, int grid, size_t lds, hipStream_t s) {
   if (quant) hipLaunchKernelGGL(k_hist_int<TPR>, dim3(grid), dim3(256), lds, s, a);
