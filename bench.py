@@ -124,8 +124,8 @@ def main():
     ap.add_argument("--max-bin", type=int, default=255)
     ap.add_argument("--num-leaves", type=int, default=255)
     ap.add_argument("--prof-iters", type=int, default=3)
-    ap.add_argument("--cpu-rows", type=int, default=2_000_000)
-    ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--cpu-rows", type=int, default=5_000_000)
+    ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="quant,plain")
     ap.add_argument("--no-graph", action="store_true", help="stream launches instead of graphs (counter passes)")
