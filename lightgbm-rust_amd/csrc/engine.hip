@@ -259,6 +259,7 @@ struct Ctx {
   DevBuf<double> p_leafval;
   DevBuf<unsigned> p_cat;
   int p_T = 0;
+  bool p_binned = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
   // profiling: HIP events around every histogram / split / partition launch
@@ -1055,8 +1056,10 @@ static void train_tree(Ctx* c, const float* g, const float* h) {
 }
 
 // ---------------------------------------------------------------- prediction
+// Flattened trees for prediction; categorical nodes carry the category
+// bitset (raw values, tree.rs:399-418) or the bin bitset (binned rows,
+// tree.rs:613-618).
 static void build_pred(Ctx* c, bool binned) {
-  (void)binned;
   std::vector<PredNode> nodes;
   std::vector<int> noff, loff, nl;
   std::vector<double> lv;
@@ -1077,6 +1080,14 @@ static void build_pred(Ctx* c, bool binned) {
       n.is_cat = (int8_t)(t.decision_type[i] & 1);
       n.cat_off = -1;
       n.cat_len = 0;
+      if (n.is_cat) {
+        const int ci = (int)t.threshold_in_bin[i];
+        const auto& bnd = binned ? t.cat_boundaries_inner : t.cat_boundaries;
+        const auto& words = binned ? t.cat_threshold_inner : t.cat_threshold;
+        n.cat_off = (int)cats.size();
+        n.cat_len = bnd[ci + 1] - bnd[ci];
+        for (int w = bnd[ci]; w < bnd[ci + 1]; ++w) cats.push_back(words[w]);
+      }
       const int fi = t.split_feature_inner[i];
       n.default_bin = c->metas[fi].default_bin;
       n.nan_bin = c->metas[fi].num_bin - 1;
@@ -1091,6 +1102,8 @@ static void build_pred(Ctx* c, bool binned) {
   c->p_leaf_off.alloc(std::max<size_t>(loff.size(), 1));
   c->p_nleaves.alloc(std::max<size_t>(nl.size(), 1));
   c->p_leafval.alloc(std::max<size_t>(lv.size(), 1));
+  c->p_cat.alloc(std::max<size_t>(cats.size(), 1));
+  if (!cats.empty()) HIP_CHECK(hipMemcpy(c->p_cat.p, cats.data(), cats.size() * 4, hipMemcpyHostToDevice));
   if (!nodes.empty()) HIP_CHECK(hipMemcpy(c->p_nodes.p, nodes.data(), nodes.size() * sizeof(PredNode), hipMemcpyHostToDevice));
   if (!noff.empty()) {
     HIP_CHECK(hipMemcpy(c->p_node_off.p, noff.data(), noff.size() * 4, hipMemcpyHostToDevice));
@@ -1099,20 +1112,19 @@ static void build_pred(Ctx* c, bool binned) {
     HIP_CHECK(hipMemcpy(c->p_leafval.p, lv.data(), lv.size() * 8, hipMemcpyHostToDevice));
   }
   c->pred_dirty = false;
+  c->p_binned = binned;
 }
 
 static void predict(Ctx* c, const float* X, const uint8_t* bins, int64_t rows, int ncol, double* out) {
   set_device(c);
-  for (const auto& t : c->trees)
-    if (t.num_cat > 0) throw Error(LGBMHIP_UNSUPPORTED, "prediction of categorical splits");
-  if (c->pred_dirty) build_pred(c, bins != nullptr);
+  if (c->pred_dirty || c->p_binned != (bins != nullptr)) build_pred(c, bins != nullptr);
   PredArgs a{};
   a.nodes = c->p_nodes.p;
   a.tree_node_off = c->p_node_off.p;
   a.tree_leaf_off = c->p_leaf_off.p;
   a.tree_num_leaves = c->p_nleaves.p;
   a.leaf_val = c->p_leafval.p;
-  a.cat_words = nullptr;
+  a.cat_words = c->p_cat.p;
   a.T = c->p_T;
   a.K = c->K;
   a.rows = rows;

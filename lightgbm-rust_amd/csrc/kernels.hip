@@ -1033,19 +1033,21 @@ __global__ void k_predict_raw(PredArgs a) {
       while (node >= 0) {
         const PredNode& n = nd[node];
         double v = n.feature < a.ncol ? (double)x[n.feature] : 0.0;
-        if (!isnan(v) && fabs(v) <= KZERO) v = 0.0;
         if (n.is_cat) {
+          // tree.rs:407-418: NaN and negative categories go right
           bool lft = false;
           if (!isnan(v)) {
-            int c = (int)v;
+            const int c = (int)v;   // saturating, as the reference's `as i32`
             if (c >= 0 && (c >> 5) < n.cat_len) lft = (a.cat_words[n.cat_off + (c >> 5)] >> (c & 31)) & 1u;
           }
           node = lft ? n.left : n.right;
           continue;
         }
-        if (isnan(v) && n.missing_type != 2) v = 0.0;
-        bool zero = v >= -KZERO && v <= KZERO;
-        if ((n.missing_type == 1 && zero) || (n.missing_type == 2 && isnan(v))) node = n.default_left ? n.left : n.right;
+        // predict.rs:95-109 step: NaN is zero unless the node splits NaN
+        const bool nan = isnan(v);
+        if (nan && n.missing_type != 2) v = 0.0;
+        const bool zero = v >= -KZERO && v <= KZERO;
+        if ((n.missing_type == 1 && zero) || (n.missing_type == 2 && nan)) node = n.default_left ? n.left : n.right;
         else node = v <= n.threshold ? n.left : n.right;
       }
       leaf = ~node;

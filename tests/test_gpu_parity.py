@@ -484,3 +484,49 @@ def test_lambdarank_boosting_improves_ndcg():
     n1 = O.ndcg(b.train_scores(), rel, qb, 10)
     n_cpu = O.ndcg(ob.scores(), rel, qb, 10)
     assert n1 > n0 and abs(n1 - n_cpu) < 1e-3
+
+
+def _categorical_data(n, seed):
+    """binary_100k with column 1 replaced by a non-monotone category code of
+    the informative column 24 (40 categories, 1% NaN); column 24 itself is
+    shuffled, so its signal is reachable only through categorical splits."""
+    X, y = datagen.binary_100k(n)
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(40)
+    q = np.quantile(X[:, 24], np.linspace(0, 1, 41)[1:-1])
+    X = X.copy()
+    X[:, 1] = perm[np.digitize(X[:, 24], q)].astype(np.float64)
+    X[rng.random(n) < 0.01, 1] = np.nan
+    X[:, 24] = rng.permutation(X[:, 24])
+    return X, y
+
+
+def test_categorical_trees_and_prediction_match_oracle():
+    X, y = _categorical_data(60000, 3)
+    params = dict(P_BIN, num_leaves=63, max_bin=255, categorical_feature=[1], min_data_per_group=50,
+                  cat_smooth=10.0)
+    ds, b = _booster(X, y, params)
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    for _ in range(4):
+        b.update()
+        ob.update()
+    ncat = 0
+    for t in range(4):
+        gt, ot = b.tree(t), ob.tree(t)
+        _compare_tree(gt, ot)
+        ncat += int(np.sum(np.asarray(gt.decision_type) & 1))
+    assert ncat > 0, "no categorical split was made"
+    np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
+    # raw-value prediction on training rows, unseen categories, negative
+    # categories, NaN and zero (tree.rs:399-418 and predict.rs:95-109)
+    Xt = np.concatenate([X[:5000], X[:200]])
+    Xt[5000:5050, 1] = 57.0
+    Xt[5050:5100, 1] = -3.0
+    Xt[5100:5150, 1] = np.nan
+    Xt[5150:5200, 0] = 0.0
+    raw = b.predict(Xt, raw_score=True)
+    np.testing.assert_allclose(raw, ob.predict_raw(Xt)[:, 0], rtol=1e-12, atol=1e-12)
+    # binned prediction equals raw prediction on the training rows
+    pb = b.predict_binned(ds.bins)[:, 0]
+    r_all = b.predict(X, raw_score=True)
+    assert np.all(np.abs(pb - r_all) <= 1e-9 * (1 + np.abs(r_all)))
