@@ -158,13 +158,15 @@ lgbmhip_status lgbmhip_build_histogram_int(lgbmhip_ctx* ctx, const uint32_t* row
                                            const int8_t* hq, int64_t* out, int32_t* out_num_bins);
 /* Threshold search on one feature's bin-indexed [B][2] histogram (mfb fixed from the sums first).
  * out_d = {gain, left_output, right_output, lsg, lsh, rsg, rsh}; out_i = {threshold, default_left,
- * left_count, right_count, splittable}. */
+ * left_count, right_count, splittable}; out_cat_bits (8 words, may be NULL) = the bins a categorical
+ * split sends left, as a bitset (zero for numerical features). */
 lgbmhip_status lgbmhip_find_best_split(lgbmhip_ctx* ctx, int32_t feature, const double* hist, double sum_g,
                                        double sum_h, int32_t num_data, double parent_output, double* out_d,
-                                       int32_t* out_i);
+                                       int32_t* out_i, uint32_t* out_cat_bits);
 lgbmhip_status lgbmhip_find_best_split_int(lgbmhip_ctx* ctx, int32_t feature, const int64_t* hist, int64_t int_sum,
                                            double grad_scale, double hess_scale, int32_t num_data,
-                                           double parent_output, double* out_d, int32_t* out_i, int64_t* out_gh);
+                                           double parent_output, double* out_d, int32_t* out_i, int64_t* out_gh,
+                                           uint32_t* out_cat_bits);
 /* Stable partition of rows[0..m) on inner feature `feature`: left rows (in order) then right rows. */
 lgbmhip_status lgbmhip_partition(lgbmhip_ctx* ctx, const uint32_t* rows, int64_t m, int32_t feature,
                                  uint32_t threshold, int32_t default_left, uint32_t* out_rows, int64_t* out_left);

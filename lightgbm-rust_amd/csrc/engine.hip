@@ -449,8 +449,6 @@ static void setup_training(Ctx* c) {
   if (c->L > 1024) throw Error(LGBMHIP_UNSUPPORTED, "num_leaves above 1024");
   if (c->F >= 65535) throw Error(LGBMHIP_UNSUPPORTED, "65535 or more used features");
   if (quant) {
-    for (auto& f : c->feats)
-      if (f.bin_type == 1) throw Error(LGBMHIP_UNSUPPORTED, "quantized gradients with categorical features");
     if (o.num_grad_quant_bins < 2 || o.num_grad_quant_bins > 126)
       throw Error(LGBMHIP_INVALID_PARAMETER, "num_grad_quant_bins out of range");
   }
@@ -1514,7 +1512,7 @@ lgbmhip_status lgbmhip_build_histogram_int(lgbmhip_ctx* ctx, const uint32_t* row
 }
 
 static void stage_search(Ctx* c, int f, const double* hist, const int64_t* ihist, LeafSums ls, bool quant,
-                         double* od, int32_t* oi, int64_t* og) {
+                         double* od, int32_t* oi, int64_t* og, uint32_t* ocat) {
   if (f < 0 || f >= c->F) throw Error(LGBMHIP_INVALID_PARAMETER, "feature out of range");
   if (!c->sp_set) throw Error(LGBMHIP_INVALID_PARAMETER, "split params not set");
   set_device(c);
@@ -1543,23 +1541,26 @@ static void stage_search(Ctx* c, int f, const double* hist, const int64_t* ihist
   od[3] = r.lsg; od[4] = r.lsh; od[5] = r.rsg; od[6] = r.rsh;
   oi[0] = r.threshold; oi[1] = r.default_left; oi[2] = r.left_count; oi[3] = r.right_count; oi[4] = spl;
   if (og) { og[0] = r.lgh; og[1] = r.rgh; }
+  if (ocat)
+    for (int w = 0; w < kCatWords; ++w) ocat[w] = spl && c->feats[f].bin_type == 1 ? r.cat_bits[w] : 0u;
 }
 
 lgbmhip_status lgbmhip_find_best_split(lgbmhip_ctx* ctx, int32_t feature, const double* hist, double sum_g,
                                        double sum_h, int32_t num_data, double parent_output, double* out_d,
-                                       int32_t* out_i) {
+                                       int32_t* out_i, uint32_t* out_cat_bits) {
   return guarded(ctx, [&] {
     LeafSums ls{};
     ls.sg = sum_g;
     ls.sh = sum_h + 2.0 * (double)1e-15f;
     ls.nd = num_data;
     ls.po = parent_output;
-    stage_search(&ctx->c, feature, hist, nullptr, ls, false, out_d, out_i, nullptr);
+    stage_search(&ctx->c, feature, hist, nullptr, ls, false, out_d, out_i, nullptr, out_cat_bits);
   });
 }
 lgbmhip_status lgbmhip_find_best_split_int(lgbmhip_ctx* ctx, int32_t feature, const int64_t* hist, int64_t int_sum,
                                            double grad_scale, double hess_scale, int32_t num_data,
-                                           double parent_output, double* out_d, int32_t* out_i, int64_t* out_gh) {
+                                           double parent_output, double* out_d, int32_t* out_i, int64_t* out_gh,
+                                           uint32_t* out_cat_bits) {
   return guarded(ctx, [&] {
     LeafSums ls{};
     ls.isum = int_sum;
@@ -1567,7 +1568,7 @@ lgbmhip_status lgbmhip_find_best_split_int(lgbmhip_ctx* ctx, int32_t feature, co
     ls.hs = hess_scale;
     ls.nd = num_data;
     ls.po = parent_output;
-    stage_search(&ctx->c, feature, nullptr, hist, ls, true, out_d, out_i, out_gh);
+    stage_search(&ctx->c, feature, nullptr, hist, ls, true, out_d, out_i, out_gh, out_cat_bits);
   });
 }
 

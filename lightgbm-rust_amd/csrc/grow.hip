@@ -644,7 +644,8 @@ __global__ void __launch_bounds__(128) k_grow_split(SplitArgs a) {
     if (parent_spl) {
       if (lane == 0) r.gain = -INFINITY;
       if (a.quant) {
-        spl = search_int(wave == 0 ? ism : ilg, fd, a.p, ls, scr[wave], r);
+        spl = fd.bin_type == 1 ? search_categorical_int(wave == 0 ? ism : ilg, fd, a.p, ls, scr[wave], r)
+                               : search_int(wave == 0 ? ism : ilg, fd, a.p, ls, scr[wave], r);
       } else if (fd.bin_type == 1) {
         spl = search_categorical(wave == 0 ? hsm[0] : hlg[0], wave == 0 ? hsm[1] : hlg[1], fd, a.p, ls, scr[wave], r);
       } else {

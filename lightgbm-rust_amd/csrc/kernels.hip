@@ -838,6 +838,154 @@ __device__ __forceinline__ bool search_categorical(const double* hg, const doubl
   return S.skipped[kMaxBins] != 0;
 }
 
+// split.rs:1194-1400 (categorical_int, no monotone constraints) on lane 0:
+// the categorical search on packed sums. Counts come from integer hessians
+// (round_int(int_hess * nd / int_hess(total))); the chosen set's counts are
+// recomputed from its packed sum; no kEpsilon. The ctr order is a stable
+// rank sort computed by all lanes.
+__device__ __forceinline__ bool search_categorical_int(const long long* hv, const FeatDev& fd, const SplitParamsDev& p0,
+                                                       const LeafSums& ls, ScanLds& S, SplitRec& out) {
+  const int lane = threadIdx.x & 63;
+  const int off = fd.offset;
+  const int bin_start = 1 - off, bin_end = fd.num_bin - off;
+  const unsigned long long total = (unsigned long long)ls.isum;
+  const double cf = (double)ls.nd / (double)(unsigned)total;
+  const double sg = (double)(int)(ls.isum >> 32) * ls.gs, sh = (double)(unsigned)total * ls.hs;
+  SplitParamsDev pc = p0;
+  const bool onehot = fd.num_bin <= p0.max_cat_to_onehot;
+  double gain_shift = p0.path_smooth > KEPS ? 0.0 : leaf_gain(sg, sh, p0, ls.nd, 0.0);
+  if (p0.path_smooth > KEPS) {
+    double s = p0.lambda_l1 > 0.0 ? thr_l1(sg, p0.lambda_l1) : sg;
+    gain_shift = -(2.0 * s * ls.po + (sh + p0.lambda_l2) * ls.po * ls.po);
+  }
+  const double min_gain_shift = gain_shift + p0.min_gain_to_split;
+  out.default_left = 0;
+  int used_bin = 0;
+  int* sorted = (int*)S.pre[2];   // scratch reused as ints
+  if (!onehot) {
+    for (int i = lane; i < bin_end - bin_start; i += 64) {
+      const unsigned hq = (unsigned)hv[bin_start + i + off];
+      S.skipped[i] = (double)round_int((double)hq * cf) >= p0.cat_smooth;
+    }
+    wave_sync();
+    if (lane == 0) {
+      int c = 0;
+      for (int i = 0; i < bin_end - bin_start; ++i)
+        if (S.skipped[i]) sorted[c++] = bin_start + i;
+      S.skipped[kMaxBins] = c;
+    }
+    wave_sync();
+    used_bin = S.skipped[kMaxBins];
+    // stable sort by ctr = g / (h + cat_smooth): rank = #less + #equal-before
+    for (int i = lane; i < used_bin; i += 64) {
+      const int ti = sorted[i];
+      const long long vi = hv[ti + off];
+      const double ci = ((double)(int)(vi >> 32) * ls.gs) / ((double)(unsigned)vi * ls.hs + p0.cat_smooth);
+      int rank = 0;
+      for (int k = 0; k < used_bin; ++k) {
+        const long long vk = hv[sorted[k] + off];
+        const double ck = ((double)(int)(vk >> 32) * ls.gs) / ((double)(unsigned)vk * ls.hs + p0.cat_smooth);
+        if (ck < ci || (!(ci < ck) && !(ck < ci) && k < i)) rank++;
+      }
+      S.val[2][rank] = (double)ti;
+    }
+    wave_sync();
+  }
+  bool is_splittable = false;
+  if (lane == 0) {
+    double best_gain = -INFINITY;
+    unsigned long long best_left = 0;
+    int best_threshold = -1, best_dir = 1;
+    if (onehot) {
+      for (int t = bin_start; t < bin_end; ++t) {
+        const unsigned long long gh = (unsigned long long)hv[t + off];
+        const int cnt = round_int((double)(unsigned)gh * cf);
+        const double hess = (double)(unsigned)gh * ls.hs;
+        if (cnt < p0.min_data_in_leaf || hess < p0.min_sum_hessian) continue;
+        const int other = ls.nd - cnt;
+        if (other < p0.min_data_in_leaf) continue;
+        const unsigned long long so = total - gh;
+        const double soh = (double)(unsigned)so * ls.hs;
+        if (soh < p0.min_sum_hessian) continue;
+        const double grad = (double)(int)(gh >> 32) * ls.gs;
+        const double sog = (double)(int)(so >> 32) * ls.gs;
+        const double cg = split_gain(sog, soh, grad, hess, pc, other, cnt, ls.po);
+        if (cg <= min_gain_shift) continue;
+        is_splittable = true;
+        if (cg > best_gain) { best_threshold = t; best_left = gh; best_gain = cg; }
+      }
+    } else {
+      pc.lambda_l2 += p0.cat_l2;
+      const int max_num_cat = min(p0.max_cat_threshold, (used_bin + 1) / 2);
+      for (int d = 0; d < 2; ++d) {
+        const int dir = d == 0 ? 1 : -1;
+        int pos = d == 0 ? 0 : used_bin - 1;
+        int cur_group = 0, lc = 0;
+        unsigned long long sl = 0;
+        for (int i = 0; i < used_bin && i < max_num_cat; ++i) {
+          const int t = (int)S.val[2][pos];
+          pos += dir;
+          const unsigned long long gh = (unsigned long long)hv[t + off];
+          const int cnt = round_int((double)(unsigned)gh * cf);
+          sl += gh;
+          lc += cnt;
+          cur_group += cnt;
+          const double slh = (double)(unsigned)sl * ls.hs;
+          if (lc < p0.min_data_in_leaf || slh < p0.min_sum_hessian) continue;
+          const int rc = ls.nd - lc;
+          if (rc < p0.min_data_in_leaf || rc < p0.min_data_per_group) break;
+          const unsigned long long sr = total - sl;
+          const double srh = (double)(unsigned)sr * ls.hs;
+          if (srh < p0.min_sum_hessian) break;
+          if (cur_group < p0.min_data_per_group) continue;
+          cur_group = 0;
+          const double slg = (double)(int)(sl >> 32) * ls.gs;
+          const double srg = (double)(int)(sr >> 32) * ls.gs;
+          const double cg = split_gain(slg, slh, srg, srh, pc, lc, rc, ls.po);
+          if (cg <= min_gain_shift) continue;
+          is_splittable = true;
+          if (cg > best_gain) { best_left = sl; best_threshold = i; best_gain = cg; best_dir = dir; }
+        }
+      }
+    }
+    if (is_splittable) {
+      const unsigned long long left = best_left, right = total - left;
+      const double lg = (double)(int)(left >> 32) * ls.gs, lh = (double)(unsigned)left * ls.hs;
+      const double rg = (double)(int)(right >> 32) * ls.gs, rh = (double)(unsigned)right * ls.hs;
+      const int lc = round_int((double)(unsigned)left * cf), rc = round_int((double)(unsigned)right * cf);
+      out.left_output = leaf_output(lg, lh, pc, lc, ls.po);
+      out.left_count = lc;
+      out.lsg = lg;
+      out.lsh = lh;
+      out.lgh = (long long)left;
+      out.right_output = leaf_output(rg, rh, pc, rc, ls.po);
+      out.right_count = rc;
+      out.rsg = rg;
+      out.rsh = rh;
+      out.rgh = (long long)right;
+      out.gain = best_gain - min_gain_shift;
+      for (int w = 0; w < kCatWords; ++w) out.cat_bits[w] = 0;
+      int nc = 0;
+      if (onehot) {
+        const int b = best_threshold + off;
+        out.cat_bits[b / 32] |= 1u << (b % 32);
+        nc = 1;
+      } else {
+        for (int i = 0; i <= best_threshold; ++i) {
+          const int t = best_dir == 1 ? (int)S.val[2][i] : (int)S.val[2][used_bin - 1 - i];
+          const int b = t + off;
+          out.cat_bits[b / 32] |= 1u << (b % 32);
+          nc++;
+        }
+      }
+      out.num_cat = nc;
+    }
+    S.skipped[kMaxBins] = is_splittable;
+  }
+  wave_sync();
+  return S.skipped[kMaxBins] != 0;
+}
+
 // split.rs:993-1190 on a wave: packed int64 prefix sums are exact, so every
 // scan position is scored in parallel from a wave-parallel prefix.
 __device__ __forceinline__ void scan_int(const long long* hv, const FeatDev& fd, const SplitParamsDev& p, const LeafSums& ls,
@@ -1128,7 +1276,7 @@ __global__ void __launch_bounds__(64) k_search_one(const double* hist, const lon
   }
   __syncthreads();
   bool spl;
-  if (quant) spl = search_int(ih, fd, p, ls, S, r);
+  if (quant) spl = fd.bin_type == 1 ? search_categorical_int(ih, fd, p, ls, S, r) : search_int(ih, fd, p, ls, S, r);
   else if (fd.bin_type == 1) spl = search_categorical(hg, hh, fd, p, ls, S, r);
   else spl = search_numerical(hg, hh, fd, p, ls, S, r);
   if (lane == 0) {

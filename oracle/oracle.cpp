@@ -971,12 +971,132 @@ static bool scan_int(const int64_t* hist, const FeatureMeta& meta, const SplitPa
   return is_splittable;
 }
 
-// split.rs:928-990 (numerical features only; categorical features with
-// quantized gradients are not restated and are rejected by the caller)
+// split.rs:1194-1400 (categorical_int, no monotone constraints, no
+// extra_trees): the categorical search on packed sums. Counts come from
+// integer hessians; the chosen set's counts are recomputed from its packed
+// sum (split.rs:1364-1367), not accumulated; no kEpsilon anywhere.
+static bool categorical_int(const int64_t* hist, const FeatureMeta& meta, const SplitParams& p, IntSums sums,
+                            int num_data, double parent_output, SplitInfo& out) {
+  const double gs = sums.grad_scale, hs = sums.hess_scale;
+  bool is_splittable = false;
+  out.default_left = false;
+  double best_gain = K_MIN_SCORE;
+  int64_t best_sum_left = 0;
+  const int64_t local_sum = sums.sum;
+  const double sum_gradient = sums.gradient(), sum_hessian = sums.hessian();
+  double gain_shift = p.use_smoothing() ? leaf_gain_given_output(sum_gradient, sum_hessian, p, parent_output)
+                                        : leaf_gain(sum_gradient, sum_hessian, p, num_data, 0.0);
+  double min_gain_shift = gain_shift + p.min_gain_to_split;
+  int offset = meta.offset;
+  int bin_start = 1 - offset, bin_end = meta.num_bin - offset;
+  int used_bin = -1;
+  std::vector<int> sorted_idx;
+  SplitParams pc = p;
+  bool use_onehot = meta.num_bin <= p.max_cat_to_onehot;
+  int best_threshold = -1, best_dir = 1;
+  const double cnt_factor = sums.cnt_factor(num_data);
+  auto bin_cnt = [&](int64_t gh) { return round_int((double)int_hess(gh) * cnt_factor); };
+  if (use_onehot) {
+    for (int t = bin_start; t < bin_end; ++t) {
+      int64_t gh = hist[t];
+      int cnt = bin_cnt(gh);
+      double hess = (double)int_hess(gh) * hs;
+      if (cnt < p.min_data_in_leaf || hess < p.min_sum_hessian_in_leaf) continue;
+      int other_count = num_data - cnt;
+      if (other_count < p.min_data_in_leaf) continue;
+      int64_t sum_other = wsub(local_sum, gh);
+      double soh = (double)int_hess(sum_other) * hs;
+      if (soh < p.min_sum_hessian_in_leaf) continue;
+      double grad = (double)int_grad(gh) * gs;
+      double sog = (double)int_grad(sum_other) * gs;
+      double cg = split_gain(sog, soh, grad, hess, pc, other_count, cnt, parent_output);
+      if (cg <= min_gain_shift) continue;
+      is_splittable = true;
+      if (cg > best_gain) { best_threshold = t; best_sum_left = gh; best_gain = cg; }
+    }
+  } else {
+    for (int i = bin_start; i < bin_end; ++i)
+      if ((double)bin_cnt(hist[i]) >= p.cat_smooth) sorted_idx.push_back(i);
+    used_bin = (int)sorted_idx.size();
+    pc.lambda_l2 += p.cat_l2;
+    auto ctr = [&](int i) {
+      return ((double)int_grad(hist[i]) * gs) / ((double)int_hess(hist[i]) * hs + p.cat_smooth);
+    };
+    std::stable_sort(sorted_idx.begin(), sorted_idx.end(), [&](int i, int j) { return ctr(i) < ctr(j); });
+    int max_num_cat = std::min(p.max_cat_threshold, (used_bin + 1) / 2);
+    is_splittable = false;
+    const int dirs[2] = {1, -1};
+    for (int d = 0; d < 2; ++d) {
+      int dir = dirs[d];
+      int start_pos = dir == 1 ? 0 : used_bin - 1;
+      int cnt_cur_group = 0;
+      int64_t sum_left = 0;
+      int lc = 0;
+      int i = 0;
+      while (i < used_bin && i < max_num_cat) {
+        int t = sorted_idx[start_pos];
+        start_pos += dir;
+        int64_t gh = hist[t];
+        int cnt = bin_cnt(gh);
+        sum_left = wadd(sum_left, gh);
+        lc += cnt;
+        cnt_cur_group += cnt;
+        double slh = (double)int_hess(sum_left) * hs;
+        if (lc < p.min_data_in_leaf || slh < p.min_sum_hessian_in_leaf) { i++; continue; }
+        int rc = num_data - lc;
+        if (rc < p.min_data_in_leaf || rc < p.min_data_per_group) break;
+        int64_t sum_right = wsub(local_sum, sum_left);
+        double srh = (double)int_hess(sum_right) * hs;
+        if (srh < p.min_sum_hessian_in_leaf) break;
+        if (cnt_cur_group < p.min_data_per_group) { i++; continue; }
+        cnt_cur_group = 0;
+        double slg = (double)int_grad(sum_left) * gs;
+        double srg = (double)int_grad(sum_right) * gs;
+        double cg = split_gain(slg, slh, srg, srh, pc, lc, rc, parent_output);
+        if (cg <= min_gain_shift) { i++; continue; }
+        is_splittable = true;
+        if (cg > best_gain) { best_sum_left = sum_left; best_threshold = i; best_gain = cg; best_dir = dir; }
+        i++;
+      }
+    }
+  }
+  if (is_splittable) {
+    int64_t left = best_sum_left, right = wsub(local_sum, left);
+    double lg = (double)int_grad(left) * gs, lh = (double)int_hess(left) * hs;
+    double rg = (double)int_grad(right) * gs, rh = (double)int_hess(right) * hs;
+    int lc = bin_cnt(left), rc = bin_cnt(right);
+    out.left_output = leaf_output(lg, lh, pc, lc, parent_output);
+    out.left_count = lc;
+    out.left_sum_gradient = lg;
+    out.left_sum_hessian = lh;
+    out.right_output = leaf_output(rg, rh, pc, rc, parent_output);
+    out.right_count = rc;
+    out.right_sum_gradient = rg;
+    out.right_sum_hessian = rh;
+    out.gain = best_gain - min_gain_shift;
+    out.left_sum_gh = left;
+    out.right_sum_gh = right;
+    out.cat_threshold.clear();
+    if (use_onehot) out.cat_threshold.push_back((uint32_t)(best_threshold + offset));
+    else if (best_dir == 1)
+      for (int i = 0; i <= best_threshold; ++i) out.cat_threshold.push_back((uint32_t)(sorted_idx[i] + offset));
+    else
+      for (int i = 0; i <= best_threshold; ++i)
+        out.cat_threshold.push_back((uint32_t)(sorted_idx[used_bin - 1 - i] + offset));
+  }
+  return is_splittable;
+}
+
+// split.rs:928-990
 static bool find_best_threshold_int(const int64_t* hist, const FeatureMeta& meta, const SplitParams& p, IntSums sums,
                                     int nd, double po, SplitInfo& out) {
   out.default_left = true;
   out.gain = K_MIN_SCORE;
+  if (meta.bin_type == BT_CAT) {
+    bool s = categorical_int(hist, meta, p, sums, nd, po, out);
+    out.gain *= meta.penalty;
+    return s;
+  }
   double min_gain_shift = leaf_gain(sums.gradient(), sums.hessian(), p, nd, po) + p.min_gain_to_split;
   bool splittable = false;
   if (meta.num_bin > 2 && meta.missing_type != MT_NONE) {
@@ -1951,7 +2071,7 @@ static SplitParams stage_params(const double* dp, const int* ip) {
 // frequent bin is fixed from (sg, sh) first, as the learner does.
 void orc_find_best_threshold(double* hist, int num_bin, int missing_type, int default_bin, int mfb, int bin_type,
                              const double* dp, const int* ip, double sg, double sh, int nd, double parent_output,
-                             double* out_d, int* out_i) {
+                             double* out_d, int* out_i, int* out_cat) {
   FeatureMeta m{num_bin, missing_type, mfb == 0 ? 1 : 0, (uint32_t)default_bin, (uint32_t)mfb, bin_type, 1.0};
   SplitParams sp = stage_params(dp, ip);
   hist_fix(m, sg, sh, hist);
@@ -1960,11 +2080,15 @@ void orc_find_best_threshold(double* hist, int num_bin, int missing_type, int de
   out_d[0] = s.gain; out_d[1] = s.left_output; out_d[2] = s.right_output;
   out_d[3] = s.left_sum_gradient; out_d[4] = s.left_sum_hessian; out_d[5] = s.right_sum_gradient; out_d[6] = s.right_sum_hessian;
   out_i[0] = (int)s.threshold; out_i[1] = s.default_left; out_i[2] = s.left_count; out_i[3] = s.right_count; out_i[4] = ok;
+  // out_cat: [n, bins sent left in the search's order] (num_bin + 1 ints)
+  out_cat[0] = ok && bin_type == BT_CAT ? (int)s.cat_threshold.size() : 0;
+  for (int i = 0; i < out_cat[0]; ++i) out_cat[1 + i] = (int)s.cat_threshold[i];
 }
-void orc_find_best_threshold_int(int64_t* hist, int num_bin, int missing_type, int default_bin, int mfb,
+void orc_find_best_threshold_int(int64_t* hist, int num_bin, int missing_type, int default_bin, int mfb, int bin_type,
                                  const double* dp, const int* ip, int64_t int_sum, double gscale, double hscale,
-                                 int nd, double parent_output, double* out_d, int* out_i, int64_t* out_gh) {
-  FeatureMeta m{num_bin, missing_type, mfb == 0 ? 1 : 0, (uint32_t)default_bin, (uint32_t)mfb, BT_NUM, 1.0};
+                                 int nd, double parent_output, double* out_d, int* out_i, int64_t* out_gh,
+                                 int* out_cat) {
+  FeatureMeta m{num_bin, missing_type, mfb == 0 ? 1 : 0, (uint32_t)default_bin, (uint32_t)mfb, bin_type, 1.0};
   SplitParams sp = stage_params(dp, ip);
   hist_fix_int(m, int_sum, hist);
   SplitInfo s;
@@ -1973,6 +2097,8 @@ void orc_find_best_threshold_int(int64_t* hist, int num_bin, int missing_type, i
   out_d[3] = s.left_sum_gradient; out_d[4] = s.left_sum_hessian; out_d[5] = s.right_sum_gradient; out_d[6] = s.right_sum_hessian;
   out_i[0] = (int)s.threshold; out_i[1] = s.default_left; out_i[2] = s.left_count; out_i[3] = s.right_count; out_i[4] = ok;
   out_gh[0] = s.left_sum_gh; out_gh[1] = s.right_sum_gh;
+  out_cat[0] = ok && bin_type == BT_CAT ? (int)s.cat_threshold.size() : 0;
+  for (int i = 0; i < out_cat[0]; ++i) out_cat[1 + i] = (int)s.cat_threshold[i];
 }
 
 void orc_binary_gradients(const double* s, const float* label, int64_t n, double sigmoid, float* g, float* h) {

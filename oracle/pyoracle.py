@@ -59,8 +59,8 @@ def lib():
             "orc_booster_predict_raw": (None, [P, PF, I64, I, PD]),
             "orc_hist_f64": (None, [PU8, I, PI, I, PU, I64, PF, PF, PD]),
             "orc_hist_int": (None, [PU8, I, PI, I, PU, I64, PI8, PI8, PI64]),
-            "orc_find_best_threshold": (None, [PD, I, I, I, I, I, PD, PI, D, D, I, D, PD, PI]),
-            "orc_find_best_threshold_int": (None, [PI64, I, I, I, I, PD, PI, I64, D, D, I, D, PD, PI, PI64]),
+            "orc_find_best_threshold": (None, [PD, I, I, I, I, I, PD, PI, D, D, I, D, PD, PI, PI]),
+            "orc_find_best_threshold_int": (None, [PI64, I, I, I, I, I, PD, PI, I64, D, D, I, D, PD, PI, PI64, PI]),
             "orc_binary_gradients": (None, [PD, PF, I64, D, PF, PF]),
             "orc_multiclass_gradients": (None, [PD, PF, I64, I, PF, PF]),
             "orc_discretize": (None, [PF, PF, I64, I, I, I, I, PI8, PI8, PD]),
@@ -263,26 +263,29 @@ def find_best_threshold(hist_bins, num_bin, missing_type, default_bin, mfb, bin_
     hb = np.ascontiguousarray(hist_bins, np.float64).reshape(-1).copy()
     od = np.zeros(7, np.float64)
     oi = np.zeros(5, np.int32)
+    oc = np.zeros(num_bin + 1, np.int32)
     lib().orc_find_best_threshold(hb, num_bin, missing_type, default_bin, mfb, bin_type, dp, ip, sg, sh, nd,
-                                  parent_output, od, oi)
+                                  parent_output, od, oi, oc)
     return dict(gain=od[0], left_output=od[1], right_output=od[2], left_sum_gradient=od[3], left_sum_hessian=od[4],
                 right_sum_gradient=od[5], right_sum_hessian=od[6], threshold=int(oi[0]), default_left=bool(oi[1]),
-                left_count=int(oi[2]), right_count=int(oi[3]), splittable=bool(oi[4]))
+                left_count=int(oi[2]), right_count=int(oi[3]), splittable=bool(oi[4]),
+                cat_threshold=[int(v) for v in oc[1:1 + oc[0]]])
 
 
 def find_best_threshold_int(hist_bins, num_bin, missing_type, default_bin, mfb, p, int_sum, gscale, hscale, nd,
-                            parent_output=0.0):
+                            parent_output=0.0, bin_type=0):
     dp, ip = split_params_arrays(p)
     hb = np.ascontiguousarray(hist_bins, np.int64).reshape(-1).copy()
     od = np.zeros(7, np.float64)
     oi = np.zeros(5, np.int32)
     og = np.zeros(2, np.int64)
-    lib().orc_find_best_threshold_int(hb, num_bin, missing_type, default_bin, mfb, dp, ip, int_sum, gscale, hscale, nd,
-                                      parent_output, od, oi, og)
+    oc = np.zeros(num_bin + 1, np.int32)
+    lib().orc_find_best_threshold_int(hb, num_bin, missing_type, default_bin, mfb, bin_type, dp, ip, int_sum, gscale,
+                                      hscale, nd, parent_output, od, oi, og, oc)
     return dict(gain=od[0], left_output=od[1], right_output=od[2], left_sum_gradient=od[3], left_sum_hessian=od[4],
                 right_sum_gradient=od[5], right_sum_hessian=od[6], threshold=int(oi[0]), default_left=bool(oi[1]),
                 left_count=int(oi[2]), right_count=int(oi[3]), splittable=bool(oi[4]), left_sum_gh=int(og[0]),
-                right_sum_gh=int(og[1]))
+                right_sum_gh=int(og[1]), cat_threshold=[int(v) for v in oc[1:1 + oc[0]]])
 
 
 def binary_gradients(scores, label, sigmoid=1.0):

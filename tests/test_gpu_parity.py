@@ -129,7 +129,7 @@ def test_split_search_bit_exact_on_identical_histograms(missing_type):
             od = np.zeros(7)
             oi = np.zeros(5, np.int32)
             lgb._capi.check(lib.lgbmhip_find_best_split(b._ctx, f, lgb.ptr(np.ascontiguousarray(hist)), sg, sh, nd,
-                                                         0.0, lgb.ptr(od), lgb.ptr(oi)), b._ctx)
+                                                         0.0, lgb.ptr(od), lgb.ptr(oi), None), b._ctx)
             r = O.find_best_threshold(hist, m["num_bin"], m["missing_type"], m["default_bin"], m["most_freq_bin"],
                                       m["bin_type"], params, sg, sh, nd)
             assert bool(oi[4]) == r["splittable"]
@@ -165,7 +165,7 @@ def test_split_search_int_bit_exact():
             og = np.zeros(2, np.int64)
             lgb._capi.check(lib.lgbmhip_find_best_split_int(b._ctx, f, lgb.ptr(packed), tot, gscale, hscale,
                                                              int(cnt.sum()), 0.0, lgb.ptr(od), lgb.ptr(oi),
-                                                             lgb.ptr(og)), b._ctx)
+                                                             lgb.ptr(og), None), b._ctx)
             r = O.find_best_threshold_int(packed, nb, m["missing_type"], m["default_bin"], m["most_freq_bin"],
                                           params, tot, gscale, hscale, int(cnt.sum()))
             assert bool(oi[4]) == r["splittable"]
@@ -174,6 +174,96 @@ def test_split_search_int_bit_exact():
             assert od[0] == r["gain"] and oi[0] == r["threshold"]
             assert (oi[2], oi[3]) == (r["left_count"], r["right_count"])
             assert (int(og[0]), int(og[1])) == (r["left_sum_gh"], r["right_sum_gh"])
+
+
+def _bits_to_bins(bits):
+    return sorted(w * 32 + k for w in range(len(bits)) for k in range(32) if (int(bits[w]) >> k) & 1)
+
+
+def _categorical_stage_booster(quant):
+    """Three categorical columns (3, 40 and 12 categories: one-hot and sorted
+    searches) and one numerical column."""
+    rng = np.random.default_rng(7)
+    n = 6000
+    X = np.column_stack([rng.integers(0, 3, n), rng.integers(0, 40, n), rng.integers(0, 12, n),
+                         rng.normal(size=n)]).astype(np.float64)
+    y = (rng.random(n) < 0.5).astype(np.float64)
+    params = dict(P_BIN, min_data_in_leaf=5, max_bin=255, categorical_feature=[0, 1, 2], min_data_per_group=10,
+                  cat_smooth=5.0)
+    if quant:
+        params.update(use_quantized_grad=True, num_grad_quant_bins=16)
+    ds, b = _booster(X, y, params)
+    return ds, b, params
+
+
+def test_categorical_split_search_bit_exact():
+    # split.rs:446-670 on identical FP64 histograms
+    ds, b, params = _categorical_stage_booster(False)
+    rng = np.random.default_rng(8)
+    lib = lgb._capi.lib()
+    seen = 0
+    for f in range(ds.num_features):
+        m = ds.feature_meta(f)
+        if m["bin_type"] != 1:
+            continue
+        for trial in range(20):
+            hist, sg, sh, nd = _random_hist(rng, m["num_bin"], m["most_freq_bin"])
+            od = np.zeros(7)
+            oi = np.zeros(5, np.int32)
+            cb = np.zeros(8, np.uint32)
+            lgb._capi.check(lib.lgbmhip_find_best_split(b._ctx, f, lgb.ptr(np.ascontiguousarray(hist)), sg, sh, nd,
+                                                         0.0, lgb.ptr(od), lgb.ptr(oi), lgb.ptr(cb)), b._ctx)
+            r = O.find_best_threshold(hist, m["num_bin"], m["missing_type"], m["default_bin"], m["most_freq_bin"],
+                                      1, params, sg, sh, nd)
+            assert bool(oi[4]) == r["splittable"]
+            if not r["splittable"]:
+                continue
+            seen += 1
+            assert od[0] == r["gain"] and not oi[1]
+            assert (oi[2], oi[3]) == (r["left_count"], r["right_count"])
+            assert od[1] == r["left_output"] and od[2] == r["right_output"]
+            assert _bits_to_bins(cb) == sorted(r["cat_threshold"])
+    assert seen >= 20
+
+
+def test_categorical_split_search_int_bit_exact():
+    # split.rs:1194-1400 on identical packed histograms
+    ds, b, params = _categorical_stage_booster(True)
+    rng = np.random.default_rng(9)
+    lib = lgb._capi.lib()
+    seen = 0
+    for f in range(ds.num_features):
+        m = ds.feature_meta(f)
+        if m["bin_type"] != 1:
+            continue
+        for trial in range(20):
+            nb = m["num_bin"]
+            cnt = rng.integers(0, 50, nb)
+            gs = rng.integers(-8, 9, nb) * cnt // 3
+            hs = cnt * 7
+            packed = (gs.astype(np.int64) << 32) + hs.astype(np.int64)
+            tot = int(packed.sum())
+            packed[m["most_freq_bin"]] = 0
+            gscale, hscale = 0.01, 0.002
+            od = np.zeros(7)
+            oi = np.zeros(5, np.int32)
+            og = np.zeros(2, np.int64)
+            cb = np.zeros(8, np.uint32)
+            lgb._capi.check(lib.lgbmhip_find_best_split_int(b._ctx, f, lgb.ptr(packed), tot, gscale, hscale,
+                                                             int(cnt.sum()), 0.0, lgb.ptr(od), lgb.ptr(oi),
+                                                             lgb.ptr(og), lgb.ptr(cb)), b._ctx)
+            r = O.find_best_threshold_int(packed, nb, m["missing_type"], m["default_bin"], m["most_freq_bin"],
+                                          params, tot, gscale, hscale, int(cnt.sum()), bin_type=1)
+            assert bool(oi[4]) == r["splittable"]
+            if not r["splittable"]:
+                continue
+            seen += 1
+            assert od[0] == r["gain"] and not oi[1]
+            assert (oi[2], oi[3]) == (r["left_count"], r["right_count"])
+            assert od[1] == r["left_output"] and od[2] == r["right_output"]
+            assert (int(og[0]), int(og[1])) == (r["left_sum_gh"], r["right_sum_gh"])
+            assert _bits_to_bins(cb) == sorted(r["cat_threshold"])
+    assert seen >= 20
 
 
 def test_partition_is_stable_and_exact():
@@ -530,3 +620,23 @@ def test_categorical_trees_and_prediction_match_oracle():
     pb = b.predict_binned(ds.bins)[:, 0]
     r_all = b.predict(X, raw_score=True)
     assert np.all(np.abs(pb - r_all) <= 1e-9 * (1 + np.abs(r_all)))
+
+
+def test_quantized_categorical_trees_and_prediction_match_oracle():
+    X, y = _categorical_data(60000, 4)
+    params = dict(P_BIN, num_leaves=63, max_bin=255, categorical_feature=[1], min_data_per_group=50,
+                  cat_smooth=10.0, use_quantized_grad=True, num_grad_quant_bins=16, quant_train_renew_leaf=True)
+    ds, b = _booster(X, y, params)
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    for _ in range(3):
+        b.update()
+        ob.update()
+    ncat = 0
+    for t in range(3):
+        gt = b.tree(t)
+        _compare_tree(gt, ob.tree(t))
+        ncat += int(np.sum(np.asarray(gt.decision_type) & 1))
+    assert ncat > 0, "no categorical split was made"
+    np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(b.predict(X[:3000], raw_score=True), ob.predict_raw(X[:3000])[:, 0],
+                               rtol=1e-12, atol=1e-12)

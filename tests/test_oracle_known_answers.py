@@ -122,6 +122,70 @@ def test_categorical_one_hot_and_sorted():
     hist = np.array([[0.0, 10.0], [-10.0, 10.0], [10.0, 10.0]])
     r = O.find_best_threshold(hist, 3, 2, 1, 1, 1, P1, 0.0, 30.0, 30)
     assert r["splittable"] and not r["default_left"] and r["left_count"] == 10
+    assert r["cat_threshold"] == [1]
+    # many-vs-many: 6 bins, offset 1 (bin 0 not stored); the two most negative ctr go left
+    hist = np.array([[0.0, 0.0], [5.0, 100.0], [-50.0, 100.0], [2.0, 100.0], [-40.0, 100.0], [3.0, 100.0]])
+    p = dict(P1, min_data_per_group=1, cat_smooth=1.0)
+    r = O.find_best_threshold(hist, 6, 2, 0, 0, 1, p, -80.0, 600.0, 600)
+    assert r["splittable"] and r["cat_threshold"] == [2, 4]
+
+
+def _packed(g, h):
+    return (np.asarray(g, np.int64) << 32) + np.asarray(h, np.int64)
+
+
+def test_categorical_int_one_hot_and_sorted():
+    # the same known answers on packed integer sums with unit scales
+    # (split.rs:1194-1400 reduces to split.rs:446-670 there, kEpsilon aside)
+    hist = _packed([0, -10, 10], [10, 10, 10])
+    r = O.find_best_threshold_int(hist, 3, 2, 1, 1, P1, int(_packed(0, 30)), 1.0, 1.0, 30, bin_type=1)
+    assert r["splittable"] and not r["default_left"] and r["left_count"] == 10
+    assert r["cat_threshold"] == [1]
+    assert r["left_sum_gh"] == int(_packed(-10, 10)) and r["right_sum_gh"] == int(_packed(10, 20))
+    hist = _packed([0, 5, -50, 2, -40, 3], [0, 100, 100, 100, 100, 100])
+    p = dict(P1, min_data_per_group=1, cat_smooth=1.0)
+    r = O.find_best_threshold_int(hist, 6, 2, 0, 0, p, int(_packed(-80, 600)), 1.0, 1.0, 600, bin_type=1)
+    assert r["splittable"] and r["cat_threshold"] == [2, 4]
+    assert r["left_sum_gh"] == int(_packed(-90, 200)) and r["left_count"] == 200
+
+
+def test_categorical_int_agrees_with_float_search():
+    """Integer-valued histograms with unit scales and count = hessian: the
+    packed-sum search and the FP64 search must choose the same category set,
+    counts and sums, and gains/outputs equal up to the FP64 form's kEpsilon."""
+    rng = np.random.default_rng(21)
+    agree = 0
+    for trial in range(300):
+        nb = int(rng.integers(2, 48))
+        onehot = nb <= 4
+        cnt = rng.integers(0, 80, nb)
+        g = (rng.normal(size=nb) * cnt * rng.uniform(0.2, 2.0, nb)).astype(np.int64)
+        mfb = int(rng.integers(0, nb))
+        tot_g, tot_h, nd = int(g.sum()), int(cnt.sum()), int(cnt.sum())
+        if nd == 0:
+            continue
+        p = dict(P1, min_data_in_leaf=int(rng.integers(1, 20)), min_data_per_group=int(rng.integers(1, 60)),
+                 cat_smooth=float(rng.choice([1.0, 5.0, 10.0])), max_cat_threshold=int(rng.integers(1, 32)))
+        hf = np.stack([g, cnt], 1).astype(np.float64)
+        hf[mfb] = 0.0
+        hi = _packed(g, cnt)
+        hi[mfb] = 0
+        rf = O.find_best_threshold(hf, nb, 2, mfb, mfb, 1, p, float(tot_g), float(tot_h), nd)
+        ri = O.find_best_threshold_int(hi, nb, 2, mfb, mfb, p, int(_packed(tot_g, tot_h)), 1.0, 1.0, nd, bin_type=1)
+        assert rf["splittable"] == ri["splittable"], trial
+        if not ri["splittable"]:
+            continue
+        assert sorted(rf["cat_threshold"]) == sorted(ri["cat_threshold"]), (trial, onehot)
+        assert (rf["left_count"], rf["right_count"]) == (ri["left_count"], ri["right_count"])
+        # the chosen set's packed sum is the wrapping sum of its bins (mfb restored from the total)
+        full = hi.copy()
+        full[mfb] = int(_packed(tot_g, tot_h)) - int(np.delete(hi, mfb).sum())
+        assert ri["left_sum_gh"] == int(full[ri["cat_threshold"]].sum())
+        assert ri["left_sum_gh"] + ri["right_sum_gh"] == int(_packed(tot_g, tot_h))
+        assert abs(rf["gain"] - ri["gain"]) <= 1e-9 * max(1.0, abs(ri["gain"]))
+        assert abs(rf["left_output"] - ri["left_output"]) <= 1e-9 * max(1.0, abs(ri["left_output"]))
+        agree += 1
+    assert agree >= 100
 
 
 def _bruteforce_best(hist, sg, sh, nd, p):
