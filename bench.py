@@ -40,7 +40,15 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_form(lgb, ds, params, warmup, steps, holdout, prof_iters):
+def holdout_auc(holdout):
+    def q(b):
+        from oracle import pyoracle as O
+        Xv, yv = holdout
+        return O.auc(b.predict(Xv, raw_score=True), yv)
+    return q
+
+
+def run_form(lgb, ds, params, warmup, steps, quality, prof_iters):
     b = lgb.Booster(params, ds)
     for _ in range(warmup):
         b.update()
@@ -57,13 +65,11 @@ def run_form(lgb, ds, params, warmup, steps, holdout, prof_iters):
     for _ in range(prof_iters):
         b.update()
     prof = b.profile(0)
-    Xv, yv = holdout
     auc = None
     try:
-        from oracle import pyoracle as O
-        auc = O.auc(b.predict(Xv, raw_score=True), yv)
-    except Exception as e:  # the AUC gate needs the oracle's metric; report it missing
-        log(f"auc unavailable: {e}")
+        auc = quality(b)
+    except Exception as e:  # the quality gate needs the oracle's metrics; report it missing
+        log(f"quality metric unavailable: {e}")
     iters_total = warmup + 1 + steps + prof_iters
     b.free()
     return dict(iters_per_sec=its, ms_per_iter=1000.0 / its, prof=prof, auc=auc, iterations=iters_total)
@@ -115,6 +121,141 @@ def recorded_traffic(form, rows, max_bin, num_leaves):
     return t.get("hbm_bytes_per_launch")
 
 
+def _line(metric, value, unit, args, ms, dtype, config, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic", "config": config}
+    out.update(extra)
+    return out
+
+
+def _train_forms(lgb, ds, base, args, quality):
+    forms = {}
+    for form in args.forms.split(","):
+        params = dict(base)
+        if form == "quant":
+            params.update(use_quantized_grad=True, num_grad_quant_bins=16, stochastic_rounding=True,
+                          quant_train_renew_leaf=True)
+        r = run_form(lgb, ds, params, args.warmup, args.steps, quality, args.prof_iters)
+        log(f"{form}: {r['iters_per_sec']:.2f} it/s ({r['ms_per_iter']:.2f} ms/it), prof {json.dumps(r['prof'])}, "
+            f"quality {r['auc']}")
+        forms[form] = r
+    return forms
+
+
+def _forms_summary(forms, quality_name):
+    return {k: {"iters_per_sec": v["iters_per_sec"], "ms_per_iter": v["ms_per_iter"], quality_name: v["auc"],
+                "hist_ms_per_iter": v["prof"]["hist_ms"] / max(v["prof"]["iters"], 1),
+                "split_ms_per_iter": v["prof"]["split_ms"] / max(v["prof"]["iters"], 1),
+                "part_ms_per_iter": v["prof"]["part_ms"] / max(v["prof"]["iters"], 1)} for k, v in forms.items()}
+
+
+def workload_multiclass(lgb, datagen, args):
+    """10-class softmax, 2M x 100 (8 categorical columns), num_leaves=127:
+    one step is one iteration = 10 trees."""
+    n, nv = 2_000_000, 100_000
+    t = time.perf_counter()
+    X, y = datagen.multiclass_2m(n + nv)
+    log(f"data {X.shape} in {time.perf_counter() - t:.1f}s")
+    Xv, yv = X[n:].copy(), y[n:].copy()
+    base = dict(objective="multiclass", num_class=10, num_leaves=127, max_bin=255, min_data_in_leaf=20,
+                learning_rate=0.1, categorical_feature=datagen.MULTICLASS_CATEGORICAL)
+    ds = lgb.Dataset(X[:n], label=y[:n], params=base)
+    del X
+
+    def accuracy(b):
+        return float((b.predict(Xv).argmax(1) == yv).mean())
+
+    forms = _train_forms(lgb, ds, base, args, accuracy)
+    main_form = "quant" if "quant" in forms else list(forms)[0]
+    m = forms[main_form]
+    return _line("boosting iters/sec (10-class softmax, 10 trees per iteration)", m["iters_per_sec"], "iters/sec",
+                 args, m["ms_per_iter"], "int64" if main_form == "quant" else "f64",
+                 {"workload": f"multiclass softmax GBDT, {n} x 100 (80 N(0,1), 12 low-cardinality, 8 categorical "
+                              f"Zipf(1.2) columns), num_class=10, num_leaves=127, max_bin=255, lr=0.1",
+                  "rows": n, "features": 100, "form": main_form, "parallelism": "1 GPU"},
+                 forms=_forms_summary(forms, "holdout_accuracy"))
+
+
+def workload_rank(lgb, datagen, args):
+    """Lambdarank NDCG, ~2M documents in ~20k queries x 136, num_leaves=255."""
+    t = time.perf_counter()
+    X, rel, qb = datagen.rank_2m()
+    log(f"data {X.shape}, {len(qb) - 1} queries in {time.perf_counter() - t:.1f}s")
+    base = dict(objective="lambdarank", num_leaves=255, max_bin=255, min_data_in_leaf=20, learning_rate=0.1)
+    ds = lgb.Dataset(X, label=rel, group=np.diff(qb), params=base)
+    nq = 2000
+    Xs, rs, qs = X[:qb[nq]].copy(), rel[:qb[nq]].copy(), qb[:nq + 1].copy()
+    del X
+
+    def ndcg(b):
+        from oracle import pyoracle as O
+        return float(O.ndcg(b.predict(Xs, raw_score=True), rs, qs, 10))
+
+    forms = _train_forms(lgb, ds, base, args, ndcg)
+    main_form = "quant" if "quant" in forms else list(forms)[0]
+    m = forms[main_form]
+    return _line("boosting iters/sec (lambdarank NDCG)", m["iters_per_sec"], "iters/sec", args, m["ms_per_iter"],
+                 "int64" if main_form == "quant" else "f64",
+                 {"workload": f"lambdarank GBDT, {int(qb[-1])} documents in {len(qb) - 1} queries x 136, "
+                              f"num_leaves=255, max_bin=255, lr=0.1",
+                  "rows": int(qb[-1]), "features": 136, "form": main_form, "parallelism": "1 GPU"},
+                 forms=_forms_summary(forms, "train_ndcg10_first_2000_queries"))
+
+
+def workload_predict(lgb, datagen, args):
+    """Prediction of a 1000-tree, 255-leaf lambdarank model on 1,000,000 rows
+    x 136 already resident in HBM: raw f32 values and u8 bins. One step is
+    one pass over all rows."""
+    import torch
+    trees, rows = 1000, 1_000_000
+    t = time.perf_counter()
+    X, rel, qb = datagen.rank_2m()
+    nq = int(np.searchsorted(qb, 200_000))
+    base = dict(objective="lambdarank", num_leaves=255, max_bin=255, min_data_in_leaf=20, learning_rate=0.1,
+                use_quantized_grad=True, num_grad_quant_bins=16, quant_train_renew_leaf=True)
+    dtrain = lgb.Dataset(X[:qb[nq]], label=rel[:qb[nq]], group=np.diff(qb[:nq + 1]), params=base)
+    b = lgb.Booster(base, dtrain)
+    for _ in range(trees):
+        b.update()
+    leaves = [b.tree(i).num_leaves for i in range(trees)]
+    log(f"model: {trees} trees, mean {np.mean(leaves):.1f} leaves, trained in {time.perf_counter() - t:.1f}s")
+    Xp = np.ascontiguousarray(X[:rows])
+    bins = lgb.Dataset(Xp, reference=dtrain).bins
+    F = bins.shape[1]
+    res = {}
+    for form, host, elem in (("raw", Xp, 4), ("binned", bins, 1)):
+        dx = torch.from_numpy(host).cuda()
+        out = torch.empty(rows, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        for _ in range(args.warmup):
+            b.predict_device(dx.data_ptr(), rows, host.shape[1], out.data_ptr(), binned=form == "binned")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.predict_device(dx.data_ptr(), rows, host.shape[1], out.data_ptr(), binned=form == "binned")
+        dt = (time.perf_counter() - t0) / args.steps
+        ref = np.ravel(b.predict(Xp[:20000], raw_score=True))
+        got = out[:20000].cpu().numpy()
+        byts = rows * host.shape[1] * elem + rows * 8
+        res[form] = dict(rows_per_sec=rows / dt, ms_per_pass=dt * 1e3, gbs=byts / dt / 1e9,
+                         max_abs_diff_vs_host_predict=float(np.max(np.abs(got - ref))))
+        log(f"predict {form}: {rows / dt / 1e6:.2f} M rows/s, {dt * 1e3:.3f} ms/pass, {byts / dt / 1e9:.1f} GB/s")
+        del dx, out
+    m = res["binned"]
+    return _line("prediction rows/sec (1000 trees x 255 leaves, u8 bins; raw f32 form beside it)",
+                 m["rows_per_sec"], "rows/sec", args, m["ms_per_pass"], "u8",
+                 {"workload": f"predict {rows} x 136 rows (lambdarank feature generator) with a {trees}-tree, "
+                              f"255-leaf model, inputs resident in HBM", "rows": rows, "features": 136,
+                  "trees": trees, "inner_features": F, "parallelism": "1 GPU"},
+                 forms=res,
+                 roofline=dict(bound="hbm", achieved=m["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                               frac=m["gbs"] / HBM_PEAK_GBS, traffic=None, kernel="k_predict<binned>"))
+
+
+WORKLOADS = {"binary_10p5m": None, "multiclass_2m": workload_multiclass, "rank_2m": workload_rank,
+             "predict_1m": workload_predict}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +272,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="stream launches instead of graphs (counter passes)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter scaling) and exit")
+    ap.add_argument("--workload", default="binary_10p5m", choices=list(WORKLOADS),
+                    help="binary_10p5m (the headline line); the others print their own line")
     args = ap.parse_args()
     if args.copy_calibrate:
         copy_calibrate(args.copy_calibrate)
@@ -142,6 +285,9 @@ def main():
 
     lgb = importlib.import_module("lightgbm-rust_amd")
     datagen = importlib.import_module("lightgbm-rust_amd.datagen")
+    if args.workload != "binary_10p5m":
+        print(json.dumps(WORKLOADS[args.workload](lgb, datagen, args)), flush=True)
+        return
     t = time.perf_counter()
     X, y = datagen.binary_10p5m(args.rows)
     Xv, yv = datagen.holdout("binary_10p5m", 200_000)
@@ -158,7 +304,7 @@ def main():
         if form == "quant":
             params.update(use_quantized_grad=True, num_grad_quant_bins=16, stochastic_rounding=True,
                           quant_train_renew_leaf=True)
-        r = run_form(lgb, ds, params, args.warmup, args.steps, (Xv, yv), args.prof_iters)
+        r = run_form(lgb, ds, params, args.warmup, args.steps, holdout_auc((Xv, yv)), args.prof_iters)
         p = r["prof"]
         hist_gbs = p["hist_bytes"] / (p["hist_ms"] * 1e-3) / 1e9 if p["hist_ms"] > 0 else None
         r["hist_gbs"] = hist_gbs

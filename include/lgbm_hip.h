@@ -174,6 +174,14 @@ lgbmhip_status lgbmhip_partition(lgbmhip_ctx* ctx, const uint32_t* rows, int64_t
 lgbmhip_status lgbmhip_predict_raw(lgbmhip_ctx* ctx, const float* X, int64_t rows, int32_t ncol, double* out);
 /* Raw scores on binned rows (rows x F u8, the mappers of the training data): out [rows][K]. */
 lgbmhip_status lgbmhip_predict_binned(lgbmhip_ctx* ctx, const uint8_t* bins, int64_t rows, double* out);
+/* Prediction over rows already resident in device memory (exactly one of d_X:
+ * rows x ncol f32 raw values, or d_bins: rows x F u8 bins) into a device
+ * [rows][K] f64 buffer; returns when the output is written. No counterpart in
+ * the reference, whose prediction takes host arrays (predict.rs:231-339); it
+ * is the same computation without the host copies, for callers whose rows
+ * already live on the GPU. */
+lgbmhip_status lgbmhip_predict_device(lgbmhip_ctx* ctx, const float* d_X, const uint8_t* d_bins, int64_t rows,
+                                      int32_t ncol, double* d_out);
 
 /* Profiling of training iterations: enable = 1 records HIP events around every
  * histogram, split and partition launch (graphs off), 0 stops, -1 only reads.

@@ -259,6 +259,15 @@ class Booster:
         check(lib().lgbmhip_predict_binned(self._ctx, ptr(b), b.shape[0], ptr(out)), self._ctx)
         return out.reshape(b.shape[0], self.K)
 
+    def predict_device(self, x_ptr, rows, ncol, out_ptr, binned=False):
+        """Raw scores of rows resident in device memory (integer device
+        addresses, e.g. torch tensors' data_ptr()): x is rows x ncol f32, or
+        rows x F u8 bins when `binned`; out is rows x K f64."""
+        if binned:
+            check(lib().lgbmhip_predict_device(self._ctx, None, x_ptr, rows, 0, out_ptr), self._ctx)
+        else:
+            check(lib().lgbmhip_predict_device(self._ctx, x_ptr, None, rows, ncol, out_ptr), self._ctx)
+
     def profile(self, enable=-1):
         """Kernel-kind timing totals (see lgbmhip_profile); enable=1 starts, 0 stops, -1 reads."""
         out = np.zeros(11, np.float64)

@@ -95,6 +95,7 @@ SIGNATURES = {
     "lgbmhip_partition": (I32, [P, P, I64, I32, C.c_uint32, I32, P, P]),
     "lgbmhip_predict_raw": (I32, [P, P, I64, I32, P]),
     "lgbmhip_predict_binned": (I32, [P, P, I64, P]),
+    "lgbmhip_predict_device": (I32, [P, P, P, I64, I32, P]),
     "lgbmhip_last_timings": (I32, [P, P]),
     "lgbmhip_profile": (I32, [P, I32, P]),
     "lgbmhip_set_tuning": (I32, [P, I32, I32, I32, I32, I32, I32]),

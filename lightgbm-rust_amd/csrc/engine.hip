@@ -254,7 +254,7 @@ struct Ctx {
   double r_min_in = 0, r_max_in = 0, r_factor = 0;
   // prediction cache
   bool pred_dirty = true;
-  DevBuf<PredNode> p_nodes;
+  DevBuf<uint4> p_nodes;
   DevBuf<int> p_node_off, p_leaf_off, p_nleaves;
   DevBuf<double> p_leafval;
   DevBuf<unsigned> p_cat;
@@ -1058,7 +1058,7 @@ static void train_tree(Ctx* c, const float* g, const float* h) {
 // bitset (raw values, tree.rs:399-418) or the bin bitset (binned rows,
 // tree.rs:613-618).
 static void build_pred(Ctx* c, bool binned) {
-  std::vector<PredNode> nodes;
+  std::vector<uint4> nodes;
   std::vector<int> noff, loff, nl;
   std::vector<double> lv;
   std::vector<unsigned> cats;
@@ -1066,30 +1066,35 @@ static void build_pred(Ctx* c, bool binned) {
     noff.push_back((int)nodes.size());
     loff.push_back((int)lv.size());
     nl.push_back(t.num_leaves);
+    if (t.num_leaves > 32768) throw Error(LGBMHIP_UNSUPPORTED, "trees above 32768 leaves in prediction");
     for (int i = 0; i < t.num_leaves - 1; ++i) {
-      PredNode n{};
-      n.threshold = t.threshold[i];
-      n.feature = t.split_feature[i];
-      n.left = t.left_child[i];
-      n.right = t.right_child[i];
-      n.thr_bin = (int)t.threshold_in_bin[i];
-      n.missing_type = (int8_t)((t.decision_type[i] >> 2) & 3);
-      n.default_left = (int8_t)((t.decision_type[i] & 2) ? 1 : 0);
-      n.is_cat = (int8_t)(t.decision_type[i] & 1);
-      n.cat_off = -1;
-      n.cat_len = 0;
-      if (n.is_cat) {
+      const int feat = binned ? t.split_feature_inner[i] : t.split_feature[i];
+      if (feat < 0 || feat > 0xFFFF) throw Error(LGBMHIP_UNSUPPORTED, "split column above 65535 in prediction");
+      const int mt = (t.decision_type[i] >> 2) & 3;
+      const bool cat = t.decision_type[i] & 1;
+      unsigned flags = (t.decision_type[i] & 2) ? 1u : 0u;
+      flags |= mt == 1 ? 2u : mt == 2 ? 4u : 0u;
+      flags |= cat ? 8u : 0u;
+      uint4 n;
+      n.x = (unsigned)feat | flags << 16 | (t.threshold_in_bin[i] & 0xFFu) << 24;
+      n.y = ((unsigned)t.left_child[i] & 0xFFFFu) | ((unsigned)t.right_child[i] & 0xFFFFu) << 16;
+      if (cat) {
         const int ci = (int)t.threshold_in_bin[i];
         const auto& bnd = binned ? t.cat_boundaries_inner : t.cat_boundaries;
         const auto& words = binned ? t.cat_threshold_inner : t.cat_threshold;
-        n.cat_off = (int)cats.size();
-        n.cat_len = bnd[ci + 1] - bnd[ci];
+        n.z = (unsigned)cats.size();
+        n.w = (unsigned)(bnd[ci + 1] - bnd[ci]);
         for (int w = bnd[ci]; w < bnd[ci + 1]; ++w) cats.push_back(words[w]);
+      } else if (binned) {
+        const auto& m = c->metas[t.split_feature_inner[i]];
+        n.z = mt == 1 ? (unsigned)m.default_bin : (unsigned)(m.num_bin - 1);
+        n.w = 0;
+      } else {
+        uint64_t bits;
+        std::memcpy(&bits, &t.threshold[i], 8);
+        n.z = (unsigned)bits;
+        n.w = (unsigned)(bits >> 32);
       }
-      const int fi = t.split_feature_inner[i];
-      n.default_bin = c->metas[fi].default_bin;
-      n.nan_bin = c->metas[fi].num_bin - 1;
-      n.pad = 0;
       nodes.push_back(n);
     }
     for (int l = 0; l < t.num_leaves; ++l) lv.push_back(t.leaf_value[l]);
@@ -1102,7 +1107,7 @@ static void build_pred(Ctx* c, bool binned) {
   c->p_leafval.alloc(std::max<size_t>(lv.size(), 1));
   c->p_cat.alloc(std::max<size_t>(cats.size(), 1));
   if (!cats.empty()) HIP_CHECK(hipMemcpy(c->p_cat.p, cats.data(), cats.size() * 4, hipMemcpyHostToDevice));
-  if (!nodes.empty()) HIP_CHECK(hipMemcpy(c->p_nodes.p, nodes.data(), nodes.size() * sizeof(PredNode), hipMemcpyHostToDevice));
+  if (!nodes.empty()) HIP_CHECK(hipMemcpy(c->p_nodes.p, nodes.data(), nodes.size() * sizeof(uint4), hipMemcpyHostToDevice));
   if (!noff.empty()) {
     HIP_CHECK(hipMemcpy(c->p_node_off.p, noff.data(), noff.size() * 4, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(c->p_leaf_off.p, loff.data(), loff.size() * 4, hipMemcpyHostToDevice));
@@ -1113,9 +1118,12 @@ static void build_pred(Ctx* c, bool binned) {
   c->p_binned = binned;
 }
 
-static void predict(Ctx* c, const float* X, const uint8_t* bins, int64_t rows, int ncol, double* out) {
-  set_device(c);
-  if (c->pred_dirty || c->p_binned != (bins != nullptr)) build_pred(c, bins != nullptr);
+// Launches prediction over device-resident rows (raw f32 X or u8 bins) into
+// a device [rows][K] output on the context's stream.
+static void launch_predict(Ctx* c, const float* dX, const uint8_t* dbins, int64_t rows, int ncol, double* dout) {
+  const bool binned = dbins != nullptr;
+  if (c->pred_dirty || c->p_binned != binned) build_pred(c, binned);
+  if (rows <= 0) return;
   PredArgs a{};
   a.nodes = c->p_nodes.p;
   a.tree_node_off = c->p_node_off.p;
@@ -1126,27 +1134,41 @@ static void predict(Ctx* c, const float* X, const uint8_t* bins, int64_t rows, i
   a.T = c->p_T;
   a.K = c->K;
   a.rows = rows;
-  a.ncol = ncol;
-  a.F = c->F;
+  a.ncol = binned ? c->F : ncol;
+  a.X = dX;
+  a.bins = dbins;
+  a.out = dout;
+  // odd word stride: lanes reading one column of their rows fall in distinct banks
+  const int words = binned ? (a.ncol + 3) / 4 : a.ncol;
+  const int sw = words | 1;
+  a.stride = binned ? sw * 4 : sw;
+  const size_t lds = (size_t)kPredRows * sw * 4;
+  const dim3 grid((unsigned)((rows + kPredRows - 1) / kPredRows));
+  if (lds <= (size_t)c->lds_max) {
+    auto kern = binned ? k_predict<true, true> : k_predict<false, true>;
+    HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, grid, dim3(kPredRows), lds, c->stream, a);
+  } else {
+    auto kern = binned ? k_predict<true, false> : k_predict<false, false>;
+    hipLaunchKernelGGL(kern, grid, dim3(kPredRows), 0, c->stream, a);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+static void predict(Ctx* c, const float* X, const uint8_t* bins, int64_t rows, int ncol, double* out) {
+  set_device(c);
   DevBuf<float> dx;
   DevBuf<uint8_t> db;
   DevBuf<double> dout;
-  dout.alloc((size_t)rows * c->K);
+  dout.alloc(std::max<size_t>((size_t)rows * c->K, 1));
   if (X) {
-    dx.alloc((size_t)rows * ncol);
+    dx.alloc(std::max<size_t>((size_t)rows * ncol, 1));
     HIP_CHECK(hipMemcpyAsync(dx.p, X, (size_t)rows * ncol * 4, hipMemcpyHostToDevice, c->stream));
-    a.X = dx.p;
   } else {
-    db.alloc((size_t)rows * c->F);
+    db.alloc(std::max<size_t>((size_t)rows * c->F, 1));
     HIP_CHECK(hipMemcpyAsync(db.p, bins, (size_t)rows * c->F, hipMemcpyHostToDevice, c->stream));
-    a.bins = db.p;
   }
-  a.out = dout.p;
-  const int grid = (int)((rows + 255) / 256);
-  if (rows > 0) {
-    if (X) hipLaunchKernelGGL(k_predict_raw, dim3(grid), dim3(256), 0, c->stream, a);
-    else hipLaunchKernelGGL(k_predict_binned, dim3(grid), dim3(256), 0, c->stream, a);
-  }
+  launch_predict(c, X ? dx.p : nullptr, X ? nullptr : db.p, rows, ncol, dout.p);
   HIP_CHECK(hipMemcpyAsync(out, dout.p, (size_t)rows * c->K * 8, hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
 }
@@ -1607,6 +1629,17 @@ lgbmhip_status lgbmhip_predict_raw(lgbmhip_ctx* ctx, const float* X, int64_t row
 }
 lgbmhip_status lgbmhip_predict_binned(lgbmhip_ctx* ctx, const uint8_t* bins, int64_t rows, double* out) {
   return guarded(ctx, [&] { predict(&ctx->c, nullptr, bins, rows, 0, out); });
+}
+lgbmhip_status lgbmhip_predict_device(lgbmhip_ctx* ctx, const float* d_X, const uint8_t* d_bins, int64_t rows,
+                                      int32_t ncol, double* d_out) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if ((d_X == nullptr) == (d_bins == nullptr)) throw Error(LGBMHIP_INVALID_PARAMETER, "exactly one of X, bins");
+    if (!d_out || rows < 0 || (d_X && ncol <= 0)) throw Error(LGBMHIP_INVALID_PARAMETER, "bad prediction buffers");
+    set_device(c);
+    launch_predict(c, d_X, d_bins, rows, ncol, d_out);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
 }
 lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out11) {
   return guarded(ctx, [&] {

@@ -241,40 +241,56 @@ __global__ void __launch_bounds__(256) k_binary_grad(const double* __restrict__ 
   }
 }
 
-// objective/multiclass.rs:16-30, 126-154 (K <= 64 classes).
-__global__ void k_multiclass_grad(const double* __restrict__ score, const float* __restrict__ label,
+// objective/multiclass.rs:16-30, 126-154 (K <= 16 classes). The softmax
+// terms exp(s_c - max) are recomputed per pass rather than held in a per-row
+// array (which would live in scratch); exp is deterministic, so the sum and
+// the probabilities are the ones a stored array would give. The per-class
+// |g|, |h| maxima are reduced per wave before one LDS atomic per class.
+__global__ void __launch_bounds__(256) k_multiclass_grad(const double* __restrict__ score, const float* __restrict__ label,
                                   const float* __restrict__ weight, long long n, int K, float* __restrict__ g,
                                   float* __restrict__ h, IterDev* it) {
-  double factor = (double)K / (double)((float)K - 1.0f);
-  __shared__ unsigned smg[64], smh[64];
-  for (int c = threadIdx.x; c < K; c += blockDim.x) { smg[c] = 0; smh[c] = 0; }
+  const double factor = (double)K / (double)((float)K - 1.0f);
+  __shared__ unsigned smg[16], smh[16];
+  if (threadIdx.x < 16) { smg[threadIdx.x] = 0; smh[threadIdx.x] = 0; }
   __syncthreads();
-  double rec[64];
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    double wmax = score[i];
+  const int lane = threadIdx.x & 63;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  // block-uniform trip count: every lane takes part in the wave reductions
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += stride) {
+    const long long i = base + threadIdx.x;
+    const bool ok = i < n;
+    const long long r = ok ? i : 0;
+    double wmax = score[r];
     for (int c = 1; c < K; ++c) {
-      double v = score[(long long)c * n + i];
+      const double v = score[(long long)c * n + r];
       wmax = v < wmax ? wmax : v;
     }
     double wsum = 0.0;
+    for (int c = 0; c < K; ++c) wsum += exp(score[(long long)c * n + r] - wmax);
+    const int li = (int)label[r];
+    const double w = weight ? (double)weight[r] : 1.0;
     for (int c = 0; c < K; ++c) {
-      rec[c] = exp(score[(long long)c * n + i] - wmax);
-      wsum += rec[c];
-    }
-    int li = (int)label[i];
-    for (int c = 0; c < K; ++c) {
-      double p = rec[c] / wsum;
-      double gg = li == c ? p - (double)1.0f : p;
-      double hh = factor * p * ((double)1.0f - p);
+      const double pc = exp(score[(long long)c * n + r] - wmax) / wsum;
+      double gg = li == c ? pc - (double)1.0f : pc;
+      double hh = factor * pc * ((double)1.0f - pc);
       if (weight) {
-        gg *= (double)weight[i];
-        hh *= (double)weight[i];
+        gg *= w;
+        hh *= w;
       }
-      float gv = (float)gg, hv = (float)hh;
-      g[(long long)c * n + i] = gv;
-      h[(long long)c * n + i] = hv;
-      atomicMax(&smg[c], __float_as_uint(fabsf(gv)));
-      atomicMax(&smh[c], __float_as_uint(fabsf(hv)));
+      const float gv = (float)gg, hv = (float)hh;
+      if (ok) {
+        g[(long long)c * n + i] = gv;
+        h[(long long)c * n + i] = hv;
+      }
+      unsigned mg = ok ? __float_as_uint(fabsf(gv)) : 0u, mh = ok ? __float_as_uint(fabsf(hv)) : 0u;
+      for (int o = 32; o > 0; o >>= 1) {
+        mg = max(mg, (unsigned)__shfl_xor((int)mg, o, 64));
+        mh = max(mh, (unsigned)__shfl_xor((int)mh, o, 64));
+      }
+      if (lane == 0) {
+        atomicMax(&smg[c], mg);
+        atomicMax(&smh[c], mh);
+      }
     }
   }
   __syncthreads();
@@ -309,7 +325,7 @@ struct RankArgs {
   float* h;
 };
 
-constexpr int kRankThreads = 128;
+constexpr int kRankThreads = 256;
 constexpr int kRankChunk = 1024;
 
 __device__ __forceinline__ double rank_sigmoid(const RankArgs& a, double s) {
@@ -352,7 +368,6 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
   const bool scale_by_delta = a.norm && best != worst;
   __shared__ float st_l[kRankChunk], st_h[kRankChunk];
   __shared__ double st_p[kRankChunk];
-  __shared__ int st_ok[kRankChunk];
   double S = 0.0;
   const int imax = min(cnt - 1, max(a.trunc, 0));
   for (int i = 0; i < imax; ++i) {
@@ -365,7 +380,7 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
       for (int j = j0 + tid; j < jn; j += kRankThreads) {
         const int bj = sorted[j];
         const float lb = lab[bj];
-        int ok = s[bj] != -INFINITY && la != lb;
+        const bool ok = s[bj] != -INFINITY && la != lb;
         if (ok) {
           const bool anchor_high = la > lb;
           const int high = anchor_high ? ai : bj, low = anchor_high ? bj : ai;
@@ -389,14 +404,37 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
           st_l[j - j0] = anchor_high ? fl : -fl;
           st_h[j - j0] = fh;
           st_p[j - j0] = pl;
+        } else {
+          // a skipped pair adds nothing: x + (-0.0f) == x and S - 2 * 0.0 == S
+          // bit for bit, so the anchor's chains need no branch
+          st_l[j - j0] = -0.0f;
+          st_h[j - j0] = -0.0f;
+          st_p[j - j0] = 0.0;
         }
-        st_ok[j - j0] = ok;
       }
       __syncthreads();
       if (tid == 0) {
-        for (int k = 0; k < jn - j0; ++k) {
-          if (!st_ok[k]) continue;
-          // anchor high: lambdas[high] += p; anchor low: lambdas[low] -= p
+        // anchor high: lambdas[high] += p; anchor low: lambdas[low] -= p.
+        // Three independent chains in j order; loads batched ahead of the adds.
+        const int m = jn - j0;
+        int k = 0;
+        for (; k + 8 <= m; k += 8) {
+          float l8[8], h8[8];
+          double p8[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            l8[u] = st_l[k + u];
+            h8[u] = st_h[k + u];
+            p8[u] = st_p[k + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            anc_l += l8[u];
+            anc_h += h8[u];
+            S -= 2.0 * p8[u];
+          }
+        }
+        for (; k < m; ++k) {
           anc_l += st_l[k];
           anc_h += st_h[k];
           S -= 2.0 * st_p[k];
@@ -1136,21 +1174,17 @@ __device__ __forceinline__ bool goes_left(int b, const FeatDev& fd, const SplitR
 }
 
 // ---------------------------------------------------------------- prediction
-// Flattened trees: nodes of all trees concatenated; a node's children are
-// node indices (>= 0) or ~leaf; leaf values in a parallel array.
-struct PredNode {
-  double threshold;
-  int feature;      // raw column (raw predict) or inner feature (binned predict)
-  int left, right;
-  int thr_bin;
-  int8_t missing_type, default_left, is_cat, pad;
-  int cat_off;      // offset of the category bitset words, -1 none
-  int cat_len;
-  int default_bin, nan_bin;  // binned predict: missing bins of the node's feature
-};
-
+// Flattened trees: the internal nodes of all trees concatenated, 16 bytes each.
+//   x = feature (16 bits) | flags << 16 | thr_bin << 24
+//       flags: 1 default_left, 2 missing type Zero, 4 missing type NaN, 8 categorical
+//   y = left (int16) | right (int16) << 16; a child >= 0 is a node, < 0 is ~leaf
+//   z, w = raw numerical: the f64 threshold's bits (low, high);
+//          categorical: offset and length of the node's bitset words;
+//          binned numerical: z = the feature's missing bin (default bin for
+//          Zero, last bin for NaN).
+// The feature is the raw column for raw rows and the inner feature for bins.
 struct PredArgs {
-  const PredNode* nodes;
+  const uint4* nodes;
   const int* tree_node_off;   // [T]
   const int* tree_leaf_off;   // [T]
   const int* tree_num_leaves; // [T]
@@ -1158,83 +1192,122 @@ struct PredArgs {
   const unsigned* cat_words;
   int T, K;
   long long rows;
-  int ncol;
+  int ncol;                   // columns of a row: raw columns, or F for bins
+  int stride;                 // staged row stride in LDS (elements)
   const float* X;             // raw rows (row-major f32)
-  const uint8_t* bins;        // binned rows (row-major u8, F columns)
-  int F;
+  const uint8_t* bins;        // binned rows (row-major u8)
   double* out;                // [rows][K]
 };
 
-// predict.rs:95-109 (Tree::decision semantics), one row per thread.
-__global__ void k_predict_raw(PredArgs a) {
-  long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.rows) return;
-  const float* x = a.X + (size_t)r * a.ncol;
-  double acc[16];
-  for (int k = 0; k < a.K; ++k) acc[k] = 0.0;
-  for (int t = 0; t < a.T; ++t) {
-    const int k = t % a.K;
-    int leaf = 0;
-    if (a.tree_num_leaves[t] > 1) {
-      const PredNode* nd = a.nodes + a.tree_node_off[t];
-      int node = 0;
-      while (node >= 0) {
-        const PredNode& n = nd[node];
-        double v = n.feature < a.ncol ? (double)x[n.feature] : 0.0;
-        if (n.is_cat) {
-          // tree.rs:407-418: NaN and negative categories go right
-          bool lft = false;
-          if (!isnan(v)) {
-            const int c = (int)v;   // saturating, as the reference's `as i32`
-            if (c >= 0 && (c >> 5) < n.cat_len) lft = (a.cat_words[n.cat_off + (c >> 5)] >> (c & 31)) & 1u;
-          }
-          node = lft ? n.left : n.right;
-          continue;
-        }
-        // predict.rs:95-109 step: NaN is zero unless the node splits NaN
-        const bool nan = isnan(v);
-        if (nan && n.missing_type != 2) v = 0.0;
-        const bool zero = v >= -KZERO && v <= KZERO;
-        if ((n.missing_type == 1 && zero) || (n.missing_type == 2 && nan)) node = n.default_left ? n.left : n.right;
-        else node = v <= n.threshold ? n.left : n.right;
-      }
-      leaf = ~node;
+constexpr int kPredRows = 64;   // rows per block (one wave, one row per lane)
+#ifndef LGBMHIP_PRED_TREES
+#define LGBMHIP_PRED_TREES 4
+#endif
+constexpr int kPredTrees = LGBMHIP_PRED_TREES;   // trees of one class walked together per lane
+
+// One decision of a node (predict.rs:95-109 numerical step, tree.rs:399-418
+// categorical; tree.rs:605-632 on bins).
+template <bool BINNED>
+__device__ __forceinline__ int pred_step(const uint4 n, const float* xr, const uint8_t* br, int ncol,
+                                         const unsigned* cat_words) {
+  const int f = n.x & 0xFFFF;
+  const unsigned flags = (n.x >> 16) & 0xFF;
+  const int left = (int)(short)(n.y & 0xFFFF), right = (int)(short)(n.y >> 16);
+  bool go_left;
+  if (BINNED) {
+    const int b = br[f];
+    if (flags & 8) {
+      go_left = (b >> 5) < (int)n.w && ((cat_words[n.z + (b >> 5)] >> (b & 31)) & 1u);
+    } else {
+      const bool is_default = (flags & 6) && b == (int)n.z;
+      go_left = is_default ? (flags & 1) != 0 : b <= (int)(n.x >> 24);
     }
-    acc[k] += a.leaf_val[a.tree_leaf_off[t] + leaf];
+  } else {
+    double v = f < ncol ? (double)xr[f] : 0.0;
+    if (flags & 8) {
+      go_left = false;
+      if (!isnan(v)) {
+        const int c = (int)v;  // saturating, as the reference's `as i32`
+        go_left = c >= 0 && (c >> 5) < (int)n.w && ((cat_words[n.z + (c >> 5)] >> (c & 31)) & 1u);
+      }
+    } else {
+      const bool nan = isnan(v);
+      if (nan && !(flags & 4)) v = 0.0;
+      const bool zero = v >= -KZERO && v <= KZERO;
+      const bool is_default = ((flags & 2) && zero) || ((flags & 4) && nan);
+      const double thr = __hiloint2double((int)n.w, (int)n.z);
+      go_left = is_default ? (flags & 1) != 0 : v <= thr;
+    }
   }
-  for (int k = 0; k < a.K; ++k) a.out[(size_t)r * a.K + k] = acc[k];
+  return go_left ? left : right;
 }
 
-// tree.rs:605-632 leaf_binned_with: bins compared with threshold_in_bin.
-__global__ void k_predict_binned(PredArgs a) {
-  long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.rows) return;
-  const uint8_t* x = a.bins + (size_t)r * a.F;
-  double acc[16];
-  for (int k = 0; k < a.K; ++k) acc[k] = 0.0;
-  for (int t = 0; t < a.T; ++t) {
-    const int k = t % a.K;
-    int leaf = 0;
-    if (a.tree_num_leaves[t] > 1) {
-      const PredNode* nd = a.nodes + a.tree_node_off[t];
-      int node = 0;
-      while (node >= 0) {
-        const PredNode& n = nd[node];
-        int b = x[n.feature];
-        if (n.is_cat) {
-          bool lft = (b >> 5) < n.cat_len && ((a.cat_words[n.cat_off + (b >> 5)] >> (b & 31)) & 1u);
-          node = lft ? n.left : n.right;
-          continue;
-        }
-        bool is_default = (n.missing_type == 1 && b == n.default_bin) || (n.missing_type == 2 && b == n.nan_bin);
-        if (is_default) node = n.default_left ? n.left : n.right;
-        else node = b <= n.thr_bin ? n.left : n.right;
-      }
-      leaf = ~node;
+// Gbdt::predict_matrix_early_stop without early stopping (predict.rs:231-339):
+// one row per lane, the block's rows staged in LDS (STAGE) at an odd word
+// stride so lanes reading one feature hit distinct banks. For each class,
+// the class's trees are summed in model order; kPredTrees of them are walked
+// together so their node loads overlap.
+template <bool BINNED, bool STAGE>
+__global__ void __launch_bounds__(kPredRows) k_predict(PredArgs a) {
+  extern __shared__ __align__(16) unsigned char pred_lds[];
+  const long long r0 = (long long)blockIdx.x * kPredRows;
+  const int nrow = (int)min((long long)kPredRows, a.rows - r0);
+  const int lane = threadIdx.x;
+  const float* xr = nullptr;
+  const uint8_t* br = nullptr;
+  if (STAGE) {
+    // copy the block's contiguous rows, element e = (r, f) -> r * stride + f
+    const long long n = (long long)nrow * a.ncol;
+    int r = 0, f = lane;
+    while (f >= a.ncol) { f -= a.ncol; ++r; }
+    const int step_r = kPredRows / a.ncol, step_f = kPredRows % a.ncol;
+    for (long long e = lane; e < n; e += kPredRows) {
+      if (BINNED) pred_lds[r * a.stride + f] = a.bins[r0 * a.ncol + e];
+      else ((float*)pred_lds)[r * a.stride + f] = a.X[r0 * a.ncol + e];
+      r += step_r;
+      f += step_f;
+      if (f >= a.ncol) { f -= a.ncol; ++r; }
     }
-    acc[k] += a.leaf_val[a.tree_leaf_off[t] + leaf];
+    __syncthreads();
+    const int rr = min(lane, nrow - 1);
+    if (BINNED) br = pred_lds + rr * a.stride;
+    else xr = (const float*)pred_lds + rr * a.stride;
+  } else {
+    const long long rr = r0 + min(lane, nrow - 1);
+    if (BINNED) br = a.bins + rr * a.ncol;
+    else xr = a.X + rr * a.ncol;
   }
-  for (int k = 0; k < a.K; ++k) a.out[(size_t)r * a.K + k] = acc[k];
+  for (int k = 0; k < a.K; ++k) {
+    double acc = 0.0;
+    for (int t0 = k; t0 < a.T; t0 += kPredTrees * a.K) {
+      int node[kPredTrees], base[kPredTrees];
+#pragma unroll
+      for (int g = 0; g < kPredTrees; ++g) {
+        const int t = t0 + g * a.K;
+        const bool walk = t < a.T && a.tree_num_leaves[t] > 1;
+        node[g] = walk ? 0 : ~0;
+        base[g] = t < a.T ? a.tree_node_off[t] : 0;
+      }
+      bool more = true;
+      while (more) {
+        more = false;
+#pragma unroll
+        for (int g = 0; g < kPredTrees; ++g) {
+          if (node[g] >= 0) {
+            const uint4 n = a.nodes[base[g] + node[g]];
+            node[g] = pred_step<BINNED>(n, xr, br, a.ncol, a.cat_words);
+            more |= node[g] >= 0;
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < kPredTrees; ++g) {
+        const int t = t0 + g * a.K;
+        if (t < a.T) acc += a.leaf_val[a.tree_leaf_off[t] + ~node[g]];
+      }
+    }
+    if (lane < nrow) a.out[(r0 + lane) * a.K + k] = acc;
+  }
 }
 
 // Stage entry point: stable partition of explicit rows (partition.rs:118-174).

@@ -8,7 +8,14 @@ Workloads (rows x features):
   binary_100k   : 100,000 x 28 N(0,1); y ~ Bernoulli(sigmoid(w.x + 0.5 x0 x1))
   binary_10p5m  : 10,500,000 x 28 (20 N(0,1) + 8 lognormal(0,1)); same label
                   model, then 5% of labels flipped
+  multiclass_2m : 2,000,000 x 100, 10 classes: 80 N(0,1) columns, 12 columns
+                  of 2-16 levels, 8 categorical columns of 50 Zipf(1.2)
+                  categories; y = argmax_c(x.W_c + cat embeddings + Gumbel)
+  rank_2m       : ~20,000 queries (~2M documents) x 136, graded relevance 0-4
 """
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 _GOLD = np.uint64(0x9E3779B97F4A7C15)
@@ -38,23 +45,39 @@ def normals(seed, start, n):
     return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
 
 
-def _features(c, n, F, lognormal_cols=(), chunk=1 << 20):
+def _chunks(n, chunk, fn):
+    """fn(r0, r1) over row chunks on a few threads (numpy releases the GIL in
+    its array loops); every chunk depends only on its own stream offsets, so
+    the result does not depend on the thread count."""
+    ranges = [(r0, min(n, r0 + chunk)) for r0 in range(0, n, chunk)]
+    workers = max(1, min(16, os.cpu_count() or 1, len(ranges)))
+    if workers == 1:
+        for r in ranges:
+            fn(*r)
+        return
+    with ThreadPoolExecutor(workers) as ex:
+        list(ex.map(lambda r: fn(*r), ranges))
+
+
+def _features(c, n, F, lognormal_cols=(), chunk=1 << 18):
     X = np.empty((n, F), np.float32)
-    for r0 in range(0, n, chunk):
-        r1 = min(n, r0 + chunk)
+
+    def fill(r0, r1):
         z = normals(1000 + c, r0 * F, (r1 - r0) * F).reshape(r1 - r0, F)
         for j in lognormal_cols:
             z[:, j] = np.exp(z[:, j])
         X[r0:r1] = z.astype(np.float32)
+
+    _chunks(n, chunk, fill)
     return X
 
 
-def _binary_labels(c, X, flip_rate=0.0, chunk=1 << 20):
+def _binary_labels(c, X, flip_rate=0.0, chunk=1 << 18):
     n, F = X.shape
     w = normals(2000 + c, 0, F)
     y = np.empty(n, np.float32)
-    for r0 in range(0, n, chunk):
-        r1 = min(n, r0 + chunk)
+
+    def fill(r0, r1):
         x = X[r0:r1].astype(np.float64)
         z = x @ w + 0.5 * x[:, 0] * x[:, 1]
         p = 1.0 / (1.0 + np.exp(-z))
@@ -64,6 +87,8 @@ def _binary_labels(c, X, flip_rate=0.0, chunk=1 << 20):
             f = uniforms(4000 + c, r0, r1 - r0) < flip_rate
             lab = lab ^ f
         y[r0:r1] = lab.astype(np.float32)
+
+    _chunks(n, chunk, fill)
     return y
 
 
@@ -75,6 +100,42 @@ def binary_100k(n=100_000):
 def binary_10p5m(n=10_500_000):
     X = _features(2, n, 28, lognormal_cols=range(20, 28))
     return X, _binary_labels(2, X, flip_rate=0.05)
+
+
+def multiclass_2m(n=2_000_000, K=10, chunk=1 << 16):
+    """The batched multiclass workload. Columns 80-91 hold floor(U k_j) with
+    k_j = 2 + (j mod 15); columns 92-99 are categories 0-49 drawn from
+    Zipf(1.2) by inverse CDF (pass them as categorical_feature). Labels are
+    argmax_c(x.W_c + E_j[cat_j]_c + Gumbel) with W ~ N(0,1)[92 x K] and one
+    N(0,1)[50 x K] embedding per categorical column."""
+    c, F, nnum, nlev, ncat, ncats = 4, 100, 80, 12, 8, 50
+    X = np.empty((n, F), np.float32)
+    pz = 1.0 / np.arange(1, ncats + 1) ** 1.2
+    cdf = np.cumsum(pz / pz.sum())
+    W = normals(2000 + c, 0, (nnum + nlev) * K).reshape(nnum + nlev, K)
+    E = normals(2100 + c, 0, ncat * ncats * K).reshape(ncat, ncats, K)
+    y = np.empty(n, np.float32)
+
+    def fill(r0, r1):
+        m = r1 - r0
+        X[r0:r1, :nnum] = normals(1000 + c, r0 * nnum, m * nnum).reshape(m, nnum)
+        u = uniforms(1100 + c, r0 * nlev, m * nlev).reshape(m, nlev)
+        lev = 2 + (np.arange(nlev) % 15)
+        X[r0:r1, nnum:nnum + nlev] = np.floor(u * lev)
+        u = uniforms(1200 + c, r0 * ncat, m * ncat).reshape(m, ncat)
+        cats = np.minimum(np.searchsorted(cdf, u, side="right"), ncats - 1)
+        X[r0:r1, nnum + nlev:] = cats
+        z = X[r0:r1, :nnum + nlev].astype(np.float64) @ W
+        for j in range(ncat):
+            z += E[j][cats[:, j]]
+        g = -np.log(-np.log(np.maximum(uniforms(3000 + c, r0 * K, m * K).reshape(m, K), 1e-300)))
+        y[r0:r1] = np.argmax(z + g, axis=1).astype(np.float32)
+
+    _chunks(n, chunk, fill)
+    return X, y
+
+
+MULTICLASS_CATEGORICAL = list(range(92, 100))
 
 
 def rank_2m(n_queries=20_000, max_size=2000, F=136):
