@@ -170,6 +170,44 @@ struct DevBuf {
   ~DevBuf() { free(); }
 };
 
+// The growth state of one tree in flight: its stream, the ordered buffers
+// 1-2 and every class's root gradients (buffer 0; buffer 0's bins are the
+// context's), the histogram units, the leaf slots and the round's plan. Lane
+// l grows the trees of classes l, l + NL, l + 2 NL, ... of an iteration, so
+// up to NL trees of one iteration grow at the same time (their gradients are
+// all computed before any tree, boosting.rs:988-1001).
+struct Lane {
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  hipEvent_t done_ev = nullptr;
+  DevBuf<uint8_t> gh[3];
+  DevBuf<uint8_t> grp[3];      // [0] unused: buffer 0's bins are Ctx::grp0
+  DevBuf<NodeDev> nodes;       // the lane's last tree's nodes
+  DevBuf<int> leaf_shift;      // [L][2] renewal fixed-point shifts
+  DevBuf<long long> acc;       // each item's histogram summed over its chunks, one unit per group
+  DevBuf<int> acc_units;       // {0, G, 2G, ...}: acc read as one unit per (item, group)
+  DevBuf<long long> slab;      // histogram units of one histogram pass
+  DevBuf<int> units;           // [kMaxBatch + 1] first unit of each item
+  DevBuf<double> pool;
+  DevBuf<long long> ipool;
+  DevBuf<SplitRec> feat_split, best, log;
+  DevBuf<uint8_t> spl;
+  DevBuf<StepDev> st;
+  DevBuf<PlanDev> plan;
+  DevBuf<PlanRep> reps;
+  DevBuf<PosRec> pos;
+  DevBuf<LeafDev> leaves;
+  DevBuf<long long> renew;
+  ~Lane() {
+    if (done_ev) (void)hipEventDestroy(done_ev);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// Trees of one iteration grown at the same time, at most: the device's
+// default number of hardware queues per process.
+constexpr int kMaxLanes = 4;
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -200,30 +238,17 @@ struct Ctx {
   DevBuf<double> score;
   DevBuf<float> g, h;
   // three buffers of (gradients, group-major bins) in leaf order: buffer 0
-  // holds the root in row order, leaves alternate between buffers 1 and 2
-  DevBuf<uint8_t> gh[3];
+  // holds the root in row order, leaves alternate between buffers 1 and 2.
+  // Buffer 0's bins never change and are shared by the lanes.
+  DevBuf<uint8_t> grp0;
   DevBuf<unsigned> rows_buf;   // stage entry points: caller row lists
-  DevBuf<NodeDev> nodes;       // the last tree's nodes
-  DevBuf<int> leaf_shift;      // [L][2] renewal fixed-point shifts
-  DevBuf<uint8_t> grp[3];
-  DevBuf<long long> acc;     // each item's histogram summed over its chunks, one unit per group
-  DevBuf<int> acc_units;     // {0, G, 2G, ...}: acc read as one unit per (item, group)
-  DevBuf<long long> slab;    // histogram units of one histogram pass
-  DevBuf<int> units;         // [kMaxBatch + 1] first unit of each item
-  DevBuf<double> pool;
-  DevBuf<long long> ipool;
-  // growth state: leaf slots (MS of them), positions, the round's plan
+  std::vector<std::unique_ptr<Lane>> lanes;
+  int NL = 1;
+  hipEvent_t grad_ev = nullptr;
+  // growth state: leaf slots (MS of them) per lane
   int MS = 0;
-  DevBuf<SplitRec> feat_split, best, log;
-  DevBuf<uint8_t> spl;
-  DevBuf<StepDev> st;
-  DevBuf<PlanDev> plan;
-  DevBuf<PlanRep> reps;
-  DevBuf<PosRec> pos;
-  DevBuf<LeafDev> leaves;
   DevBuf<IterDev> it;
   DevBuf<double> leafval, leafraw;
-  DevBuf<long long> renew;
   DevBuf<double> grv, hrv;
   DevBuf<int> starts;
   DevBuf<int> call_ctr;
@@ -269,6 +294,8 @@ struct Ctx {
                                  // rows partitioned, splits computed, splits used
 
   ~Ctx() {
+    lanes.clear();
+    if (grad_ev) (void)hipEventDestroy(grad_ev);
     for (auto e : pev) if (e) (void)hipEventDestroy(e);
     for (auto g : graphs) if (g) (void)hipGraphExecDestroy(g);
     for (auto g : graphs_cont) if (g) (void)hipGraphExecDestroy(g);
@@ -282,6 +309,9 @@ struct Ctx {
 };
 
 static void set_device(Ctx* c) { HIP_CHECK(hipSetDevice(c->device)); }
+
+// The lane that grows class k's trees.
+static Lane& lane_of(Ctx* c, int k) { return *c->lanes[(size_t)(k % c->NL)]; }
 
 static int grid_for(int64_t n, int threads, int cap) {
   int64_t g = (n + threads - 1) / threads;
@@ -458,38 +488,54 @@ static void setup_training(Ctx* c) {
   c->score.zero(c->stream);
   c->g.alloc((size_t)K * N);
   c->h.alloc((size_t)K * N);
-  for (int b = 0; b < 3; ++b) c->gh[b].alloc((size_t)N * (quant ? 2 : 8));
-  c->nodes.alloc(L);
-  c->leaf_shift.alloc((size_t)2 * L);
   set_walk_lds_limit(L);
   // leaf slots: at most L - 1 replayed splits plus at most L splits made
   // ahead of order and never replayed, two slots each, plus the root
   c->MS = 4 * L;
   const int MS = c->MS;
-  c->units.alloc(kMaxBatch + 2);
-  c->units.zero(c->stream);
-  c->acc_units.alloc(kMaxBatch + 1);
-  if (quant) { c->ipool.alloc((size_t)MS * F * B); c->pool.free(); }
-  else { c->pool.alloc((size_t)MS * F * B * 2); c->ipool.free(); }
-  c->feat_split.alloc((size_t)2 * kMaxBatch * F);
-  c->best.alloc(MS);
-  c->log.alloc(L);
-  c->spl.alloc((size_t)MS * F);
-  c->st.alloc(1);
-  c->st.zero(c->stream);
-  c->plan.alloc(1);
-  c->plan.zero(c->stream);
-  c->reps.alloc(L);
-  c->pos.alloc((size_t)2 * L);
-  c->leaves.alloc(MS);
-  c->leaves.zero(c->stream);
+  // lanes: lane 0 runs on the context's stream; more lanes get streams of
+  // their own (default priority)
+  c->NL = std::min(K, kMaxLanes);
+  c->lanes.clear();
+  for (int l = 0; l < c->NL; ++l) {
+    c->lanes.emplace_back(new Lane());
+    Lane& ln = *c->lanes.back();
+    if (l == 0) {
+      ln.stream = c->stream;
+    } else {
+      HIP_CHECK(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
+      ln.own_stream = true;
+    }
+    HIP_CHECK(hipEventCreateWithFlags(&ln.done_ev, hipEventDisableTiming));
+    for (int b = 0; b < 3; ++b) ln.gh[b].alloc((size_t)N * (quant ? 2 : 8));
+    ln.nodes.alloc(L);
+    ln.leaf_shift.alloc((size_t)2 * L);
+    ln.units.alloc(kMaxBatch + 2);
+    ln.units.zero(c->stream);
+    ln.acc_units.alloc(kMaxBatch + 1);
+    if (quant) ln.ipool.alloc((size_t)MS * F * B);
+    else ln.pool.alloc((size_t)MS * F * B * 2);
+    ln.feat_split.alloc((size_t)2 * kMaxBatch * F);
+    ln.best.alloc(MS);
+    ln.log.alloc(L);
+    ln.spl.alloc((size_t)MS * F);
+    ln.st.alloc(1);
+    ln.st.zero(c->stream);
+    ln.plan.alloc(1);
+    ln.plan.zero(c->stream);
+    ln.reps.alloc(L);
+    ln.pos.alloc((size_t)2 * L);
+    ln.leaves.alloc(MS);
+    ln.leaves.zero(c->stream);
+    ln.renew.alloc((size_t)2 * L);
+    ln.renew.zero(c->stream);
+  }
+  if (!c->grad_ev) HIP_CHECK(hipEventCreateWithFlags(&c->grad_ev, hipEventDisableTiming));
   c->it.alloc(K);
   c->it.zero(c->stream);
   c->leafval.alloc((size_t)K * L);
   c->leafraw.alloc((size_t)K * L);
   set_part_lds_limit(part_lds(L));
-  c->renew.alloc((size_t)2 * L);
-  c->renew.zero(c->stream);
   // defaults from sweeps on 10.5M x 28 (quantized and FP64 forms)
   c->part_grid = (int)std::min<int64_t>((N + part_tile() - 1) / part_tile(),
                                         c->tune_part_grid > 0 ? c->tune_part_grid : 4 * c->cus);
@@ -526,24 +572,24 @@ static void setup_training(Ctx* c) {
     c->n_calls = 0;
   }
   plan_hist(c, quant);
-  c->acc.alloc((size_t)kMaxBatch * c->hist_ng * c->hist_fb * B * (quant ? 1 : 2));
   {
     std::vector<int> au(kMaxBatch + 1);
     for (int w = 0; w <= kMaxBatch; ++w) au[w] = w * c->hist_ng;
-    HIP_CHECK(hipMemcpy(c->acc_units.p, au.data(), au.size() * sizeof(int), hipMemcpyHostToDevice));
-  }
-  {
     // histogram units of one pass: at most (rows / chunk + splits) per group
     const int64_t max_units = ((N + c->hist_min_rows - 1) / c->hist_min_rows + kMaxBatch + 1) * c->hist_ng;
-    c->slab.alloc((size_t)max_units * c->hist_fb * B * (quant ? 1 : 2));
-  }
-  // buffer 0: the root's rows in row order, bins group-major
-  {
     const size_t gbytes = (size_t)c->hist_ng * N * c->hist_fb;
-    for (int b = 0; b < 3; ++b) c->grp[b].alloc(gbytes);
+    for (auto& lp : c->lanes) {
+      Lane& ln = *lp;
+      ln.acc.alloc((size_t)kMaxBatch * c->hist_ng * c->hist_fb * B * (quant ? 1 : 2));
+      HIP_CHECK(hipMemcpy(ln.acc_units.p, au.data(), au.size() * sizeof(int), hipMemcpyHostToDevice));
+      ln.slab.alloc((size_t)max_units * c->hist_fb * B * (quant ? 1 : 2));
+      for (int b = 1; b < 3; ++b) ln.grp[b].alloc(gbytes);
+    }
+    // buffer 0: the root's rows in row order, bins group-major
+    c->grp0.alloc(gbytes);
     hipLaunchKernelGGL(k_group_rows, dim3(grid_for(N * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
                        c->stream, (const unsigned*)nullptr, (long long)N, c->bins_rm.p, c->S, c->F, c->hist_fb,
-                       c->hist_ng, (long long)N, c->grp[0].p);
+                       c->hist_ng, (long long)N, c->grp0.p);
     HIP_CHECK(hipGetLastError());
   }
   for (auto p : c->h_log) if (p) (void)hipHostFree(p);
@@ -569,18 +615,18 @@ static void setup_training(Ctx* c) {
   c->state_ready = true;
 }
 
-static HistArgs hist_args(Ctx* c, bool quant, int k, int mode) {
+static HistArgs hist_args(Ctx* c, Lane& ln, bool quant, int k, int mode) {
   HistArgs a{};
   for (int b = 0; b < 3; ++b) {
-    a.grp[b] = c->grp[b].p;
-    a.gh[b] = c->gh[b].p;
+    a.grp[b] = b == 0 ? c->grp0.p : ln.grp[b].p;
+    a.gh[b] = ln.gh[b].p;
   }
-  a.st = c->st.p;
-  a.plan = c->plan.p;
-  a.reps = c->reps.p;
-  a.leaves = c->leaves.p;
-  a.log = c->log.p;
-  a.best = c->best.p;
+  a.st = ln.st.p;
+  a.plan = ln.plan.p;
+  a.reps = ln.reps.p;
+  a.leaves = ln.leaves.p;
+  a.log = ln.log.p;
+  a.best = ln.best.p;
   a.feats = c->d_feats.p;
   a.F = c->F;
   a.B = c->B;
@@ -590,8 +636,8 @@ static HistArgs hist_args(Ctx* c, bool quant, int k, int mode) {
   a.N = c->N;
   a.max_chunks = c->hist_chunks;
   a.min_rows_per_chunk = c->hist_min_rows;
-  a.slab = c->slab.p;
-  a.units = c->units.p;
+  a.slab = ln.slab.p;
+  a.units = ln.units.p;
   a.it = c->it.p + k;
   a.p = c->sp;
   a.quant = quant ? 1 : 0;
@@ -599,43 +645,43 @@ static HistArgs hist_args(Ctx* c, bool quant, int k, int mode) {
   return a;
 }
 
-static SplitArgs split_args(Ctx* c, int k, bool root) {
+static SplitArgs split_args(Ctx* c, Lane& ln, int k, bool root) {
   SplitArgs a{};
-  a.st = c->st.p;
-  a.plan = c->plan.p;
-  a.leaves = c->leaves.p;
+  a.st = ln.st.p;
+  a.plan = ln.plan.p;
+  a.leaves = ln.leaves.p;
   a.feats = c->d_feats.p;
   a.p = c->sp;
   a.F = c->F;
   a.B = c->B;
   a.L = c->L;
-  a.slab = c->acc.p;          // summed by the reduction after each histogram pass
-  a.units = c->acc_units.p;
+  a.slab = ln.acc.p;          // summed by the reduction after each histogram pass
+  a.units = ln.acc_units.p;
   a.FB = c->hist_fb;
   a.NG = c->hist_ng;
-  a.pool = c->pool.p;
-  a.ipool = c->ipool.p;
-  a.feat_split = c->feat_split.p;
-  a.splittable = c->spl.p;
+  a.pool = ln.pool.p;
+  a.ipool = ln.ipool.p;
+  a.feat_split = ln.feat_split.p;
+  a.splittable = ln.spl.p;
   a.it = c->it.p + k;
   a.quant = c->opts.use_quantized_grad != 0;
   a.root = root ? 1 : 0;
   return a;
 }
 
-static PartArgs part_args(Ctx* c) {
+static PartArgs part_args(Ctx* c, Lane& ln) {
   PartArgs a{};
-  a.st = c->st.p;
-  a.plan = c->plan.p;
-  a.reps = c->reps.p;
-  a.pos = c->pos.p;
-  a.leaves = c->leaves.p;
+  a.st = ln.st.p;
+  a.plan = ln.plan.p;
+  a.reps = ln.reps.p;
+  a.pos = ln.pos.p;
+  a.leaves = ln.leaves.p;
   a.feats = c->d_feats.p;
-  a.best = c->best.p;
-  a.feat_split = c->feat_split.p;
+  a.best = ln.best.p;
+  a.feat_split = ln.feat_split.p;
   for (int b = 0; b < 3; ++b) {
-    a.grp[b] = c->grp[b].p;
-    a.gh[b] = c->gh[b].p;
+    a.grp[b] = b == 0 ? c->grp0.p : ln.grp[b].p;
+    a.gh[b] = ln.gh[b].p;
   }
   a.FB = c->hist_fb;
   a.NG = c->hist_ng;
@@ -654,56 +700,57 @@ static hipEvent_t pevent(Ctx* c, int k, int step, int kind, int end) {
   return c->pev[i];
 }
 static void prof_mark(Ctx* c, int k, int step, int kind, int end) {
-  if (c->profile && step < prof_steps(c)) HIP_CHECK(hipEventRecord(pevent(c, k, step, kind, end), c->stream));
+  if (c->profile && step < prof_steps(c)) HIP_CHECK(hipEventRecord(pevent(c, k, step, kind, end), lane_of(c, k).stream));
 }
 
 // Every item's histogram units of the last histogram pass summed into acc
 // (one unit per item and feature group).
-static void slab_total(Ctx* c, bool quant, hipStream_t s) {
+static void slab_total(Ctx* c, Lane& ln, bool quant, hipStream_t s) {
   const int per_unit = c->hist_fb * c->B * (quant ? 1 : 2);
   const int64_t eblocks = ((int64_t)c->hist_ng * per_unit + 31) / 32;
   const int grid = (int)std::min<int64_t>(eblocks * kMaxBatch, 4LL * c->cus);
-  hipLaunchKernelGGL(k_slab_reduce, dim3(grid), dim3(256), 0, s, c->slab.p, c->units.p, c->hist_ng, per_unit,
-                     c->acc.p);
+  hipLaunchKernelGGL(k_slab_reduce, dim3(grid), dim3(256), 0, s, ln.slab.p, ln.units.p, c->hist_ng, per_unit,
+                     ln.acc.p);
 }
 
 // The first launches of a tree for class k (gradients already in g/h and
 // max |g|, |h| in it[k]): identity partition or quantization with the root
 // sums, the growth state, the root's histogram and search.
-static void enqueue_head(Ctx* c, int k) {
-  hipStream_t s = c->stream;
+static void enqueue_head(Ctx* c, Lane& ln, int k) {
+  hipStream_t s = ln.stream;
   const int64_t N = c->N;
   const bool quant = c->opts.use_quantized_grad != 0;
   const float* gk = c->g.p + (size_t)k * N;
   const float* hk = c->h.p + (size_t)k * N;
   const int ig = grid_for(N, 256, c->cus * 4);
   if (quant) {
+    // the k-th discretizer call of the iteration (the counter moves on by
+    // the iteration's trees once they are all grown)
     hipLaunchKernelGGL(k_discretize, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, c->grv.p, c->hrv.p,
-                       c->opts.num_grad_quant_bins, c->opts.stochastic_rounding, 0, c->starts.p, c->call_ctr.p,
-                       c->it.p + k, (unsigned*)nullptr, (char2*)c->gh[0].p, (int8_t*)nullptr, (int8_t*)nullptr);
-    hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, c->call_ctr.p);
+                       c->opts.num_grad_quant_bins, c->opts.stochastic_rounding, 0, c->starts.p, c->call_ctr.p, k,
+                       c->it.p + k, (unsigned*)nullptr, (char2*)ln.gh[0].p, (int8_t*)nullptr, (int8_t*)nullptr);
   } else {
     hipLaunchKernelGGL(k_tree_init, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, (unsigned*)nullptr,
-                       (float2*)c->gh[0].p, c->it.p + k);
+                       (float2*)ln.gh[0].p, c->it.p + k);
   }
-  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(256), 0, s, c->st.p, c->pos.p, c->leaves.p, c->spl.p, (long long)N,
+  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(256), 0, s, ln.st.p, ln.pos.p, ln.leaves.p, ln.spl.p, (long long)N,
                      c->F);
   const int hgrid = c->hist_chunks * c->hist_ng;
   prof_mark(c, k, 0, 0, 0);
-  launch_hist(hist_args(c, quant, k, 2), quant, c->hist_tpr, hgrid, c->hist_lds, s);
-  slab_total(c, quant, s);
+  launch_hist(hist_args(c, ln, quant, k, 2), quant, c->hist_tpr, hgrid, c->hist_lds, s);
+  slab_total(c, ln, quant, s);
   prof_mark(c, k, 0, 0, 1);
-  launch_split(split_args(c, k, true), s);
+  launch_split(split_args(c, ln, k, true), s);
   prof_mark(c, k, 0, 1, 1);
 }
 
 // Growth rounds r0 .. r0 + n - 1 (a round after the tree is done returns at once).
-static void enqueue_rounds(Ctx* c, int k, int r0, int n) {
-  hipStream_t s = c->stream;
+static void enqueue_rounds(Ctx* c, Lane& ln, int k, int r0, int n) {
+  hipStream_t s = ln.stream;
   const bool quant = c->opts.use_quantized_grad != 0;
-  const HistArgs ha = hist_args(c, quant, k, 3);
-  const SplitArgs sa = split_args(c, k, false);
-  const PartArgs pa = part_args(c);
+  const HistArgs ha = hist_args(c, ln, quant, k, 3);
+  const SplitArgs sa = split_args(c, ln, k, false);
+  const PartArgs pa = part_args(c, ln);
   const int hgrid = c->hist_chunks * c->hist_ng;
   for (int r = r0; r < r0 + n; ++r) {
     const int step = r + 1;
@@ -711,7 +758,7 @@ static void enqueue_rounds(Ctx* c, int k, int r0, int n) {
     launch_part(pa, c->part_grid, s);
     prof_mark(c, k, step, 2, 1);
     launch_hist(ha, quant, c->hist_tpr, hgrid, c->hist_lds, s);
-    slab_total(c, quant, s);
+    slab_total(c, ln, quant, s);
     prof_mark(c, k, step, 0, 1);
     launch_split(sa, s);
     prof_mark(c, k, step, 1, 1);
@@ -720,17 +767,17 @@ static void enqueue_rounds(Ctx* c, int k, int r0, int n) {
 
 // The tree's end: quantized leaf renewal, leaf values, score update. No-ops
 // while the tree is still growing.
-static WalkArgs walk_args(Ctx* c, int mode) {
+static WalkArgs walk_args(Ctx* c, Lane& ln, int mode) {
   WalkArgs w{};
-  w.st = c->st.p;
-  w.nodes = c->nodes.p;
+  w.st = ln.st.p;
+  w.nodes = ln.nodes.p;
   w.feats = c->d_feats.p;
   w.bins = c->bins_rm.p;
   w.S = c->S;
   w.L = c->L;
   w.N = c->N;
-  w.shifts = c->leaf_shift.p;
-  w.sums = c->renew.p;
+  w.shifts = ln.leaf_shift.p;
+  w.sums = ln.renew.p;
   w.mode = mode;
   return w;
 }
@@ -738,86 +785,128 @@ static WalkArgs walk_args(Ctx* c, int mode) {
 // The tree's end: its nodes, quantized leaf renewal, leaf values, score
 // update (the last two walk the tree over every row's bins in row order).
 // No-ops while the tree is still growing.
-static void enqueue_tail(Ctx* c, int k, bool score_update, double lr) {
-  hipStream_t s = c->stream;
+static void enqueue_tail(Ctx* c, Lane& ln, int k, bool score_update, double lr) {
+  hipStream_t s = ln.stream;
   const int64_t N = c->N;
   const bool quant = c->opts.use_quantized_grad != 0;
   const bool renew = quant && c->opts.quant_train_renew_leaf;
   const int wgrid = grid_for(N, 256, c->cus * 4);
-  hipLaunchKernelGGL(k_tree_nodes, dim3(1), dim3(256), 0, s, c->st.p, c->log.p, c->pos.p, c->leaves.p, c->it.p + k,
-                     c->L, c->nodes.p, c->leaf_shift.p);
+  hipLaunchKernelGGL(k_tree_nodes, dim3(1), dim3(256), 0, s, ln.st.p, ln.log.p, ln.pos.p, ln.leaves.p, c->it.p + k,
+                     c->L, ln.nodes.p, ln.leaf_shift.p);
   if (renew) {
-    WalkArgs w = walk_args(c, 0);
+    WalkArgs w = walk_args(c, ln, 0);
     w.g = c->g.p + (size_t)k * N;
     w.h = c->h.p + (size_t)k * N;
     launch_walk(w, wgrid, s);
   }
-  hipLaunchKernelGGL(k_leaf_values, dim3((c->L + 255) / 256), dim3(256), 0, s, c->st.p, c->pos.p, c->leaves.p,
-                     c->it.p + k, c->renew.p, renew ? 1 : 0, c->sp, lr, c->L, c->leafval.p + (size_t)k * c->L,
+  hipLaunchKernelGGL(k_leaf_values, dim3((c->L + 255) / 256), dim3(256), 0, s, ln.st.p, ln.pos.p, ln.leaves.p,
+                     c->it.p + k, ln.renew.p, renew ? 1 : 0, c->sp, lr, c->L, c->leafval.p + (size_t)k * c->L,
                      c->leafraw.p + (size_t)k * c->L);
   if (score_update) {
-    WalkArgs w = walk_args(c, 1);
+    WalkArgs w = walk_args(c, ln, 1);
     w.val = c->leafval.p + (size_t)k * c->L;
     w.score = c->score.p + (size_t)k * N;
     launch_walk(w, wgrid, s);
   }
 }
 
-static hipGraphExec_t capture(Ctx* c, const std::function<void()>& body) {
+static hipGraphExec_t capture(hipStream_t s, const std::function<void()>& body) {
   hipGraph_t g;
-  HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   body();
-  HIP_CHECK(hipStreamEndCapture(c->stream, &g));
+  HIP_CHECK(hipStreamEndCapture(s, &g));
   hipGraphExec_t e = nullptr;
   HIP_CHECK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
   (void)hipGraphDestroy(g);
   return e;
 }
 
-// One tree for class k, to completion: the main launch (graph or stream),
-// then continuation rounds while the device says the tree still grows.
-// Leaves the tree's log, raw leaf outputs and state in the pinned copies.
-static void run_tree(Ctx* c, int k, bool score_update, double lr) {
+// Class k's tree on its lane: the main launch (graph or stream: init, root,
+// rounds_main rounds, tail), then a copy of the growth state for the host.
+static void launch_main(Ctx* c, int k, bool score_update, double lr) {
+  Lane& ln = lane_of(c, k);
   const bool graph = c->opts.use_graph && !c->profile;
   if (graph) {
     if (!c->graphs[k])
-      c->graphs[k] = capture(c, [&] {
-        enqueue_head(c, k);
-        enqueue_rounds(c, k, 0, c->rounds_main);
-        enqueue_tail(c, k, score_update, lr);
+      c->graphs[k] = capture(ln.stream, [&] {
+        enqueue_head(c, ln, k);
+        enqueue_rounds(c, ln, k, 0, c->rounds_main);
+        enqueue_tail(c, ln, k, score_update, lr);
       });
-    HIP_CHECK(hipGraphLaunch(c->graphs[k], c->stream));
+    HIP_CHECK(hipGraphLaunch(c->graphs[k], ln.stream));
   } else {
-    enqueue_head(c, k);
-    enqueue_rounds(c, k, 0, c->rounds_main);
-    enqueue_tail(c, k, score_update, lr);
+    enqueue_head(c, ln, k);
+    enqueue_rounds(c, ln, k, 0, c->rounds_main);
+    enqueue_tail(c, ln, k, score_update, lr);
   }
   HIP_CHECK(hipGetLastError());
-  int launched = c->rounds_main;
-  for (;;) {
-    HIP_CHECK(hipMemcpyAsync(c->h_st[k], c->st.p, sizeof(StepDev), hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(hipStreamSynchronize(c->stream));
-    if (c->h_st[k]->done) break;
-    // every round replays or splits at least one leaf: L + 1 rounds finish any tree
-    if (launched > c->L + 1) throw Error(LGBMHIP_INTERNAL, "tree growth did not finish");
-    if (graph) {
-      if (!c->graphs_cont[k])
-        c->graphs_cont[k] = capture(c, [&] {
-          enqueue_rounds(c, k, 0, c->rounds_cont);
-          enqueue_tail(c, k, score_update, lr);
-        });
-      HIP_CHECK(hipGraphLaunch(c->graphs_cont[k], c->stream));
-    } else {
-      enqueue_rounds(c, k, launched, c->rounds_cont);
-      enqueue_tail(c, k, score_update, lr);
-    }
-    HIP_CHECK(hipGetLastError());
-    launched += c->rounds_cont;
+  HIP_CHECK(hipMemcpyAsync(c->h_st[k], ln.st.p, sizeof(StepDev), hipMemcpyDeviceToHost, ln.stream));
+}
+
+// rounds_cont more rounds of class k's tree and its tail again.
+static void launch_cont(Ctx* c, int k, int launched, bool score_update, double lr) {
+  Lane& ln = lane_of(c, k);
+  const bool graph = c->opts.use_graph && !c->profile;
+  if (graph) {
+    if (!c->graphs_cont[k])
+      c->graphs_cont[k] = capture(ln.stream, [&] {
+        enqueue_rounds(c, ln, k, 0, c->rounds_cont);
+        enqueue_tail(c, ln, k, score_update, lr);
+      });
+    HIP_CHECK(hipGraphLaunch(c->graphs_cont[k], ln.stream));
+  } else {
+    enqueue_rounds(c, ln, k, launched, c->rounds_cont);
+    enqueue_tail(c, ln, k, score_update, lr);
   }
-  c->last_rounds[k] = launched;
-  HIP_CHECK(hipMemcpyAsync(c->h_log[k], c->log.p, sizeof(SplitRec) * c->L, hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->h_raw[k], c->leafraw.p + (size_t)k * c->L, sizeof(double) * c->L,
-                           hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(c->h_st[k], ln.st.p, sizeof(StepDev), hipMemcpyDeviceToHost, ln.stream));
+}
+
+// The trees of classes 0 .. ntrees-1 (gradients already enqueued on the
+// context's stream), each to completion: lane l grows classes l, l + NL,
+// ... in order while the other lanes grow theirs; the host tops up a tree
+// with continuation rounds while its device state says it still grows.
+// Leaves each tree's log, raw leaf outputs and state in the pinned copies,
+// and the context's stream ordered after every lane's last tree.
+static void grow_trees(Ctx* c, int ntrees, bool score_update, double lr) {
+  const int NL = std::min(ntrees, c->NL);
+  HIP_CHECK(hipEventRecord(c->grad_ev, c->stream));
+  for (int l = 1; l < NL; ++l) HIP_CHECK(hipStreamWaitEvent(c->lanes[l]->stream, c->grad_ev, 0));
+  std::vector<int> cur(NL, -1), launched(NL, 0);
+  int active = 0;
+  auto start = [&](int l, int k) {
+    cur[l] = k;
+    launch_main(c, k, score_update, lr);
+    launched[l] = c->rounds_main;
+    active++;
+  };
+  for (int l = 0; l < NL; ++l) start(l, l);
+  while (active > 0) {
+    for (int l = 0; l < NL; ++l) {
+      const int k = cur[l];
+      if (k < 0) continue;
+      Lane& ln = *c->lanes[l];
+      HIP_CHECK(hipStreamSynchronize(ln.stream));
+      if (!c->h_st[k]->done) {
+        // every round replays or splits at least one leaf: L + 1 rounds finish any tree
+        if (launched[l] > c->L + 1) throw Error(LGBMHIP_INTERNAL, "tree growth did not finish");
+        launch_cont(c, k, launched[l], score_update, lr);
+        launched[l] += c->rounds_cont;
+        continue;
+      }
+      c->last_rounds[k] = launched[l];
+      HIP_CHECK(hipMemcpyAsync(c->h_log[k], ln.log.p, sizeof(SplitRec) * c->L, hipMemcpyDeviceToHost, ln.stream));
+      HIP_CHECK(hipMemcpyAsync(c->h_raw[k], c->leafraw.p + (size_t)k * c->L, sizeof(double) * c->L,
+                               hipMemcpyDeviceToHost, ln.stream));
+      cur[l] = -1;
+      active--;
+      if (k + NL < ntrees) start(l, k + NL);
+    }
+  }
+  for (int l = 1; l < NL; ++l) {
+    HIP_CHECK(hipEventRecord(c->lanes[l]->done_ev, c->lanes[l]->stream));
+    HIP_CHECK(hipStreamWaitEvent(c->stream, c->lanes[l]->done_ev, 0));
+  }
 }
 
 static HostTree build_tree(Ctx* c, int k) {
@@ -995,9 +1084,10 @@ static int train_iteration(Ctx* c) {
     throw Error(LGBMHIP_UNSUPPORTED, "more quantized trees than the start-offset table holds (65536)");
   HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   enqueue_gradients(c);
-  for (int k = 0; k < K; ++k) {
-    run_tree(c, k, true, c->opts.learning_rate);
-    if (c->opts.use_quantized_grad) c->n_calls++;
+  grow_trees(c, K, true, c->opts.learning_rate);
+  if (c->opts.use_quantized_grad) {
+    hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, c->stream, c->call_ctr.p, K);
+    c->n_calls += K;
   }
   HIP_CHECK(hipEventRecord(c->ev1, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1046,8 +1136,11 @@ static void train_tree(Ctx* c, const float* g, const float* h) {
                      (long long)N, c->it.p);
   if (c->opts.use_quantized_grad && c->n_calls + 1 > (int)c->starts.n)
     throw Error(LGBMHIP_UNSUPPORTED, "more quantized trees than the start-offset table holds (65536)");
-  run_tree(c, 0, false, 1.0);
-  if (c->opts.use_quantized_grad) c->n_calls++;
+  grow_trees(c, 1, false, 1.0);
+  if (c->opts.use_quantized_grad) {
+    hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, c->stream, c->call_ctr.p, 1);
+    c->n_calls++;
+  }
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->trees.push_back(build_tree(c, 0));
   c->pred_dirty = true;
@@ -1370,7 +1463,8 @@ lgbmhip_status lgbmhip_leaf_of_row(lgbmhip_ctx* ctx, int32_t* out) {
     set_device(c);
     DevBuf<int> d;
     d.alloc(c->N);
-    WalkArgs w = walk_args(c, 2);
+    // the last tree grown: class K - 1's
+    WalkArgs w = walk_args(c, lane_of(c, c->K - 1), 2);
     w.out = d.p;
     launch_walk(w, grid_for(c->N, 256, c->cus * 4), c->stream);
     HIP_CHECK(hipMemcpyAsync(out, d.p, c->N * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1398,9 +1492,9 @@ lgbmhip_status lgbmhip_quant_state(lgbmhip_ctx* ctx, int8_t* gq, int8_t* hq, dou
   return guarded(ctx, [&] {
     Ctx* c = &ctx->c;
     if (!c->state_ready || !c->opts.use_quantized_grad) throw Error(LGBMHIP_INVALID_PARAMETER, "no quantized state");
-    // buffer 0 holds the last tree's (gq, hq) pairs in row order
+    // buffer 0 of the last tree's lane holds its (gq, hq) pairs in row order
     std::vector<int8_t> q((size_t)2 * c->N);
-    HIP_CHECK(hipMemcpyAsync(q.data(), c->gh[0].p, q.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(q.data(), lane_of(c, c->K - 1).gh[0].p, q.size(), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     for (int64_t i = 0; i < c->N; ++i) {
       gq[i] = q[2 * i];
@@ -1445,9 +1539,9 @@ lgbmhip_status lgbmhip_discretize(lgbmhip_ctx* ctx, const float* grad, const flo
                          (long long)N, c->it.p);
       hipLaunchKernelGGL(k_discretize, dim3(grid_for(N, 256, c->cus * 4)), dim3(256), 0, c->stream, (long long)N,
                          c->g.p, c->h.p, c->grv.p, c->hrv.p, c->opts.num_grad_quant_bins, c->opts.stochastic_rounding,
-                         0, c->starts.p, c->call_ctr.p, c->it.p, (unsigned*)nullptr, (char2*)c->gh[0].p, c->gq.p,
-                         c->hq.p);
-      hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, c->stream, c->call_ctr.p);
+                         0, c->starts.p, c->call_ctr.p, 0, c->it.p, (unsigned*)nullptr, (char2*)c->lanes[0]->gh[0].p,
+                         c->gq.p, c->hq.p);
+      hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, c->stream, c->call_ctr.p, 1);
       c->n_calls++;
     }
     HIP_CHECK(hipMemcpyAsync(gq, c->gq.p, N, hipMemcpyDeviceToHost, c->stream));
@@ -1465,6 +1559,7 @@ static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc
   if (!c->state_ready) setup_training(c);
   set_device(c);
   if (quant != (c->opts.use_quantized_grad != 0)) throw Error(LGBMHIP_INVALID_PARAMETER, "histogram kind differs from train opts");
+  Lane& ln = *c->lanes[0];   // lane 0 runs on the context's stream
   const int64_t N = c->N;
   if (m < 0 || m > N) throw Error(LGBMHIP_INVALID_DATA, "row count out of range");
   for (int64_t i = 0; i < m; ++i)
@@ -1493,12 +1588,12 @@ static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc
     // the rows' bins gathered into positions 0..m-1 of ordered buffer 1
     c->rows_buf.alloc(m);
     HIP_CHECK(hipMemcpyAsync(c->rows_buf.p, rows, m * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(hipMemcpyAsync(c->gh[1].p, pk.data(), pk.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(ln.gh[1].p, pk.data(), pk.size(), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_group_rows, dim3(grid_for(m * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
                        c->stream, (const unsigned*)c->rows_buf.p, (long long)m, c->bins_rm.p, c->S, c->F, c->hist_fb,
-                       c->hist_ng, (long long)N, c->grp[1].p);
+                       c->hist_ng, (long long)N, ln.grp[1].p);
   }
-  HistArgs a = hist_args(c, quant, 0, 1);
+  HistArgs a = hist_args(c, ln, quant, 0, 1);
   a.d_begin = 0;
   a.d_count = m;
   a.d_buf = 1;
@@ -1506,12 +1601,12 @@ static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc
   a.d_shift_h = fix_shift(m, hbits);
   const int C = quant ? 1 : 2;
   const size_t cnt = (size_t)c->F * c->B * C;
-  std::vector<long long> acc(c->acc.n, 0);
+  std::vector<long long> acc(ln.acc.n, 0);
   if (m > 0) {
     launch_hist(a, quant, c->hist_tpr, c->hist_chunks * c->hist_ng, c->hist_lds, c->stream);
-    slab_total(c, quant, c->stream);
+    slab_total(c, ln, quant, c->stream);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(acc.data(), c->acc.p, c->acc.n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(acc.data(), ln.acc.p, ln.acc.n * 8, hipMemcpyDeviceToHost, c->stream));
   }
   HIP_CHECK(hipStreamSynchronize(c->stream));
   // acc is [group][feature in group][bin][C]: out is [feature][bin][C]

@@ -22,7 +22,10 @@
 namespace lgbmhip {
 
 constexpr int kPartThreads = 256;
-constexpr int kPartRowsPerThread = 4;
+#ifndef LGBMHIP_PART_ROWS
+#define LGBMHIP_PART_ROWS 4
+#endif
+constexpr int kPartRowsPerThread = LGBMHIP_PART_ROWS;
 constexpr int kPartTile = kPartThreads * kPartRowsPerThread;
 constexpr int kRowThreads = kPartThreads / (2 * kMaxBatch);   // threads reducing one searched row
 static_assert(kRowThreads >= 1, "a searched row needs at least one thread");

@@ -167,7 +167,7 @@ __device__ bool last_block(int* counter) {
 }
 
 // Advances the quantizer's call counter (one start offset per discretize call).
-__global__ void k_bump(int* p) { *p += 1; }
+__global__ void k_bump(int* p, int by) { *p += by; }
 
 // Clears the per-iteration reductions of K classes (a fill kernel, so the
 // iteration can be captured in a graph).
@@ -510,7 +510,8 @@ __global__ void __launch_bounds__(256) k_tree_init(long long n, const float* __r
 // are written when given (the stage entry point).
 __global__ void __launch_bounds__(256) k_discretize(long long n, const float* __restrict__ g, const float* __restrict__ h,
                              const double* __restrict__ grv, const double* __restrict__ hrv, int bins, int stochastic,
-                             int constant_hessian, const int* __restrict__ starts, const int* call_ctr, IterDev* it,
+                             int constant_hessian, const int* __restrict__ starts, const int* call_ctr, int call_off,
+                             IterDev* it,
                              unsigned* __restrict__ idx0, char2* __restrict__ q0, int8_t* __restrict__ gq_out,
                              int8_t* __restrict__ hq_out) {
   const double mg = (double)__uint_as_float(it->max_abs_g_bits);
@@ -518,7 +519,7 @@ __global__ void __launch_bounds__(256) k_discretize(long long n, const float* __
   const double gs = mg / (double)(bins / 2);
   const double hs = constant_hessian ? mh : mh / (double)bins;
   const double inv_g = 1.0 / gs, inv_h = 1.0 / hs;
-  const long long start = starts[*call_ctr];
+  const long long start = starts[*call_ctr + call_off];
   long long ai = 0, agq = 0, ahq = 0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const double gv = (double)g[i];
