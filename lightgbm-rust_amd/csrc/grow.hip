@@ -1,3 +1,4 @@
+#include <type_traits>
 // Tree growth on the device: histograms, split search, partition and the
 // round driver (included by engine.hip after kernels.hip).
 //
@@ -155,30 +156,38 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
   const int rpp = 256 / TPR;
   // select, not index: a runtime index into a kernel-argument array makes
   // the compiler copy the arguments to scratch memory
-  const uint8_t* __restrict__ gb =
-      sel3(a.grp[0], a.grp[1], a.grp[2], buf) + ((size_t)fg * a.N + begin) * a.FB + j * 4;
-  const void* ghb = sel3(a.gh[0], a.gh[1], a.gh[2], buf);
+  // lanes past the group's last feature read the row's first word (unused)
   const bool active = j * 4 < FBg;
+  const uint8_t* __restrict__ gb =
+      sel3(a.grp[0], a.grp[1], a.grp[2], buf) + ((size_t)fg * a.N + begin) * a.FB + (active ? j * 4 : 0);
+  using GT = typename std::conditional<Q, char2, float2>::type;
+  const GT* __restrict__ ghb = (const GT*)sel3(a.gh[0], a.gh[1], a.gh[2], buf) + begin;
   for (long long base = r0; base < r1; base += kHistUnroll * rpp) {
     unsigned w[kHistUnroll];
+    GT graw[kHistUnroll];
     long long gv[kHistUnroll], hv[kHistUnroll];
     bool ok[kHistUnroll];
+    // every slot loads (one past the chunk reloads its last row), all loads
+    // before any use, so a thread's loads are in flight together
 #pragma unroll
     for (int u = 0; u < kHistUnroll; ++u) {
       const long long p = base + u * rpp + sub;
       ok[u] = active && p < r1;
-      if (ok[u]) {
-        w[u] = *(const unsigned*)(gb + (size_t)p * a.FB);
-        if (Q) {
-          // multi_val_bin.rs:710-791: the row's packed (g << 32) | u8(h)
-          const char2 v = ((const char2*)ghb)[begin + p];
-          gv[u] = (long long)(((unsigned long long)(long long)v.x << 32) | (unsigned long long)(uint8_t)v.y);
-          hv[u] = 0;
-        } else {
-          const float2 v = ((const float2*)ghb)[begin + p];
-          gv[u] = to_fixed(v.x, sg);
-          hv[u] = to_fixed(v.y, sh);
-        }
+      const long long pc = p < r1 ? p : r1 - 1;
+      w[u] = *(const unsigned*)(gb + (size_t)pc * a.FB);
+      graw[u] = ghb[pc];
+    }
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      if (Q) {
+        // multi_val_bin.rs:710-791: the row's packed (g << 32) | u8(h)
+        const char2 v = *(const char2*)&graw[u];
+        gv[u] = (long long)(((unsigned long long)(long long)v.x << 32) | (unsigned long long)(uint8_t)v.y);
+        hv[u] = 0;
+      } else {
+        const float2 v = *(const float2*)&graw[u];
+        gv[u] = to_fixed(v.x, sg);
+        hv[u] = to_fixed(v.y, sh);
       }
     }
 #pragma unroll
@@ -1170,9 +1179,20 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
 // partition.rs:107-174). The row order inside a child is unspecified: every
 // consumer of a segment (exact fixed-point or integer histograms, renewal
 // sums, score update) is order-independent.
-template <int VW>
+template <int GB>
+struct GradVec;   // one row's gradients: char2 (quantized) or float2 (plain)
+template <>
+struct GradVec<2> { using T = unsigned short; };
+template <>
+struct GradVec<8> { using T = uint2; };
+
+// Every load of a tile is unconditional (slots past the segment load its
+// last row) and issued before any of the loaded values is used, so a
+// thread's loads are in flight together; stores follow, masked.
+template <int VW, int GB>
 __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
   using VT = typename RowVec<VW>::T;
+  using GT = typename GradVec<GB>::T;
   constexpr int VWW = VW / 4;   // words per vector
   StepDev* st = a.st;
   constexpr int NW = kPartThreads / 64;
@@ -1194,7 +1214,7 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
   const int k = s_hdr[1], total_tiles = s_hdr[2];
   if (k == 0) return;
   // ---- tiles
-  const int FB = a.FB, gbytes = a.gh_bytes;
+  const int FB = a.FB;
   const size_t gstride = (size_t)a.N * FB;   // bytes of one feature group's array
   const int words = FB / 4, tw = a.NG * words;   // 4-byte words of one row over all groups
   constexpr int R = kPartRowsPerThread;
@@ -1209,32 +1229,39 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
     const FeatDev& fd = items[w].fd;
     const long long begin = items[w].begin, count = items[w].count;
     const int src = items[w].buf, dst = child_buf(src);   // buffer 0 is never a destination
-    const uint8_t* igh = (const uint8_t*)sel3(a.gh[0], a.gh[1], a.gh[2], src) + (size_t)begin * gbytes;
-    uint8_t* ogh = (uint8_t*)sel3(a.gh[0], a.gh[1], a.gh[2], dst) + (size_t)begin * gbytes;
+    const GT* igh = (const GT*)sel3(a.gh[0], a.gh[1], a.gh[2], src) + begin;
+    GT* ogh = (GT*)sel3(a.gh[0], a.gh[1], a.gh[2], dst) + begin;
     const uint8_t* gsrc = sel3(a.grp[0], a.grp[1], a.grp[2], src) + (size_t)begin * FB;
     uint8_t* gdst = sel3(a.grp[0], a.grp[1], a.grp[2], dst) + (size_t)begin * FB;
     const uint8_t* scol = gsrc + (size_t)(s.feature / FB) * gstride + (s.feature % FB);
     const long long t0 = (long long)(t - items[w].tile0) * kPartTile;
     // one round of independent loads: gradients, the split feature's bin and
-    // the first words of every row (lanes take consecutive rows)
+    // the first words of every row (lanes take consecutive rows); a slot past
+    // the segment reloads the segment's last row, a word past the row its
+    // last word
     bool valid[R], left[R];
-    uint2 gv[R];
+    long long pc[R];
+    GT gv[R];
+    uint8_t bin[R];
     VT v[R][CW / VWW];
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       const long long p = t0 + u * kPartThreads + tid;
       valid[u] = p < count;
-      left[u] = false;
-      if (valid[u]) {
-        if (gbytes == 8) gv[u] = ((const uint2*)igh)[p];
-        else gv[u].x = ((const unsigned short*)igh)[p];
-        left[u] = goes_left(scol[(size_t)p * FB], fd, s);
+      pc[u] = valid[u] ? p : count - 1;
+    }
 #pragma unroll
-        for (int i = 0; i < CW; i += VWW)
-          if (i < tw)
-            v[u][i / VWW] = *(const VT*)(gsrc + (size_t)(i / words) * gstride + (size_t)p * FB + 4 * (i % words));
+    for (int u = 0; u < R; ++u) {
+      gv[u] = igh[pc[u]];
+      bin[u] = scol[(size_t)pc[u] * FB];
+#pragma unroll
+      for (int i = 0; i < CW; i += VWW) {
+        const int ii = min(i, tw - VWW);
+        v[u][i / VWW] = *(const VT*)(gsrc + (size_t)(ii / words) * gstride + (size_t)pc[u] * FB + 4 * (ii % words));
       }
     }
+#pragma unroll
+    for (int u = 0; u < R; ++u) left[u] = valid[u] && goes_left(bin[u], fd, s);
     // ranks inside each 64-row slice, slice counts to LDS
     int rk[R];
 #pragma unroll
@@ -1281,8 +1308,7 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
 #pragma unroll
     for (int u = 0; u < R; ++u) {
       if (!valid[u]) continue;
-      if (gbytes == 8) ((uint2*)ogh)[d[u]] = gv[u];
-      else ((unsigned short*)ogh)[d[u]] = (unsigned short)gv[u].x;
+      ogh[d[u]] = gv[u];
 #pragma unroll
       for (int i = 0; i < CW; i += VWW)
         if (i < tw)
@@ -1292,13 +1318,10 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
     for (int k0 = CW; k0 < tw; k0 += CW) {
 #pragma unroll
       for (int u = 0; u < R; ++u) {
-        if (!valid[u]) continue;
-        const long long p = t0 + u * kPartThreads + tid;
 #pragma unroll
         for (int i = 0; i < CW; i += VWW) {
-          const int kk = k0 + i;
-          if (kk < tw)
-            v[u][i / VWW] = *(const VT*)(gsrc + (size_t)(kk / words) * gstride + (size_t)p * FB + 4 * (kk % words));
+          const int kk = min(k0 + i, tw - VWW);
+          v[u][i / VWW] = *(const VT*)(gsrc + (size_t)(kk / words) * gstride + (size_t)pc[u] * FB + 4 * (kk % words));
         }
       }
 #pragma unroll
@@ -1712,9 +1735,15 @@ static void launch_part(const PartArgs& a, int grid, hipStream_t s) {
   if (a.L <= 256) hipLaunchKernelGGL(k_grow_plan<4>, dim3(1), dim3(kPartThreads), part_lds(a.L), s, a);
   else if (a.L <= 512) hipLaunchKernelGGL(k_grow_plan<8>, dim3(1), dim3(kPartThreads), part_lds(a.L), s, a);
   else hipLaunchKernelGGL(k_grow_plan<16>, dim3(1), dim3(kPartThreads), part_lds(a.L), s, a);
-  if (a.FB % 16 == 0) hipLaunchKernelGGL(k_grow_part<16>, dim3(grid), dim3(kPartThreads), 0, s, a);
-  else if (a.FB % 8 == 0) hipLaunchKernelGGL(k_grow_part<8>, dim3(grid), dim3(kPartThreads), 0, s, a);
-  else hipLaunchKernelGGL(k_grow_part<4>, dim3(grid), dim3(kPartThreads), 0, s, a);
+  if (a.gh_bytes == 2) {
+    if (a.FB % 16 == 0) hipLaunchKernelGGL((k_grow_part<16, 2>), dim3(grid), dim3(kPartThreads), 0, s, a);
+    else if (a.FB % 8 == 0) hipLaunchKernelGGL((k_grow_part<8, 2>), dim3(grid), dim3(kPartThreads), 0, s, a);
+    else hipLaunchKernelGGL((k_grow_part<4, 2>), dim3(grid), dim3(kPartThreads), 0, s, a);
+  } else {
+    if (a.FB % 16 == 0) hipLaunchKernelGGL((k_grow_part<16, 8>), dim3(grid), dim3(kPartThreads), 0, s, a);
+    else if (a.FB % 8 == 0) hipLaunchKernelGGL((k_grow_part<8, 8>), dim3(grid), dim3(kPartThreads), 0, s, a);
+    else hipLaunchKernelGGL((k_grow_part<4, 8>), dim3(grid), dim3(kPartThreads), 0, s, a);
+  }
 }
 
 static int part_tile() { return kPartTile; }
