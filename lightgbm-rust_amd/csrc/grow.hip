@@ -1503,7 +1503,7 @@ __host__ __device__ constexpr size_t walk_lds(int L) {
   return (size_t)L * (16 + 4 * kCatWords) + (size_t)L * (2 * 8 + 8 + 2 * 4);
 }
 
-template <int RW>
+template <int RW, int MODE>
 __global__ void __launch_bounds__(256) k_walk(WalkArgs a) {
   extern __shared__ unsigned long long walk_sm[];
   uint4* nq = (uint4*)walk_sm;                                      // [L]
@@ -1512,7 +1512,7 @@ __global__ void __launch_bounds__(256) k_walk(WalkArgs a) {
   unsigned* ncat = (unsigned*)(lval + a.L);                        // [L][kCatWords]
   int* lsh = (int*)(ncat + (size_t)a.L * kCatWords);               // [2 L]
   const StepDev* st = a.st;
-  if (a.mode != 2 && !st->done) return;
+  if (MODE != 2 && !st->done) return;
   const int ns = st->num_splits, nl = ns + 1;
   const int tid = threadIdx.x;
   for (int i = tid; i < ns; i += blockDim.x) {
@@ -1521,26 +1521,35 @@ __global__ void __launch_bounds__(256) k_walk(WalkArgs a) {
 #pragma unroll
     for (int w = 0; w < kCatWords; ++w) ncat[(size_t)i * kCatWords + w] = n.cat[w];
   }
-  if (a.mode == 0)
+  if (MODE == 0)
     for (int l = tid; l < nl; l += blockDim.x) {
       lsum[2 * l] = lsum[2 * l + 1] = 0ull;
       lsh[2 * l] = a.shifts[2 * l];
       lsh[2 * l + 1] = a.shifts[2 * l + 1];
     }
-  if (a.mode == 1)
+  if (MODE == 1)
     for (int l = tid; l < nl; l += blockDim.x) lval[l] = a.val[l];
   __syncthreads();
   constexpr int U = 4;   // rows walked together per thread
   const long long step = (long long)gridDim.x * blockDim.x * U;
   for (long long base = (long long)blockIdx.x * blockDim.x * U + tid; base < a.N; base += step) {
+    // every load first (a slot past the last row reloads the last row):
+    // the row's bins and what the row's leaf will be combined with
     unsigned words[U][RW > 0 ? RW : 1];
     int node[U];
+    long long rc[U];
+    double sc[U];
+    float gg[U], hh[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const long long r = base + (long long)u * blockDim.x;
       node[u] = (ns == 0 || r >= a.N) ? -1 : 0;   // ~0: leaf 0
-      if (RW > 0 && r < a.N) {
-        const uint8_t* row = a.bins + (size_t)r * a.S;
+      rc[u] = r < a.N ? r : a.N - 1;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (RW > 0) {
+        const uint8_t* row = a.bins + (size_t)rc[u] * a.S;
 #pragma unroll
         for (int i = 0; i < RW; i += 4) {
           if (i + 4 <= RW) {
@@ -1554,6 +1563,11 @@ __global__ void __launch_bounds__(256) k_walk(WalkArgs a) {
           }
         }
       }
+      if (MODE == 0) {
+        gg[u] = a.g[rc[u]];
+        hh[u] = a.h[rc[u]];
+      }
+      if (MODE == 1) sc[u] = a.score[rc[u]];
     }
     for (;;) {
       bool any = false;
@@ -1570,7 +1584,7 @@ __global__ void __launch_bounds__(256) k_walk(WalkArgs a) {
             for (int i = 1; i < RW; ++i) w = wi == i ? words[u][i] : w;
             b = (int)((w >> (8 * (f & 3))) & 0xFF);
           } else {
-            b = a.bins[(size_t)(base + (long long)u * blockDim.x) * a.S + f];
+            b = a.bins[(size_t)rc[u] * a.S + f];
           }
           node[u] = walk_step(q, b, ncat + (size_t)node[u] * kCatWords);
           any = true;
@@ -1583,17 +1597,17 @@ __global__ void __launch_bounds__(256) k_walk(WalkArgs a) {
       const long long r = base + (long long)u * blockDim.x;
       if (r >= a.N) continue;
       const int l = ~node[u];
-      if (a.mode == 0) {
-        atomicAdd(&lsum[2 * l], (unsigned long long)to_fixed(a.g[r], lsh[2 * l]));
-        atomicAdd(&lsum[2 * l + 1], (unsigned long long)to_fixed(a.h[r], lsh[2 * l + 1]));
-      } else if (a.mode == 1) {
-        a.score[r] += lval[l];
+      if (MODE == 0) {
+        atomicAdd(&lsum[2 * l], (unsigned long long)to_fixed(gg[u], lsh[2 * l]));
+        atomicAdd(&lsum[2 * l + 1], (unsigned long long)to_fixed(hh[u], lsh[2 * l + 1]));
+      } else if (MODE == 1) {
+        a.score[r] = sc[u] + lval[l];
       } else {
         a.out[r] = l;
       }
     }
   }
-  if (a.mode == 0) {
+  if (MODE == 0) {
     __syncthreads();
     for (int l = tid; l < nl; l += blockDim.x) {
       const unsigned long long vg = lsum[2 * l], vh = lsum[2 * l + 1];
@@ -1677,20 +1691,32 @@ __global__ void __launch_bounds__(256) k_slab_reduce(const long long* __restrict
 
 // ---------------------------------------------------------------- launchers
 // Rows of at most 32 bytes are walked from registers.
-static void launch_walk(const WalkArgs& a, int grid, hipStream_t s) {
+template <int MODE>
+static void launch_walk_m(const WalkArgs& a, int grid, hipStream_t s) {
   const size_t lds = walk_lds(a.L);
   switch (a.S) {
-    case 4: hipLaunchKernelGGL(k_walk<1>, dim3(grid), dim3(256), lds, s, a); break;
-    case 8: hipLaunchKernelGGL(k_walk<2>, dim3(grid), dim3(256), lds, s, a); break;
-    case 16: hipLaunchKernelGGL(k_walk<4>, dim3(grid), dim3(256), lds, s, a); break;
-    case 32: hipLaunchKernelGGL(k_walk<8>, dim3(grid), dim3(256), lds, s, a); break;
-    default: hipLaunchKernelGGL(k_walk<0>, dim3(grid), dim3(256), lds, s, a); break;
+    case 4: hipLaunchKernelGGL((k_walk<1, MODE>), dim3(grid), dim3(256), lds, s, a); break;
+    case 8: hipLaunchKernelGGL((k_walk<2, MODE>), dim3(grid), dim3(256), lds, s, a); break;
+    case 16: hipLaunchKernelGGL((k_walk<4, MODE>), dim3(grid), dim3(256), lds, s, a); break;
+    case 32: hipLaunchKernelGGL((k_walk<8, MODE>), dim3(grid), dim3(256), lds, s, a); break;
+    default: hipLaunchKernelGGL((k_walk<0, MODE>), dim3(grid), dim3(256), lds, s, a); break;
   }
 }
-static void set_walk_lds_limit(int L) {
-  const void* ks[] = {(const void*)k_walk<0>, (const void*)k_walk<1>, (const void*)k_walk<2>, (const void*)k_walk<4>,
-                      (const void*)k_walk<8>};
+static void launch_walk(const WalkArgs& a, int grid, hipStream_t s) {
+  if (a.mode == 0) launch_walk_m<0>(a, grid, s);
+  else if (a.mode == 1) launch_walk_m<1>(a, grid, s);
+  else launch_walk_m<2>(a, grid, s);
+}
+template <int MODE>
+static void set_walk_lds_limit_m(int L) {
+  const void* ks[] = {(const void*)k_walk<0, MODE>, (const void*)k_walk<1, MODE>, (const void*)k_walk<2, MODE>,
+                      (const void*)k_walk<4, MODE>, (const void*)k_walk<8, MODE>};
   for (const void* k : ks) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)walk_lds(L));
+}
+static void set_walk_lds_limit(int L) {
+  set_walk_lds_limit_m<0>(L);
+  set_walk_lds_limit_m<1>(L);
+  set_walk_lds_limit_m<2>(L);
 }
 
 template <int TPR>
