@@ -14,8 +14,9 @@ Prints one JSON line (the driver's contract) and exits 0.
 
 Kernel timing: the timed iterations run as captured HIP graphs with no
 instrumentation; --prof-iters further iterations then run with HIP events
-around every histogram / split / partition launch on the context's stream,
-which gives the histogram kernel's average launch duration and GB/s. HBM
+around every histogram / slab-reduction / split / partition launch on the
+stream the kernels run on, which gives the histogram kernel's average launch
+duration (its own launch only, plus the dispatch gap before it) and GB/s. HBM
 traffic per histogram launch comes from rocprofv3 FETCH_SIZE / WRITE_SIZE
 passes over this same command, scaled by the counters' reading on a copy of
 known size (--copy-calibrate), recorded in profiles/hist_traffic.json.
@@ -101,10 +102,15 @@ def copy_calibrate(mib):
     a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(3):
         b.copy_(a)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_calibrate_mib": mib, "copies": 3}), flush=True)
+    ms = e0.elapsed_time(e1) / 3
+    print(json.dumps({"copy_calibrate_mib": mib, "copies": 3, "ms_per_copy": ms,
+                      "copy_gbs_read_plus_write": 2 * mib * (1 << 20) / (ms * 1e-3) / 1e9}), flush=True)
 
 
 def recorded_traffic(form, rows, max_bin, num_leaves):
@@ -147,6 +153,7 @@ def _forms_summary(forms, quality_name):
     return {k: {"iters_per_sec": v["iters_per_sec"], "ms_per_iter": v["ms_per_iter"], quality_name: v["auc"],
                 "hist_ms_per_iter": v["prof"]["hist_ms"] / max(v["prof"]["iters"], 1),
                 "split_ms_per_iter": v["prof"]["split_ms"] / max(v["prof"]["iters"], 1),
+                "slab_ms_per_iter": v["prof"]["slab_ms"] / max(v["prof"]["iters"], 1),
                 "part_ms_per_iter": v["prof"]["part_ms"] / max(v["prof"]["iters"], 1)} for k, v in forms.items()}
 
 
@@ -356,6 +363,7 @@ def main():
                           hist_avg_launch_us=v["hist_avg_launch_us"], auc_holdout=v["auc"],
                           iterations_trained=v["iterations"],
                           split_ms_per_iter=v["prof"]["split_ms"] / max(v["prof"]["iters"], 1),
+                          slab_ms_per_iter=v["prof"]["slab_ms"] / max(v["prof"]["iters"], 1),
                           part_ms_per_iter=v["prof"]["part_ms"] / max(v["prof"]["iters"], 1),
                           hist_ms_per_iter=v["prof"]["hist_ms"] / max(v["prof"]["iters"], 1))
                   for k, v in forms.items()},

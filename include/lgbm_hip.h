@@ -184,14 +184,16 @@ lgbmhip_status lgbmhip_predict_device(lgbmhip_ctx* ctx, const float* d_X, const 
                                       int32_t ncol, double* d_out);
 
 /* Profiling of training iterations: enable = 1 records HIP events around every
- * histogram, split and partition launch (graphs off), 0 stops, -1 only reads.
- * out11 (may be NULL) receives the totals since enabling: {histogram ms,
- * histogram launches (one per growth round plus the root's), rows summed into
- * histograms, algorithmic histogram bytes, split ms, partition ms, iteration
- * ms, iterations, rows moved by partitions, splits computed, splits in the
- * trees}. No counterpart in the reference (it has no profiler hooks); for
- * bench and rocprof cross-checks. */
-lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out11);
+ * histogram, slab-reduction, split and partition launch (graphs off), 0 stops,
+ * -1 only reads. out12 (may be NULL) receives the totals since enabling:
+ * {histogram-kernel ms, histogram launches (one per growth round plus the
+ * root's), rows summed into histograms, algorithmic histogram bytes, split ms,
+ * partition ms (with the round's plan), iteration ms, iterations, rows moved
+ * by partitions, splits computed, splits in the trees, slab-reduction ms}.
+ * Each span runs from the previous event on the same stream, so it includes
+ * that launch's dispatch gap. No counterpart in the reference (it has no
+ * profiler hooks); for bench and rocprof cross-checks. */
+lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out12);
 
 /* Launch shapes of the tree-growth kernels (0 keeps the current value):
  * partition blocks, histogram row chunks, minimum rows per histogram chunk,

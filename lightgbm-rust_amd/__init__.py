@@ -270,10 +270,10 @@ class Booster:
 
     def profile(self, enable=-1):
         """Kernel-kind timing totals (see lgbmhip_profile); enable=1 starts, 0 stops, -1 reads."""
-        out = np.zeros(11, np.float64)
+        out = np.zeros(12, np.float64)
         check(lib().lgbmhip_profile(self._ctx, int(enable), ptr(out)), self._ctx)
         keys = ("hist_ms", "hist_launches", "hist_rows", "hist_bytes", "split_ms", "part_ms", "iter_ms", "iters",
-                "part_rows", "splits_computed", "splits_used")
+                "part_rows", "splits_computed", "splits_used", "slab_ms")
         return dict(zip(keys, out.tolist()))
 
     def set_tuning(self, part_grid=0, hist_chunks=0, hist_min_rows=0, hist_lds_budget=0, rounds_main=0,

@@ -289,8 +289,8 @@ struct Ctx {
   double last_ms = 0.0;
   // profiling: HIP events around every histogram / split / partition launch
   int profile = 0;
-  std::vector<hipEvent_t> pev;   // [K][L][3 kinds][2]
-  double prof[11] = {0};         // hist_ms, hist_launches, hist_rows, hist_bytes, split_ms, part_ms, iter_ms, iters,
+  std::vector<hipEvent_t> pev;   // [K][L + 2 steps][4 kinds][2]
+  double prof[12] = {0};         // hist_ms, hist_launches, hist_rows, hist_bytes, split_ms, part_ms, iter_ms, iters,
                                  // rows partitioned, splits computed, splits used
 
   ~Ctx() {
@@ -693,10 +693,11 @@ static PartArgs part_args(Ctx* c, Lane& ln) {
 }
 
 // Profiling events: per class, step 0 = the root's histogram and search,
-// step r + 1 = growth round r; kinds 0 histogram, 1 search, 2 partition.
+// step r + 1 = growth round r; kinds 0 histogram, 1 search, 2 partition,
+// 3 slab reduction.
 static int prof_steps(const Ctx* c) { return c->L + 2; }
 static hipEvent_t pevent(Ctx* c, int k, int step, int kind, int end) {
-  size_t i = ((((size_t)k * prof_steps(c) + step) * 3 + kind) * 2 + end);
+  size_t i = ((((size_t)k * prof_steps(c) + step) * 4 + kind) * 2 + end);
   return c->pev[i];
 }
 static void prof_mark(Ctx* c, int k, int step, int kind, int end) {
@@ -738,8 +739,9 @@ static void enqueue_head(Ctx* c, Lane& ln, int k) {
   const int hgrid = c->hist_chunks * c->hist_ng;
   prof_mark(c, k, 0, 0, 0);
   launch_hist(hist_args(c, ln, quant, k, 2), quant, c->hist_tpr, hgrid, c->hist_lds, s);
-  slab_total(c, ln, quant, s);
   prof_mark(c, k, 0, 0, 1);
+  slab_total(c, ln, quant, s);
+  prof_mark(c, k, 0, 3, 1);
   launch_split(split_args(c, ln, k, true), s);
   prof_mark(c, k, 0, 1, 1);
 }
@@ -758,8 +760,9 @@ static void enqueue_rounds(Ctx* c, Lane& ln, int k, int r0, int n) {
     launch_part(pa, c->part_grid, s);
     prof_mark(c, k, step, 2, 1);
     launch_hist(ha, quant, c->hist_tpr, hgrid, c->hist_lds, s);
-    slab_total(c, ln, quant, s);
     prof_mark(c, k, step, 0, 1);
+    slab_total(c, ln, quant, s);
+    prof_mark(c, k, step, 3, 1);
     launch_split(sa, s);
     prof_mark(c, k, step, 1, 1);
   }
@@ -1044,12 +1047,14 @@ static void account_profile(Ctx* c, int k) {
   };
   c->prof[0] += el(pevent(c, k, 0, 0, 0), pevent(c, k, 0, 0, 1));
   c->prof[1] += 1;
-  c->prof[4] += el(pevent(c, k, 0, 0, 1), pevent(c, k, 0, 1, 1));
+  c->prof[11] += el(pevent(c, k, 0, 0, 1), pevent(c, k, 0, 3, 1));
+  c->prof[4] += el(pevent(c, k, 0, 3, 1), pevent(c, k, 0, 1, 1));
   const int steps = std::min(c->last_rounds[k] + 1, prof_steps(c));
   for (int step = 1; step < steps; ++step) {
     c->prof[0] += el(pevent(c, k, step, 2, 1), pevent(c, k, step, 0, 1));
     c->prof[1] += 1;
-    c->prof[4] += el(pevent(c, k, step, 0, 1), pevent(c, k, step, 1, 1));
+    c->prof[11] += el(pevent(c, k, step, 0, 1), pevent(c, k, step, 3, 1));
+    c->prof[4] += el(pevent(c, k, step, 3, 1), pevent(c, k, step, 1, 1));
     c->prof[5] += el(pevent(c, k, step, 2, 0), pevent(c, k, step, 2, 1));
   }
   c->prof[2] += (double)st.hist_rows;
@@ -1736,17 +1741,17 @@ lgbmhip_status lgbmhip_predict_device(lgbmhip_ctx* ctx, const float* d_X, const 
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
 }
-lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out11) {
+lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out12) {
   return guarded(ctx, [&] {
     Ctx* c = &ctx->c;
-    if (out11) for (int i = 0; i < 11; ++i) out11[i] = c->prof[i];
+    if (out12) for (int i = 0; i < 12; ++i) out12[i] = c->prof[i];
     if (enable >= 0) {
       c->profile = enable;
-      for (int i = 0; i < 11; ++i) c->prof[i] = 0.0;
+      for (int i = 0; i < 12; ++i) c->prof[i] = 0.0;
       if (enable && c->pev.empty()) {
         set_device(c);
         if (!c->state_ready) setup_training(c);
-        c->pev.resize((size_t)c->K * prof_steps(c) * 3 * 2);
+        c->pev.resize((size_t)c->K * prof_steps(c) * 4 * 2);
         for (auto& e : c->pev) HIP_CHECK(hipEventCreate(&e));
       }
     }
