@@ -200,7 +200,7 @@ lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out12);
  * LDS budget in bytes of one histogram block (sets features per block),
  * growth rounds in a tree's first launch and in each continuation launch.
  * Defaults: part_grid = min(tiles, 4 x CUs), hist_chunks = 2 x CUs
- * (quantized) or CUs (FP64), 2048 (quantized) or 4096 (FP64) rows, 32 KiB,
+ * (quantized) or CUs (FP64), 4096 rows, 32 KiB,
  * 24 and 8 rounds. Trees do not depend on any of them.
  * No counterpart in the reference (its thread count is its only knob); for
  * launch-shape sweeps on other row/feature shapes (Booster.set_tuning). */

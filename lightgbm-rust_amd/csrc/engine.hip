@@ -540,7 +540,7 @@ static void setup_training(Ctx* c) {
   c->part_grid = (int)std::min<int64_t>((N + part_tile() - 1) / part_tile(),
                                         c->tune_part_grid > 0 ? c->tune_part_grid : 4 * c->cus);
   c->hist_chunks = c->tune_hist_chunks > 0 ? c->tune_hist_chunks : (quant ? 2 : 1) * c->cus;
-  c->hist_min_rows = c->tune_hist_min_rows > 0 ? c->tune_hist_min_rows : (quant ? 2048 : 4096);
+  c->hist_min_rows = c->tune_hist_min_rows > 0 ? c->tune_hist_min_rows : 4096;
   if (quant) {
     // quant.rs:45-79: per-row tables from block generators of `quant_table_threads` blocks
     std::vector<double> grv(N), hrv(N);

@@ -129,6 +129,11 @@ struct HistArgs {
 
 // rows each thread has in flight per pass (all loads issued before the atomics)
 constexpr int kHistUnroll = 8;
+// Threads of a histogram block: the quantized form's 512-thread blocks take
+// 4096-row chunks, so a unit's slab (one histogram per chunk) stays a small
+// share of the bytes it reads; the FP64 form's wider LDS histogram keeps 256.
+template <bool Q>
+struct HistBlock { static constexpr int threads = Q ? 512 : 256; };
 
 // One unit: rows [r0, r1) of the segment's chunk, feature group fg, into LDS,
 // then stored to the unit's slab (an empty chunk stores zeros).
@@ -153,7 +158,7 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
     mfb[k] = fl < FBg ? a.feats[f0 + fl].mfb : 256;
   }
   __syncthreads();
-  const int rpp = 256 / TPR;
+  const int rpp = HistBlock<Q>::threads / TPR;
   // select, not index: a runtime index into a kernel-argument array makes
   // the compiler copy the arguments to scratch memory
   // lanes past the group's last feature read the row's first word (unused)
@@ -369,13 +374,13 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
 }
 
 template <int TPR>
-__global__ void __launch_bounds__(256) k_hist_f64(HistArgs a) {
+__global__ void __launch_bounds__(HistBlock<false>::threads) k_hist_f64(HistArgs a) {
   extern __shared__ unsigned long long lds_hist[];
   hist_kernel<TPR, false>(a, lds_hist);
 }
 
 template <int TPR>
-__global__ void __launch_bounds__(256) k_hist_int(HistArgs a) {
+__global__ void __launch_bounds__(HistBlock<true>::threads) k_hist_int(HistArgs a) {
   extern __shared__ unsigned long long lds_hist[];
   hist_kernel<TPR, true>(a, lds_hist);
 }
@@ -1721,8 +1726,8 @@ static void set_walk_lds_limit(int L) {
 
 template <int TPR>
 static void launch_hist_t(const HistArgs& a, bool quant, int grid, size_t lds, hipStream_t s) {
-  if (quant) hipLaunchKernelGGL(k_hist_int<TPR>, dim3(grid), dim3(256), lds, s, a);
-  else hipLaunchKernelGGL(k_hist_f64<TPR>, dim3(grid), dim3(256), lds, s, a);
+  if (quant) hipLaunchKernelGGL(k_hist_int<TPR>, dim3(grid), dim3(HistBlock<true>::threads), lds, s, a);
+  else hipLaunchKernelGGL(k_hist_f64<TPR>, dim3(grid), dim3(HistBlock<false>::threads), lds, s, a);
 }
 
 static void launch_hist(const HistArgs& a, bool quant, int tpr, int grid, size_t lds, hipStream_t s) {
