@@ -275,7 +275,8 @@ struct Ctx {
   // lambdarank state (rank.rs:129-341)
   DevBuf<int> d_qb, r_rank, r_sorted;
   DevBuf<double> r_imd, r_table, r_gain, r_disc;
-  DevBuf<float> r_lam, r_hes;
+  DevBuf<float> r_lam, r_hes, r_bl;
+  DevBuf<double> r_bs;
   double r_min_in = 0, r_max_in = 0, r_factor = 0;
   // prediction cache
   bool pred_dirty = true;
@@ -458,6 +459,8 @@ static void setup_rank(Ctx* c) {
   c->r_sorted.alloc(c->N);
   c->r_lam.alloc(c->N);
   c->r_hes.alloc(c->N);
+  c->r_bs.alloc(c->N);
+  c->r_bl.alloc(c->N);
   HIP_CHECK(hipMemcpy(c->d_qb.p, qb.data(), qb.size() * 4, hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->r_imd.p, imd.data(), Q * 8, hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->r_table.p, table.data(), bins * 8, hipMemcpyHostToDevice));
@@ -1014,6 +1017,8 @@ static void enqueue_gradients(Ctx* c) {
     a.sorted = c->r_sorted.p;
     a.lam = c->r_lam.p;
     a.hes = c->r_hes.p;
+    a.by_rank_s = c->r_bs.p;
+    a.by_rank_l = c->r_bl.p;
     a.g = c->g.p;
     a.h = c->h.p;
     if (a.Q > 0) hipLaunchKernelGGL(k_lambdarank, dim3(a.Q), dim3(kRankThreads), 0, s, a);

@@ -319,6 +319,8 @@ struct RankArgs {
   const double* discount;    // 10000 entries
   int* rank;                 // scratch [N]: rank of each row within its query
   int* sorted;               // scratch [N]: row (query-local) at each rank
+  double* by_rank_s;         // scratch [N]: score at each rank
+  float* by_rank_l;          // scratch [N]: label at each rank
   float* lam;                // scratch [N]: accumulators by rank
   float* hes;
   float* g;
@@ -343,9 +345,13 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
   const float* lab = a.label + start;
   int* rank = a.rank + start;
   int* sorted = a.sorted + start;
+  double* rs = a.by_rank_s + start;   // by rank: contiguous, no index chase in the pair loop
+  float* rl = a.by_rank_l + start;
   float* lam = a.lam + start;
   float* hes = a.hes + start;
   if (cnt <= 0) return;
+  __shared__ double sgain[31];
+  if (tid < 31) sgain[tid] = a.label_gain[tid];
   // stable descending order: rank = #{j : s_j > s_i or (s_j == s_i and j < i)}
   for (int i = tid; i < cnt; i += kRankThreads) {
     const double si = s[i];
@@ -356,14 +362,16 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
     }
     rank[i] = r;
     sorted[r] = i;
+    rs[r] = si;
+    rl[r] = lab[i];
     lam[i] = 0.0f;
     hes[i] = 0.0f;
   }
   __syncthreads();
-  const double best = s[sorted[0]];
+  const double best = rs[0];
   int worst_idx = cnt - 1;
-  if (worst_idx > 0 && s[sorted[worst_idx]] == -INFINITY) worst_idx--;
-  const double worst = s[sorted[worst_idx]];
+  if (worst_idx > 0 && rs[worst_idx] == -INFINITY) worst_idx--;
+  const double worst = rs[worst_idx];
   const double imd = a.inv_max_dcg[q];
   const bool scale_by_delta = a.norm && best != worst;
   __shared__ float st_l[kRankChunk], st_h[kRankChunk];
@@ -371,22 +379,21 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
   double S = 0.0;
   const int imax = min(cnt - 1, max(a.trunc, 0));
   for (int i = 0; i < imax; ++i) {
-    const int ai = sorted[i];
-    if (s[ai] == -INFINITY) continue;
-    const float la = lab[ai];
+    const double sa = rs[i];
+    if (sa == -INFINITY) continue;
+    const float la = rl[i];
     float anc_l = lam[i], anc_h = hes[i];   // lane 0 only
     for (int j0 = i + 1; j0 < cnt; j0 += kRankChunk) {
       const int jn = min(cnt, j0 + kRankChunk);
       for (int j = j0 + tid; j < jn; j += kRankThreads) {
-        const int bj = sorted[j];
-        const float lb = lab[bj];
-        const bool ok = s[bj] != -INFINITY && la != lb;
+        const double sb = rs[j];
+        const float lb = rl[j];
+        const bool ok = sb != -INFINITY && la != lb;
         if (ok) {
           const bool anchor_high = la > lb;
-          const int high = anchor_high ? ai : bj, low = anchor_high ? bj : ai;
           const int hr = anchor_high ? i : j, lr = anchor_high ? j : i;
-          const double hs = s[high], ls = s[low];
-          const double hg = a.label_gain[(int)lab[high]], lg = a.label_gain[(int)lab[low]];
+          const double hs = anchor_high ? sa : sb, ls = anchor_high ? sb : sa;
+          const double hg = sgain[(int)(anchor_high ? la : lb)], lg = sgain[(int)(anchor_high ? lb : la)];
           const double ds = hs - ls;
           const double gap = hg - lg;
           const double pd = fabs(a.discount[hr] - a.discount[lr]);
