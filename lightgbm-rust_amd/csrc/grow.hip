@@ -827,6 +827,29 @@ struct RowVec<8> { using T = u32x2; };
 template <>
 struct RowVec<16> { using T = u32x4; };
 
+// Copies n records of RW words, record w from src(w) to dst(w) (a null dst
+// skips the record); each thread issues up to 8 word loads before its stores.
+template <int RW, class Src, class Dst>
+__device__ __forceinline__ void copy_recs(int n, Src src, Dst dst) {
+  const int total = n * RW;
+  for (int e0 = threadIdx.x; e0 < total; e0 += kPartThreads * 8) {
+    unsigned v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = min(e0 + u * kPartThreads, total - 1);
+      v[u] = src(e / RW)[e % RW];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * kPartThreads;
+      if (e < total) {
+        unsigned* d = dst(e / RW);
+        if (d) d[e % RW] = v[u];
+      }
+    }
+  }
+}
+
 template <int RP>   // positions per lane: num_leaves <= 64 RP
 __global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
   StepDev* st = a.st;
@@ -845,20 +868,41 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
   const int nseq0 = st->nseq, npend0 = st->npending, nslots0 = st->nslots, nspl0 = st->num_splits;
   const int par = st->parity, sbase = st->search_base, nsearched = st->nsearched;
   const int F = a.F, L = a.L;
-  // ---- 1. positions and searched rows (independent loads)
-  for (int i = tid; i < nseq0; i += kPartThreads) P[i] = a.pos[(size_t)par * L + i];
+  // ---- 1. positions and searched rows (independent loads, all issued
+  // before any is used: clamped slots reload a valid entry)
+  for (int i0 = tid; i0 < nseq0; i0 += kPartThreads * 4) {
+    PosRec pr[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pr[u] = a.pos[(size_t)par * L + min(i0 + u * kPartThreads, nseq0 - 1)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (i0 + u * kPartThreads < nseq0) P[i0 + u * kPartThreads] = pr[u];
+  }
   {
     const int r = tid / kRowThreads, q = tid % kRowThreads;
     Cand c = cand_none();
-    if (r < nsearched)
-      for (int f = q; f < F; f += kRowThreads) {
-        const SplitRec& rec = a.feat_split[(size_t)r * F + f];
-        Cand d;
-        d.g = rec.gain;
-        d.f = rec.feature;
-        d.l = f;
-        c = pick(c, d);
+    if (r < nsearched) {
+      const SplitRec* row = a.feat_split + (size_t)r * F;
+      for (int f0 = q; f0 < F; f0 += kRowThreads * 8) {
+        double g8[8];
+        int f8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int f = min(f0 + u * kRowThreads, F - 1);
+          g8[u] = row[f].gain;
+          f8[u] = row[f].feature;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (f0 + u * kRowThreads >= F) continue;
+          Cand d;
+          d.g = g8[u];
+          d.f = f8[u];
+          d.l = f0 + u * kRowThreads;
+          c = pick(c, d);
+        }
       }
+    }
     for (int o = 1; o < kRowThreads; o <<= 1) {
       Cand d;
       d.g = __shfl_xor(c.g, o, 64);
@@ -1112,13 +1156,16 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
     const int feat = (rs >= 0 && rs < nsearched) ? rowbest[rs].f : P[ipos[tid]].feat;
     ifd[tid] = a.feats[feat];
   }
-  for (int e = tid; e < k * RW; e += kPartThreads) {
-    const int w = e / RW, wd = e % RW;
-    const int pslot = P[ipos[w]].slot;
-    const int rs = pslot - sbase;
-    const SplitRec* src = (rs >= 0 && rs < nsearched) ? a.feat_split + (size_t)rs * F + rowbest[rs].l : a.best + pslot;
-    ((unsigned*)&irec[w])[wd] = ((const unsigned*)src)[wd];
-  }
+  copy_recs<RW>(
+      k,
+      [&](int w) {
+        const int pslot = P[ipos[w]].slot;
+        const int rs = pslot - sbase;
+        const SplitRec* src =
+            (rs >= 0 && rs < nsearched) ? a.feat_split + (size_t)rs * F + rowbest[rs].l : a.best + pslot;
+        return (const unsigned*)src;
+      },
+      [&](int w) { return (unsigned*)&irec[w]; });
   __syncthreads();
   if (tid == 0) {
     int t = 0;
@@ -1163,11 +1210,10 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
     }
     PosRec* nxt = a.pos + (size_t)(par ^ 1) * L;
     for (int i = tid; i < s_nseq; i += kPartThreads) nxt[i] = P[i];
-    for (int e = tid; e < nsearched * RW; e += kPartThreads) {
-      const int r = e / RW, wd = e % RW;
-      if (rowbest[r].l >= 0)
-        ((unsigned*)(a.best + sbase + r))[wd] = ((const unsigned*)(a.feat_split + (size_t)r * F + rowbest[r].l))[wd];
-    }
+    copy_recs<RW>(
+        nsearched,
+        [&](int r) { return (const unsigned*)(a.feat_split + (size_t)r * F + max(rowbest[r].l, 0)); },
+        [&](int r) { return rowbest[r].l >= 0 ? (unsigned*)(a.best + sbase + r) : (unsigned*)nullptr; });
     for (int r = tid; r < nsearched; r += kPartThreads)
       if (rowbest[r].l < 0) {
         SplitRec z;
