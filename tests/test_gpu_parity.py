@@ -640,3 +640,31 @@ def test_quantized_categorical_trees_and_prediction_match_oracle():
     np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(b.predict(X[:3000], raw_score=True), ob.predict_raw(X[:3000])[:, 0],
                                rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("trunc", [30, 100])
+def test_lambdarank_gradients_every_query_size_class(trunc):
+    # queries of 1 to 2600 documents (one and several 1024-partner chunks,
+    # sizes at and around powers of two), about 1% of scores -inf, and anchors
+    # past the first 64 ranks (trunc 100)
+    sizes = np.array([1, 2, 3, 63, 64, 65, 130, 255, 256, 257, 700, 2047, 2048, 2049, 2600])
+    qb = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    n = int(qb[-1])
+    rng = np.random.default_rng(31 + trunc)
+    X = rng.normal(size=(n, 6)).astype(np.float32)
+    rel = rng.integers(0, 5, n).astype(np.float32)
+    params = dict(objective="lambdarank", num_leaves=15, max_bin=63, lambdarank_truncation_level=trunc)
+    ds = lgb.Dataset(X, label=rel, group=sizes, params=params)
+    b = lgb.Booster(params, ds)
+    s = np.round(rng.normal(size=n), 2)
+    s[rng.random(n) < 0.01] = -np.inf
+    g = np.zeros(n, np.float32)
+    h = np.zeros(n, np.float32)
+    lib = lgb._capi.lib()
+    lgb._capi.check(lib.lgbmhip_gradients(b._ctx, lgb.ptr(s), lgb.ptr(g), lgb.ptr(h)), b._ctx)
+    og, oh = O.lambdarank_gradients(s, rel, qb, trunc=trunc)
+    for a, c in ((g, og), (h, oh)):
+        diff = a != c
+        assert diff.sum() <= max(2, int(1e-4 * len(a)))
+        if diff.any():
+            assert np.all(np.abs(a[diff].view(np.int32) - c[diff].view(np.int32)) <= 1)
