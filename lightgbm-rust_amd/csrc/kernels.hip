@@ -1255,21 +1255,28 @@ __device__ __forceinline__ int pred_step(const uint4 n, const float* xr, const u
 // stride so lanes reading one feature hit distinct banks. For each class,
 // the class's trees are summed in model order; kPredTrees of them are walked
 // together so their node loads overlap.
-template <bool BINNED, bool STAGE>
-__global__ void __launch_bounds__(kPredRows) k_predict(PredArgs a) {
+template <bool BINNED, bool STAGE, int W>
+__global__ void __launch_bounds__(kPredRows * W) k_predict(PredArgs a) {
+  // W waves share one 64-row tile: wave w walks trees w*G .. w*G + G - 1 of
+  // every chunk of W*G trees of a class and leaves their values in LDS, then
+  // wave 0 adds the chunk's values to each row's sum in model order.
+  constexpr int G = kPredTrees;
+  constexpr int TC = W * G;
   extern __shared__ __align__(16) unsigned char pred_lds[];
+  __shared__ double vals[W > 1 ? TC : 1][kPredRows];
   const long long r0 = (long long)blockIdx.x * kPredRows;
   const int nrow = (int)min((long long)kPredRows, a.rows - r0);
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NT = kPredRows * W;
   const float* xr = nullptr;
   const uint8_t* br = nullptr;
   if (STAGE) {
     // copy the block's contiguous rows, element e = (r, f) -> r * stride + f
     const long long n = (long long)nrow * a.ncol;
-    int r = 0, f = lane;
+    int r = 0, f = tid;
     while (f >= a.ncol) { f -= a.ncol; ++r; }
-    const int step_r = kPredRows / a.ncol, step_f = kPredRows % a.ncol;
-    for (long long e = lane; e < n; e += kPredRows) {
+    const int step_r = NT / a.ncol, step_f = NT % a.ncol;
+    for (long long e = tid; e < n; e += NT) {
       if (BINNED) pred_lds[r * a.stride + f] = a.bins[r0 * a.ncol + e];
       else ((float*)pred_lds)[r * a.stride + f] = a.X[r0 * a.ncol + e];
       r += step_r;
@@ -1287,20 +1294,22 @@ __global__ void __launch_bounds__(kPredRows) k_predict(PredArgs a) {
   }
   for (int k = 0; k < a.K; ++k) {
     double acc = 0.0;
-    for (int t0 = k; t0 < a.T; t0 += kPredTrees * a.K) {
-      int node[kPredTrees], base[kPredTrees];
+    const int nk = a.T > k ? (a.T - k + a.K - 1) / a.K : 0;   // trees of class k
+    for (int c0 = 0; c0 < nk; c0 += TC) {
+      int node[G], base[G];
 #pragma unroll
-      for (int g = 0; g < kPredTrees; ++g) {
-        const int t = t0 + g * a.K;
-        const bool walk = t < a.T && a.tree_num_leaves[t] > 1;
+      for (int g = 0; g < G; ++g) {
+        const int ci = c0 + wave * G + g;
+        const int t = k + ci * a.K;
+        const bool walk = ci < nk && a.tree_num_leaves[t] > 1;
         node[g] = walk ? 0 : ~0;
-        base[g] = t < a.T ? a.tree_node_off[t] : 0;
+        base[g] = ci < nk ? a.tree_node_off[t] : 0;
       }
       bool more = true;
       while (more) {
         more = false;
 #pragma unroll
-        for (int g = 0; g < kPredTrees; ++g) {
+        for (int g = 0; g < G; ++g) {
           if (node[g] >= 0) {
             const uint4 n = a.nodes[base[g] + node[g]];
             node[g] = pred_step<BINNED>(n, xr, br, a.ncol, a.cat_words);
@@ -1308,13 +1317,27 @@ __global__ void __launch_bounds__(kPredRows) k_predict(PredArgs a) {
           }
         }
       }
+      if (W == 1) {
 #pragma unroll
-      for (int g = 0; g < kPredTrees; ++g) {
-        const int t = t0 + g * a.K;
-        if (t < a.T) acc += a.leaf_val[a.tree_leaf_off[t] + ~node[g]];
+        for (int g = 0; g < G; ++g) {
+          const int ci = c0 + g;
+          if (ci < nk) acc += a.leaf_val[a.tree_leaf_off[k + ci * a.K] + ~node[g]];
+        }
+      } else {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const int ci = c0 + wave * G + g;
+          if (ci < nk) vals[wave * G + g][lane] = a.leaf_val[a.tree_leaf_off[k + ci * a.K] + ~node[g]];
+        }
+        __syncthreads();
+        if (wave == 0) {
+          const int m = min(TC, nk - c0);
+          for (int i = 0; i < m; ++i) acc += vals[i][lane];
+        }
+        __syncthreads();
       }
     }
-    if (lane < nrow) a.out[(r0 + lane) * a.K + k] = acc;
+    if (wave == 0 && lane < nrow) a.out[(r0 + lane) * a.K + k] = acc;
   }
 }
 
