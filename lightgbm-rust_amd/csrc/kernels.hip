@@ -351,21 +351,33 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
   float* hes = a.hes + start;
   if (cnt <= 0) return;
   __shared__ double sgain[31];
+  __shared__ float st_l[kRankChunk], st_h[kRankChunk];
+  __shared__ double st_p[kRankChunk];
   if (tid < 31) sgain[tid] = a.label_gain[tid];
-  // stable descending order: rank = #{j : s_j > s_i or (s_j == s_i and j < i)}
-  for (int i = tid; i < cnt; i += kRankThreads) {
-    const double si = s[i];
+  // stable descending order: rank = #{j : s_j > s_i or (s_j == s_i and j < i)},
+  // the scores compared from LDS, kRankChunk at a time
+  for (int i0 = 0; i0 < cnt; i0 += kRankThreads) {
+    const int i = i0 + tid;
+    const double si = s[i < cnt ? i : cnt - 1];
     int r = 0;
-    for (int j = 0; j < cnt; ++j) {
-      const double sj = s[j];
-      r += (sj > si) || (!(sj < si) && !(sj > si) && j < i);
+    for (int j0 = 0; j0 < cnt; j0 += kRankChunk) {
+      const int jn = min(cnt, j0 + kRankChunk);
+      __syncthreads();
+      for (int j = j0 + tid; j < jn; j += kRankThreads) st_p[j - j0] = s[j];
+      __syncthreads();
+      for (int j = j0; j < jn; ++j) {
+        const double sj = st_p[j - j0];
+        r += (sj > si) || (!(sj < si) && !(sj > si) && j < i);
+      }
     }
-    rank[i] = r;
-    sorted[r] = i;
-    rs[r] = si;
-    rl[r] = lab[i];
-    lam[i] = 0.0f;
-    hes[i] = 0.0f;
+    if (i < cnt) {
+      rank[i] = r;
+      sorted[r] = i;
+      rs[r] = si;
+      rl[r] = lab[i];
+      lam[i] = 0.0f;
+      hes[i] = 0.0f;
+    }
   }
   __syncthreads();
   const double best = rs[0];
@@ -374,8 +386,6 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
   const double worst = rs[worst_idx];
   const double imd = a.inv_max_dcg[q];
   const bool scale_by_delta = a.norm && best != worst;
-  __shared__ float st_l[kRankChunk], st_h[kRankChunk];
-  __shared__ double st_p[kRankChunk];
   double S = 0.0;
   const int imax = min(cnt - 1, max(a.trunc, 0));
   for (int i = 0; i < imax; ++i) {
@@ -410,9 +420,9 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
           hes[j] += fh;
           st_l[j - j0] = anchor_high ? fl : -fl;
           st_h[j - j0] = fh;
-          st_p[j - j0] = pl;
+          st_p[j - j0] = 2.0 * pl;   // exact: the chain subtracts 2 p
         } else {
-          // a skipped pair adds nothing: x + (-0.0f) == x and S - 2 * 0.0 == S
+          // a skipped pair adds nothing: x + (-0.0f) == x and S - 0.0 == S
           // bit for bit, so the anchor's chains need no branch
           st_l[j - j0] = -0.0f;
           st_h[j - j0] = -0.0f;
@@ -438,13 +448,13 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
           for (int u = 0; u < 8; ++u) {
             anc_l += l8[u];
             anc_h += h8[u];
-            S -= 2.0 * p8[u];
+            S -= p8[u];
           }
         }
         for (; k < m; ++k) {
           anc_l += st_l[k];
           anc_h += st_h[k];
-          S -= 2.0 * st_p[k];
+          S -= st_p[k];
         }
       }
       __syncthreads();
