@@ -28,6 +28,9 @@
  *   lgbmhip_partition         <- DataPartition::split           learner/partition.rs:107-174
  *   lgbmhip_predict_raw       <- Gbdt::predict_matrix_early_stop predict.rs:231-339
  *   lgbmhip_predict_binned    <- Tree::add_prediction_to_score  tree.rs:605-652
+ *   lgbmhip_add_valid         <- Gbdt::add_valid                boosting.rs:378-411
+ *   lgbmhip_copy_valid_scores <- Gbdt::valid_scores             boosting.rs:1282-1284
+ *   lgbmhip_eval              <- Metric::eval                   metric.rs:525-545, 551-729
  */
 #ifndef LGBM_HIP_H_
 #define LGBM_HIP_H_
@@ -97,6 +100,23 @@ typedef struct {
   int32_t lambdarank_norm;
   int32_t use_graph;            /* capture each tree's launches in a hipGraph */
 } lgbmhip_train_opts;
+
+/* One metric of lgbmhip_eval (metric.rs:17-43 MetricKind, the Config fields it reads). */
+#define LGBMHIP_METRIC_BINARY_LOGLOSS 0
+#define LGBMHIP_METRIC_BINARY_ERROR 1
+#define LGBMHIP_METRIC_AUC 2
+#define LGBMHIP_METRIC_MULTI_LOGLOSS 3
+#define LGBMHIP_METRIC_MULTI_ERROR 4
+#define LGBMHIP_METRIC_NDCG 5
+typedef struct {
+  int32_t metric;              /* LGBMHIP_METRIC_* */
+  int32_t multi_error_top_k;   /* multi_error: a row counts as an error above this many classes scoring >= its own */
+  int32_t num_threads;         /* NDCG: the reference's per-thread partial sums over queries (1: query order) */
+  int32_t num_eval_at;         /* NDCG cut-offs, ascending; 0: 1..5 */
+  const int32_t* eval_at;
+  int32_t num_label_gain;      /* 0: 2^i - 1 for i < 31 */
+  const double* label_gain;
+} lgbmhip_metric_opts;
 
 const char* lgbmhip_last_error(const lgbmhip_ctx* ctx);
 const char* lgbmhip_global_last_error(void);
@@ -182,6 +202,27 @@ lgbmhip_status lgbmhip_predict_binned(lgbmhip_ctx* ctx, const uint8_t* bins, int
  * already live on the GPU. */
 lgbmhip_status lgbmhip_predict_device(lgbmhip_ctx* ctx, const float* d_X, const uint8_t* d_bins, int64_t rows,
                                       int32_t ncol, double* d_out);
+
+/* A validation set binned with the training data's mappers (rows x F u8,
+ * the layout of lgbmhip_predict_binned): labels (weights, query boundaries
+ * and class-major init scores [K][rows] may be NULL). Its scores start from
+ * init_score (or zeros), take every tree trained so far, one at a time in
+ * model order, and then every later tree on the device as the training
+ * scores do. *out_index receives its data index for lgbmhip_eval (1, 2, ...;
+ * 0 is the training data). */
+lgbmhip_status lgbmhip_add_valid(lgbmhip_ctx* ctx, const uint8_t* bins, int64_t rows, const float* label,
+                                 const float* weight, const int32_t* query_boundaries, int32_t num_queries,
+                                 const double* init_score, int32_t* out_index);
+/* Scores of validation set `index` (1-based), class-major [K][rows]. */
+lgbmhip_status lgbmhip_copy_valid_scores(lgbmhip_ctx* ctx, int32_t index, double* out);
+/* One metric over the scores of data set `data` (0 training, i >= 1 the
+ * i-th validation set), computed on the device. out receives one value, or
+ * one per cut-off for NDCG; *out_count (may be NULL) the number written.
+ * Unweighted AUC, binary_error, multi_error and NDCG equal the reference's
+ * single-thread values exactly; the logloss metrics and weighted AUC sum in
+ * a fixed order of their own (within 1e-10 relative). */
+lgbmhip_status lgbmhip_eval(lgbmhip_ctx* ctx, int32_t data, const lgbmhip_metric_opts* opts, double* out,
+                            int32_t* out_count);
 
 /* Profiling of training iterations: enable = 1 records HIP events around every
  * histogram, slab-reduction, split and partition launch (graphs off), 0 stops,

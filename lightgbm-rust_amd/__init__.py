@@ -10,6 +10,7 @@ Binning (the BinMapper) runs on the host; the packed bin matrix is uploaded
 to HBM once; gradients, histograms, threshold search, partition, score
 update and prediction run as HIP kernels on gfx950.
 """
+import collections
 import ctypes as C
 
 import numpy as np
@@ -17,7 +18,7 @@ import numpy as np
 from . import _capi
 from ._capi import LightGBMError, check, lib, ptr
 
-__all__ = ["Dataset", "Booster", "train", "LightGBMError", "DEFAULT_PARAMS"]
+__all__ = ["Dataset", "Booster", "train", "LightGBMError", "DEFAULT_PARAMS", "EvalResult", "CallbackEnv"]
 
 # lgbm-core defaults of the parameters this path reads (config.rs:508-620).
 DEFAULT_PARAMS = dict(
@@ -29,6 +30,8 @@ DEFAULT_PARAMS = dict(
     stochastic_rounding=True, quant_train_renew_leaf=False, max_cat_to_onehot=4, max_cat_threshold=32,
     cat_l2=10.0, cat_smooth=10.0, min_data_per_group=100, lambdarank_truncation_level=30, lambdarank_norm=True,
     num_threads=0, device_type="gpu", gpu_device_id=0, use_graph=True, quant_table_threads=1,
+    metric="", eval_at=(1, 2, 3, 4, 5), multi_error_top_k=1, label_gain=None, early_stopping_round=0,
+    first_metric_only=False, early_stopping_min_delta=0.0,
 )
 
 ALIASES = {
@@ -43,7 +46,43 @@ ALIASES = {
     "random_seed": "seed", "random_state": "seed", "data_seed": "data_random_seed", "subsample_for_bin":
     "bin_construct_sample_cnt", "num_thread": "num_threads", "nthread": "num_threads", "n_jobs": "num_threads",
     "cat_feature": "categorical_feature", "categorical_column": "categorical_feature",
+    "metrics": "metric", "metric_types": "metric", "ndcg_eval_at": "eval_at", "ndcg_at": "eval_at",
+    "map_eval_at": "eval_at", "map_at": "eval_at", "early_stopping_rounds": "early_stopping_round",
+    "early_stopping": "early_stopping_round", "n_iter_no_change": "early_stopping_round",
 }
+
+# metric names and aliases on this path (config.rs ParseMetrics, metric.rs:44-76)
+METRIC_ALIASES = {
+    "binary_logloss": "binary_logloss", "binary": "binary_logloss", "binary_error": "binary_error",
+    "auc": "auc", "multi_logloss": "multi_logloss", "multiclass": "multi_logloss", "softmax": "multi_logloss",
+    "multiclassova": "multi_logloss", "multiclass_ova": "multi_logloss", "ova": "multi_logloss",
+    "ovr": "multi_logloss", "multi_error": "multi_error", "ndcg": "ndcg", "lambdarank": "ndcg",
+    "rank_xendcg": "ndcg", "xendcg": "ndcg", "xe_ndcg": "ndcg", "xe_ndcg_mart": "ndcg", "xendcg_mart": "ndcg",
+}
+DEFAULT_METRIC = {"binary": "binary_logloss", "multiclass": "multi_logloss", "softmax": "multi_logloss",
+                  "lambdarank": "ndcg"}
+HIGHER_BETTER = {"auc", "ndcg"}
+
+EvalResult = collections.namedtuple("EvalResult", ["dataset_name", "metric_name", "metric_value", "maximize"])
+CallbackEnv = collections.namedtuple(
+    "CallbackEnv", ["model", "params", "iteration", "begin_iteration", "end_iteration", "evaluation_result_list"])
+
+
+def _metrics(p):
+    m = p.get("metric")
+    names = [x.strip() for x in m.split(",")] if isinstance(m, str) else list(m or [])
+    names = [x for x in names if x]
+    if not names:
+        names = [DEFAULT_METRIC[str(p["objective"])]]
+    out = []
+    for x in names:
+        if x in ("None", "none", "null", "na", "custom"):
+            return []
+        if x not in METRIC_ALIASES:
+            raise LightGBMError(f"metric {x!r} is not evaluated on the GPU path")
+        if METRIC_ALIASES[x] not in out:
+            out.append(METRIC_ALIASES[x])
+    return out
 
 OBJECTIVES = {"binary": 0, "multiclass": 1, "softmax": 1, "lambdarank": 2}
 DEVICES = {"gpu", "hip", "cuda"}
@@ -172,6 +211,11 @@ class Booster:
                              int(_b(p["use_graph"])))
         check(lib().lgbmhip_set_train_opts(h, C.byref(to)), h)
         self._finished = False
+        self._metrics = _metrics(p)
+        self._valid = []          # (name, index, Dataset)
+        self._train_name = "training"
+        self.best_iteration = 0
+        self.best_score = {}
 
     # --- training (basic.py:1519 update -> py/lib.rs:479-482 -> Gbdt::train_one_iter)
     def update(self):
@@ -235,6 +279,66 @@ class Booster:
     def leaf_of_row(self):
         out = np.zeros(self.train_set.num_data, np.int32)
         check(lib().lgbmhip_leaf_of_row(self._ctx, ptr(out)), self._ctx)
+        return out
+
+    # --- validation and metrics (basic.py add_valid / eval_train / eval_valid -> boosting.rs:378-411, 1242-1280)
+    def add_valid(self, data, name):
+        """Attach a validation Dataset built with reference=train_set."""
+        if data.reference is not self.train_set and data is not self.train_set:
+            raise LightGBMError("validation data must be constructed with reference to the training data")
+        if data.label is None:
+            raise LightGBMError("validation data has no label")
+        qb, nq = None, 0
+        if data.group is not None:
+            qb = np.ascontiguousarray(np.concatenate([[0], np.cumsum(data.group)]), dtype=np.int32)
+            nq = len(data.group)
+        idx = C.c_int32(0)
+        check(lib().lgbmhip_add_valid(self._ctx, ptr(data.bins), data.num_data, ptr(data.label), ptr(data.weight),
+                                      ptr(qb), nq, None, C.byref(idx)), self._ctx)
+        self._valid.append((name, idx.value, data))
+        return self
+
+    def set_train_data_name(self, name):
+        self._train_name = name
+        return self
+
+    def valid_scores(self, i):
+        """Raw scores of validation set i (0-based), class-major."""
+        name, idx, data = self._valid[i]
+        out = np.zeros(self.K * data.num_data, np.float64)
+        check(lib().lgbmhip_copy_valid_scores(self._ctx, idx, ptr(out)), self._ctx)
+        return out
+
+    def _eval(self, data_index, data_name):
+        p = self.params
+        res = []
+        for m in self._metrics:
+            at = np.ascontiguousarray(sorted(int(k) for k in (p["eval_at"] or (1, 2, 3, 4, 5))), np.int32)
+            gain = None if p["label_gain"] is None else np.ascontiguousarray(p["label_gain"], np.float64)
+            threads = int(p["num_threads"]) if int(p["num_threads"]) > 0 else 1
+            o = _capi.MetricOpts(_capi.METRIC_IDS[m], int(p["multi_error_top_k"]), threads,
+                                 len(at) if m == "ndcg" else 0, at.ctypes.data if m == "ndcg" else None,
+                                 0 if gain is None else len(gain), None if gain is None else gain.ctypes.data)
+            out = np.zeros(max(len(at), 1), np.float64)
+            cnt = C.c_int32(0)
+            check(lib().lgbmhip_eval(self._ctx, data_index, C.byref(o), ptr(out), C.byref(cnt)), self._ctx)
+            if m == "ndcg":
+                names = [f"ndcg@{k}" for k in at]
+            elif m == "multi_error" and int(p["multi_error_top_k"]) != 1:
+                names = [f"multi_error@{int(p['multi_error_top_k'])}"]
+            else:
+                names = [m]
+            for nm, v in zip(names, out[: cnt.value]):
+                res.append(EvalResult(data_name, nm, float(v), m in HIGHER_BETTER))
+        return res
+
+    def eval_train(self):
+        return self._eval(0, self._train_name)
+
+    def eval_valid(self):
+        out = []
+        for name, idx, _ in self._valid:
+            out.extend(self._eval(idx, name))
         return out
 
     # --- prediction (basic.py:1641 -> py/lib.rs:557-579 -> predict.rs:231)
@@ -304,14 +408,66 @@ class Booster:
         self.free()
 
 
-def train(params, train_set, num_boost_round=100, callbacks=None):
-    """engine.py:50: boosting loop calling Booster.update until done."""
+def train(params, train_set, num_boost_round=100, valid_sets=None, valid_names=None, callbacks=None):
+    """engine.py:50-200: the boosting loop, evaluation of the validation sets
+    after each iteration, and early stopping (callback.py:161-305 semantics:
+    a validation metric without improvement for early_stopping_round
+    iterations stops training; best_iteration is 1-based)."""
     p = _params(params)
     n = int(p.get("num_iterations", num_boost_round)) if "num_iterations" in (params or {}) else num_boost_round
+    if n <= 0:
+        raise ValueError(f"Number of boosting rounds must be greater than 0. Got {n}.")
     b = Booster(params, train_set)
+    if isinstance(valid_sets, Dataset):
+        valid_sets = [valid_sets]
+    if isinstance(valid_names, str):
+        valid_names = [valid_names]
+    eval_train = False
+    for i, v in enumerate(valid_sets or []):
+        name = valid_names[i] if valid_names is not None and len(valid_names) > i else f"valid_{i}"
+        if v is train_set:
+            eval_train = True
+            b.set_train_data_name(name)
+            continue
+        b.add_valid(v, name)
+    rounds = int(p["early_stopping_round"] or 0)
+    first_only = _b(p["first_metric_only"])
+    delta = float(p["early_stopping_min_delta"])
+    best, best_iter, best_list = [], [], []
+    results = []
     for i in range(n):
         if b.update():
             break
+        results = []
+        if valid_sets is not None:
+            if eval_train:
+                results.extend(b.eval_train())
+            results.extend(b.eval_valid())
         for cb in callbacks or []:
-            cb(b, i)
+            cb(CallbackEnv(b, p, i, 0, n, results))
+        if rounds <= 0 or not results:
+            continue
+        if not best:
+            best = [-np.inf if r.maximize else np.inf for r in results]
+            best_iter = [0] * len(results)
+            best_list = [results] * len(results)
+        stop = None
+        for j, r in enumerate(results):
+            better = r.metric_value > best[j] + delta if r.maximize else r.metric_value < best[j] - delta
+            if better or i == 0:
+                best[j], best_iter[j], best_list[j] = r.metric_value, i, results
+            if first_only and r.metric_name != results[0].metric_name:
+                continue
+            if r.dataset_name == b._train_name and eval_train:
+                continue
+            if i - best_iter[j] >= rounds or i == n - 1:
+                stop = j
+                break
+        if stop is not None:
+            b.best_iteration = best_iter[stop] + 1
+            results = best_list[stop]
+            break
+    b.best_score = {}
+    for r in results:
+        b.best_score.setdefault(r.dataset_name, {})[r.metric_name] = r.metric_value
     return b

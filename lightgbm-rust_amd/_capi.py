@@ -53,6 +53,14 @@ class TrainOpts(C.Structure):
                 ("lambdarank_norm", C.c_int32), ("use_graph", C.c_int32)]
 
 
+class MetricOpts(C.Structure):
+    _fields_ = [("metric", C.c_int32), ("multi_error_top_k", C.c_int32), ("num_threads", C.c_int32),
+                ("num_eval_at", C.c_int32), ("eval_at", C.c_void_p), ("num_label_gain", C.c_int32),
+                ("label_gain", C.c_void_p)]
+
+
+METRIC_IDS = {"binary_logloss": 0, "binary_error": 1, "auc": 2, "multi_logloss": 3, "multi_error": 4, "ndcg": 5}
+
 P = C.c_void_p
 I32 = C.c_int32
 I64 = C.c_int64
@@ -97,6 +105,9 @@ SIGNATURES = {
     "lgbmhip_predict_binned": (I32, [P, P, I64, P]),
     "lgbmhip_predict_device": (I32, [P, P, P, I64, I32, P]),
     "lgbmhip_last_timings": (I32, [P, P]),
+    "lgbmhip_add_valid": (I32, [P, P, I64, P, P, P, I32, P, C.POINTER(I32)]),
+    "lgbmhip_copy_valid_scores": (I32, [P, I32, P]),
+    "lgbmhip_eval": (I32, [P, I32, C.POINTER(MetricOpts), P, C.POINTER(I32)]),
     "lgbmhip_profile": (I32, [P, I32, P]),
     "lgbmhip_set_tuning": (I32, [P, I32, I32, I32, I32, I32, I32]),
     "lgbmhip_get_tuning": (I32, [P, P]),

@@ -28,10 +28,12 @@ def build():
     os.makedirs(OUT_DIR, exist_ok=True)
     obj_bin = os.path.join(OUT_DIR, "binning.o")
     obj_eng = os.path.join(OUT_DIR, "engine.o")
+    obj_met = os.path.join(OUT_DIR, "metrics.o")
     run(["g++", *COMMON, "-Wall", "-c", os.path.join(CSRC, "binning.cpp"), "-o", obj_bin, "-pthread"])
     run([HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "engine.hip"), "-o", obj_eng])
+    run([HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "metrics.hip"), "-o", obj_met])
     tmp = LIB + ".tmp"
-    run([HIPCC, "-shared", *DEVICE, obj_eng, obj_bin, "-o", tmp, "-pthread"])
+    run([HIPCC, "-shared", *DEVICE, obj_eng, obj_met, obj_bin, "-o", tmp, "-pthread"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "metrics.h"
 #include "kernels.hip"
 #include "grow.hip"
 
@@ -208,6 +209,22 @@ struct Lane {
 // default number of hardware queues per process.
 constexpr int kMaxLanes = 4;
 
+// A validation set (boosting.rs:378-411): rows binned with the training
+// data's mappers, row-major [n][S] like the training bins, and its scores
+// [K][n], which every tree's score walk also updates (boosting.rs:1381-1408).
+struct ValidData {
+  int64_t n = 0;
+  DevBuf<uint8_t> bins;
+  DevBuf<double> score, init;
+  bool has_init = false;
+  std::vector<float> label;
+  DevBuf<float> d_label, d_weight;
+  bool has_weight = false;
+  double sum_weights = 0.0;
+  std::vector<int32_t> qb;
+  DevBuf<int> d_qb;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -278,6 +295,10 @@ struct Ctx {
   DevBuf<float> r_lam, r_hes, r_bl;
   DevBuf<double> r_bs;
   double r_min_in = 0, r_max_in = 0, r_factor = 0;
+  // validation sets and the metric workspace
+  std::vector<std::unique_ptr<ValidData>> valid;
+  MetricWs* mws = nullptr;
+  DevBuf<int> d_qb_eval;   // training query boundaries for NDCG when the objective keeps none
   // prediction cache
   bool pred_dirty = true;
   DevBuf<uint4> p_nodes;
@@ -296,6 +317,7 @@ struct Ctx {
 
   ~Ctx() {
     lanes.clear();
+    if (mws) metric_ws_free(mws);
     if (grad_ev) (void)hipEventDestroy(grad_ev);
     for (auto e : pev) if (e) (void)hipEventDestroy(e);
     for (auto g : graphs) if (g) (void)hipGraphExecDestroy(g);
@@ -468,6 +490,17 @@ static void setup_rank(Ctx* c) {
   HIP_CHECK(hipMemcpy(c->r_disc.p, disc.data(), 10000 * 8, hipMemcpyHostToDevice));
 }
 
+// A validation set's scores back to their start: its init scores or zeros
+// (boosting.rs:389-396).
+static void reset_valid_scores(Ctx* c, ValidData& v) {
+  const size_t cnt = (size_t)c->K * v.n;
+  if (v.has_init && v.init.n != cnt)
+    throw Error(LGBMHIP_INVALID_DATA, "validation init_score size does not match num_class x rows");
+  if (v.score.n != cnt) v.score.alloc(cnt);
+  if (v.has_init) HIP_CHECK(hipMemcpyAsync(v.score.p, v.init.p, cnt * 8, hipMemcpyDeviceToDevice, c->stream));
+  else v.score.zero(c->stream);
+}
+
 static void setup_training(Ctx* c) {
   if (!c->N) throw Error(LGBMHIP_INVALID_DATA, "no bins uploaded");
   if (c->label.empty()) throw Error(LGBMHIP_INVALID_DATA, "no labels set");
@@ -614,6 +647,7 @@ static void setup_training(Ctx* c) {
   c->trees.clear();
   c->iter = 0;
   c->pred_dirty = true;
+  for (auto& v : c->valid) reset_valid_scores(c, *v);
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->state_ready = true;
 }
@@ -813,6 +847,14 @@ static void enqueue_tail(Ctx* c, Lane& ln, int k, bool score_update, double lr) 
     w.val = c->leafval.p + (size_t)k * c->L;
     w.score = c->score.p + (size_t)k * N;
     launch_walk(w, wgrid, s);
+    // every validation set: the same walk over its bins (tree.rs:635-652)
+    for (auto& v : c->valid) {
+      WalkArgs wv = w;
+      wv.bins = v->bins.p;
+      wv.N = v->n;
+      wv.score = v->score.p + (size_t)k * v->n;
+      launch_walk(wv, grid_for(v->n, 256, c->cus * 4), s);
+    }
   }
 }
 
@@ -1082,6 +1124,9 @@ static int train_iteration(Ctx* c) {
       if (std::fabs(v) > (double)1e-15f) {
         hipLaunchKernelGGL(k_add_const, dim3(grid_for(N, 256, 2048)), dim3(256), 0, c->stream,
                            c->score.p + (size_t)k * N, (long long)N, v);
+        for (auto& vd : c->valid)
+          hipLaunchKernelGGL(k_add_const, dim3(grid_for(vd->n, 256, 2048)), dim3(256), 0, c->stream,
+                             vd->score.p + (size_t)k * vd->n, (long long)vd->n, v);
         init[k] = v;
       }
     }
@@ -1118,6 +1163,17 @@ static int train_iteration(Ctx* c) {
       t.shrink(c->opts.learning_rate);
       if (std::fabs(init[k]) > (double)1e-15f) t.add_bias(init[k]);
     } else if ((int)c->trees.size() < K) {
+      if (!c->opts.boost_from_average) {
+        // boosting.rs:1118-1126: a first tree without a split starts the
+        // scores from the objective's score
+        init[k] = boost_from_score(c, k);
+        hipLaunchKernelGGL(k_add_const, dim3(grid_for(N, 256, 2048)), dim3(256), 0, c->stream,
+                           c->score.p + (size_t)k * N, (long long)N, init[k]);
+        for (auto& vd : c->valid)
+          hipLaunchKernelGGL(k_add_const, dim3(grid_for(vd->n, 256, 2048)), dim3(256), 0, c->stream,
+                             vd->score.p + (size_t)k * vd->n, (long long)vd->n, init[k]);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+      }
       t.as_constant(init[k], (int)N);
     } else {
       t.as_constant(0.0, (int)N);
@@ -1266,6 +1322,203 @@ static void launch_predict(Ctx* c, const float* dX, const uint8_t* dbins, int64_
     hipLaunchKernelGGL(kern, grid, dim3(kPredRows), 0, c->stream, a);
   }
   HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- validation sets and metrics
+// Gbdt::add_valid (boosting.rs:378-411): bins checked and stored like the
+// training bins, scores from init_score (or zeros), then every tree trained
+// so far added one at a time in model order (tree.rs:635-652).
+static int add_valid(Ctx* c, const uint8_t* bins, int64_t n, const float* label, const float* weight,
+                     const int32_t* qb, int32_t nq, const double* init_score) {
+  if (!c->N) throw Error(LGBMHIP_INVALID_DATA, "upload the training bins first");
+  if (!c->opts_set) throw Error(LGBMHIP_INVALID_PARAMETER, "set the train options first");
+  if (n <= 0 || n > INT32_MAX) throw Error(LGBMHIP_INVALID_DATA, "validation rows out of range");
+  if (!bins || !label) throw Error(LGBMHIP_INVALID_PARAMETER, "validation bins and labels are required");
+  set_device(c);
+  const int K = c->opts.objective == 1 ? c->opts.num_class : 1;
+  const int F = c->F;
+  for (int64_t i = 0; i < n * F; ++i)
+    if (bins[i] >= (uint8_t)c->metas[i % F].num_bin && c->metas[i % F].num_bin < 256)
+      throw Error(LGBMHIP_INVALID_DATA, "validation bin value >= num_bin");
+  if (qb) {
+    if (nq <= 0 || qb[0] != 0 || qb[nq] != (int32_t)n) throw Error(LGBMHIP_INVALID_DATA, "query boundaries do not cover the validation data");
+    for (int q = 0; q < nq; ++q)
+      if (qb[q + 1] < qb[q]) throw Error(LGBMHIP_INVALID_DATA, "query boundaries must not decrease");
+  }
+  auto v = std::make_unique<ValidData>();
+  v->n = n;
+  std::vector<uint8_t> rm((size_t)n * c->S, 0);
+  for (int64_t r = 0; r < n; ++r) std::memcpy(&rm[(size_t)r * c->S], bins + (size_t)r * F, F);
+  v->bins.alloc(rm.size());
+  HIP_CHECK(hipMemcpy(v->bins.p, rm.data(), rm.size(), hipMemcpyHostToDevice));
+  v->label.assign(label, label + n);
+  v->d_label.alloc(n);
+  HIP_CHECK(hipMemcpy(v->d_label.p, label, n * 4, hipMemcpyHostToDevice));
+  v->has_weight = weight != nullptr;
+  // Metric::new (metric.rs:238-247): the weight total in row order
+  v->sum_weights = (double)n;
+  if (weight) {
+    v->d_weight.alloc(n);
+    HIP_CHECK(hipMemcpy(v->d_weight.p, weight, n * 4, hipMemcpyHostToDevice));
+    double sw = 0.0;
+    for (int64_t i = 0; i < n; ++i) sw += (double)weight[i];
+    v->sum_weights = sw;
+  }
+  if (qb) {
+    v->qb.assign(qb, qb + nq + 1);
+    v->d_qb.alloc(nq + 1);
+    HIP_CHECK(hipMemcpy(v->d_qb.p, qb, (nq + 1) * 4, hipMemcpyHostToDevice));
+  }
+  const size_t cnt = (size_t)K * n;
+  v->has_init = init_score != nullptr;
+  if (init_score) {
+    v->init.alloc(cnt);
+    HIP_CHECK(hipMemcpy(v->init.p, init_score, cnt * 8, hipMemcpyHostToDevice));
+  }
+  const int saved_k = c->K;
+  c->K = K;
+  reset_valid_scores(c, *v);
+  c->K = saved_k;
+  if (c->state_ready && !c->trees.empty()) {
+    if (c->pred_dirty || !c->p_binned) build_pred(c, true);
+    const int words = (F + 3) / 4, sw = words | 1;
+    const size_t lds = (size_t)kPredRows * sw * 4;
+    if (lds > (size_t)c->lds_max) throw Error(LGBMHIP_UNSUPPORTED, "validation rows too wide to stage");
+    auto kern = k_predict<true, true, LGBMHIP_PRED_BIN_W>;
+    HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const dim3 grid((unsigned)((n + kPredRows - 1) / kPredRows));
+    // the prediction kernel reads rows of F bytes (the stored rows have stride S)
+    DevBuf<uint8_t> packed;
+    packed.alloc((size_t)n * F);
+    HIP_CHECK(hipMemcpy(packed.p, bins, (size_t)n * F, hipMemcpyHostToDevice));
+    for (int t = 0; t < c->p_T; ++t) {
+      PredArgs a{};
+      a.nodes = c->p_nodes.p;
+      a.tree_node_off = c->p_node_off.p + t;
+      a.tree_leaf_off = c->p_leaf_off.p + t;
+      a.tree_num_leaves = c->p_nleaves.p + t;
+      a.leaf_val = c->p_leafval.p;
+      a.cat_words = c->p_cat.p;
+      a.T = 1;
+      a.K = 1;
+      a.rows = n;
+      a.ncol = F;
+      a.stride = sw * 4;
+      a.bins = packed.p;
+      a.out = v->score.p + (size_t)(t % K) * n;
+      a.accum = 1;
+      hipLaunchKernelGGL(kern, grid, dim3(kPredRows * LGBMHIP_PRED_BIN_W), lds, c->stream, a);
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->valid.push_back(std::move(v));
+  // captured trees hold the score walks of the sets they were captured with
+  for (auto& g : c->graphs) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  for (auto& g : c->graphs_cont) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  return (int)c->valid.size();
+}
+
+// Metric ids of lgbmhip_eval.
+enum { kMetricBinaryLogloss = 0, kMetricBinaryError = 1, kMetricAuc = 2, kMetricMultiLogloss = 3,
+       kMetricMultiError = 4, kMetricNdcg = 5 };
+
+// Metric::eval (metric.rs:525-545) on the scores of the training data
+// (data 0) or validation set data.
+static int eval_metric(Ctx* c, int data, const lgbmhip_metric_opts* o, double* out) {
+  if (!c->state_ready) setup_training(c);
+  set_device(c);
+  if (data < 0 || data > (int)c->valid.size()) throw Error(LGBMHIP_INVALID_PARAMETER, "no such data set");
+  const int K = c->K;
+  const double* score;
+  const float *label, *weight;
+  const std::vector<float>* hlabel;
+  const std::vector<int32_t>* hqb;
+  const int* dqb = nullptr;
+  int64_t n;
+  double sum_weights;
+  if (data == 0) {
+    n = c->N;
+    score = c->score.p;
+    label = c->d_label.p;
+    weight = c->has_weight ? c->d_weight.p : nullptr;
+    hlabel = &c->label;
+    hqb = &c->qb;
+    sum_weights = (double)n;
+    if (c->has_weight) {
+      std::vector<float> w(n);
+      HIP_CHECK(hipMemcpy(w.data(), c->d_weight.p, n * 4, hipMemcpyDeviceToHost));
+      double sw = 0.0;
+      for (int64_t i = 0; i < n; ++i) sw += (double)w[i];
+      sum_weights = sw;
+    }
+    if (!c->qb.empty()) {
+      if (c->d_qb_eval.n != c->qb.size()) {
+        c->d_qb_eval.alloc(c->qb.size());
+        HIP_CHECK(hipMemcpy(c->d_qb_eval.p, c->qb.data(), c->qb.size() * 4, hipMemcpyHostToDevice));
+      }
+      dqb = c->d_qb_eval.p;
+    }
+  } else {
+    const ValidData& v = *c->valid[data - 1];
+    n = v.n;
+    score = v.score.p;
+    label = v.d_label.p;
+    weight = v.has_weight ? v.d_weight.p : nullptr;
+    hlabel = &v.label;
+    hqb = &v.qb;
+    dqb = v.qb.empty() ? nullptr : v.d_qb.p;
+    sum_weights = v.sum_weights;
+  }
+  if (!c->mws) c->mws = metric_ws_new();
+  const int obj = c->opts.objective;
+  const double nle = -(double)std::log(1e-15f);   // consts.rs:12-14, the single-precision log
+  switch (o->metric) {
+    case kMetricBinaryLogloss:
+    case kMetricBinaryError:
+      if (obj != 0) throw Error(LGBMHIP_UNSUPPORTED, "binary metrics need the binary objective");
+      out[0] = metric_point(c->mws, c->stream, o->metric == kMetricBinaryLogloss ? kBinaryLogloss : kBinaryError,
+                            score, label, weight, n, 1, c->opts.sigmoid, 0, nle, sum_weights);
+      return 1;
+    case kMetricAuc:
+      if (K != 1) throw Error(LGBMHIP_UNSUPPORTED, "AUC of a multiclass model");
+      out[0] = metric_auc(c->mws, c->stream, score, label, weight, n, sum_weights);
+      return 1;
+    case kMetricMultiLogloss:
+    case kMetricMultiError: {
+      if (obj != 1) throw Error(LGBMHIP_UNSUPPORTED, "multiclass metrics need the multiclass objective");
+      for (float l : *hlabel)
+        if (!(l >= 0.0f && (int)l < K)) throw Error(LGBMHIP_INVALID_DATA, "multiclass label out of [0, num_class)");
+      out[0] = metric_point(c->mws, c->stream, o->metric == kMetricMultiLogloss ? kMultiLogloss : kMultiError, score,
+                            label, weight, n, K, 1.0, o->multi_error_top_k, nle, sum_weights);
+      return 1;
+    }
+    case kMetricNdcg: {
+      if (K != 1) throw Error(LGBMHIP_UNSUPPORTED, "NDCG of a multiclass model");
+      if (!dqb || hqb->size() < 2) throw Error(LGBMHIP_INVALID_DATA, "The NDCG metric requires query information");
+      if (weight) throw Error(LGBMHIP_UNSUPPORTED, "NDCG with row weights");
+      // dcg.rs:27-52: default eval_at 1..5 and label gains 2^i - 1
+      std::vector<int> at = o->num_eval_at > 0 ? std::vector<int>(o->eval_at, o->eval_at + o->num_eval_at)
+                                               : std::vector<int>{1, 2, 3, 4, 5};
+      std::vector<double> gain;
+      if (o->num_label_gain > 0) gain.assign(o->label_gain, o->label_gain + o->num_label_gain);
+      else {
+        gain.push_back(0.0);
+        for (int i = 1; i < 31; ++i) gain.push_back((double)((1 << i) - 1));
+      }
+      for (float l : *hlabel)
+        if (!(l >= 0.0f) || (float)(int)l != l || (size_t)l >= gain.size())
+          throw Error(LGBMHIP_INVALID_DATA, "NDCG labels must be integers below the label gain count");
+      std::vector<double> disc(10000);
+      for (int i = 0; i < 10000; ++i) disc[i] = 1.0 / std::log2(2.0 + (double)i);
+      metric_ndcg(c->mws, c->stream, score, label, dqb, (int)hqb->size() - 1, at.data(), (int)at.size(), gain.data(),
+                  (int)gain.size(), disc.data(), (int)disc.size(), o->num_threads, out);
+      return (int)at.size();
+    }
+    default:
+      throw Error(LGBMHIP_INVALID_PARAMETER, "unknown metric id");
+  }
 }
 
 static void predict(Ctx* c, const float* X, const uint8_t* bins, int64_t rows, int ncol, double* out) {
@@ -1754,6 +2007,32 @@ lgbmhip_status lgbmhip_predict_device(lgbmhip_ctx* ctx, const float* d_X, const 
     set_device(c);
     launch_predict(c, d_X, d_bins, rows, ncol, d_out);
     HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+lgbmhip_status lgbmhip_add_valid(lgbmhip_ctx* ctx, const uint8_t* bins, int64_t rows, const float* label,
+                                 const float* weight, const int32_t* query_boundaries, int32_t num_queries,
+                                 const double* init_score, int32_t* out_index) {
+  return guarded(ctx, [&] {
+    const int idx = add_valid(&ctx->c, bins, rows, label, weight, query_boundaries, num_queries, init_score);
+    if (out_index) *out_index = idx;
+  });
+}
+lgbmhip_status lgbmhip_copy_valid_scores(lgbmhip_ctx* ctx, int32_t index, double* out) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (index < 1 || index > (int32_t)c->valid.size()) throw Error(LGBMHIP_INVALID_PARAMETER, "no such validation set");
+    set_device(c);
+    const ValidData& v = *c->valid[index - 1];
+    HIP_CHECK(hipMemcpyAsync(out, v.score.p, v.score.n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+lgbmhip_status lgbmhip_eval(lgbmhip_ctx* ctx, int32_t data, const lgbmhip_metric_opts* opts, double* out,
+                            int32_t* out_count) {
+  return guarded(ctx, [&] {
+    if (!opts || !out) throw Error(LGBMHIP_INVALID_PARAMETER, "metric options and output are required");
+    const int cnt = eval_metric(&ctx->c, data, opts, out);
+    if (out_count) *out_count = cnt;
   });
 }
 lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out12) {

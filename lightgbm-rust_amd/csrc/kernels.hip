@@ -1215,6 +1215,7 @@ struct PredArgs {
   const float* X;             // raw rows (row-major f32)
   const uint8_t* bins;        // binned rows (row-major u8)
   double* out;                // [rows][K]
+  int accum;                  // 1: out[row][k] += the trees' sum (one tree: its leaf value, exactly)
 };
 
 constexpr int kPredRows = 64;   // rows per block (one wave, one row per lane)
@@ -1303,7 +1304,8 @@ __global__ void __launch_bounds__(kPredRows * W) k_predict(PredArgs a) {
     else xr = a.X + rr * a.ncol;
   }
   for (int k = 0; k < a.K; ++k) {
-    double acc = 0.0;
+    // -0.0 + v == v for every v, so an accumulated single tree adds its leaf value unchanged
+    double acc = a.accum ? -0.0 : 0.0;
     const int nk = a.T > k ? (a.T - k + a.K - 1) / a.K : 0;   // trees of class k
     for (int c0 = 0; c0 < nk; c0 += TC) {
       int node[G], base[G];
@@ -1347,7 +1349,10 @@ __global__ void __launch_bounds__(kPredRows * W) k_predict(PredArgs a) {
         __syncthreads();
       }
     }
-    if (wave == 0 && lane < nrow) a.out[(r0 + lane) * a.K + k] = acc;
+    if (wave == 0 && lane < nrow) {
+      double* o = a.out + (r0 + lane) * a.K + k;
+      *o = a.accum ? *o + acc : acc;
+    }
   }
 }
 

@@ -20,6 +20,9 @@
 //   binary/multiclass gradients: closed forms (binary.rs:101-127, multiclass.rs:126-154)
 //   lambdarank gradients       : two-document known answer (objective/rank.rs:491-509)
 //   AUC                        : sklearn roc_auc_score on tie-free and tied inputs
+//   metric_auc (weighted)      : sklearn roc_auc_score(sample_weight) + brute force over pairs
+//   metric_point               : sklearn log_loss / accuracy (binary and multiclass, weighted)
+//   metric_ndcg                : dcg.rs:195-209 known answers + sklearn ndcg_score (gains 2^rel-1)
 //   full-pipeline model text   : parity unpinned (no committed artifact in the reference)
 
 #include <algorithm>
@@ -2225,6 +2228,152 @@ double orc_binary_logloss(const double* raw, const float* label, int64_t n, doub
     sum += l;
   }
   return sum / (double)n;
+}
+
+// ---------------------------------------------------------------- metrics
+// Metric::new / Metric::auc (metric.rs:238-247, 701-729): the weight total in
+// row order; documents by descending score (ties in index order, one of the
+// orders the reference's unstable sort may produce); runs of equal scores.
+double orc_metric_auc(const double* score, const float* label, const float* weight, int64_t n) {
+  double sum_weights = (double)n;
+  if (weight) {
+    sum_weights = 0.0;
+    for (int64_t i = 0; i < n; ++i) sum_weights += (double)weight[i];
+  }
+  std::vector<int64_t> idx(n);
+  for (int64_t i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return score[a] > score[b]; });
+  double cur_pos = 0.0, sum_pos = 0.0, accum = 0.0, cur_neg = 0.0;
+  double threshold = score[idx[0]];
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t i = idx[k];
+    const double s = score[i];
+    if (s != threshold) {
+      threshold = s;
+      accum += cur_neg * (cur_pos * 0.5 + sum_pos);
+      sum_pos += cur_pos;
+      cur_neg = 0.0;
+      cur_pos = 0.0;
+    }
+    const double w = weight ? (double)weight[i] : 1.0;
+    cur_neg += (double)(label[i] <= 0.0f ? 1 : 0) * w;
+    cur_pos += (double)(label[i] > 0.0f ? 1 : 0) * w;
+  }
+  accum += cur_neg * (cur_pos * 0.5 + sum_pos);
+  sum_pos += cur_pos;
+  return (sum_pos > 0.0 && sum_pos != sum_weights) ? accum / (sum_pos * (sum_weights - sum_pos)) : 1.0;
+}
+
+// metric.rs:602-625 (binary, binary.rs:164-166 sigmoid) and :629-653 with
+// multiclass_loss :133-153 (softmax multiclass.rs:16-30). kind: 0
+// binary_logloss, 1 binary_error, 2 multi_logloss, 3 multi_error. score is
+// class-major [K][n].
+double orc_metric_point(int kind, const double* score, const float* label, const float* weight, int64_t n, int K,
+                        double sigmoid, int top_k) {
+  double sum_weights = (double)n;
+  if (weight) {
+    sum_weights = 0.0;
+    for (int64_t i = 0; i < n; ++i) sum_weights += (double)weight[i];
+  }
+  const double nle = -(double)std::log(1e-15f);
+  double sum = 0.0;
+  std::vector<double> raw(K), rec(K);
+  for (int64_t i = 0; i < n; ++i) {
+    double l;
+    if (kind <= 1) {
+      const double p = 1.0 / (1.0 + std::exp(-sigmoid * score[i]));
+      if (kind == 0) {
+        if (label[i] <= 0.0f) l = 1.0 - p > K_EPSILON ? -std::log(1.0 - p) : nle;
+        else l = p > K_EPSILON ? -std::log(p) : nle;
+      } else {
+        l = p <= 0.5 ? (double)(label[i] > 0.0f ? 1 : 0) : (double)(label[i] <= 0.0f ? 1 : 0);
+      }
+    } else {
+      for (int c = 0; c < K; ++c) raw[c] = score[(size_t)c * n + i];
+      double wmax = raw[0];
+      for (int c = 1; c < K; ++c) wmax = raw[c] < wmax ? wmax : raw[c];
+      double wsum = 0.0;
+      for (int c = 0; c < K; ++c) {
+        rec[c] = std::exp(raw[c] - wmax);
+        wsum += rec[c];
+      }
+      for (int c = 0; c < K; ++c) rec[c] /= wsum;
+      const int k = (int)label[i];
+      if (kind == 2) {
+        l = rec[k] > K_EPSILON ? -std::log(rec[k]) : nle;
+      } else {
+        int larger = 0;
+        l = 0.0;
+        for (int c = 0; c < K; ++c) {
+          if (rec[c] >= rec[k]) ++larger;
+          if (larger > top_k) {
+            l = 1.0;
+            break;
+          }
+        }
+      }
+    }
+    sum += weight ? l * (double)weight[i] : l;
+  }
+  return sum / sum_weights;
+}
+
+// Metric::rank_eval for NDCG (metric.rs:551-600) with DcgCalculator::dcg /
+// max_dcg (dcg.rs:88-118): per query, DCG at every cut-off of the ranking by
+// descending score (ties in index order), times 1 / max DCG (1 for a query
+// without relevant documents); libgomp static chunks of queries over
+// `threads`, chunk sums added in chunk order, divided by the query count.
+void orc_metric_ndcg(const double* score, const float* label, const int* qb, int nq, const int* eval_at, int nk,
+                     int threads, double* out) {
+  Dcg d;
+  threads = std::max(1, threads);
+  std::vector<std::vector<double>> bufs(threads, std::vector<double>(nk, 0.0));
+  std::vector<double> tmp(nk), inv(nk);
+  const int qc = nq / threads, r = nq % threads;
+  for (int t = 0; t < threads; ++t) {
+    const int start = t * qc + std::min(t, r), len = qc + (t < r ? 1 : 0);
+    for (int q = start; q < start + len; ++q) {
+      const int s0 = qb[q], cnt = qb[q + 1] - s0;
+      const float* lq = label + s0;
+      // max DCG at every cut-off (dcg.rs:88-103)
+      std::vector<int> c(d.label_gain.size(), 0);
+      for (int i = 0; i < cnt; ++i) c[(int)lq[i]]++;
+      int top = (int)d.label_gain.size() - 1, left = 0;
+      double cur = 0.0;
+      for (int m = 0; m < nk; ++m) {
+        const int k = std::min(eval_at[m], cnt);
+        for (int j = left; j < k; ++j) {
+          while (top > 0 && c[top] <= 0) top--;
+          cur += d.discount[j] * d.label_gain[top];
+          c[top]--;
+        }
+        inv[m] = cur > 0.0 ? 1.0 / cur : -1.0;
+        left = k;
+      }
+      if (inv[0] <= 0.0) {
+        for (int m = 0; m < nk; ++m) bufs[t][m] += 1.0;
+        continue;
+      }
+      // DCG at every cut-off (dcg.rs:105-118)
+      std::vector<int> idx(cnt);
+      for (int i = 0; i < cnt; ++i) idx[i] = i;
+      std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return score[s0 + a] > score[s0 + b]; });
+      cur = 0.0;
+      left = 0;
+      for (int m = 0; m < nk; ++m) {
+        const int k = std::min(eval_at[m], cnt);
+        for (int j = left; j < k; ++j) cur += d.label_gain[(int)lq[idx[j]]] * d.discount[j];
+        tmp[m] = cur;
+        left = k;
+      }
+      for (int m = 0; m < nk; ++m) bufs[t][m] += tmp[m] * inv[m];
+    }
+  }
+  for (int m = 0; m < nk; ++m) {
+    double v = 0.0;
+    for (int t = 0; t < threads; ++t) v += bufs[t][m];
+    out[m] = v / (double)nq;
+  }
 }
 
 }  // extern "C"

@@ -76,6 +76,9 @@ def lib():
             "orc_ndcg": (D, [PD, PF, PI, I, I]),
             "orc_max_dcg_at_k": (D, [PF, I, I]),
             "orc_binary_logloss": (D, [PD, PF, I64, D]),
+            "orc_metric_auc": (D, [PD, PF, P, I64]),
+            "orc_metric_point": (D, [I, PD, PF, P, I64, I, D, I]),
+            "orc_metric_ndcg": (None, [PD, PF, PI, I, PI, I, I, PD]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -347,6 +350,39 @@ def quant_tables(n, seed, count):
 
 def auc(score, label):
     return lib().orc_auc(np.ascontiguousarray(score, np.float64), np.ascontiguousarray(label, np.float32), len(label))
+
+
+def _f32_or_none(w):
+    return None if w is None else np.ascontiguousarray(w, np.float32)
+
+
+def metric_auc(score, label, weight=None):
+    """metric.rs:701-729 (weighted when weight is given)."""
+    w = _f32_or_none(weight)
+    return lib().orc_metric_auc(np.ascontiguousarray(score, np.float64), np.ascontiguousarray(label, np.float32),
+                                None if w is None else w.ctypes.data, len(label))
+
+
+POINT_KINDS = {"binary_logloss": 0, "binary_error": 1, "multi_logloss": 2, "multi_error": 3}
+
+
+def metric_point(kind, score, label, weight=None, K=1, sigmoid=1.0, top_k=1):
+    """metric.rs:602-653; score class-major [K][n]."""
+    w = _f32_or_none(weight)
+    return lib().orc_metric_point(POINT_KINDS[kind], np.ascontiguousarray(score, np.float64).ravel(),
+                                  np.ascontiguousarray(label, np.float32),
+                                  None if w is None else w.ctypes.data, len(label), K,
+                                  sigmoid, top_k)
+
+
+def metric_ndcg(score, label, qb, eval_at=(1, 2, 3, 4, 5), threads=1):
+    """metric.rs:551-600: NDCG at each cut-off."""
+    at = np.ascontiguousarray(eval_at, np.int32)
+    qb = np.ascontiguousarray(qb, np.int32)
+    out = np.zeros(len(at), np.float64)
+    lib().orc_metric_ndcg(np.ascontiguousarray(score, np.float64), np.ascontiguousarray(label, np.float32), qb,
+                          len(qb) - 1, at, len(at), threads, out)
+    return out
 
 
 def binary_logloss(raw, label, sigmoid=1.0):

@@ -441,3 +441,81 @@ def test_ndcg_matches_sklearn_on_tie_free_queries():
             continue
         ref.append(ndcg_score([2.0 ** lab[a:b] - 1], [s[a:b]], k=10))
     assert abs(ours - np.mean(ref)) < 1e-12
+
+
+# ---------------------------------------------------------------- metrics (metric.rs:551-729)
+def test_metric_auc_matches_sklearn_and_pairs():
+    from sklearn.metrics import roc_auc_score
+    rng = np.random.default_rng(30)
+    for n, ties in ((500, False), (3000, True)):
+        s = rng.normal(size=n)
+        if ties:
+            s = np.round(s, 1)
+        y = (rng.random(n) < 0.35).astype(np.float32)
+        w = (rng.random(n) * 3).astype(np.float32)
+        assert O.metric_auc(s, y) == pytest.approx(roc_auc_score(y, s), rel=1e-13)
+        assert O.metric_auc(s, y, w) == pytest.approx(roc_auc_score(y, s, sample_weight=w), rel=1e-12)
+    # brute force over (positive, negative) pairs, ties count one half
+    s = np.round(rng.normal(size=60), 1)
+    y = (rng.random(60) < 0.5).astype(np.float32)
+    w = rng.random(60).astype(np.float32)
+    pos, neg = np.where(y > 0)[0], np.where(y <= 0)[0]
+    num = sum(float(w[p]) * float(w[q]) * (1.0 if s[p] > s[q] else 0.5 if s[p] == s[q] else 0.0)
+              for p in pos for q in neg)
+    den = float(w[pos].astype(np.float64).sum()) * float(w[neg].astype(np.float64).sum())
+    assert O.metric_auc(s, y, w) == pytest.approx(num / den, rel=1e-12)
+    # one class only: 1 (metric.rs:724-728)
+    assert O.metric_auc(s, np.ones(60, np.float32)) == 1.0
+    # -0.0 and +0.0 are one score
+    assert O.metric_auc(np.array([0.0, -0.0]), np.array([1, 0], np.float32)) == 0.5
+
+
+def test_metric_point_matches_sklearn():
+    from sklearn.metrics import log_loss
+    rng = np.random.default_rng(31)
+    n = 4000
+    s = rng.normal(size=n) * 2
+    y = (rng.random(n) < 0.5).astype(np.float32)
+    w = (rng.random(n) + 0.2).astype(np.float32)
+    p = 1.0 / (1.0 + np.exp(-s))
+    assert O.metric_point("binary_logloss", s, y, w) == pytest.approx(log_loss(y, p, sample_weight=w), rel=1e-12)
+    err = np.where(p <= 0.5, y > 0, y <= 0).astype(np.float64)
+    assert O.metric_point("binary_error", s, y) == err.mean()
+    # the clamp: -log(float eps) when the probability underflows
+    big = O.metric_point("binary_logloss", np.array([800.0]), np.array([0.0], np.float32))
+    assert big == -float(np.log(np.float32(1e-15)))
+    K = 4
+    S = rng.normal(size=(K, n))
+    yk = rng.integers(0, K, n).astype(np.float32)
+    P = np.exp(S - S.max(0)) / np.exp(S - S.max(0)).sum(0)
+    assert O.metric_point("multi_logloss", S, yk, K=K) == pytest.approx(
+        log_loss(yk, P.T, labels=list(range(K))), rel=1e-12)
+    rank = (P >= P[yk.astype(int), np.arange(n)]).sum(0)
+    for top_k in (1, 2, 3):
+        assert O.metric_point("multi_error", S, yk, K=K, top_k=top_k) == (rank > top_k).mean()
+
+
+def test_metric_ndcg_known_answers_and_sklearn():
+    from sklearn.metrics import ndcg_score
+    # dcg.rs:195-209: labels [0, 2, 1], scores [1, 0, 0] (the tie keeps index order)
+    lab = np.array([0, 2, 1], np.float32)
+    out = O.metric_ndcg(np.array([1.0, 0.0, 0.0]), lab, [0, 3], (1, 3))
+    ideal = 3.0 + 1.0 / np.log2(3.0)
+    assert out[0] == 0.0
+    assert out[1] == pytest.approx((3.0 / np.log2(3.0) + 1.0 / np.log2(4.0)) / ideal, rel=1e-15)
+    # a query without relevant documents counts 1
+    assert O.metric_ndcg(np.zeros(4), np.zeros(4, np.float32), [0, 4], (1, 2)).tolist() == [1.0, 1.0]
+    rng = np.random.default_rng(32)
+    sizes = [5, 12, 30, 7, 50]
+    qb = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    lab = rng.integers(0, 5, qb[-1]).astype(np.float32)
+    lab[qb[1]] = 4
+    s = rng.normal(size=qb[-1])
+    at = (1, 3, 10)
+    ours = O.metric_ndcg(s, lab, qb, at)
+    for m, k in enumerate(at):
+        ref = [ndcg_score([2.0 ** lab[a:b] - 1], [s[a:b]], k=k) if lab[a:b].max() > 0 else 1.0
+               for a, b in zip(qb[:-1], qb[1:])]
+        assert ours[m] == pytest.approx(np.mean(ref), rel=1e-12)
+    # per-thread partial sums change only the rounding
+    assert O.metric_ndcg(s, lab, qb, at, threads=3) == pytest.approx(ours, rel=1e-14)
