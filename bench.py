@@ -259,8 +259,72 @@ def workload_predict(lgb, datagen, args):
                                frac=m["gbs"] / HBM_PEAK_GBS, traffic=None, kernel="k_predict<binned>"))
 
 
+def workload_eval(lgb, datagen, args):
+    """Boosting with evaluation on the GPU: the headline quantized binary
+    model on --rows x 28 with a 1,000,000-row validation set attached, whose
+    scores every tree updates on the device. One step is one iteration plus
+    the validation AUC (what early stopping reads each iteration). Also
+    reports AUC and logloss over the training scores, and the oracle's AUC
+    (one CPU thread, metric.rs:701-729) over the same training scores."""
+    n, nv = args.rows, 1_000_000
+    t = time.perf_counter()
+    X, y = datagen.binary_10p5m(n)
+    Xv, yv = datagen.holdout("binary_10p5m", nv)
+    log(f"data {X.shape} + valid {Xv.shape} in {time.perf_counter() - t:.1f}s")
+    base = dict(objective="binary", num_leaves=args.num_leaves, max_bin=args.max_bin, min_data_in_leaf=20,
+                learning_rate=0.1, use_quantized_grad=True, num_grad_quant_bins=16, stochastic_rounding=True,
+                quant_train_renew_leaf=True, metric="auc")
+    ds = lgb.Dataset(X, label=y, params=base)
+    dv = lgb.Dataset(Xv, label=yv, reference=ds, params=base)
+    del X
+    b = lgb.Booster(base, ds)
+    b.add_valid(dv, "valid")
+    for _ in range(args.warmup):
+        b.update()
+        b.eval_valid()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.update()
+        res = b.eval_valid()
+    dt = (time.perf_counter() - t0) / args.steps
+    log(f"iteration + valid AUC: {1 / dt:.2f} it/s ({dt * 1e3:.2f} ms), valid auc {res[0].metric_value:.6f}")
+    timings = {}
+    for m in ("auc", "binary_logloss"):
+        b._metrics = [m]
+        b.eval_train()
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            v = b.eval_train()[0].metric_value
+        timings[m] = dict(ms=(time.perf_counter() - t0) / reps * 1e3, value=v)
+        log(f"train {m} over {n} rows: {timings[m]['ms']:.2f} ms ({v:.9f})")
+    out = dict(iters_with_valid_auc_per_sec=1 / dt, ms_per_iter=dt * 1e3, valid_auc=res[0].metric_value,
+               train_metric_ms=timings)
+    if not args.no_cpu:
+        from oracle import pyoracle as O
+        O.build()
+        ts = b.train_scores()
+        t0 = time.perf_counter()
+        ref = O.metric_auc(ts, y)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_auc_s"] = cpu_s
+        out["cpu_auc_equal"] = bool(ref == timings["auc"]["value"])
+        log(f"oracle AUC over {n} rows: {cpu_s:.2f} s, equal: {out['cpu_auc_equal']}")
+    auc_ms = timings["auc"]["ms"]
+    byts = n * (8 + 4)   # one read of the scores and labels
+    return _line("boosting iters/sec with a 1M-row validation set and its AUC every iteration", 1 / dt, "iters/sec",
+                 args, dt * 1e3, "int64",
+                 {"workload": f"binary logloss GBDT, {n} x 28, num_leaves={args.num_leaves}, max_bin={args.max_bin}, "
+                              f"quantized (16 levels), validation set {nv} rows, metric=auc",
+                  "rows": n, "valid_rows": nv, "parallelism": "1 GPU"},
+                 eval=out,
+                 roofline=dict(bound="hbm", achieved=byts / auc_ms / 1e6, peak=HBM_PEAK_GBS, unit="GB/s",
+                               frac=byts / auc_ms / 1e6 / HBM_PEAK_GBS, traffic=None,
+                               kernel="AUC over the training scores (radix sort of 64-bit keys and runs)"))
+
+
 WORKLOADS = {"binary_10p5m": None, "multiclass_2m": workload_multiclass, "rank_2m": workload_rank,
-             "predict_1m": workload_predict}
+             "predict_1m": workload_predict, "eval_10p5m": workload_eval}
 
 
 def main():

@@ -12,6 +12,9 @@
  * crates/lgbm-core/src/ of zrluety/lightgbm-rust):
  *   lgbmhip_status codes      <- LgbmError variants            error.rs:6-18
  *   lgbmhip_bin_dense         <- Dataset::from_matrix (binning) dataset.rs:360-422
+ *   lgbmhip_find_mappers      <- BinMapper::find_bin (sample)   dataset.rs:393-397, binning.rs:398-517
+ *   lgbmhip_upload_raw        <- Dataset::from_matrix push pass dataset.rs:594-663, binning.rs:637-667
+ *   lgbmhip_bin_rows_device   <- Dataset::from_matrix_with_reference (bins) dataset.rs:456-520
  *   lgbmhip_upload_bins       <- SerialTreeLearner::new         learner/mod.rs:220-310
  *   lgbmhip_set_metadata      <- Objective::init / Metadata     objective/mod.rs:43-102, dataset.rs:104-130
  *   lgbmhip_set_split_params  <- SplitParams                    learner/split.rs:33-46
@@ -134,6 +137,12 @@ lgbmhip_status lgbmhip_bins_matrix(const lgbmhip_bins* b, uint8_t* out);
 /* Bin raw values with the mappers of `b` (validation / prediction data). */
 lgbmhip_status lgbmhip_bins_apply(const lgbmhip_bins* b, const float* X, int64_t n, int32_t ncol, uint8_t* out);
 
+/* The mappers of lgbmhip_bin_dense without the bin matrix (the sample-based
+ * BinMapper search stays on the host); lgbmhip_upload_raw then bins the rows
+ * on the device. lgbmhip_bins_matrix fails on such a handle. */
+lgbmhip_status lgbmhip_find_mappers(const float* X, int64_t n, int32_t ncol, const int32_t* categorical,
+                                    int32_t n_cat, const lgbmhip_bin_params* params, lgbmhip_bins** out);
+
 /* ---- context ---- */
 lgbmhip_status lgbmhip_create(int32_t device, lgbmhip_ctx** out);
 void lgbmhip_destroy(lgbmhip_ctx* ctx);
@@ -141,6 +150,15 @@ void lgbmhip_destroy(lgbmhip_ctx* ctx);
 /* Uploads the binned matrix once (row-major and feature-major copies in HBM). */
 lgbmhip_status lgbmhip_upload_bins(lgbmhip_ctx* ctx, const lgbmhip_feature_meta* metas, int32_t num_features,
                                    const double* upper, const uint8_t* bins, int64_t num_rows, int32_t num_total_features);
+/* lgbmhip_upload_bins with the binning pass on the device: rows x ncol f32
+ * values (host memory, or device memory when x_on_device = 1) binned with
+ * the mappers of `mappers` (value_to_bin) straight into HBM. */
+lgbmhip_status lgbmhip_upload_raw(lgbmhip_ctx* ctx, const lgbmhip_bins* mappers, const float* X, int64_t rows,
+                                  int32_t ncol, int32_t x_on_device);
+/* Device rows (rows x ncol f32) binned with the uploaded training data's
+ * mappers into device u8 [rows][F] (the input of binned prediction). */
+lgbmhip_status lgbmhip_bin_rows_device(lgbmhip_ctx* ctx, const float* d_X, int64_t rows, int32_t ncol,
+                                       uint8_t* d_out);
 lgbmhip_status lgbmhip_set_metadata(lgbmhip_ctx* ctx, const float* label, const float* weight,
                                     const int32_t* query_boundaries, int32_t num_queries);
 lgbmhip_status lgbmhip_set_split_params(lgbmhip_ctx* ctx, const lgbmhip_split_params* p);

@@ -31,7 +31,7 @@ DEFAULT_PARAMS = dict(
     cat_l2=10.0, cat_smooth=10.0, min_data_per_group=100, lambdarank_truncation_level=30, lambdarank_norm=True,
     num_threads=0, device_type="gpu", gpu_device_id=0, use_graph=True, quant_table_threads=1,
     metric="", eval_at=(1, 2, 3, 4, 5), multi_error_top_k=1, label_gain=None, early_stopping_round=0,
-    first_metric_only=False, early_stopping_min_delta=0.0,
+    first_metric_only=False, early_stopping_min_delta=0.0, device_binning=False,
 )
 
 ALIASES = {
@@ -124,12 +124,14 @@ class Dataset:
             cats = [int(c) for c in cats.split(",") if c.strip()] if cats else []
         self.categorical = np.ascontiguousarray(cats or [], dtype=np.int32)
         self.reference = reference
+        self._raw = None
+        self._bins = None
         if reference is not None:
             self._handle = reference._handle
             self._owns = False
-            self.bins = np.zeros((self.num_data, reference.num_features), np.uint8)
+            self._bins = np.zeros((self.num_data, reference.num_features), np.uint8)
             check(lib().lgbmhip_bins_apply(self._handle, ptr(X), self.num_data, self.num_total_features,
-                                           ptr(self.bins)))
+                                           ptr(self._bins)))
             self.metas, self.upper = reference.metas, reference.upper
             self.num_features = reference.num_features
             return
@@ -139,17 +141,34 @@ class Dataset:
                              int(_b(p["feature_pre_filter"])), int(_b(p["use_missing"])),
                              int(_b(p["zero_as_missing"])), int(p["num_threads"]))
         h = C.c_void_p()
-        check(lib().lgbmhip_bin_dense(ptr(X), self.num_data, self.num_total_features,
-                                      ptr(self.categorical) if len(self.categorical) else None,
-                                      len(self.categorical), C.byref(bp), C.byref(h)))
+        # device_binning: only the sample-based mappers on the host; the
+        # Booster bins the rows on the GPU (lgbmhip_upload_raw)
+        device = _b(p["device_binning"])
+        find = lib().lgbmhip_find_mappers if device else lib().lgbmhip_bin_dense
+        check(find(ptr(X), self.num_data, self.num_total_features,
+                   ptr(self.categorical) if len(self.categorical) else None, len(self.categorical), C.byref(bp),
+                   C.byref(h)))
         self._handle = h
         self._owns = True
         self.num_features = lib().lgbmhip_bins_num_features(h)
         self.metas = (_capi.FeatureMeta * max(self.num_features, 1))()
         self.upper = np.zeros(max(lib().lgbmhip_bins_num_upper(h), 1), np.float64)
         check(lib().lgbmhip_bins_metas(h, C.cast(self.metas, C.c_void_p), ptr(self.upper)))
-        self.bins = np.zeros((self.num_data, self.num_features), np.uint8)
-        check(lib().lgbmhip_bins_matrix(h, ptr(self.bins)))
+        if device:
+            self._raw = X
+            return
+        self._bins = np.zeros((self.num_data, self.num_features), np.uint8)
+        check(lib().lgbmhip_bins_matrix(h, ptr(self._bins)))
+
+    @property
+    def bins(self):
+        """The n x F u8 bin matrix (binned on the host on first use when the
+        rows are binned on the device for training)."""
+        if self._bins is None:
+            self._bins = np.zeros((self.num_data, self.num_features), np.uint8)
+            check(lib().lgbmhip_bins_apply(self._handle, ptr(self._raw), self.num_data, self.num_total_features,
+                                           ptr(self._bins)))
+        return self._bins
 
     def feature_meta(self, f):
         m = self.metas[f]
@@ -189,8 +208,11 @@ class Booster:
         check(lib().lgbmhip_create(int(p["gpu_device_id"]), C.byref(h)))
         self._ctx = h
         ds = train_set
-        check(lib().lgbmhip_upload_bins(h, C.cast(ds.metas, C.c_void_p), ds.num_features, ptr(ds.upper),
-                                        ptr(ds.bins), ds.num_data, ds.num_total_features), h)
+        if ds._raw is not None:
+            check(lib().lgbmhip_upload_raw(h, ds._handle, ptr(ds._raw), ds.num_data, ds.num_total_features, 0), h)
+        else:
+            check(lib().lgbmhip_upload_bins(h, C.cast(ds.metas, C.c_void_p), ds.num_features, ptr(ds.upper),
+                                            ptr(ds.bins), ds.num_data, ds.num_total_features), h)
         qb = None
         nq = 0
         if ds.group is not None:
@@ -356,6 +378,11 @@ class Booster:
             e = np.exp(raw - raw.max(axis=1, keepdims=True))
             return e / e.sum(axis=1, keepdims=True)
         return raw[:, 0]
+
+    def bin_rows_device(self, x_ptr, rows, ncol, out_ptr):
+        """Bins rows resident in device memory (rows x ncol f32, integer device
+        addresses) into device u8 [rows][F] with the training mappers."""
+        check(lib().lgbmhip_bin_rows_device(self._ctx, x_ptr, rows, ncol, out_ptr), self._ctx)
 
     def predict_binned(self, bins):
         b = np.ascontiguousarray(bins, np.uint8)

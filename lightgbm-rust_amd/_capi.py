@@ -72,6 +72,7 @@ SIGNATURES = {
     "lgbmhip_last_error": (C.c_char_p, [P]),
     "lgbmhip_global_last_error": (C.c_char_p, []),
     "lgbmhip_bin_dense": (I32, [P, I64, I32, P, I32, C.POINTER(BinParams), PP]),
+    "lgbmhip_find_mappers": (I32, [P, I64, I32, P, I32, C.POINTER(BinParams), PP]),
     "lgbmhip_bins_free": (None, [P]),
     "lgbmhip_bins_num_features": (I32, [P]),
     "lgbmhip_bins_metas": (I32, [P, P, P]),
@@ -81,6 +82,8 @@ SIGNATURES = {
     "lgbmhip_create": (I32, [I32, PP]),
     "lgbmhip_destroy": (None, [P]),
     "lgbmhip_upload_bins": (I32, [P, P, I32, P, P, I64, I32]),
+    "lgbmhip_upload_raw": (I32, [P, P, P, I64, I32, I32]),
+    "lgbmhip_bin_rows_device": (I32, [P, P, I64, I32, P]),
     "lgbmhip_set_metadata": (I32, [P, P, P, P, I32]),
     "lgbmhip_set_split_params": (I32, [P, C.POINTER(SplitParams)]),
     "lgbmhip_set_train_opts": (I32, [P, C.POINTER(TrainOpts)]),

@@ -300,7 +300,7 @@ void parallel_for(int64_t n, int threads, F&& fn) {
 uint32_t Mapper::to_bin(double value) const {
   if (bin_type == 1) {
     if (std::isnan(value)) return 0;
-    int v = (int)value;
+    int v = sat_i32(value);
     if (v < 0) return 0;
     auto it = cat_to_bin.find(v);
     return it == cat_to_bin.end() ? 0u : (uint32_t)it->second;
@@ -317,7 +317,7 @@ uint32_t Mapper::to_bin(double value) const {
   return (uint32_t)lo;
 }
 
-Binned* bin_dense(const float* X, int64_t n, int ncol, const int32_t* cats, int n_cat, const lgbmhip_bin_params& p) {
+Binned* find_mappers(const float* X, int64_t n, int ncol, const int32_t* cats, int n_cat, const lgbmhip_bin_params& p) {
   if (n <= 0 || ncol <= 0) throw Error(LGBMHIP_INVALID_DATA, "dataset has no rows or no feature columns");
   if (n > INT32_MAX) throw Error(LGBMHIP_INVALID_DATA, "more than 2^31-1 rows");
   std::unique_ptr<Binned> out(new Binned());
@@ -345,6 +345,11 @@ Binned* bin_dense(const float* X, int64_t n, int ncol, const int32_t* cats, int 
       throw Error(LGBMHIP_UNSUPPORTED, "a feature has more than 256 bins (u8 bin storage)");
     out->used.push_back(c);
   }
+  return out.release();
+}
+
+Binned* bin_dense(const float* X, int64_t n, int ncol, const int32_t* cats, int n_cat, const lgbmhip_bin_params& p) {
+  std::unique_ptr<Binned> out(find_mappers(X, n, ncol, cats, n_cat, p));
   const int F = (int)out->used.size();
   out->bins.resize((size_t)n * F);
   const int64_t chunk = 16384;

@@ -43,11 +43,27 @@ struct Binned {
 };
 
 Binned* bin_dense(const float* X, int64_t n, int ncol, const int32_t* cats, int n_cat, const lgbmhip_bin_params& p);
+// The mappers only (no bin matrix): rows are binned on the device.
+Binned* find_mappers(const float* X, int64_t n, int ncol, const int32_t* cats, int n_cat, const lgbmhip_bin_params& p);
+// Rust's `f64 as i32`: saturating, NaN to 0 (binning.rs:641).
+inline int sat_i32(double v) {
+  if (!(v == v)) return 0;
+  if (v >= 2147483647.0) return 2147483647;
+  if (v <= -2147483648.0) return -2147483647 - 1;
+  return (int)v;
+}
 void apply_bins(const Binned& b, const float* X, int64_t n, int ncol, uint8_t* out);
 
 // ---------------------------------------------------------------- device records
 constexpr int kMaxBins = 256;        // u8 bins
 constexpr int kCatWords = kMaxBins / 32;
+
+// Per used feature, what value_to_bin reads on the device (binning.rs:637-667):
+// its raw column, and at `off` its bin upper bounds (numerical) or its
+// categories sorted ascending with their bins (categorical, ncat of them).
+struct BinFeat {
+  int col, num_bin, missing_type, bin_type, off, ncat;
+};
 
 // Per used feature, as the split search and partition read it (split.rs:18-29).
 struct FeatDev {

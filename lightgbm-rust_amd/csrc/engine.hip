@@ -239,6 +239,10 @@ struct Ctx {
   std::vector<FeatDev> feats;
   DevBuf<uint8_t> bins_rm, bins_cm;
   DevBuf<FeatDev> d_feats;
+  // value_to_bin tables of the used features (device binning)
+  DevBuf<BinFeat> d_binfeat;
+  DevBuf<double> d_bounds;
+  DevBuf<int> d_ckeys, d_cbins;
   // metadata
   std::vector<float> label;
   DevBuf<float> d_label, d_weight;
@@ -352,8 +356,10 @@ static int row_stride(int F) {
   return (F + 15) / 16 * 16;
 }
 
-static void upload_bins(Ctx* c, const lgbmhip_feature_meta* metas, int F, const double* upper, const uint8_t* bins,
-                        int64_t N, int ncol) {
+// The used features of the training data: metas, bin bounds, split-search
+// records and the device tables of value_to_bin (bounds of numerical
+// features; categories of categorical ones, sorted, with their bins).
+static void set_features(Ctx* c, const lgbmhip_feature_meta* metas, int F, const double* upper, int64_t N, int ncol) {
   if (F <= 0 || N <= 0) throw Error(LGBMHIP_INVALID_DATA, "no rows or no used features");
   if (N > INT32_MAX) throw Error(LGBMHIP_INVALID_DATA, "more than 2^31-1 rows");
   set_device(c);
@@ -376,10 +382,55 @@ static void upload_bins(Ctx* c, const lgbmhip_feature_meta* metas, int F, const 
     c->feats.push_back(d);
     c->B = std::max(c->B, m.num_bin);
   }
+  c->S = row_stride(F);
+  std::vector<BinFeat> bfs;
+  std::vector<double> bounds;
+  std::vector<int> ck, cb;
+  for (int f = 0; f < F; ++f) {
+    const auto& m = metas[f];
+    if (m.real_index < 0 || m.real_index >= ncol) throw Error(LGBMHIP_INVALID_DATA, "feature column out of range");
+    BinFeat bf{m.real_index, m.num_bin, m.missing_type, m.bin_type, 0, 0};
+    if (m.bin_type == 1) {
+      std::vector<std::pair<int, int>> cats;
+      for (int b = 0; b < m.num_bin; ++b)
+        if (c->upper[f][b] >= 0.0) cats.emplace_back(sat_i32(c->upper[f][b]), b);
+      std::sort(cats.begin(), cats.end());
+      bf.off = (int)ck.size();
+      bf.ncat = (int)cats.size();
+      for (auto& pr : cats) {
+        ck.push_back(pr.first);
+        cb.push_back(pr.second);
+      }
+    } else {
+      bf.off = (int)bounds.size();
+      bounds.insert(bounds.end(), c->upper[f].begin(), c->upper[f].end());
+    }
+    bfs.push_back(bf);
+  }
+  c->d_binfeat.alloc(F);
+  c->d_bounds.alloc(std::max<size_t>(bounds.size(), 1));
+  c->d_ckeys.alloc(std::max<size_t>(ck.size(), 1));
+  c->d_cbins.alloc(std::max<size_t>(cb.size(), 1));
+  HIP_CHECK(hipMemcpy(c->d_binfeat.p, bfs.data(), F * sizeof(BinFeat), hipMemcpyHostToDevice));
+  if (!bounds.empty()) HIP_CHECK(hipMemcpy(c->d_bounds.p, bounds.data(), bounds.size() * 8, hipMemcpyHostToDevice));
+  if (!ck.empty()) {
+    HIP_CHECK(hipMemcpy(c->d_ckeys.p, ck.data(), ck.size() * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->d_cbins.p, cb.data(), cb.size() * 4, hipMemcpyHostToDevice));
+  }
+  c->d_feats.alloc(F);
+  HIP_CHECK(hipMemcpy(c->d_feats.p, c->feats.data(), F * sizeof(FeatDev), hipMemcpyHostToDevice));
+  c->state_ready = false;
+  c->trees.clear();
+  c->pred_dirty = true;
+  c->valid.clear();
+}
+
+static void upload_bins(Ctx* c, const lgbmhip_feature_meta* metas, int F, const double* upper, const uint8_t* bins,
+                        int64_t N, int ncol) {
+  set_features(c, metas, F, upper, N, ncol);
   for (int64_t i = 0; i < N * F; ++i)
     if (bins[i] >= (uint8_t)c->metas[i % F].num_bin && c->metas[i % F].num_bin < 256)
       throw Error(LGBMHIP_INVALID_DATA, "bin value >= num_bin");
-  c->S = row_stride(F);
   std::vector<uint8_t> rm((size_t)N * c->S, 0), cm((size_t)N * F);
   for (int64_t r = 0; r < N; ++r)
     for (int f = 0; f < F; ++f) {
@@ -389,13 +440,56 @@ static void upload_bins(Ctx* c, const lgbmhip_feature_meta* metas, int F, const 
     }
   c->bins_rm.alloc(rm.size());
   c->bins_cm.alloc(cm.size());
-  c->d_feats.alloc(F);
   HIP_CHECK(hipMemcpy(c->bins_rm.p, rm.data(), rm.size(), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->bins_cm.p, cm.data(), cm.size(), hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(c->d_feats.p, c->feats.data(), F * sizeof(FeatDev), hipMemcpyHostToDevice));
-  c->state_ready = false;
-  c->trees.clear();
-  c->pred_dirty = true;
+}
+
+// Bins device rows (rows x ncol f32) with the training mappers into u8 rows
+// of `stride` bytes on the context's stream.
+static void launch_bin_rows(Ctx* c, const float* dX, int64_t rows, int ncol, uint8_t* dout, int stride) {
+  if (ncol != c->ncol) throw Error(LGBMHIP_INVALID_DATA, "column count differs from the training data");
+  if (rows <= 0) return;
+  const int64_t work = rows * ((stride + 3) / 4);
+  hipLaunchKernelGGL(k_bin_rows, dim3(grid_for(work, 256, c->cus * 16)), dim3(256), 0, c->stream, dX, (long long)rows,
+                     ncol, c->d_binfeat.p, c->F, c->d_bounds.p, c->d_ckeys.p, c->d_cbins.p, dout, stride);
+  HIP_CHECK(hipGetLastError());
+}
+
+// Dataset::from_matrix's binning pass (dataset.rs:355-422, value_to_bin
+// binning.rs:637-667) on the device: the raw rows (host memory, staged in
+// chunks, or device memory) binned straight into the row-major bins, then
+// the column-major copy.
+static void upload_raw(Ctx* c, const Binned& b, const float* X, int64_t N, int ncol, bool on_device) {
+  if (ncol != b.ncol) throw Error(LGBMHIP_INVALID_DATA, "column count differs from the mappers");
+  const int F = (int)b.used.size();
+  std::vector<lgbmhip_feature_meta> metas(F);
+  std::vector<double> upper;
+  for (int f = 0; f < F; ++f) {
+    const Mapper& m = b.mappers[b.used[f]];
+    metas[f] = lgbmhip_feature_meta{b.used[f], m.num_bin, m.missing_type, (int32_t)m.default_bin,
+                                    (int32_t)m.most_freq_bin, m.bin_type};
+    for (int i = 0; i < m.num_bin; ++i) upper.push_back(m.bin_type == 1 ? (double)m.categories[i] : m.bounds[i]);
+  }
+  set_features(c, metas.data(), F, upper.data(), N, ncol);
+  c->bins_rm.alloc((size_t)N * c->S);
+  c->bins_cm.alloc((size_t)N * F);
+  if (on_device) {
+    launch_bin_rows(c, X, N, ncol, c->bins_rm.p, c->S);
+  } else {
+    const int64_t chunk = std::max<int64_t>(1, ((int64_t)64 << 20) / ncol);   // 256 MiB of f32 per copy
+    DevBuf<float> stage;
+    stage.alloc((size_t)std::min(N, chunk) * ncol);
+    for (int64_t r0 = 0; r0 < N; r0 += chunk) {
+      const int64_t m = std::min(chunk, N - r0);
+      HIP_CHECK(hipMemcpyAsync(stage.p, X + (size_t)r0 * ncol, (size_t)m * ncol * 4, hipMemcpyHostToDevice, c->stream));
+      launch_bin_rows(c, stage.p, m, ncol, c->bins_rm.p + (size_t)r0 * c->S, c->S);
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+  }
+  hipLaunchKernelGGL(k_rm_to_cm, dim3(grid_for(N * F, 256, c->cus * 16)), dim3(256), 0, c->stream, c->bins_rm.p,
+                     (long long)N, c->S, F, c->bins_cm.p);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
 // Histogram launch shape: features per block so the block's LDS histogram
@@ -1592,6 +1686,20 @@ lgbmhip_status lgbmhip_bin_dense(const float* X, int64_t n, int32_t ncol, const 
     *out = b;
   });
 }
+lgbmhip_status lgbmhip_find_mappers(const float* X, int64_t n, int32_t ncol, const int32_t* categorical,
+                                    int32_t n_cat, const lgbmhip_bin_params* params, lgbmhip_bins** out) {
+  return guarded(nullptr, [&] {
+    if (!X || !params || !out) throw Error(LGBMHIP_INVALID_PARAMETER, "null argument");
+    auto* b = new lgbmhip_bins();
+    try {
+      b->b.reset(find_mappers(X, n, ncol, categorical, n_cat, *params));
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
 void lgbmhip_bins_free(lgbmhip_bins* b) { delete b; }
 int32_t lgbmhip_bins_num_features(const lgbmhip_bins* b) { return b ? (int32_t)b->b->used.size() : 0; }
 int64_t lgbmhip_bins_num_upper(const lgbmhip_bins* b) {
@@ -1612,7 +1720,11 @@ lgbmhip_status lgbmhip_bins_metas(const lgbmhip_bins* b, lgbmhip_feature_meta* m
   });
 }
 lgbmhip_status lgbmhip_bins_matrix(const lgbmhip_bins* b, uint8_t* out) {
-  return guarded(nullptr, [&] { std::memcpy(out, b->b->bins.data(), b->b->bins.size()); });
+  return guarded(nullptr, [&] {
+    if (b->b->bins.size() != (size_t)b->b->n * b->b->used.size())
+      throw Error(LGBMHIP_INVALID_PARAMETER, "these mappers hold no bin matrix (rows are binned on the device)");
+    std::memcpy(out, b->b->bins.data(), b->b->bins.size());
+  });
 }
 lgbmhip_status lgbmhip_bins_apply(const lgbmhip_bins* b, const float* X, int64_t n, int32_t ncol, uint8_t* out) {
   return guarded(nullptr, [&] { apply_bins(*b->b, X, n, ncol, out); });
@@ -2006,6 +2118,24 @@ lgbmhip_status lgbmhip_predict_device(lgbmhip_ctx* ctx, const float* d_X, const 
     if (!d_out || rows < 0 || (d_X && ncol <= 0)) throw Error(LGBMHIP_INVALID_PARAMETER, "bad prediction buffers");
     set_device(c);
     launch_predict(c, d_X, d_bins, rows, ncol, d_out);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+lgbmhip_status lgbmhip_upload_raw(lgbmhip_ctx* ctx, const lgbmhip_bins* mappers, const float* X, int64_t rows,
+                                  int32_t ncol, int32_t x_on_device) {
+  return guarded(ctx, [&] {
+    if (!mappers || !X) throw Error(LGBMHIP_INVALID_PARAMETER, "mappers and rows are required");
+    upload_raw(&ctx->c, *mappers->b, X, rows, ncol, x_on_device != 0);
+  });
+}
+lgbmhip_status lgbmhip_bin_rows_device(lgbmhip_ctx* ctx, const float* d_X, int64_t rows, int32_t ncol,
+                                       uint8_t* d_out) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (!c->N) throw Error(LGBMHIP_INVALID_DATA, "upload the training data first");
+    if (!d_X || !d_out || rows < 0) throw Error(LGBMHIP_INVALID_PARAMETER, "bad device buffers");
+    set_device(c);
+    launch_bin_rows(c, d_X, rows, ncol, d_out, c->F);
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
 }

@@ -1404,4 +1404,79 @@ __global__ void __launch_bounds__(64) k_search_one(const double* hist, const lon
   }
 }
 
+// ---------------------------------------------------------------- device binning
+// BinMapper::value_to_bin (binning.rs:637-667) of one value.
+__device__ __forceinline__ unsigned value_to_bin_dev(const BinFeat& bf, const double* bounds, const int* ckeys,
+                                                     const int* cbins, double v) {
+  if (bf.bin_type == 1) {
+    if (isnan(v)) return 0u;
+    // Rust `as i32`: saturating
+    const int iv = v >= 2147483647.0 ? 2147483647 : v <= -2147483648.0 ? (-2147483647 - 1) : (int)v;
+    if (iv < 0) return 0u;
+    const int* k = ckeys + bf.off;
+    int lo = 0, hi = bf.ncat;
+    while (lo < hi) {
+      const int m = (lo + hi) >> 1;
+      if (k[m] < iv) lo = m + 1; else hi = m;
+    }
+    return (lo < bf.ncat && k[lo] == iv) ? (unsigned)cbins[bf.off + lo] : 0u;
+  }
+  if (isnan(v)) {
+    if (bf.missing_type == 2) return (unsigned)(bf.num_bin - 1);
+    v = 0.0;
+  }
+  const double* ub = bounds + bf.off;
+  int l = 0, r = bf.num_bin - 1 - (bf.missing_type == 2 ? 1 : 0);
+  while (l < r) {
+    const int m = (r + l - 1) / 2;
+    if (v <= ub[m]) r = m; else l = m + 1;
+  }
+  return (unsigned)l;
+}
+
+// Bins rows x ncol f32 values into u8 rows of `stride` bytes (features F..stride-1
+// zero): one thread per 4 bytes of an output row, values loaded before use.
+__global__ void __launch_bounds__(256) k_bin_rows(const float* X, long long n, int ncol, const BinFeat* feats, int F,
+                                                  const double* bounds, const int* ckeys, const int* cbins,
+                                                  uint8_t* out, int stride) {
+  const int W = (stride + 3) / 4;
+  const long long total = n * W;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long r = t / W;
+    const int w = (int)(t - r * W);
+    float x[4];
+    BinFeat bf[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int f = min(4 * w + j, F - 1);
+      bf[j] = feats[f];
+      x[j] = X[r * ncol + bf[j].col];
+    }
+    unsigned word = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (4 * w + j < F) word |= (value_to_bin_dev(bf[j], bounds, ckeys, cbins, (double)x[j]) & 0xFFu) << (8 * j);
+    uint8_t* o = out + r * stride + 4 * w;
+    if ((stride & 3) == 0) {
+      *(unsigned*)o = word;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (4 * w + j < stride) o[j] = (uint8_t)(word >> (8 * j));
+    }
+  }
+}
+
+// Column-major copy [F][n] of row-major bins [n][S].
+__global__ void k_rm_to_cm(const uint8_t* rm, long long n, int S, int F, uint8_t* cm) {
+  const long long total = n * F;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(t / n);
+    const long long r = t - (long long)f * n;
+    cm[t] = rm[r * S + f];
+  }
+}
+
 }  // namespace lgbmhip

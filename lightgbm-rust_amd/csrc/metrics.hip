@@ -74,28 +74,31 @@ __device__ __forceinline__ unsigned long long desc_key(double s) {
   return ~asc;
 }
 
-__global__ void k_auc_keys(const double* score, long long n, unsigned long long* keys, int* rows) {
+// Sort input: keys, and the row (weighted) or the positive flag (unweighted,
+// so the sort carries the label and no gather follows).
+template <bool W>
+__global__ void k_auc_keys(const double* score, const float* label, long long n, unsigned long long* keys, int* rows,
+                           uint8_t* flags) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     keys[i] = desc_key(score[i]);
-    rows[i] = (int)i;
+    if (W) rows[i] = (int)i;
+    else flags[i] = label[i] > 0.0f ? 1 : 0;
   }
 }
 
-// Per sorted position: positive flag (or positive / negative weights) and
-// the position itself where a run of equal keys starts (0 elsewhere).
+// Per sorted position: the position itself where a run of equal keys starts
+// (0 elsewhere) and, weighted, the positive / negative weights.
 template <bool W>
 __global__ void k_auc_marks(const unsigned long long* keys, const int* rows, const float* label, const float* weight,
-                            long long n, int* pos, int* head, double* wpos, double* wneg) {
+                            long long n, int* head, double* wpos, double* wneg) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const int r = rows[i];
-    const float l = label[r];
     head[i] = (i == 0 || keys[i] != keys[i - 1]) ? (int)i : 0;
     if (W) {
+      const int r = rows[i];
+      const float l = label[r];
       const double w = (double)weight[r];
       wneg[i] = (double)(l <= 0.0f ? 1 : 0) * w;
       wpos[i] = (double)(l > 0.0f ? 1 : 0) * w;
-    } else {
-      pos[i] = l > 0.0f ? 1 : 0;
     }
   }
 }
@@ -105,7 +108,7 @@ __global__ void k_auc_marks(const unsigned long long* keys, const int* rows, con
 // P_x the positives in [0, x); weighted, neg_w (pos_w / 2 + before_w).
 template <bool W>
 __global__ void __launch_bounds__(kRedThreads) k_auc_runs(const unsigned long long* keys, long long n, const int* pre,
-                                                         const int* pos, const int* head_max, const double* pwpre,
+                                                         const uint8_t* pos, const int* head_max, const double* pwpre,
                                                          const double* nwpre, const double* wpos, const double* wneg,
                                                          long long* ipart, double* dpart) {
   __shared__ long long ism[kRedThreads];
@@ -138,7 +141,7 @@ __global__ void __launch_bounds__(kRedThreads) k_auc_runs(const unsigned long lo
 // metric.rs:724-728: accum / (sum_pos (sum_weights - sum_pos)), or 1.
 template <bool W>
 __global__ void __launch_bounds__(kRedThreads) k_auc_final(const long long* ipart, const double* dpart, int nparts,
-                                                          const int* pre, const int* pos, const double* pwpre,
+                                                          const int* pre, const uint8_t* pos, const double* pwpre,
                                                           const double* wpos, long long n, double sum_weights,
                                                           double* out) {
   __shared__ long long ism[kRedThreads];
@@ -343,7 +346,7 @@ __global__ void __launch_bounds__(256) k_ndcg_sum(const double* terms, int nq, i
 }  // namespace
 
 struct MetricWs {
-  Buf keys0, keys1, rows0, rows1, pos, pre, head, headmax, wpos, wneg, pwpre, nwpre, part, temp, res;
+  Buf keys0, keys1, rows0, rows1, pre, head, headmax, wpos, wneg, pwpre, nwpre, part, temp, res;
   Buf terms, bufs, qtab;
 };
 
@@ -361,58 +364,61 @@ double metric_auc(MetricWs* ws, hipStream_t s, const double* score, const float*
   const bool W = weight != nullptr;
   auto* k0 = (unsigned long long*)ws->keys0.get(n * 8);
   auto* k1 = (unsigned long long*)ws->keys1.get(n * 8);
-  auto* r0 = (int*)ws->rows0.get(n * 4);
-  auto* r1 = (int*)ws->rows1.get(n * 4);
   auto* head = (int*)ws->head.get(n * 4);
   auto* hmax = (int*)ws->headmax.get(n * 4);
-  int* pos = nullptr;
-  int* pre = nullptr;
+  int *r0 = nullptr, *r1 = nullptr, *pre = nullptr;
+  uint8_t *f0 = nullptr, *f1 = nullptr;
   double *wpos = nullptr, *wneg = nullptr, *pwpre = nullptr, *nwpre = nullptr;
   if (W) {
+    r0 = (int*)ws->rows0.get(n * 4);
+    r1 = (int*)ws->rows1.get(n * 4);
     wpos = (double*)ws->wpos.get(n * 8);
     wneg = (double*)ws->wneg.get(n * 8);
     pwpre = (double*)ws->pwpre.get(n * 8);
     nwpre = (double*)ws->nwpre.get(n * 8);
   } else {
-    pos = (int*)ws->pos.get(n * 4);
+    f0 = (uint8_t*)ws->rows0.get(n);
+    f1 = (uint8_t*)ws->rows1.get(n);
     pre = (int*)ws->pre.get(n * 4);
   }
   const int g = red_grid(n);
   auto* ipart = (long long*)ws->part.get(kRedBlocks * 8);
   auto* dpart = (double*)ipart;
   auto* res = (double*)ws->res.get(8);
-  hipLaunchKernelGGL(k_auc_keys, dim3(g), dim3(kRedThreads), 0, s, score, (long long)n, k0, r0);
-  size_t tb = 0;
-  MCHECK(rocprim::radix_sort_pairs(nullptr, tb, k0, k1, r0, r1, (unsigned)n, 0, 64, s));
-  size_t tb2 = 0;
+  size_t tb = 0, tb2 = 0, tb3 = 0;
   if (W) {
+    hipLaunchKernelGGL(k_auc_keys<true>, dim3(g), dim3(kRedThreads), 0, s, score, label, (long long)n, k0, r0, f0);
+    MCHECK(rocprim::radix_sort_pairs(nullptr, tb, k0, k1, r0, r1, (unsigned)n, 0, 64, s));
     MCHECK(rocprim::deterministic_exclusive_scan(nullptr, tb2, wpos, pwpre, 0.0, (size_t)n, rocprim::plus<double>(), s));
   } else {
-    MCHECK(rocprim::exclusive_scan(nullptr, tb2, pos, pre, 0, (size_t)n, rocprim::plus<int>(), s));
+    hipLaunchKernelGGL(k_auc_keys<false>, dim3(g), dim3(kRedThreads), 0, s, score, label, (long long)n, k0, r0, f0);
+    MCHECK(rocprim::radix_sort_pairs(nullptr, tb, k0, k1, f0, f1, (unsigned)n, 0, 64, s));
+    MCHECK(rocprim::exclusive_scan(nullptr, tb2, f1, pre, 0, (size_t)n, rocprim::plus<int>(), s));
   }
-  size_t tb3 = 0;
   MCHECK(rocprim::inclusive_scan(nullptr, tb3, head, hmax, (size_t)n, MaxOp(), s));
   void* temp = ws->temp.get(std::max(tb, std::max(tb2, tb3)));
-  MCHECK(rocprim::radix_sort_pairs(temp, tb, k0, k1, r0, r1, (unsigned)n, 0, 64, s));
-  if (W)
+  if (W) {
+    MCHECK(rocprim::radix_sort_pairs(temp, tb, k0, k1, r0, r1, (unsigned)n, 0, 64, s));
     hipLaunchKernelGGL(k_auc_marks<true>, dim3(g), dim3(kRedThreads), 0, s, k1, r1, label, weight, (long long)n,
-                       pos, head, wpos, wneg);
-  else
+                       head, wpos, wneg);
+  } else {
+    MCHECK(rocprim::radix_sort_pairs(temp, tb, k0, k1, f0, f1, (unsigned)n, 0, 64, s));
     hipLaunchKernelGGL(k_auc_marks<false>, dim3(g), dim3(kRedThreads), 0, s, k1, r1, label, weight, (long long)n,
-                       pos, head, wpos, wneg);
+                       head, wpos, wneg);
+  }
   MCHECK(rocprim::inclusive_scan(temp, tb3, head, hmax, (size_t)n, MaxOp(), s));
   if (W) {
     MCHECK(rocprim::deterministic_exclusive_scan(temp, tb2, wpos, pwpre, 0.0, (size_t)n, rocprim::plus<double>(), s));
     MCHECK(rocprim::deterministic_exclusive_scan(temp, tb2, wneg, nwpre, 0.0, (size_t)n, rocprim::plus<double>(), s));
-    hipLaunchKernelGGL(k_auc_runs<true>, dim3(g), dim3(kRedThreads), 0, s, k1, (long long)n, pre, pos, hmax, pwpre,
+    hipLaunchKernelGGL(k_auc_runs<true>, dim3(g), dim3(kRedThreads), 0, s, k1, (long long)n, pre, f1, hmax, pwpre,
                        nwpre, wpos, wneg, ipart, dpart);
-    hipLaunchKernelGGL(k_auc_final<true>, dim3(1), dim3(kRedThreads), 0, s, ipart, dpart, g, pre, pos, pwpre, wpos,
+    hipLaunchKernelGGL(k_auc_final<true>, dim3(1), dim3(kRedThreads), 0, s, ipart, dpart, g, pre, f1, pwpre, wpos,
                        (long long)n, sum_weights, res);
   } else {
-    MCHECK(rocprim::exclusive_scan(temp, tb2, pos, pre, 0, (size_t)n, rocprim::plus<int>(), s));
-    hipLaunchKernelGGL(k_auc_runs<false>, dim3(g), dim3(kRedThreads), 0, s, k1, (long long)n, pre, pos, hmax, pwpre,
+    MCHECK(rocprim::exclusive_scan(temp, tb2, f1, pre, 0, (size_t)n, rocprim::plus<int>(), s));
+    hipLaunchKernelGGL(k_auc_runs<false>, dim3(g), dim3(kRedThreads), 0, s, k1, (long long)n, pre, f1, hmax, pwpre,
                        nwpre, wpos, wneg, ipart, dpart);
-    hipLaunchKernelGGL(k_auc_final<false>, dim3(1), dim3(kRedThreads), 0, s, ipart, dpart, g, pre, pos, pwpre, wpos,
+    hipLaunchKernelGGL(k_auc_final<false>, dim3(1), dim3(kRedThreads), 0, s, ipart, dpart, g, pre, f1, pwpre, wpos,
                        (long long)n, sum_weights, res);
   }
   MCHECK(hipGetLastError());

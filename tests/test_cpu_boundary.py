@@ -87,3 +87,24 @@ def test_datagen_is_deterministic_and_shaped():
     assert abs(a.mean()) < 0.1 and abs(a.std() - 1) < 0.1
     X, y = datagen.binary_100k(1000)
     assert X.shape == (1000, 28) and set(np.unique(y)) <= {0.0, 1.0}
+
+
+def test_find_mappers_are_the_bin_dense_mappers():
+    rng = np.random.default_rng(40)
+    X = rng.normal(size=(30000, 6)).astype(np.float32)
+    X[::5, 1] = np.nan
+    X[:, 2] = rng.integers(-2, 30, len(X)).astype(np.float32)
+    X[:7, 2] = 1e12   # saturating cast: an unseen category
+    y = (X[:, 0] > 0).astype(np.float32)
+    params = dict(max_bin=63, categorical_feature=[2])
+    host = lgb.Dataset(X, label=y, params=params)
+    dev = lgb.Dataset(X, label=y, params=dict(params, device_binning=True))
+    assert dev._bins is None and dev.num_features == host.num_features
+    assert [dev.feature_meta(f) for f in range(dev.num_features)] == \
+        [host.feature_meta(f) for f in range(host.num_features)]
+    np.testing.assert_array_equal(dev.upper, host.upper)
+    np.testing.assert_array_equal(dev.bins, host.bins)   # host apply of the same mappers
+    # a mapper-only handle has no bin matrix
+    out = np.zeros(dev.bins.size, np.uint8)
+    with pytest.raises(lgb.LightGBMError, match="no bin matrix"):
+        capi.check(capi.lib().lgbmhip_bins_matrix(dev._handle, capi.ptr(out)))
