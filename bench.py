@@ -323,8 +323,54 @@ def workload_eval(lgb, datagen, args):
                                kernel="AUC over the training scores (radix sort of 64-bit keys and runs)"))
 
 
+def workload_binning(lgb, datagen, args):
+    """Dataset construction for --rows x 28 f32 rows: the host's binning
+    (mappers from a 200,000-row sample, then every value binned on host
+    threads) against the mappers on the host and the binning pass on the
+    device, from rows in host memory (with their PCIe copy) and from rows
+    already in HBM. One step is one binning pass over all rows."""
+    import ctypes as C
+    import torch
+    n = args.rows
+    X, y = datagen.binary_10p5m(n)
+    p = dict(objective="binary", max_bin=args.max_bin, num_leaves=args.num_leaves)
+    t0 = time.perf_counter()
+    host = lgb.Dataset(X, label=y, params=p)
+    host_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dev = lgb.Dataset(X, label=y, params=dict(p, device_binning=True))
+    mapper_s = time.perf_counter() - t0
+    b = lgb.Booster(p, dev)   # uploads through the device binning pass
+    lib, check = lgb._capi.lib(), lgb._capi.check
+    t0 = time.perf_counter()
+    for _ in range(3):
+        check(lib.lgbmhip_upload_raw(b._ctx, dev._handle, lgb.ptr(X), n, X.shape[1], 0), b._ctx)
+    from_host_s = (time.perf_counter() - t0) / 3
+    dx = torch.from_numpy(X).cuda()
+    out = torch.empty((n, dev.num_features), dtype=torch.uint8, device="cuda")
+    for _ in range(args.warmup):
+        b.bin_rows_device(dx.data_ptr(), n, X.shape[1], out.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.bin_rows_device(dx.data_ptr(), n, X.shape[1], out.data_ptr())
+    dt = (time.perf_counter() - t0) / args.steps
+    same = bool(np.array_equal(out[:200000].cpu().numpy(), host.bins[:200000]))
+    byts = n * X.shape[1] * 4 + n * dev.num_features
+    log(f"host binning {host_s:.2f}s; mappers {mapper_s:.2f}s + device pass from host rows {from_host_s * 1e3:.1f} ms; "
+        f"device pass on HBM rows {dt * 1e3:.3f} ms ({byts / dt / 1e9:.0f} GB/s), equal: {same}")
+    return _line("rows binned per second (device value_to_bin pass, rows resident in HBM)", n / dt, "rows/sec", args,
+                 dt * 1e3, "f32->u8",
+                 {"workload": f"binning {n} x 28 f32 rows (20 N(0,1) + 8 lognormal) with max_bin={args.max_bin} "
+                              f"mappers", "rows": n, "features": 28, "parallelism": "1 GPU"},
+                 binning=dict(host_total_s=host_s, host_mappers_s=mapper_s, device_from_host_rows_ms=from_host_s * 1e3,
+                              device_from_hbm_rows_ms=dt * 1e3, bins_equal_host=same),
+                 roofline=dict(bound="hbm", achieved=byts / dt / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                               frac=byts / dt / 1e9 / HBM_PEAK_GBS, traffic=None, kernel="k_bin_rows"))
+
+
 WORKLOADS = {"binary_10p5m": None, "multiclass_2m": workload_multiclass, "rank_2m": workload_rank,
-             "predict_1m": workload_predict, "eval_10p5m": workload_eval}
+             "predict_1m": workload_predict, "eval_10p5m": workload_eval, "binning_10p5m": workload_binning}
 
 
 def main():

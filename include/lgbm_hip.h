@@ -102,6 +102,9 @@ typedef struct {
   int32_t lambdarank_truncation_level;
   int32_t lambdarank_norm;
   int32_t use_graph;            /* capture each tree's launches in a hipGraph */
+  double bagging_fraction;      /* row bagging (sample_strategy.rs): on when < 1 and bagging_freq > 0 */
+  int32_t bagging_freq;
+  int32_t bagging_seed;
 } lgbmhip_train_opts;
 
 /* One metric of lgbmhip_eval (metric.rs:17-43 MetricKind, the Config fields it reads). */

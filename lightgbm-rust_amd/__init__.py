@@ -31,7 +31,8 @@ DEFAULT_PARAMS = dict(
     cat_l2=10.0, cat_smooth=10.0, min_data_per_group=100, lambdarank_truncation_level=30, lambdarank_norm=True,
     num_threads=0, device_type="gpu", gpu_device_id=0, use_graph=True, quant_table_threads=1,
     metric="", eval_at=(1, 2, 3, 4, 5), multi_error_top_k=1, label_gain=None, early_stopping_round=0,
-    first_metric_only=False, early_stopping_min_delta=0.0, device_binning=False,
+    first_metric_only=False, early_stopping_min_delta=0.0, device_binning=False, bagging_fraction=1.0,
+    bagging_freq=0, bagging_seed=3,
 )
 
 ALIASES = {
@@ -49,6 +50,8 @@ ALIASES = {
     "metrics": "metric", "metric_types": "metric", "ndcg_eval_at": "eval_at", "ndcg_at": "eval_at",
     "map_eval_at": "eval_at", "map_at": "eval_at", "early_stopping_rounds": "early_stopping_round",
     "early_stopping": "early_stopping_round", "n_iter_no_change": "early_stopping_round",
+    "sub_row": "bagging_fraction", "subsample": "bagging_fraction", "bagging": "bagging_fraction",
+    "subsample_freq": "bagging_freq", "bagging_fraction_seed": "bagging_seed",
 }
 
 # metric names and aliases on this path (config.rs ParseMetrics, metric.rs:44-76)
@@ -88,10 +91,22 @@ OBJECTIVES = {"binary": 0, "multiclass": 1, "softmax": 1, "lambdarank": 2}
 DEVICES = {"gpu", "hip", "cuda"}
 
 
+def _lcg_short(x):
+    """random.rs:19-33: the next state and next_short(0, 32767) of the upstream LCG."""
+    x = (x * 214013 + 2531011) & 0xFFFFFFFF
+    return x, ((x >> 16) & 0x7FFF) % 32767
+
+
 def _params(params):
     out = dict(DEFAULT_PARAMS)
-    for k, v in (params or {}).items():
-        k = ALIASES.get(k, k)
+    given = {ALIASES.get(k, k): v for k, v in (params or {}).items()}
+    if "seed" in given:
+        # config.rs:795-806: an explicit seed derives data_random_seed and
+        # bagging_seed (explicit values of those still win)
+        x = int(given["seed"]) & 0xFFFFFFFF
+        x, out["data_random_seed"] = _lcg_short(x)
+        x, out["bagging_seed"] = _lcg_short(x)
+    for k, v in given.items():
         out[k] = v
     if str(out["device_type"]).lower() not in DEVICES:
         raise LightGBMError("device_type must be gpu/hip for this engine (the reference's CPU engine handles cpu)")
@@ -230,7 +245,8 @@ class Booster:
                              int(p["num_grad_quant_bins"]), int(_b(p["stochastic_rounding"])),
                              int(_b(p["quant_train_renew_leaf"])), int(p["seed"]), int(p["quant_table_threads"]),
                              int(p["lambdarank_truncation_level"]), int(_b(p["lambdarank_norm"])),
-                             int(_b(p["use_graph"])))
+                             int(_b(p["use_graph"])), float(p["bagging_fraction"]), int(p["bagging_freq"]),
+                             int(p["bagging_seed"]))
         check(lib().lgbmhip_set_train_opts(h, C.byref(to)), h)
         self._finished = False
         self._metrics = _metrics(p)
@@ -393,7 +409,9 @@ class Booster:
     def predict_device(self, x_ptr, rows, ncol, out_ptr, binned=False):
         """Raw scores of rows resident in device memory (integer device
         addresses, e.g. torch tensors' data_ptr()): x is rows x ncol f32, or
-        rows x F u8 bins when `binned`; out is rows x K f64."""
+        rows x F u8 bins when `binned`; out is rows x K f64. With torch,
+        import torch before this package first loads its library, so the
+        process holds one HIP runtime."""
         if binned:
             check(lib().lgbmhip_predict_device(self._ctx, None, x_ptr, rows, 0, out_ptr), self._ctx)
         else:

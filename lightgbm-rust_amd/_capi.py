@@ -50,7 +50,8 @@ class TrainOpts(C.Structure):
                 ("use_quantized_grad", C.c_int32), ("num_grad_quant_bins", C.c_int32),
                 ("stochastic_rounding", C.c_int32), ("quant_train_renew_leaf", C.c_int32), ("seed", C.c_int32),
                 ("quant_table_threads", C.c_int32), ("lambdarank_truncation_level", C.c_int32),
-                ("lambdarank_norm", C.c_int32), ("use_graph", C.c_int32)]
+                ("lambdarank_norm", C.c_int32), ("use_graph", C.c_int32), ("bagging_fraction", C.c_double),
+                ("bagging_freq", C.c_int32), ("bagging_seed", C.c_int32)]
 
 
 class MetricOpts(C.Structure):

@@ -299,6 +299,15 @@ struct Ctx {
   DevBuf<float> r_lam, r_hes, r_bl;
   DevBuf<double> r_bs;
   double r_min_in = 0, r_max_in = 0, r_factor = 0;
+  // row bagging (sample_strategy.rs:186-318): the bag drawn on the device,
+  // the root's rows of every tree until the next draw, and buffer 0's bins
+  // for it (bag_grp0, group-major, positions 0 .. bag_m - 1)
+  bool bag_on = false, bag_subset = false, need_re_bagging = false;
+  int64_t bag_cnt = 0, bag_m = 0;
+  DevBuf<unsigned> bag_rows, bag_lcg;
+  DevBuf<uint8_t> bag_flags, bag_grp0;
+  DevBuf<int> bag_counts, bag_offsets;
+  DevBuf<long long> bag_total;
   // validation sets and the metric workspace
   std::vector<std::unique_ptr<ValidData>> valid;
   MetricWs* mws = nullptr;
@@ -715,6 +724,33 @@ static void setup_training(Ctx* c) {
       ln.slab.alloc((size_t)max_units * c->hist_fb * B * (quant ? 1 : 2));
       for (int b = 1; b < 3; ++b) ln.grp[b].alloc(gbytes);
     }
+    // row bagging: Bagger setup (sample_strategy.rs:186-214); the feature
+    // count stands for the reference's feature groups (dense data)
+    c->bag_on = o.bagging_fraction < 1.0 && o.bagging_freq > 0;
+    c->bag_cnt = N;
+    if (c->bag_on) {
+      if (o.objective == 2) throw Error(LGBMHIP_UNSUPPORTED, "bagging with the lambdarank objective");
+      c->bag_cnt = (int64_t)(o.bagging_fraction * (double)N);
+      c->bag_subset = ((double)c->bag_cnt / (double)N) / (double)o.bagging_freq <= 0.5 && F < 100;
+      if (quant && !c->bag_subset)
+        throw Error(LGBMHIP_UNSUPPORTED,
+                    "quantized gradients with a full-data bag (an average bag rate above 0.5; the reference's "
+                    "root sums read rows by position there)");
+      c->need_re_bagging = true;
+      const int64_t nb = (N + 1023) / 1024;
+      c->bag_rows.alloc(N);
+      c->bag_flags.alloc(N);
+      c->bag_counts.alloc(nb);
+      c->bag_offsets.alloc(nb);
+      c->bag_total.alloc(1);
+      c->bag_grp0.alloc(gbytes);
+      c->bag_m = 0;
+      // one generator per 1024-row block, Random(bagging_seed + block)
+      std::vector<unsigned> lcg(nb);
+      for (int64_t b = 0; b < nb; ++b) lcg[b] = (unsigned)o.bagging_seed + (unsigned)b;
+      c->bag_lcg.alloc(nb);
+      HIP_CHECK(hipMemcpy(c->bag_lcg.p, lcg.data(), nb * 4, hipMemcpyHostToDevice));
+    }
     // buffer 0: the root's rows in row order, bins group-major
     c->grp0.alloc(gbytes);
     hipLaunchKernelGGL(k_group_rows, dim3(grid_for(N * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
@@ -749,7 +785,7 @@ static void setup_training(Ctx* c) {
 static HistArgs hist_args(Ctx* c, Lane& ln, bool quant, int k, int mode) {
   HistArgs a{};
   for (int b = 0; b < 3; ++b) {
-    a.grp[b] = b == 0 ? c->grp0.p : ln.grp[b].p;
+    a.grp[b] = b == 0 ? (c->bag_on ? c->bag_grp0.p : c->grp0.p) : ln.grp[b].p;
     a.gh[b] = ln.gh[b].p;
   }
   a.st = ln.st.p;
@@ -773,6 +809,7 @@ static HistArgs hist_args(Ctx* c, Lane& ln, bool quant, int k, int mode) {
   a.p = c->sp;
   a.quant = quant ? 1 : 0;
   a.mode = mode;
+  a.root_n = c->bag_on ? c->bag_m : c->N;
   return a;
 }
 
@@ -811,7 +848,7 @@ static PartArgs part_args(Ctx* c, Lane& ln) {
   a.best = ln.best.p;
   a.feat_split = ln.feat_split.p;
   for (int b = 0; b < 3; ++b) {
-    a.grp[b] = b == 0 ? c->grp0.p : ln.grp[b].p;
+    a.grp[b] = b == 0 ? (c->bag_on ? c->bag_grp0.p : c->grp0.p) : ln.grp[b].p;
     a.gh[b] = ln.gh[b].p;
   }
   a.FB = c->hist_fb;
@@ -854,18 +891,26 @@ static void enqueue_head(Ctx* c, Lane& ln, int k) {
   const bool quant = c->opts.use_quantized_grad != 0;
   const float* gk = c->g.p + (size_t)k * N;
   const float* hk = c->h.p + (size_t)k * N;
-  const int ig = grid_for(N, 256, c->cus * 4);
+  // the root: every row, or the bag's rows (positions 0 .. bag_m - 1)
+  const int64_t R = c->bag_on ? c->bag_m : N;
+  const unsigned* bag = c->bag_on ? c->bag_rows.p : nullptr;
+  const int ig = grid_for(R, 256, c->cus * 4);
+  if (bag) {
+    // max |g|, |h| over the bag (quant.rs:100-108 with rows)
+    hipLaunchKernelGGL(k_clear_max, dim3(1), dim3(1), 0, s, c->it.p + k);
+    hipLaunchKernelGGL(k_absmax_it, dim3(ig), dim3(256), 0, s, gk, hk, (long long)R, c->it.p + k, bag);
+  }
   if (quant) {
     // the k-th discretizer call of the iteration (the counter moves on by
     // the iteration's trees once they are all grown)
-    hipLaunchKernelGGL(k_discretize, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, c->grv.p, c->hrv.p,
+    hipLaunchKernelGGL(k_discretize, dim3(ig), dim3(256), 0, s, (long long)R, gk, hk, c->grv.p, c->hrv.p,
                        c->opts.num_grad_quant_bins, c->opts.stochastic_rounding, 0, c->starts.p, c->call_ctr.p, k,
-                       c->it.p + k, (unsigned*)nullptr, (char2*)ln.gh[0].p, (int8_t*)nullptr, (int8_t*)nullptr);
+                       c->it.p + k, (unsigned*)nullptr, (char2*)ln.gh[0].p, (int8_t*)nullptr, (int8_t*)nullptr, bag);
   } else {
-    hipLaunchKernelGGL(k_tree_init, dim3(ig), dim3(256), 0, s, (long long)N, gk, hk, (unsigned*)nullptr,
-                       (float2*)ln.gh[0].p, c->it.p + k);
+    hipLaunchKernelGGL(k_tree_init, dim3(ig), dim3(256), 0, s, (long long)R, gk, hk, (unsigned*)nullptr,
+                       (float2*)ln.gh[0].p, c->it.p + k, bag);
   }
-  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(256), 0, s, ln.st.p, ln.pos.p, ln.leaves.p, ln.spl.p, (long long)N,
+  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(256), 0, s, ln.st.p, ln.pos.p, ln.leaves.p, ln.spl.p, (long long)R,
                      c->F);
   const int hgrid = c->hist_chunks * c->hist_ng;
   prof_mark(c, k, 0, 0, 0);
@@ -928,10 +973,15 @@ static void enqueue_tail(Ctx* c, Lane& ln, int k, bool score_update, double lr) 
   hipLaunchKernelGGL(k_tree_nodes, dim3(1), dim3(256), 0, s, ln.st.p, ln.log.p, ln.pos.p, ln.leaves.p, c->it.p + k,
                      c->L, ln.nodes.p, ln.leaf_shift.p);
   if (renew) {
+    // leaf sums over the root's rows: every row, or the bag's
     WalkArgs w = walk_args(c, ln, 0);
     w.g = c->g.p + (size_t)k * N;
     w.h = c->h.p + (size_t)k * N;
-    launch_walk(w, wgrid, s);
+    if (c->bag_on) {
+      w.rows = c->bag_rows.p;
+      w.N = c->bag_m;
+    }
+    launch_walk(w, c->bag_on ? grid_for(c->bag_m, 256, c->cus * 4) : wgrid, s);
   }
   hipLaunchKernelGGL(k_leaf_values, dim3((c->L + 255) / 256), dim3(256), 0, s, ln.st.p, ln.pos.p, ln.leaves.p,
                      c->it.p + k, ln.renew.p, renew ? 1 : 0, c->sp, lr, c->L, c->leafval.p + (size_t)k * c->L,
@@ -967,7 +1017,8 @@ static hipGraphExec_t capture(hipStream_t s, const std::function<void()>& body) 
 // rounds_main rounds, tail), then a copy of the growth state for the host.
 static void launch_main(Ctx* c, int k, bool score_update, double lr) {
   Lane& ln = lane_of(c, k);
-  const bool graph = c->opts.use_graph && !c->profile;
+  // a bag changes the root's size and rows: stream launches
+  const bool graph = c->opts.use_graph && !c->profile && !c->bag_on;
   if (graph) {
     if (!c->graphs[k])
       c->graphs[k] = capture(ln.stream, [&] {
@@ -988,7 +1039,7 @@ static void launch_main(Ctx* c, int k, bool score_update, double lr) {
 // rounds_cont more rounds of class k's tree and its tail again.
 static void launch_cont(Ctx* c, int k, int launched, bool score_update, double lr) {
   Lane& ln = lane_of(c, k);
-  const bool graph = c->opts.use_graph && !c->profile;
+  const bool graph = c->opts.use_graph && !c->profile && !c->bag_on;
   if (graph) {
     if (!c->graphs_cont[k])
       c->graphs_cont[k] = capture(ln.stream, [&] {
@@ -1158,7 +1209,8 @@ static void enqueue_gradients(Ctx* c) {
     a.g = c->g.p;
     a.h = c->h.p;
     if (a.Q > 0) hipLaunchKernelGGL(k_lambdarank, dim3(a.Q), dim3(kRankThreads), 0, s, a);
-    hipLaunchKernelGGL(k_absmax_it, dim3(grid), dim3(256), 0, s, c->g.p, c->h.p, (long long)N, c->it.p);
+    hipLaunchKernelGGL(k_absmax_it, dim3(grid), dim3(256), 0, s, c->g.p, c->h.p, (long long)N, c->it.p,
+                       (const unsigned*)nullptr);
   } else {
     hipLaunchKernelGGL(k_multiclass_grad, dim3(grid), dim3(256), 0, s, c->score.p, c->d_label.p, w, (long long)N,
                        c->K, c->g.p, c->h.p, c->it.p);
@@ -1205,6 +1257,33 @@ static void account_profile(Ctx* c, int k) {
   c->prof[10] += (double)st.num_splits;
 }
 
+// SampleStrategy::bagging (sample_strategy.rs:278-318) on the device: every
+// 1024-row block's generator draws its rows (the generators run on from draw
+// to draw), the in-bag rows are compacted in ascending order, and buffer 0's
+// bins are gathered for them.
+static void draw_bag(Ctx* c) {
+  const int64_t N = c->N, nb = (N + 1023) / 1024;
+  hipStream_t s = c->stream;
+  c->need_re_bagging = false;
+  hipLaunchKernelGGL(k_bag_draw, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, (long long)N,
+                     c->bag_lcg.p, c->opts.bagging_fraction, c->bag_flags.p, c->bag_counts.p);
+  hipLaunchKernelGGL(k_bag_offsets, dim3(1), dim3(1024), 0, s, c->bag_counts.p, (long long)nb, c->bag_offsets.p,
+                     c->bag_total.p);
+  hipLaunchKernelGGL(k_bag_scatter, dim3((unsigned)nb), dim3(1024), 0, s, (long long)N, c->bag_flags.p,
+                     c->bag_offsets.p, c->bag_rows.p);
+  HIP_CHECK(hipGetLastError());
+  long long m = 0;
+  HIP_CHECK(hipMemcpyAsync(&m, c->bag_total.p, 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (m <= 0) throw Error(LGBMHIP_UNSUPPORTED, "an empty bag");
+  c->bag_m = m;
+  c->bag_cnt = m;
+  hipLaunchKernelGGL(k_group_rows, dim3(grid_for(m * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
+                     s, (const unsigned*)c->bag_rows.p, (long long)m, c->bins_rm.p, c->S, c->F, c->hist_fb,
+                     c->hist_ng, (long long)N, c->bag_grp0.p);
+  HIP_CHECK(hipGetLastError());
+}
+
 // boosting.rs:957-1139 for device=gpu.
 static int train_iteration(Ctx* c) {
   if (!c->state_ready) setup_training(c);
@@ -1231,6 +1310,7 @@ static int train_iteration(Ctx* c) {
   }
   if (c->opts.use_quantized_grad && c->n_calls + K > (int)c->starts.n)
     throw Error(LGBMHIP_UNSUPPORTED, "more quantized trees than the start-offset table holds (65536)");
+  if (c->bag_on && ((c->bag_cnt < N && c->iter % c->opts.bagging_freq == 0) || c->need_re_bagging)) draw_bag(c);
   HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   enqueue_gradients(c);
   grow_trees(c, K, true, c->opts.learning_rate);
@@ -1287,13 +1367,14 @@ static int train_iteration(Ctx* c) {
 // One tree from caller gradients (SerialTreeLearner::train), no score update.
 static void train_tree(Ctx* c, const float* g, const float* h) {
   if (!c->state_ready) setup_training(c);
+  if (c->bag_on) throw Error(LGBMHIP_UNSUPPORTED, "one tree from caller gradients with bagging on");
   set_device(c);
   const int64_t N = c->N;
   HIP_CHECK(hipMemcpyAsync(c->g.p, g, N * 4, hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(hipMemcpyAsync(c->h.p, h, N * 4, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(64), 0, c->stream, c->it.p, c->K);
   hipLaunchKernelGGL(k_absmax_it, dim3(grid_for(N, 256, c->cus * 4)), dim3(256), 0, c->stream, c->g.p, c->h.p,
-                     (long long)N, c->it.p);
+                     (long long)N, c->it.p, (const unsigned*)nullptr);
   if (c->opts.use_quantized_grad && c->n_calls + 1 > (int)c->starts.n)
     throw Error(LGBMHIP_UNSUPPORTED, "more quantized trees than the start-offset table holds (65536)");
   grow_trees(c, 1, false, 1.0);
@@ -1921,11 +2002,11 @@ lgbmhip_status lgbmhip_discretize(lgbmhip_ctx* ctx, const float* grad, const flo
     for (int call = 0; call < num_calls; ++call) {
       hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(64), 0, c->stream, c->it.p, 1);
       hipLaunchKernelGGL(k_absmax_it, dim3(grid_for(N, 256, c->cus * 4)), dim3(256), 0, c->stream, c->g.p, c->h.p,
-                         (long long)N, c->it.p);
+                         (long long)N, c->it.p, (const unsigned*)nullptr);
       hipLaunchKernelGGL(k_discretize, dim3(grid_for(N, 256, c->cus * 4)), dim3(256), 0, c->stream, (long long)N,
                          c->g.p, c->h.p, c->grv.p, c->hrv.p, c->opts.num_grad_quant_bins, c->opts.stochastic_rounding,
                          0, c->starts.p, c->call_ctr.p, 0, c->it.p, (unsigned*)nullptr, (char2*)c->lanes[0]->gh[0].p,
-                         c->gq.p, c->hq.p);
+                         c->gq.p, c->hq.p, (const unsigned*)nullptr);
       hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, c->stream, c->call_ctr.p, 1);
       c->n_calls++;
     }

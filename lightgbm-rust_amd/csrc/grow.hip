@@ -123,6 +123,7 @@ struct HistArgs {
   SplitParamsDev p;
   int quant;
   int mode;                // 1: explicit segment, 2: the root, 3: the round's items
+  long long root_n;        // rows of the root (all rows, or the bag)
   long long d_begin, d_count;
   int d_buf, d_shift_g, d_shift_h;
 };
@@ -295,7 +296,7 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
   if (a.mode != 3) {
     if (tid == 0) {
       const bool root = a.mode == 2;
-      const long long cnt = root ? a.N : a.d_count;
+      const long long cnt = root ? a.root_n : a.d_count;
       s_begin[0] = root ? 0 : a.d_begin;
       s_count[0] = cnt;
       s_buf[0] = root ? 0 : a.d_buf;
@@ -1540,6 +1541,7 @@ struct WalkArgs {
   long long N;
   const float* g;
   const float* h;
+  const unsigned* rows;  // MODE 0 over a bag: row of each position (null: every row in order)
   const int* shifts;
   long long* sums;       // [L][2]
   const double* val;     // [L]
@@ -1596,6 +1598,7 @@ __global__ void __launch_bounds__(256) k_walk(WalkArgs a) {
       const long long r = base + (long long)u * blockDim.x;
       node[u] = (ns == 0 || r >= a.N) ? -1 : 0;   // ~0: leaf 0
       rc[u] = r < a.N ? r : a.N - 1;
+      if (MODE == 0 && a.rows) rc[u] = (long long)a.rows[rc[u]];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {

@@ -484,12 +484,15 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
   }
 }
 
-// max |g|, |h| after the ranking gradients (order-free).
-__global__ void __launch_bounds__(256) k_absmax_it(const float* g, const float* h, long long n, IterDev* it) {
+// max |g|, |h| after the ranking gradients, or over the rows of a bag
+// (rows[0..n), when given) (order-free).
+__global__ void __launch_bounds__(256) k_absmax_it(const float* g, const float* h, long long n, IterDev* it,
+                                                   const unsigned* rows) {
   unsigned mg = 0, mh = 0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    mg = max(mg, __float_as_uint(fabsf(g[i])));
-    mh = max(mh, __float_as_uint(fabsf(h[i])));
+    const long long r = rows ? (long long)rows[i] : i;
+    mg = max(mg, __float_as_uint(fabsf(g[r])));
+    mh = max(mh, __float_as_uint(fabsf(h[r])));
   }
   mg = block_max_u(mg);
   mh = block_max_u(mh);
@@ -504,11 +507,13 @@ __global__ void __launch_bounds__(256) k_absmax_it(const float* g, const float* 
 // row index of buffer 0 is written once per training run), fixed-point root
 // sums (learner/mod.rs:580-628).
 __global__ void __launch_bounds__(256) k_tree_init(long long n, const float* __restrict__ g, const float* __restrict__ h,
-                            unsigned* __restrict__ idx0, float2* __restrict__ gh0, IterDev* it) {
+                            unsigned* __restrict__ idx0, float2* __restrict__ gh0, IterDev* it,
+                            const unsigned* __restrict__ bag) {
   const int sg = fix_shift(n, it->max_abs_g_bits), sh = fix_shift(n, it->max_abs_h_bits);
   long long ag = 0, ah = 0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const float gv = g[i], hv = h[i];
+    const long long r = bag ? (long long)bag[i] : i;   // position i of the root holds row r
+    const float gv = g[r], hv = h[r];
     if (idx0) idx0[i] = (unsigned)i;
     gh0[i] = make_float2(gv, hv);
     ag += to_fixed(gv, sg);
@@ -530,25 +535,28 @@ __global__ void __launch_bounds__(256) k_discretize(long long n, const float* __
                              int constant_hessian, const int* __restrict__ starts, const int* call_ctr, int call_off,
                              IterDev* it,
                              unsigned* __restrict__ idx0, char2* __restrict__ q0, int8_t* __restrict__ gq_out,
-                             int8_t* __restrict__ hq_out) {
+                             int8_t* __restrict__ hq_out, const unsigned* __restrict__ bag) {
   const double mg = (double)__uint_as_float(it->max_abs_g_bits);
   const double mh = (double)__uint_as_float(it->max_abs_h_bits);
   const double gs = mg / (double)(bins / 2);
   const double hs = constant_hessian ? mh : mh / (double)bins;
   const double inv_g = 1.0 / gs, inv_h = 1.0 / hs;
   const long long start = starts[*call_ctr + call_off];
+  const long long start_n = n > 0 ? start % n : 0;   // a bag has fewer rows than the offsets' range
   long long ai = 0, agq = 0, ahq = 0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const double gv = (double)g[i];
+    // position i of the root (of the bag: row bag[i], quant.rs:92-129 with rows)
+    const long long r = bag ? (long long)bag[i] : i;
+    const double gv = (double)g[r];
     double rg = 0.5, rh = 0.5;
     if (stochastic) {
-      long long pos = i + start;   // (i + start) mod n, with i, start < n
+      long long pos = i + start_n;   // (i + start) mod n
       if (pos >= n) pos -= n;
       rg = grv[pos];
       rh = hrv[pos];
     }
     const int8_t a = gv >= 0.0 ? to_i8(gv * inv_g + rg) : to_i8(gv * inv_g - rg);
-    const int8_t b = constant_hessian ? (int8_t)1 : to_i8((double)h[i] * inv_h + rh);
+    const int8_t b = constant_hessian ? (int8_t)1 : to_i8((double)h[r] * inv_h + rh);
     if (idx0) idx0[i] = (unsigned)i;
     q0[i] = make_char2(a, b);
     if (gq_out) {
@@ -1402,6 +1410,81 @@ __global__ void __launch_bounds__(64) k_search_one(const double* hist, const lon
     *out = r;
     *splittable = spl ? 1 : 0;
   }
+}
+
+// ---------------------------------------------------------------- bagging
+// sample_strategy.rs:285-318: row i is in the bag when the next draw of the
+// generator of its 1024-row block (Random(seed + block), random.rs:15-40,
+// its state kept in lcg[block] from draw to draw) is below the fraction.
+// One thread per block draws its rows in order.
+constexpr int kBagBlock = 1024;
+
+__global__ void k_bag_draw(long long n, unsigned* lcg, double fraction, uint8_t* flags, int* counts) {
+  const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nb = (n + kBagBlock - 1) / kBagBlock;
+  if (b >= nb) return;
+  unsigned x = lcg[b];
+  const long long r0 = b * kBagBlock, r1 = min(n, r0 + kBagBlock);
+  int cnt = 0;
+  for (long long i = r0; i < r1; ++i) {
+    x = x * 214013u + 2531011u;
+    const float f = (float)(int)((x >> 16) & 0x7FFFu) / 32768.0f;
+    const int in = (double)f < fraction ? 1 : 0;
+    flags[i] = (uint8_t)in;
+    cnt += in;
+  }
+  lcg[b] = x;
+  counts[b] = cnt;
+}
+
+// Exclusive offsets of the per-block counts (one 1024-thread block), and the bag size.
+__global__ void __launch_bounds__(1024) k_bag_offsets(const int* counts, long long nb, int* offsets, long long* total) {
+  __shared__ int sm[1024];
+  __shared__ int carry;
+  const int t = threadIdx.x;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (long long base = 0; base < nb; base += 1024) {
+    const long long i = base + t;
+    const int v = i < nb ? counts[i] : 0;
+    sm[t] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int add = t >= o ? sm[t - o] : 0;
+      __syncthreads();
+      sm[t] += add;
+      __syncthreads();
+    }
+    if (i < nb) offsets[i] = carry + sm[t] - v;
+    __syncthreads();
+    if (t == 1023) carry += sm[1023];
+    __syncthreads();
+  }
+  if (t == 0) *total = carry;
+}
+
+// In-bag rows ascending: one 1024-thread block per generator block.
+__global__ void __launch_bounds__(1024) k_bag_scatter(long long n, const uint8_t* flags, const int* offsets,
+                                                      unsigned* rows) {
+  __shared__ int sm[1024];
+  const int t = threadIdx.x;
+  const long long i = (long long)blockIdx.x * kBagBlock + t;
+  const int v = i < n ? flags[i] : 0;
+  sm[t] = v;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int add = t >= o ? sm[t - o] : 0;
+    __syncthreads();
+    sm[t] += add;
+    __syncthreads();
+  }
+  if (v) rows[offsets[blockIdx.x] + sm[t] - 1] = (unsigned)i;
+}
+
+// Clears class k's max |g|, |h| before they are taken over a bag.
+__global__ void k_clear_max(IterDev* it) {
+  it->max_abs_g_bits = 0;
+  it->max_abs_h_bits = 0;
 }
 
 // ---------------------------------------------------------------- device binning

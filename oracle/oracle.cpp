@@ -455,12 +455,15 @@ struct Params {
   int lambdarank_truncation_level = 30;
   bool lambdarank_norm = true;
   int num_threads = 1;
+  double bagging_fraction = 1.0;   // sample_strategy.rs (row bagging)
+  int bagging_freq = 0, bagging_seed = 3;
   std::vector<int> categorical;
 };
 
 static bool parse_params(const char* s, Params& p) {
   std::string str(s ? s : "");
   size_t i = 0;
+  bool seed_given = false, drs_given = false, bs_given = false;
   while (i < str.size()) {
     while (i < str.size() && (str[i] == ' ' || str[i] == ',')) ++i;
     size_t j = i;
@@ -504,6 +507,9 @@ static bool parse_params(const char* s, Params& p) {
       else if (k == "lambdarank_truncation_level") p.lambdarank_truncation_level = atoi(v.c_str());
       else if (k == "lambdarank_norm") p.lambdarank_norm = v == "true" || v == "1";
       else if (k == "num_threads") p.num_threads = atoi(v.c_str());
+      else if (k == "bagging_fraction") p.bagging_fraction = atof(v.c_str());
+      else if (k == "bagging_freq") p.bagging_freq = atoi(v.c_str());
+      else if (k == "bagging_seed") p.bagging_seed = atoi(v.c_str());
       else if (k == "categorical_feature") {
         size_t a = 0;
         while (a < v.size()) {
@@ -515,8 +521,19 @@ static bool parse_params(const char* s, Params& p) {
       } else if (k == "num_iterations" || k == "force_row_wise" || k == "deterministic" || k == "verbose") {
       } else return false;
       if (p.objective < 0) return false;
+      seed_given |= k == "seed";
+      drs_given |= k == "data_random_seed";
+      bs_given |= k == "bagging_seed";
     }
     i = j;
+  }
+  if (seed_given) {
+    // config.rs:795-806: an explicit seed derives the other seeds; explicit
+    // values of those still win
+    Random r(p.seed);
+    const int drs = r.next_short(0, 32767), bs = r.next_short(0, 32767);
+    if (!drs_given) p.data_random_seed = drs;
+    if (!bs_given) p.bagging_seed = bs;
   }
   return true;
 }
@@ -1303,6 +1320,33 @@ struct Discretizer {
     hq.assign(n, 0);
     packed.assign(n, 0);
   }
+  // quant.rs:92-129 with rows: position i of the bag is row rows[i]; the
+  // random tables are read at (i + start) mod (bag size).
+  void discretize_rows(const float* grad, const float* hess, const std::vector<uint32_t>& rows) {
+    const int64_t n = (int64_t)rows.size();
+    if (n == 0) return;
+    double mg = std::fabs((double)grad[rows[0]]), mh = std::fabs((double)hess[rows[0]]);
+    for (int64_t i = 0; i < n; ++i) {
+      double g = std::fabs((double)grad[rows[i]]), h = std::fabs((double)hess[rows[i]]);
+      if (g > mg) mg = g;
+      if (h > mh) mh = h;
+    }
+    grad_scale = mg / (double)(bins / 2);
+    hess_scale = constant_hessian ? mh : mh / (double)bins;
+    double inv_g = 1.0 / grad_scale, inv_h = 1.0 / hess_scale;
+    int64_t start = start_eng.uniform_int(0, start_max);
+    for (int64_t i = 0; i < n; ++i) {
+      const uint32_t r = rows[i];
+      double g = (double)grad[r];
+      double rg = 0.5, rh = 0.5;
+      if (stochastic) { int64_t pos = (i + start) % n; rg = grv[pos]; rh = hrv[pos]; }
+      int8_t gi = g >= 0.0 ? to_i8(g * inv_g + rg) : to_i8(g * inv_g - rg);
+      int8_t hi = constant_hessian ? 1 : to_i8((double)hess[r] * inv_h + rh);
+      gq[r] = gi;
+      hq[r] = hi;
+      packed[r] = (int64_t)(((uint64_t)(int64_t)gi << 32) | (uint64_t)(uint8_t)hi);
+    }
+  }
   void discretize(const float* grad, const float* hess, int64_t n) {
     if (n == 0) return;
     double mg = std::fabs((double)grad[0]), mh = std::fabs((double)hess[0]);
@@ -1358,6 +1402,10 @@ struct Learner {
   LeafSplits smaller, larger;
   Discretizer* quant = nullptr;
   std::vector<TraceEntry> trace;
+  // learner/mod.rs:545-568 set_bagging_data: the root's rows (ascending);
+  // `bag_subset` is upstream's subset mode (the quantizer draws over the bag)
+  std::vector<uint32_t> bag;
+  bool use_bag = false, bag_subset = false;
   // the last tree's smaller-leaf histograms, for stage parity checks
   Learner(const Dataset* d, const Params& p) : ds(d), prm(p) {
     sp = SplitParams{p.lambda_l1, p.lambda_l2, p.max_delta_step, p.path_smooth, p.min_data_in_leaf,
@@ -1591,10 +1639,13 @@ struct Learner {
 
   // learner/mod.rs:572-695
   Tree train(const float* g, const float* h) {
-    int64_t n = ds->n;
+    int64_t n = use_bag ? (int64_t)bag.size() : ds->n;
     bool q = quant != nullptr;
-    if (q) quant->discretize(g, h, n);
-    for (int64_t i = 0; i < n; ++i) indices[i] = (uint32_t)i;
+    if (q) {
+      if (use_bag) quant->discretize_rows(g, h, bag);   // subset mode only (checked at set-up)
+      else quant->discretize(g, h, n);
+    }
+    for (int64_t i = 0; i < n; ++i) indices[i] = use_bag ? bag[i] : (uint32_t)i;
     std::fill(leaf_begin.begin(), leaf_begin.end(), 0);
     std::fill(leaf_count.begin(), leaf_count.end(), 0);
     leaf_count[0] = n;
@@ -1602,13 +1653,14 @@ struct Learner {
     double sg = 0, sh = 0;
     int64_t is = 0;
     if (q) {
-      for (int64_t i = 0; i < n; ++i) {
+      for (int64_t k = 0; k < n; ++k) {
+        const uint32_t i = indices[k];
         sg += (double)quant->gq[i] * quant->grad_scale;
         sh += (double)quant->hq[i] * quant->hess_scale;
         is = wadd(is, quant->packed[i]);
       }
     } else {
-      for (int64_t i = 0; i < n; ++i) { sg += (double)g[i]; sh += (double)h[i]; }
+      for (int64_t k = 0; k < n; ++k) { sg += (double)g[indices[k]]; sh += (double)h[indices[k]]; }
     }
     smaller = LeafSplits{0, (int)n, sg, sh, is, smaller.weight};
     larger = LeafSplits{};
@@ -1639,6 +1691,30 @@ struct Learner {
       }
     }
     return tree;
+  }
+
+  // tree.rs:605-632 leaf_binned_with: the leaf of row r from its bins
+  int leaf_binned(const Tree& t, int64_t r) const {
+    if (t.num_leaves <= 1) return 0;
+    int node = 0;
+    while (node >= 0) {
+      const int f = t.split_feature_inner[node];
+      const FeatureMeta& m = metas[f];
+      const uint32_t b = ds->bin(r, f);
+      if (t.decision_type[node] & 1) {
+        const int c = (int)t.threshold_in_bin[node];
+        const int lo = t.cat_boundaries_inner[c], hi = t.cat_boundaries_inner[c + 1];
+        const bool in = (int)(b / 32) < hi - lo && ((t.cat_threshold_inner[lo + b / 32] >> (b % 32)) & 1);
+        node = in ? t.left_child[node] : t.right_child[node];
+        continue;
+      }
+      const int mt = (t.decision_type[node] >> 2) & 3;
+      const bool dl = (t.decision_type[node] & 2) != 0;
+      const bool def = (mt == MT_ZERO && b == (uint32_t)m.default_bin) || (mt == MT_NAN && b == (uint32_t)(m.num_bin - 1));
+      node = def ? (dl ? t.left_child[node] : t.right_child[node])
+                 : (b <= t.threshold_in_bin[node] ? t.left_child[node] : t.right_child[node]);
+    }
+    return ~node;
   }
 
   // partition.rs:178-191
@@ -1821,6 +1897,45 @@ struct Lambdarank {
 using namespace orc;
 
 // ---------------------------------------------------------------- booster
+// sample_strategy.rs:186-214, 278-318: row bagging (no balanced fractions,
+// no query bagging, no GOSS). One LCG per 1024-row block seeded
+// bagging_seed + block; a row is in the bag when its draw is below the
+// fraction.
+struct Bagger {
+  bool on = false, subset = false, need_re_bagging = false;
+  double fraction = 1.0;
+  int freq = 0;
+  int64_t n = 0, bag_cnt = 0;
+  std::vector<Random> rands;
+  std::vector<uint32_t> in_bag, out_of_bag;
+  void setup(const Params& p, int64_t rows, int num_feature_groups) {
+    n = rows;
+    bag_cnt = rows;
+    if (!(p.bagging_fraction < 1.0 && p.bagging_freq > 0)) return;
+    on = true;
+    fraction = p.bagging_fraction;
+    freq = p.bagging_freq;
+    bag_cnt = (int64_t)(fraction * (double)n);
+    for (int64_t b = 0; b < (n + 1023) / 1024; ++b) rands.emplace_back((int)((uint32_t)p.bagging_seed + (uint32_t)b));
+    const double rate = ((double)bag_cnt / (double)n) / (double)freq;
+    subset = rate <= 0.5 && num_feature_groups < 100;
+    need_re_bagging = true;
+  }
+  bool bagging(int iter) {
+    if (!on || !((bag_cnt < n && iter % freq == 0) || need_re_bagging)) return false;
+    need_re_bagging = false;
+    in_bag.clear();
+    out_of_bag.clear();
+    for (int64_t i = 0; i < n; ++i) {
+      const double r = (double)rands[i / 1024].next_float();
+      if (r < fraction) in_bag.push_back((uint32_t)i);
+      else out_of_bag.push_back((uint32_t)i);
+    }
+    bag_cnt = (int64_t)in_bag.size();
+    return true;
+  }
+};
+
 struct OrcBooster {
   Dataset* ds;
   Params prm;
@@ -1833,6 +1948,7 @@ struct OrcBooster {
   int iter = 0;
   std::vector<std::vector<TraceEntry>> traces;
   Lambdarank* rank = nullptr;
+  Bagger bagger;
 };
 
 // boosting.rs:957-1139 (gbdt, no bagging, built-in objective)
@@ -1853,6 +1969,11 @@ static int booster_train_one_iter(OrcBooster* b) {
       }
     }
   }
+  if (b->bagger.bagging(b->iter)) {
+    b->learner->bag = b->bagger.in_bag;
+    b->learner->use_bag = true;
+    b->learner->bag_subset = b->bagger.subset;
+  }
   if (b->prm.objective == 0) binary_gradients(b->scores.data(), b->ds->label.data(), n, b->prm.sigmoid, b->grad.data(), b->hess.data());
   else if (b->prm.objective == 1) multiclass_gradients(b->scores.data(), b->ds->label.data(), n, K, b->grad.data(), b->hess.data());
   else b->rank->gradients(b->scores.data(), b->ds->label.data(), b->ds->query_boundaries, b->grad.data(), b->hess.data());
@@ -1864,6 +1985,9 @@ static int booster_train_one_iter(OrcBooster* b) {
       cont = true;
       t.shrink(b->prm.learning_rate);
       b->learner->add_leaf_outputs(t, b->scores.data() + (size_t)k * n);
+      // boosting.rs:1395-1398: out-of-bag rows by the binned tree
+      if (b->bagger.on && b->bagger.bag_cnt < n)
+        for (uint32_t r : b->bagger.out_of_bag) b->scores[(size_t)k * n + r] += t.leaf_value[b->learner->leaf_binned(t, r)];
       if (std::fabs(init[k]) > K_EPSILON) t.add_bias(init[k]);
     } else if (b->models.size() < (size_t)K) {
       t.as_constant(init[k], (int)n);
@@ -1932,7 +2056,22 @@ void* orc_booster_create(void* d, const char* params) {
   b->scores.assign((size_t)b->K * b->ds->n, 0.0);
   b->grad.assign((size_t)b->K * b->ds->n, 0.0f);
   b->hess.assign((size_t)b->K * b->ds->n, 0.0f);
+  b->bagger.setup(p, b->ds->n, (int)b->ds->used.size());
+  if (b->bagger.on && p.use_quantized_grad && !b->bagger.subset) {
+    // the full-data bag of the quantized learner is not restated
+    delete b->learner;
+    delete b->rank;
+    delete b;
+    return nullptr;
+  }
   return b;
+}
+// The current bag (in-bag rows ascending); returns its size, or -1 without bagging.
+int64_t orc_booster_bag(void* p, uint32_t* out) {
+  OrcBooster* b = (OrcBooster*)p;
+  if (!b->bagger.on) return -1;
+  if (out) std::memcpy(out, b->bagger.in_bag.data(), b->bagger.in_bag.size() * 4);
+  return (int64_t)b->bagger.in_bag.size();
 }
 void orc_booster_free(void* p) {
   OrcBooster* b = (OrcBooster*)p;

@@ -76,6 +76,7 @@ def lib():
             "orc_ndcg": (D, [PD, PF, PI, I, I]),
             "orc_max_dcg_at_k": (D, [PF, I, I]),
             "orc_binary_logloss": (D, [PD, PF, I64, D]),
+            "orc_booster_bag": (I64, [P, P]),
             "orc_metric_auc": (D, [PD, PF, P, I64]),
             "orc_metric_point": (D, [I, PD, PF, P, I64, I, D, I]),
             "orc_metric_ndcg": (None, [PD, PF, PI, I, PI, I, I, PD]),
@@ -220,6 +221,15 @@ class Booster:
         hq = np.zeros(self.ds.n, np.int8)
         lib().orc_learner_quant_packed(self.h, gq, hq)
         return gq, hq
+
+    def bag(self):
+        """The current bag (in-bag rows ascending), or None without bagging."""
+        k = lib().orc_booster_bag(self.h, None)
+        if k < 0:
+            return None
+        out = np.zeros(max(k, 1), np.uint32)
+        lib().orc_booster_bag(self.h, out.ctypes.data)
+        return out[:k]
 
     def predict_raw(self, X):
         X = np.ascontiguousarray(X, np.float32)

@@ -1,0 +1,67 @@
+"""Row bagging (sample_strategy.rs:186-318) on the GPU against the oracle:
+the same bag (one generator per 1024-row block, running on from draw to
+draw), trees grown on the bag's rows, out-of-bag rows scored by the binned
+tree. Tree structure, thresholds and leaf counts exact; leaf values within
+FP64 summation noise; scores within 1e-9 relative."""
+import numpy as np
+import pytest
+
+from tests._pkg import datagen, lgb
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+P = dict(objective="binary", num_leaves=31, learning_rate=0.1, min_data_in_leaf=20, max_bin=63)
+
+
+def _compare(b, ob, trees):
+    for t in range(trees):
+        gt, ot = b.tree(t), ob.tree(t)
+        assert gt.num_leaves == ot.num_leaves, t
+        for a in ("split_feature", "threshold_in_bin", "decision_type", "left_child", "right_child", "leaf_count",
+                  "threshold"):
+            np.testing.assert_array_equal(getattr(gt, a), getattr(ot, a))
+        np.testing.assert_allclose(gt.leaf_value, ot.leaf_value, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("quant,fraction,freq", [(False, 0.3, 2), (False, 0.8, 1), (True, 0.4, 1), (True, 0.7, 3)])
+def test_bagging_matches_oracle(quant, fraction, freq):
+    X, y = datagen.binary_100k(60000)
+    params = dict(P, bagging_fraction=fraction, bagging_freq=freq, bagging_seed=11)
+    if quant:
+        params.update(use_quantized_grad=True, num_grad_quant_bins=16, quant_train_renew_leaf=True)
+    print(f"bagging quant={quant} fraction={fraction} freq={freq}", flush=True)
+    b = lgb.Booster(params, lgb.Dataset(X, label=y, params=params))
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    iters = 6
+    for _ in range(iters):
+        b.update()
+        ob.update()
+    bag = ob.bag()
+    assert 0 < len(bag) < len(y)
+    _compare(b, ob, iters)
+    np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
+
+
+def test_multiclass_bagging_matches_oracle():
+    rng = np.random.default_rng(12)
+    n, K = 40000, 3
+    X = rng.normal(size=(n, 8)).astype(np.float32)
+    y = np.argmax(X[:, :K] + rng.gumbel(size=(n, K)), 1).astype(np.float32)
+    params = dict(objective="multiclass", num_class=K, num_leaves=15, max_bin=63, bagging_fraction=0.5,
+                  bagging_freq=1)
+    b = lgb.Booster(params, lgb.Dataset(X, label=y, params=params))
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    for _ in range(3):
+        b.update()
+        ob.update()
+    _compare(b, ob, 3 * K)
+    np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
+
+
+def test_quantized_full_data_bag_is_unsupported():
+    X, y = datagen.binary_100k(5000)
+    params = dict(P, use_quantized_grad=True, bagging_fraction=0.9, bagging_freq=1)
+    b = lgb.Booster(params, lgb.Dataset(X, label=y, params=params))
+    with pytest.raises(lgb.LightGBMError, match="Unsupported"):
+        b.update()

@@ -4,7 +4,7 @@ trees trained from them identical to the host-binned ones."""
 import numpy as np
 import pytest
 
-from tests._pkg import lgb
+from tests._pkg import DeviceArray, lgb
 
 pytestmark = pytest.mark.gpu
 
@@ -27,15 +27,15 @@ def _data(n, seed):
 
 @pytest.mark.parametrize("max_bin", [15, 255])
 def test_device_bins_equal_host_bins(max_bin):
-    import torch
     X, y = _data(120001, 1)
     params = dict(objective="binary", num_leaves=31, max_bin=max_bin, categorical_feature=[3])
     host = lgb.Dataset(X, label=y, params=params)
     b = lgb.Booster(params, host)
-    dx = torch.from_numpy(X).cuda()
-    out = torch.zeros((len(X), host.num_features), dtype=torch.uint8, device="cuda")
-    b.bin_rows_device(dx.data_ptr(), len(X), X.shape[1], out.data_ptr())
-    np.testing.assert_array_equal(out.cpu().numpy(), host.bins)
+    dx = DeviceArray(X)
+    out = DeviceArray(nbytes=len(X) * host.num_features)
+    b.bin_rows_device(dx.ptr, len(X), X.shape[1], out.ptr)
+    got = out.to_host(np.zeros((len(X), host.num_features), np.uint8))
+    np.testing.assert_array_equal(got, host.bins)
 
 
 @pytest.mark.parametrize("quant", [False, True])

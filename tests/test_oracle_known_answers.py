@@ -519,3 +519,44 @@ def test_metric_ndcg_known_answers_and_sklearn():
         assert ours[m] == pytest.approx(np.mean(ref), rel=1e-12)
     # per-thread partial sums change only the rounding
     assert O.metric_ndcg(s, lab, qb, at, threads=3) == pytest.approx(ours, rel=1e-14)
+
+
+# ---------------------------------------------------------------- bagging (sample_strategy.rs:186-318)
+def _oracle_bag(n, seed, fraction):
+    X = np.random.default_rng(0).normal(size=(n, 3)).astype(np.float32)
+    y = (X[:, 0] > 0).astype(np.float32)
+    p = dict(objective="binary", num_leaves=4, bagging_fraction=fraction, bagging_freq=1, bagging_seed=seed)
+    ob = O.Booster(O.Dataset(X, y, p), p)
+    ob.update()
+    return ob.bag(), ob
+
+
+def test_bagging_draw_is_seeded_per_1024_row_block():
+    n = 5 * 1024
+    a, _ = _oracle_bag(n, 7, 0.37)
+    b, _ = _oracle_bag(n, 8, 0.37)
+    # block j's generator is Random(seed + j): seed 8's block j draws what seed 7's block j + 1 draws
+    for j in range(4):
+        aj = a[(a >= (j + 1) * 1024) & (a < (j + 2) * 1024)] - 1024
+        bj = b[(b >= j * 1024) & (b < (j + 1) * 1024)]
+        np.testing.assert_array_equal(aj, bj)
+    assert np.all(np.diff(a) > 0)
+    assert abs(len(a) / n - 0.37) < 0.03
+    # the first draw of a block is Random(seed + j)'s first float (random.rs:19-33)
+    for j in range(5):
+        x = ((7 + j) * 214013 + 2531011) & 0xFFFFFFFF
+        first = np.float32(((x >> 16) & 0x7FFF) / np.float32(32768.0))
+        assert (j * 1024 in set(a.tolist())) == (float(first) < 0.37)
+
+
+def test_bagging_generators_run_on_between_draws():
+    n = 3000
+    X = np.random.default_rng(1).normal(size=(n, 3)).astype(np.float32)
+    y = (X[:, 0] > 0).astype(np.float32)
+    p = dict(objective="binary", num_leaves=4, bagging_fraction=0.5, bagging_freq=1, bagging_seed=3)
+    ob = O.Booster(O.Dataset(X, y, p), p)
+    ob.update()
+    first = ob.bag().copy()
+    ob.update()
+    second = ob.bag()
+    assert not np.array_equal(first, second)
