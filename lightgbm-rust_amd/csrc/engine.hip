@@ -459,8 +459,20 @@ static void launch_bin_rows(Ctx* c, const float* dX, int64_t rows, int ncol, uin
   if (ncol != c->ncol) throw Error(LGBMHIP_INVALID_DATA, "column count differs from the training data");
   if (rows <= 0) return;
   const int64_t work = rows * ((stride + 3) / 4);
-  hipLaunchKernelGGL(k_bin_rows, dim3(grid_for(work, 256, c->cus * 16)), dim3(256), 0, c->stream, dX, (long long)rows,
-                     ncol, c->d_binfeat.p, c->F, c->d_bounds.p, c->d_ckeys.p, c->d_cbins.p, dout, stride);
+  const int nbounds = (int)c->d_bounds.n;
+  const size_t lds = (size_t)nbounds * 8;
+  if (lds <= 64 * 1024) {
+    // every feature's bounds in LDS: 1024-thread blocks, two per CU at 64 KiB
+    HIP_CHECK(hipFuncSetAttribute((const void*)k_bin_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
+    hipLaunchKernelGGL(k_bin_rows<true>, dim3(grid_for(work, 1024, c->cus * 4)), dim3(1024), lds, c->stream, dX,
+                       (long long)rows, ncol, c->d_binfeat.p, c->F, c->d_bounds.p, nbounds, c->d_ckeys.p,
+                       c->d_cbins.p, dout, stride);
+  } else {
+    hipLaunchKernelGGL(k_bin_rows<false>, dim3(grid_for(work, 256, c->cus * 16)), dim3(256), 0, c->stream, dX,
+                       (long long)rows, ncol, c->d_binfeat.p, c->F, c->d_bounds.p, nbounds, c->d_ckeys.p,
+                       c->d_cbins.p, dout, stride);
+  }
   HIP_CHECK(hipGetLastError());
 }
 

@@ -1519,9 +1519,19 @@ __device__ __forceinline__ unsigned value_to_bin_dev(const BinFeat& bf, const do
 
 // Bins rows x ncol f32 values into u8 rows of `stride` bytes (features F..stride-1
 // zero): one thread per 4 bytes of an output row, values loaded before use.
-__global__ void __launch_bounds__(256) k_bin_rows(const float* X, long long n, int ncol, const BinFeat* feats, int F,
-                                                  const double* bounds, const int* ckeys, const int* cbins,
-                                                  uint8_t* out, int stride) {
+// STAGE: the bin bounds of every feature (nbounds doubles) are copied to LDS
+// first, so the binary searches read LDS.
+template <bool STAGE>
+__global__ void __launch_bounds__(1024) k_bin_rows(const float* X, long long n, int ncol, const BinFeat* feats, int F,
+                                                   const double* gbounds, int nbounds, const int* ckeys,
+                                                   const int* cbins, uint8_t* out, int stride) {
+  extern __shared__ double bin_lds[];
+  const double* bounds = gbounds;
+  if (STAGE) {
+    for (int i = threadIdx.x; i < nbounds; i += blockDim.x) bin_lds[i] = gbounds[i];
+    __syncthreads();
+    bounds = bin_lds;
+  }
   const int W = (stride + 3) / 4;
   const long long total = n * W;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
