@@ -199,6 +199,7 @@ struct Lane {
   DevBuf<PosRec> pos;
   DevBuf<LeafDev> leaves;
   DevBuf<long long> renew;
+  DevBuf<uint16_t> leaf16;     // each row's leaf from the renewal walk (quantized renew)
   ~Lane() {
     if (done_ev) (void)hipEventDestroy(done_ev);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -679,6 +680,7 @@ static void setup_training(Ctx* c) {
     ln.leaves.alloc(MS);
     ln.leaves.zero(c->stream);
     ln.renew.alloc((size_t)2 * L);
+    if (quant && o.quant_train_renew_leaf) ln.leaf16.alloc(N);
     ln.renew.zero(c->stream);
   }
   if (!c->grad_ev) HIP_CHECK(hipEventCreateWithFlags(&c->grad_ev, hipEventDisableTiming));
@@ -992,6 +994,8 @@ static void enqueue_tail(Ctx* c, Lane& ln, int k, bool score_update, double lr) 
     if (c->bag_on) {
       w.rows = c->bag_rows.p;
       w.N = c->bag_m;
+    } else {
+      w.leaf16 = ln.leaf16.p;   // every row's leaf, for the score update below
     }
     launch_walk(w, c->bag_on ? grid_for(c->bag_m, 256, c->cus * 4) : wgrid, s);
   }
@@ -1002,7 +1006,11 @@ static void enqueue_tail(Ctx* c, Lane& ln, int k, bool score_update, double lr) 
     WalkArgs w = walk_args(c, ln, 1);
     w.val = c->leafval.p + (size_t)k * c->L;
     w.score = c->score.p + (size_t)k * N;
-    launch_walk(w, wgrid, s);
+    if (renew && !c->bag_on)
+      hipLaunchKernelGGL(k_score_leaf, dim3(grid_for(N, 1024, c->cus * 4)), dim3(256), 0, s, ln.st.p, ln.leaf16.p,
+                         w.val, w.score, (long long)N);
+    else
+      launch_walk(w, wgrid, s);
     // every validation set: the same walk over its bins (tree.rs:635-652)
     for (auto& v : c->valid) {
       WalkArgs wv = w;

@@ -1542,6 +1542,7 @@ struct WalkArgs {
   const float* g;
   const float* h;
   const unsigned* rows;  // MODE 0 over a bag: row of each position (null: every row in order)
+  uint16_t* leaf16;      // MODE 0: each row's leaf, when given (the score update then reads it)
   const int* shifts;
   long long* sums;       // [L][2]
   const double* val;     // [L]
@@ -1654,6 +1655,7 @@ __global__ void __launch_bounds__(256) k_walk(WalkArgs a) {
       if (MODE == 0) {
         atomicAdd(&lsum[2 * l], (unsigned long long)to_fixed(gg[u], lsh[2 * l]));
         atomicAdd(&lsum[2 * l + 1], (unsigned long long)to_fixed(hh[u], lsh[2 * l + 1]));
+        if (a.leaf16) a.leaf16[rc[u]] = (uint16_t)l;
       } else if (MODE == 1) {
         a.score[r] = sc[u] + lval[l];
       } else {
@@ -1667,6 +1669,36 @@ __global__ void __launch_bounds__(256) k_walk(WalkArgs a) {
       const unsigned long long vg = lsum[2 * l], vh = lsum[2 * l + 1];
       if (vg) atomicAdd((unsigned long long*)&a.sums[2 * l], vg);
       if (vh) atomicAdd((unsigned long long*)&a.sums[2 * l + 1], vh);
+    }
+  }
+}
+
+// Score update from the leaf ids the renewal walk left (boosting.rs:1381-1408,
+// partition.rs:178-191): score[i] += val[leaf[i]], the score walk's own f64
+// addition, without walking the tree again. No-op while the tree grows.
+__global__ void __launch_bounds__(256) k_score_leaf(const StepDev* st, const uint16_t* __restrict__ leaf,
+                                                    const double* __restrict__ val, double* __restrict__ score,
+                                                    long long n) {
+  __shared__ double lv[1024];
+  if (!st->done) return;
+  const int nl = st->num_splits + 1;
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) lv[l] = val[l];
+  __syncthreads();
+  constexpr int U = 4;
+  const long long step = (long long)gridDim.x * blockDim.x * U;
+  for (long long base = (long long)blockIdx.x * blockDim.x * U + threadIdx.x; base < n; base += step) {
+    int lf[U];
+    double sc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = min(base + (long long)u * blockDim.x, n - 1);
+      lf[u] = leaf[i];
+      sc[u] = score[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = base + (long long)u * blockDim.x;
+      if (i < n) score[i] = sc[u] + lv[lf[u]];
     }
   }
 }
