@@ -34,6 +34,9 @@
  *   lgbmhip_add_valid         <- Gbdt::add_valid                boosting.rs:378-411
  *   lgbmhip_copy_valid_scores <- Gbdt::valid_scores             boosting.rs:1282-1284
  *   lgbmhip_eval              <- Metric::eval                   metric.rs:525-545, 551-729
+ *   lgbmhip_save_model        <- GBDT::SaveModelToString        io.rs:13-88, tree.rs:718-770
+ *   lgbmhip_load_model        <- GBDT::LoadModelFromString      io.rs:156-343
+ *   lgbmhip_bins_feature_infos <- Dataset::feature_infos        dataset.rs:770-776
  */
 #ifndef LGBM_HIP_H_
 #define LGBM_HIP_H_
@@ -244,6 +247,24 @@ lgbmhip_status lgbmhip_copy_valid_scores(lgbmhip_ctx* ctx, int32_t index, double
  * a fixed order of their own (within 1e-10 relative). */
 lgbmhip_status lgbmhip_eval(lgbmhip_ctx* ctx, int32_t data, const lgbmhip_metric_opts* opts, double* out,
                             int32_t* out_count);
+
+/* Model text of the trained trees (GBDT::SaveModelToString, io.rs:13-88):
+ * header, tree_sizes, every tree's block (tree.rs:718-770, numbers as C's
+ * %.17g / %g), "end of trees" and the split-count feature importances. The
+ * parameters section (the whole config) is not written. feature_names:
+ * space-separated, one per column (NULL: Column_0 ...); feature_infos: as
+ * lgbmhip_bins_feature_infos gives them. num_iteration <= 0: every tree.
+ * *out_len receives the text's length; out (capacity bytes) receives the
+ * NUL-terminated text when it fits (out = NULL only sizes it). */
+lgbmhip_status lgbmhip_save_model(lgbmhip_ctx* ctx, int32_t num_iteration, const char* feature_names,
+                                  const char* feature_infos, char* out, int64_t capacity, int64_t* out_len);
+/* Trees of a model text (GBDT::LoadModelFromString, io.rs:156-343) into an
+ * empty context, for raw prediction (lgbmhip_predict_raw / _device with X);
+ * linear trees are Unsupported. */
+lgbmhip_status lgbmhip_load_model(lgbmhip_ctx* ctx, const char* text);
+/* Dataset::feature_infos (dataset.rs:770-776): per raw column "[min:max]",
+ * the categories of a categorical column joined by ':', or "none". */
+lgbmhip_status lgbmhip_bins_feature_infos(const lgbmhip_bins* b, char* out, int64_t capacity, int64_t* out_len);
 
 /* Profiling of training iterations: enable = 1 records HIP events around every
  * histogram, slab-reduction, split and partition launch (graphs off), 0 stops,

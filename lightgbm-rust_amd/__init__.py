@@ -185,6 +185,14 @@ class Dataset:
                                            ptr(self._bins)))
         return self._bins
 
+    def feature_infos(self):
+        """Dataset::feature_infos: '[min:max]', categories 'a:b', or 'none' per raw column."""
+        n = C.c_int64(0)
+        check(lib().lgbmhip_bins_feature_infos(self._handle, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(lib().lgbmhip_bins_feature_infos(self._handle, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
+
     def feature_meta(self, f):
         m = self.metas[f]
         return dict(real_index=m.real_index, num_bin=m.num_bin, missing_type=m.missing_type,
@@ -207,9 +215,12 @@ class Tree:
 class Booster:
     """Training booster on the GPU engine (basic.py:1364 / py/lib.rs:435 RsBooster)."""
 
-    def __init__(self, params=None, train_set=None):
+    def __init__(self, params=None, train_set=None, model_file=None, model_str=None):
         if train_set is None:
-            raise LightGBMError("Booster needs a train_set (model-file loading is outside the GPU path)")
+            if model_file is None and model_str is None:
+                raise LightGBMError("Booster needs a train_set, a model_file or a model_str")
+            self._load(params, model_file, model_str)
+            return
         self.params = _params({**(train_set.params or {}), **(params or {})})
         p = self.params
         obj = str(p["objective"])
@@ -254,6 +265,44 @@ class Booster:
         self._train_name = "training"
         self.best_iteration = 0
         self.best_score = {}
+
+    def _load(self, params, model_file, model_str):
+        """A prediction-only booster from model text (GBDT::LoadModelFromString)."""
+        if model_str is None:
+            with open(model_file, "r") as f:
+                model_str = f.read()
+        self.params = _params(params)
+        h = C.c_void_p()
+        check(lib().lgbmhip_create(int(self.params["gpu_device_id"]), C.byref(h)))
+        self._ctx = h
+        check(lib().lgbmhip_load_model(h, model_str.encode()), h)
+        head = dict(l.split("=", 1) for l in model_str.split("end of trees")[0].splitlines() if "=" in l and
+                    not l.startswith("Tree="))
+        self.K = int(head["num_tree_per_iteration"])
+        obj = head.get("objective", "").split(" ")
+        self.params["objective"] = obj[0] if obj[0] else "binary"
+        for tok in obj[1:]:
+            if tok.startswith("sigmoid:"):
+                self.params["sigmoid"] = float(tok.split(":", 1)[1])
+        self.train_set = None
+        self._valid, self._metrics = [], []
+        self.best_iteration = 0
+        self.best_score = {}
+
+    def model_to_string(self, num_iteration=None):
+        """Model text (GBDT::SaveModelToString) without the parameters section."""
+        ni = -1 if num_iteration is None else int(num_iteration)
+        infos = self.train_set.feature_infos().encode() if self.train_set is not None else b""
+        n = C.c_int64(0)
+        check(lib().lgbmhip_save_model(self._ctx, ni, None, infos, None, 0, C.byref(n)), self._ctx)
+        buf = C.create_string_buffer(n.value + 1)
+        check(lib().lgbmhip_save_model(self._ctx, ni, None, infos, buf, n.value + 1, C.byref(n)), self._ctx)
+        return buf.value.decode()
+
+    def save_model(self, filename, num_iteration=None):
+        with open(filename, "w") as f:
+            f.write(self.model_to_string(num_iteration))
+        return self
 
     # --- training (basic.py:1519 update -> py/lib.rs:479-482 -> Gbdt::train_one_iter)
     def update(self):

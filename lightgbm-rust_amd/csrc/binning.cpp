@@ -218,6 +218,8 @@ Mapper build_mapper(std::vector<double> vals, int64_t total, bool categorical, c
   Mapper m;
   m.missing_type = mt;
   m.bin_type = categorical ? 1 : 0;
+  m.min_val = d.v.front();
+  m.max_val = d.v.back();
   if (categorical) {
     m.bounds.clear();
     std::vector<std::pair<int, int>> cats;  // (count, category) in ascending category order

@@ -25,6 +25,7 @@ struct Mapper {
   std::vector<double> bounds{INFINITY};
   bool is_trivial = true;
   double sparse_rate = 1.0;
+  double min_val = 0.0, max_val = 0.0;   // smallest / largest distinct sample value (binning.rs:453-454)
   int bin_type = 0;
   uint32_t default_bin = 0, most_freq_bin = 0;
   std::vector<int> categories;

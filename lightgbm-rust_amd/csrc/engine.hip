@@ -313,6 +313,7 @@ struct Ctx {
   std::vector<std::unique_ptr<ValidData>> valid;
   MetricWs* mws = nullptr;
   DevBuf<int> d_qb_eval;   // training query boundaries for NDCG when the objective keeps none
+  bool loaded_model = false;   // trees from model text: raw prediction only
   // prediction cache
   bool pred_dirty = true;
   DevBuf<uint4> p_nodes;
@@ -1476,6 +1477,8 @@ static void build_pred(Ctx* c, bool binned) {
 // a device [rows][K] output on the context's stream.
 static void launch_predict(Ctx* c, const float* dX, const uint8_t* dbins, int64_t rows, int ncol, double* dout) {
   const bool binned = dbins != nullptr;
+  if (binned && c->loaded_model) throw Error(LGBMHIP_UNSUPPORTED, "binned prediction of a model loaded from text");
+  if (c->trees.empty()) throw Error(LGBMHIP_INVALID_PARAMETER, "no trees to predict with");
   if (c->pred_dirty || c->p_binned != binned) build_pred(c, binned);
   if (rows <= 0) return;
   PredArgs a{};
@@ -1714,6 +1717,219 @@ static int eval_metric(Ctx* c, int data, const lgbmhip_metric_opts* o, double* o
     default:
       throw Error(LGBMHIP_INVALID_PARAMETER, "unknown metric id");
   }
+}
+
+// ---------------------------------------------------------------- model text
+// C printf's %.{p}g, as upstream writes model numbers (fmt.rs:6-44).
+static std::string fmt_g(double v, int prec) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.*g", prec, v);
+  return buf;
+}
+template <class T, class F>
+static std::string join_fmt(const std::vector<T>& v, size_t n, F f) {
+  std::string s;
+  for (size_t i = 0; i < n && i < v.size(); ++i) {
+    if (i) s += ' ';
+    s += f(v[i]);
+  }
+  return s;
+}
+
+// Tree::to_model_string (tree.rs:718-770) of a non-linear tree.
+static std::string tree_text(const HostTree& t) {
+  const size_t n = (size_t)t.num_leaves, ni = n - 1;
+  auto i2s = [](auto x) { return std::to_string((long long)x); };
+  auto g17 = [](double x) { return fmt_g(x, 17); };
+  auto g6 = [](double x) { return fmt_g(x, 6); };
+  std::string s;
+  s += "num_leaves=" + std::to_string(n) + "\n";
+  s += "num_cat=" + std::to_string(t.num_cat) + "\n";
+  s += "split_feature=" + join_fmt(t.split_feature, ni, i2s) + "\n";
+  s += "split_gain=" + join_fmt(t.split_gain, ni, [&](float x) { return g6((double)x); }) + "\n";
+  s += "threshold=" + join_fmt(t.threshold, ni, g17) + "\n";
+  s += "decision_type=" + join_fmt(t.decision_type, ni, [](int8_t x) { return std::to_string((int)x); }) + "\n";
+  s += "left_child=" + join_fmt(t.left_child, ni, i2s) + "\n";
+  s += "right_child=" + join_fmt(t.right_child, ni, i2s) + "\n";
+  s += "leaf_value=" + join_fmt(t.leaf_value, n, g17) + "\n";
+  s += "leaf_weight=" + join_fmt(t.leaf_weight, n, g17) + "\n";
+  s += "leaf_count=" + join_fmt(t.leaf_count, n, i2s) + "\n";
+  s += "internal_value=" + join_fmt(t.internal_value, ni, g6) + "\n";
+  s += "internal_weight=" + join_fmt(t.internal_weight, ni, g6) + "\n";
+  s += "internal_count=" + join_fmt(t.internal_count, ni, i2s) + "\n";
+  if (t.num_cat > 0) {
+    s += "cat_boundaries=" + join_fmt(t.cat_boundaries, (size_t)t.num_cat + 1, i2s) + "\n";
+    s += "cat_threshold=" + join_fmt(t.cat_threshold, t.cat_threshold.size(), i2s) + "\n";
+  }
+  s += "is_linear=0\n";
+  s += "shrinkage=" + g6(t.shrinkage) + "\n\n";
+  return s;
+}
+
+// GBDT::SaveModelToString (io.rs:13-88) up to the feature importances; the
+// parameters section (the whole config, config.rs:1169) is not written.
+static std::string save_model(Ctx* c, int num_iteration, const char* names_in, const char* infos_in) {
+  if (c->trees.empty()) throw Error(LGBMHIP_INVALID_PARAMETER, "no trees to save");
+  const int K = c->K, ncol = c->ncol;
+  std::vector<std::string> names;
+  if (names_in && *names_in) {
+    std::string cur;
+    for (const char* p = names_in;; ++p) {
+      if (*p == ' ' || *p == 0) {
+        if (!cur.empty()) names.push_back(cur);
+        cur.clear();
+        if (*p == 0) break;
+      } else {
+        cur += *p;
+      }
+    }
+  } else {
+    for (int i = 0; i < ncol; ++i) names.push_back("Column_" + std::to_string(i));
+  }
+  if ((int)names.size() != ncol) throw Error(LGBMHIP_INVALID_PARAMETER, "one feature name per column");
+  std::string s = "tree\nversion=v4\n";
+  s += "num_class=" + std::to_string(c->opts.objective == 1 ? c->opts.num_class : 1) + "\n";
+  s += "num_tree_per_iteration=" + std::to_string(K) + "\n";
+  s += "label_index=0\n";
+  s += "max_feature_idx=" + std::to_string(ncol - 1) + "\n";
+  // objective to_model_string: binary.rs:168-170, multiclass.rs:169-171, rank.rs:338-340
+  if (c->opts.objective == 0) s += "objective=binary sigmoid:" + fmt_g(c->opts.sigmoid, 6) + "\n";
+  else if (c->opts.objective == 1) s += "objective=multiclass num_class:" + std::to_string(c->opts.num_class) + "\n";
+  else s += "objective=lambdarank\n";
+  std::string nj;
+  for (size_t i = 0; i < names.size(); ++i) nj += (i ? " " : "") + names[i];
+  s += "feature_names=" + nj + "\n";
+  s += "feature_infos=" + std::string(infos_in ? infos_in : "") + "\n";
+  size_t used = c->trees.size();
+  if (num_iteration > 0) used = std::min(used, (size_t)num_iteration * K);
+  std::vector<std::string> ts;
+  std::string sizes;
+  for (size_t i = 0; i < used; ++i) {
+    ts.push_back("Tree=" + std::to_string(i) + "\n" + tree_text(c->trees[i]) + "\n");
+    sizes += (i ? " " : "") + std::to_string(ts.back().size());
+  }
+  s += "tree_sizes=" + sizes + "\n\n";
+  for (auto& t : ts) s += t;
+  s += "end of trees\n";
+  // split-count importances (boosting.rs:1337-1360), stable by count descending
+  std::vector<double> imp(ncol, 0.0);
+  for (size_t i = 0; i < used; ++i) {
+    const HostTree& t = c->trees[i];
+    for (int k = 0; k + 1 < t.num_leaves; ++k)
+      if (t.split_gain[k] > 0.0f && t.split_feature[k] >= 0 && t.split_feature[k] < ncol) imp[t.split_feature[k]] += 1.0;
+  }
+  std::vector<std::pair<size_t, int>> pairs;
+  for (int f = 0; f < ncol; ++f)
+    if ((size_t)imp[f] > 0) pairs.emplace_back((size_t)imp[f], f);
+  std::stable_sort(pairs.begin(), pairs.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  s += "\nfeature_importances:\n";
+  for (auto& pr : pairs) s += names[pr.second] + "=" + std::to_string(pr.first) + "\n";
+  return s;
+}
+
+// GBDT::LoadModelFromString (io.rs:156-343) for raw prediction: the header's
+// class and column counts and each tree's arrays (numbers parsed correctly
+// rounded, as parse_f64 does).
+static void load_model(Ctx* c, const char* text) {
+  if (!text) throw Error(LGBMHIP_INVALID_PARAMETER, "no model text");
+  std::vector<std::pair<std::string, std::string>> lines;
+  {
+    const char* p = text;
+    while (*p) {
+      const char* e = std::strchr(p, '\n');
+      std::string ln = e ? std::string(p, e - p) : std::string(p);
+      if (!ln.empty() && ln.back() == '\r') ln.pop_back();
+      const size_t eq = ln.find('=');
+      if (eq != std::string::npos) lines.emplace_back(ln.substr(0, eq), ln.substr(eq + 1));
+      else lines.emplace_back(ln, "");
+      if (!e) break;
+      p = e + 1;
+    }
+  }
+  auto nums = [](const std::string& v) {
+    std::vector<double> out;
+    const char* p = v.c_str();
+    while (*p) {
+      while (*p == ' ') ++p;
+      if (!*p) break;
+      char* e;
+      const double d = std::strtod(p, &e);
+      if (e == p) throw Error(LGBMHIP_MODEL_FORMAT, "bad number in model text");
+      out.push_back(d);
+      p = e;
+    }
+    return out;
+  };
+  int ntpi = -1, max_feat = -1, num_class = 1;
+  std::string objective;
+  std::vector<HostTree> trees;
+  size_t i = 0;
+  for (; i < lines.size() && lines[i].first != "Tree"; ++i) {
+    const auto& kv = lines[i];
+    if (kv.first == "num_tree_per_iteration") ntpi = std::atoi(kv.second.c_str());
+    else if (kv.first == "max_feature_idx") max_feat = std::atoi(kv.second.c_str());
+    else if (kv.first == "num_class") num_class = std::atoi(kv.second.c_str());
+    else if (kv.first == "objective") objective = kv.second;
+  }
+  if (ntpi < 1 || max_feat < 0) throw Error(LGBMHIP_MODEL_FORMAT, "model text without num_tree_per_iteration / max_feature_idx");
+  while (i < lines.size() && lines[i].first == "Tree") {
+    HostTree t;
+    std::unordered_map<std::string, std::string> kv;
+    for (++i; i < lines.size() && !lines[i].first.empty() && lines[i].first != "Tree" &&
+              lines[i].first != "end of trees";
+         ++i)
+      kv[lines[i].first] = lines[i].second;
+    while (i < lines.size() && lines[i].first.empty()) ++i;
+    if (!kv.count("num_leaves")) throw Error(LGBMHIP_MODEL_FORMAT, "tree without num_leaves");
+    if (kv.count("is_linear") && std::atoi(kv["is_linear"].c_str()) != 0)
+      throw Error(LGBMHIP_UNSUPPORTED, "linear trees");
+    t.num_leaves = std::atoi(kv["num_leaves"].c_str());
+    t.num_cat = kv.count("num_cat") ? std::atoi(kv["num_cat"].c_str()) : 0;
+    const int n = t.num_leaves, ni = n - 1;
+    auto take = [&](const char* key, size_t want) {
+      std::vector<double> v = kv.count(key) ? nums(kv[key]) : std::vector<double>();
+      if (v.size() < want) throw Error(LGBMHIP_MODEL_FORMAT, std::string("tree field ") + key + " is short");
+      return v;
+    };
+    if (n < 1) throw Error(LGBMHIP_MODEL_FORMAT, "num_leaves < 1");
+    const auto lv = take("leaf_value", n);
+    t.leaf_value.assign(lv.begin(), lv.begin() + n);
+    t.leaf_count.assign(n, 0);
+    if (ni > 0) {
+      const auto sf = take("split_feature", ni), th = take("threshold", ni), dt = take("decision_type", ni),
+                 lc = take("left_child", ni), rc = take("right_child", ni);
+      for (int k = 0; k < ni; ++k) {
+        t.split_feature.push_back((int)sf[k]);
+        t.split_feature_inner.push_back((int)sf[k]);
+        t.threshold.push_back(th[k]);
+        t.threshold_in_bin.push_back(0);
+        t.decision_type.push_back((int8_t)(int)dt[k]);
+        t.left_child.push_back((int)lc[k]);
+        t.right_child.push_back((int)rc[k]);
+        t.split_gain.push_back(0.0f);
+        if ((int)sf[k] > max_feat) throw Error(LGBMHIP_MODEL_FORMAT, "split feature beyond max_feature_idx");
+        if (lc[k] >= ni || rc[k] >= ni || lc[k] < -n || rc[k] < -n) throw Error(LGBMHIP_MODEL_FORMAT, "child out of range");
+      }
+    }
+    if (t.num_cat > 0) {
+      const auto cb = take("cat_boundaries", (size_t)t.num_cat + 1);
+      t.cat_boundaries.assign(cb.begin(), cb.begin() + t.num_cat + 1);
+      const auto ct = nums(kv["cat_threshold"]);
+      for (double x : ct) t.cat_threshold.push_back((uint32_t)(long long)x);
+      if ((size_t)t.cat_boundaries.back() != t.cat_threshold.size()) throw Error(LGBMHIP_MODEL_FORMAT, "cat_threshold size");
+    }
+    trees.push_back(std::move(t));
+  }
+  if (trees.empty()) throw Error(LGBMHIP_MODEL_FORMAT, "model text without trees");
+  c->trees = std::move(trees);
+  c->K = ntpi;
+  c->ncol = max_feat + 1;
+  c->opts.objective = objective.rfind("multiclass", 0) == 0 ? 1 : objective.rfind("lambdarank", 0) == 0 ? 2 : 0;
+  c->opts.num_class = num_class;
+  c->opts_set = true;
+  c->pred_dirty = true;
+  c->state_ready = false;
+  c->loaded_model = true;
 }
 
 static void predict(Ctx* c, const float* X, const uint8_t* bins, int64_t rows, int ncol, double* out) {
@@ -2264,6 +2480,40 @@ lgbmhip_status lgbmhip_eval(lgbmhip_ctx* ctx, int32_t data, const lgbmhip_metric
     if (!opts || !out) throw Error(LGBMHIP_INVALID_PARAMETER, "metric options and output are required");
     const int cnt = eval_metric(&ctx->c, data, opts, out);
     if (out_count) *out_count = cnt;
+  });
+}
+// A string result: its length always, the text (NUL-terminated) when it fits.
+static void put_text(const std::string& s, char* out, int64_t capacity, int64_t* out_len) {
+  if (out_len) *out_len = (int64_t)s.size();
+  if (!out) return;
+  if (capacity <= (int64_t)s.size()) throw Error(LGBMHIP_INVALID_PARAMETER, "output buffer too small for the text");
+  std::memcpy(out, s.c_str(), s.size() + 1);
+}
+lgbmhip_status lgbmhip_save_model(lgbmhip_ctx* ctx, int32_t num_iteration, const char* feature_names,
+                                  const char* feature_infos, char* out, int64_t capacity, int64_t* out_len) {
+  return guarded(ctx, [&] {
+    put_text(save_model(&ctx->c, num_iteration, feature_names, feature_infos), out, capacity, out_len);
+  });
+}
+lgbmhip_status lgbmhip_load_model(lgbmhip_ctx* ctx, const char* text) {
+  return guarded(ctx, [&] { load_model(&ctx->c, text); });
+}
+lgbmhip_status lgbmhip_bins_feature_infos(const lgbmhip_bins* b, char* out, int64_t capacity, int64_t* out_len) {
+  return guarded(nullptr, [&] {
+    // Dataset::feature_infos (dataset.rs:770-776), BinMapper::bin_info_string (binning.rs:680-686)
+    std::string s;
+    for (int c = 0; c < b->b->ncol; ++c) {
+      const Mapper& m = b->b->mappers[c];
+      if (c) s += ' ';
+      if (m.is_trivial) {
+        s += "none";
+      } else if (m.bin_type == 1) {
+        for (size_t i = 0; i < m.categories.size(); ++i) s += (i ? ":" : "") + std::to_string(m.categories[i]);
+      } else {
+        s += "[" + fmt_g(m.min_val, 17) + ":" + fmt_g(m.max_val, 17) + "]";
+      }
+    }
+    put_text(s, out, capacity, out_len);
   });
 }
 lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out12) {

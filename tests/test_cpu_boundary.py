@@ -108,3 +108,19 @@ def test_find_mappers_are_the_bin_dense_mappers():
     out = np.zeros(dev.bins.size, np.uint8)
     with pytest.raises(lgb.LightGBMError, match="no bin matrix"):
         capi.check(capi.lib().lgbmhip_bins_matrix(dev._handle, capi.ptr(out)))
+
+
+def test_feature_infos_are_sample_min_max_and_categories():
+    rng = np.random.default_rng(41)
+    X = rng.normal(size=(2000, 4)).astype(np.float32)
+    X[:, 1] = np.abs(X[:, 1]) + 1.0       # positive only: 0 is not a sample value (no zeros)
+    X[:, 2] = 7.0                          # constant: unused
+    X[:, 3] = rng.integers(0, 4, len(X)).astype(np.float32)
+    ds = lgb.Dataset(X, label=(X[:, 0] > 0).astype(np.float32), params=dict(categorical_feature=[3]))
+    infos = ds.feature_infos().split(" ")
+    assert infos[0] == "[%.17g:%.17g]" % (float(X[:, 0].min()), float(X[:, 0].max()))
+    assert infos[1] == "[%.17g:%.17g]" % (float(X[:, 1].min()), float(X[:, 1].max()))
+    assert infos[2] == "none"
+    # the categories of the bins in bin order, bin 0 holding -1 (binning.rs:704-779 order [-1, 3, 1, 0, 2])
+    cats = [int(c) for c in infos[3].split(":")]
+    assert cats[0] == -1 and sorted(cats[1:]) == [0, 1, 2, 3]
