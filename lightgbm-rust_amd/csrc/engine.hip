@@ -356,6 +356,31 @@ static int grid_for(int64_t n, int threads, int cap) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
 }
 
+// Buffer-0 bins of rows[0..m) (every row when rows is null) into the
+// group-major layout of the ordered buffers (row stride N), on stream s.
+static void launch_group_rows(Ctx* c, const unsigned* rows, int64_t m, uint8_t* out, hipStream_t s) {
+  if (m <= 0) return;
+  const int CH = c->hist_ng * c->hist_fb / 16;
+  const bool vec = c->S % 16 == 0 && c->hist_fb % 16 == 0 && c->hist_ng * c->hist_fb <= c->S && CH >= 1 && CH <= 8;
+  if (vec) {
+    const dim3 g(grid_for(m, 256, c->cus * 16)), b(256);
+    switch (CH) {
+      case 1: hipLaunchKernelGGL(k_group_rows_v<1>, g, b, 0, s, rows, (long long)m, c->bins_rm.p, c->S, c->hist_fb, (long long)c->N, out); break;
+      case 2: hipLaunchKernelGGL(k_group_rows_v<2>, g, b, 0, s, rows, (long long)m, c->bins_rm.p, c->S, c->hist_fb, (long long)c->N, out); break;
+      case 3: hipLaunchKernelGGL(k_group_rows_v<3>, g, b, 0, s, rows, (long long)m, c->bins_rm.p, c->S, c->hist_fb, (long long)c->N, out); break;
+      case 4: hipLaunchKernelGGL(k_group_rows_v<4>, g, b, 0, s, rows, (long long)m, c->bins_rm.p, c->S, c->hist_fb, (long long)c->N, out); break;
+      case 5: hipLaunchKernelGGL(k_group_rows_v<5>, g, b, 0, s, rows, (long long)m, c->bins_rm.p, c->S, c->hist_fb, (long long)c->N, out); break;
+      case 6: hipLaunchKernelGGL(k_group_rows_v<6>, g, b, 0, s, rows, (long long)m, c->bins_rm.p, c->S, c->hist_fb, (long long)c->N, out); break;
+      case 7: hipLaunchKernelGGL(k_group_rows_v<7>, g, b, 0, s, rows, (long long)m, c->bins_rm.p, c->S, c->hist_fb, (long long)c->N, out); break;
+      default: hipLaunchKernelGGL(k_group_rows_v<8>, g, b, 0, s, rows, (long long)m, c->bins_rm.p, c->S, c->hist_fb, (long long)c->N, out); break;
+    }
+  } else {
+    hipLaunchKernelGGL(k_group_rows, dim3(grid_for(m * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
+                       s, rows, (long long)m, c->bins_rm.p, c->S, c->F, c->hist_fb, c->hist_ng, (long long)c->N, out);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
 // Row-major stride: rows of <= 32 bytes padded to a power of two (one 32-byte
 // sector per row gather), wider rows to 16 bytes.
 static int row_stride(int F) {
@@ -768,10 +793,7 @@ static void setup_training(Ctx* c) {
     }
     // buffer 0: the root's rows in row order, bins group-major
     c->grp0.alloc(gbytes);
-    hipLaunchKernelGGL(k_group_rows, dim3(grid_for(N * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
-                       c->stream, (const unsigned*)nullptr, (long long)N, c->bins_rm.p, c->S, c->F, c->hist_fb,
-                       c->hist_ng, (long long)N, c->grp0.p);
-    HIP_CHECK(hipGetLastError());
+    launch_group_rows(c, nullptr, N, c->grp0.p, c->stream);
   }
   for (auto p : c->h_log) if (p) (void)hipHostFree(p);
   for (auto p : c->h_raw) if (p) (void)hipHostFree(p);
@@ -1286,7 +1308,7 @@ static void draw_bag(Ctx* c) {
   const int64_t N = c->N, nb = (N + 1023) / 1024;
   hipStream_t s = c->stream;
   c->need_re_bagging = false;
-  hipLaunchKernelGGL(k_bag_draw, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, (long long)N,
+  hipLaunchKernelGGL(k_bag_draw, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, (long long)N,
                      c->bag_lcg.p, c->opts.bagging_fraction, c->bag_flags.p, c->bag_counts.p);
   hipLaunchKernelGGL(k_bag_offsets, dim3(1), dim3(1024), 0, s, c->bag_counts.p, (long long)nb, c->bag_offsets.p,
                      c->bag_total.p);
@@ -1299,10 +1321,7 @@ static void draw_bag(Ctx* c) {
   if (m <= 0) throw Error(LGBMHIP_UNSUPPORTED, "an empty bag");
   c->bag_m = m;
   c->bag_cnt = m;
-  hipLaunchKernelGGL(k_group_rows, dim3(grid_for(m * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
-                     s, (const unsigned*)c->bag_rows.p, (long long)m, c->bins_rm.p, c->S, c->F, c->hist_fb,
-                     c->hist_ng, (long long)N, c->bag_grp0.p);
-  HIP_CHECK(hipGetLastError());
+  launch_group_rows(c, c->bag_rows.p, m, c->bag_grp0.p, s);
 }
 
 // boosting.rs:957-1139 for device=gpu.
@@ -2291,9 +2310,7 @@ static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc
     c->rows_buf.alloc(m);
     HIP_CHECK(hipMemcpyAsync(c->rows_buf.p, rows, m * 4, hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(ln.gh[1].p, pk.data(), pk.size(), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_group_rows, dim3(grid_for(m * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
-                       c->stream, (const unsigned*)c->rows_buf.p, (long long)m, c->bins_rm.p, c->S, c->F, c->hist_fb,
-                       c->hist_ng, (long long)N, ln.grp[1].p);
+    launch_group_rows(c, c->rows_buf.p, m, ln.grp[1].p, c->stream);
   }
   HistArgs a = hist_args(c, ln, quant, 0, 1);
   a.d_begin = 0;

@@ -1413,6 +1413,26 @@ __global__ void k_group_rows(const unsigned* rows, long long m, const uint8_t* _
   }
 }
 
+// k_group_rows for rows whose groups are whole 16-byte chunks inside the row
+// (S and FB multiples of 16, NG * FB <= S): one thread per row, its groups
+// moved as 16-byte vectors, every load issued before the stores.
+template <int CH>   // 16-byte chunks per row (NG * FB / 16)
+__global__ void __launch_bounds__(256) k_group_rows_v(const unsigned* rows, long long m, const uint8_t* __restrict__ rm,
+                                                      int S, int FB, long long N, uint8_t* __restrict__ out) {
+  const int cpg = FB / 16;   // chunks per group
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x) {
+    const long long r = rows ? (long long)rows[i] : i;
+    uint4 v[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) v[c] = *(const uint4*)(rm + r * S + 16 * c);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int g = c / cpg, q = c % cpg;
+      *(uint4*)(out + (size_t)g * N * FB + (size_t)i * FB + 16 * q) = v[c];
+    }
+  }
+}
+
 // ---------------------------------------------------------------- tree end
 // After the last round the tree's leaves are the positions 0..nseq of the
 // current position array. The kernels below run after every launch of

@@ -1419,22 +1419,54 @@ __global__ void __launch_bounds__(64) k_search_one(const double* hist, const lon
 // One thread per block draws its rows in order.
 constexpr int kBagBlock = 1024;
 
-__global__ void k_bag_draw(long long n, unsigned* lcg, double fraction, uint8_t* flags, int* counts) {
-  const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// LCG jump-ahead: the map x -> a x + c applied k times is x -> A x + C (mod 2^32).
+__device__ __forceinline__ void lcg_jump(unsigned k, unsigned& A, unsigned& C) {
+  unsigned ma = 214013u, mc = 2531011u;   // the map applied 2^i times
+  A = 1u;
+  C = 0u;
+  while (k) {
+    if (k & 1u) {
+      A = ma * A;
+      C = ma * C + mc;
+    }
+    mc = ma * mc + mc;
+    ma = ma * ma;
+    k >>= 1;
+  }
+}
+
+// One wave per generator block: lane j draws rows 16 j .. 16 j + 15 of the
+// block from the generator state jumped ahead by 16 j steps, so the draws are
+// the sequential ones; lane 0 stores the state after the block's rows.
+__global__ void __launch_bounds__(256) k_bag_draw(long long n, unsigned* lcg, double fraction, uint8_t* flags,
+                                                  int* counts) {
+  const int lane = threadIdx.x & 63;
+  const long long b = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nb = (n + kBagBlock - 1) / kBagBlock;
   if (b >= nb) return;
-  unsigned x = lcg[b];
-  const long long r0 = b * kBagBlock, r1 = min(n, r0 + kBagBlock);
+  const unsigned x0 = lcg[b];
+  const long long r0 = b * kBagBlock, rows = min((long long)kBagBlock, n - r0);
+  constexpr int PER = kBagBlock / 64;
+  unsigned A, C;
+  lcg_jump((unsigned)(lane * PER), A, C);
+  unsigned x = A * x0 + C;
   int cnt = 0;
-  for (long long i = r0; i < r1; ++i) {
+  for (int t = 0; t < PER; ++t) {
+    const long long i = (long long)lane * PER + t;
+    if (i >= rows) break;
     x = x * 214013u + 2531011u;
     const float f = (float)(int)((x >> 16) & 0x7FFFu) / 32768.0f;
     const int in = (double)f < fraction ? 1 : 0;
-    flags[i] = (uint8_t)in;
+    flags[r0 + i] = (uint8_t)in;
     cnt += in;
   }
-  lcg[b] = x;
-  counts[b] = cnt;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+  if (lane == 0) {
+    lcg_jump((unsigned)rows, A, C);
+    lcg[b] = A * x0 + C;
+    counts[b] = cnt;
+  }
 }
 
 // Exclusive offsets of the per-block counts (one 1024-thread block), and the bag size.
