@@ -108,6 +108,8 @@ typedef struct {
   double bagging_fraction;      /* row bagging (sample_strategy.rs): on when < 1 and bagging_freq > 0 */
   int32_t bagging_freq;
   int32_t bagging_seed;
+  double pos_bagging_fraction;  /* balanced bagging (binary objective): fractions of rows with label > 0 / <= 0 */
+  double neg_bagging_fraction;
 } lgbmhip_train_opts;
 
 /* One metric of lgbmhip_eval (metric.rs:17-43 MetricKind, the Config fields it reads). */

@@ -32,7 +32,7 @@ DEFAULT_PARAMS = dict(
     num_threads=0, device_type="gpu", gpu_device_id=0, use_graph=True, quant_table_threads=1,
     metric="", eval_at=(1, 2, 3, 4, 5), multi_error_top_k=1, label_gain=None, early_stopping_round=0,
     first_metric_only=False, early_stopping_min_delta=0.0, device_binning=False, bagging_fraction=1.0,
-    bagging_freq=0, bagging_seed=3,
+    bagging_freq=0, bagging_seed=3, pos_bagging_fraction=1.0, neg_bagging_fraction=1.0,
 )
 
 ALIASES = {
@@ -52,6 +52,8 @@ ALIASES = {
     "early_stopping": "early_stopping_round", "n_iter_no_change": "early_stopping_round",
     "sub_row": "bagging_fraction", "subsample": "bagging_fraction", "bagging": "bagging_fraction",
     "subsample_freq": "bagging_freq", "bagging_fraction_seed": "bagging_seed",
+    "pos_sub_row": "pos_bagging_fraction", "pos_subsample": "pos_bagging_fraction", "pos_bagging": "pos_bagging_fraction",
+    "neg_sub_row": "neg_bagging_fraction", "neg_subsample": "neg_bagging_fraction", "neg_bagging": "neg_bagging_fraction",
 }
 
 # metric names and aliases on this path (config.rs ParseMetrics, metric.rs:44-76)
@@ -257,7 +259,8 @@ class Booster:
                              int(_b(p["quant_train_renew_leaf"])), int(p["seed"]), int(p["quant_table_threads"]),
                              int(p["lambdarank_truncation_level"]), int(_b(p["lambdarank_norm"])),
                              int(_b(p["use_graph"])), float(p["bagging_fraction"]), int(p["bagging_freq"]),
-                             int(p["bagging_seed"]))
+                             int(p["bagging_seed"]), float(p["pos_bagging_fraction"]),
+                             float(p["neg_bagging_fraction"]))
         check(lib().lgbmhip_set_train_opts(h, C.byref(to)), h)
         self._finished = False
         self._metrics = _metrics(p)

@@ -51,7 +51,8 @@ class TrainOpts(C.Structure):
                 ("stochastic_rounding", C.c_int32), ("quant_train_renew_leaf", C.c_int32), ("seed", C.c_int32),
                 ("quant_table_threads", C.c_int32), ("lambdarank_truncation_level", C.c_int32),
                 ("lambdarank_norm", C.c_int32), ("use_graph", C.c_int32), ("bagging_fraction", C.c_double),
-                ("bagging_freq", C.c_int32), ("bagging_seed", C.c_int32)]
+                ("bagging_freq", C.c_int32), ("bagging_seed", C.c_int32), ("pos_bagging_fraction", C.c_double),
+                ("neg_bagging_fraction", C.c_double)]
 
 
 class MetricOpts(C.Structure):

@@ -303,7 +303,7 @@ struct Ctx {
   // row bagging (sample_strategy.rs:186-318): the bag drawn on the device,
   // the root's rows of every tree until the next draw, and buffer 0's bins
   // for it (bag_grp0, group-major, positions 0 .. bag_m - 1)
-  bool bag_on = false, bag_subset = false, need_re_bagging = false;
+  bool bag_on = false, bag_subset = false, need_re_bagging = false, bag_balanced = false;
   int64_t bag_cnt = 0, bag_m = 0;
   DevBuf<unsigned> bag_rows, bag_lcg;
   DevBuf<uint8_t> bag_flags, bag_grp0;
@@ -766,11 +766,18 @@ static void setup_training(Ctx* c) {
     }
     // row bagging: Bagger setup (sample_strategy.rs:186-214); the feature
     // count stands for the reference's feature groups (dense data)
-    c->bag_on = o.bagging_fraction < 1.0 && o.bagging_freq > 0;
+    // balanced bagging needs the objective's positive count (binary.rs:73-99; 0 otherwise)
+    int64_t num_pos = 0;
+    if (o.objective == 0)
+      for (float l : c->label) num_pos += l > 0.0f ? 1 : 0;
+    c->bag_balanced = (o.pos_bagging_fraction < 1.0 || o.neg_bagging_fraction < 1.0) && num_pos > 0;
+    c->bag_on = (o.bagging_fraction < 1.0 || c->bag_balanced) && o.bagging_freq > 0;
     c->bag_cnt = N;
     if (c->bag_on) {
       if (o.objective == 2) throw Error(LGBMHIP_UNSUPPORTED, "bagging with the lambdarank objective");
-      c->bag_cnt = (int64_t)(o.bagging_fraction * (double)N);
+      c->bag_cnt = c->bag_balanced ? (int64_t)((double)num_pos * o.pos_bagging_fraction) +
+                                         (int64_t)((double)(N - num_pos) * o.neg_bagging_fraction)
+                                   : (int64_t)(o.bagging_fraction * (double)N);
       c->bag_subset = ((double)c->bag_cnt / (double)N) / (double)o.bagging_freq <= 0.5 && F < 100;
       if (quant && !c->bag_subset)
         throw Error(LGBMHIP_UNSUPPORTED,
@@ -1309,7 +1316,8 @@ static void draw_bag(Ctx* c) {
   hipStream_t s = c->stream;
   c->need_re_bagging = false;
   hipLaunchKernelGGL(k_bag_draw, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, (long long)N,
-                     c->bag_lcg.p, c->opts.bagging_fraction, c->bag_flags.p, c->bag_counts.p);
+                     c->bag_lcg.p, c->opts.bagging_fraction, c->bag_balanced ? c->d_label.p : (const float*)nullptr,
+                     c->opts.pos_bagging_fraction, c->opts.neg_bagging_fraction, c->bag_flags.p, c->bag_counts.p);
   hipLaunchKernelGGL(k_bag_offsets, dim3(1), dim3(1024), 0, s, c->bag_counts.p, (long long)nb, c->bag_offsets.p,
                      c->bag_total.p);
   hipLaunchKernelGGL(k_bag_scatter, dim3((unsigned)nb), dim3(1024), 0, s, (long long)N, c->bag_flags.p,

@@ -1438,7 +1438,8 @@ __device__ __forceinline__ void lcg_jump(unsigned k, unsigned& A, unsigned& C) {
 // One wave per generator block: lane j draws rows 16 j .. 16 j + 15 of the
 // block from the generator state jumped ahead by 16 j steps, so the draws are
 // the sequential ones; lane 0 stores the state after the block's rows.
-__global__ void __launch_bounds__(256) k_bag_draw(long long n, unsigned* lcg, double fraction, uint8_t* flags,
+__global__ void __launch_bounds__(256) k_bag_draw(long long n, unsigned* lcg, double fraction, const float* label,
+                                                  double pos_fraction, double neg_fraction, uint8_t* flags,
                                                   int* counts) {
   const int lane = threadIdx.x & 63;
   const long long b = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -1456,7 +1457,9 @@ __global__ void __launch_bounds__(256) k_bag_draw(long long n, unsigned* lcg, do
     if (i >= rows) break;
     x = x * 214013u + 2531011u;
     const float f = (float)(int)((x >> 16) & 0x7FFFu) / 32768.0f;
-    const int in = (double)f < fraction ? 1 : 0;
+    // balanced bagging: the fraction of the row's class (label > 0 or not)
+    const double frac = label ? (label[r0 + i] > 0.0f ? pos_fraction : neg_fraction) : fraction;
+    const int in = (double)f < frac ? 1 : 0;
     flags[r0 + i] = (uint8_t)in;
     cnt += in;
   }

@@ -457,6 +457,7 @@ struct Params {
   int num_threads = 1;
   double bagging_fraction = 1.0;   // sample_strategy.rs (row bagging)
   int bagging_freq = 0, bagging_seed = 3;
+  double pos_bagging_fraction = 1.0, neg_bagging_fraction = 1.0;
   std::vector<int> categorical;
 };
 
@@ -510,6 +511,8 @@ static bool parse_params(const char* s, Params& p) {
       else if (k == "bagging_fraction") p.bagging_fraction = atof(v.c_str());
       else if (k == "bagging_freq") p.bagging_freq = atoi(v.c_str());
       else if (k == "bagging_seed") p.bagging_seed = atoi(v.c_str());
+      else if (k == "pos_bagging_fraction") p.pos_bagging_fraction = atof(v.c_str());
+      else if (k == "neg_bagging_fraction") p.neg_bagging_fraction = atof(v.c_str());
       else if (k == "categorical_feature") {
         size_t a = 0;
         while (a < v.size()) {
@@ -1902,20 +1905,30 @@ using namespace orc;
 // bagging_seed + block; a row is in the bag when its draw is below the
 // fraction.
 struct Bagger {
-  bool on = false, subset = false, need_re_bagging = false;
-  double fraction = 1.0;
+  bool on = false, subset = false, need_re_bagging = false, balanced = false;
+  double fraction = 1.0, pos_fraction = 1.0, neg_fraction = 1.0;
+  std::vector<float> label;
   int freq = 0;
   int64_t n = 0, bag_cnt = 0;
   std::vector<Random> rands;
   std::vector<uint32_t> in_bag, out_of_bag;
-  void setup(const Params& p, int64_t rows, int num_feature_groups) {
+  // num_pos: the objective's num_positive_data (binary: labels > 0; 0 otherwise)
+  void setup(const Params& p, int64_t rows, int num_feature_groups, int64_t num_pos, const float* lab) {
     n = rows;
     bag_cnt = rows;
-    if (!(p.bagging_fraction < 1.0 && p.bagging_freq > 0)) return;
+    balanced = (p.pos_bagging_fraction < 1.0 || p.neg_bagging_fraction < 1.0) && num_pos > 0;
+    if (!((p.bagging_fraction < 1.0 || balanced) && p.bagging_freq > 0)) return;
     on = true;
     fraction = p.bagging_fraction;
+    pos_fraction = p.pos_bagging_fraction;
+    neg_fraction = p.neg_bagging_fraction;
     freq = p.bagging_freq;
-    bag_cnt = (int64_t)(fraction * (double)n);
+    if (balanced) {
+      bag_cnt = (int64_t)((double)num_pos * pos_fraction) + (int64_t)((double)(n - num_pos) * neg_fraction);
+      label.assign(lab, lab + n);
+    } else {
+      bag_cnt = (int64_t)(fraction * (double)n);
+    }
     for (int64_t b = 0; b < (n + 1023) / 1024; ++b) rands.emplace_back((int)((uint32_t)p.bagging_seed + (uint32_t)b));
     const double rate = ((double)bag_cnt / (double)n) / (double)freq;
     subset = rate <= 0.5 && num_feature_groups < 100;
@@ -1928,7 +1941,8 @@ struct Bagger {
     out_of_bag.clear();
     for (int64_t i = 0; i < n; ++i) {
       const double r = (double)rands[i / 1024].next_float();
-      if (r < fraction) in_bag.push_back((uint32_t)i);
+      const double frac = balanced ? (label[i] > 0.0f ? pos_fraction : neg_fraction) : fraction;
+      if (r < frac) in_bag.push_back((uint32_t)i);
       else out_of_bag.push_back((uint32_t)i);
     }
     bag_cnt = (int64_t)in_bag.size();
@@ -2056,7 +2070,10 @@ void* orc_booster_create(void* d, const char* params) {
   b->scores.assign((size_t)b->K * b->ds->n, 0.0);
   b->grad.assign((size_t)b->K * b->ds->n, 0.0f);
   b->hess.assign((size_t)b->K * b->ds->n, 0.0f);
-  b->bagger.setup(p, b->ds->n, (int)b->ds->used.size());
+  int64_t num_pos = 0;
+  if (p.objective == 0)
+    for (float l : b->ds->label) num_pos += l > 0.0f ? 1 : 0;
+  b->bagger.setup(p, b->ds->n, (int)b->ds->used.size(), num_pos, b->ds->label.data());
   if (b->bagger.on && p.use_quantized_grad && !b->bagger.subset) {
     // the full-data bag of the quantized learner is not restated
     delete b->learner;

@@ -43,6 +43,24 @@ def test_bagging_matches_oracle(quant, fraction, freq):
     np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
 
 
+@pytest.mark.parametrize("quant", [False, True])
+def test_balanced_bagging_matches_oracle(quant):
+    X, y = datagen.binary_100k(50000)
+    params = dict(P, pos_bagging_fraction=0.3, neg_bagging_fraction=0.6, bagging_freq=1, bagging_seed=5)
+    if quant:
+        params.update(use_quantized_grad=True, num_grad_quant_bins=16, quant_train_renew_leaf=True)
+    b = lgb.Booster(params, lgb.Dataset(X, label=y, params=params))
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    for _ in range(4):
+        b.update()
+        ob.update()
+    bag = ob.bag()
+    pos_rate = (y[bag] > 0).sum() / max((y > 0).sum(), 1)
+    assert abs(pos_rate - 0.3) < 0.03
+    _compare(b, ob, 4)
+    np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
+
+
 def test_multiclass_bagging_matches_oracle():
     rng = np.random.default_rng(12)
     n, K = 40000, 3
