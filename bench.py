@@ -369,32 +369,41 @@ def workload_binning(lgb, datagen, args):
                                frac=byts / dt / 1e9 / HBM_PEAK_GBS, traffic=None, kernel="k_bin_rows"))
 
 
-def workload_bagging(lgb, datagen, args):
+def workload_bagging(lgb, datagen, args, goss=False):
     """The headline binary workload with row bagging (bagging_fraction=0.5,
     bagging_freq=1: a new bag every iteration, trees grown on its rows,
-    out-of-bag rows scored by the tree walk)."""
+    out-of-bag rows scored by the tree walk), or with GOSS (top_rate 0.2,
+    other_rate 0.1, 256 sampling blocks; warm-up iterations are part of the
+    untimed warmup)."""
     t = time.perf_counter()
     X, y = datagen.binary_10p5m(args.rows)
     Xv, yv = datagen.holdout("binary_10p5m", 200_000)
     log(f"data {X.shape} in {time.perf_counter() - t:.1f}s")
     base = dict(objective="binary", num_leaves=args.num_leaves, max_bin=args.max_bin, min_data_in_leaf=20,
                 learning_rate=0.1, bagging_fraction=0.5, bagging_freq=1, bagging_seed=3)
+    if goss:
+        base.update(bagging_fraction=1.0, bagging_freq=0, data_sample_strategy="goss", top_rate=0.2, other_rate=0.1,
+                    num_threads=256)
+        args.warmup = max(args.warmup, 10)
     ds = lgb.Dataset(X, label=y, params=base)
     del X
     forms = _train_forms(lgb, ds, base, args, holdout_auc((Xv, yv)))
     main_form = "quant" if "quant" in forms else list(forms)[0]
     m = forms[main_form]
-    return _line("boosting iters/sec with row bagging (fraction 0.5, a new bag each iteration)", m["iters_per_sec"],
+    what = "GOSS (top_rate 0.2, other_rate 0.1)" if goss else "row bagging (fraction 0.5, a new bag each iteration)"
+    cfg = "GOSS top_rate=0.2 other_rate=0.1" if goss else "bagging_fraction=0.5, bagging_freq=1"
+    return _line(f"boosting iters/sec with {what}", m["iters_per_sec"],
                  "iters/sec", args, m["ms_per_iter"], "int64" if main_form == "quant" else "f64",
                  {"workload": f"binary logloss GBDT, {args.rows} x 28, num_leaves={args.num_leaves}, "
-                              f"max_bin={args.max_bin}, bagging_fraction=0.5, bagging_freq=1",
+                              f"max_bin={args.max_bin}, {cfg}",
                   "rows": args.rows, "features": 28, "form": main_form, "parallelism": "1 GPU"},
                  forms=_forms_summary(forms, "holdout_auc"))
 
 
 WORKLOADS = {"binary_10p5m": None, "multiclass_2m": workload_multiclass, "rank_2m": workload_rank,
              "predict_1m": workload_predict, "eval_10p5m": workload_eval, "binning_10p5m": workload_binning,
-             "bagging_10p5m": workload_bagging}
+             "bagging_10p5m": workload_bagging,
+             "goss_10p5m": lambda lgb, datagen, args: workload_bagging(lgb, datagen, args, goss=True)}
 
 
 def main():

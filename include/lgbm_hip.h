@@ -110,6 +110,10 @@ typedef struct {
   int32_t bagging_seed;
   double pos_bagging_fraction;  /* balanced bagging (binary objective): fractions of rows with label > 0 / <= 0 */
   double neg_bagging_fraction;
+  int32_t data_sample_strategy; /* 0 bagging, 1 goss */
+  int32_t sample_threads;       /* GOSS: the reference's num_threads, which sets its row blocks (>= 1) */
+  double top_rate;              /* GOSS: share of rows kept by |g h| and share sampled from the rest */
+  double other_rate;
 } lgbmhip_train_opts;
 
 /* One metric of lgbmhip_eval (metric.rs:17-43 MetricKind, the Config fields it reads). */

@@ -33,6 +33,7 @@ DEFAULT_PARAMS = dict(
     metric="", eval_at=(1, 2, 3, 4, 5), multi_error_top_k=1, label_gain=None, early_stopping_round=0,
     first_metric_only=False, early_stopping_min_delta=0.0, device_binning=False, bagging_fraction=1.0,
     bagging_freq=0, bagging_seed=3, pos_bagging_fraction=1.0, neg_bagging_fraction=1.0,
+    data_sample_strategy="bagging", top_rate=0.2, other_rate=0.1,
 )
 
 ALIASES = {
@@ -260,7 +261,10 @@ class Booster:
                              int(p["lambdarank_truncation_level"]), int(_b(p["lambdarank_norm"])),
                              int(_b(p["use_graph"])), float(p["bagging_fraction"]), int(p["bagging_freq"]),
                              int(p["bagging_seed"]), float(p["pos_bagging_fraction"]),
-                             float(p["neg_bagging_fraction"]))
+                             float(p["neg_bagging_fraction"]),
+                             1 if str(p["data_sample_strategy"]) == "goss" or str(p.get("boosting")) == "goss" else 0,
+                             int(p["num_threads"]) if int(p["num_threads"]) > 0 else 256,
+                             float(p["top_rate"]), float(p["other_rate"]))
         check(lib().lgbmhip_set_train_opts(h, C.byref(to)), h)
         self._finished = False
         self._metrics = _metrics(p)

@@ -304,6 +304,11 @@ struct Ctx {
   // the root's rows of every tree until the next draw, and buffer 0's bins
   // for it (bag_grp0, group-major, positions 0 .. bag_m - 1)
   bool bag_on = false, bag_subset = false, need_re_bagging = false, bag_balanced = false;
+  bool bag_active = false;   // the current trees grow on the bag
+  bool goss_on = false;
+  DevBuf<GossBlock> goss_blocks;
+  DevBuf<float> goss_abs, goss_thr;
+  int goss_nblock = 0;
   int64_t bag_cnt = 0, bag_m = 0;
   DevBuf<unsigned> bag_rows, bag_lcg;
   DevBuf<uint8_t> bag_flags, bag_grp0;
@@ -772,7 +777,57 @@ static void setup_training(Ctx* c) {
       for (float l : c->label) num_pos += l > 0.0f ? 1 : 0;
     c->bag_balanced = (o.pos_bagging_fraction < 1.0 || o.neg_bagging_fraction < 1.0) && num_pos > 0;
     c->bag_on = (o.bagging_fraction < 1.0 || c->bag_balanced) && o.bagging_freq > 0;
+    c->goss_on = o.data_sample_strategy == 1;
+    c->bag_active = false;
     c->bag_cnt = N;
+    if (c->goss_on) {
+      // GOSSStrategy::ResetSampleConfig (sample_strategy.rs:140-160)
+      if (o.top_rate + o.other_rate > 1.0)
+        throw Error(LGBMHIP_INVALID_PARAMETER, "Check failed: (config_->top_rate + config_->other_rate) <= (1.0f)");
+      if (!(o.top_rate > 0.0 && o.other_rate > 0.0))
+        throw Error(LGBMHIP_INVALID_PARAMETER, "Check failed: config_->top_rate > 0.0f && config_->other_rate > 0.0f");
+      if (o.bagging_freq > 0 && o.bagging_fraction != 1.0) throw Error(LGBMHIP_INVALID_PARAMETER, "Cannot use bagging in GOSS");
+      if (o.objective == 2) throw Error(LGBMHIP_UNSUPPORTED, "GOSS with the lambdarank objective");
+      c->bag_subset = o.top_rate + o.other_rate <= 0.5;
+      if (quant && !c->bag_subset)
+        throw Error(LGBMHIP_UNSUPPORTED, "quantized gradients with GOSS above top_rate + other_rate = 0.5 (a full-data bag)");
+      c->bag_on = false;
+      // threading.rs block_info_force_size(num_threads, n, 1024)
+      const int64_t T = std::max(1, o.sample_threads);
+      int64_t nblock = std::min<int64_t>(T, (N + 1023) / 1024), block = N;
+      if (nblock > 1) {
+        block = (N + nblock - 1) / nblock;
+        block = (block + 1023) / 1024 * 1024;
+      }
+      std::vector<GossBlock> gbs;
+      for (int64_t b = 0; b < nblock; ++b) {
+        const int64_t start = b * block;
+        if (start >= N) break;
+        GossBlock gb{};
+        gb.start = start;
+        gb.cnt = std::min(block, N - start);
+        gb.top_k = std::max((int)((double)gb.cnt * o.top_rate), 1);
+        gb.other_k = (int)((double)gb.cnt * o.other_rate);
+        gb.multiply = (float)(int)(gb.cnt - gb.top_k) / (float)gb.other_k;
+        gbs.push_back(gb);
+      }
+      c->goss_nblock = (int)gbs.size();
+      c->goss_blocks.alloc(gbs.size());
+      HIP_CHECK(hipMemcpy(c->goss_blocks.p, gbs.data(), gbs.size() * sizeof(GossBlock), hipMemcpyHostToDevice));
+      c->goss_abs.alloc(N);
+      c->goss_thr.alloc(gbs.size());
+      const int64_t nb = (N + 1023) / 1024;
+      c->bag_rows.alloc(N);
+      c->bag_flags.alloc(N);
+      c->bag_counts.alloc(nb);
+      c->bag_offsets.alloc(nb);
+      c->bag_total.alloc(1);
+      c->bag_grp0.alloc(gbytes);
+      std::vector<unsigned> lcg(nb);
+      for (int64_t b = 0; b < nb; ++b) lcg[b] = (unsigned)o.bagging_seed + (unsigned)b;
+      c->bag_lcg.alloc(nb);
+      HIP_CHECK(hipMemcpy(c->bag_lcg.p, lcg.data(), nb * 4, hipMemcpyHostToDevice));
+    }
     if (c->bag_on) {
       if (o.objective == 2) throw Error(LGBMHIP_UNSUPPORTED, "bagging with the lambdarank objective");
       c->bag_cnt = c->bag_balanced ? (int64_t)((double)num_pos * o.pos_bagging_fraction) +
@@ -829,7 +884,7 @@ static void setup_training(Ctx* c) {
 static HistArgs hist_args(Ctx* c, Lane& ln, bool quant, int k, int mode) {
   HistArgs a{};
   for (int b = 0; b < 3; ++b) {
-    a.grp[b] = b == 0 ? (c->bag_on ? c->bag_grp0.p : c->grp0.p) : ln.grp[b].p;
+    a.grp[b] = b == 0 ? (c->bag_active ? c->bag_grp0.p : c->grp0.p) : ln.grp[b].p;
     a.gh[b] = ln.gh[b].p;
   }
   a.st = ln.st.p;
@@ -853,7 +908,7 @@ static HistArgs hist_args(Ctx* c, Lane& ln, bool quant, int k, int mode) {
   a.p = c->sp;
   a.quant = quant ? 1 : 0;
   a.mode = mode;
-  a.root_n = c->bag_on ? c->bag_m : c->N;
+  a.root_n = c->bag_active ? c->bag_m : c->N;
   return a;
 }
 
@@ -892,7 +947,7 @@ static PartArgs part_args(Ctx* c, Lane& ln) {
   a.best = ln.best.p;
   a.feat_split = ln.feat_split.p;
   for (int b = 0; b < 3; ++b) {
-    a.grp[b] = b == 0 ? (c->bag_on ? c->bag_grp0.p : c->grp0.p) : ln.grp[b].p;
+    a.grp[b] = b == 0 ? (c->bag_active ? c->bag_grp0.p : c->grp0.p) : ln.grp[b].p;
     a.gh[b] = ln.gh[b].p;
   }
   a.FB = c->hist_fb;
@@ -936,8 +991,8 @@ static void enqueue_head(Ctx* c, Lane& ln, int k) {
   const float* gk = c->g.p + (size_t)k * N;
   const float* hk = c->h.p + (size_t)k * N;
   // the root: every row, or the bag's rows (positions 0 .. bag_m - 1)
-  const int64_t R = c->bag_on ? c->bag_m : N;
-  const unsigned* bag = c->bag_on ? c->bag_rows.p : nullptr;
+  const int64_t R = c->bag_active ? c->bag_m : N;
+  const unsigned* bag = c->bag_active ? c->bag_rows.p : nullptr;
   const int ig = grid_for(R, 256, c->cus * 4);
   if (bag) {
     // max |g|, |h| over the bag (quant.rs:100-108 with rows)
@@ -1021,13 +1076,13 @@ static void enqueue_tail(Ctx* c, Lane& ln, int k, bool score_update, double lr) 
     WalkArgs w = walk_args(c, ln, 0);
     w.g = c->g.p + (size_t)k * N;
     w.h = c->h.p + (size_t)k * N;
-    if (c->bag_on) {
+    if (c->bag_active) {
       w.rows = c->bag_rows.p;
       w.N = c->bag_m;
     } else {
       w.leaf16 = ln.leaf16.p;   // every row's leaf, for the score update below
     }
-    launch_walk(w, c->bag_on ? grid_for(c->bag_m, 256, c->cus * 4) : wgrid, s);
+    launch_walk(w, c->bag_active ? grid_for(c->bag_m, 256, c->cus * 4) : wgrid, s);
   }
   hipLaunchKernelGGL(k_leaf_values, dim3((c->L + 255) / 256), dim3(256), 0, s, ln.st.p, ln.pos.p, ln.leaves.p,
                      c->it.p + k, ln.renew.p, renew ? 1 : 0, c->sp, lr, c->L, c->leafval.p + (size_t)k * c->L,
@@ -1036,7 +1091,7 @@ static void enqueue_tail(Ctx* c, Lane& ln, int k, bool score_update, double lr) 
     WalkArgs w = walk_args(c, ln, 1);
     w.val = c->leafval.p + (size_t)k * c->L;
     w.score = c->score.p + (size_t)k * N;
-    if (renew && !c->bag_on)
+    if (renew && !c->bag_active)
       hipLaunchKernelGGL(k_score_leaf, dim3(grid_for(N, 1024, c->cus * 4)), dim3(256), 0, s, ln.st.p, ln.leaf16.p,
                          w.val, w.score, (long long)N);
     else
@@ -1068,7 +1123,7 @@ static hipGraphExec_t capture(hipStream_t s, const std::function<void()>& body) 
 static void launch_main(Ctx* c, int k, bool score_update, double lr) {
   Lane& ln = lane_of(c, k);
   // a bag changes the root's size and rows: stream launches
-  const bool graph = c->opts.use_graph && !c->profile && !c->bag_on;
+  const bool graph = c->opts.use_graph && !c->profile && !c->bag_on && !c->goss_on;
   if (graph) {
     if (!c->graphs[k])
       c->graphs[k] = capture(ln.stream, [&] {
@@ -1089,7 +1144,7 @@ static void launch_main(Ctx* c, int k, bool score_update, double lr) {
 // rounds_cont more rounds of class k's tree and its tail again.
 static void launch_cont(Ctx* c, int k, int launched, bool score_update, double lr) {
   Lane& ln = lane_of(c, k);
-  const bool graph = c->opts.use_graph && !c->profile && !c->bag_on;
+  const bool graph = c->opts.use_graph && !c->profile && !c->bag_on && !c->goss_on;
   if (graph) {
     if (!c->graphs_cont[k])
       c->graphs_cont[k] = capture(ln.stream, [&] {
@@ -1307,6 +1362,8 @@ static void account_profile(Ctx* c, int k) {
   c->prof[10] += (double)st.num_splits;
 }
 
+static void compact_bag(Ctx* c);
+
 // SampleStrategy::bagging (sample_strategy.rs:278-318) on the device: every
 // 1024-row block's generator draws its rows (the generators run on from draw
 // to draw), the in-bag rows are compacted in ascending order, and buffer 0's
@@ -1318,6 +1375,14 @@ static void draw_bag(Ctx* c) {
   hipLaunchKernelGGL(k_bag_draw, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, (long long)N,
                      c->bag_lcg.p, c->opts.bagging_fraction, c->bag_balanced ? c->d_label.p : (const float*)nullptr,
                      c->opts.pos_bagging_fraction, c->opts.neg_bagging_fraction, c->bag_flags.p, c->bag_counts.p);
+  compact_bag(c);
+}
+
+// The in-bag rows (flags set) ascending into bag_rows, their count to the
+// host, and the bag's buffer 0 bins.
+static void compact_bag(Ctx* c) {
+  const int64_t N = c->N, nb = (N + 1023) / 1024;
+  hipStream_t s = c->stream;
   hipLaunchKernelGGL(k_bag_offsets, dim3(1), dim3(1024), 0, s, c->bag_counts.p, (long long)nb, c->bag_offsets.p,
                      c->bag_total.p);
   hipLaunchKernelGGL(k_bag_scatter, dim3((unsigned)nb), dim3(1024), 0, s, (long long)N, c->bag_flags.p,
@@ -1329,7 +1394,28 @@ static void draw_bag(Ctx* c) {
   if (m <= 0) throw Error(LGBMHIP_UNSUPPORTED, "an empty bag");
   c->bag_m = m;
   c->bag_cnt = m;
+  c->bag_active = true;
   launch_group_rows(c, c->bag_rows.p, m, c->bag_grp0.p, s);
+}
+
+// GOSSStrategy::Bagging (sample_strategy.rs:319-350, 395-442) after the
+// gradients: from iteration 1 / learning_rate on, every iteration's bag.
+static void goss_bag(Ctx* c) {
+  const int64_t N = c->N, nb = (N + 1023) / 1024;
+  hipStream_t s = c->stream;
+  c->bag_active = false;
+  c->bag_cnt = N;
+  if ((int64_t)c->iter < (int64_t)((double)1.0f / c->opts.learning_rate)) return;
+  hipLaunchKernelGGL(k_goss_abs, dim3(grid_for(N, 256, c->cus * 4)), dim3(256), 0, s, c->g.p, c->h.p, (long long)N,
+                     c->K, c->goss_abs.p);
+  hipLaunchKernelGGL(k_goss_threshold, dim3(c->goss_nblock), dim3(1024), 0, s, c->goss_abs.p, c->goss_blocks.p,
+                     c->goss_thr.p);
+  hipLaunchKernelGGL(k_goss_chain, dim3(c->goss_nblock), dim3(64), 0, s, c->goss_abs.p, c->goss_blocks.p,
+                     c->goss_thr.p, c->bag_lcg.p, c->g.p, c->h.p, (long long)N, c->K, c->bag_flags.p);
+  hipLaunchKernelGGL(k_bag_counts, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, c->bag_flags.p, (long long)N,
+                     c->bag_counts.p);
+  HIP_CHECK(hipGetLastError());
+  compact_bag(c);
 }
 
 // boosting.rs:957-1139 for device=gpu.
@@ -1361,6 +1447,7 @@ static int train_iteration(Ctx* c) {
   if (c->bag_on && ((c->bag_cnt < N && c->iter % c->opts.bagging_freq == 0) || c->need_re_bagging)) draw_bag(c);
   HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   enqueue_gradients(c);
+  if (c->goss_on) goss_bag(c);
   grow_trees(c, K, true, c->opts.learning_rate);
   if (c->opts.use_quantized_grad) {
     hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, c->stream, c->call_ctr.p, K);
@@ -1415,7 +1502,7 @@ static int train_iteration(Ctx* c) {
 // One tree from caller gradients (SerialTreeLearner::train), no score update.
 static void train_tree(Ctx* c, const float* g, const float* h) {
   if (!c->state_ready) setup_training(c);
-  if (c->bag_on) throw Error(LGBMHIP_UNSUPPORTED, "one tree from caller gradients with bagging on");
+  if (c->bag_on || c->goss_on) throw Error(LGBMHIP_UNSUPPORTED, "one tree from caller gradients with bagging on");
   set_device(c);
   const int64_t N = c->N;
   HIP_CHECK(hipMemcpyAsync(c->g.p, g, N * 4, hipMemcpyHostToDevice, c->stream));

@@ -1516,6 +1516,158 @@ __global__ void __launch_bounds__(1024) k_bag_scatter(long long n, const uint8_t
   if (v) rows[offsets[blockIdx.x] + sm[t] - 1] = (unsigned)i;
 }
 
+// ---------------------------------------------------------------- GOSS
+// sample_strategy.rs:395-442 on the device. |g h| summed over the K classes
+// in f32, class by class.
+__global__ void __launch_bounds__(256) k_goss_abs(const float* g, const float* h, long long n, int K, float* out) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    float s = 0.0f;
+    for (int k = 0; k < K; ++k) s += fabsf(g[(size_t)k * n + i] * h[(size_t)k * n + i]);
+    out[i] = s;
+  }
+}
+
+// Per GOSS block (one 1024-thread workgroup each): the top_k-th largest
+// |g h| of rows start .. start + cnt (ArgMaxAtK, array_args.rs:65-79), by
+// a most-significant-digit radix select over the f32 bits (non-negative
+// floats order as their bits).
+struct GossBlock {
+  long long start, cnt;
+  int top_k, other_k;
+  float multiply;
+  int pad;
+};
+__global__ void __launch_bounds__(1024) k_goss_threshold(const float* v, const GossBlock* blocks, float* thr) {
+  __shared__ unsigned hist[256];
+  __shared__ unsigned s_prefix, s_k;
+  const GossBlock gb = blocks[blockIdx.x];
+  const int t = threadIdx.x;
+  if (t == 0) {
+    s_prefix = 0u;
+    s_k = (unsigned)gb.top_k;   // rank from the top still to find
+  }
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    if (t < 256) hist[t] = 0u;
+    __syncthreads();
+    const unsigned prefix = s_prefix;
+    const unsigned mask_hi = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+    for (long long i = t; i < gb.cnt; i += blockDim.x) {
+      const unsigned x = __float_as_uint(v[gb.start + i]);
+      if ((x & mask_hi) == prefix) atomicAdd(&hist[(x >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    if (t == 0) {
+      unsigned k = s_k;
+      int d = 255;
+      for (; d > 0; --d) {
+        if (hist[d] >= k) break;
+        k -= hist[d];
+      }
+      s_prefix = prefix | ((unsigned)d << shift);
+      s_k = k;
+    }
+    __syncthreads();
+  }
+  if (t == 0) thr[blockIdx.x] = __uint_as_float(s_prefix);
+}
+
+// One wave per GOSS block: rows in 1024-row chunks (the generator blocks;
+// GOSS blocks are multiples of 1024 rows). Lanes mark the rows at or above
+// the threshold and draw, for every other row, the next value of its chunk's
+// generator (jumped ahead by the chunk's earlier non-top rows); lane 0 then
+// runs the reference's sequential sampling (rest_need / rest_all, compared
+// exactly in integers: draw k / 32768 < a / b iff k b < 32768 a); sampled
+// rows' g, h of every class are scaled by the block's multiplier.
+__global__ void __launch_bounds__(64) k_goss_chain(const float* v, const GossBlock* blocks, const float* thr,
+                                                   unsigned* lcg, float* g, float* h, long long n, int K,
+                                                   uint8_t* flags) {
+  __shared__ unsigned short draw[kBagBlock];   // 15-bit draw, or 0xFFFF for a top row
+  __shared__ uint8_t take[kBagBlock];
+  const GossBlock gb = blocks[blockIdx.x];
+  const float th = thr[blockIdx.x];
+  const int lane = threadIdx.x;
+  constexpr int PER = kBagBlock / 64;
+  int big = 0, left = 0;   // lane 0's running counts over the block
+  for (long long c0 = 0; c0 < gb.cnt; c0 += kBagBlock) {
+    const long long base = gb.start + c0;
+    const int rows = (int)min((long long)kBagBlock, gb.cnt - c0);
+    const long long gen = base / kBagBlock;
+    bool top[PER];
+    int nb = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int i = lane * PER + j;
+      top[j] = i < rows && v[base + i] >= th;
+      nb += (i < rows && !top[j]) ? 1 : 0;
+    }
+    int incl = nb;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o);
+      if (lane >= o) incl += y;
+    }
+    const int before = incl - nb, total = __shfl(incl, 63);
+    const unsigned x0 = lcg[gen];
+    unsigned A, C;
+    lcg_jump((unsigned)before, A, C);
+    unsigned x = A * x0 + C;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int i = lane * PER + j;
+      if (i >= rows) continue;
+      if (top[j]) {
+        draw[i] = 0xFFFFu;
+      } else {
+        x = x * 214013u + 2531011u;
+        draw[i] = (unsigned short)((x >> 16) & 0x7FFFu);
+      }
+    }
+    __syncthreads();
+    if (lane == 0) {
+      lcg_jump((unsigned)total, A, C);
+      lcg[gen] = A * x0 + C;
+      for (int i = 0; i < rows; ++i) {
+        const unsigned d = draw[i];
+        uint8_t in;
+        if (d == 0xFFFFu) {
+          in = 1;
+          ++left;
+          ++big;
+        } else {
+          const long long rest_need = (long long)gb.other_k - (left - big);
+          const long long rest_all = (gb.cnt - (c0 + i)) - ((long long)gb.top_k - big);
+          in = (long long)d * rest_all < rest_need * 32768LL ? 2 : 0;
+          if (in) ++left;
+        }
+        take[i] = in;
+      }
+    }
+    __syncthreads();
+    for (int i = lane; i < rows; i += 64) {
+      const uint8_t in = take[i];
+      flags[base + i] = in ? 1 : 0;
+      if (in == 2)
+        for (int k = 0; k < K; ++k) {
+          g[(size_t)k * n + base + i] *= gb.multiply;
+          h[(size_t)k * n + base + i] *= gb.multiply;
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// Rows per generator block that the flags put in the bag (for k_bag_offsets).
+__global__ void __launch_bounds__(256) k_bag_counts(const uint8_t* flags, long long n, int* counts) {
+  const int lane = threadIdx.x & 63;
+  const long long b = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nb = (n + kBagBlock - 1) / kBagBlock;
+  if (b >= nb) return;
+  int cnt = 0;
+  for (long long i = b * kBagBlock + lane; i < min(n, (b + 1) * kBagBlock); i += 64) cnt += flags[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+  if (lane == 0) counts[b] = cnt;
+}
+
 // Clears class k's max |g|, |h| before they are taken over a bag.
 __global__ void k_clear_max(IterDev* it) {
   it->max_abs_g_bits = 0;

@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <set>
 #include <string>
@@ -458,6 +459,8 @@ struct Params {
   double bagging_fraction = 1.0;   // sample_strategy.rs (row bagging)
   int bagging_freq = 0, bagging_seed = 3;
   double pos_bagging_fraction = 1.0, neg_bagging_fraction = 1.0;
+  int goss = 0;                    // data_sample_strategy=goss
+  double top_rate = 0.2, other_rate = 0.1;
   std::vector<int> categorical;
 };
 
@@ -512,6 +515,9 @@ static bool parse_params(const char* s, Params& p) {
       else if (k == "bagging_freq") p.bagging_freq = atoi(v.c_str());
       else if (k == "bagging_seed") p.bagging_seed = atoi(v.c_str());
       else if (k == "pos_bagging_fraction") p.pos_bagging_fraction = atof(v.c_str());
+      else if (k == "data_sample_strategy") p.goss = v == "goss";
+      else if (k == "top_rate") p.top_rate = atof(v.c_str());
+      else if (k == "other_rate") p.other_rate = atof(v.c_str());
       else if (k == "neg_bagging_fraction") p.neg_bagging_fraction = atof(v.c_str());
       else if (k == "categorical_feature") {
         size_t a = 0;
@@ -1906,6 +1912,10 @@ using namespace orc;
 // fraction.
 struct Bagger {
   bool on = false, subset = false, need_re_bagging = false, balanced = false;
+  // GOSS (sample_strategy.rs:319-464): blocks of block_info_force_size(num_threads, n, 1024)
+  bool goss = false;
+  double top_rate = 0, other_rate = 0, learning_rate = 0.1;
+  int num_threads = 1, K = 1;
   double fraction = 1.0, pos_fraction = 1.0, neg_fraction = 1.0;
   std::vector<float> label;
   int freq = 0;
@@ -1916,6 +1926,17 @@ struct Bagger {
   void setup(const Params& p, int64_t rows, int num_feature_groups, int64_t num_pos, const float* lab) {
     n = rows;
     bag_cnt = rows;
+    if (p.goss) {
+      goss = on = true;
+      top_rate = p.top_rate;
+      other_rate = p.other_rate;
+      learning_rate = p.learning_rate;
+      num_threads = std::max(1, p.num_threads);
+      K = p.objective == 1 ? p.num_class : 1;
+      for (int64_t b = 0; b < (n + 1023) / 1024; ++b) rands.emplace_back((int)((uint32_t)p.bagging_seed + (uint32_t)b));
+      subset = top_rate + other_rate <= 0.5;
+      return;
+    }
     balanced = (p.pos_bagging_fraction < 1.0 || p.neg_bagging_fraction < 1.0) && num_pos > 0;
     if (!((p.bagging_fraction < 1.0 || balanced) && p.bagging_freq > 0)) return;
     on = true;
@@ -1933,6 +1954,66 @@ struct Bagger {
     const double rate = ((double)bag_cnt / (double)n) / (double)freq;
     subset = rate <= 0.5 && num_feature_groups < 100;
     need_re_bagging = true;
+  }
+  // GOSSStrategy::Helper over rows start .. start + cnt (sample_strategy.rs:395-442)
+  void goss_block(int64_t start, int64_t cnt, float* grad, float* hess) {
+    auto abs_gh = [&](int64_t row) {
+      float s = 0.0f;
+      for (int k = 0; k < K; ++k) s += std::fabs(grad[(size_t)k * n + row] * hess[(size_t)k * n + row]);
+      return s;
+    };
+    std::vector<float> tmp(cnt);
+    for (int64_t i = 0; i < cnt; ++i) tmp[i] = abs_gh(start + i);
+    const int top_k = std::max((int)((double)cnt * top_rate), 1);
+    const int other_k = (int)((double)cnt * other_rate);
+    // array_args.rs:65-79 ArgMaxAtK: position top_k - 1 of the descending order
+    std::nth_element(tmp.begin(), tmp.begin() + (top_k - 1), tmp.end(), std::greater<float>());
+    const float threshold = tmp[top_k - 1];
+    const float multiply = (float)(int)(cnt - top_k) / (float)other_k;
+    int big = 0, left = 0;
+    for (int64_t i = 0; i < cnt; ++i) {
+      const int64_t cur = start + i;
+      const float g = abs_gh(cur);
+      if (g >= threshold) {
+        in_bag.push_back((uint32_t)cur);
+        ++left;
+        ++big;
+      } else {
+        const int sampled = left - big;
+        const int rest_need = other_k - sampled;
+        const int rest_all = (int)(cnt - i) - (top_k - big);
+        const double prob = (double)rest_need / (double)rest_all;
+        if ((double)rands[cur / 1024].next_float() < prob) {
+          in_bag.push_back((uint32_t)cur);
+          ++left;
+          for (int k = 0; k < K; ++k) {
+            grad[(size_t)k * n + cur] *= multiply;
+            hess[(size_t)k * n + cur] *= multiply;
+          }
+        } else {
+          out_of_bag.push_back((uint32_t)cur);
+        }
+      }
+    }
+  }
+  bool bagging_goss(int iter, float* grad, float* hess) {
+    bag_cnt = n;
+    if ((int64_t)iter < (int64_t)((double)1.0f / learning_rate)) return false;
+    // threading.rs block_info_force_size(num_threads, n, 1024)
+    int64_t nblock = std::min<int64_t>(num_threads, (n + 1023) / 1024), block = n;
+    if (nblock > 1) {
+      block = (n + nblock - 1) / nblock;
+      block = (block + 1023) / 1024 * 1024;
+    }
+    in_bag.clear();
+    out_of_bag.clear();
+    for (int64_t b = 0; b < nblock; ++b) {
+      const int64_t start = b * block;
+      if (start >= n) break;
+      goss_block(start, std::min(block, n - start), grad, hess);
+    }
+    bag_cnt = (int64_t)in_bag.size();
+    return true;
   }
   bool bagging(int iter) {
     if (!on || !((bag_cnt < n && iter % freq == 0) || need_re_bagging)) return false;
@@ -1983,7 +2064,7 @@ static int booster_train_one_iter(OrcBooster* b) {
       }
     }
   }
-  if (b->bagger.bagging(b->iter)) {
+  if (!b->bagger.goss && b->bagger.bagging(b->iter)) {
     b->learner->bag = b->bagger.in_bag;
     b->learner->use_bag = true;
     b->learner->bag_subset = b->bagger.subset;
@@ -1991,6 +2072,12 @@ static int booster_train_one_iter(OrcBooster* b) {
   if (b->prm.objective == 0) binary_gradients(b->scores.data(), b->ds->label.data(), n, b->prm.sigmoid, b->grad.data(), b->hess.data());
   else if (b->prm.objective == 1) multiclass_gradients(b->scores.data(), b->ds->label.data(), n, K, b->grad.data(), b->hess.data());
   else b->rank->gradients(b->scores.data(), b->ds->label.data(), b->ds->query_boundaries, b->grad.data(), b->hess.data());
+  // boosting.rs:1029-1035: the sampler after the gradients (GOSS rescales them)
+  if (b->bagger.goss && b->bagger.bagging_goss(b->iter, b->grad.data(), b->hess.data())) {
+    b->learner->bag = b->bagger.in_bag;
+    b->learner->use_bag = true;
+    b->learner->bag_subset = b->bagger.subset;
+  }
   bool cont = false;
   for (int k = 0; k < K; ++k) {
     Tree t = b->learner->train(b->grad.data() + (size_t)k * n, b->hess.data() + (size_t)k * n);

@@ -83,3 +83,28 @@ def test_quantized_full_data_bag_is_unsupported():
     b = lgb.Booster(params, lgb.Dataset(X, label=y, params=params))
     with pytest.raises(lgb.LightGBMError, match="Unsupported"):
         b.update()
+
+
+@pytest.mark.parametrize("quant,threads,K", [(False, 4, 1), (True, 1, 1), (False, 3, 3)])
+def test_goss_matches_oracle(quant, threads, K):
+    rng = np.random.default_rng(14)
+    X, y = datagen.binary_100k(40000)
+    params = dict(P, learning_rate=0.5, data_sample_strategy="goss", top_rate=0.2, other_rate=0.1,
+                  num_threads=threads, bagging_seed=9)
+    if K > 1:
+        y = np.minimum(np.floor(np.abs(X[:, 0]) * 2), K - 1).astype(np.float32)
+        params.update(objective="multiclass", num_class=K)
+    if quant:
+        params.update(use_quantized_grad=True, num_grad_quant_bins=16, quant_train_renew_leaf=True)
+    print(f"goss quant={quant} threads={threads} K={K}", flush=True)
+    b = lgb.Booster(params, lgb.Dataset(X, label=y, params=params))
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    iters = 5   # GOSS starts at iteration 1 / learning_rate = 2
+    for _ in range(iters):
+        b.update()
+        ob.update()
+    bag = ob.bag()
+    # top 20% by |g h| (more where |g h| ties at the threshold) plus 10% sampled from the rest
+    assert bag is not None and 0.25 * len(y) < len(bag) < 0.5 * len(y)
+    _compare(b, ob, iters * K)
+    np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
