@@ -1581,8 +1581,8 @@ __global__ void __launch_bounds__(1024) k_goss_threshold(const float* v, const G
 __global__ void __launch_bounds__(64) k_goss_chain(const float* v, const GossBlock* blocks, const float* thr,
                                                    unsigned* lcg, float* g, float* h, long long n, int K,
                                                    uint8_t* flags) {
-  __shared__ unsigned short draw[kBagBlock];   // 15-bit draw, or 0xFFFF for a top row
-  __shared__ uint8_t take[kBagBlock];
+  __shared__ __align__(16) unsigned short draw[kBagBlock];   // 15-bit draw, 0xFFFF a top row, 0xFFFE no row
+  __shared__ __align__(16) uint8_t take[kBagBlock];
   const GossBlock gb = blocks[blockIdx.x];
   const float th = thr[blockIdx.x];
   const int lane = threadIdx.x;
@@ -1613,8 +1613,9 @@ __global__ void __launch_bounds__(64) k_goss_chain(const float* v, const GossBlo
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       const int i = lane * PER + j;
-      if (i >= rows) continue;
-      if (top[j]) {
+      if (i >= rows) {
+        draw[i] = 0xFFFEu;   // past the block's rows
+      } else if (top[j]) {
         draw[i] = 0xFFFFu;
       } else {
         x = x * 214013u + 2531011u;
@@ -1625,20 +1626,29 @@ __global__ void __launch_bounds__(64) k_goss_chain(const float* v, const GossBlo
     if (lane == 0) {
       lcg_jump((unsigned)total, A, C);
       lcg[gen] = A * x0 + C;
-      for (int i = 0; i < rows; ++i) {
-        const unsigned d = draw[i];
-        uint8_t in;
-        if (d == 0xFFFFu) {
-          in = 1;
-          ++left;
-          ++big;
-        } else {
-          const long long rest_need = (long long)gb.other_k - (left - big);
-          const long long rest_all = (gb.cnt - (c0 + i)) - ((long long)gb.top_k - big);
-          in = (long long)d * rest_all < rest_need * 32768LL ? 2 : 0;
-          if (in) ++left;
+      // 8 draws per LDS read; rest_all counts down with the rows
+      long long rest_base = gb.cnt - c0 - (long long)gb.top_k;
+      for (int i0 = 0; i0 < rows; i0 += 8) {
+        const uint4 q = *(const uint4*)&draw[i0];
+        const unsigned w4[4] = {q.x, q.y, q.z, q.w};
+        unsigned long long tk = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const unsigned d = (w4[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+          unsigned long long in = 0;
+          if (d == 0xFFFFu) {
+            in = 1;
+            ++left;
+            ++big;
+          } else if (d != 0xFFFEu) {
+            const long long rest_need = (long long)gb.other_k - (left - big);
+            const long long rest_all = rest_base - (i0 + j) + big;
+            in = (long long)d * rest_all < rest_need * 32768LL ? 2 : 0;
+            left += in ? 1 : 0;
+          }
+          tk |= in << (8 * j);
         }
-        take[i] = in;
+        *(unsigned long long*)&take[i0] = tk;
       }
     }
     __syncthreads();
