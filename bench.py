@@ -373,7 +373,7 @@ def workload_bagging(lgb, datagen, args, goss=False):
     """The headline binary workload with row bagging (bagging_fraction=0.5,
     bagging_freq=1: a new bag every iteration, trees grown on its rows,
     out-of-bag rows scored by the tree walk), or with GOSS (top_rate 0.2,
-    other_rate 0.1, 1024 sampling blocks; warm-up iterations are part of the
+    other_rate 0.1, 4096 sampling blocks; warm-up iterations are part of the
     untimed warmup)."""
     t = time.perf_counter()
     X, y = datagen.binary_10p5m(args.rows)
@@ -383,7 +383,7 @@ def workload_bagging(lgb, datagen, args, goss=False):
                 learning_rate=0.1, bagging_fraction=0.5, bagging_freq=1, bagging_seed=3)
     if goss:
         base.update(bagging_fraction=1.0, bagging_freq=0, data_sample_strategy="goss", top_rate=0.2, other_rate=0.1,
-                    num_threads=1024)
+                    num_threads=4096)
         args.warmup = max(args.warmup, 10)
     ds = lgb.Dataset(X, label=y, params=base)
     del X

@@ -263,7 +263,7 @@ class Booster:
                              int(p["bagging_seed"]), float(p["pos_bagging_fraction"]),
                              float(p["neg_bagging_fraction"]),
                              1 if str(p["data_sample_strategy"]) == "goss" or str(p.get("boosting")) == "goss" else 0,
-                             int(p["num_threads"]) if int(p["num_threads"]) > 0 else 1024,
+                             int(p["num_threads"]) if int(p["num_threads"]) > 0 else 4096,
                              float(p["top_rate"]), float(p["other_rate"]))
         check(lib().lgbmhip_set_train_opts(h, C.byref(to)), h)
         self._finished = False

@@ -248,6 +248,7 @@ struct Ctx {
   std::vector<float> label;
   DevBuf<float> d_label, d_weight;
   bool has_weight = false;
+  double sum_weights = 0.0;   // Metric::new's weight total (metric.rs:238-247)
   std::vector<int32_t> qb;
   // parameters
   SplitParamsDev sp{};
@@ -1757,14 +1758,7 @@ static int eval_metric(Ctx* c, int data, const lgbmhip_metric_opts* o, double* o
     weight = c->has_weight ? c->d_weight.p : nullptr;
     hlabel = &c->label;
     hqb = &c->qb;
-    sum_weights = (double)n;
-    if (c->has_weight) {
-      std::vector<float> w(n);
-      HIP_CHECK(hipMemcpy(w.data(), c->d_weight.p, n * 4, hipMemcpyDeviceToHost));
-      double sw = 0.0;
-      for (int64_t i = 0; i < n; ++i) sw += (double)w[i];
-      sum_weights = sw;
-    }
+    sum_weights = c->sum_weights;
     if (!c->qb.empty()) {
       if (c->d_qb_eval.n != c->qb.size()) {
         c->d_qb_eval.alloc(c->qb.size());
@@ -2202,9 +2196,13 @@ lgbmhip_status lgbmhip_set_metadata(lgbmhip_ctx* ctx, const float* label, const 
     c->d_label.alloc(c->N);
     HIP_CHECK(hipMemcpy(c->d_label.p, label, c->N * 4, hipMemcpyHostToDevice));
     c->has_weight = weight != nullptr;
+    c->sum_weights = (double)c->N;
     if (weight) {
       c->d_weight.alloc(c->N);
       HIP_CHECK(hipMemcpy(c->d_weight.p, weight, c->N * 4, hipMemcpyHostToDevice));
+      double sw = 0.0;
+      for (int64_t i = 0; i < c->N; ++i) sw += (double)weight[i];
+      c->sum_weights = sw;
     }
     if (query_boundaries) c->qb.assign(query_boundaries, query_boundaries + num_queries + 1);
     c->state_ready = false;
