@@ -788,7 +788,6 @@ static void setup_training(Ctx* c) {
       if (!(o.top_rate > 0.0 && o.other_rate > 0.0))
         throw Error(LGBMHIP_INVALID_PARAMETER, "Check failed: config_->top_rate > 0.0f && config_->other_rate > 0.0f");
       if (o.bagging_freq > 0 && o.bagging_fraction != 1.0) throw Error(LGBMHIP_INVALID_PARAMETER, "Cannot use bagging in GOSS");
-      if (o.objective == 2) throw Error(LGBMHIP_UNSUPPORTED, "GOSS with the lambdarank objective");
       c->bag_subset = o.top_rate + o.other_rate <= 0.5;
       if (quant && !c->bag_subset)
         throw Error(LGBMHIP_UNSUPPORTED, "quantized gradients with GOSS above top_rate + other_rate = 0.5 (a full-data bag)");
@@ -830,7 +829,6 @@ static void setup_training(Ctx* c) {
       HIP_CHECK(hipMemcpy(c->bag_lcg.p, lcg.data(), nb * 4, hipMemcpyHostToDevice));
     }
     if (c->bag_on) {
-      if (o.objective == 2) throw Error(LGBMHIP_UNSUPPORTED, "bagging with the lambdarank objective");
       c->bag_cnt = c->bag_balanced ? (int64_t)((double)num_pos * o.pos_bagging_fraction) +
                                          (int64_t)((double)(N - num_pos) * o.neg_bagging_fraction)
                                    : (int64_t)(o.bagging_fraction * (double)N);

@@ -108,3 +108,25 @@ def test_goss_matches_oracle(quant, threads, K):
     assert bag is not None and 0.25 * len(y) < len(bag) < 0.5 * len(y)
     _compare(b, ob, iters * K)
     np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("goss", [False, True])
+def test_lambdarank_row_sampling_matches_oracle(goss):
+    # row bagging / GOSS with the ranking objective: gradients of every query,
+    # trees on the sampled rows (bagging_by_query is not built)
+    X, rel, qb = datagen.rank_2m(400, 300)
+    params = dict(objective="lambdarank", num_leaves=31, max_bin=63, learning_rate=0.5)
+    if goss:
+        params.update(data_sample_strategy="goss", top_rate=0.3, other_rate=0.2, num_threads=2)
+    else:
+        params.update(bagging_fraction=0.5, bagging_freq=1, bagging_seed=2)
+    b = lgb.Booster(params, lgb.Dataset(X, label=rel, group=np.diff(qb), params=params))
+    ods = O.Dataset(X, rel, params)
+    ods.set_query_boundaries(qb)
+    ob = O.Booster(ods, params)
+    for _ in range(4):
+        b.update()
+        ob.update()
+    assert ob.bag() is not None
+    _compare(b, ob, 4)
+    np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
