@@ -34,6 +34,7 @@
  *   lgbmhip_add_valid         <- Gbdt::add_valid                boosting.rs:378-411
  *   lgbmhip_copy_valid_scores <- Gbdt::valid_scores             boosting.rs:1282-1284
  *   lgbmhip_eval              <- Metric::eval                   metric.rs:525-545, 551-729
+ *   lgbmhip_set_predict_window <- predict_window / prediction_early_stop predict.rs:178-202
  *   lgbmhip_save_model        <- GBDT::SaveModelToString        io.rs:13-88, tree.rs:718-770
  *   lgbmhip_load_model        <- GBDT::LoadModelFromString      io.rs:156-343
  *   lgbmhip_bins_feature_infos <- Dataset::feature_infos        dataset.rs:770-776
@@ -253,6 +254,15 @@ lgbmhip_status lgbmhip_copy_valid_scores(lgbmhip_ctx* ctx, int32_t index, double
  * a fixed order of their own (within 1e-10 relative). */
 lgbmhip_status lgbmhip_eval(lgbmhip_ctx* ctx, int32_t data, const lgbmhip_metric_opts* opts, double* out,
                             int32_t* out_count);
+
+/* The prediction window and early stopping of later predict_* calls on this
+ * context (Gbdt::predict_window / prediction_early_stop, predict.rs:178-202):
+ * iterations start_iteration .. start_iteration + num_iteration (<= 0: to
+ * the end); early_stop_freq > 0 stops a row's sum once, at a multiple of
+ * freq iterations, its margin (2|raw| or the top-two gap of the classes)
+ * exceeds early_stop_margin. Defaults: 0, 0, 0 (every tree, no stopping). */
+lgbmhip_status lgbmhip_set_predict_window(lgbmhip_ctx* ctx, int32_t start_iteration, int32_t num_iteration,
+                                          int32_t early_stop_freq, double early_stop_margin);
 
 /* Model text of the trained trees (GBDT::SaveModelToString, io.rs:13-88):
  * header, tree_sizes, every tree's block (tree.rs:718-770, numbers as C's

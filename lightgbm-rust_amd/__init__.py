@@ -436,9 +436,20 @@ class Booster:
         return out
 
     # --- prediction (basic.py:1641 -> py/lib.rs:557-579 -> predict.rs:231)
-    def predict(self, data, raw_score=False):
+    def _window(self, start_iteration, num_iteration, pred_early_stop, pred_early_stop_freq,
+                pred_early_stop_margin):
+        # basic.py predict: num_iteration None means best_iteration when set, else every iteration
+        if num_iteration is None:
+            num_iteration = self.best_iteration if getattr(self, "best_iteration", 0) > 0 else -1
+        freq = int(pred_early_stop_freq) if pred_early_stop else 0
+        check(lib().lgbmhip_set_predict_window(self._ctx, int(start_iteration), int(num_iteration), freq,
+                                               float(pred_early_stop_margin)), self._ctx)
+
+    def predict(self, data, raw_score=False, start_iteration=0, num_iteration=None, pred_early_stop=False,
+                pred_early_stop_freq=10, pred_early_stop_margin=10.0):
         X = np.ascontiguousarray(data, np.float32)
         out = np.zeros(X.shape[0] * self.K, np.float64)
+        self._window(start_iteration, num_iteration, pred_early_stop, pred_early_stop_freq, pred_early_stop_margin)
         check(lib().lgbmhip_predict_raw(self._ctx, ptr(X), X.shape[0], X.shape[1], ptr(out)), self._ctx)
         raw = out.reshape(X.shape[0], self.K)
         if raw_score:
@@ -456,9 +467,10 @@ class Booster:
         addresses) into device u8 [rows][F] with the training mappers."""
         check(lib().lgbmhip_bin_rows_device(self._ctx, x_ptr, rows, ncol, out_ptr), self._ctx)
 
-    def predict_binned(self, bins):
+    def predict_binned(self, bins, start_iteration=0, num_iteration=None):
         b = np.ascontiguousarray(bins, np.uint8)
         out = np.zeros(b.shape[0] * self.K, np.float64)
+        self._window(start_iteration, num_iteration, False, 0, 0.0)
         check(lib().lgbmhip_predict_binned(self._ctx, ptr(b), b.shape[0], ptr(out)), self._ctx)
         return out.reshape(b.shape[0], self.K)
 
@@ -468,6 +480,7 @@ class Booster:
         rows x F u8 bins when `binned`; out is rows x K f64. With torch,
         import torch before this package first loads its library, so the
         process holds one HIP runtime."""
+        self._window(0, None, False, 0, 0.0)
         if binned:
             check(lib().lgbmhip_predict_device(self._ctx, None, x_ptr, rows, 0, out_ptr), self._ctx)
         else:

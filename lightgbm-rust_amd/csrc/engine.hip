@@ -320,6 +320,10 @@ struct Ctx {
   MetricWs* mws = nullptr;
   DevBuf<int> d_qb_eval;   // training query boundaries for NDCG when the objective keeps none
   bool loaded_model = false;   // trees from model text: raw prediction only
+  // prediction window and early stopping (predict.rs:178-202): iterations
+  // start .. start + num (num <= 0: to the end), margin test every freq (0: off)
+  int pred_start = 0, pred_num = 0, pred_es_freq = 0;
+  double pred_es_margin = 0.0;
   // prediction cache
   bool pred_dirty = true;
   DevBuf<uint4> p_nodes;
@@ -1594,15 +1598,20 @@ static void launch_predict(Ctx* c, const float* dX, const uint8_t* dbins, int64_
   if (c->trees.empty()) throw Error(LGBMHIP_INVALID_PARAMETER, "no trees to predict with");
   if (c->pred_dirty || c->p_binned != binned) build_pred(c, binned);
   if (rows <= 0) return;
+  // GBDT::InitPredict's iteration window (predict.rs:193-202)
+  const int K = c->K;
+  const int total = c->p_T / K;
+  const int start = std::min(std::max(c->pred_start, 0), total);
+  const int num = c->pred_num > 0 ? std::min(c->pred_num, total - start) : total - start;
   PredArgs a{};
   a.nodes = c->p_nodes.p;
-  a.tree_node_off = c->p_node_off.p;
-  a.tree_leaf_off = c->p_leaf_off.p;
-  a.tree_num_leaves = c->p_nleaves.p;
+  a.tree_node_off = c->p_node_off.p + (size_t)start * K;
+  a.tree_leaf_off = c->p_leaf_off.p + (size_t)start * K;
+  a.tree_num_leaves = c->p_nleaves.p + (size_t)start * K;
   a.leaf_val = c->p_leafval.p;
   a.cat_words = c->p_cat.p;
-  a.T = c->p_T;
-  a.K = c->K;
+  a.T = num * K;
+  a.K = K;
   a.rows = rows;
   a.ncol = binned ? c->F : ncol;
   a.X = dX;
@@ -1614,6 +1623,16 @@ static void launch_predict(Ctx* c, const float* dX, const uint8_t* dbins, int64_
   a.stride = binned ? sw * 4 : sw;
   const size_t lds = (size_t)kPredRows * sw * 4;
   const dim3 grid((unsigned)((rows + kPredRows - 1) / kPredRows));
+  // binary, multiclass and ranking objectives all allow it (predict.rs:174-190)
+  if (c->pred_es_freq > 0) {
+    if (K > 16) throw Error(LGBMHIP_UNSUPPORTED, "early-stopped prediction of more than 16 classes");
+    if (lds > (size_t)c->lds_max) throw Error(LGBMHIP_UNSUPPORTED, "rows too wide for early-stopped prediction");
+    auto kern = binned ? k_predict_es<true> : k_predict_es<false>;
+    HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, grid, dim3(kPredRows), lds, c->stream, a, c->pred_es_freq, c->pred_es_margin);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
   // waves sharing one staged 64-row tile: a raw f32 tile (~35 KB at 136
   // columns) allows few tiles per CU, so 16 waves share it; 4 for bin tiles
   // (1M x 136 rows, 1000 trees: 42 and 55 M rows/s)
@@ -2596,6 +2615,20 @@ static void put_text(const std::string& s, char* out, int64_t capacity, int64_t*
   if (!out) return;
   if (capacity <= (int64_t)s.size()) throw Error(LGBMHIP_INVALID_PARAMETER, "output buffer too small for the text");
   std::memcpy(out, s.c_str(), s.size() + 1);
+}
+lgbmhip_status lgbmhip_set_predict_window(lgbmhip_ctx* ctx, int32_t start_iteration, int32_t num_iteration,
+                                          int32_t early_stop_freq, double early_stop_margin) {
+  return guarded(ctx, [&] {
+    // Gbdt::prediction_early_stop (predict.rs:178-190)
+    if (early_stop_freq < 0) throw Error(LGBMHIP_INVALID_PARAMETER, "Check failed: (early_stop_freq) > (0)");
+    if (early_stop_freq > 0 && !(early_stop_margin >= 0.0))
+      throw Error(LGBMHIP_INVALID_PARAMETER, "Check failed: (early_stop_margin) >= (0)");
+    Ctx* c = &ctx->c;
+    c->pred_start = start_iteration;
+    c->pred_num = num_iteration;
+    c->pred_es_freq = early_stop_freq;
+    c->pred_es_margin = early_stop_margin;
+  });
 }
 lgbmhip_status lgbmhip_save_model(lgbmhip_ctx* ctx, int32_t num_iteration, const char* feature_names,
                                   const char* feature_infos, char* out, int64_t capacity, int64_t* out_len) {

@@ -1259,14 +1259,76 @@ __device__ __forceinline__ int pred_step(const uint4 n, const float* xr, const u
       }
     } else {
       const bool nan = isnan(v);
-      if (nan && !(flags & 4)) v = 0.0;
-      const bool zero = v >= -KZERO && v <= KZERO;
+      // the reference's predictor drops |v| <= kZeroThreshold (predict.rs:262-267)
+      if ((nan && !(flags & 4)) || (v >= -KZERO && v <= KZERO)) v = 0.0;
+      const bool zero = v == 0.0;
       const bool is_default = ((flags & 2) && zero) || ((flags & 4) && nan);
       const double thr = __hiloint2double((int)n.w, (int)n.z);
       go_left = is_default ? (flags & 1) != 0 : v <= thr;
     }
   }
   return go_left ? left : right;
+}
+
+// Gbdt::predict_matrix_early_stop with early stopping (predict.rs:231-339,
+// the margin test :155-171): one row per lane, the rows staged in LDS, the
+// iterations in order (each class's tree added to the row's raw score of
+// that class, as the reference adds them); every `freq` iterations a row
+// whose margin exceeds the threshold stops taking trees.
+template <bool BINNED>
+__global__ void __launch_bounds__(kPredRows) k_predict_es(PredArgs a, int freq, double margin) {
+  extern __shared__ __align__(16) unsigned char pred_lds[];
+  __shared__ double raw[16][kPredRows];
+  const long long r0 = (long long)blockIdx.x * kPredRows;
+  const int nrow = (int)min((long long)kPredRows, a.rows - r0);
+  const int lane = threadIdx.x;
+  {
+    const long long n = (long long)nrow * a.ncol;
+    for (long long e = lane; e < n; e += kPredRows) {
+      const int r = (int)(e / a.ncol), f = (int)(e % a.ncol);
+      if (BINNED) pred_lds[r * a.stride + f] = a.bins[r0 * a.ncol + e];
+      else ((float*)pred_lds)[r * a.stride + f] = a.X[r0 * a.ncol + e];
+    }
+  }
+  for (int k = 0; k < a.K; ++k) raw[k][lane] = 0.0;
+  __syncthreads();
+  const int rr = min(lane, nrow - 1);
+  const uint8_t* br = BINNED ? pred_lds + rr * a.stride : nullptr;
+  const float* xr = BINNED ? nullptr : (const float*)pred_lds + rr * a.stride;
+  bool stopped = lane >= nrow;
+  const int iters = a.T / a.K;
+  for (int it = 0; it < iters; ++it) {
+    if (!stopped) {
+      for (int k = 0; k < a.K; ++k) {
+        const int t = it * a.K + k;
+        int node = a.tree_num_leaves[t] > 1 ? 0 : ~0;
+        const int base = a.tree_node_off[t];
+        while (node >= 0) node = pred_step<BINNED>(a.nodes[base + node], xr, br, a.ncol, a.cat_words);
+        raw[k][lane] += a.leaf_val[a.tree_leaf_off[t] + ~node];
+      }
+      if ((it + 1) % freq == 0) {
+        double m;
+        if (a.K > 1) {
+          double x = -INFINITY, y = -INFINITY;
+          for (int k = 0; k < a.K; ++k) {
+            const double v = raw[k][lane];
+            if (v > x) {
+              y = x;
+              x = v;
+            } else if (v > y) {
+              y = v;
+            }
+          }
+          m = x - y;
+        } else {
+          m = 2.0 * fabs(raw[0][lane]);
+        }
+        if (m > margin) stopped = true;
+      }
+    }
+  }
+  if (lane < nrow)
+    for (int k = 0; k < a.K; ++k) a.out[(r0 + lane) * a.K + k] = raw[k][lane];
 }
 
 // Gbdt::predict_matrix_early_stop without early stopping (predict.rs:231-339):

@@ -2271,6 +2271,49 @@ void orc_booster_predict_raw(void* p, const float* X, int64_t rows, int ncol, do
   }
 }
 
+// predict.rs:193-202 window and :231-339 with early stopping (:155-171):
+// iterations start .. start + num in order, each class's tree added to its
+// raw score; every freq iterations (freq > 0) a row whose margin exceeds the
+// threshold stops.
+void orc_booster_predict_window(void* p, const float* X, int64_t rows, int ncol, int start_iteration,
+                                int num_iteration, int freq, double margin, double* out) {
+  OrcBooster* b = (OrcBooster*)p;
+  const int K = b->K;
+  const int total = (int)b->models.size() / K;
+  const int start = std::min(std::max(start_iteration, 0), total);
+  const int num = num_iteration > 0 ? std::min(num_iteration, total - start) : total - start;
+  std::vector<double> row(ncol);
+  for (int64_t r = 0; r < rows; ++r) {
+    for (int c = 0; c < ncol; ++c) {
+      double v = (double)X[(size_t)r * ncol + c];
+      if (!std::isnan(v) && std::fabs(v) <= K_ZERO_THRESHOLD) v = 0.0;
+      row[c] = v;
+    }
+    double* raw = out + (size_t)r * K;
+    for (int k = 0; k < K; ++k) raw[k] = 0.0;
+    for (int it = 0; it < num; ++it) {
+      for (int k = 0; k < K; ++k) {
+        const Tree& t = b->models[(size_t)(start + it) * K + k];
+        raw[k] += t.leaf_value[t.get_leaf(row.data(), ncol)];
+      }
+      if (freq > 0 && (it + 1) % freq == 0) {
+        double m;
+        if (K > 1) {
+          double x = -INFINITY, y = -INFINITY;
+          for (int k = 0; k < K; ++k) {
+            if (raw[k] > x) { y = x; x = raw[k]; }
+            else if (raw[k] > y) y = raw[k];
+          }
+          m = x - y;
+        } else {
+          m = 2.0 * std::fabs(raw[0]);
+        }
+        if (m > margin) break;
+      }
+    }
+  }
+}
+
 // --------------------------- stage functions (same layouts as the HIP engine)
 // Feature metas are passed as int arrays of length F:
 //   num_bin, missing_type, default_bin, most_freq_bin, bin_type.

@@ -77,6 +77,7 @@ def lib():
             "orc_max_dcg_at_k": (D, [PF, I, I]),
             "orc_binary_logloss": (D, [PD, PF, I64, D]),
             "orc_booster_bag": (I64, [P, P]),
+            "orc_booster_predict_window": (None, [P, PF, I64, I, I, I, I, D, PD]),
             "orc_metric_auc": (D, [PD, PF, P, I64]),
             "orc_metric_point": (D, [I, PD, PF, P, I64, I, D, I]),
             "orc_metric_ndcg": (None, [PD, PF, PI, I, PI, I, I, PD]),
@@ -221,6 +222,13 @@ class Booster:
         hq = np.zeros(self.ds.n, np.int8)
         lib().orc_learner_quant_packed(self.h, gq, hq)
         return gq, hq
+
+    def predict_window(self, X, start_iteration=0, num_iteration=-1, freq=0, margin=0.0):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros(X.shape[0] * self.K, np.float64)
+        lib().orc_booster_predict_window(self.h, X, X.shape[0], X.shape[1], start_iteration, num_iteration, freq,
+                                         margin, out)
+        return out.reshape(X.shape[0], self.K)
 
     def bag(self):
         """The current bag (in-bag rows ascending), or None without bagging."""

@@ -668,3 +668,35 @@ def test_lambdarank_gradients_every_query_size_class(trunc):
         assert diff.sum() <= max(2, int(1e-4 * len(a)))
         if diff.any():
             assert np.all(np.abs(a[diff].view(np.int32) - c[diff].view(np.int32)) <= 1)
+
+
+@pytest.mark.parametrize("K", [1, 3])
+def test_prediction_window_and_early_stop_match_oracle(K):
+    rng = np.random.default_rng(21)
+    n = 20000
+    X = rng.normal(size=(n, 8)).astype(np.float32)
+    X[:50, 2] = 1e-38          # below kZeroThreshold: predicted as 0
+    if K == 1:
+        y = (X[:, 0] + X[:, 1] > 0).astype(np.float32)
+        params = dict(objective="binary", num_leaves=15, max_bin=63, learning_rate=0.3)
+    else:
+        y = np.argmax(X[:, :K] + 0.5 * rng.gumbel(size=(n, K)), 1).astype(np.float32)
+        params = dict(objective="multiclass", num_class=K, num_leaves=15, max_bin=63, learning_rate=0.3)
+    ds, b = _booster(X, y, params)
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    for _ in range(12):
+        b.update()
+        ob.update()
+    Xt = X[:4000]
+    # trees equal the oracle's; leaf values within FP64 summation noise (1e-9)
+    close = dict(rtol=1e-9, atol=1e-12)
+    full = b.predict(Xt, raw_score=True).reshape(len(Xt), K)
+    np.testing.assert_allclose(full, ob.predict_window(Xt), **close)
+    win = b.predict(Xt, raw_score=True, start_iteration=2, num_iteration=5).reshape(len(Xt), K)
+    np.testing.assert_allclose(win, ob.predict_window(Xt, 2, 5), **close)
+    es = b.predict(Xt, raw_score=True, pred_early_stop=True, pred_early_stop_freq=3,
+                   pred_early_stop_margin=1.0).reshape(len(Xt), K)
+    ref = ob.predict_window(Xt, 0, -1, 3, 1.0)
+    np.testing.assert_allclose(es, ref, **close)
+    stopped = np.any(np.abs(es - full) > 1e-6, axis=1)
+    assert 0 < stopped.sum() < len(Xt)   # some rows stopped early, not all
