@@ -35,6 +35,8 @@
  *   lgbmhip_copy_valid_scores <- Gbdt::valid_scores             boosting.rs:1282-1284
  *   lgbmhip_eval              <- Metric::eval                   metric.rs:525-545, 551-729
  *   lgbmhip_set_predict_window <- predict_window / prediction_early_stop predict.rs:178-202
+ *   lgbmhip_predict_leaf      <- PredictKind::LeafIndex         predict.rs:272-279
+ *   lgbmhip_rollback_one_iter <- Gbdt::rollback_one_iter        boosting.rs:1215-1239
  *   lgbmhip_save_model        <- GBDT::SaveModelToString        io.rs:13-88, tree.rs:718-770
  *   lgbmhip_load_model        <- GBDT::LoadModelFromString      io.rs:156-343
  *   lgbmhip_bins_feature_infos <- Dataset::feature_infos        dataset.rs:770-776
@@ -254,6 +256,15 @@ lgbmhip_status lgbmhip_copy_valid_scores(lgbmhip_ctx* ctx, int32_t index, double
  * a fixed order of their own (within 1e-10 relative). */
 lgbmhip_status lgbmhip_eval(lgbmhip_ctx* ctx, int32_t data, const lgbmhip_metric_opts* opts, double* out,
                             int32_t* out_count);
+
+/* PredictKind::LeafIndex (predict.rs:272-279): each raw row's leaf in every
+ * tree of the prediction window, out [rows][trees] as f64 (out must hold
+ * rows x the window's trees); *out_trees (may be NULL) receives the tree count. */
+lgbmhip_status lgbmhip_predict_leaf(lgbmhip_ctx* ctx, const float* X, int64_t rows, int32_t ncol, double* out,
+                                    int32_t* out_trees);
+/* Gbdt::rollback_one_iter (boosting.rs:1215-1239): the last iteration's trees
+ * leave the training and validation scores and the model. */
+lgbmhip_status lgbmhip_rollback_one_iter(lgbmhip_ctx* ctx);
 
 /* The prediction window and early stopping of later predict_* calls on this
  * context (Gbdt::predict_window / prediction_early_stop, predict.rs:178-202):

@@ -328,6 +328,32 @@ class Booster:
     def num_trees(self):
         return lib().lgbmhip_num_trees(self._ctx)
 
+    def num_model_per_iteration(self):
+        return self.K
+
+    def rollback_one_iter(self):
+        """Drops the last iteration's trees from the model and the scores (Gbdt::rollback_one_iter)."""
+        check(lib().lgbmhip_rollback_one_iter(self._ctx), self._ctx)
+        return self
+
+    def feature_importance(self, importance_type="split", iteration=None):
+        """Split counts or total gains per raw column (boosting.rs:1337-1360)."""
+        if importance_type not in ("split", "gain"):
+            raise LightGBMError("Unknown importance type: only support split=0 and gain=1")
+        ncol = self.train_set.num_total_features if self.train_set is not None else None
+        n_models = self.num_trees()
+        if iteration is not None and int(iteration) > 0:
+            n_models = min(n_models, int(iteration) * self.K)
+        imp = np.zeros(ncol if ncol is not None else 0, np.float64)
+        for t in range(n_models):
+            tr = self.tree(t)
+            for f, g in zip(tr.split_feature, tr.split_gain):
+                if g > 0:
+                    if f >= len(imp):
+                        imp = np.pad(imp, (0, f + 1 - len(imp)))
+                    imp[f] += 1.0 if importance_type == "split" else float(g)
+        return imp
+
     def current_iteration(self):
         return self.num_trees() // self.K
 
@@ -445,11 +471,19 @@ class Booster:
         check(lib().lgbmhip_set_predict_window(self._ctx, int(start_iteration), int(num_iteration), freq,
                                                float(pred_early_stop_margin)), self._ctx)
 
-    def predict(self, data, raw_score=False, start_iteration=0, num_iteration=None, pred_early_stop=False,
-                pred_early_stop_freq=10, pred_early_stop_margin=10.0):
+    def predict(self, data, raw_score=False, start_iteration=0, num_iteration=None, pred_leaf=False,
+                pred_early_stop=False, pred_early_stop_freq=10, pred_early_stop_margin=10.0):
         X = np.ascontiguousarray(data, np.float32)
-        out = np.zeros(X.shape[0] * self.K, np.float64)
         self._window(start_iteration, num_iteration, pred_early_stop, pred_early_stop_freq, pred_early_stop_margin)
+        if pred_leaf:
+            # PredictKind::LeafIndex: one column per tree of the window
+            total = self.num_trees()
+            out = np.zeros(max(X.shape[0] * total, 1), np.float64)
+            nt = C.c_int32(0)
+            check(lib().lgbmhip_predict_leaf(self._ctx, ptr(X), X.shape[0], X.shape[1], ptr(out), C.byref(nt)),
+                  self._ctx)
+            return out[: X.shape[0] * nt.value].reshape(X.shape[0], nt.value).astype(np.int32)
+        out = np.zeros(X.shape[0] * self.K, np.float64)
         check(lib().lgbmhip_predict_raw(self._ctx, ptr(X), X.shape[0], X.shape[1], ptr(out)), self._ctx)
         raw = out.reshape(X.shape[0], self.K)
         if raw_score:

@@ -2057,6 +2057,103 @@ static void load_model(Ctx* c, const char* text) {
   c->loaded_model = true;
 }
 
+// PredictKind::LeafIndex over host raw rows: out [rows][trees of the window].
+static int64_t predict_leaf(Ctx* c, const float* X, int64_t rows, int ncol, double* out) {
+  set_device(c);
+  if (c->trees.empty()) throw Error(LGBMHIP_INVALID_PARAMETER, "no trees to predict with");
+  if (c->pred_dirty || c->p_binned) build_pred(c, false);
+  const int K = c->K, total = c->p_T / K;
+  const int start = std::min(std::max(c->pred_start, 0), total);
+  const int num = c->pred_num > 0 ? std::min(c->pred_num, total - start) : total - start;
+  const int T = num * K;
+  if (rows <= 0 || T == 0) return T;
+  PredArgs a{};
+  a.nodes = c->p_nodes.p;
+  a.tree_node_off = c->p_node_off.p + (size_t)start * K;
+  a.tree_leaf_off = c->p_leaf_off.p + (size_t)start * K;
+  a.tree_num_leaves = c->p_nleaves.p + (size_t)start * K;
+  a.leaf_val = c->p_leafval.p;
+  a.cat_words = c->p_cat.p;
+  a.T = T;
+  a.K = K;
+  a.rows = rows;
+  a.ncol = ncol;
+  const int sw = ncol | 1;
+  a.stride = sw;
+  const size_t lds = (size_t)kPredRows * sw * 4;
+  if (lds > (size_t)c->lds_max) throw Error(LGBMHIP_UNSUPPORTED, "rows too wide for leaf prediction");
+  DevBuf<float> dx;
+  DevBuf<double> dout;
+  dx.alloc((size_t)rows * ncol);
+  dout.alloc((size_t)rows * T);
+  HIP_CHECK(hipMemcpyAsync(dx.p, X, (size_t)rows * ncol * 4, hipMemcpyHostToDevice, c->stream));
+  a.X = dx.p;
+  a.out = dout.p;
+  HIP_CHECK(hipFuncSetAttribute((const void*)k_predict_leaf<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+  hipLaunchKernelGGL(k_predict_leaf<false>, dim3((unsigned)((rows + kPredRows - 1) / kPredRows)), dim3(kPredRows),
+                     lds, c->stream, a);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(out, dout.p, (size_t)rows * T * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return T;
+}
+
+// Gbdt::rollback_one_iter (boosting.rs:1215-1239): the last iteration's trees,
+// shrunk by -1, added to the training and validation scores (binned walks),
+// then dropped.
+static void rollback_one_iter(Ctx* c) {
+  if (!c->state_ready || c->iter == 0 || (int)c->trees.size() < c->K) return;
+  set_device(c);
+  const int K = c->K;
+  if (c->pred_dirty || !c->p_binned) build_pred(c, true);
+  const int first = c->p_T - K;
+  // leaf values of the K trees as shrink(-1) leaves them (tree.rs:303-319)
+  std::vector<int> loff(c->p_T);
+  HIP_CHECK(hipMemcpy(loff.data(), c->p_leaf_off.p, c->p_T * 4, hipMemcpyDeviceToHost));
+  DevBuf<double> neg;
+  neg.alloc(c->p_leafval.n);
+  HIP_CHECK(hipMemcpy(neg.p, c->p_leafval.p, c->p_leafval.n * 8, hipMemcpyDeviceToDevice));
+  for (int k = 0; k < K; ++k) {
+    const HostTree& t = c->trees[(size_t)first + k];
+    std::vector<double> v(t.num_leaves);
+    for (int l = 0; l < t.num_leaves; ++l) v[l] = HostTree::r0(t.leaf_value[l] * -1.0);
+    HIP_CHECK(hipMemcpy(neg.p + loff[first + k], v.data(), v.size() * 8, hipMemcpyHostToDevice));
+  }
+  auto add_tree = [&](const uint8_t* bins, int64_t n, int stride_cols, double* score, int t) {
+    const int words = (stride_cols + 3) / 4, sw = words | 1;
+    const size_t lds = (size_t)kPredRows * sw * 4;
+    auto kern = k_predict<true, true, LGBMHIP_PRED_BIN_W>;
+    HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    PredArgs a{};
+    a.nodes = c->p_nodes.p;
+    a.tree_node_off = c->p_node_off.p + t;
+    a.tree_leaf_off = c->p_leaf_off.p + t;
+    a.tree_num_leaves = c->p_nleaves.p + t;
+    a.leaf_val = neg.p;
+    a.cat_words = c->p_cat.p;
+    a.T = 1;
+    a.K = 1;
+    a.rows = n;
+    a.ncol = stride_cols;   // rows of S bytes: the padding columns are never read by a node
+    a.stride = sw * 4;
+    a.bins = bins;
+    a.out = score;
+    a.accum = 1;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((n + kPredRows - 1) / kPredRows)), dim3(kPredRows * LGBMHIP_PRED_BIN_W),
+                       lds, c->stream, a);
+    HIP_CHECK(hipGetLastError());
+  };
+  for (int k = 0; k < K; ++k) {
+    add_tree(c->bins_rm.p, c->N, c->S, c->score.p + (size_t)k * c->N, first + k);
+    for (auto& v : c->valid) add_tree(v->bins.p, v->n, c->S, v->score.p + (size_t)k * v->n, first + k);
+  }
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->trees.resize(c->trees.size() - K);
+  c->iter--;
+  c->pred_dirty = true;
+}
+
 static void predict(Ctx* c, const float* X, const uint8_t* bins, int64_t rows, int ncol, double* out) {
   set_device(c);
   DevBuf<float> dx;
@@ -2615,6 +2712,17 @@ static void put_text(const std::string& s, char* out, int64_t capacity, int64_t*
   if (!out) return;
   if (capacity <= (int64_t)s.size()) throw Error(LGBMHIP_INVALID_PARAMETER, "output buffer too small for the text");
   std::memcpy(out, s.c_str(), s.size() + 1);
+}
+lgbmhip_status lgbmhip_predict_leaf(lgbmhip_ctx* ctx, const float* X, int64_t rows, int32_t ncol, double* out,
+                                    int32_t* out_trees) {
+  return guarded(ctx, [&] {
+    if (!X || !out || rows < 0 || ncol <= 0) throw Error(LGBMHIP_INVALID_PARAMETER, "bad prediction buffers");
+    const int64_t T = predict_leaf(&ctx->c, X, rows, ncol, out);
+    if (out_trees) *out_trees = (int32_t)T;
+  });
+}
+lgbmhip_status lgbmhip_rollback_one_iter(lgbmhip_ctx* ctx) {
+  return guarded(ctx, [&] { rollback_one_iter(&ctx->c); });
 }
 lgbmhip_status lgbmhip_set_predict_window(lgbmhip_ctx* ctx, int32_t start_iteration, int32_t num_iteration,
                                           int32_t early_stop_freq, double early_stop_margin) {

@@ -1331,6 +1331,32 @@ __global__ void __launch_bounds__(kPredRows) k_predict_es(PredArgs a, int freq, 
     for (int k = 0; k < a.K; ++k) a.out[(r0 + lane) * a.K + k] = raw[k][lane];
 }
 
+// PredictKind::LeafIndex (predict.rs:272-279): each row's leaf in every tree
+// of the window, out[row][tree] as f64; one row per lane, rows in LDS.
+template <bool BINNED>
+__global__ void __launch_bounds__(kPredRows) k_predict_leaf(PredArgs a) {
+  extern __shared__ __align__(16) unsigned char pred_lds[];
+  const long long r0 = (long long)blockIdx.x * kPredRows;
+  const int nrow = (int)min((long long)kPredRows, a.rows - r0);
+  const int lane = threadIdx.x;
+  const long long n = (long long)nrow * a.ncol;
+  for (long long e = lane; e < n; e += kPredRows) {
+    const int r = (int)(e / a.ncol), f = (int)(e % a.ncol);
+    if (BINNED) pred_lds[r * a.stride + f] = a.bins[r0 * a.ncol + e];
+    else ((float*)pred_lds)[r * a.stride + f] = a.X[r0 * a.ncol + e];
+  }
+  __syncthreads();
+  if (lane >= nrow) return;
+  const uint8_t* br = BINNED ? pred_lds + lane * a.stride : nullptr;
+  const float* xr = BINNED ? nullptr : (const float*)pred_lds + lane * a.stride;
+  for (int t = 0; t < a.T; ++t) {
+    int node = a.tree_num_leaves[t] > 1 ? 0 : ~0;
+    const int base = a.tree_node_off[t];
+    while (node >= 0) node = pred_step<BINNED>(a.nodes[base + node], xr, br, a.ncol, a.cat_words);
+    a.out[(r0 + lane) * (long long)a.T + t] = (double)~node;
+  }
+}
+
 // Gbdt::predict_matrix_early_stop without early stopping (predict.rs:231-339):
 // one row per lane, the block's rows staged in LDS (STAGE) at an odd word
 // stride so lanes reading one feature hit distinct banks. For each class,

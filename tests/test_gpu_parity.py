@@ -700,3 +700,34 @@ def test_prediction_window_and_early_stop_match_oracle(K):
     np.testing.assert_allclose(es, ref, **close)
     stopped = np.any(np.abs(es - full) > 1e-6, axis=1)
     assert 0 < stopped.sum() < len(Xt)   # some rows stopped early, not all
+
+
+def test_leaf_prediction_and_rollback():
+    X, y = datagen.binary_100k(30000)
+    params = dict(P_BIN, num_leaves=31, max_bin=63)
+    ds, b = _booster(X, y, params)
+    dv = lgb.Dataset(X[:5000], label=y[:5000], reference=ds, params=params)
+    b.add_valid(dv, "v")
+    for _ in range(4):
+        b.update()
+    s3 = None
+    # leaf indices: the training partition's leaf of each row for the last tree
+    leaves = b.predict(X, pred_leaf=True)
+    assert leaves.shape == (len(X), 4)
+    np.testing.assert_array_equal(leaves[:, 3], b.leaf_of_row())
+    np.testing.assert_array_equal(b.predict(X[:100], pred_leaf=True, start_iteration=1, num_iteration=2),
+                                  leaves[:100, 1:3])
+    # rollback: scores go back to those after 3 iterations (up to the f64 add/subtract rounding)
+    b2 = lgb.Booster(params, ds)
+    for _ in range(3):
+        b2.update()
+    s3 = b2.train_scores()
+    b.rollback_one_iter()
+    assert b.num_trees() == 3
+    np.testing.assert_allclose(b.train_scores(), s3, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(b.valid_scores(0), b.predict_binned(dv.bins)[:, 0], rtol=0, atol=1e-12)
+    # training continues from the rolled-back state
+    b.update()
+    assert b.num_trees() == 4
+    imp = b.feature_importance()
+    assert imp.sum() == sum(b.tree(t).num_leaves - 1 for t in range(4))
