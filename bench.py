@@ -18,8 +18,11 @@ around every histogram / slab-reduction / split / partition launch on the
 stream the kernels run on, which gives the histogram kernel's average launch
 duration (its own launch only, plus the dispatch gap before it) and GB/s. HBM
 traffic per histogram launch comes from rocprofv3 FETCH_SIZE / WRITE_SIZE
-passes over this same command, scaled by the counters' reading on a copy of
-known size (--copy-calibrate), recorded in profiles/hist_traffic.json.
+passes over this same command with --prof-all (every iteration's algorithmic
+histogram bytes counted), scaled by the counters' reading on a copy of known
+size (--copy-calibrate): their ratio of HBM bytes to algorithmic bytes over
+the same dispatches is recorded in profiles/hist_traffic.json, and `traffic`
+is that ratio times this run's algorithmic bytes per launch.
 """
 import argparse
 import importlib
@@ -49,8 +52,10 @@ def holdout_auc(holdout):
     return q
 
 
-def run_form(lgb, ds, params, warmup, steps, quality, prof_iters):
+def run_form(lgb, ds, params, warmup, steps, quality, prof_iters, prof_all=False):
     b = lgb.Booster(params, ds)
+    if prof_all:
+        b.profile(1)   # every iteration's launches counted (counter passes)
     for _ in range(warmup):
         b.update()
     b.update()  # one more untimed iteration (graph already captured)
@@ -62,7 +67,8 @@ def run_form(lgb, ds, params, warmup, steps, quality, prof_iters):
     t1 = time.perf_counter()
     its = done / (t1 - t0)
     # kernel-level profile over a few more iterations (HIP events around each launch)
-    b.profile(1)
+    if not prof_all:
+        b.profile(1)
     for _ in range(prof_iters):
         b.update()
     prof = b.profile(0)
@@ -114,8 +120,8 @@ def copy_calibrate(mib):
 
 
 def recorded_traffic(form, rows, max_bin, num_leaves):
-    """HBM bytes per histogram launch from the committed counter passes, when
-    they were taken on this exact workload."""
+    """HBM bytes per algorithmic byte of the histogram launches, from the
+    committed counter passes, when they were taken on this exact workload."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
@@ -124,7 +130,7 @@ def recorded_traffic(form, rows, max_bin, num_leaves):
     w = t.get("workload", {})
     if (w.get("form"), w.get("rows"), w.get("max_bin"), w.get("num_leaves")) != (form, rows, max_bin, num_leaves):
         return None
-    return t.get("hbm_bytes_per_launch")
+    return t.get("hbm_per_algorithmic_byte")
 
 
 def _line(metric, value, unit, args, ms, dtype, config, **extra):
@@ -420,6 +426,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="quant,plain")
     ap.add_argument("--no-graph", action="store_true", help="stream launches instead of graphs (counter passes)")
+    ap.add_argument("--prof-all", action="store_true",
+                    help="count every iteration's histogram bytes, warm-up included (counter passes)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter scaling) and exit")
     ap.add_argument("--workload", default="binary_10p5m", choices=list(WORKLOADS),
@@ -454,7 +462,8 @@ def main():
         if form == "quant":
             params.update(use_quantized_grad=True, num_grad_quant_bins=16, stochastic_rounding=True,
                           quant_train_renew_leaf=True)
-        r = run_form(lgb, ds, params, args.warmup, args.steps, holdout_auc((Xv, yv)), args.prof_iters)
+        r = run_form(lgb, ds, params, args.warmup, args.steps, holdout_auc((Xv, yv)), args.prof_iters,
+                     args.prof_all)
         p = r["prof"]
         hist_gbs = p["hist_bytes"] / (p["hist_ms"] * 1e-3) / 1e9 if p["hist_ms"] > 0 else None
         r["hist_gbs"] = hist_gbs
@@ -467,9 +476,11 @@ def main():
     main_form = "quant" if "quant" in forms else list(forms)[0]
     m = forms[main_form]
     p = m["prof"]
+    ratio = recorded_traffic(main_form, args.rows, args.max_bin, args.num_leaves)
+    traffic = ratio * m["hist_bytes_per_launch"] if ratio else None
     roof = dict(bound="hbm", achieved=m["hist_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=(m["hist_gbs"] / HBM_PEAK_GBS) if m["hist_gbs"] else None,
-                traffic=recorded_traffic(main_form, args.rows, args.max_bin, args.num_leaves),
+                traffic=traffic, traffic_per_algorithmic_byte=ratio,
                 kernel="k_hist_int" if main_form == "quant" else "k_hist_f64",
                 bytes_per_launch=m["hist_bytes_per_launch"], avg_launch_us=m["hist_avg_launch_us"])
     cpu = None

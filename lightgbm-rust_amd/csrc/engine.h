@@ -172,6 +172,7 @@ struct IterDev {
   double grad_scale, hess_scale;            // quantizer scales
   int start;                                // stochastic-rounding start offset
   int pad;
+  unsigned qmax_g, qmax_h;                  // max |gq| and max u8(hq) of the tree's discretized rows
 };
 
 }  // namespace lgbmhip

@@ -574,7 +574,8 @@ static void plan_hist(Ctx* c, bool quant) {
   c->hist_fb = fb;
   c->hist_ng = ng;
   c->hist_tpr = tpr;
-  c->hist_lds = (size_t)std::min(fb, c->F) * per_feat;
+  // + 64 trash entries of the quantized form's 32-bit histogram
+  c->hist_lds = (size_t)std::min(fb, c->F) * per_feat + (quant ? 256 : 0);
   if (c->hist_lds > c->lds_max) throw Error(LGBMHIP_UNSUPPORTED, "histogram does not fit in LDS");
   set_hist_lds_limits(c->hist_lds);
 }
@@ -2495,11 +2496,14 @@ static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc
   // pack (g, h) in the row order of `rows` into gh buffer 1 positions 0..m-1
   std::vector<uint8_t> pk((size_t)std::max<int64_t>(m, 1) * (quant ? 2 : 8));
   unsigned gbits = 0, hbits = 0;
+  int qg = 0, qh = 0;   // max |gq| and max u8(hq) of the rows
   for (int64_t i = 0; i < m; ++i) {
     const uint32_t r = rows[i];
     if (quant) {
       pk[2 * i] = ((const uint8_t*)gsrc)[r];
       pk[2 * i + 1] = ((const uint8_t*)hsrc)[r];
+      qg = std::max(qg, std::abs((int)((const int8_t*)gsrc)[r]));
+      qh = std::max(qh, (int)((const uint8_t*)hsrc)[r]);
     } else {
       float gv = ((const float*)gsrc)[r], hv = ((const float*)hsrc)[r];
       std::memcpy(&pk[8 * i], &gv, 4);
@@ -2525,6 +2529,9 @@ static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc
   a.d_buf = 1;
   a.d_shift_g = fix_shift(m, gbits);
   a.d_shift_h = fix_shift(m, hbits);
+  a.q_host = 1;
+  a.q_gmax = qg;
+  a.q_hmax = qh;
   const int C = quant ? 1 : 2;
   const size_t cnt = (size_t)c->F * c->B * C;
   std::vector<long long> acc(ln.acc.n, 0);

@@ -89,8 +89,10 @@ __global__ void k_state_init(StepDev* st, PosRec* pos, LeafDev* leaves, uint8_t*
 // Row-wise histograms (multi_val_bin.rs:556-611 FP64, :710-791 integer).
 // FP64 values are summed as exact fixed-point int64 (ds_add_u64 in LDS):
 // exact, so the result is independent of order and block count. The
-// quantized form adds each row's packed (g << 32) | u8(h) with one wrapping
-// ds_add_u64 per bin, the reference's own sum of packed rows.
+// quantized form's result is the reference's own wrapping sum of packed
+// (g << 32) | u8(h) rows; it accumulates in 32-bit LDS bins (16-bit fields,
+// folded to int64 before they can overflow) and falls back to one wrapping
+// ds_add_u64 per bin when a row's values are too wide for that.
 //
 // Every leaf's rows are one contiguous segment of an ordered buffer (buffer
 // 0 holds the root in row order; a leaf in buffer 1 splits into buffer 2 and
@@ -126,6 +128,8 @@ struct HistArgs {
   long long root_n;        // rows of the root (all rows, or the bag)
   long long d_begin, d_count;
   int d_buf, d_shift_g, d_shift_h;
+  int q_host;              // 1: the quantized maxima below (an explicit segment), 0: the tree's, from it
+  int q_gmax, q_hmax;      // max |gq| and max u8(hq) of the segment's rows
 };
 
 // rows each thread has in flight per pass (all loads issued before the atomics)
@@ -218,6 +222,148 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
   __syncthreads();
   for (int e = threadIdx.x; e < n_lds; e += blockDim.x) out_slab[e] = (long long)lds64[e];
   __syncthreads();   // the next unit reuses the LDS histogram
+}
+
+// The quantized form in 32-bit LDS bins. A row adds gq * 65536 + u8(hq)
+// with one wrapping ds_add_u32. While an entry's sums stay within
+// |sum gq| <= 32767 and sum u8(hq) <= 65535, the 32-bit word holds them
+// exactly: as a signed int it equals sum_gq * 65536 + sum_hq, so its low 16
+// bits are sum_hq and its arithmetic shift right by 16 is sum_gq. Rows are
+// taken in epochs short enough for that bound (from the rows' max |gq| and
+// max u8(hq)); after each epoch a thread folds its entries into 64-bit
+// registers as (sum_gq << 32) + sum_hq, the reference's packed sum of the
+// same rows, and clears them. The unit then stores the registers to its
+// slab, the same int64 entries as the 64-bit form.
+constexpr int kQ32Ept = 16;   // LDS entries per thread a block can fold (FB * B <= 16 * threads)
+
+// Passes (of kHistUnroll * rows-per-pass rows) per epoch, 0 when one pass can
+// overflow a 16-bit field.
+__device__ __forceinline__ int q32_epoch_passes(unsigned gmax, unsigned hmax, int pass_rows) {
+  const unsigned g = max(gmax, 1u), h = max(hmax, 1u);
+  const unsigned rows = min(32767u / g, 65535u / h);
+  return (int)(rows / (unsigned)pass_rows);
+}
+
+// A wave-uniform copy of a 64-bit value (kept in scalar registers, so loads
+// from a uniform base take 32-bit per-lane offsets).
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// The row loop has no branches per bin: a bin equal to the feature's most
+// frequent bin (never accumulated), and every byte of a feature the lane
+// does not hold, adds to a per-lane trash entry past the histogram; rows
+// past the chunk's end (the last pass) add 0.
+template <int TPR>
+__device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32, long long begin, long long count,
+                                              int buf, int chunk, int nchunks, int fg, long long* out_slab,
+                                              int epoch_passes) {
+  constexpr int T = HistBlock<true>::threads;
+  constexpr int rpp = T / TPR;
+  constexpr int PR = kHistUnroll * rpp;   // rows per pass
+  const long long per = (count + nchunks - 1) / nchunks;
+  const long long r0 = (long long)chunk * per, r1 = min(count, r0 + per);
+  const int rows = (int)max(0LL, r1 - r0);
+  const int f0 = fg * a.FB;
+  const int FBg = min(a.FB, a.F - f0);
+  const int B = a.B;
+  const int n_lds = FBg * B;
+  const int sub = threadIdx.x / TPR, j = threadIdx.x % TPR;
+  const unsigned trash = (unsigned)(min(a.FB, a.F) * B + (threadIdx.x & 63)) * 4u;   // byte offset
+  unsigned fbase[4], mfb[4], bytemask = 0u;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int fl = j * 4 + k;
+    const bool on = fl < FBg;
+    fbase[k] = on ? (unsigned)(fl * B) * 4u : 0u;
+    mfb[k] = on ? (unsigned)a.feats[f0 + fl].mfb : 0u;   // a masked byte reads 0: the trash
+    bytemask |= on ? (0xFFu << (8 * k)) : 0u;
+  }
+  unsigned long long acc[kQ32Ept];
+#pragma unroll
+  for (int i = 0; i < kQ32Ept; ++i) acc[i] = 0ull;
+  // folds the block's 32-bit entries into acc and clears them (the LDS
+  // histogram is all zero between units)
+  auto fold = [&]() {
+#pragma unroll
+    for (int i = 0; i < kQ32Ept; ++i) {
+      const int e = threadIdx.x + i * T;
+      if (e < n_lds) {
+        const unsigned x = lds32[e];
+        lds32[e] = 0u;
+        const long long sg = (long long)((int)x >> 16);
+        acc[i] += ((unsigned long long)sg << 32) + (unsigned long long)(x & 0xFFFFu);
+      }
+    }
+  };
+  const int FB = a.FB;
+  const unsigned jofs = bytemask ? (unsigned)(j * 4) : 0u;
+  typedef const __attribute__((address_space(1))) uint8_t* GBytes;
+  typedef const __attribute__((address_space(1))) unsigned short* GHalf;
+  typedef __attribute__((address_space(3))) unsigned* LWord;
+  const GBytes gu = (GBytes)uniform_u64(
+      (unsigned long long)(sel3(a.grp[0], a.grp[1], a.grp[2], buf) + ((size_t)fg * a.N + begin + r0) * FB));
+  const GHalf hu = (GHalf)uniform_u64(
+      (unsigned long long)((const char2*)sel3(a.gh[0], a.gh[1], a.gh[2], buf) + begin + r0));
+  // LDS byte addresses, the histogram's base folded in
+  const unsigned lbase = (unsigned)(size_t)(LWord)lds32;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) fbase[k] += lbase;
+  const unsigned trash_a = trash + lbase;
+  // one pass's loads (a row past the chunk reloads its last row)
+  auto load_pass = [&](int q0, unsigned* w, unsigned* hr) {
+    const bool full = q0 + PR <= rows;
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      int q = q0 + u * rpp + sub;
+      if (!full) q = min(q, rows - 1);
+      w[u] = *(const __attribute__((address_space(1))) unsigned*)(gu + ((unsigned)q * (unsigned)FB + jofs));
+      hr[u] = hu[(unsigned)q];
+    }
+  };
+  int passes = 0;
+  // the next pass's loads are issued before this pass's atomics
+  unsigned w[kHistUnroll], hr[kHistUnroll], wn[kHistUnroll], hn[kHistUnroll];
+  if (rows > 0) load_pass(0, w, hr);
+  for (int q0 = 0; q0 < rows; q0 += PR) {
+    const bool full = q0 + PR <= rows;
+    load_pass(q0 + PR, wn, hn);   // past the end: the chunk's last row again (no branch, so loads stay counted)
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      // multi_val_bin.rs:710-791 with the row's (gq, hq) as gq * 65536 + u8(hq)
+      const unsigned v0 = ((unsigned)(int)(signed char)(hr[u] & 0xFFu) << 16) + (hr[u] >> 8);
+      const unsigned v = (full || q0 + u * rpp + sub < rows) ? v0 : 0u;
+      const unsigned ww = w[u] & bytemask;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const unsigned b = (ww >> (8 * k)) & 0xFFu;
+        const unsigned addr = b == mfb[k] ? trash_a : fbase[k] + b * 4u;
+        __hip_atomic_fetch_add((LWord)(size_t)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);   // wrapping
+      }
+    }
+    // the loop bounds are the block's own, so every thread takes this branch alike
+    if (++passes == epoch_passes && q0 + PR < rows) {
+      __syncthreads();
+      fold();
+      __syncthreads();
+      passes = 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kHistUnroll; ++u) {
+      w[u] = wn[u];
+      hr[u] = hn[u];
+    }
+  }
+  __syncthreads();
+  fold();
+  __syncthreads();   // the next unit's atomics start on a cleared histogram
+#pragma unroll
+  for (int i = 0; i < kQ32Ept; ++i) {
+    const int e = threadIdx.x + i * T;
+    if (e < n_lds) out_slab[e] = (long long)acc[i];
+  }
 }
 
 // Applies the round's plan to the tree state (block 0 of the histogram pass,
@@ -366,6 +512,25 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
   __syncthreads();
   const int units = s_u0[nitems];
   int w = 0;
+  if (Q) {
+    constexpr int T = HistBlock<true>::threads;
+    const int passes = a.FB * a.B <= kQ32Ept * T
+                           ? q32_epoch_passes(a.q_host ? (unsigned)a.q_gmax : a.it->qmax_g,
+                                              a.q_host ? (unsigned)a.q_hmax : a.it->qmax_h, kHistUnroll * (T / TPR))
+                           : 0;
+    if (passes > 0) {
+      unsigned* lds32 = (unsigned*)lds;
+      for (int e = tid; e < min(a.FB, a.F) * a.B; e += T) lds32[e] = 0u;
+      __syncthreads();
+      for (int u = blockIdx.x; u < units; u += gridDim.x) {
+        while (u >= s_u0[w + 1]) ++w;
+        const int r = u - s_u0[w];
+        hist_unit_q32<TPR>(a, lds32, s_begin[w], s_count[w], s_buf[w], r / a.NG, s_nch[w], r % a.NG,
+                           a.slab + (size_t)u * ustride, passes);
+      }
+      return;
+    }
+  }
   for (int u = blockIdx.x; u < units; u += gridDim.x) {
     while (u >= s_u0[w + 1]) ++w;
     const int r = u - s_u0[w];

@@ -181,6 +181,8 @@ __global__ void k_iter_reset(IterDev* it, int K) {
     it[k].root_int_sum = 0;
     it[k].root_gq_sum = 0;
     it[k].root_hq_sum = 0;
+    it[k].qmax_g = 0;
+    it[k].qmax_h = 0;
   }
 }
 
@@ -544,6 +546,7 @@ __global__ void __launch_bounds__(256) k_discretize(long long n, const float* __
   const long long start = starts[*call_ctr + call_off];
   const long long start_n = n > 0 ? start % n : 0;   // a bag has fewer rows than the offsets' range
   long long ai = 0, agq = 0, ahq = 0;
+  unsigned mq = 0;   // max |gq| (low 16 bits) and max u8(hq) (high 16 bits), for the histogram's 32-bit bins
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     // position i of the root (of the bag: row bag[i], quant.rs:92-129 with rows)
     const long long r = bag ? (long long)bag[i] : i;
@@ -566,14 +569,19 @@ __global__ void __launch_bounds__(256) k_discretize(long long n, const float* __
     ai += (long long)(((unsigned long long)(long long)a << 32) | (unsigned long long)(uint8_t)b);
     agq += a;
     ahq += b;
+    const unsigned ag = (unsigned)(a < 0 ? -(int)a : (int)a), bh = (unsigned)(uint8_t)b;
+    mq = max(mq & 0xFFFFu, ag) | (max(mq >> 16, bh) << 16);
   }
   ai = block_sum_ll(ai);
   agq = block_sum_ll(agq);
   ahq = block_sum_ll(ahq);
+  const unsigned qg = block_max_u(mq & 0xFFFFu), qh = block_max_u(mq >> 16);
   if (threadIdx.x == 0) {
     atomicAdd((unsigned long long*)&it->root_int_sum, (unsigned long long)ai);
     atomicAdd((unsigned long long*)&it->root_gq_sum, (unsigned long long)agq);
     atomicAdd((unsigned long long*)&it->root_hq_sum, (unsigned long long)ahq);
+    atomicMax(&it->qmax_g, qg);
+    atomicMax(&it->qmax_h, qh);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     it->grad_scale = gs;
@@ -1770,6 +1778,8 @@ __global__ void __launch_bounds__(256) k_bag_counts(const uint8_t* flags, long l
 __global__ void k_clear_max(IterDev* it) {
   it->max_abs_g_bits = 0;
   it->max_abs_h_bits = 0;
+  it->qmax_g = 0;
+  it->qmax_h = 0;
 }
 
 // ---------------------------------------------------------------- device binning

@@ -84,13 +84,17 @@ def test_histogram_is_deterministic_and_order_free():
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
 
 
-def test_histogram_int_bit_exact():
+# (max |gq|, max hq): the 16-bin quantizer's range (32-bit LDS bins, folded
+# mid-chunk), a range that folds after every pass, and full int8 values (a
+# negative hq is a u8 up to 255: the 64-bit LDS bins)
+@pytest.mark.parametrize("gmax,hmax", [(8, 16), (16, 31), (128, 127)])
+def test_histogram_int_bit_exact(gmax, hmax):
     X, y = datagen.binary_100k(40000)
     params = dict(P_BIN, use_quantized_grad=True, num_grad_quant_bins=16)
     ds, b = _booster(X, y, params)
     rng = np.random.default_rng(3)
-    gq = rng.integers(-8, 9, len(y)).astype(np.int8)
-    hq = rng.integers(0, 17, len(y)).astype(np.int8)
+    gq = rng.integers(-gmax, min(gmax, 127) + 1, len(y)).astype(np.int8)
+    hq = rng.integers(-128 if gmax == 128 else 0, hmax + 1, len(y)).astype(np.int8)
     rows = np.sort(rng.choice(len(y), 25000, replace=False)).astype(np.uint32)
     B = max(ds.feature_meta(f)["num_bin"] for f in range(ds.num_features))
     out = np.zeros(ds.num_features * B, np.int64)
