@@ -574,8 +574,9 @@ static void plan_hist(Ctx* c, bool quant) {
   c->hist_fb = fb;
   c->hist_ng = ng;
   c->hist_tpr = tpr;
-  // + 64 trash entries of the quantized form's 32-bit histogram
-  c->hist_lds = (size_t)std::min(fb, c->F) * per_feat + (quant ? 256 : 0);
+  c->hist_lds = (size_t)std::min(fb, c->F) * per_feat;
+  // the quantized form's 32-bit bins, trash words and int64 sums
+  if (quant) c->hist_lds = std::max(c->hist_lds, q32_lds_bytes(std::min(fb, c->F) * c->B));
   if (c->hist_lds > c->lds_max) throw Error(LGBMHIP_UNSUPPORTED, "histogram does not fit in LDS");
   set_hist_lds_limits(c->hist_lds);
 }

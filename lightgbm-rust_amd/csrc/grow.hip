@@ -230,11 +230,15 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
 // exactly: as a signed int it equals sum_gq * 65536 + sum_hq, so its low 16
 // bits are sum_hq and its arithmetic shift right by 16 is sum_gq. Rows are
 // taken in epochs short enough for that bound (from the rows' max |gq| and
-// max u8(hq)); after each epoch a thread folds its entries into 64-bit
-// registers as (sum_gq << 32) + sum_hq, the reference's packed sum of the
-// same rows, and clears them. The unit then stores the registers to its
-// slab, the same int64 entries as the 64-bit form.
-constexpr int kQ32Ept = 16;   // LDS entries per thread a block can fold (FB * B <= 16 * threads)
+// max u8(hq)); after each epoch a thread folds its entries into an int64
+// LDS copy as (sum_gq << 32) + sum_hq, the reference's packed sum of the
+// same rows, and clears them. The unit then stores the int64 copy to its
+// slab, the same entries as the 64-bit form.
+//
+// LDS of a block: [n32 32-bit bins][64 trash words][n32 int64 sums], n32 =
+// min(FB, F) * B, the int64 part 8-byte aligned.
+__host__ __device__ inline size_t q32_acc_offset(int n32) { return ((size_t)(n32 + 64) * 4 + 7) / 8 * 8; }
+__host__ __device__ inline size_t q32_lds_bytes(int n32) { return q32_acc_offset(n32) + (size_t)n32 * 8; }
 
 // Passes (of kHistUnroll * rows-per-pass rows) per epoch, 0 when one pass can
 // overflow a 16-bit field.
@@ -270,8 +274,12 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
   const int FBg = min(a.FB, a.F - f0);
   const int B = a.B;
   const int n_lds = FBg * B;
+  const int n32 = min(a.FB, a.F) * B;
   const int sub = threadIdx.x / TPR, j = threadIdx.x % TPR;
-  const unsigned trash = (unsigned)(min(a.FB, a.F) * B + (threadIdx.x & 63)) * 4u;   // byte offset
+  const unsigned trash = (unsigned)(n32 + (threadIdx.x & 63)) * 4u;   // byte offset
+  // this thread's entries e = threadIdx.x + i * T: only it reads and writes
+  // their int64 sums, so those need no barrier
+  unsigned long long* acc = (unsigned long long*)((char*)lds32 + q32_acc_offset(n32));
   unsigned fbase[4], mfb[4], bytemask = 0u;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -281,21 +289,15 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
     mfb[k] = on ? (unsigned)a.feats[f0 + fl].mfb : 0u;   // a masked byte reads 0: the trash
     bytemask |= on ? (0xFFu << (8 * k)) : 0u;
   }
-  unsigned long long acc[kQ32Ept];
-#pragma unroll
-  for (int i = 0; i < kQ32Ept; ++i) acc[i] = 0ull;
-  // folds the block's 32-bit entries into acc and clears them (the LDS
+  for (int e = threadIdx.x; e < n_lds; e += T) acc[e] = 0ull;
+  // folds the block's 32-bit entries into acc and clears them (the 32-bit
   // histogram is all zero between units)
   auto fold = [&]() {
-#pragma unroll
-    for (int i = 0; i < kQ32Ept; ++i) {
-      const int e = threadIdx.x + i * T;
-      if (e < n_lds) {
-        const unsigned x = lds32[e];
-        lds32[e] = 0u;
-        const long long sg = (long long)((int)x >> 16);
-        acc[i] += ((unsigned long long)sg << 32) + (unsigned long long)(x & 0xFFFFu);
-      }
+    for (int e = threadIdx.x; e < n_lds; e += T) {
+      const unsigned x = lds32[e];
+      lds32[e] = 0u;
+      const long long sg = (long long)((int)x >> 16);
+      acc[e] += ((unsigned long long)sg << 32) + (unsigned long long)(x & 0xFFFFu);
     }
   };
   const int FB = a.FB;
@@ -359,11 +361,7 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
   __syncthreads();
   fold();
   __syncthreads();   // the next unit's atomics start on a cleared histogram
-#pragma unroll
-  for (int i = 0; i < kQ32Ept; ++i) {
-    const int e = threadIdx.x + i * T;
-    if (e < n_lds) out_slab[e] = (long long)acc[i];
-  }
+  for (int e = threadIdx.x; e < n_lds; e += T) out_slab[e] = (long long)acc[e];
 }
 
 // Applies the round's plan to the tree state (block 0 of the histogram pass,
@@ -514,10 +512,8 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
   int w = 0;
   if (Q) {
     constexpr int T = HistBlock<true>::threads;
-    const int passes = a.FB * a.B <= kQ32Ept * T
-                           ? q32_epoch_passes(a.q_host ? (unsigned)a.q_gmax : a.it->qmax_g,
-                                              a.q_host ? (unsigned)a.q_hmax : a.it->qmax_h, kHistUnroll * (T / TPR))
-                           : 0;
+    const int passes = q32_epoch_passes(a.q_host ? (unsigned)a.q_gmax : a.it->qmax_g,
+                                        a.q_host ? (unsigned)a.q_hmax : a.it->qmax_h, kHistUnroll * (T / TPR));
     if (passes > 0) {
       unsigned* lds32 = (unsigned*)lds;
       for (int e = tid; e < min(a.FB, a.F) * a.B; e += T) lds32[e] = 0u;
@@ -545,8 +541,11 @@ __global__ void __launch_bounds__(HistBlock<false>::threads) k_hist_f64(HistArgs
   hist_kernel<TPR, false>(a, lds_hist);
 }
 
+// three 512-thread blocks per CU (6 waves per SIMD: at most 80 VGPRs), the
+// LDS of three 48 KiB histograms
 template <int TPR>
-__global__ void __launch_bounds__(HistBlock<true>::threads) k_hist_int(HistArgs a) {
+__global__ void __launch_bounds__(HistBlock<true>::threads) __attribute__((amdgpu_waves_per_eu(6)))
+k_hist_int(HistArgs a) {
   extern __shared__ unsigned long long lds_hist[];
   hist_kernel<TPR, true>(a, lds_hist);
 }
