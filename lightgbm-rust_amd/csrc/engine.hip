@@ -731,7 +731,16 @@ static void setup_training(Ctx* c) {
   c->part_grid = (int)std::min<int64_t>((N + part_tile() - 1) / part_tile(),
                                         c->tune_part_grid > 0 ? c->tune_part_grid : 4 * c->cus);
   c->hist_chunks = c->tune_hist_chunks > 0 ? c->tune_hist_chunks : (quant ? 2 : 1) * c->cus;
-  c->hist_min_rows = c->tune_hist_min_rows > 0 ? c->tune_hist_min_rows : 4096;
+  // quantized: chunks no longer than an epoch of the 32-bit histogram bins
+  // for the quantizer's range (|gq| <= bins / 2, hq <= bins): 4095 rows for
+  // 16 bins, so a chunk's bins go to its slab without an int64 fold
+  int rows_q = 4096;
+  if (quant) {
+    const int qb = std::max(2, c->opts.num_grad_quant_bins);
+    const int epoch = std::min(32767 / std::max(1, qb / 2), 65535 / qb);
+    if (epoch >= 2048) rows_q = std::min(rows_q, epoch);
+  }
+  c->hist_min_rows = c->tune_hist_min_rows > 0 ? c->tune_hist_min_rows : rows_q;
   if (quant) {
     // quant.rs:45-79: per-row tables from block generators of `quant_table_threads` blocks
     std::vector<double> grv(N), hrv(N);

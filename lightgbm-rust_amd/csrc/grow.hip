@@ -240,12 +240,10 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
 __host__ __device__ inline size_t q32_acc_offset(int n32) { return ((size_t)(n32 + 64) * 4 + 7) / 8 * 8; }
 __host__ __device__ inline size_t q32_lds_bytes(int n32) { return q32_acc_offset(n32) + (size_t)n32 * 8; }
 
-// Passes (of kHistUnroll * rows-per-pass rows) per epoch, 0 when one pass can
-// overflow a 16-bit field.
-__device__ __forceinline__ int q32_epoch_passes(unsigned gmax, unsigned hmax, int pass_rows) {
+// Rows per epoch: the most rows whose sums cannot overflow a 16-bit field.
+__device__ __forceinline__ int q32_epoch_rows(unsigned gmax, unsigned hmax) {
   const unsigned g = max(gmax, 1u), h = max(hmax, 1u);
-  const unsigned rows = min(32767u / g, 65535u / h);
-  return (int)(rows / (unsigned)pass_rows);
+  return (int)min(32767u / g, 65535u / h);
 }
 
 // A wave-uniform copy of a 64-bit value (kept in scalar registers, so loads
@@ -263,7 +261,7 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) 
 template <int TPR>
 __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32, long long begin, long long count,
                                               int buf, int chunk, int nchunks, int fg, long long* out_slab,
-                                              int epoch_passes) {
+                                              int epoch_rows) {
   constexpr int T = HistBlock<true>::threads;
   constexpr int rpp = T / TPR;
   constexpr int PR = kHistUnroll * rpp;   // rows per pass
@@ -280,6 +278,9 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
   // this thread's entries e = threadIdx.x + i * T: only it reads and writes
   // their int64 sums, so those need no barrier
   unsigned long long* acc = (unsigned long long*)((char*)lds32 + q32_acc_offset(n32));
+  // a chunk within one epoch goes from the 32-bit bins to its slab directly
+  const bool one_epoch = rows <= epoch_rows;
+  const int epoch_passes = epoch_rows / PR;
   unsigned fbase[4], mfb[4], bytemask = 0u;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -289,7 +290,8 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
     mfb[k] = on ? (unsigned)a.feats[f0 + fl].mfb : 0u;   // a masked byte reads 0: the trash
     bytemask |= on ? (0xFFu << (8 * k)) : 0u;
   }
-  for (int e = threadIdx.x; e < n_lds; e += T) acc[e] = 0ull;
+  if (!one_epoch)
+    for (int e = threadIdx.x; e < n_lds; e += T) acc[e] = 0ull;
   // folds the block's 32-bit entries into acc and clears them (the 32-bit
   // histogram is all zero between units)
   auto fold = [&]() {
@@ -346,7 +348,7 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
       }
     }
     // the loop bounds are the block's own, so every thread takes this branch alike
-    if (++passes == epoch_passes && q0 + PR < rows) {
+    if (!one_epoch && ++passes == epoch_passes && q0 + PR < rows) {
       __syncthreads();
       fold();
       __syncthreads();
@@ -359,6 +361,16 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
     }
   }
   __syncthreads();
+  if (one_epoch) {
+    for (int e = threadIdx.x; e < n_lds; e += T) {
+      const unsigned x = lds32[e];
+      lds32[e] = 0u;
+      const long long sg = (long long)((int)x >> 16);
+      out_slab[e] = (long long)(((unsigned long long)sg << 32) + (unsigned long long)(x & 0xFFFFu));
+    }
+    __syncthreads();   // the next unit's atomics start on a cleared histogram
+    return;
+  }
   fold();
   __syncthreads();   // the next unit's atomics start on a cleared histogram
   for (int e = threadIdx.x; e < n_lds; e += T) out_slab[e] = (long long)acc[e];
@@ -512,9 +524,9 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
   int w = 0;
   if (Q) {
     constexpr int T = HistBlock<true>::threads;
-    const int passes = q32_epoch_passes(a.q_host ? (unsigned)a.q_gmax : a.it->qmax_g,
-                                        a.q_host ? (unsigned)a.q_hmax : a.it->qmax_h, kHistUnroll * (T / TPR));
-    if (passes > 0) {
+    const int erows = q32_epoch_rows(a.q_host ? (unsigned)a.q_gmax : a.it->qmax_g,
+                                     a.q_host ? (unsigned)a.q_hmax : a.it->qmax_h);
+    if (erows >= kHistUnroll * (T / TPR)) {   // at least one pass per epoch
       unsigned* lds32 = (unsigned*)lds;
       for (int e = tid; e < min(a.FB, a.F) * a.B; e += T) lds32[e] = 0u;
       __syncthreads();
@@ -522,7 +534,7 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
         while (u >= s_u0[w + 1]) ++w;
         const int r = u - s_u0[w];
         hist_unit_q32<TPR>(a, lds32, s_begin[w], s_count[w], s_buf[w], r / a.NG, s_nch[w], r % a.NG,
-                           a.slab + (size_t)u * ustride, passes);
+                           a.slab + (size_t)u * ustride, erows);
       }
       return;
     }
