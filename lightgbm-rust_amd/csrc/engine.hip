@@ -188,7 +188,7 @@ struct Lane {
   DevBuf<long long> acc;       // each item's histogram summed over its chunks, one unit per group
   DevBuf<int> acc_units;       // {0, G, 2G, ...}: acc read as one unit per (item, group)
   DevBuf<long long> slab;      // histogram units of one histogram pass
-  DevBuf<int> units;           // [kMaxBatch + 1] first unit of each item
+  DevBuf<int> units;           // first unit of each item, total, item count, slab form of each item
   DevBuf<double> pool;
   DevBuf<long long> ipool;
   DevBuf<SplitRec> feat_split, best, log;
@@ -700,7 +700,7 @@ static void setup_training(Ctx* c) {
     for (int b = 0; b < 3; ++b) ln.gh[b].alloc((size_t)N * (quant ? 2 : 8));
     ln.nodes.alloc(L);
     ln.leaf_shift.alloc((size_t)2 * L);
-    ln.units.alloc(kMaxBatch + 2);
+    ln.units.alloc(2 * kMaxBatch + 2);   // first units, total, item count, each item's slab form
     ln.units.zero(c->stream);
     ln.acc_units.alloc(kMaxBatch + 1);
     if (quant) ln.ipool.alloc((size_t)MS * F * B);

@@ -240,6 +240,10 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
 __host__ __device__ inline size_t q32_acc_offset(int n32) { return ((size_t)(n32 + 64) * 4 + 7) / 8 * 8; }
 __host__ __device__ inline size_t q32_lds_bytes(int n32) { return q32_acc_offset(n32) + (size_t)n32 * 8; }
 
+// units[kUnitForm + w]: 1 when item w's slabs hold packed 32-bit bins (the
+// quantized form within one epoch), 0 for int64 entries.
+constexpr int kUnitForm = kMaxBatch + 2;
+
 // Rows per epoch: the most rows whose sums cannot overflow a 16-bit field.
 __device__ __forceinline__ int q32_epoch_rows(unsigned gmax, unsigned hmax) {
   const unsigned g = max(gmax, 1u), h = max(hmax, 1u);
@@ -278,8 +282,10 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
   // this thread's entries e = threadIdx.x + i * T: only it reads and writes
   // their int64 sums, so those need no barrier
   unsigned long long* acc = (unsigned long long*)((char*)lds32 + q32_acc_offset(n32));
-  // a chunk within one epoch goes from the 32-bit bins to its slab directly
-  const bool one_epoch = rows <= epoch_rows;
+  // an item whose chunks fit one epoch stores its 32-bit bins to the slab
+  // as they are (the reduction unpacks them), decided by the item's chunk
+  // length so that every chunk of the item has the same form
+  const bool one_epoch = per <= epoch_rows;
   const int epoch_passes = epoch_rows / PR;
   unsigned fbase[4], mfb[4], bytemask = 0u;
 #pragma unroll
@@ -362,11 +368,10 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
   }
   __syncthreads();
   if (one_epoch) {
+    unsigned* out32 = (unsigned*)out_slab;
     for (int e = threadIdx.x; e < n_lds; e += T) {
-      const unsigned x = lds32[e];
+      out32[e] = lds32[e];
       lds32[e] = 0u;
-      const long long sg = (long long)((int)x >> 16);
-      out_slab[e] = (long long)(((unsigned long long)sg << 32) + (unsigned long long)(x & 0xFFFFu));
     }
     __syncthreads();   // the next unit's atomics start on a cleared histogram
     return;
@@ -526,7 +531,12 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
     constexpr int T = HistBlock<true>::threads;
     const int erows = q32_epoch_rows(a.q_host ? (unsigned)a.q_gmax : a.it->qmax_g,
                                      a.q_host ? (unsigned)a.q_hmax : a.it->qmax_h);
-    if (erows >= kHistUnroll * (T / TPR)) {   // at least one pass per epoch
+    const bool q32 = erows >= kHistUnroll * (T / TPR);   // at least one pass per epoch
+    if (blockIdx.x == 0 && tid < nitems) {
+      const long long per = s_nch[tid] > 0 ? (s_count[tid] + s_nch[tid] - 1) / s_nch[tid] : 0;
+      a.units[kUnitForm + tid] = q32 && per <= erows ? 1 : 0;
+    }
+    if (q32) {
       unsigned* lds32 = (unsigned*)lds;
       for (int e = tid; e < min(a.FB, a.F) * a.B; e += T) lds32[e] = 0u;
       __syncthreads();
@@ -539,6 +549,7 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
       return;
     }
   }
+  if (!Q && blockIdx.x == 0 && tid < nitems) a.units[kUnitForm + tid] = 0;
   for (int u = blockIdx.x; u < units; u += gridDim.x) {
     while (u >= s_u0[w + 1]) ++w;
     const int r = u - s_u0[w];
@@ -1946,7 +1957,25 @@ __global__ void __launch_bounds__(256) k_slab_reduce(const long long* __restrict
     const long long e = (bi % eblocks) * 32 + bo;   // entry of item w: [NG][per_unit]
     const int u0 = units[w], nch = (units[w + 1] - u0) / NG;
     unsigned long long acc = 0;
-    if (e < per_item) {
+    if (e < per_item && units[kUnitForm + w]) {
+      // packed 32-bit bins: sum_gq * 65536 + sum_hq of one chunk each
+      const int g = (int)(e / per_unit), r = (int)(e % per_unit);
+      const unsigned* q = (const unsigned*)(slab + (size_t)(u0 + g) * per_unit) + r;
+      const size_t cs32 = cstride * 2;
+      int c = sl;
+      for (; c + 8 * 7 < nch; c += 8 * 8) {
+        unsigned x[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = q[(size_t)(c + 8 * j) * cs32];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          acc += ((unsigned long long)(long long)((int)x[j] >> 16) << 32) + (unsigned long long)(x[j] & 0xFFFFu);
+      }
+      for (; c < nch; c += 8) {
+        const unsigned x = q[(size_t)c * cs32];
+        acc += ((unsigned long long)(long long)((int)x >> 16) << 32) + (unsigned long long)(x & 0xFFFFu);
+      }
+    } else if (e < per_item) {
       const int g = (int)(e / per_unit), r = (int)(e % per_unit);
       const long long* q = slab + (size_t)(u0 + g) * per_unit + r;
       int c = sl;
