@@ -2354,6 +2354,17 @@ static SplitParams stage_params(const double* dp, const int* ip) {
   return SplitParams{dp[0], dp[1], dp[2], dp[3], ip[0], dp[4], dp[5], ip[1], ip[2], dp[6], dp[7], ip[3]};
 }
 
+// histogram.rs:42-85: the most-frequent-bin fix on a bin-indexed histogram
+// ([num_bin][2] FP64, or [num_bin] packed int64), on its own for tests.
+void orc_hist_fix(int num_bin, int mfb, double sg, double sh, double* hist) {
+  FeatureMeta m{num_bin, 0, mfb == 0 ? 1 : 0, 0u, (uint32_t)mfb, 0, 1.0};
+  hist_fix(m, sg, sh, hist);
+}
+void orc_hist_fix_int(int num_bin, int mfb, int64_t s, int64_t* hist) {
+  FeatureMeta m{num_bin, 0, mfb == 0 ? 1 : 0, 0u, (uint32_t)mfb, 0, 1.0};
+  hist_fix_int(m, s, hist);
+}
+
 // Split record: out_d = {gain, left_output, right_output, lsg, lsh, rsg, rsh},
 // out_i = {threshold, default_left, left_count, right_count, splittable}.
 // `hist` is the bin-indexed [B][2] histogram of one feature; the most

@@ -59,6 +59,8 @@ def lib():
             "orc_booster_predict_raw": (None, [P, PF, I64, I, PD]),
             "orc_hist_f64": (None, [PU8, I, PI, I, PU, I64, PF, PF, PD]),
             "orc_hist_int": (None, [PU8, I, PI, I, PU, I64, PI8, PI8, PI64]),
+            "orc_hist_fix": (None, [I, I, D, D, PD]),
+            "orc_hist_fix_int": (None, [I, I, I64, PI64]),
             "orc_find_best_threshold": (None, [PD, I, I, I, I, I, PD, PI, D, D, I, D, PD, PI, PI]),
             "orc_find_best_threshold_int": (None, [PI64, I, I, I, I, I, PD, PI, I64, D, D, I, D, PD, PI, PI64, PI]),
             "orc_binary_gradients": (None, [PD, PF, I64, D, PF, PF]),
@@ -276,6 +278,20 @@ def split_params_arrays(p):
     ip = np.array([p.get("min_data_in_leaf", 20), p.get("max_cat_to_onehot", 4), p.get("max_cat_threshold", 32),
                    p.get("min_data_per_group", 100)], np.int32)
     return dp, ip
+
+
+def hist_fix(hist_bins, mfb, sg, sh):
+    """histogram.rs:42-56 on a bin-indexed [B][2] histogram; returns the fixed copy."""
+    hb = np.ascontiguousarray(hist_bins, np.float64).reshape(-1).copy()
+    lib().orc_hist_fix(len(hb) // 2, mfb, sg, sh, hb)
+    return hb
+
+
+def hist_fix_int(hist_bins, mfb, total):
+    """histogram.rs:67-78 on a bin-indexed packed int64 histogram; returns the fixed copy."""
+    hb = np.ascontiguousarray(hist_bins, np.int64).copy()
+    lib().orc_hist_fix_int(len(hb), mfb, int(total), hb)
+    return hb
 
 
 def find_best_threshold(hist_bins, num_bin, missing_type, default_bin, mfb, bin_type, p, sg, sh, nd, parent_output=0.0):

@@ -35,6 +35,32 @@ def test_to_i8_casts_like_x86():
     assert O.lib().orc_to_i8(1e300) == 0
 
 
+def test_hist_fix_known_answers():
+    # histogram.rs:91-98: bins 0,1,1,2 with gradients 1,2,3,4 (hessian 1
+    # each) and most-frequent bin 1, which is not stored
+    got = O.hist_fix(np.array([[1.0, 1.0], [0.0, 0.0], [4.0, 1.0]]), 1, 10.0, 4.0)
+    assert got.tolist() == [1.0, 1.0, 5.0, 2.0, 4.0, 1.0]
+    # histogram.rs:100-106: most-frequent bin 0 (offset 1) leaves the histogram alone
+    got = O.hist_fix(np.array([[0.0, 0.0], [2.0, 1.0], [3.0, 1.0]]), 0, 7.0, 3.0)
+    assert got.tolist() == [0.0, 0.0, 2.0, 1.0, 3.0, 1.0]
+
+
+def _pk(g, h):
+    return ((g << 32) + h + (1 << 63)) % (1 << 64) - (1 << 63)   # packed (g << 32) | u8(h) as int64
+
+
+def test_hist_fix_int_on_the_same_rows():
+    # the packed form of the rows above: bin 1 = total - bin 0 - bin 2
+    got = O.hist_fix_int(np.array([_pk(1, 1), 0, _pk(4, 1)]), 1, _pk(10, 4))
+    assert got.tolist() == [_pk(1, 1), _pk(5, 2), _pk(4, 1)]
+    # gradients 1, -2, -3, 4: the total's gradient is 0, so bin 1 needs the
+    # wrapping subtraction to come out as (-5 << 32) + 2
+    got = O.hist_fix_int(np.array([_pk(1, 1), 0, _pk(4, 1)]), 1, _pk(0, 4))
+    assert got.tolist() == [_pk(1, 1), _pk(-5, 2), _pk(4, 1)]
+    got = O.hist_fix_int(np.array([0, _pk(2, 1), _pk(3, 1)]), 0, _pk(7, 3))
+    assert got.tolist() == [0, _pk(2, 1), _pk(3, 1)]
+
+
 def test_discretize_packing_known_answer():
     # quant.rs:175-184: bins 4, seed 0, no stochastic rounding
     gq, hq, gs, hs = O.discretize(np.array([1.0, -0.5], np.float32), np.array([0.5, 0.25], np.float32), 4, 0, False)
@@ -310,6 +336,37 @@ def test_multiclass_gradients_closed_form():
     onehot = np.eye(K)[y.astype(int)]
     np.testing.assert_allclose(g.reshape(K, n).T, P - onehot, atol=1e-6)
     np.testing.assert_allclose(h.reshape(K, n).T, K / (K - 1.0) * P * (1 - P), atol=1e-6)
+
+
+def test_gradients_are_finite_differences_of_the_losses():
+    # objective/tests.rs:18-56: g = dL/ds, h = d2L/ds2 by central differences
+    # of the loss itself (binary logloss; softmax cross-entropy, whose
+    # hessian the reference scales by K / (K - 1), multiclass.rs:72)
+    rng = np.random.default_rng(5)
+    s = rng.normal(size=200) * 2
+    y = (rng.random(200) > 0.5).astype(np.float32)
+    g, h = O.binary_gradients(s, y)
+    lab = np.where(y > 0, 1.0, -1.0)
+    loss = lambda t: np.log1p(np.exp(-lab * t))
+    e = 1e-4
+    np.testing.assert_allclose(g, (loss(s + e) - loss(s - e)) / (2 * e), rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(h, (loss(s + e) - 2 * loss(s) + loss(s - e)) / e ** 2, rtol=1e-4, atol=1e-6)
+    K, n = 3, 100
+    s = rng.normal(size=K * n)
+    y = rng.integers(0, K, size=n)
+    g, h = O.multiclass_gradients(s, y.astype(np.float32), K)
+    S = s.reshape(K, n).T
+
+    def ce(S):
+        m = S.max(1, keepdims=True)
+        return -(S[np.arange(n), y] - m[:, 0] - np.log(np.exp(S - m).sum(1)))
+    for c in range(K):
+        d = np.zeros_like(S)
+        d[:, c] = e
+        fd1 = (ce(S + d) - ce(S - d)) / (2 * e)
+        fd2 = (ce(S + d) - 2 * ce(S) + ce(S - d)) / e ** 2
+        np.testing.assert_allclose(g.reshape(K, n)[c], fd1, rtol=1e-5, atol=1e-7)
+        np.testing.assert_allclose(h.reshape(K, n)[c] * (K - 1.0) / K, fd2, rtol=1e-4, atol=1e-6)
 
 
 def test_auc_matches_sklearn():
