@@ -78,8 +78,11 @@ def run_form(lgb, ds, params, warmup, steps, quality, prof_iters, prof_all=False
     except Exception as e:  # the quality gate needs the oracle's metrics; report it missing
         log(f"quality metric unavailable: {e}")
     iters_total = warmup + 1 + steps + prof_iters
+    tun = b.get_tuning()
+    lds_peak = b.lds_atomic_peak()
     b.free()
-    return dict(iters_per_sec=its, ms_per_iter=1000.0 / its, prof=prof, auc=auc, iterations=iters_total)
+    return dict(iters_per_sec=its, ms_per_iter=1000.0 / its, prof=prof, auc=auc, iterations=iters_total,
+                hist_fb=int(tun["hist_features_per_block"]), hist_ng=int(tun["hist_groups"]), lds_peak=lds_peak)
 
 
 def cpu_baseline(rows, iters, params, full_rows):
@@ -483,6 +486,16 @@ def main():
                 traffic=traffic, traffic_per_algorithmic_byte=ratio,
                 kernel="k_hist_int" if main_form == "quant" else "k_hist_f64",
                 bytes_per_launch=m["hist_bytes_per_launch"], avg_launch_us=m["hist_avg_launch_us"])
+    # LDS-atomic throughput of the histogram kernel against the device's
+    # conflict-free ds_add_u32 rate: one update per (row, feature) is the
+    # algorithmic count; the kernel issues one per (row, feature slot of its
+    # groups), the most frequent bin's and padding slots' adds included
+    hist_s = p["hist_ms"] * 1e-3
+    lds = None
+    if hist_s > 0:
+        alg = p["hist_rows"] * ds.num_features / hist_s
+        lds = dict(achieved=alg, issued=p["hist_rows"] * m["hist_fb"] * m["hist_ng"] / hist_s, peak=m["lds_peak"],
+                   unit="lane atomics/s", frac=alg / m["lds_peak"], kernel=roof["kernel"])
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
@@ -522,6 +535,7 @@ def main():
                           hist_ms_per_iter=v["prof"]["hist_ms"] / max(v["prof"]["iters"], 1))
                   for k, v in forms.items()},
         "roofline": roof,
+        "lds_atomics": lds,
         "cpu_baseline": cpu,
     }
     if "quant" in forms and "plain" in forms:

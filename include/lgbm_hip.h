@@ -305,12 +305,20 @@ lgbmhip_status lgbmhip_bins_feature_infos(const lgbmhip_bins* b, char* out, int6
  * profiler hooks); for bench and rocprof cross-checks. */
 lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out12);
 
+/* The device's LDS-atomic peak: conflict-free ds_add_u32 (each lane of a
+ * wave adds to its own word) from three 512-thread blocks per CU, timed with
+ * HIP events on the context's stream. out: lane atomics per second. No
+ * counterpart in the reference; the denominator of the histogram kernel's
+ * LDS-atomic throughput in bench reports. */
+lgbmhip_status lgbmhip_lds_atomic_peak(lgbmhip_ctx* ctx, double* lane_atomics_per_sec);
+
 /* Launch shapes of the tree-growth kernels (0 keeps the current value):
  * partition blocks, histogram row chunks, minimum rows per histogram chunk,
  * LDS budget in bytes of one histogram block (sets features per block),
  * growth rounds in a tree's first launch and in each continuation launch.
  * Defaults: part_grid = min(tiles, 4 x CUs), hist_chunks = 2 x CUs
- * (quantized) or CUs (FP64), 4096 rows, 32 KiB,
+ * (quantized) or CUs (FP64), 4096 rows (quantized: at most one epoch of the
+ * 32-bit histogram bins, 4095 for 16 quantization bins), 32 KiB,
  * 24 and 8 rounds. Trees do not depend on any of them.
  * No counterpart in the reference (its thread count is its only knob); for
  * launch-shape sweeps on other row/feature shapes (Booster.set_tuning). */

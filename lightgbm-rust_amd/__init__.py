@@ -528,6 +528,12 @@ class Booster:
                 "part_rows", "splits_computed", "splits_used", "slab_ms")
         return dict(zip(keys, out.tolist()))
 
+    def lds_atomic_peak(self):
+        """Lane atomics per second of conflict-free ds_add_u32 on this device (lgbmhip_lds_atomic_peak)."""
+        out = np.zeros(1, np.float64)
+        check(lib().lgbmhip_lds_atomic_peak(self._ctx, ptr(out)), self._ctx)
+        return float(out[0])
+
     def set_tuning(self, part_grid=0, hist_chunks=0, hist_min_rows=0, hist_lds_budget=0, rounds_main=0,
                    rounds_cont=0):
         """Launch shapes of the tree-growth kernels (before the first update;

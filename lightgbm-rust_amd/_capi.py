@@ -121,6 +121,7 @@ SIGNATURES = {
     "lgbmhip_load_model": (I32, [P, C.c_char_p]),
     "lgbmhip_bins_feature_infos": (I32, [P, P, I64, C.POINTER(I64)]),
     "lgbmhip_profile": (I32, [P, I32, P]),
+    "lgbmhip_lds_atomic_peak": (I32, [P, P]),
     "lgbmhip_set_tuning": (I32, [P, I32, I32, I32, I32, I32, I32]),
     "lgbmhip_get_tuning": (I32, [P, P]),
 }

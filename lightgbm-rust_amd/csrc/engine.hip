@@ -2799,6 +2799,32 @@ lgbmhip_status lgbmhip_profile(lgbmhip_ctx* ctx, int32_t enable, double* out12) 
   });
 }
 
+lgbmhip_status lgbmhip_lds_atomic_peak(lgbmhip_ctx* ctx, double* lane_atomics_per_sec) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (!lane_atomics_per_sec) throw Error(LGBMHIP_INVALID_PARAMETER, "null argument");
+    set_device(c);
+    const int blocks = 3 * c->cus, iters = 4096;
+    DevBuf<unsigned> sink;
+    sink.alloc((size_t)blocks * 64);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_lds_atomic_peak, dim3(blocks), dim3(512), 0, c->stream, 64, sink.p);   // warm
+    HIP_CHECK(hipEventRecord(e0, c->stream));
+    hipLaunchKernelGGL(k_lds_atomic_peak, dim3(blocks), dim3(512), 0, c->stream, iters, sink.p);
+    HIP_CHECK(hipEventRecord(e1, c->stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipGetLastError());
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    HIP_CHECK(hipEventDestroy(e0));
+    HIP_CHECK(hipEventDestroy(e1));
+    if (!(ms > 0.0f)) throw Error(LGBMHIP_INTERNAL, "LDS-atomic timing failed");
+    *lane_atomics_per_sec = (double)blocks * 512.0 * iters * 16.0 / (ms * 1e-3);
+  });
+}
+
 lgbmhip_status lgbmhip_set_tuning(lgbmhip_ctx* ctx, int32_t part_grid, int32_t hist_chunks, int32_t hist_min_rows,
                                   int32_t hist_lds_budget, int32_t rounds_main, int32_t rounds_cont) {
   return guarded(ctx, [&] {

@@ -1782,6 +1782,28 @@ __global__ void k_clear_max(IterDev* it) {
   it->qmax_h = 0;
 }
 
+// ---------------------------------------------------------------- LDS-atomic peak
+// Conflict-free LDS atomics: lane l of each wave adds to its own word, so a
+// wave's 64 adds spread evenly over the banks. The block's words are summed
+// into sink so that no add is dead.
+__global__ void __launch_bounds__(512) k_lds_atomic_peak(int iters, unsigned* sink) {
+  __shared__ unsigned cells[512];
+  cells[threadIdx.x] = 0u;
+  __syncthreads();
+  unsigned* cell = cells + threadIdx.x;
+  const unsigned v = threadIdx.x | 1u;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) __hip_atomic_fetch_add(cell, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    unsigned s = 0;
+    for (int i = threadIdx.x; i < 512; i += 64) s += cells[i];
+    sink[(size_t)blockIdx.x * 64 + threadIdx.x] = s;
+  }
+}
+
 // ---------------------------------------------------------------- device binning
 // BinMapper::value_to_bin (binning.rs:637-667) of one value.
 __device__ __forceinline__ unsigned value_to_bin_dev(const BinFeat& bf, const double* bounds, const int* ckeys,

@@ -735,3 +735,12 @@ def test_leaf_prediction_and_rollback():
     assert b.num_trees() == 4
     imp = b.feature_importance()
     assert imp.sum() == sum(b.tree(t).num_leaves - 1 for t in range(4))
+
+
+def test_lds_atomic_peak_is_a_rate():
+    # the denominator of the histogram kernel's LDS-atomic fraction in bench
+    # reports: lane atomics per second of conflict-free ds_add_u32
+    X, y = small_binary(2000, 4)
+    _, b = _booster(X, y, P_BIN)
+    r = b.lds_atomic_peak()
+    assert np.isfinite(r) and 1e11 < r < 1e15
