@@ -2807,19 +2807,23 @@ lgbmhip_status lgbmhip_lds_atomic_peak(lgbmhip_ctx* ctx, double* lane_atomics_pe
     const int blocks = 3 * c->cus, iters = 4096;
     DevBuf<unsigned> sink;
     sink.alloc((size_t)blocks * 64);
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
+    struct Events {   // destroyed on every exit
+      hipEvent_t e[2] = {nullptr, nullptr};
+      ~Events() {
+        for (hipEvent_t x : e)
+          if (x) (void)hipEventDestroy(x);
+      }
+    } ev;
+    HIP_CHECK(hipEventCreate(&ev.e[0]));
+    HIP_CHECK(hipEventCreate(&ev.e[1]));
     hipLaunchKernelGGL(k_lds_atomic_peak, dim3(blocks), dim3(512), 0, c->stream, 64, sink.p);   // warm
-    HIP_CHECK(hipEventRecord(e0, c->stream));
+    HIP_CHECK(hipEventRecord(ev.e[0], c->stream));
     hipLaunchKernelGGL(k_lds_atomic_peak, dim3(blocks), dim3(512), 0, c->stream, iters, sink.p);
-    HIP_CHECK(hipEventRecord(e1, c->stream));
-    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipEventRecord(ev.e[1], c->stream));
+    HIP_CHECK(hipEventSynchronize(ev.e[1]));
     HIP_CHECK(hipGetLastError());
     float ms = 0.0f;
-    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    HIP_CHECK(hipEventDestroy(e0));
-    HIP_CHECK(hipEventDestroy(e1));
+    HIP_CHECK(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
     if (!(ms > 0.0f)) throw Error(LGBMHIP_INTERNAL, "LDS-atomic timing failed");
     *lane_atomics_per_sec = (double)blocks * 512.0 * iters * 16.0 / (ms * 1e-3);
   });
