@@ -772,6 +772,9 @@ static void setup_training(Ctx* c) {
     c->n_calls = 0;
   }
   plan_hist(c, quant);
+  // histogram kernels address a chunk's rows with 32-bit byte offsets
+  if (((N + c->hist_chunks - 1) / c->hist_chunks) * (int64_t)c->hist_fb >= ((int64_t)1 << 31))
+    throw Error(LGBMHIP_UNSUPPORTED, "histogram chunks too long for 32-bit row offsets: raise hist_chunks");
   {
     std::vector<int> au(kMaxBatch + 1);
     for (int w = 0; w <= kMaxBatch; ++w) au[w] = w * c->hist_ng;
