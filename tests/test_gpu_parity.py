@@ -432,6 +432,17 @@ def test_growth_rounds_per_launch_do_not_change_trees():
     _same_trees_as_oracle(dict(params, use_graph=False), trees=1, tuning=dict(rounds_main=2, rounds_cont=3))
 
 
+# Launch shapes of the quantized histogram: one 28-feature group at 8 lanes
+# per row (a 64 KiB LDS budget), leaves cut into few long chunks that fold
+# their 32-bit bins into int64 sums several times (8 chunks per leaf), and
+# short chunks (1024 rows). Trees must not depend on any of them.
+@pytest.mark.parametrize("tuning", [dict(hist_lds_budget=65536), dict(hist_chunks=8), dict(hist_min_rows=1024)])
+def test_quantized_histogram_launch_shapes_do_not_change_trees(tuning):
+    params = dict(P_BIN, num_leaves=63, max_bin=255, min_data_in_leaf=20, use_quantized_grad=True,
+                  num_grad_quant_bins=16, quant_train_renew_leaf=True)
+    _same_trees_as_oracle(params, trees=2, tuning=tuning)
+
+
 def test_batched_growth_with_max_depth_and_min_data():
     # children that may not be searched (depth, min_data) inside batched rounds
     params = dict(P_BIN, num_leaves=255, max_bin=63, max_depth=6, min_data_in_leaf=300)
