@@ -39,6 +39,7 @@
  *   lgbmhip_rollback_one_iter <- Gbdt::rollback_one_iter        boosting.rs:1215-1239
  *   lgbmhip_save_model        <- GBDT::SaveModelToString        io.rs:13-88, tree.rs:718-770
  *   lgbmhip_load_model        <- GBDT::LoadModelFromString      io.rs:156-343
+ *   lgbmhip_check_model       <- LoadModelFromString / Tree::from_model_str checks io.rs:156-343, tree.rs:869-1047
  *   lgbmhip_bins_feature_infos <- Dataset::feature_infos        dataset.rs:770-776
  */
 #ifndef LGBM_HIP_H_
@@ -275,20 +276,33 @@ lgbmhip_status lgbmhip_rollback_one_iter(lgbmhip_ctx* ctx);
 lgbmhip_status lgbmhip_set_predict_window(lgbmhip_ctx* ctx, int32_t start_iteration, int32_t num_iteration,
                                           int32_t early_stop_freq, double early_stop_margin);
 
-/* Model text of the trained trees (GBDT::SaveModelToString, io.rs:13-88):
- * header, tree_sizes, every tree's block (tree.rs:718-770, numbers as C's
- * %.17g / %g), "end of trees" and the split-count feature importances. The
- * parameters section (the whole config) is not written. feature_names:
- * space-separated, one per column (NULL: Column_0 ...); feature_infos: as
- * lgbmhip_bins_feature_infos gives them. num_iteration <= 0: every tree.
- * *out_len receives the text's length; out (capacity bytes) receives the
- * NUL-terminated text when it fits (out = NULL only sizes it). */
+/* Model text (GBDT::SaveModelToString, io.rs:13-88): header, tree_sizes,
+ * every tree's block (tree.rs:718-770, numbers as C's %.17g / %g), "end of
+ * trees", the split-count feature importances and, when there is one, the
+ * parameters block. feature_names (space-separated, one per column),
+ * feature_infos (as lgbmhip_bins_feature_infos gives them) and parameters
+ * (the host config's `[name: value]` lines, config.rs:1169-1187) may be
+ * NULL: a model loaded from text then keeps its own, a trained one writes
+ * Column_0 ..., "none" per column and no parameters block. num_iteration
+ * <= 0: every tree. *out_len receives the text's length; out (capacity
+ * bytes) receives the NUL-terminated text when it fits (out = NULL only
+ * sizes it). */
 lgbmhip_status lgbmhip_save_model(lgbmhip_ctx* ctx, int32_t num_iteration, const char* feature_names,
-                                  const char* feature_infos, char* out, int64_t capacity, int64_t* out_len);
+                                  const char* feature_infos, const char* parameters, char* out, int64_t capacity,
+                                  int64_t* out_len);
 /* Trees of a model text (GBDT::LoadModelFromString, io.rs:156-343) into an
  * empty context, for raw prediction (lgbmhip_predict_raw / _device with X);
- * linear trees are Unsupported. */
+ * linear trees, forests (average_output) and objectives other than binary,
+ * multiclass and lambdarank are Unsupported. */
 lgbmhip_status lgbmhip_load_model(lgbmhip_ctx* ctx, const char* text);
+/* The parse and checks of lgbmhip_load_model without a context or device
+ * (io.rs:156-343, tree.rs:869-1047): field value counts, numbers as
+ * parse_f64 / atof_legacy read them (fmt.rs:57-150), child ranges, each node
+ * referenced once, categorical indices and bitset boundaries. Out pointers
+ * may be NULL; out / out_len receive the text lgbmhip_save_model would write
+ * for the parsed model (as lgbmhip_save_model's out / out_len). */
+lgbmhip_status lgbmhip_check_model(const char* text, int32_t* out_num_trees, int32_t* out_num_tree_per_iteration,
+                                   char* out, int64_t capacity, int64_t* out_len);
 /* Dataset::feature_infos (dataset.rs:770-776): per raw column "[min:max]",
  * the categories of a categorical column joined by ':', or "none". */
 lgbmhip_status lgbmhip_bins_feature_infos(const lgbmhip_bins* b, char* out, int64_t capacity, int64_t* out_len);

@@ -296,14 +296,17 @@ class Booster:
         self.best_iteration = 0
         self.best_score = {}
 
-    def model_to_string(self, num_iteration=None):
-        """Model text (GBDT::SaveModelToString) without the parameters section."""
+    def model_to_string(self, num_iteration=None, parameters=None):
+        """Model text (GBDT::SaveModelToString). `parameters`: the parameters
+        block's `[name: value]` lines (the config dump the host owns); a model
+        loaded from text keeps its own block."""
         ni = -1 if num_iteration is None else int(num_iteration)
-        infos = self.train_set.feature_infos().encode() if self.train_set is not None else b""
+        infos = self.train_set.feature_infos().encode() if self.train_set is not None else None
+        par = parameters.encode() if parameters is not None else None
         n = C.c_int64(0)
-        check(lib().lgbmhip_save_model(self._ctx, ni, None, infos, None, 0, C.byref(n)), self._ctx)
+        check(lib().lgbmhip_save_model(self._ctx, ni, None, infos, par, None, 0, C.byref(n)), self._ctx)
         buf = C.create_string_buffer(n.value + 1)
-        check(lib().lgbmhip_save_model(self._ctx, ni, None, infos, buf, n.value + 1, C.byref(n)), self._ctx)
+        check(lib().lgbmhip_save_model(self._ctx, ni, None, infos, par, buf, n.value + 1, C.byref(n)), self._ctx)
         return buf.value.decode()
 
     def save_model(self, filename, num_iteration=None):

@@ -117,8 +117,9 @@ SIGNATURES = {
     "lgbmhip_set_predict_window": (I32, [P, I32, I32, I32, D]),
     "lgbmhip_predict_leaf": (I32, [P, P, I64, I32, P, C.POINTER(I32)]),
     "lgbmhip_rollback_one_iter": (I32, [P]),
-    "lgbmhip_save_model": (I32, [P, I32, C.c_char_p, C.c_char_p, P, I64, C.POINTER(I64)]),
+    "lgbmhip_save_model": (I32, [P, I32, C.c_char_p, C.c_char_p, C.c_char_p, P, I64, C.POINTER(I64)]),
     "lgbmhip_load_model": (I32, [P, C.c_char_p]),
+    "lgbmhip_check_model": (I32, [C.c_char_p, P, P, P, I64, C.POINTER(I64)]),
     "lgbmhip_bins_feature_infos": (I32, [P, P, I64, C.POINTER(I64)]),
     "lgbmhip_profile": (I32, [P, I32, P]),
     "lgbmhip_lds_atomic_peak": (I32, [P, P]),

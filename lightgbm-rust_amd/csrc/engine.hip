@@ -8,17 +8,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "engine.h"
+#include "textnum.h"
 #include "metrics.h"
 #include "kernels.hip"
 #include "grow.hip"
@@ -148,6 +151,16 @@ struct HostTree {
     leaf_value[0] = v;
     leaf_count[0] = count;
   }
+};
+
+// The header of a model text (io.rs:19-37) and its parameters block.
+struct ModelHeader {
+  int num_class = 1, ntpi = 1, label_index = 0, ncol = 0;
+  bool has_objective = true;                    // false: a custom objective's model (no objective line)
+  std::string objective;                        // as written: "binary sigmoid:1", "multiclass num_class:3", ...
+  std::string feature_names, feature_infos;     // space-separated, one per column
+  bool has_parameters = false;
+  std::string parameters;                       // the block's lines, each ending in '\n'
 };
 
 template <class T>
@@ -320,6 +333,7 @@ struct Ctx {
   MetricWs* mws = nullptr;
   DevBuf<int> d_qb_eval;   // training query boundaries for NDCG when the objective keeps none
   bool loaded_model = false;   // trees from model text: raw prediction only
+  ModelHeader loaded_head;     // the loaded text's header and parameters block
   // prediction window and early stopping (predict.rs:178-202): iterations
   // start .. start + num (num <= 0: to the end), margin test every freq (0: off)
   int pred_start = 0, pred_num = 0, pred_es_freq = 0;
@@ -562,6 +576,10 @@ static void plan_hist(Ctx* c, bool quant) {
   // at most 128 bytes (32 threads of 4 bins) of a row per block
   int fb = (int)std::min<size_t>(std::min<size_t>((size_t)c->F, 128), budget / per_feat);
   fb = std::max(4, fb / 4 * 4);
+  // the quantized form also holds 32-bit bins and trash words beside its
+  // int64 sums (about 12 bytes per bin): narrow the group until that fits
+  if (quant)
+    while (fb > 4 && q32_lds_bytes(std::min(fb, c->F) * c->B) > std::min<size_t>(c->lds_max, 160 * 1024)) fb -= 4;
   if (fb >= c->F) fb = (c->F + 3) / 4 * 4;
   int ng = (c->F + fb - 1) / fb;
   // balance the groups
@@ -772,9 +790,15 @@ static void setup_training(Ctx* c) {
     c->n_calls = 0;
   }
   plan_hist(c, quant);
-  // histogram kernels address a chunk's rows with 32-bit byte offsets
-  if (((N + c->hist_chunks - 1) / c->hist_chunks) * (int64_t)c->hist_fb >= ((int64_t)1 << 31))
-    throw Error(LGBMHIP_UNSUPPORTED, "histogram chunks too long for 32-bit row offsets: raise hist_chunks");
+  // histogram kernels address a chunk's rows with 32-bit byte offsets. A leaf
+  // of cnt rows takes min(hist_chunks, ceil(cnt / hist_min_rows)) chunks, so
+  // no chunk is longer than max(hist_min_rows, ceil(N / hist_chunks)) rows.
+  {
+    const int64_t chunk_max = std::max<int64_t>(c->hist_min_rows, (N + c->hist_chunks - 1) / c->hist_chunks);
+    if (chunk_max * (int64_t)c->hist_fb >= ((int64_t)1 << 31))
+      throw Error(LGBMHIP_UNSUPPORTED,
+                  "histogram chunks too long for 32-bit row offsets: raise hist_chunks or lower hist_min_rows");
+  }
   {
     std::vector<int> au(kMaxBatch + 1);
     for (int w = 0; w <= kMaxBatch; ++w) au[w] = w * c->hist_ng;
@@ -1637,8 +1661,9 @@ static void launch_predict(Ctx* c, const float* dX, const uint8_t* dbins, int64_
   a.stride = binned ? sw * 4 : sw;
   const size_t lds = (size_t)kPredRows * sw * 4;
   const dim3 grid((unsigned)((rows + kPredRows - 1) / kPredRows));
-  // binary, multiclass and ranking objectives all allow it (predict.rs:174-190)
-  if (c->pred_es_freq > 0) {
+  // prediction_early_stop (predict.rs:175-190): binary, multiclass and ranking
+  // objectives allow it; a model without an objective never stops early
+  if (c->pred_es_freq > 0 && !(c->loaded_model && !c->loaded_head.has_objective)) {
     if (K > 16) throw Error(LGBMHIP_UNSUPPORTED, "early-stopped prediction of more than 16 classes");
     if (lds > (size_t)c->lds_max) throw Error(LGBMHIP_UNSUPPORTED, "rows too wide for early-stopped prediction");
     auto kern = binned ? k_predict_es<true> : k_predict_es<false>;
@@ -1891,6 +1916,7 @@ static std::string tree_text(const HostTree& t) {
   s += "left_child=" + join_fmt(t.left_child, ni, i2s) + "\n";
   s += "right_child=" + join_fmt(t.right_child, ni, i2s) + "\n";
   s += "leaf_value=" + join_fmt(t.leaf_value, n, g17) + "\n";
+  // at most the stored weights: a loaded one-leaf tree stores none (tree.rs:734-736)
   s += "leaf_weight=" + join_fmt(t.leaf_weight, n, g17) + "\n";
   s += "leaf_count=" + join_fmt(t.leaf_count, n, i2s) + "\n";
   s += "internal_value=" + join_fmt(t.internal_value, ni, g6) + "\n";
@@ -1905,170 +1931,293 @@ static std::string tree_text(const HostTree& t) {
   return s;
 }
 
-// GBDT::SaveModelToString (io.rs:13-88) up to the feature importances; the
-// parameters section (the whole config, config.rs:1169) is not written.
-static std::string save_model(Ctx* c, int num_iteration, const char* names_in, const char* infos_in) {
-  if (c->trees.empty()) throw Error(LGBMHIP_INVALID_PARAMETER, "no trees to save");
-  const int K = c->K, ncol = c->ncol;
+// GBDT::SaveModelToString (io.rs:13-88): header, tree blocks with their byte
+// sizes, split-count importances (boosting.rs:1337-1360), parameters block.
+static std::string model_text(const ModelHeader& h, const std::vector<HostTree>& trees, int num_iteration) {
+  if (trees.empty()) throw Error(LGBMHIP_INVALID_PARAMETER, "no trees to save");
   std::vector<std::string> names;
-  if (names_in && *names_in) {
+  {
     std::string cur;
-    for (const char* p = names_in;; ++p) {
+    for (const char* p = h.feature_names.c_str();; ++p) {
       if (*p == ' ' || *p == 0) {
-        if (!cur.empty()) names.push_back(cur);
+        names.push_back(cur);
         cur.clear();
         if (*p == 0) break;
       } else {
         cur += *p;
       }
     }
-  } else {
-    for (int i = 0; i < ncol; ++i) names.push_back("Column_" + std::to_string(i));
   }
-  if ((int)names.size() != ncol) throw Error(LGBMHIP_INVALID_PARAMETER, "one feature name per column");
+  if ((int)names.size() != h.ncol) throw Error(LGBMHIP_INVALID_PARAMETER, "one feature name per column");
   std::string s = "tree\nversion=v4\n";
-  s += "num_class=" + std::to_string(c->opts.objective == 1 ? c->opts.num_class : 1) + "\n";
-  s += "num_tree_per_iteration=" + std::to_string(K) + "\n";
-  s += "label_index=0\n";
-  s += "max_feature_idx=" + std::to_string(ncol - 1) + "\n";
-  // objective to_model_string: binary.rs:168-170, multiclass.rs:169-171, rank.rs:338-340
-  if (c->opts.objective == 0) s += "objective=binary sigmoid:" + fmt_g(c->opts.sigmoid, 6) + "\n";
-  else if (c->opts.objective == 1) s += "objective=multiclass num_class:" + std::to_string(c->opts.num_class) + "\n";
-  else s += "objective=lambdarank\n";
-  std::string nj;
-  for (size_t i = 0; i < names.size(); ++i) nj += (i ? " " : "") + names[i];
-  s += "feature_names=" + nj + "\n";
-  s += "feature_infos=" + std::string(infos_in ? infos_in : "") + "\n";
-  size_t used = c->trees.size();
-  if (num_iteration > 0) used = std::min(used, (size_t)num_iteration * K);
+  s += "num_class=" + std::to_string(h.num_class) + "\n";
+  s += "num_tree_per_iteration=" + std::to_string(h.ntpi) + "\n";
+  s += "label_index=" + std::to_string(h.label_index) + "\n";
+  s += "max_feature_idx=" + std::to_string(h.ncol - 1) + "\n";
+  if (h.has_objective) s += "objective=" + h.objective + "\n";
+  s += "feature_names=" + h.feature_names + "\n";
+  s += "feature_infos=" + h.feature_infos + "\n";
+  size_t used = trees.size();
+  if (num_iteration > 0) used = std::min(used, (size_t)num_iteration * h.ntpi);
   std::vector<std::string> ts;
   std::string sizes;
   for (size_t i = 0; i < used; ++i) {
-    ts.push_back("Tree=" + std::to_string(i) + "\n" + tree_text(c->trees[i]) + "\n");
+    ts.push_back("Tree=" + std::to_string(i) + "\n" + tree_text(trees[i]) + "\n");
     sizes += (i ? " " : "") + std::to_string(ts.back().size());
   }
   s += "tree_sizes=" + sizes + "\n\n";
   for (auto& t : ts) s += t;
   s += "end of trees\n";
-  // split-count importances (boosting.rs:1337-1360), stable by count descending
-  std::vector<double> imp(ncol, 0.0);
+  std::vector<double> imp(h.ncol, 0.0);
   for (size_t i = 0; i < used; ++i) {
-    const HostTree& t = c->trees[i];
+    const HostTree& t = trees[i];
     for (int k = 0; k + 1 < t.num_leaves; ++k)
-      if (t.split_gain[k] > 0.0f && t.split_feature[k] >= 0 && t.split_feature[k] < ncol) imp[t.split_feature[k]] += 1.0;
+      if (t.split_gain[k] > 0.0f && t.split_feature[k] >= 0 && t.split_feature[k] < h.ncol) imp[t.split_feature[k]] += 1.0;
   }
   std::vector<std::pair<size_t, int>> pairs;
-  for (int f = 0; f < ncol; ++f)
+  for (int f = 0; f < h.ncol; ++f)
     if ((size_t)imp[f] > 0) pairs.emplace_back((size_t)imp[f], f);
   std::stable_sort(pairs.begin(), pairs.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
   s += "\nfeature_importances:\n";
   for (auto& pr : pairs) s += names[pr.second] + "=" + std::to_string(pr.first) + "\n";
+  if (h.has_parameters) s += "\nparameters:\n" + h.parameters + "\nend of parameters\n";
   return s;
 }
 
-// GBDT::LoadModelFromString (io.rs:156-343) for raw prediction: the header's
-// class and column counts and each tree's arrays (numbers parsed correctly
-// rounded, as parse_f64 does).
-static void load_model(Ctx* c, const char* text) {
-  if (!text) throw Error(LGBMHIP_INVALID_PARAMETER, "no model text");
-  std::vector<std::pair<std::string, std::string>> lines;
-  {
-    const char* p = text;
-    while (*p) {
-      const char* e = std::strchr(p, '\n');
-      std::string ln = e ? std::string(p, e - p) : std::string(p);
-      if (!ln.empty() && ln.back() == '\r') ln.pop_back();
-      const size_t eq = ln.find('=');
-      if (eq != std::string::npos) lines.emplace_back(ln.substr(0, eq), ln.substr(eq + 1));
-      else lines.emplace_back(ln, "");
-      if (!e) break;
-      p = e + 1;
-    }
-  }
-  auto nums = [](const std::string& v) {
-    std::vector<double> out;
-    const char* p = v.c_str();
-    while (*p) {
-      while (*p == ' ') ++p;
-      if (!*p) break;
-      char* e;
-      const double d = std::strtod(p, &e);
-      if (e == p) throw Error(LGBMHIP_MODEL_FORMAT, "bad number in model text");
-      out.push_back(d);
-      p = e;
-    }
-    return out;
-  };
-  int ntpi = -1, max_feat = -1, num_class = 1;
-  std::string objective;
+// A model text's trees and header.
+struct ParsedModel {
   std::vector<HostTree> trees;
-  size_t i = 0;
-  for (; i < lines.size() && lines[i].first != "Tree"; ++i) {
-    const auto& kv = lines[i];
-    if (kv.first == "num_tree_per_iteration") ntpi = std::atoi(kv.second.c_str());
-    else if (kv.first == "max_feature_idx") max_feat = std::atoi(kv.second.c_str());
-    else if (kv.first == "num_class") num_class = std::atoi(kv.second.c_str());
-    else if (kv.first == "objective") objective = kv.second;
+  ModelHeader head;
+  int objective = 0;   // 0 binary, 1 multiclass, 2 lambdarank
+};
+
+// GBDT::LoadModelFromString (io.rs:156-343) and Tree::from_model_str
+// (tree.rs:869-1047): header fields, every tree field the writer writes
+// (exactly the expected count of values each), the parameters block; the
+// checks the reference makes, plus a tree-shape check (each node but the
+// root, and each leaf, referenced exactly once). No device work.
+static ParsedModel parse_model(const char* text) {
+  if (!text) throw Error(LGBMHIP_INVALID_PARAMETER, "no model text");
+  std::vector<std::string> lines;
+  for (const char* p = text;;) {
+    const char* e = std::strchr(p, '\n');
+    std::string ln = e ? std::string(p, e - p) : std::string(p);
+    if (!ln.empty() && ln.back() == '\r') ln.pop_back();
+    lines.push_back(std::move(ln));
+    if (!e) break;
+    p = e + 1;
   }
-  if (ntpi < 1 || max_feat < 0) throw Error(LGBMHIP_MODEL_FORMAT, "model text without num_tree_per_iteration / max_feature_idx");
-  while (i < lines.size() && lines[i].first == "Tree") {
+  auto key_of = [](const std::string& ln) { return ln.substr(0, ln.find('=')); };
+  auto val_of = [](const std::string& ln) {
+    const size_t eq = ln.find('=');
+    return eq == std::string::npos ? std::string() : ln.substr(eq + 1);
+  };
+  std::unordered_map<std::string, std::string> hk;
+  size_t i = 0;
+  for (; i < lines.size() && lines[i].rfind("Tree=", 0) != 0; ++i) {
+    if (lines[i].empty()) continue;
+    if (lines[i] == "end of trees") break;
+    hk[key_of(lines[i])] = val_of(lines[i]);
+  }
+  auto head_int = [&](const char* k, long long lo, long long hi) {
+    long long v;
+    if (!hk.count(k)) throw Error(LGBMHIP_MODEL_FORMAT, std::string("Model file doesn't specify ") + k);
+    if (!parse_int(hk[k], lo, hi, &v)) throw Error(LGBMHIP_MODEL_FORMAT, std::string("bad ") + k);
+    return (int)v;
+  };
+  ParsedModel m;
+  ModelHeader& h = m.head;
+  h.num_class = head_int("num_class", 1, 1 << 20);
+  h.ntpi = hk.count("num_tree_per_iteration") ? head_int("num_tree_per_iteration", 1, 1 << 20) : h.num_class;
+  h.label_index = head_int("label_index", -(1LL << 31), (1LL << 31) - 1);
+  h.ncol = head_int("max_feature_idx", 0, (1LL << 31) - 2) + 1;
+  if (!hk.count("feature_names")) throw Error(LGBMHIP_MODEL_FORMAT, "Model file doesn't contain feature_names");
+  if (!hk.count("feature_infos")) throw Error(LGBMHIP_MODEL_FORMAT, "Model file doesn't contain feature_infos");
+  h.feature_names = hk["feature_names"];
+  h.feature_infos = hk["feature_infos"];
+  auto count_fields = [](const std::string& s) { return (int)std::count(s.begin(), s.end(), ' ') + 1; };
+  if (count_fields(h.feature_names) != h.ncol) throw Error(LGBMHIP_MODEL_FORMAT, "Wrong size of feature_names");
+  if (count_fields(h.feature_infos) != h.ncol) throw Error(LGBMHIP_MODEL_FORMAT, "Wrong size of feature_infos");
+  h.has_objective = hk.count("objective") > 0;
+  h.objective = h.has_objective ? hk["objective"] : std::string();
+  // The prediction here is the raw sum of binary, multiclass and lambdarank
+  // models, with their early-stopping margins (predict.rs:152-190), or of a
+  // model without an objective (a custom one), which never stops early.
+  // Forests (average_output) and other objectives are not served.
+  if (hk.count("average_output")) throw Error(LGBMHIP_UNSUPPORTED, "models with average_output (random forest)");
+  const std::string& obj = h.objective;
+  const bool obj_binary = obj.rfind("binary", 0) == 0, obj_multi = obj.rfind("multiclass", 0) == 0,
+             obj_rank = obj.rfind("lambdarank", 0) == 0;
+  if (h.has_objective && !obj_binary && !obj_multi && !obj_rank)
+    throw Error(LGBMHIP_UNSUPPORTED, "model objective '" + obj + "' (binary, multiclass and lambdarank are served)");
+  m.objective = obj_multi ? 1 : obj_rank ? 2 : h.has_objective ? 0 : h.ntpi > 1 ? 1 : 0;
+
+  while (i < lines.size() && lines[i].rfind("Tree=", 0) == 0) {
     HostTree t;
     std::unordered_map<std::string, std::string> kv;
-    for (++i; i < lines.size() && !lines[i].first.empty() && lines[i].first != "Tree" &&
-              lines[i].first != "end of trees";
-         ++i)
-      kv[lines[i].first] = lines[i].second;
-    while (i < lines.size() && lines[i].first.empty()) ++i;
-    if (!kv.count("num_leaves")) throw Error(LGBMHIP_MODEL_FORMAT, "tree without num_leaves");
-    if (kv.count("is_linear") && std::atoi(kv["is_linear"].c_str()) != 0)
-      throw Error(LGBMHIP_UNSUPPORTED, "linear trees");
-    t.num_leaves = std::atoi(kv["num_leaves"].c_str());
-    t.num_cat = kv.count("num_cat") ? std::atoi(kv["num_cat"].c_str()) : 0;
-    const int n = t.num_leaves, ni = n - 1;
-    auto take = [&](const char* key, size_t want) {
-      std::vector<double> v = kv.count(key) ? nums(kv[key]) : std::vector<double>();
-      if (v.size() < want) throw Error(LGBMHIP_MODEL_FORMAT, std::string("tree field ") + key + " is short");
+    for (++i; i < lines.size() && !lines[i].empty() && lines[i].rfind("Tree=", 0) != 0 && lines[i] != "end of trees";
+         ++i) {
+      if (lines[i].find('=') == std::string::npos) throw Error(LGBMHIP_MODEL_FORMAT, "bad tree line: " + lines[i]);
+      kv[key_of(lines[i])] = val_of(lines[i]);
+    }
+    while (i < lines.size() && lines[i].empty()) ++i;
+    auto need = [&](const char* k) -> const std::string& {
+      if (!kv.count(k))
+        throw Error(LGBMHIP_MODEL_FORMAT, std::string("Tree model string format error, should contain ") + k + " field");
+      return kv[k];
+    };
+    auto ints = [&](const std::string& s, size_t n, const char* name, long long lo, long long hi) {
+      const auto toks = split_ws(s);
+      if (toks.size() != n) throw Error(LGBMHIP_MODEL_FORMAT, std::string(name) + " has a wrong number of values");
+      std::vector<long long> v(n);
+      for (size_t k = 0; k < n; ++k)
+        if (!parse_int(toks[k], lo, hi, &v[k])) throw Error(LGBMHIP_MODEL_FORMAT, std::string("bad value in ") + name);
       return v;
     };
-    if (n < 1) throw Error(LGBMHIP_MODEL_FORMAT, "num_leaves < 1");
-    const auto lv = take("leaf_value", n);
-    t.leaf_value.assign(lv.begin(), lv.begin() + n);
+    auto reals = [&](const std::string& s, size_t n, const char* name, bool legacy) {
+      const auto toks = split_ws(s);
+      if (toks.size() != n) throw Error(LGBMHIP_MODEL_FORMAT, std::string(name) + " has a wrong number of values");
+      std::vector<double> v(n);
+      for (size_t k = 0; k < n; ++k)
+        if (!(legacy ? atof_legacy(toks[k], &v[k]) : parse_f64(toks[k], &v[k])))
+          throw Error(LGBMHIP_MODEL_FORMAT, std::string("bad value in ") + name);
+      return v;
+    };
+    const long long I32MAX = (1LL << 31) - 1, I32MIN = -(1LL << 31);
+    const int n = (int)ints(need("num_leaves"), 1, "num_leaves", 1, 1 << 24)[0];
+    t.num_cat = (int)ints(need("num_cat"), 1, "num_cat", 0, I32MAX)[0];
+    if (kv.count("is_linear") && kv["is_linear"] != "0") throw Error(LGBMHIP_UNSUPPORTED, "linear trees");
+    const int ni = n - 1;
+    t.num_leaves = n;
+    t.leaf_value = reals(need("leaf_value"), n, "leaf_value", false);
+    t.shrinkage = 1.0;
+    if (kv.count("shrinkage")) {
+      double sh;
+      if (atof_legacy(kv["shrinkage"], &sh)) t.shrinkage = sh;
+    }
     t.leaf_count.assign(n, 0);
+    if (kv.count("leaf_count")) {
+      const auto lc = ints(kv["leaf_count"], n, "leaf_count", I32MIN, I32MAX);
+      for (int k = 0; k < n; ++k) t.leaf_count[k] = (int)lc[k];
+    }
+    t.leaf_parent.assign(n, -1);
+    t.leaf_depth.assign(n, 0);
+    t.leaf_weight.clear();   // a one-leaf tree stores no weights (tree.rs:962-963)
     if (ni > 0) {
-      const auto sf = take("split_feature", ni), th = take("threshold", ni), dt = take("decision_type", ni),
-                 lc = take("left_child", ni), rc = take("right_child", ni);
+      t.leaf_weight = kv.count("leaf_weight") ? reals(kv["leaf_weight"], n, "leaf_weight", false)
+                                              : std::vector<double>(n, 0.0);
+      const auto lc = ints(need("left_child"), ni, "left_child", I32MIN, I32MAX);
+      const auto rc = ints(need("right_child"), ni, "right_child", I32MIN, I32MAX);
+      const auto sf = ints(need("split_feature"), ni, "split_feature", I32MIN, I32MAX);
+      const auto th = reals(need("threshold"), ni, "threshold", false);
+      const auto sg = kv.count("split_gain") ? reals(kv["split_gain"], ni, "split_gain", true) : std::vector<double>(ni, 0.0);
+      const auto ic = kv.count("internal_count") ? ints(kv["internal_count"], ni, "internal_count", I32MIN, I32MAX)
+                                                 : std::vector<long long>(ni, 0);
+      t.internal_value = kv.count("internal_value") ? reals(kv["internal_value"], ni, "internal_value", true)
+                                                    : std::vector<double>(ni, 0.0);
+      t.internal_weight = kv.count("internal_weight") ? reals(kv["internal_weight"], ni, "internal_weight", true)
+                                                      : std::vector<double>(ni, 0.0);
+      const auto dt = kv.count("decision_type") ? ints(kv["decision_type"], ni, "decision_type", -128, 127)
+                                                : std::vector<long long>(ni, 0);
+      if (t.num_cat > 0) {
+        const size_t nb = (size_t)t.num_cat + 1;
+        if (!kv.count("cat_boundaries")) throw Error(LGBMHIP_MODEL_FORMAT, "Tree model should contain cat_boundaries field.");
+        const auto cb = ints(kv["cat_boundaries"], nb, "cat_boundaries", I32MIN, I32MAX);
+        // tree.rs:993-1009: boundaries ascend from 0 and end at the bitset words' count
+        if (cb[0] != 0) throw Error(LGBMHIP_MODEL_FORMAT, "cat_boundaries must be non-decreasing from 0");
+        for (size_t k = 1; k < nb; ++k)
+          if (cb[k] < cb[k - 1]) throw Error(LGBMHIP_MODEL_FORMAT, "cat_boundaries must be non-decreasing from 0");
+        if (!kv.count("cat_threshold")) throw Error(LGBMHIP_MODEL_FORMAT, "Tree model should contain cat_threshold field");
+        const auto ct = ints(kv["cat_threshold"], (size_t)cb[nb - 1], "cat_threshold", 0, 4294967295LL);
+        t.cat_boundaries.assign(cb.begin(), cb.end());
+        t.cat_threshold.assign(ct.begin(), ct.end());
+      }
       for (int k = 0; k < ni; ++k) {
+        // a categorical node's threshold is its bitset's index (tree.rs:400-403, 1011-1018);
+        // prediction reads that index from threshold_in_bin
+        const bool cat = (dt[k] & 1) != 0;
+        if (cat && !(th[k] >= 0.0 && (int)th[k] < t.num_cat))
+          throw Error(LGBMHIP_MODEL_FORMAT, "categorical split index out of range");
+        for (long long ch : {lc[k], rc[k]})
+          if (ch >= 0 ? ch >= ni : ~ch >= n) throw Error(LGBMHIP_MODEL_FORMAT, "child index out of range");
+        if (sf[k] < 0 || sf[k] >= h.ncol) throw Error(LGBMHIP_MODEL_FORMAT, "split_feature out of range");
         t.split_feature.push_back((int)sf[k]);
         t.split_feature_inner.push_back((int)sf[k]);
         t.threshold.push_back(th[k]);
-        t.threshold_in_bin.push_back(0);
-        t.decision_type.push_back((int8_t)(int)dt[k]);
+        t.threshold_in_bin.push_back(cat ? (uint32_t)th[k] : 0u);
+        t.decision_type.push_back((int8_t)dt[k]);
         t.left_child.push_back((int)lc[k]);
         t.right_child.push_back((int)rc[k]);
-        t.split_gain.push_back(0.0f);
-        if ((int)sf[k] > max_feat) throw Error(LGBMHIP_MODEL_FORMAT, "split feature beyond max_feature_idx");
-        if (lc[k] >= ni || rc[k] >= ni || lc[k] < -n || rc[k] < -n) throw Error(LGBMHIP_MODEL_FORMAT, "child out of range");
+        t.split_gain.push_back((float)sg[k]);
+        t.internal_count.push_back((int)ic[k]);
       }
+      // Every internal node but the root, and every leaf, is some node's child
+      // exactly once. A walk from the root then meets each node at most once,
+      // so the device's node-chain loops end within num_leaves - 1 steps.
+      std::vector<int> refs((size_t)(ni + n), 0);
+      for (int k = 0; k < ni; ++k)
+        for (int ch : {t.left_child[k], t.right_child[k]}) ++refs[ch >= 0 ? (size_t)ch : (size_t)(ni + ~ch)];
+      if (refs[0] != 0) throw Error(LGBMHIP_MODEL_FORMAT, "the root is a child");
+      for (size_t r = 1; r < refs.size(); ++r)
+        if (refs[r] != 1) throw Error(LGBMHIP_MODEL_FORMAT, "tree node referenced other than once");
     }
-    if (t.num_cat > 0) {
-      const auto cb = take("cat_boundaries", (size_t)t.num_cat + 1);
-      t.cat_boundaries.assign(cb.begin(), cb.begin() + t.num_cat + 1);
-      const auto ct = nums(kv["cat_threshold"]);
-      for (double x : ct) t.cat_threshold.push_back((uint32_t)(long long)x);
-      if ((size_t)t.cat_boundaries.back() != t.cat_threshold.size()) throw Error(LGBMHIP_MODEL_FORMAT, "cat_threshold size");
-    }
-    trees.push_back(std::move(t));
+    m.trees.push_back(std::move(t));
   }
-  if (trees.empty()) throw Error(LGBMHIP_MODEL_FORMAT, "model text without trees");
-  c->trees = std::move(trees);
-  c->K = ntpi;
-  c->ncol = max_feat + 1;
-  c->opts.objective = objective.rfind("multiclass", 0) == 0 ? 1 : objective.rfind("lambdarank", 0) == 0 ? 2 : 0;
-  c->opts.num_class = num_class;
+  if (m.trees.empty()) throw Error(LGBMHIP_MODEL_FORMAT, "model text without trees");
+  // the parameters block after the trees (io.rs:272-290): its lines, blank lines skipped
+  for (; i < lines.size(); ++i) {
+    if (lines[i] == "parameters:") {
+      h.has_parameters = true;
+      for (++i; i < lines.size() && lines[i] != "end of parameters"; ++i)
+        if (!lines[i].empty()) h.parameters += lines[i] + "\n";
+      break;
+    }
+  }
+  return m;
+}
+
+static void load_model(Ctx* c, const char* text) {
+  ParsedModel m = parse_model(text);
+  c->trees = std::move(m.trees);
+  c->K = m.head.ntpi;
+  c->ncol = m.head.ncol;
+  c->opts.objective = m.objective;
+  c->opts.num_class = m.head.num_class;
+  c->loaded_head = m.head;
   c->opts_set = true;
   c->pred_dirty = true;
   c->state_ready = false;
   c->loaded_model = true;
+}
+
+// The header of the context's model: the loaded text's, or the trained
+// booster's (objective to_model_string: binary.rs:168-170,
+// multiclass.rs:169-171, rank.rs:338-340). names / infos / parameters
+// replace the header's when given.
+static std::string save_model(Ctx* c, int num_iteration, const char* names, const char* infos, const char* params) {
+  ModelHeader h;
+  if (c->loaded_model) {
+    h = c->loaded_head;
+  } else {
+    h.ncol = c->ncol;
+    h.ntpi = c->K;
+    h.num_class = c->opts.objective == 1 ? c->opts.num_class : 1;
+    if (c->opts.objective == 0) h.objective = "binary sigmoid:" + fmt_g(c->opts.sigmoid, 6);
+    else if (c->opts.objective == 1) h.objective = "multiclass num_class:" + std::to_string(c->opts.num_class);
+    else h.objective = "lambdarank";
+    for (int f = 0; f < h.ncol; ++f) {
+      h.feature_names += (f ? " Column_" : "Column_") + std::to_string(f);
+      h.feature_infos += f ? " none" : "none";
+    }
+  }
+  if (names) h.feature_names = names;
+  if (infos) h.feature_infos = infos;
+  if (params) {
+    h.has_parameters = true;
+    h.parameters = params;
+    if (!h.parameters.empty() && h.parameters.back() != '\n') h.parameters += '\n';
+  }
+  return model_text(h, c->trees, num_iteration);
 }
 
 // PredictKind::LeafIndex over host raw rows: out [rows][trees of the window].
@@ -2759,9 +2908,19 @@ lgbmhip_status lgbmhip_set_predict_window(lgbmhip_ctx* ctx, int32_t start_iterat
   });
 }
 lgbmhip_status lgbmhip_save_model(lgbmhip_ctx* ctx, int32_t num_iteration, const char* feature_names,
-                                  const char* feature_infos, char* out, int64_t capacity, int64_t* out_len) {
+                                  const char* feature_infos, const char* parameters, char* out, int64_t capacity,
+                                  int64_t* out_len) {
   return guarded(ctx, [&] {
-    put_text(save_model(&ctx->c, num_iteration, feature_names, feature_infos), out, capacity, out_len);
+    put_text(save_model(&ctx->c, num_iteration, feature_names, feature_infos, parameters), out, capacity, out_len);
+  });
+}
+lgbmhip_status lgbmhip_check_model(const char* text, int32_t* out_num_trees, int32_t* out_num_tree_per_iteration,
+                                   char* out, int64_t capacity, int64_t* out_len) {
+  return guarded(nullptr, [&] {
+    const ParsedModel m = parse_model(text);
+    if (out_num_trees) *out_num_trees = (int32_t)m.trees.size();
+    if (out_num_tree_per_iteration) *out_num_tree_per_iteration = m.head.ntpi;
+    if (out || out_len) put_text(model_text(m.head, m.trees, 0), out, capacity, out_len);
   });
 }
 lgbmhip_status lgbmhip_load_model(lgbmhip_ctx* ctx, const char* text) {

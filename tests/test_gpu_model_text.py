@@ -75,3 +75,39 @@ def test_hand_written_model_known_answers():
     # tree 1: categories 1 and 3 (bitset 0b1010) go left; NaN and negatives go right
     want = np.array([-1 + 0.25, 2 + 0.25, -1 + 0.5, 2 + 0.5, -1 + 0.5])
     np.testing.assert_array_equal(b.predict(X, raw_score=True), want)
+
+
+def test_two_categorical_bitsets_known_answers():
+    # one tree with two categorical nodes whose bitsets differ: node 0 sends
+    # categories {1, 3} of column 0 left (bitset 0: 0b1010), node 1 sends
+    # category {2} of column 1 left (bitset 1: 0b100). A node's bitset is the
+    # one its threshold names (tree.rs:400-403).
+    text = "\n".join([
+        "tree", "version=v4", "num_class=1", "num_tree_per_iteration=1", "label_index=0", "max_feature_idx=1",
+        "objective=binary sigmoid:1", "feature_names=a b", "feature_infos=none none", "",
+        "Tree=0", "num_leaves=3", "num_cat=2", "split_feature=0 1", "split_gain=1 1", "threshold=0 1",
+        "decision_type=1 1", "left_child=-1 -2", "right_child=1 -3", "leaf_value=10 20 30",
+        "leaf_weight=1 1 1", "leaf_count=1 1 1", "internal_value=0 0", "internal_weight=0 0",
+        "internal_count=3 2", "cat_boundaries=0 1 2", "cat_threshold=10 4", "is_linear=0", "shrinkage=1", "", "",
+        "end of trees", ""])
+    b = lgb.Booster(model_str=text)
+    X = np.array([[1, 0], [3, 2], [0, 2], [2, 1], [5, 2], [0, 3], [np.nan, 2]], np.float32)
+    want = np.array([10, 10, 20, 30, 20, 30, 20], np.float64)
+    np.testing.assert_array_equal(b.predict(X, raw_score=True), want)
+
+
+def test_save_of_load_of_save_is_stable():
+    # tests/end_to_end.rs:59-61: save(load(save(trained))) equals save(trained)
+    X, y = datagen.binary_100k(20000)
+    X = X.copy()
+    X[:, 5] = np.floor(np.abs(X[:, 5]) * 4)
+    params = dict(objective="binary", num_leaves=15, max_bin=63, categorical_feature=[5])
+    b = lgb.Booster(params, lgb.Dataset(X, label=y, params=params))
+    for _ in range(4):
+        b.update()
+    block = "[boosting: gbdt]\n[objective: binary]\n[device_type: cpu]\n[num_leaves: 15]\n"
+    text = b.model_to_string(parameters=block)
+    assert text.endswith("\nparameters:\n" + block + "\nend of parameters\n")
+    loaded = lgb.Booster(model_str=text)
+    assert loaded.model_to_string() == text
+    np.testing.assert_array_equal(loaded.predict(X[:3000], raw_score=True), b.predict(X[:3000], raw_score=True))
