@@ -224,6 +224,19 @@ lgbmhip_status lgbmhip_find_best_split_int(lgbmhip_ctx* ctx, int32_t feature, co
 /* Stable partition of rows[0..m) on inner feature `feature`: left rows (in order) then right rows. */
 lgbmhip_status lgbmhip_partition(lgbmhip_ctx* ctx, const uint32_t* rows, int64_t m, int32_t feature,
                                  uint32_t threshold, int32_t default_left, uint32_t* out_rows, int64_t* out_left);
+/* lgbmhip_partition on device memory that allocates nothing: d_rows[0..m)
+ * (device u32 row indices) partitioned stably into d_out_rows (device, m
+ * entries: left rows in order, then right rows in order); *out_left (host)
+ * receives the left count. No counterpart in the reference; it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc): its tile counts live in a static
+ * 256 KiB device buffer, one per GPU, so m is at most 67,106,816 rows
+ * (InvalidParameter above). Every call on a GPU shares that buffer: calls that
+ * overlap on one GPU (two contexts, or two host threads) race. A row index
+ * >= the uploaded rows is InvalidData; numerical features only. */
+lgbmhip_status lgbmhip_partition_device_noalloc(lgbmhip_ctx* ctx, const uint32_t* d_rows, int64_t m, int32_t feature,
+                                                uint32_t threshold, int32_t default_left, uint32_t* d_out_rows,
+                                                int64_t* out_left);
 /* Raw scores of the trained model on raw rows (rows x ncol f32): out [rows][K]. */
 lgbmhip_status lgbmhip_predict_raw(lgbmhip_ctx* ctx, const float* X, int64_t rows, int32_t ncol, double* out);
 /* Raw scores on binned rows (rows x F u8, the mappers of the training data): out [rows][K]. */

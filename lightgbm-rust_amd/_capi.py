@@ -107,6 +107,7 @@ SIGNATURES = {
     "lgbmhip_find_best_split": (I32, [P, I32, P, D, D, I32, D, P, P, P]),
     "lgbmhip_find_best_split_int": (I32, [P, I32, P, I64, D, D, I32, D, P, P, P, P]),
     "lgbmhip_partition": (I32, [P, P, I64, I32, C.c_uint32, I32, P, P]),
+    "lgbmhip_partition_device_noalloc": (I32, [P, P, I64, I32, C.c_uint32, I32, P, C.POINTER(I64)]),
     "lgbmhip_predict_raw": (I32, [P, P, I64, I32, P]),
     "lgbmhip_predict_binned": (I32, [P, P, I64, P]),
     "lgbmhip_predict_device": (I32, [P, P, P, I64, I32, P]),

@@ -2340,6 +2340,26 @@ static void predict(Ctx* c, const float* X, const uint8_t* bins, int64_t rows, i
 // ==================================================================== C ABI
 using namespace lgbmhip;
 
+// The no-allocation variant's work buffer: one per device (each GPU holds its
+// own copy at its own address), shared by every call on that device.
+static __device__ __align__(256) unsigned scratch[kNoAllocScratchWords];
+static std::mutex g_scratch_mu;
+static std::vector<unsigned*> g_scratch_ptr;   // per device: scratch's device address, resolved once
+
+// Resolves scratch's address on `device` (current) once; host code then uses
+// that pointer, never the array's name (a host-side stand-in).
+static unsigned* scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (!g_scratch_ptr[(size_t)device]) {
+    void* ptr = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error(LGBMHIP_INTERNAL, "no device address for the partition scratch");
+    g_scratch_ptr[(size_t)device] = (unsigned*)ptr;
+  }
+  return g_scratch_ptr[(size_t)device];
+}
+
 struct lgbmhip_ctx {
   Ctx c;
 };
@@ -2445,6 +2465,7 @@ lgbmhip_status lgbmhip_create(int32_t device, lgbmhip_ctx** out) {
       hipDeviceProp_t prop;
       HIP_CHECK(hipGetDeviceProperties(&prop, device));
       x->c.cus = prop.multiProcessorCount;
+      scratch_of(device);   // the no-allocation entry point's work buffer on this device
       x->c.lds_max = prop.sharedMemPerBlock;
       if (prop.maxSharedMemoryPerMultiProcessor > 0)
         x->c.lds_max = std::min<size_t>(std::max<size_t>(x->c.lds_max, 65536), prop.maxSharedMemoryPerMultiProcessor);
@@ -2784,33 +2805,77 @@ lgbmhip_status lgbmhip_find_best_split_int(lgbmhip_ctx* ctx, int32_t feature, co
   });
 }
 
+// The stable partition of device rows d_rows[0..m) into d_out (left rows in
+// order, then right rows in order) with the kSpartTile kernels on the
+// context's stream; work: at least ntiles + 2 words of device memory. Returns
+// the left count; a row index >= N is an INVALID_DATA error (the kernels
+// read nothing for it).
+static int64_t stable_partition(Ctx* c, const unsigned* d_rows, int64_t m, int feature, uint32_t threshold,
+                                int default_left, unsigned* d_out, unsigned* work) {
+  const int ntiles = (int)((m + kSpartTile - 1) / kSpartTile);
+  SplitRec s{};
+  s.threshold = (int)threshold;
+  s.default_left = default_left;
+  const uint8_t* col = c->bins_cm.p + (size_t)feature * c->N;
+  // the out-of-range flag word starts at zero on every call (a fill kernel, no memset node)
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, work + ntiles + 1, 1LL, 0u);
+  if (m > 0) {
+    hipLaunchKernelGGL(k_spart_count, dim3(ntiles), dim3(256), 0, c->stream, d_rows, (long long)m, (unsigned)c->N,
+                       col, c->feats[feature], s, work, ntiles);
+    hipLaunchKernelGGL(k_spart_scan, dim3(1), dim3(1024), 0, c->stream, work, ntiles);
+    hipLaunchKernelGGL(k_spart_scatter, dim3(ntiles), dim3(256), 0, c->stream, d_rows, (long long)m,
+                       (unsigned)c->N, col, c->feats[feature], s, work, ntiles, d_out);
+  } else {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, work, 1LL, 0u);
+  }
+  HIP_CHECK(hipGetLastError());
+  unsigned tail[2] = {0, 0};   // {left rows, out-of-range flag}
+  HIP_CHECK(hipMemcpyAsync(tail, work + ntiles, sizeof tail, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (tail[1]) throw Error(LGBMHIP_INVALID_DATA, "row index out of range");
+  return (int64_t)tail[0];
+}
+
+static void check_partition_args(Ctx* c, int64_t m, int32_t feature) {
+  if (!c->bins_cm.p) throw Error(LGBMHIP_INVALID_PARAMETER, "no bins uploaded");
+  if (feature < 0 || feature >= c->F) throw Error(LGBMHIP_INVALID_PARAMETER, "feature out of range");
+  if (m < 0 || m > c->N) throw Error(LGBMHIP_INVALID_DATA, "row count out of range");
+  if (c->feats[feature].bin_type != 0) throw Error(LGBMHIP_UNSUPPORTED, "partition on a categorical feature");
+}
+
 lgbmhip_status lgbmhip_partition(lgbmhip_ctx* ctx, const uint32_t* rows, int64_t m, int32_t feature,
                                  uint32_t threshold, int32_t default_left, uint32_t* out_rows, int64_t* out_left) {
   return guarded(ctx, [&] {
     Ctx* c = &ctx->c;
-    if (feature < 0 || feature >= c->F) throw Error(LGBMHIP_INVALID_PARAMETER, "feature out of range");
+    check_partition_args(c, m, feature);
     set_device(c);
     for (int64_t i = 0; i < m; ++i)
       if (rows[i] >= (uint64_t)c->N) throw Error(LGBMHIP_INVALID_DATA, "row index out of range");
-    DevBuf<unsigned> dr;
-    DevBuf<uint8_t> flag;
+    DevBuf<unsigned> dr, dout, work;
     dr.alloc(std::max<int64_t>(m, 1));
-    flag.alloc(std::max<int64_t>(m, 1));
+    dout.alloc(std::max<int64_t>(m, 1));
+    work.alloc((size_t)(m + kSpartTile - 1) / kSpartTile + 2);
     if (m > 0) HIP_CHECK(hipMemcpyAsync(dr.p, rows, m * 4, hipMemcpyHostToDevice, c->stream));
-    SplitRec s{};
-    s.threshold = (int)threshold;
-    s.default_left = default_left;
-    if (m > 0)
-      hipLaunchKernelGGL(k_partition_rows, dim3((int)((m + 255) / 256)), dim3(256), 0, c->stream, dr.p, (long long)m,
-                         c->bins_cm.p + (size_t)feature * c->N, c->feats[feature], s, flag.p);
-    std::vector<uint8_t> fl(std::max<int64_t>(m, 1));
-    HIP_CHECK(hipMemcpyAsync(fl.data(), flag.p, std::max<int64_t>(m, 1), hipMemcpyDeviceToHost, c->stream));
+    *out_left = stable_partition(c, dr.p, m, feature, threshold, default_left, dout.p, work.p);
+    if (m > 0) HIP_CHECK(hipMemcpyAsync(out_rows, dout.p, m * 4, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
-    int64_t nl = 0;
-    for (int64_t i = 0; i < m; ++i) if (fl[i]) out_rows[nl++] = rows[i];
-    int64_t k = nl;
-    for (int64_t i = 0; i < m; ++i) if (!fl[i]) out_rows[k++] = rows[i];
-    *out_left = nl;
+  });
+}
+
+lgbmhip_status lgbmhip_partition_device_noalloc(lgbmhip_ctx* ctx, const uint32_t* d_rows, int64_t m, int32_t feature,
+                                                uint32_t threshold, int32_t default_left, uint32_t* d_out_rows,
+                                                int64_t* out_left) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (m > kNoAllocMaxRows)
+      throw Error(LGBMHIP_INVALID_PARAMETER, "lgbmhip_partition_device_noalloc takes at most " +
+                                                 std::to_string(kNoAllocMaxRows) + " rows");
+    check_partition_args(c, m, feature);
+    if (m > 0 && (!d_rows || !d_out_rows)) throw Error(LGBMHIP_INVALID_PARAMETER, "no device rows");
+    if (!out_left) throw Error(LGBMHIP_INVALID_PARAMETER, "no output for the left count");
+    set_device(c);
+    unsigned* work = scratch_of(c->device);
+    *out_left = stable_partition(c, d_rows, m, feature, threshold, default_left, d_out_rows, work);
   });
 }
 

@@ -1460,11 +1460,124 @@ __global__ void __launch_bounds__(kPredRows * W) k_predict(PredArgs a) {
   }
 }
 
-// Stage entry point: stable partition of explicit rows (partition.rs:118-174).
-__global__ void k_partition_rows(const unsigned* rows, long long m, const uint8_t* col, FeatDev fd, SplitRec s,
-                                 uint8_t* flag) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m) flag[i] = goes_left(col[rows[i]], fd, s) ? 1 : 0;
+// Stage entry points: stable partition of explicit rows (partition.rs:107-174)
+// in tiles of kSpartTile rows, 256 threads of 4 consecutive rows each.
+//   k_spart_count   one block per tile: the tile's left rows -> cnt[tile];
+//                   a row index >= n sets cnt[ntiles + 1] (nothing is read for it)
+//   k_spart_scan    one block: cnt[t] -> first left slot of tile t (exclusive
+//                   prefix), cnt[ntiles] = all left rows
+//   k_spart_scatter one block per tile: left rows in order from the tile's
+//                   slot, right rows in order after all left rows
+// Work buffer: ntiles + 2 words; cnt[ntiles + 1] must be zero before the count.
+constexpr int kSpartTile = 1024;
+// lgbmhip_partition_device_noalloc's work buffer: 256 KiB per device, enough
+// for the tile counts of (kNoAllocScratchWords - 2) tiles of rows
+constexpr int kNoAllocScratchWords = 1 << 16;
+constexpr long long kNoAllocMaxRows = (long long)(kNoAllocScratchWords - 2) * kSpartTile;
+
+// words p[0..n) = v (a kernel, so a caller's graph capture records no memset node)
+__global__ void k_fill_u32(unsigned* p, long long n, unsigned v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__device__ __forceinline__ void spart_load(const unsigned* rows, long long m, unsigned n, const uint8_t* col,
+                                           const FeatDev& fd, const SplitRec& s, long long first, unsigned r[4],
+                                           bool valid[4], bool left[4], bool* bad) {
+  unsigned b[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    valid[u] = first + u < m;
+    r[u] = valid[u] ? rows[first + u] : 0u;
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const bool in = valid[u] && r[u] < n;
+    if (valid[u] && !in) *bad = true;
+    b[u] = in ? col[r[u]] : 0u;
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) left[u] = valid[u] && r[u] < n && goes_left((int)b[u], fd, s);
+}
+
+__global__ void __launch_bounds__(256) k_spart_count(const unsigned* rows, long long m, unsigned n,
+                                                     const uint8_t* col, FeatDev fd, SplitRec s, unsigned* cnt,
+                                                     int ntiles) {
+  __shared__ unsigned tot;
+  if (threadIdx.x == 0) tot = 0;
+  __syncthreads();
+  unsigned r[4];
+  bool valid[4], left[4], bad = false;
+  spart_load(rows, m, n, col, fd, s, (long long)blockIdx.x * kSpartTile + threadIdx.x * 4, r, valid, left, &bad);
+  unsigned c = (unsigned)left[0] + (unsigned)left[1] + (unsigned)left[2] + (unsigned)left[3];
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(&tot, c);
+  if (bad) atomicOr(&cnt[ntiles + 1], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+}
+
+// Exclusive scan of cnt[0..ntiles) in place by one 1024-thread block, in
+// passes of 1024 entries with a running carry; cnt[ntiles] = the total.
+__global__ void __launch_bounds__(1024) k_spart_scan(unsigned* cnt, int ntiles) {
+  __shared__ unsigned wsum[16];
+  __shared__ unsigned carry;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < ntiles; base += 1024) {
+    const int i = base + t;
+    const unsigned v = i < ntiles ? cnt[i] : 0u;
+    unsigned x = v;   // inclusive wave scan
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    unsigned before = carry;
+    for (int k = 0; k < w; ++k) before += wsum[k];
+    if (i < ntiles) cnt[i] = before + x - v;
+    __syncthreads();
+    if (t == 0) {
+      unsigned add = 0;
+      for (int k = 0; k < 16; ++k) add += wsum[k];
+      carry += add;
+    }
+    __syncthreads();
+  }
+  if (t == 0) cnt[ntiles] = carry;
+}
+
+__global__ void __launch_bounds__(256) k_spart_scatter(const unsigned* rows, long long m, unsigned n,
+                                                       const uint8_t* col, FeatDev fd, SplitRec s,
+                                                       const unsigned* cnt, int ntiles, unsigned* out) {
+  __shared__ unsigned wsum[4];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const long long tile0 = (long long)blockIdx.x * kSpartTile;
+  unsigned r[4];
+  bool valid[4], left[4], bad = false;
+  spart_load(rows, m, n, col, fd, s, tile0 + t * 4, r, valid, left, &bad);
+  const unsigned c = (unsigned)left[0] + (unsigned)left[1] + (unsigned)left[2] + (unsigned)left[3];
+  unsigned x = c;   // inclusive wave scan of the threads' left counts
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  unsigned lbefore = x - c;   // left rows of this tile before this thread's rows
+  for (int k = 0; k < w; ++k) lbefore += wsum[k];
+  const unsigned tile_left = cnt[blockIdx.x], total_left = cnt[ntiles];
+  unsigned lslot = tile_left + lbefore;
+  // right rows before this thread's: the tile's earlier rows minus its earlier left rows
+  unsigned rslot = total_left + ((unsigned)tile0 - tile_left) + ((unsigned)t * 4u - lbefore);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (!valid[u]) continue;
+    if (left[u]) out[lslot++] = r[u];
+    else out[rslot++] = r[u];
+  }
 }
 
 // Stage entry point: one feature's threshold search on a given histogram.
