@@ -2183,6 +2183,82 @@ void orc_booster_free(void* p) {
   delete b->rank;
   delete b;
 }
+
+// A prediction-only booster from model text (io.rs:156-343 header and tree
+// blocks, tree.rs:869-1047 fields): num_tree_per_iteration (or num_class) and
+// each tree's num_leaves, num_cat, split_feature, threshold, decision_type,
+// children, leaf_value and categorical bitsets. Numbers are read with strtod
+// (correctly rounded, as parse_f64). Returns null on a malformed text.
+void* orc_model_load(const char* text) {
+  std::vector<std::string> lines;
+  std::string cur;
+  for (const char* q = text; *q; ++q) {
+    if (*q == '\n') { lines.push_back(cur); cur.clear(); }
+    else if (*q != '\r') cur += *q;
+  }
+  lines.push_back(cur);
+  auto vals = [](const std::string& v) {
+    std::vector<double> out;
+    const char* q = v.c_str();
+    while (*q) {
+      while (*q == ' ') ++q;
+      if (!*q) break;
+      char* e;
+      out.push_back(std::strtod(q, &e));
+      if (e == q) return std::vector<double>();
+      q = e;
+    }
+    return out;
+  };
+  int K = -1, num_class = 1;
+  size_t i = 0;
+  for (; i < lines.size() && lines[i].rfind("Tree=", 0) != 0; ++i) {
+    if (lines[i].rfind("num_tree_per_iteration=", 0) == 0) K = std::atoi(lines[i].c_str() + 23);
+    if (lines[i].rfind("num_class=", 0) == 0) num_class = std::atoi(lines[i].c_str() + 10);
+  }
+  if (K < 1) K = num_class;
+  OrcBooster* b = new OrcBooster();
+  b->ds = nullptr;
+  b->learner = nullptr;
+  b->K = K;
+  while (i < lines.size() && lines[i].rfind("Tree=", 0) == 0) {
+    std::map<std::string, std::string> kv;
+    for (++i; i < lines.size() && !lines[i].empty(); ++i) {
+      const size_t eq = lines[i].find('=');
+      if (eq != std::string::npos) kv[lines[i].substr(0, eq)] = lines[i].substr(eq + 1);
+    }
+    while (i < lines.size() && lines[i].empty()) ++i;
+    Tree t;
+    const int n = std::atoi(kv["num_leaves"].c_str());
+    if (n < 1) { delete b; return nullptr; }
+    t.num_leaves = n;
+    t.num_cat = std::atoi(kv["num_cat"].c_str());
+    const auto lv = vals(kv["leaf_value"]);
+    if ((int)lv.size() != n) { delete b; return nullptr; }
+    t.leaf_value = lv;
+    if (n > 1) {
+      const auto sf = vals(kv["split_feature"]), th = vals(kv["threshold"]), dt = vals(kv["decision_type"]),
+                 lc = vals(kv["left_child"]), rc = vals(kv["right_child"]);
+      if ((int)sf.size() != n - 1 || (int)th.size() != n - 1 || (int)dt.size() != n - 1 || (int)lc.size() != n - 1 ||
+          (int)rc.size() != n - 1) { delete b; return nullptr; }
+      for (int k = 0; k < n - 1; ++k) {
+        t.split_feature.push_back((int)sf[k]);
+        t.threshold.push_back(th[k]);
+        t.decision_type.push_back((int8_t)(int)dt[k]);
+        t.left_child.push_back((int)lc[k]);
+        t.right_child.push_back((int)rc[k]);
+      }
+      if (t.num_cat > 0) {
+        const auto cb = vals(kv["cat_boundaries"]), ct = vals(kv["cat_threshold"]);
+        t.cat_boundaries.assign(cb.begin(), cb.end());
+        for (double x : ct) t.cat_threshold.push_back((uint32_t)(long long)x);
+      }
+    }
+    b->models.push_back(std::move(t));
+  }
+  if (b->models.empty()) { delete b; return nullptr; }
+  return b;
+}
 int orc_booster_train_one_iter(void* p) { return booster_train_one_iter((OrcBooster*)p); }
 int orc_booster_num_trees(void* p) { return (int)((OrcBooster*)p)->models.size(); }
 void orc_booster_scores(void* p, double* out) {

@@ -83,6 +83,7 @@ def lib():
             "orc_metric_auc": (D, [PD, PF, P, I64]),
             "orc_metric_point": (D, [I, PD, PF, P, I64, I, D, I]),
             "orc_metric_ndcg": (None, [PD, PF, PI, I, PI, I, I, PD]),
+            "orc_model_load": (P, [C.c_char_p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -251,6 +252,19 @@ class Booster:
         if getattr(self, "h", None) and _lib is not None:
             _lib.orc_booster_free(self.h)
             self.h = None
+
+
+class LoadedModel(Booster):
+    """A prediction-only oracle booster from model text (io.rs:156-343):
+    predict_raw and predict_window of Booster on the same trees."""
+
+    def __init__(self, text, K=None):
+        self.ds = None
+        self.h = lib().orc_model_load(text.encode())
+        if not self.h:
+            raise ValueError("oracle could not read the model text")
+        head = dict(l.split("=", 1) for l in text.split("Tree=")[0].splitlines() if "=" in l)
+        self.K = int(head.get("num_tree_per_iteration", head.get("num_class", "1")))
 
 
 def hist_f64(bins_u8, mfb, B, idx, g, h):
