@@ -1910,6 +1910,19 @@ using namespace orc;
 // no query bagging, no GOSS). One LCG per 1024-row block seeded
 // bagging_seed + block; a row is in the bag when its draw is below the
 // fraction.
+// threading.rs block_info_force_size(num_threads, n, min_size): at most
+// num_threads blocks of a multiple of min_size rows (one block of n rows when
+// a single block results).
+static void block_info_force_size(int num_threads, int64_t n, int64_t min_size, int64_t* nblock, int64_t* block) {
+  *nblock = std::min<int64_t>(std::max(num_threads, 1), (n + min_size - 1) / min_size);
+  *block = n;
+  if (*nblock > 1) {
+    *block = (n + *nblock - 1) / *nblock;
+    *block = (*block + min_size - 1) / min_size * min_size;
+  }
+  if (*nblock < 1) *nblock = 1;
+}
+
 struct Bagger {
   bool on = false, subset = false, need_re_bagging = false, balanced = false;
   // GOSS (sample_strategy.rs:319-464): blocks of block_info_force_size(num_threads, n, 1024)
@@ -1999,12 +2012,8 @@ struct Bagger {
   bool bagging_goss(int iter, float* grad, float* hess) {
     bag_cnt = n;
     if ((int64_t)iter < (int64_t)((double)1.0f / learning_rate)) return false;
-    // threading.rs block_info_force_size(num_threads, n, 1024)
-    int64_t nblock = std::min<int64_t>(num_threads, (n + 1023) / 1024), block = n;
-    if (nblock > 1) {
-      block = (n + nblock - 1) / nblock;
-      block = (block + 1023) / 1024 * 1024;
-    }
+    int64_t nblock, block;
+    block_info_force_size(num_threads, n, 1024, &nblock, &block);
     in_bag.clear();
     out_of_bag.clear();
     for (int64_t b = 0; b < nblock; ++b) {
@@ -2108,6 +2117,9 @@ static int booster_train_one_iter(OrcBooster* b) {
 extern "C" {
 
 int orc_version() { return 1; }
+void orc_block_info_force_size(int num_threads, int64_t n, int64_t min_size, int64_t* nblock, int64_t* block) {
+  block_info_force_size(num_threads, n, min_size, nblock, block);
+}
 
 void* orc_dataset_create(const float* X, int64_t n, int ncol, const float* label, const char* params) {
   Params p;
@@ -2260,6 +2272,27 @@ void* orc_model_load(const char* text) {
   return b;
 }
 int orc_booster_train_one_iter(void* p) { return booster_train_one_iter((OrcBooster*)p); }
+// One GOSS draw (sample_strategy.rs:320-442) on caller gradients g, h [K][n]
+// (rescaled in place): a fresh sampler of the given seed, thread count and
+// rates at an iteration past the warm-up. Writes the in-bag rows in order and
+// returns their count.
+int64_t orc_goss(float* g, float* h, int64_t n, int K, int num_threads, double top_rate, double other_rate,
+                 int seed, uint32_t* in_bag) {
+  Params p;
+  p.goss = 1;
+  p.top_rate = top_rate;
+  p.other_rate = other_rate;
+  p.num_threads = num_threads;
+  p.bagging_seed = seed;
+  p.learning_rate = 1.0;
+  p.objective = K > 1 ? 1 : 0;
+  p.num_class = K;
+  Bagger bg;
+  bg.setup(p, n, 1, 0, nullptr);
+  bg.bagging_goss(1, g, h);
+  std::memcpy(in_bag, bg.in_bag.data(), bg.in_bag.size() * 4);
+  return (int64_t)bg.in_bag.size();
+}
 int orc_booster_num_trees(void* p) { return (int)((OrcBooster*)p)->models.size(); }
 void orc_booster_scores(void* p, double* out) {
   OrcBooster* b = (OrcBooster*)p;

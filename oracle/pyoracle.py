@@ -84,6 +84,8 @@ def lib():
             "orc_metric_point": (D, [I, PD, PF, P, I64, I, D, I]),
             "orc_metric_ndcg": (None, [PD, PF, PI, I, PI, I, I, PD]),
             "orc_model_load": (P, [C.c_char_p]),
+            "orc_goss": (I64, [PF, PF, I64, I, I, D, D, I, PU]),
+            "orc_block_info_force_size": (None, [I, I64, I64, C.POINTER(I64), C.POINTER(I64)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -265,6 +267,22 @@ class LoadedModel(Booster):
             raise ValueError("oracle could not read the model text")
         head = dict(l.split("=", 1) for l in text.split("Tree=")[0].splitlines() if "=" in l)
         self.K = int(head.get("num_tree_per_iteration", head.get("num_class", "1")))
+
+
+def block_info_force_size(num_threads, n, min_size):
+    a, b = I64(), I64()
+    lib().orc_block_info_force_size(num_threads, n, min_size, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def goss(g, h, K, num_threads, top_rate, other_rate, seed):
+    """One GOSS draw on gradients [K][n] (copies): (in-bag rows, rescaled g, rescaled h)."""
+    g = np.ascontiguousarray(g, np.float32).copy()
+    h = np.ascontiguousarray(h, np.float32).copy()
+    n = g.size // K
+    bag = np.zeros(max(n, 1), np.uint32)
+    m = lib().orc_goss(g, h, n, K, num_threads, top_rate, other_rate, seed, bag)
+    return bag[:m], g, h
 
 
 def hist_f64(bins_u8, mfb, B, idx, g, h):

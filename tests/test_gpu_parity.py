@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from tests._pkg import datagen, lgb, small_binary
+from tests._trees import compare_or_near_tie
 from oracle import pyoracle as O
 
 pytestmark = pytest.mark.gpu
@@ -464,30 +465,35 @@ def test_quantized_boosting_255_leaves_matches_oracle():
     np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
 
 
+def _boost_both(b, ob, n_iter, K=1):
+    """n_iter iterations on both sides, each computing its own gradients.
+    Every tree must equal the oracle's; a differing split is accepted only as
+    a reported near-tie (equal f32 gains), after which the runs diverge and
+    the comparison stops. Returns the iterations compared."""
+    for it in range(n_iter):
+        b.update()
+        ob.update()
+        for k in range(K):
+            if not compare_or_near_tie(b.tree(it * K + k), ob.tree(it * K + k)):
+                return it
+    return n_iter
+
+
 def test_boosting_matches_oracle_and_predicts():
     X, y = datagen.binary_100k(100000)
     params = dict(P_BIN, num_leaves=31, max_bin=255)
     ds, b = _booster(X, y, params)
     ob = O.Booster(O.Dataset(X, y, params), params)
     n_iter = 10
-    identical = 0
-    for it in range(n_iter):
-        b.update()
-        ob.update()
-        gt, ot = b.tree(it), ob.tree(it)
-        if (gt.num_leaves == ot.num_leaves and np.array_equal(gt.split_feature, ot.split_feature)
-                and np.array_equal(gt.threshold_in_bin, ot.threshold_in_bin)):
-            identical += 1
-    # tree structures are expected to coincide (near-ties aside)
-    assert identical >= n_iter - 1
+    done = _boost_both(b, ob, n_iter)
     s_gpu, s_cpu = b.train_scores(), ob.scores()
-    assert np.max(np.abs(s_gpu - s_cpu)) < 1e-6 or identical < n_iter
+    if done == n_iter:
+        np.testing.assert_allclose(s_gpu, s_cpu, rtol=1e-9, atol=1e-12)
     raw = b.predict(X, raw_score=True)
     # training scores equal raw predictions (end_to_end.rs:53-57)
-    assert np.all(np.abs(raw - s_gpu) <= 1e-9 * (1 + np.abs(s_gpu)))
-    # binned prediction agrees with raw prediction on training rows
-    pb = b.predict_binned(ds.bins)[:, 0]
-    assert np.all(np.abs(pb - raw) <= 1e-9 * (1 + np.abs(raw)))
+    np.testing.assert_array_equal(raw, s_gpu)
+    # binned prediction equals raw prediction on training rows
+    np.testing.assert_array_equal(b.predict_binned(ds.bins)[:, 0], raw)
     assert abs(O.auc(s_gpu, y) - O.auc(s_cpu, y)) < 1e-4
 
 
@@ -525,10 +531,12 @@ def test_quantized_boosting_quality_close_to_plain():
         params = dict(P_BIN, num_leaves=31, max_bin=63, use_quantized_grad=q, num_grad_quant_bins=16,
                       quant_train_renew_leaf=True)
         ds, b = _booster(X, y, params)
-        for _ in range(20):
+        # past the first steep iterations (at 20 the AUC still climbs 0.89 -> 0.93 and
+        # one iteration's noise moves it by ~1e-3)
+        for _ in range(60):
             b.update()
         aucs[q] = O.auc(b.predict(Xv, raw_score=True), yv)
-    assert abs(aucs[True] - aucs[False]) < 5e-3, aucs
+    assert abs(aucs[True] - aucs[False]) <= 1e-3, aucs
 
 
 def test_multiclass_boosting_trains():
@@ -543,8 +551,9 @@ def test_multiclass_boosting_trains():
     for it in range(3):
         b.update()
         ob.update()
-    same = sum(np.array_equal(b.tree(t).split_feature, ob.tree(t).split_feature) for t in range(3 * K))
-    assert same >= 3 * K - 1
+    for t in range(3 * K):
+        if not compare_or_near_tie(b.tree(t), ob.tree(t)):
+            break
     p = b.predict(X)
     assert p.shape == (n, K) and np.allclose(p.sum(1), 1.0)
     acc = (p.argmax(1) == y).mean()
@@ -581,14 +590,15 @@ def test_lambdarank_boosting_improves_ndcg():
     ods.set_query_boundaries(qb)
     ob = O.Booster(ods, params)
     n0 = O.ndcg(np.zeros(len(rel)), rel, qb, 10)
-    for it in range(5):
-        b.update()
-        ob.update()
-    same = sum(np.array_equal(b.tree(t).split_feature, ob.tree(t).split_feature) for t in range(5))
-    assert same >= 4
+    done = _boost_both(b, ob, 5)
     n1 = O.ndcg(b.train_scores(), rel, qb, 10)
     n_cpu = O.ndcg(ob.scores(), rel, qb, 10)
-    assert n1 > n0 and abs(n1 - n_cpu) < 1e-3
+    assert n1 > n0
+    if done == 5:
+        np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
+        assert n1 == n_cpu
+    else:
+        assert abs(n1 - n_cpu) < 1e-3
 
 
 def _categorical_data(n, seed):

@@ -617,3 +617,145 @@ def test_bagging_generators_run_on_between_draws():
     ob.update()
     second = ob.bag()
     assert not np.array_equal(first, second)
+
+
+def _lcg_streams(seeds, count):
+    """count draws of Random(seed) for each seed (random.rs:19-33, 97-105):
+    x' = 214013 x + 2531011 (mod 2^32), next_float = ((x' >> 16) & 0x7FFF) / 32768 in f32."""
+    x = np.array(seeds, np.uint64) & np.uint64(0xFFFFFFFF)
+    out = np.empty((len(seeds), count), np.float32)
+    for i in range(count):
+        x = (x * np.uint64(214013) + np.uint64(2531011)) & np.uint64(0xFFFFFFFF)
+        out[:, i] = ((x >> np.uint64(16)) & np.uint64(0x7FFF)).astype(np.float32) / np.float32(32768.0)
+    return out
+
+
+def _expected_bags(n, seed, fracs_of_row, bags):
+    """The bags of `bags` consecutive draws: block j's generator is
+    Random(seed + j) and runs on from one bag to the next; row i is in the bag
+    when its draw (as f64) is below its fraction (sample_strategy.rs:285-315)."""
+    nb = (n + 1023) // 1024
+    streams = _lcg_streams([seed + j for j in range(nb)], 1024 * bags)
+    out = []
+    for k in range(bags):
+        draws = np.empty(n, np.float32)
+        for j in range(nb):
+            lo, hi = j * 1024, min(n, (j + 1) * 1024)
+            draws[lo:hi] = streams[j, k * (hi - lo):(k + 1) * (hi - lo)]
+        out.append(np.flatnonzero(draws.astype(np.float64) < fracs_of_row))
+    return out
+
+
+@pytest.mark.parametrize("balanced", [False, True])
+def test_bag_draws_follow_the_lcg_streams(balanced):
+    n = 20 * 1024 + 300   # a ragged last block
+    rng = np.random.default_rng(21)
+    X = rng.normal(size=(n, 3)).astype(np.float32)
+    y = (rng.random(n) < 0.35).astype(np.float32)
+    p = dict(objective="binary", num_leaves=4, bagging_freq=1, bagging_seed=5)
+    if balanced:
+        p.update(pos_bagging_fraction=0.3, neg_bagging_fraction=0.6)
+        fr = np.where(y > 0, 0.3, 0.6)
+    else:
+        p.update(bagging_fraction=0.45)
+        fr = np.full(n, 0.45)
+    ob = O.Booster(O.Dataset(X, y, p), p)
+    want = _expected_bags(n, 5, fr, 3)
+    for k in range(3):
+        ob.update()
+        np.testing.assert_array_equal(ob.bag(), want[k])
+
+
+def test_goss_block_info_known_answers():
+    # sample_strategy.rs:457-463 (block_info_force_size)
+    assert O.block_info_force_size(1, 5000, 1024) == (1, 5000)
+    assert O.block_info_force_size(4, 5000, 1024) == (4, 2048)
+    assert O.block_info_force_size(8, 3000, 1024) == (3, 1024)
+    assert O.block_info_force_size(4, 500, 1024) == (1, 500)
+
+
+def _check_goss_block_properties(v, g0, h0, g1, h1, bag, n, K, threads, top_rate, other_rate):
+    """Per sampling block of block_info_force_size(threads, n, 1024): every row
+    whose |g h| reaches the block's top_k-th largest value is kept; at most
+    other_k other rows are sampled, exactly other_k when no value ties the
+    threshold (selection sampling with probability rest_need / rest_all keeps
+    exactly that many); sampled rows' g and h are multiplied by
+    f32(cnt - top_k) / f32(other_k), every other row's are untouched."""
+    in_bag = np.zeros(n, bool)
+    in_bag[bag] = True
+    assert np.all(np.diff(bag.astype(np.int64)) > 0)
+    nblock, block = O.block_info_force_size(threads, n, 1024)
+    scaled = np.zeros(n, bool)
+    exact_blocks = 0
+    for b in range(nblock):
+        lo = b * block
+        if lo >= n:
+            break
+        hi = min(n, lo + block)
+        cnt = hi - lo
+        top_k = max(int(cnt * top_rate), 1)
+        other_k = int(cnt * other_rate)
+        vb = v[lo:hi]
+        thr = np.sort(vb)[::-1][top_k - 1]
+        top = vb >= thr
+        assert in_bag[lo:hi][top].all()
+        small_in = in_bag[lo:hi] & ~top
+        if top.sum() == top_k:
+            assert small_in.sum() == other_k
+            exact_blocks += 1
+        else:
+            assert small_in.sum() <= other_k
+        scaled[lo:hi] = small_in
+        mult = np.float32(cnt - top_k) / np.float32(other_k)
+        for k in range(K):
+            np.testing.assert_array_equal(g1[k, lo:hi][small_in], g0[k, lo:hi][small_in] * mult)
+            np.testing.assert_array_equal(h1[k, lo:hi][small_in], h0[k, lo:hi][small_in] * mult)
+    for k in range(K):
+        np.testing.assert_array_equal(g1[k][~scaled], g0[k][~scaled])
+        np.testing.assert_array_equal(h1[k][~scaled], h0[k][~scaled])
+    return exact_blocks
+
+
+def _abs_gh(g, h, K):
+    v = np.zeros(g.shape[1], np.float32)
+    for k in range(K):   # f32 sum over the classes in order (sample_strategy.rs:404-411)
+        v = v + np.abs(g[k] * h[k])
+    return v
+
+
+@pytest.mark.parametrize("K,threads", [(1, 3), (1, 1), (3, 2), (2, 8)])
+def test_goss_sample_properties_on_continuous_gradients(K, threads):
+    n = 9 * 1024 + 77
+    rng = np.random.default_rng(30 + K + threads)
+    g0 = rng.normal(size=(K, n)).astype(np.float32)
+    h0 = (0.05 + rng.random(size=(K, n))).astype(np.float32)
+    bag, g1, h1 = O.goss(g0, h0, K, threads, 0.2, 0.15, 7)
+    g1, h1 = g1.reshape(K, n), h1.reshape(K, n)
+    exact = _check_goss_block_properties(_abs_gh(g0, h0, K), g0, h0, g1, h1, bag, n, K, threads, 0.2, 0.15)
+    nblock, block = O.block_info_force_size(threads, n, 1024)
+    assert exact == sum(1 for b in range(nblock) if b * block < n)   # no ties: exactly other_k in every block
+
+
+@pytest.mark.parametrize("K,threads", [(1, 3), (3, 2)])
+def test_goss_sample_properties_in_boosting(K, threads):
+    # boosting scores take few distinct values, so |g h| ties at the threshold
+    n = 6 * 1024 + 100
+    rng = np.random.default_rng(22 + K)
+    X = rng.normal(size=(n, 4)).astype(np.float32)
+    if K == 1:
+        y = (X[:, 0] + 0.5 * rng.normal(size=n) > 0).astype(np.float32)
+        p = dict(objective="binary")
+    else:
+        y = np.argmax(X[:, :K] + rng.gumbel(size=(n, K)), 1).astype(np.float32)
+        p = dict(objective="multiclass", num_class=K)
+    p.update(num_leaves=7, learning_rate=0.5, data_sample_strategy="goss", top_rate=0.2, other_rate=0.15,
+             num_threads=threads, bagging_seed=3)
+    ob = O.Booster(O.Dataset(X, y, p), p)
+    for _ in range(2):   # GOSS starts at iteration 1 / learning_rate = 2
+        ob.update()
+    scores = ob.scores()
+    ob.update()
+    g1, h1 = ob.gradients()
+    g0, h0 = O.binary_gradients(scores, y) if K == 1 else O.multiclass_gradients(scores, y, K)
+    g0, h0, g1, h1 = (a.reshape(K, n) for a in (g0, h0, g1, h1))
+    _check_goss_block_properties(_abs_gh(g0, h0, K), g0, h0, g1, h1, ob.bag(), n, K, threads, 0.2, 0.15)
