@@ -10,6 +10,13 @@ A step is one boosting iteration (gradients, one 255-leaf tree, score
 update) over the full matrix resident in HBM. Warmup iterations run first;
 then exactly --steps iterations are timed between device synchronizations.
 
+Beside the timed steps, each form also trains a fresh booster for the full
+--rounds (500) with its train loop timed end to end, and reports the
+holdout AUC at that last round (|delta AUC| between the forms); the same is
+done at each --also-max-bin value (63). The CPU baseline is the oracle's
+threaded C++ engine on all rows, at the host's cores (at most 16) and at one
+thread, best of --cpu-repeats iterations each.
+
 Prints one JSON line (the driver's contract) and exits 0.
 
 Kernel timing: the timed iterations run as captured HIP graphs with no
@@ -85,22 +92,68 @@ def run_form(lgb, ds, params, warmup, steps, quality, prof_iters, prof_all=False
                 hist_fb=int(tun["hist_features_per_block"]), hist_ng=int(tun["hist_groups"]), lds_peak=lds_peak)
 
 
-def cpu_baseline(rows, iters, params, full_rows):
-    from tests._pkg import datagen
-    from oracle import pyoracle as O
-    X, y = datagen.binary_10p5m(rows)
-    ds = O.Dataset(X, y, params)
-    b = O.Booster(ds, params)
-    b.update()  # warm (first iteration includes boost-from-average)
+def host_cores():
+    """Host threads this process may use: its CPU affinity, at most 16 (the
+    host share of one GPU on the benchmark machines)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def full_run(lgb, ds, params, rounds, quality):
+    """A fresh booster trained for `rounds` iterations, the train loop timed
+    end to end (graph capture of the first tree included), then the quality
+    metric at that round."""
+    b = lgb.Booster(params, ds)
     t0 = time.perf_counter()
-    for _ in range(iters):
-        b.update()
+    done = 0
+    for _ in range(rounds):
+        if b.update():
+            break
+        done += 1
     dt = time.perf_counter() - t0
-    its = iters / dt
-    return dict(value=its * rows / full_rows, unit=f"iters/sec at {full_rows} rows (linear scale from the sample)",
-                cores=1, kind="port",
-                sample=f"oracle (single-thread C++ restatement), {rows} rows x 28 of the same generator, {iters} "
-                       f"timed iterations after 1 warm, {its:.4f} it/s on the sample")
+    q = None
+    try:
+        q = quality(b)
+    except Exception as e:
+        log(f"quality metric unavailable: {e}")
+    b.free()
+    return dict(rounds=done, seconds=dt, iters_per_sec=done / dt, quality=q)
+
+
+def cpu_baseline(X, y, params, threads, repeats):
+    """The repository's CPU engine (the oracle's C++ learner, threaded: its
+    trees do not depend on the thread count) on the full workload: one warm
+    iteration, then `repeats` timed iterations, the best kept, at `threads`
+    host threads and at one thread (the reference's timing method: train loop
+    only, best of 3; crates/lgbm-core/benches/train.rs:61-83)."""
+    from oracle import pyoracle as O
+    t = time.perf_counter()
+    ds = O.Dataset(X, y, params)
+    log(f"cpu: oracle dataset in {time.perf_counter() - t:.1f}s")
+    best = {}
+    try:
+        for th in (threads, 1):
+            O.set_num_threads(th)
+            b = O.Booster(ds, params)
+            b.update()   # warm: the first iteration includes boost-from-average and the quantizer tables
+            ts = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                b.update()
+                ts.append(time.perf_counter() - t0)
+            best[th] = min(ts)
+            log(f"cpu: {th} threads, iterations {['%.3f' % x for x in ts]} s")
+            del b
+    finally:
+        O.set_num_threads(1)
+    return dict(value=1.0 / best[threads], unit="iters/sec", cores=threads, kind="port",
+                one_thread_iters_per_sec=1.0 / best[1], threaded_speedup=best[1] / best[threads],
+                sample=f"the oracle's CPU engine (C++ restatement of the reference learner, quantized form) on all "
+                       f"{len(y)} rows x 28 of the same generator, {threads} host threads (one thread beside it): "
+                       f"best of {repeats} iterations after one warm iteration")
 
 
 def copy_calibrate(mib):
@@ -424,8 +477,10 @@ def main():
     ap.add_argument("--max-bin", type=int, default=255)
     ap.add_argument("--num-leaves", type=int, default=255)
     ap.add_argument("--prof-iters", type=int, default=3)
-    ap.add_argument("--cpu-rows", type=int, default=5_000_000)
-    ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--cpu-repeats", type=int, default=3, help="timed CPU iterations per thread count (best kept)")
+    ap.add_argument("--rounds", type=int, default=500,
+                    help="full training run per form (train loop timed, holdout AUC at its last round); 0 skips")
+    ap.add_argument("--also-max-bin", default="63", help="further max_bin values measured the same way")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="quant,plain")
     ap.add_argument("--no-graph", action="store_true", help="stream launches instead of graphs (counter passes)")
@@ -458,23 +513,38 @@ def main():
     t = time.perf_counter()
     ds = lgb.Dataset(X, label=y, params=base)
     log(f"host binning in {time.perf_counter() - t:.1f}s, {ds.num_features} features")
-    del X
     forms = {}
-    for form in args.forms.split(","):
-        params = dict(base)
+    datasets = {args.max_bin: ds}
+    extra_bins = [int(v) for v in args.also_max_bin.split(",") if v] if args.also_max_bin else []
+    for mb in extra_bins:
+        if mb not in datasets:
+            t = time.perf_counter()
+            datasets[mb] = lgb.Dataset(X, label=y, params=dict(base, max_bin=mb))
+            log(f"host binning max_bin={mb} in {time.perf_counter() - t:.1f}s")
+    runs = [(f, args.max_bin) for f in args.forms.split(",")] + \
+           [(f, mb) for mb in extra_bins for f in args.forms.split(",")]
+    for form, mb in runs:
+        params = dict(base, max_bin=mb)
         if form == "quant":
             params.update(use_quantized_grad=True, num_grad_quant_bins=16, stochastic_rounding=True,
                           quant_train_renew_leaf=True)
-        r = run_form(lgb, ds, params, args.warmup, args.steps, holdout_auc((Xv, yv)), args.prof_iters,
+        key = form if mb == args.max_bin else f"{form}_max_bin_{mb}"
+        r = run_form(lgb, datasets[mb], params, args.warmup, args.steps, holdout_auc((Xv, yv)), args.prof_iters,
                      args.prof_all)
         p = r["prof"]
         hist_gbs = p["hist_bytes"] / (p["hist_ms"] * 1e-3) / 1e9 if p["hist_ms"] > 0 else None
         r["hist_gbs"] = hist_gbs
         r["hist_avg_launch_us"] = 1000.0 * p["hist_ms"] / max(p["hist_launches"], 1)
         r["hist_bytes_per_launch"] = p["hist_bytes"] / max(p["hist_launches"], 1)
-        log(f"{form}: {r['iters_per_sec']:.2f} it/s ({r['ms_per_iter']:.2f} ms/it), hist {hist_gbs} GB/s, "
+        r["max_bin"] = mb
+        log(f"{key}: {r['iters_per_sec']:.2f} it/s ({r['ms_per_iter']:.2f} ms/it), hist {hist_gbs} GB/s, "
             f"prof {json.dumps(p)}, auc {r['auc']}")
-        forms[form] = r
+        if args.rounds > 0:
+            fr = full_run(lgb, datasets[mb], params, args.rounds, holdout_auc((Xv, yv)))
+            r["full"] = fr
+            log(f"{key}: {fr['rounds']} rounds in {fr['seconds']:.2f}s ({fr['iters_per_sec']:.2f} it/s), "
+                f"holdout auc {fr['quality']}")
+        forms[key] = r
 
     main_form = "quant" if "quant" in forms else list(forms)[0]
     m = forms[main_form]
@@ -501,12 +571,14 @@ def main():
         t = time.perf_counter()
         cparams = {k: v for k, v in base.items() if k not in ("device_type", "use_graph")}
         if main_form == "quant":
-            cparams.update(use_quantized_grad=True, num_grad_quant_bins=16, quant_train_renew_leaf=True)
+            cparams.update(use_quantized_grad=True, num_grad_quant_bins=16, stochastic_rounding=True,
+                           quant_train_renew_leaf=True)
         try:
-            cpu = cpu_baseline(args.cpu_rows, args.cpu_iters, cparams, args.rows)
+            cpu = cpu_baseline(X, y, cparams, host_cores(), args.cpu_repeats)
         except Exception as e:
             log(f"cpu baseline failed: {e}")
         log(f"cpu baseline in {time.perf_counter() - t:.1f}s")
+    del X
     out = {
         "metric": "boosting iters/sec (int8-quantized histogram form; FP64 form and hist GB/s beside it)",
         "value": m["iters_per_sec"],
@@ -528,7 +600,10 @@ def main():
         },
         "forms": {k: dict(iters_per_sec=v["iters_per_sec"], ms_per_iter=v["ms_per_iter"], hist_gbs=v["hist_gbs"],
                           hist_avg_launch_us=v["hist_avg_launch_us"], auc_holdout=v["auc"],
-                          iterations_trained=v["iterations"],
+                          iterations_trained=v["iterations"], max_bin=v["max_bin"],
+                          full_run=(dict(rounds=v["full"]["rounds"], seconds=v["full"]["seconds"],
+                                         iters_per_sec=v["full"]["iters_per_sec"],
+                                         auc_holdout=v["full"]["quality"]) if "full" in v else None),
                           split_ms_per_iter=v["prof"]["split_ms"] / max(v["prof"]["iters"], 1),
                           slab_ms_per_iter=v["prof"]["slab_ms"] / max(v["prof"]["iters"], 1),
                           part_ms_per_iter=v["prof"]["part_ms"] / max(v["prof"]["iters"], 1),
@@ -538,6 +613,21 @@ def main():
         "lds_atomics": lds,
         "cpu_baseline": cpu,
     }
+    # quantized against plain at each max_bin: the timed steps, the full runs
+    # (iters/sec over all rounds) and |delta AUC| on the holdout at the last round
+    cmp = {}
+    for mb in [args.max_bin] + extra_bins:
+        q, pl = ("quant", "plain") if mb == args.max_bin else (f"quant_max_bin_{mb}", f"plain_max_bin_{mb}")
+        if q in forms and pl in forms:
+            c = dict(steps_speedup=forms[q]["iters_per_sec"] / forms[pl]["iters_per_sec"])
+            if "full" in forms[q] and "full" in forms[pl]:
+                fq, fp = forms[q]["full"], forms[pl]["full"]
+                c["rounds"] = fq["rounds"]
+                c["full_run_speedup"] = fq["iters_per_sec"] / fp["iters_per_sec"]
+                if fq["quality"] is not None and fp["quality"] is not None:
+                    c["abs_delta_auc_at_last_round"] = abs(fq["quality"] - fp["quality"])
+            cmp[f"max_bin_{mb}"] = c
+    out["quant_vs_plain"] = cmp
     if "quant" in forms and "plain" in forms:
         out["quant_vs_plain_speedup"] = forms["quant"]["iters_per_sec"] / forms["plain"]["iters_per_sec"]
         if forms["quant"]["auc"] is not None and forms["plain"]["auc"] is not None:

@@ -23,6 +23,7 @@
  *   lgbmhip_train_tree        <- SerialTreeLearner::train       learner/mod.rs:572-695
  *   lgbmhip_tree_*            <- Tree arrays / SplitArgs        tree.rs:120-178, 254-297
  *   lgbmhip_copy_scores       <- Gbdt::train_scores             boosting.rs:339-341
+ *   lgbmhip_add_leaf_outputs  <- SerialTreeLearner::add_leaf_outputs learner/mod.rs:537, partition.rs:178-191
  *   lgbmhip_gradients         <- RowObjective::gradients        objective/mod.rs:52, binary.rs:101-127,
  *                                                               multiclass.rs:126-154, rank.rs:63-108
  *   lgbmhip_discretize        <- GradientDiscretizer::discretize learner/quant.rs:92-129
@@ -197,6 +198,11 @@ lgbmhip_status lgbmhip_tree_arrays(lgbmhip_ctx* ctx, int32_t t, int32_t* split_f
 /* Leaf index of every training row for the last tree (partition state). */
 lgbmhip_status lgbmhip_leaf_of_row(lgbmhip_ctx* ctx, int32_t* out);
 lgbmhip_status lgbmhip_copy_scores(lgbmhip_ctx* ctx, double* out);
+/* score[i] += leaf_values[leaf of row i in the last tree grown], for every
+ * training row (score: host, num_rows values; leaf_values: at least the
+ * tree's num_leaves values). */
+lgbmhip_status lgbmhip_add_leaf_outputs(lgbmhip_ctx* ctx, const double* leaf_values, int32_t num_values,
+                                        double* score);
 lgbmhip_status lgbmhip_copy_gradients(lgbmhip_ctx* ctx, float* grad, float* hess);
 lgbmhip_status lgbmhip_quant_state(lgbmhip_ctx* ctx, int8_t* gq, int8_t* hq, double* scales);
 

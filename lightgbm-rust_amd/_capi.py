@@ -97,6 +97,7 @@ SIGNATURES = {
     "lgbmhip_tree_num_leaves": (I32, [P, I32]),
     "lgbmhip_tree_arrays": (I32, [P, I32, P, P, P, P, P, P, P, P, P, P, P]),
     "lgbmhip_leaf_of_row": (I32, [P, P]),
+    "lgbmhip_add_leaf_outputs": (I32, [P, P, I32, P]),
     "lgbmhip_copy_scores": (I32, [P, P]),
     "lgbmhip_copy_gradients": (I32, [P, P, P]),
     "lgbmhip_quant_state": (I32, [P, P, P, P]),

@@ -301,7 +301,10 @@ struct Ctx {
   std::vector<StepDev*> h_st;
   // graphs: per class, one tree (init, root, rounds_main rounds, tail) and
   // a continuation (rounds_cont rounds, tail) for trees that need more rounds
-  std::vector<hipGraphExec_t> graphs, graphs_cont;
+  // per class: main-launch graphs by their round count, and the continuation graph
+  std::vector<std::vector<std::pair<int, hipGraphExec_t>>> graphs;
+  std::vector<hipGraphExec_t> graphs_cont;
+  std::vector<int> want_rounds;   // rounds the class's previous tree ran to completion
   int rounds_main = 24, rounds_cont = 8;
   std::vector<int> last_rounds;   // rounds launched for each class's last tree
   // model
@@ -359,7 +362,8 @@ struct Ctx {
     if (mws) metric_ws_free(mws);
     if (grad_ev) (void)hipEventDestroy(grad_ev);
     for (auto e : pev) if (e) (void)hipEventDestroy(e);
-    for (auto g : graphs) if (g) (void)hipGraphExecDestroy(g);
+    for (auto& v : graphs)
+      for (auto& g : v) (void)hipGraphExecDestroy(g.second);
     for (auto g : graphs_cont) if (g) (void)hipGraphExecDestroy(g);
     for (auto p : h_log) if (p) (void)hipHostFree(p);
     for (auto p : h_raw) if (p) (void)hipHostFree(p);
@@ -909,11 +913,13 @@ static void setup_training(Ctx* c) {
     HIP_CHECK(hipHostMalloc((void**)&c->h_raw[k], sizeof(double) * L));
     HIP_CHECK(hipHostMalloc((void**)&c->h_st[k], sizeof(StepDev)));
   }
-  for (auto g : c->graphs) if (g) (void)hipGraphExecDestroy(g);
+  for (auto& v : c->graphs)
+    for (auto& g : v) (void)hipGraphExecDestroy(g.second);
   for (auto g : c->graphs_cont) if (g) (void)hipGraphExecDestroy(g);
-  c->graphs.assign(K, nullptr);
+  c->graphs.assign(K, {});
   c->graphs_cont.assign(K, nullptr);
   c->last_rounds.assign(K, 0);
+  c->want_rounds.assign(K, 0);
   c->trees.clear();
   c->iter = 0;
   c->pred_dirty = true;
@@ -1159,23 +1165,37 @@ static hipGraphExec_t capture(hipStream_t s, const std::function<void()>& body) 
   return e;
 }
 
+// Rounds of class k's next main launch: at least rounds_main, else what its
+// previous tree ran to completion plus two, rounded up to a multiple of 8
+// (consecutive trees of a class take similar round counts, so most trees
+// finish in one launch; rounds after the tree is done do nothing).
+static int main_rounds(const Ctx* c, int k) {
+  const int want = c->want_rounds[k] > 0 ? c->want_rounds[k] + 2 : 0;
+  return std::max(c->rounds_main, (want + 7) / 8 * 8);
+}
+
 // Class k's tree on its lane: the main launch (graph or stream: init, root,
-// rounds_main rounds, tail), then a copy of the growth state for the host.
-static void launch_main(Ctx* c, int k, bool score_update, double lr) {
+// `rounds` rounds, tail), then a copy of the growth state for the host.
+static void launch_main(Ctx* c, int k, int rounds, bool score_update, double lr) {
   Lane& ln = lane_of(c, k);
   // a bag changes the root's size and rows: stream launches
   const bool graph = c->opts.use_graph && !c->profile && !c->bag_on && !c->goss_on;
   if (graph) {
-    if (!c->graphs[k])
-      c->graphs[k] = capture(ln.stream, [&] {
+    hipGraphExec_t e = nullptr;
+    for (auto& g : c->graphs[k])
+      if (g.first == rounds) e = g.second;
+    if (!e) {
+      e = capture(ln.stream, [&] {
         enqueue_head(c, ln, k);
-        enqueue_rounds(c, ln, k, 0, c->rounds_main);
+        enqueue_rounds(c, ln, k, 0, rounds);
         enqueue_tail(c, ln, k, score_update, lr);
       });
-    HIP_CHECK(hipGraphLaunch(c->graphs[k], ln.stream));
+      c->graphs[k].emplace_back(rounds, e);
+    }
+    HIP_CHECK(hipGraphLaunch(e, ln.stream));
   } else {
     enqueue_head(c, ln, k);
-    enqueue_rounds(c, ln, k, 0, c->rounds_main);
+    enqueue_rounds(c, ln, k, 0, rounds);
     enqueue_tail(c, ln, k, score_update, lr);
   }
   HIP_CHECK(hipGetLastError());
@@ -1215,8 +1235,8 @@ static void grow_trees(Ctx* c, int ntrees, bool score_update, double lr) {
   int active = 0;
   auto start = [&](int l, int k) {
     cur[l] = k;
-    launch_main(c, k, score_update, lr);
-    launched[l] = c->rounds_main;
+    launched[l] = main_rounds(c, k);
+    launch_main(c, k, launched[l], score_update, lr);
     active++;
   };
   for (int l = 0; l < NL; ++l) start(l, l);
@@ -1234,6 +1254,7 @@ static void grow_trees(Ctx* c, int ntrees, bool score_update, double lr) {
         continue;
       }
       c->last_rounds[k] = launched[l];
+      c->want_rounds[k] = c->h_st[k]->rounds;
       HIP_CHECK(hipMemcpyAsync(c->h_log[k], ln.log.p, sizeof(SplitRec) * c->L, hipMemcpyDeviceToHost, ln.stream));
       HIP_CHECK(hipMemcpyAsync(c->h_raw[k], c->leafraw.p + (size_t)k * c->L, sizeof(double) * c->L,
                                hipMemcpyDeviceToHost, ln.stream));
@@ -1784,7 +1805,10 @@ static int add_valid(Ctx* c, const uint8_t* bins, int64_t n, const float* label,
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->valid.push_back(std::move(v));
   // captured trees hold the score walks of the sets they were captured with
-  for (auto& g : c->graphs) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  for (auto& v : c->graphs) {
+    for (auto& g : v) (void)hipGraphExecDestroy(g.second);
+    v.clear();
+  }
   for (auto& g : c->graphs_cont) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
   return (int)c->valid.size();
 }
@@ -2580,6 +2604,33 @@ lgbmhip_status lgbmhip_leaf_of_row(lgbmhip_ctx* ctx, int32_t* out) {
     w.out = d.p;
     launch_walk(w, grid_for(c->N, 256, c->cus * 4), c->stream);
     HIP_CHECK(hipMemcpyAsync(out, d.p, c->N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+lgbmhip_status lgbmhip_add_leaf_outputs(lgbmhip_ctx* ctx, const double* leaf_values, int32_t num_values,
+                                        double* score) {
+  return guarded(ctx, [&] {
+    Ctx* c = &ctx->c;
+    if (!c->state_ready || c->trees.empty()) throw Error(LGBMHIP_INVALID_PARAMETER, "no tree trained");
+    if (!leaf_values || !score) throw Error(LGBMHIP_INVALID_PARAMETER, "no leaf values or scores");
+    const int nl = c->trees.back().num_leaves;
+    if (num_values < nl) throw Error(LGBMHIP_INVALID_PARAMETER, "fewer leaf values than the tree's leaves");
+    set_device(c);
+    DevBuf<int> leaf;
+    DevBuf<double> lv, sc;
+    leaf.alloc(c->N);
+    lv.alloc(nl);
+    sc.alloc(c->N);
+    // each row's leaf in the last tree grown (class K - 1's), as the partition left it
+    WalkArgs w = walk_args(c, lane_of(c, c->K - 1), 2);
+    w.out = leaf.p;
+    launch_walk(w, grid_for(c->N, 256, c->cus * 4), c->stream);
+    HIP_CHECK(hipMemcpyAsync(lv.p, leaf_values, (size_t)nl * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(sc.p, score, (size_t)c->N * 8, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_add_leaf_outputs, dim3(grid_for(c->N, 256, c->cus * 4)), dim3(256), 0, c->stream, leaf.p,
+                       lv.p, sc.p, (long long)c->N);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(score, sc.p, (size_t)c->N * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
 }

@@ -1475,6 +1475,14 @@ constexpr int kSpartTile = 1024;
 constexpr int kNoAllocScratchWords = 1 << 16;
 constexpr long long kNoAllocMaxRows = (long long)(kNoAllocScratchWords - 2) * kSpartTile;
 
+// Stage entry point: score[i] += leaf_values[leaf[i]] (partition.rs:178-191),
+// one add per row, so the order of rows does not matter.
+__global__ void k_add_leaf_outputs(const int* __restrict__ leaf, const double* __restrict__ lv,
+                                   double* __restrict__ score, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    score[i] += lv[leaf[i]];
+}
+
 // words p[0..n) = v (a kernel, so a caller's graph capture records no memset node)
 __global__ void k_fill_u32(unsigned* p, long long n, unsigned v) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)

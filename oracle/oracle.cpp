@@ -33,6 +33,8 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
+#include <omp.h>
 #include <set>
 #include <string>
 #include <unordered_map>
@@ -1294,6 +1296,15 @@ static bool find_in_bitset(const std::vector<uint32_t>& bits, int pos) {
   return ((bits[i1] >> (pos % 32)) & 1) == 1;
 }
 
+// Host threads of the CPU engine (orc_set_num_threads; 1 by default). Every
+// parallel loop below splits work whose results do not depend on the split:
+// histograms by feature (each bin still sums its rows in partition order),
+// threshold searches by feature (the best split is then picked in feature
+// order), the stable partition by row chunks, per-row gradients and
+// quantization, per-leaf score updates and renewals. So trees and scores are
+// bit-identical at every thread count.
+static int g_threads = 1;
+
 // ---------------------------------------------------------------- quantizer
 // quant.rs:37-40
 static inline int8_t to_i8(double x) {
@@ -1359,6 +1370,8 @@ struct Discretizer {
   void discretize(const float* grad, const float* hess, int64_t n) {
     if (n == 0) return;
     double mg = std::fabs((double)grad[0]), mh = std::fabs((double)hess[0]);
+    // a maximum does not depend on the order of its operands
+#pragma omp parallel for num_threads(g_threads) schedule(static) reduction(max : mg, mh) if (g_threads > 1)
     for (int64_t i = 0; i < n; ++i) {
       double g = std::fabs((double)grad[i]), h = std::fabs((double)hess[i]);
       if (g > mg) mg = g;
@@ -1368,6 +1381,7 @@ struct Discretizer {
     hess_scale = constant_hessian ? mh : mh / (double)bins;
     double inv_g = 1.0 / grad_scale, inv_h = 1.0 / hess_scale;
     int64_t start = start_eng.uniform_int(0, start_max);
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
     for (int64_t i = 0; i < n; ++i) {
       double g = (double)grad[i];
       double rg = 0.5, rh = 0.5;
@@ -1393,6 +1407,7 @@ struct TraceEntry { int leaf, feature; uint32_t threshold; double gain; };
 
 // learner/mod.rs:86-153, 572-706, 816-1225 (single thread, no bagging,
 // feature_fraction = 1, no constraints/CEGB/forced splits/linear trees)
+
 struct Learner {
   const Dataset* ds;
   Params prm;
@@ -1456,6 +1471,27 @@ struct Learner {
   void construct_f64(int leaf, const float* g, const float* h, std::vector<double>& out) {
     out.assign((size_t)F * B * 2, 0.0);
     int64_t b = leaf_begin[leaf], c = leaf_count[leaf];
+    if (g_threads > 1 && c >= 4096) {
+      // by feature range: one pass over the rows per thread, and every bin of
+      // its features sums the rows in the same order as the loop below
+      const int T = std::min(g_threads, F);
+#pragma omp parallel num_threads(T)
+      {
+        const int t = omp_get_thread_num(), f0 = F * t / T, f1 = F * (t + 1) / T;
+        for (int64_t k = 0; k < c; ++k) {
+          uint32_t r = indices[b + k];
+          double gg = (double)g[r], hh = (double)h[r];
+          for (int f = f0; f < f1; ++f) {
+            uint32_t bin = ds->bin(r, f);
+            if (bin == metas[f].most_freq_bin) continue;
+            size_t o = ((size_t)f * B + bin) * 2;
+            out[o] += gg;
+            out[o + 1] += hh;
+          }
+        }
+      }
+      return;
+    }
     for (int64_t k = 0; k < c; ++k) {
       uint32_t r = indices[b + k];
       double gg = (double)g[r], hh = (double)h[r];
@@ -1472,6 +1508,30 @@ struct Learner {
   void construct_int(int leaf, std::vector<int64_t>& out) {
     out.assign((size_t)F * B, 0);
     int64_t b = leaf_begin[leaf], c = leaf_count[leaf];
+    if (g_threads > 1 && c >= 4096) {
+      // by row chunks into private histograms, then summed: wrapping integer
+      // sums do not depend on the order of their terms
+      const int T = g_threads;
+      std::vector<std::vector<int64_t>> part(T);
+#pragma omp parallel num_threads(T)
+      {
+        const int t = omp_get_thread_num();
+        std::vector<int64_t>& o = part[t];
+        o.assign((size_t)F * B, 0);
+        for (int64_t k = c * t / T; k < c * (t + 1) / T; ++k) {
+          uint32_t r = indices[b + k];
+          int64_t v = quant->packed[r];
+          for (int f = 0; f < F; ++f) {
+            uint32_t bin = ds->bin(r, f);
+            if (bin == metas[f].most_freq_bin) continue;
+            o[(size_t)f * B + bin] = wadd(o[(size_t)f * B + bin], v);
+          }
+        }
+      }
+      for (int t = 0; t < T; ++t)
+        for (size_t i = 0; i < out.size(); ++i) out[i] = wadd(out[i], part[t][i]);
+      return;
+    }
     for (int64_t k = 0; k < c; ++k) {
       uint32_t r = indices[b + k];
       int64_t v = quant->packed[r];
@@ -1503,6 +1563,10 @@ struct Learner {
     double lpo = larger.leaf >= 0 ? parent_output(tree, larger) : 0.0;
     SplitInfo s_best, l_best;
     std::vector<char> s_spl(F, 0), l_spl = right >= 0 ? parent_spl : std::vector<char>(F, 0);
+    std::vector<SplitInfo> s_res(F), l_res(F);
+    std::vector<char> s_has(F, 0);
+    // each feature's search is independent; the best is picked below in feature order
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1) if (g_threads > 1)
     for (int f = 0; f < F; ++f) {
       bool used = right >= 0 ? parent_spl[f] != 0 : true;
       if (!used) { s_spl[f] = 0; continue; }
@@ -1539,12 +1603,16 @@ struct Learner {
           ls.feature = real;
         }
       }
-      if (ss.better_than(s_best)) s_best = ss;
+      s_res[f] = ss;
+      l_res[f] = ls;
+      s_has[f] = 1;
       s_spl[f] = s_ok;
-      if (larger.leaf >= 0) {
-        if (ls.better_than(l_best)) l_best = ls;
-        l_spl[f] = l_ok;
-      }
+      if (larger.leaf >= 0) l_spl[f] = l_ok;
+    }
+    for (int f = 0; f < F; ++f) {
+      if (!s_has[f]) continue;
+      if (s_res[f].better_than(s_best)) s_best = s_res[f];
+      if (larger.leaf >= 0 && l_res[f].better_than(l_best)) l_best = l_res[f];
     }
     best[sm] = s_best;
     if (q) ipool[sm] = sibuf; else pool[sm] = sbuf;
@@ -1581,17 +1649,45 @@ struct Learner {
   // partition.rs:107-174 (one thread)
   int64_t partition_split(int leaf, int f, const std::vector<char>& lut, int right_leaf) {
     int64_t b = leaf_begin[leaf], c = leaf_count[leaf];
-    std::vector<uint32_t> L, R;
-    for (int64_t k = 0; k < c; ++k) {
-      uint32_t r = indices[b + k];
-      if (lut[ds->bin(r, f)]) L.push_back(r); else R.push_back(r);
+    int64_t nl = 0;
+    if (g_threads > 1 && c >= 8192) {
+      // row chunks in order: each chunk's left / right rows go after the
+      // earlier chunks' (the same stable partition as one pass)
+      const int T = g_threads;
+      std::vector<int64_t> lc(T + 1, 0);
+      std::unique_ptr<uint32_t[]> tmp(new uint32_t[(size_t)c]);
+#pragma omp parallel num_threads(T)
+      {
+        const int t = omp_get_thread_num();
+        const int64_t lo = c * t / T, hi = c * (t + 1) / T;
+        int64_t k = 0;
+        for (int64_t i = lo; i < hi; ++i) k += lut[ds->bin(indices[b + i], f)] ? 1 : 0;
+        lc[t + 1] = k;
+#pragma omp barrier
+#pragma omp single
+        for (int u = 0; u < T; ++u) lc[u + 1] += lc[u];
+        int64_t lp = lc[t], rp = lc[T] + (lo - lc[t]);
+        for (int64_t i = lo; i < hi; ++i) {
+          const uint32_t r = indices[b + i];
+          if (lut[ds->bin(r, f)]) tmp[lp++] = r; else tmp[rp++] = r;
+        }
+      }
+      std::memcpy(indices.data() + b, tmp.get(), (size_t)c * 4);
+      nl = lc[T];
+    } else {
+      std::vector<uint32_t> L, R;
+      for (int64_t k = 0; k < c; ++k) {
+        uint32_t r = indices[b + k];
+        if (lut[ds->bin(r, f)]) L.push_back(r); else R.push_back(r);
+      }
+      std::copy(L.begin(), L.end(), indices.begin() + b);
+      std::copy(R.begin(), R.end(), indices.begin() + b + (int64_t)L.size());
+      nl = (int64_t)L.size();
     }
-    std::copy(L.begin(), L.end(), indices.begin() + b);
-    std::copy(R.begin(), R.end(), indices.begin() + b + (int64_t)L.size());
-    leaf_count[leaf] = (int64_t)L.size();
-    leaf_begin[right_leaf] = b + (int64_t)L.size();
-    leaf_count[right_leaf] = c - (int64_t)L.size();
-    return (int64_t)L.size();
+    leaf_count[leaf] = nl;
+    leaf_begin[right_leaf] = b + nl;
+    leaf_count[right_leaf] = c - nl;
+    return nl;
   }
 
   // learner/mod.rs:1105-1225
@@ -1689,6 +1785,7 @@ struct Learner {
     }
     if (q && prm.quant_train_renew_leaf) {
       // learner/mod.rs:736-747
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1) if (g_threads > 1)
       for (int leaf = 0; leaf < tree.num_leaves; ++leaf) {
         double a = 0, b = 0;
         for (int64_t k = 0; k < leaf_count[leaf]; ++k) {
@@ -1728,6 +1825,7 @@ struct Learner {
 
   // partition.rs:178-191
   void add_leaf_outputs(const Tree& t, double* score) {
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1) if (g_threads > 1)
     for (int leaf = 0; leaf < t.num_leaves; ++leaf)
       for (int64_t k = 0; k < leaf_count[leaf]; ++k) score[indices[leaf_begin[leaf] + k]] += t.leaf_value[leaf];
   }
@@ -1736,6 +1834,7 @@ struct Learner {
 // ---------------------------------------------------------------- objectives
 // objective/binary.rs:101-127
 static void binary_gradients(const double* s, const float* label, int64_t n, double sigmoid, float* g, float* h) {
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
   for (int64_t i = 0; i < n; ++i) {
     bool pos = label[i] > 0.0f;
     double lab = pos ? 1.0 : -1.0;
@@ -2117,6 +2216,8 @@ static int booster_train_one_iter(OrcBooster* b) {
 extern "C" {
 
 int orc_version() { return 1; }
+// Host threads of the CPU engine's loops (results are the same at any count).
+void orc_set_num_threads(int n) { g_threads = std::max(1, n); }
 void orc_block_info_force_size(int num_threads, int64_t n, int64_t min_size, int64_t* nblock, int64_t* block) {
   block_info_force_size(num_threads, n, min_size, nblock, block);
 }

@@ -84,6 +84,7 @@ def lib():
             "orc_metric_point": (D, [I, PD, PF, P, I64, I, D, I]),
             "orc_metric_ndcg": (None, [PD, PF, PI, I, PI, I, I, PD]),
             "orc_model_load": (P, [C.c_char_p]),
+            "orc_set_num_threads": (None, [I]),
             "orc_goss": (I64, [PF, PF, I64, I, I, D, D, I, PU]),
             "orc_block_info_force_size": (None, [I, I64, I64, C.POINTER(I64), C.POINTER(I64)]),
         }
@@ -273,6 +274,11 @@ def block_info_force_size(num_threads, n, min_size):
     a, b = I64(), I64()
     lib().orc_block_info_force_size(num_threads, n, min_size, C.byref(a), C.byref(b))
     return a.value, b.value
+
+
+def set_num_threads(n):
+    """Host threads of the oracle's loops (trees and scores do not depend on it)."""
+    lib().orc_set_num_threads(int(n))
 
 
 def goss(g, h, K, num_threads, top_rate, other_rate, seed):

@@ -292,6 +292,28 @@ def test_partition_is_stable_and_exact():
             assert nl[0] == left.sum()
 
 
+def test_add_leaf_outputs_follows_the_partition():
+    # partition.rs:178-191: score[i] += leaf_values[leaf of row i], with the
+    # rows' leaves those of the oracle's tree grown on the same gradients
+    X, y = datagen.binary_100k(50000)
+    params = dict(P_BIN, num_leaves=63, max_bin=255)
+    ds, b = _booster(X, y, params)
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    rng = np.random.default_rng(16)
+    g, h = O.binary_gradients(rng.normal(size=len(y)), y)
+    b.train_tree(g, h)
+    ob.train_tree(g, h)
+    nl = b.tree(0).num_leaves
+    lv = rng.normal(size=nl)
+    score = rng.normal(size=len(y))
+    out = score.copy()
+    lib = lgb._capi.lib()
+    lgb._capi.check(lib.lgbmhip_add_leaf_outputs(b._ctx, lgb.ptr(lv), nl, lgb.ptr(out)), b._ctx)
+    np.testing.assert_array_equal(out, score + lv[ob.leaf_of_row()])
+    st = lib.lgbmhip_add_leaf_outputs(b._ctx, lgb.ptr(lv), nl - 1, lgb.ptr(out))
+    assert st == 1
+
+
 def test_binary_gradients_match_oracle():
     X, y = datagen.binary_100k(200000)
     ds, b = _booster(X, y, P_BIN)

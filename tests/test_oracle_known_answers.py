@@ -759,3 +759,36 @@ def test_goss_sample_properties_in_boosting(K, threads):
     g0, h0 = O.binary_gradients(scores, y) if K == 1 else O.multiclass_gradients(scores, y, K)
     g0, h0, g1, h1 = (a.reshape(K, n) for a in (g0, h0, g1, h1))
     _check_goss_block_properties(_abs_gh(g0, h0, K), g0, h0, g1, h1, ob.bag(), n, K, threads, 0.2, 0.15)
+
+
+@pytest.mark.parametrize("form", ["plain", "quant", "multiclass_cat"])
+def test_oracle_threads_do_not_change_trees_or_scores(form):
+    """The threaded CPU engine (histograms and searches by feature, stable
+    partition by row chunks, per-row and per-leaf loops) gives the
+    single-thread trees and scores bit for bit."""
+    rng = np.random.default_rng(40)
+    n = 150000
+    X = rng.normal(size=(n, 10)).astype(np.float32)
+    p = dict(objective="binary", num_leaves=63, max_bin=255, min_data_in_leaf=20, learning_rate=0.1)
+    if form == "quant":
+        p.update(use_quantized_grad=True, num_grad_quant_bins=16, quant_train_renew_leaf=True)
+    if form == "multiclass_cat":
+        X[:, 3] = rng.integers(0, 30, n)
+        y = np.argmax(X[:, :3] + 0.1 * (X[:, 3:4] % 7) + rng.gumbel(size=(n, 3)), 1).astype(np.float32)
+        p.update(objective="multiclass", num_class=3, categorical_feature=[3])
+    else:
+        y = (X[:, 0] + 0.5 * X[:, 1] * X[:, 2] + rng.normal(size=n) > 0).astype(np.float32)
+    runs = []
+    try:
+        for threads in (1, 4):
+            O.set_num_threads(threads)
+            ob = O.Booster(O.Dataset(X, y, p), p)
+            for _ in range(3):
+                ob.update()
+            runs.append((ob.scores(), [ob.tree(t) for t in range(ob.num_trees())]))
+    finally:
+        O.set_num_threads(1)
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    for a, b in zip(runs[0][1], runs[1][1]):
+        for f in ("split_feature", "threshold_in_bin", "left_child", "right_child", "leaf_value", "leaf_count"):
+            np.testing.assert_array_equal(getattr(a, f), getattr(b, f))
