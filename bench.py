@@ -87,9 +87,11 @@ def run_form(lgb, ds, params, warmup, steps, quality, prof_iters, prof_all=False
     iters_total = warmup + 1 + steps + prof_iters
     tun = b.get_tuning()
     lds_peak = b.lds_atomic_peak()
+    autotuned = b.tuning
     b.free()
     return dict(iters_per_sec=its, ms_per_iter=1000.0 / its, prof=prof, auc=auc, iterations=iters_total,
-                hist_fb=int(tun["hist_features_per_block"]), hist_ng=int(tun["hist_groups"]), lds_peak=lds_peak)
+                hist_fb=int(tun["hist_features_per_block"]), hist_ng=int(tun["hist_groups"]), lds_peak=lds_peak,
+                tuning=dict(tun, autotune=autotuned))
 
 
 def host_cores():
@@ -484,6 +486,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="quant,plain")
     ap.add_argument("--no-graph", action="store_true", help="stream launches instead of graphs (counter passes)")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
     ap.add_argument("--prof-all", action="store_true",
                     help="count every iteration's histogram bytes, warm-up included (counter passes)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
@@ -509,7 +512,7 @@ def main():
     Xv, yv = datagen.holdout("binary_10p5m", 200_000)
     log(f"data {X.shape} in {time.perf_counter() - t:.1f}s")
     base = dict(objective="binary", num_leaves=args.num_leaves, max_bin=args.max_bin, min_data_in_leaf=20,
-                learning_rate=0.1, device_type="gpu", use_graph=not args.no_graph)
+                learning_rate=0.1, device_type="gpu", use_graph=not args.no_graph, autotune=not args.no_autotune)
     t = time.perf_counter()
     ds = lgb.Dataset(X, label=y, params=base)
     log(f"host binning in {time.perf_counter() - t:.1f}s, {ds.num_features} features")
@@ -569,7 +572,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
-        cparams = {k: v for k, v in base.items() if k not in ("device_type", "use_graph")}
+        cparams = {k: v for k, v in base.items() if k not in ("device_type", "use_graph", "autotune")}
         if main_form == "quant":
             cparams.update(use_quantized_grad=True, num_grad_quant_bins=16, stochastic_rounding=True,
                            quant_train_renew_leaf=True)
@@ -600,7 +603,7 @@ def main():
         },
         "forms": {k: dict(iters_per_sec=v["iters_per_sec"], ms_per_iter=v["ms_per_iter"], hist_gbs=v["hist_gbs"],
                           hist_avg_launch_us=v["hist_avg_launch_us"], auc_holdout=v["auc"],
-                          iterations_trained=v["iterations"], max_bin=v["max_bin"],
+                          iterations_trained=v["iterations"], max_bin=v["max_bin"], launch_shapes=v["tuning"],
                           full_run=(dict(rounds=v["full"]["rounds"], seconds=v["full"]["seconds"],
                                          iters_per_sec=v["full"]["iters_per_sec"],
                                          auc_holdout=v["full"]["quality"]) if "full" in v else None),

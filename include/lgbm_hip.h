@@ -357,6 +357,22 @@ lgbmhip_status lgbmhip_lds_atomic_peak(lgbmhip_ctx* ctx, double* lane_atomics_pe
  * launch-shape sweeps on other row/feature shapes (Booster.set_tuning). */
 lgbmhip_status lgbmhip_set_tuning(lgbmhip_ctx* ctx, int32_t part_grid, int32_t hist_chunks, int32_t hist_min_rows,
                                   int32_t hist_lds_budget, int32_t rounds_main, int32_t rounds_cont);
+/* The autotuner: before the first iteration, times boosting iterations of
+ * the workload itself (trees do not depend on launch shapes) at candidate
+ * launch shapes, by coordinate descent over the histogram LDS budget
+ * (features per block and the row layout), histogram row chunks, minimum
+ * rows per chunk and the partition grid: each candidate runs two warm and
+ * trees_per_candidate timed iterations from the starting scores, the
+ * training state is reset after each, and the fastest is kept (a change
+ * must win by 2%). The choice is cached per
+ * process for the workload shape (rows, used features, bins, form,
+ * classes, leaves, CUs): a second context of the same shape takes it
+ * without timing. out5 (may be NULL): {part_grid, hist_chunks,
+ * hist_min_rows, hist_lds_budget (0: default), candidates timed (0: from
+ * the cache)}. No counterpart in the reference, whose only shape rule is
+ * its row-block size from the thread count (multi_val_bin.rs:337-351).
+ * Unsupported with bagging or GOSS. */
+lgbmhip_status lgbmhip_autotune(lgbmhip_ctx* ctx, int32_t trees_per_candidate, int32_t* out5);
 /* Current launch shapes: {part_grid, hist_chunks, hist_min_rows, hist_lds_budget, features per hist block,
  * hist feature groups, threads per row, rounds_main, rounds_cont}. */
 lgbmhip_status lgbmhip_get_tuning(lgbmhip_ctx* ctx, int32_t* out9);

@@ -33,7 +33,7 @@ DEFAULT_PARAMS = dict(
     metric="", eval_at=(1, 2, 3, 4, 5), multi_error_top_k=1, label_gain=None, early_stopping_round=0,
     first_metric_only=False, early_stopping_min_delta=0.0, device_binning=False, bagging_fraction=1.0,
     bagging_freq=0, bagging_seed=3, pos_bagging_fraction=1.0, neg_bagging_fraction=1.0,
-    data_sample_strategy="bagging", top_rate=0.2, other_rate=0.1,
+    data_sample_strategy="bagging", top_rate=0.2, other_rate=0.1, autotune=False, autotune_trees=4,
 )
 
 ALIASES = {
@@ -266,6 +266,13 @@ class Booster:
                              int(p["num_threads"]) if int(p["num_threads"]) > 0 else 4096,
                              float(p["top_rate"]), float(p["other_rate"]))
         check(lib().lgbmhip_set_train_opts(h, C.byref(to)), h)
+        self.tuning = None
+        if _b(p["autotune"]):
+            # launch shapes timed on this workload's first trees (cached per process and shape)
+            out = np.zeros(5, np.int32)
+            check(lib().lgbmhip_autotune(h, int(p["autotune_trees"]), ptr(out)), h)
+            self.tuning = dict(zip(("part_grid", "hist_chunks", "hist_min_rows", "hist_lds", "candidates_timed"),
+                                   out.tolist()))
         self._finished = False
         self._metrics = _metrics(p)
         self._valid = []          # (name, index, Dataset)

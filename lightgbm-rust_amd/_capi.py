@@ -126,6 +126,7 @@ SIGNATURES = {
     "lgbmhip_profile": (I32, [P, I32, P]),
     "lgbmhip_lds_atomic_peak": (I32, [P, P]),
     "lgbmhip_set_tuning": (I32, [P, I32, I32, I32, I32, I32, I32]),
+    "lgbmhip_autotune": (I32, [P, I32, P]),
     "lgbmhip_get_tuning": (I32, [P, P]),
 }
 

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cctype>
 #include <chrono>
 #include <cmath>
@@ -15,9 +16,11 @@
 #include <cstring>
 #include <functional>
 #include <limits>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "engine.h"
@@ -301,9 +304,16 @@ struct Ctx {
   std::vector<StepDev*> h_st;
   // graphs: per class, one tree (init, root, rounds_main rounds, tail) and
   // a continuation (rounds_cont rounds, tail) for trees that need more rounds
-  // per class: main-launch graphs by their round count, and the continuation graph
-  std::vector<std::vector<std::pair<int, hipGraphExec_t>>> graphs;
-  std::vector<hipGraphExec_t> graphs_cont;
+  // per class: captured main launches and continuations, by (rounds, score update, learning rate)
+  struct GraphKey {
+    int rounds;
+    bool score_update;
+    double lr;
+    bool operator==(const GraphKey& o) const {
+      return rounds == o.rounds && score_update == o.score_update && lr == o.lr;
+    }
+  };
+  std::vector<std::vector<std::pair<GraphKey, hipGraphExec_t>>> graphs, graphs_cont;
   std::vector<int> want_rounds;   // rounds the class's previous tree ran to completion
   int rounds_main = 24, rounds_cont = 8;
   std::vector<int> last_rounds;   // rounds launched for each class's last tree
@@ -362,9 +372,9 @@ struct Ctx {
     if (mws) metric_ws_free(mws);
     if (grad_ev) (void)hipEventDestroy(grad_ev);
     for (auto e : pev) if (e) (void)hipEventDestroy(e);
-    for (auto& v : graphs)
-      for (auto& g : v) (void)hipGraphExecDestroy(g.second);
-    for (auto g : graphs_cont) if (g) (void)hipGraphExecDestroy(g);
+    for (auto* gs : {&graphs, &graphs_cont})
+      for (auto& v : *gs)
+        for (auto& g : v) (void)hipGraphExecDestroy(g.second);
     for (auto p : h_log) if (p) (void)hipHostFree(p);
     for (auto p : h_raw) if (p) (void)hipHostFree(p);
     for (auto p : h_st) if (p) (void)hipHostFree(p);
@@ -386,8 +396,12 @@ static int grid_for(int64_t n, int threads, int cap) {
 
 // Buffer-0 bins of rows[0..m) (every row when rows is null) into the
 // group-major layout of the ordered buffers (row stride N), on stream s.
-static void launch_group_rows(Ctx* c, const unsigned* rows, int64_t m, uint8_t* out, hipStream_t s) {
+static void launch_group_rows(Ctx* c, const unsigned* rows, int64_t m, uint8_t* out, size_t out_bytes,
+                              hipStream_t s) {
   if (m <= 0) return;
+  // the kernels write groups [NG][N][FB] at row stride N: the buffer must hold them
+  if ((size_t)c->hist_ng * (size_t)c->N * (size_t)c->hist_fb > out_bytes)
+    throw Error(LGBMHIP_INTERNAL, "group-major row buffer smaller than the feature groups");
   const int CH = c->hist_ng * c->hist_fb / 16;
   const bool vec = c->S % 16 == 0 && c->hist_fb % 16 == 0 && c->hist_ng * c->hist_fb <= c->S && CH >= 1 && CH <= 8;
   if (vec) {
@@ -677,6 +691,65 @@ static void reset_valid_scores(Ctx* c, ValidData& v) {
   else v.score.zero(c->stream);
 }
 
+// Destroys every captured tree launch (their kernels' arguments or the
+// validation sets they update changed).
+static void drop_graphs(Ctx* c) {
+  for (auto* gs : {&c->graphs, &c->graphs_cont})
+    for (auto& v : *gs) {
+      for (auto& g : v) (void)hipGraphExecDestroy(g.second);
+      v.clear();
+    }
+}
+
+// Launch shapes of the tree-growth kernels (the tuning fields, else the
+// defaults) and the buffers whose layout depends on them: the histogram
+// feature groups, units and slabs, and the lanes' group-major row buffers.
+// Buffer 0 (the root's rows) is gathered by the caller.
+static void set_shapes(Ctx* c, bool quant) {
+  const int64_t N = c->N;
+  const int B = c->B;
+  // defaults from sweeps on 10.5M x 28 (quantized and FP64 forms)
+  c->part_grid = (int)std::min<int64_t>((N + part_tile() - 1) / part_tile(),
+                                        c->tune_part_grid > 0 ? c->tune_part_grid : 4 * c->cus);
+  c->hist_chunks = c->tune_hist_chunks > 0 ? c->tune_hist_chunks : (quant ? 2 : 1) * c->cus;
+  // quantized: chunks no longer than an epoch of the 32-bit histogram bins
+  // for the quantizer's range (|gq| <= bins / 2, hq <= bins): 4095 rows for
+  // 16 bins, so a chunk's bins go to its slab without an int64 fold
+  int rows_q = 4096;
+  if (quant) {
+    const int qb = std::max(2, c->opts.num_grad_quant_bins);
+    const int epoch = std::min(32767 / std::max(1, qb / 2), 65535 / qb);
+    if (epoch >= 2048) rows_q = std::min(rows_q, epoch);
+  }
+  c->hist_min_rows = c->tune_hist_min_rows > 0 ? c->tune_hist_min_rows : rows_q;
+  plan_hist(c, quant);
+  // histogram kernels address a chunk's rows with 32-bit byte offsets. A leaf
+  // of cnt rows takes min(hist_chunks, ceil(cnt / hist_min_rows)) chunks, so
+  // no chunk is longer than max(hist_min_rows, ceil(N / hist_chunks)) rows.
+  const int64_t chunk_max = std::max<int64_t>(c->hist_min_rows, (N + c->hist_chunks - 1) / c->hist_chunks);
+  if (chunk_max * (int64_t)c->hist_fb >= ((int64_t)1 << 31))
+    throw Error(LGBMHIP_UNSUPPORTED,
+                "histogram chunks too long for 32-bit row offsets: raise hist_chunks or lower hist_min_rows");
+  std::vector<int> au(kMaxBatch + 1);
+  for (int w = 0; w <= kMaxBatch; ++w) au[w] = w * c->hist_ng;
+  // histogram units of one pass: at most (rows / chunk + splits) per group
+  const int64_t max_units = ((N + c->hist_min_rows - 1) / c->hist_min_rows + kMaxBatch + 1) * c->hist_ng;
+  const size_t gbytes = (size_t)c->hist_ng * N * c->hist_fb;
+  for (auto& lp : c->lanes) {
+    Lane& ln = *lp;
+    ln.acc.alloc((size_t)kMaxBatch * c->hist_ng * c->hist_fb * B * (quant ? 1 : 2));
+    HIP_CHECK(hipMemcpy(ln.acc_units.p, au.data(), au.size() * sizeof(int), hipMemcpyHostToDevice));
+    ln.slab.alloc((size_t)max_units * c->hist_fb * B * (quant ? 1 : 2));
+    for (int b = 1; b < 3; ++b) ln.grp[b].alloc(gbytes);
+  }
+  // buffer 0 (the root's rows, gathered by the caller) and a bag's buffer 0
+  // in the same group layout
+  if (c->grp0.n != gbytes) c->grp0.alloc(gbytes);
+  if (c->bag_grp0.n && c->bag_grp0.n != gbytes) c->bag_grp0.alloc(gbytes);
+  // captured trees hold the previous shapes' launches
+  drop_graphs(c);
+}
+
 static void setup_training(Ctx* c) {
   if (!c->N) throw Error(LGBMHIP_INVALID_DATA, "no bins uploaded");
   if (c->label.empty()) throw Error(LGBMHIP_INVALID_DATA, "no labels set");
@@ -749,20 +822,6 @@ static void setup_training(Ctx* c) {
   c->leafval.alloc((size_t)K * L);
   c->leafraw.alloc((size_t)K * L);
   set_part_lds_limit(part_lds(L));
-  // defaults from sweeps on 10.5M x 28 (quantized and FP64 forms)
-  c->part_grid = (int)std::min<int64_t>((N + part_tile() - 1) / part_tile(),
-                                        c->tune_part_grid > 0 ? c->tune_part_grid : 4 * c->cus);
-  c->hist_chunks = c->tune_hist_chunks > 0 ? c->tune_hist_chunks : (quant ? 2 : 1) * c->cus;
-  // quantized: chunks no longer than an epoch of the 32-bit histogram bins
-  // for the quantizer's range (|gq| <= bins / 2, hq <= bins): 4095 rows for
-  // 16 bins, so a chunk's bins go to its slab without an int64 fold
-  int rows_q = 4096;
-  if (quant) {
-    const int qb = std::max(2, c->opts.num_grad_quant_bins);
-    const int epoch = std::min(32767 / std::max(1, qb / 2), 65535 / qb);
-    if (epoch >= 2048) rows_q = std::min(rows_q, epoch);
-  }
-  c->hist_min_rows = c->tune_hist_min_rows > 0 ? c->tune_hist_min_rows : rows_q;
   if (quant) {
     // quant.rs:45-79: per-row tables from block generators of `quant_table_threads` blocks
     std::vector<double> grv(N), hrv(N);
@@ -793,29 +852,9 @@ static void setup_training(Ctx* c) {
     c->hq.alloc(N);
     c->n_calls = 0;
   }
-  plan_hist(c, quant);
-  // histogram kernels address a chunk's rows with 32-bit byte offsets. A leaf
-  // of cnt rows takes min(hist_chunks, ceil(cnt / hist_min_rows)) chunks, so
-  // no chunk is longer than max(hist_min_rows, ceil(N / hist_chunks)) rows.
+  set_shapes(c, quant);
   {
-    const int64_t chunk_max = std::max<int64_t>(c->hist_min_rows, (N + c->hist_chunks - 1) / c->hist_chunks);
-    if (chunk_max * (int64_t)c->hist_fb >= ((int64_t)1 << 31))
-      throw Error(LGBMHIP_UNSUPPORTED,
-                  "histogram chunks too long for 32-bit row offsets: raise hist_chunks or lower hist_min_rows");
-  }
-  {
-    std::vector<int> au(kMaxBatch + 1);
-    for (int w = 0; w <= kMaxBatch; ++w) au[w] = w * c->hist_ng;
-    // histogram units of one pass: at most (rows / chunk + splits) per group
-    const int64_t max_units = ((N + c->hist_min_rows - 1) / c->hist_min_rows + kMaxBatch + 1) * c->hist_ng;
     const size_t gbytes = (size_t)c->hist_ng * N * c->hist_fb;
-    for (auto& lp : c->lanes) {
-      Lane& ln = *lp;
-      ln.acc.alloc((size_t)kMaxBatch * c->hist_ng * c->hist_fb * B * (quant ? 1 : 2));
-      HIP_CHECK(hipMemcpy(ln.acc_units.p, au.data(), au.size() * sizeof(int), hipMemcpyHostToDevice));
-      ln.slab.alloc((size_t)max_units * c->hist_fb * B * (quant ? 1 : 2));
-      for (int b = 1; b < 3; ++b) ln.grp[b].alloc(gbytes);
-    }
     // row bagging: Bagger setup (sample_strategy.rs:186-214); the feature
     // count stands for the reference's feature groups (dense data)
     // balanced bagging needs the objective's positive count (binary.rs:73-99; 0 otherwise)
@@ -900,7 +939,7 @@ static void setup_training(Ctx* c) {
     }
     // buffer 0: the root's rows in row order, bins group-major
     c->grp0.alloc(gbytes);
-    launch_group_rows(c, nullptr, N, c->grp0.p, c->stream);
+    launch_group_rows(c, nullptr, N, c->grp0.p, c->grp0.n, c->stream);
   }
   for (auto p : c->h_log) if (p) (void)hipHostFree(p);
   for (auto p : c->h_raw) if (p) (void)hipHostFree(p);
@@ -913,11 +952,9 @@ static void setup_training(Ctx* c) {
     HIP_CHECK(hipHostMalloc((void**)&c->h_raw[k], sizeof(double) * L));
     HIP_CHECK(hipHostMalloc((void**)&c->h_st[k], sizeof(StepDev)));
   }
-  for (auto& v : c->graphs)
-    for (auto& g : v) (void)hipGraphExecDestroy(g.second);
-  for (auto g : c->graphs_cont) if (g) (void)hipGraphExecDestroy(g);
+  drop_graphs(c);
   c->graphs.assign(K, {});
-  c->graphs_cont.assign(K, nullptr);
+  c->graphs_cont.assign(K, {});
   c->last_rounds.assign(K, 0);
   c->want_rounds.assign(K, 0);
   c->trees.clear();
@@ -1181,16 +1218,17 @@ static void launch_main(Ctx* c, int k, int rounds, bool score_update, double lr)
   // a bag changes the root's size and rows: stream launches
   const bool graph = c->opts.use_graph && !c->profile && !c->bag_on && !c->goss_on;
   if (graph) {
+    const Ctx::GraphKey key{rounds, score_update, lr};
     hipGraphExec_t e = nullptr;
     for (auto& g : c->graphs[k])
-      if (g.first == rounds) e = g.second;
+      if (g.first == key) e = g.second;
     if (!e) {
       e = capture(ln.stream, [&] {
         enqueue_head(c, ln, k);
         enqueue_rounds(c, ln, k, 0, rounds);
         enqueue_tail(c, ln, k, score_update, lr);
       });
-      c->graphs[k].emplace_back(rounds, e);
+      c->graphs[k].emplace_back(key, e);
     }
     HIP_CHECK(hipGraphLaunch(e, ln.stream));
   } else {
@@ -1207,12 +1245,18 @@ static void launch_cont(Ctx* c, int k, int launched, bool score_update, double l
   Lane& ln = lane_of(c, k);
   const bool graph = c->opts.use_graph && !c->profile && !c->bag_on && !c->goss_on;
   if (graph) {
-    if (!c->graphs_cont[k])
-      c->graphs_cont[k] = capture(ln.stream, [&] {
+    const Ctx::GraphKey key{c->rounds_cont, score_update, lr};
+    hipGraphExec_t e = nullptr;
+    for (auto& g : c->graphs_cont[k])
+      if (g.first == key) e = g.second;
+    if (!e) {
+      e = capture(ln.stream, [&] {
         enqueue_rounds(c, ln, k, 0, c->rounds_cont);
         enqueue_tail(c, ln, k, score_update, lr);
       });
-    HIP_CHECK(hipGraphLaunch(c->graphs_cont[k], ln.stream));
+      c->graphs_cont[k].emplace_back(key, e);
+    }
+    HIP_CHECK(hipGraphLaunch(e, ln.stream));
   } else {
     enqueue_rounds(c, ln, k, launched, c->rounds_cont);
     enqueue_tail(c, ln, k, score_update, lr);
@@ -1457,7 +1501,7 @@ static void compact_bag(Ctx* c) {
   c->bag_m = m;
   c->bag_cnt = m;
   c->bag_active = true;
-  launch_group_rows(c, c->bag_rows.p, m, c->bag_grp0.p, s);
+  launch_group_rows(c, c->bag_rows.p, m, c->bag_grp0.p, c->bag_grp0.n, s);
 }
 
 // GOSSStrategy::Bagging (sample_strategy.rs:319-350, 395-442) after the
@@ -1559,6 +1603,151 @@ static int train_iteration(Ctx* c) {
   }
   c->iter++;
   return 0;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Launch shapes of the tree-growth kernels picked by timing, once per
+// workload shape: rows, used features, bins, form, classes, leaves and the
+// device's CUs. The reference's only shape rule is its row-block size from
+// the thread count (multi_val_bin.rs:337-351 block_info / min_block_size);
+// here the knobs are the histogram LDS budget (features per block and the
+// group-major row layout), the histogram row chunks and their minimum rows,
+// and the partition grid. Trees do not depend on any of them, so candidates
+// are timed on the workload itself: each runs boosting iterations from the
+// starting scores (two warm ones, which capture its launches, then `trees`
+// timed ones) and the training state is reset after it; coordinate descent
+// over the knobs in that order.
+struct TuneKey {
+  int64_t N;
+  int F, B, quant, K, L, cus;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(N, F, B, quant, K, L, cus) < std::tie(o.N, o.F, o.B, o.quant, o.K, o.L, o.cus);
+  }
+};
+struct TuneChoice {
+  int part_grid, hist_chunks, hist_min_rows, hist_lds;
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TuneChoice> g_tune_cache;   // per process: one search per workload shape
+
+static void apply_tuning(Ctx* c, const TuneChoice& t) {
+  c->tune_part_grid = t.part_grid;
+  c->tune_hist_chunks = t.hist_chunks;
+  c->tune_hist_min_rows = t.hist_min_rows;
+  c->tune_hist_lds = t.hist_lds;
+}
+
+// Back to the state before the first iteration: scores (training and
+// validation) from zero, no trees, the quantizer's call counter at zero.
+static void reset_training(Ctx* c) {
+  c->score.zero(c->stream);
+  for (auto& v : c->valid) reset_valid_scores(c, *v);
+  c->trees.clear();
+  c->iter = 0;
+  c->pred_dirty = true;
+  if (c->opts.use_quantized_grad) {
+    c->call_ctr.zero(c->stream);
+    c->n_calls = 0;
+  }
+  std::fill(c->want_rounds.begin(), c->want_rounds.end(), 0);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+// Time (ms) of `iters` boosting iterations at the current shapes, after
+// `warm` iterations that capture their launches, from the starting scores.
+static double time_iterations(Ctx* c, int warm, int iters) {
+  const bool quant = c->opts.use_quantized_grad != 0;
+  set_shapes(c, quant);
+  launch_group_rows(c, nullptr, c->N, c->grp0.p, c->grp0.n, c->stream);
+  reset_training(c);
+  for (int i = 0; i < warm; ++i) train_iteration(c);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < iters; ++i) train_iteration(c);
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  reset_training(c);
+  return ms;
+}
+
+// Returns {part_grid, hist_chunks, hist_min_rows, hist_lds_budget, candidates timed}.
+static std::array<int, 5> autotune(Ctx* c, int trees) {
+  if (!c->state_ready) setup_training(c);
+  if (!c->trees.empty()) throw Error(LGBMHIP_INVALID_PARAMETER, "autotune before the first iteration only");
+  if (c->bag_on || c->goss_on) throw Error(LGBMHIP_UNSUPPORTED, "autotune with bagging or GOSS");
+  set_device(c);
+  const bool quant = c->opts.use_quantized_grad != 0;
+  const TuneKey key{c->N, c->F, c->B, quant ? 1 : 0, c->K, c->L, c->cus};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      apply_tuning(c, it->second);
+      set_shapes(c, quant);
+      launch_group_rows(c, nullptr, c->N, c->grp0.p, c->grp0.n, c->stream);
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+      return {c->part_grid, c->hist_chunks, c->hist_min_rows, c->tune_hist_lds, 0};
+    }
+  }
+  const int iters = std::max(1, trees);
+  const int cus = c->cus;
+  TuneChoice best{0, 0, 0, 0};
+  apply_tuning(c, best);
+  double best_ms = time_iterations(c, 2, iters);
+  int timed = 1;
+  auto consider = [&](TuneChoice t) {
+    apply_tuning(c, t);
+    double ms;
+    try {
+      ms = time_iterations(c, 2, iters);
+    } catch (const Error& e) {
+      if (e.code != LGBMHIP_UNSUPPORTED) throw;   // a shape this device cannot run
+      return;
+    }
+    ++timed;
+    if (ms < best_ms * 0.98) {   // a change must win by more than run-to-run noise
+      best_ms = ms;
+      best = t;
+    }
+  };
+  // 1. histogram LDS budget (features per block, the row layout): only budgets that change the groups
+  {
+    std::vector<std::pair<int, int>> seen;
+    plan_hist(c, quant);
+    seen.emplace_back(c->hist_fb, c->hist_ng);
+    for (int kib : {8, 16, 48, 64, 96}) {
+      c->tune_hist_lds = kib * 1024;
+      plan_hist(c, quant);
+      if (std::find(seen.begin(), seen.end(), std::make_pair(c->hist_fb, c->hist_ng)) != seen.end()) continue;
+      seen.emplace_back(c->hist_fb, c->hist_ng);
+      TuneChoice t = best;
+      t.hist_lds = kib * 1024;
+      consider(t);
+    }
+  }
+  // 2. histogram row chunks per pass, 3. their minimum rows, 4. partition grid
+  for (int m : {1, 2, 4}) {
+    TuneChoice t = best;
+    t.hist_chunks = m * cus;
+    consider(t);
+  }
+  for (int r : {1024, 2048, 8192}) {
+    TuneChoice t = best;
+    t.hist_min_rows = r;
+    consider(t);
+  }
+  for (int m : {1, 2, 8}) {
+    TuneChoice t = best;
+    t.part_grid = m * cus;
+    consider(t);
+  }
+  apply_tuning(c, best);
+  set_shapes(c, quant);
+  launch_group_rows(c, nullptr, c->N, c->grp0.p, c->grp0.n, c->stream);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+  }
+  return {c->part_grid, c->hist_chunks, c->hist_min_rows, c->tune_hist_lds, timed};
 }
 
 // One tree from caller gradients (SerialTreeLearner::train), no score update.
@@ -1805,11 +1994,7 @@ static int add_valid(Ctx* c, const uint8_t* bins, int64_t n, const float* label,
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->valid.push_back(std::move(v));
   // captured trees hold the score walks of the sets they were captured with
-  for (auto& v : c->graphs) {
-    for (auto& g : v) (void)hipGraphExecDestroy(g.second);
-    v.clear();
-  }
-  for (auto& g : c->graphs_cont) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  drop_graphs(c);
   return (int)c->valid.size();
 }
 
@@ -2755,7 +2940,7 @@ static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc
     c->rows_buf.alloc(m);
     HIP_CHECK(hipMemcpyAsync(c->rows_buf.p, rows, m * 4, hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(ln.gh[1].p, pk.data(), pk.size(), hipMemcpyHostToDevice, c->stream));
-    launch_group_rows(c, c->rows_buf.p, m, ln.grp[1].p, c->stream);
+    launch_group_rows(c, c->rows_buf.p, m, ln.grp[1].p, ln.grp[1].n, c->stream);
   }
   HistArgs a = hist_args(c, ln, quant, 0, 1);
   a.d_begin = 0;
@@ -3126,6 +3311,13 @@ lgbmhip_status lgbmhip_set_tuning(lgbmhip_ctx* ctx, int32_t part_grid, int32_t h
   });
 }
 
+lgbmhip_status lgbmhip_autotune(lgbmhip_ctx* ctx, int32_t trees_per_candidate, int32_t* out5) {
+  return guarded(ctx, [&] {
+    const auto r = autotune(&ctx->c, trees_per_candidate);
+    if (out5)
+      for (int i = 0; i < 5; ++i) out5[i] = r[(size_t)i];
+  });
+}
 lgbmhip_status lgbmhip_get_tuning(lgbmhip_ctx* ctx, int32_t* out9) {
   return guarded(ctx, [&] {
     Ctx* c = &ctx->c;
