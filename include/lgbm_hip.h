@@ -119,6 +119,7 @@ typedef struct {
   int32_t sample_threads;       /* GOSS: the reference's num_threads, which sets its row blocks (>= 1) */
   double top_rate;              /* GOSS: share of rows kept by |g h| and share sampled from the rest */
   double other_rate;
+  int32_t bagging_by_query;     /* row bagging draws whole queries (query boundaries required) */
 } lgbmhip_train_opts;
 
 /* One metric of lgbmhip_eval (metric.rs:17-43 MetricKind, the Config fields it reads). */

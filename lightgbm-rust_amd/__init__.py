@@ -29,11 +29,11 @@ DEFAULT_PARAMS = dict(
     boost_from_average=True, num_class=1, use_quantized_grad=False, num_grad_quant_bins=4,
     stochastic_rounding=True, quant_train_renew_leaf=False, max_cat_to_onehot=4, max_cat_threshold=32,
     cat_l2=10.0, cat_smooth=10.0, min_data_per_group=100, lambdarank_truncation_level=30, lambdarank_norm=True,
-    num_threads=0, device_type="gpu", gpu_device_id=0, use_graph=True, quant_table_threads=1,
+    num_threads=0, device_type="gpu", gpu_device_id=0, use_graph=True, quant_table_threads=None,
     metric="", eval_at=(1, 2, 3, 4, 5), multi_error_top_k=1, label_gain=None, early_stopping_round=0,
     first_metric_only=False, early_stopping_min_delta=0.0, device_binning=False, bagging_fraction=1.0,
     bagging_freq=0, bagging_seed=3, pos_bagging_fraction=1.0, neg_bagging_fraction=1.0,
-    data_sample_strategy="bagging", top_rate=0.2, other_rate=0.1, autotune=False, autotune_trees=4,
+    data_sample_strategy="bagging", top_rate=0.2, other_rate=0.1, autotune=False, autotune_trees=4, bagging_by_query=False,
 )
 
 ALIASES = {
@@ -215,6 +215,16 @@ class Tree:
     """Tree arrays (tree.rs:120-159)."""
 
 
+def _quant_table_threads(p):
+    """The thread count the quantizer's random tables are drawn for
+    (learner/quant.rs:45-79: one generator pair per thread block): the given
+    quant_table_threads, else num_threads when it is set, else 1 (the
+    reference's default of all host threads has no fixed value here)."""
+    if p.get("quant_table_threads") is not None:
+        return int(p["quant_table_threads"])
+    return int(p["num_threads"]) if int(p["num_threads"]) > 0 else 1
+
+
 class Booster:
     """Training booster on the GPU engine (basic.py:1364 / py/lib.rs:435 RsBooster)."""
 
@@ -257,14 +267,14 @@ class Booster:
         to = _capi.TrainOpts(OBJECTIVES[obj], self.K, float(p["sigmoid"]), float(p["learning_rate"]),
                              int(_b(p["boost_from_average"])), int(_b(p["use_quantized_grad"])),
                              int(p["num_grad_quant_bins"]), int(_b(p["stochastic_rounding"])),
-                             int(_b(p["quant_train_renew_leaf"])), int(p["seed"]), int(p["quant_table_threads"]),
+                             int(_b(p["quant_train_renew_leaf"])), int(p["seed"]), _quant_table_threads(p),
                              int(p["lambdarank_truncation_level"]), int(_b(p["lambdarank_norm"])),
                              int(_b(p["use_graph"])), float(p["bagging_fraction"]), int(p["bagging_freq"]),
                              int(p["bagging_seed"]), float(p["pos_bagging_fraction"]),
                              float(p["neg_bagging_fraction"]),
                              1 if str(p["data_sample_strategy"]) == "goss" or str(p.get("boosting")) == "goss" else 0,
                              int(p["num_threads"]) if int(p["num_threads"]) > 0 else 4096,
-                             float(p["top_rate"]), float(p["other_rate"]))
+                             float(p["top_rate"]), float(p["other_rate"]), int(_b(p["bagging_by_query"])))
         check(lib().lgbmhip_set_train_opts(h, C.byref(to)), h)
         self.tuning = None
         if _b(p["autotune"]):

@@ -53,7 +53,8 @@ class TrainOpts(C.Structure):
                 ("lambdarank_norm", C.c_int32), ("use_graph", C.c_int32), ("bagging_fraction", C.c_double),
                 ("bagging_freq", C.c_int32), ("bagging_seed", C.c_int32), ("pos_bagging_fraction", C.c_double),
                 ("neg_bagging_fraction", C.c_double), ("data_sample_strategy", C.c_int32),
-                ("sample_threads", C.c_int32), ("top_rate", C.c_double), ("other_rate", C.c_double)]
+                ("sample_threads", C.c_int32), ("top_rate", C.c_double), ("other_rate", C.c_double),
+                ("bagging_by_query", C.c_int32)]
 
 
 class MetricOpts(C.Structure):

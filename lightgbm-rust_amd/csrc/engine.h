@@ -173,6 +173,14 @@ struct IterDev {
   int start;                                // stochastic-rounding start offset
   int pad;
   unsigned qmax_g, qmax_h;                  // max |gq| and max u8(hq) of the tree's discretized rows
+  // a quantized tree on a full-data bag (quant.rs:144-158 bag_root_sums): the
+  // root's sums of f32(gq * f32 grad_scale) and f32(hq * f32 hess_scale) as
+  // exact integers in units of 2^bag_eg / 2^bag_eh, and the packed sum read
+  // by position 0 .. bag size - 1
+  int full_bag;
+  int bag_eg, bag_eh;
+  int pad2;
+  long long bag_sg_fix, bag_sh_fix, bag_isum;
 };
 
 }  // namespace lgbmhip

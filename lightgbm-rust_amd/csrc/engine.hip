@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <limits>
@@ -199,6 +200,7 @@ struct Lane {
   hipEvent_t done_ev = nullptr;
   DevBuf<uint8_t> gh[3];
   DevBuf<uint8_t> grp[3];      // [0] unused: buffer 0's bins are Ctx::grp0
+  DevBuf<int16_t> q_all;       // quantized (gq, hq) of every row (a quantized tree on a full-data bag)
   DevBuf<NodeDev> nodes;       // the lane's last tree's nodes
   DevBuf<int> leaf_shift;      // [L][2] renewal fixed-point shifts
   DevBuf<long long> acc;       // each item's histogram summed over its chunks, one unit per group
@@ -224,7 +226,15 @@ struct Lane {
 
 // Trees of one iteration grown at the same time, at most: the device's
 // default number of hardware queues per process.
-constexpr int kMaxLanes = 4;
+constexpr int kMaxLanes = 16;
+
+// Growth lanes run on concurrent streams: one per hardware queue the HIP
+// runtime gives this process (GPU_MAX_HW_QUEUES, 4 when unset), at most 16.
+static int hw_queue_lanes() {
+  const char* v = std::getenv("GPU_MAX_HW_QUEUES");
+  const int q = v ? std::atoi(v) : 4;
+  return std::max(1, std::min(kMaxLanes, q > 0 ? q : 4));
+}
 
 // A validation set (boosting.rs:378-411): rows binned with the training
 // data's mappers, row-major [n][S] like the training bins, and its scores
@@ -339,6 +349,8 @@ struct Ctx {
   int64_t bag_cnt = 0, bag_m = 0;
   DevBuf<unsigned> bag_rows, bag_lcg;
   DevBuf<uint8_t> bag_flags, bag_grp0;
+  bool bag_by_query = false;   // bagging_by_query: whole queries drawn
+  DevBuf<int> bag_qb;          // their boundaries on the device
   DevBuf<int> bag_counts, bag_offsets;
   DevBuf<long long> bag_total;
   // validation sets and the metric workspace
@@ -780,7 +792,7 @@ static void setup_training(Ctx* c) {
   const int MS = c->MS;
   // lanes: lane 0 runs on the context's stream; more lanes get streams of
   // their own (default priority)
-  c->NL = std::min(K, kMaxLanes);
+  c->NL = std::min(K, hw_queue_lanes());
   c->lanes.clear();
   for (int l = 0; l < c->NL; ++l) {
     c->lanes.emplace_back(new Lane());
@@ -874,8 +886,7 @@ static void setup_training(Ctx* c) {
         throw Error(LGBMHIP_INVALID_PARAMETER, "Check failed: config_->top_rate > 0.0f && config_->other_rate > 0.0f");
       if (o.bagging_freq > 0 && o.bagging_fraction != 1.0) throw Error(LGBMHIP_INVALID_PARAMETER, "Cannot use bagging in GOSS");
       c->bag_subset = o.top_rate + o.other_rate <= 0.5;
-      if (quant && !c->bag_subset)
-        throw Error(LGBMHIP_UNSUPPORTED, "quantized gradients with GOSS above top_rate + other_rate = 0.5 (a full-data bag)");
+
       c->bag_on = false;
       // threading.rs block_info_force_size(num_threads, n, 1024)
       const int64_t T = std::max(1, o.sample_threads);
@@ -918,11 +929,17 @@ static void setup_training(Ctx* c) {
                                          (int64_t)((double)(N - num_pos) * o.neg_bagging_fraction)
                                    : (int64_t)(o.bagging_fraction * (double)N);
       c->bag_subset = ((double)c->bag_cnt / (double)N) / (double)o.bagging_freq <= 0.5 && F < 100;
-      if (quant && !c->bag_subset)
-        throw Error(LGBMHIP_UNSUPPORTED,
-                    "quantized gradients with a full-data bag (an average bag rate above 0.5; the reference's "
-                    "root sums read rows by position there)");
+
       c->need_re_bagging = true;
+      c->bag_by_query = o.bagging_by_query != 0;
+      if (c->bag_by_query) {
+        // sample_strategy.rs:239-276; without query data every bag would be empty
+        if (c->qb.size() < 2) throw Error(LGBMHIP_UNSUPPORTED, "bagging_by_query without query boundaries");
+        if (c->qb.front() != 0 || c->qb.back() != (int32_t)N)
+          throw Error(LGBMHIP_INVALID_DATA, "query boundaries do not cover the data");
+        c->bag_qb.alloc(c->qb.size());
+        HIP_CHECK(hipMemcpy(c->bag_qb.p, c->qb.data(), c->qb.size() * 4, hipMemcpyHostToDevice));
+      }
       const int64_t nb = (N + 1023) / 1024;
       c->bag_rows.alloc(N);
       c->bag_flags.alloc(N);
@@ -936,6 +953,15 @@ static void setup_training(Ctx* c) {
       for (int64_t b = 0; b < nb; ++b) lcg[b] = (unsigned)o.bagging_seed + (unsigned)b;
       c->bag_lcg.alloc(nb);
       HIP_CHECK(hipMemcpy(c->bag_lcg.p, lcg.data(), nb * 4, hipMemcpyHostToDevice));
+    }
+    if (quant && (c->bag_on || c->goss_on) && !c->bag_subset) {
+      // a full-data bag: the quantizer runs over every row, and the root's
+      // f32-scaled sums (quant.rs:144-158) are summed exactly; the
+      // reference's f64 running sums are exact (and so equal them) while
+      // rows x quantization bins < 2^29
+      if ((double)N * (double)std::max(2, o.num_grad_quant_bins) >= 536870912.0)
+        throw Error(LGBMHIP_UNSUPPORTED, "quantized gradients with a full-data bag above 2^29 / num_grad_quant_bins rows");
+      for (auto& lp : c->lanes) lp->q_all.alloc(N);
     }
     // buffer 0: the root's rows in row order, bins group-major
     c->grp0.alloc(gbytes);
@@ -1078,12 +1104,27 @@ static void enqueue_head(Ctx* c, Lane& ln, int k) {
   const int64_t R = c->bag_active ? c->bag_m : N;
   const unsigned* bag = c->bag_active ? c->bag_rows.p : nullptr;
   const int ig = grid_for(R, 256, c->cus * 4);
-  if (bag) {
+  // a quantized tree on a full-data bag: the quantizer over every row, the
+  // root's positions and sums from the bag (quant.rs:92-158)
+  const bool full_bag = quant && bag && !c->bag_subset;
+  if (full_bag) {
+    const int ag = grid_for(N, 256, c->cus * 4);
+    hipLaunchKernelGGL(k_clear_max, dim3(1), dim3(1), 0, s, c->it.p + k);
+    hipLaunchKernelGGL(k_absmax_it, dim3(ag), dim3(256), 0, s, gk, hk, (long long)N, c->it.p + k,
+                       (const unsigned*)nullptr);
+    hipLaunchKernelGGL(k_discretize, dim3(ag), dim3(256), 0, s, (long long)N, gk, hk, c->grv.p, c->hrv.p,
+                       c->opts.num_grad_quant_bins, c->opts.stochastic_rounding, 0, c->starts.p, c->call_ctr.p, k,
+                       c->it.p + k, (unsigned*)nullptr, (char2*)ln.q_all.p, (int8_t*)nullptr, (int8_t*)nullptr,
+                       (const unsigned*)nullptr);
+    hipLaunchKernelGGL(k_full_bag_root, dim3(ig), dim3(256), 0, s, (long long)R, bag, (const char2*)ln.q_all.p,
+                       c->it.p + k, (char2*)ln.gh[0].p);
+  } else if (bag) {
     // max |g|, |h| over the bag (quant.rs:100-108 with rows)
     hipLaunchKernelGGL(k_clear_max, dim3(1), dim3(1), 0, s, c->it.p + k);
     hipLaunchKernelGGL(k_absmax_it, dim3(ig), dim3(256), 0, s, gk, hk, (long long)R, c->it.p + k, bag);
   }
-  if (quant) {
+  if (full_bag) {
+  } else if (quant) {
     // the k-th discretizer call of the iteration (the counter moves on by
     // the iteration's trees once they are all grown)
     hipLaunchKernelGGL(k_discretize, dim3(ig), dim3(256), 0, s, (long long)R, gk, hk, c->grv.p, c->hrv.p,
@@ -1478,6 +1519,17 @@ static void draw_bag(Ctx* c) {
   const int64_t N = c->N, nb = (N + 1023) / 1024;
   hipStream_t s = c->stream;
   c->need_re_bagging = false;
+  if (c->bag_by_query) {
+    // whole queries: row flags from the query draws, then the row blocks' counts
+    const int64_t nq = (int64_t)c->qb.size() - 1, qbk = (nq + 1023) / 1024;
+    hipLaunchKernelGGL(k_bag_draw_queries, dim3((unsigned)((qbk + 3) / 4)), dim3(256), 0, s, (long long)nq,
+                       c->bag_lcg.p, c->opts.bagging_fraction, c->bag_qb.p, c->bag_flags.p);
+    hipLaunchKernelGGL(k_bag_counts, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, c->bag_flags.p, (long long)N,
+                       c->bag_counts.p);
+    HIP_CHECK(hipGetLastError());
+    compact_bag(c);
+    return;
+  }
   hipLaunchKernelGGL(k_bag_draw, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, (long long)N,
                      c->bag_lcg.p, c->opts.bagging_fraction, c->bag_balanced ? c->d_label.p : (const float*)nullptr,
                      c->opts.pos_bagging_fraction, c->opts.neg_bagging_fraction, c->bag_flags.p, c->bag_counts.p);

@@ -604,7 +604,11 @@ __device__ __forceinline__ LeafSums leaf_sums(const SplitArgs& a, int slot, bool
   s.hs = a.it->hess_scale;
   double sg = L.sum_g, sh = L.sum_h;
   if (root) {
-    if (a.quant) {
+    if (a.quant && a.it->full_bag) {
+      s.isum = a.it->bag_isum;
+      sg = ldexp((double)a.it->bag_sg_fix, a.it->bag_eg);
+      sh = ldexp((double)a.it->bag_sh_fix, a.it->bag_eh);
+    } else if (a.quant) {
       s.isum = a.it->root_int_sum;
       sg = (double)a.it->root_gq_sum * s.gs;
       sh = (double)a.it->root_hq_sum * s.hs;

@@ -183,6 +183,10 @@ __global__ void k_iter_reset(IterDev* it, int K) {
     it[k].root_hq_sum = 0;
     it[k].qmax_g = 0;
     it[k].qmax_h = 0;
+    it[k].full_bag = 0;
+    it[k].bag_sg_fix = 0;
+    it[k].bag_sh_fix = 0;
+    it[k].bag_isum = 0;
   }
 }
 
@@ -587,6 +591,47 @@ __global__ void __launch_bounds__(256) k_discretize(long long n, const float* __
     it->grad_scale = gs;
     it->hess_scale = hs;
     it->start = (int)start;
+  }
+}
+
+// A quantized tree on a full-data bag (not subset mode): the quantizer ran
+// over every row into q_all (learner/quant.rs:92-129 with rows None); the
+// root's positions take the bag rows' values, and the root sums are
+// quant.rs:144-158 bag_root_sums: sums over the bag's rows of f32(gq * gs)
+// and f32(hq * hs) with the scales rounded to f32, and the packed sum of
+// positions 0 .. m - 1 (rows 0 .. m - 1, not the bag's). Every product is an
+// integer multiple of 2^e (gs = M 2^e, M < 2^24), so the sums are exact
+// int64 in those units; the host bounds m * bins so that the reference's
+// f64 running sums are exact too.
+__device__ __forceinline__ int f32_unit_exp(float s) { return s > 0.0f ? ilogbf(s) - 23 : 0; }
+
+__global__ void __launch_bounds__(256) k_full_bag_root(long long m, const unsigned* __restrict__ bag,
+                                                       const char2* __restrict__ q_all, IterDev* it,
+                                                       char2* __restrict__ q0) {
+  const float gs = (float)it->grad_scale, hs = (float)it->hess_scale;
+  const int eg = f32_unit_exp(gs), eh = f32_unit_exp(hs);
+  long long ag = 0, ah = 0, ai = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x) {
+    const char2 v = q_all[bag[i]];
+    q0[i] = v;
+    const float pg = (float)v.x * gs, ph = (float)v.y * hs;
+    ag += (long long)ldexp((double)pg, -eg);
+    ah += (long long)ldexp((double)ph, -eh);
+    const char2 u = q_all[i];
+    ai += (long long)(((unsigned long long)(long long)u.x << 32) | (unsigned long long)(uint8_t)u.y);
+  }
+  ag = block_sum_ll(ag);
+  ah = block_sum_ll(ah);
+  ai = block_sum_ll(ai);
+  if (threadIdx.x == 0) {
+    atomicAdd((unsigned long long*)&it->bag_sg_fix, (unsigned long long)ag);
+    atomicAdd((unsigned long long*)&it->bag_sh_fix, (unsigned long long)ah);
+    atomicAdd((unsigned long long*)&it->bag_isum, (unsigned long long)ai);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    it->full_bag = 1;
+    it->bag_eg = eg;
+    it->bag_eh = eh;
   }
 }
 
@@ -1686,6 +1731,37 @@ __global__ void __launch_bounds__(256) k_bag_draw(long long n, unsigned* lcg, do
     lcg_jump((unsigned)rows, A, C);
     lcg[b] = A * x0 + C;
     counts[b] = cnt;
+  }
+}
+
+// bagging_by_query (sample_strategy.rs:239-276): query q is in the bag when
+// the next draw of the generator of its 1024-query block is below the
+// fraction, and then all of its rows are. One wave per generator block: lane
+// j draws queries 16 j .. 16 j + 15 of the block from the state jumped ahead
+// by 16 j steps and flags their rows (every row is some query's).
+__global__ void __launch_bounds__(256) k_bag_draw_queries(long long nq, unsigned* lcg, double fraction,
+                                                          const int* __restrict__ qb, uint8_t* __restrict__ flags) {
+  const int lane = threadIdx.x & 63;
+  const long long b = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nb = (nq + kBagBlock - 1) / kBagBlock;
+  if (b >= nb) return;
+  const unsigned x0 = lcg[b];
+  const long long q0 = b * kBagBlock, qs = min((long long)kBagBlock, nq - q0);
+  constexpr int PER = kBagBlock / 64;
+  unsigned A, C;
+  lcg_jump((unsigned)(lane * PER), A, C);
+  unsigned x = A * x0 + C;
+  for (int t = 0; t < PER; ++t) {
+    const long long i = (long long)lane * PER + t;
+    if (i >= qs) break;
+    x = x * 214013u + 2531011u;
+    const float f = (float)(int)((x >> 16) & 0x7FFFu) / 32768.0f;
+    const uint8_t in = (double)f < fraction ? 1 : 0;
+    for (int r = qb[q0 + i]; r < qb[q0 + i + 1]; ++r) flags[r] = in;
+  }
+  if (lane == 0) {
+    lcg_jump((unsigned)qs, A, C);
+    lcg[b] = A * x0 + C;
   }
 }
 

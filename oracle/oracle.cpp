@@ -463,6 +463,7 @@ struct Params {
   double pos_bagging_fraction = 1.0, neg_bagging_fraction = 1.0;
   int goss = 0;                    // data_sample_strategy=goss
   double top_rate = 0.2, other_rate = 0.1;
+  bool bagging_by_query = false;
   std::vector<int> categorical;
 };
 
@@ -521,6 +522,7 @@ static bool parse_params(const char* s, Params& p) {
       else if (k == "top_rate") p.top_rate = atof(v.c_str());
       else if (k == "other_rate") p.other_rate = atof(v.c_str());
       else if (k == "neg_bagging_fraction") p.neg_bagging_fraction = atof(v.c_str());
+      else if (k == "bagging_by_query") p.bagging_by_query = v == "true" || v == "1";
       else if (k == "categorical_feature") {
         size_t a = 0;
         while (a < v.size()) {
@@ -1426,6 +1428,7 @@ struct Learner {
   LeafSplits smaller, larger;
   Discretizer* quant = nullptr;
   std::vector<TraceEntry> trace;
+  LeafSplits root_sums;   // the last tree's root sums (stage checks)
   // learner/mod.rs:545-568 set_bagging_data: the root's rows (ascending);
   // `bag_subset` is upstream's subset mode (the quantizer draws over the bag)
   std::vector<uint32_t> bag;
@@ -1746,8 +1749,12 @@ struct Learner {
   Tree train(const float* g, const float* h) {
     int64_t n = use_bag ? (int64_t)bag.size() : ds->n;
     bool q = quant != nullptr;
+    // a full-data bag (not subset mode) keeps the learner's view of every
+    // row: the quantizer runs over all rows (quant.rs:92-129 with rows None)
+    const bool full_bag = use_bag && !bag_subset;
     if (q) {
-      if (use_bag) quant->discretize_rows(g, h, bag);   // subset mode only (checked at set-up)
+      if (full_bag) quant->discretize(g, h, ds->n);
+      else if (use_bag) quant->discretize_rows(g, h, bag);
       else quant->discretize(g, h, n);
     }
     for (int64_t i = 0; i < n; ++i) indices[i] = use_bag ? bag[i] : (uint32_t)i;
@@ -1757,7 +1764,17 @@ struct Learner {
     for (auto& s : best) s.reset();
     double sg = 0, sh = 0;
     int64_t is = 0;
-    if (q) {
+    if (q && full_bag) {
+      // quant.rs:144-158 bag_root_sums: f32 scales, and the packed sum read
+      // by position 0 .. n - 1 rather than over the bag's rows
+      const float gs = (float)quant->grad_scale, hs = (float)quant->hess_scale;
+      for (int64_t k = 0; k < n; ++k) {
+        const uint32_t i = indices[k];
+        sg += (double)((float)quant->gq[i] * gs);
+        sh += (double)((float)quant->hq[i] * hs);
+        is = wadd(is, quant->packed[k]);
+      }
+    } else if (q) {
       for (int64_t k = 0; k < n; ++k) {
         const uint32_t i = indices[k];
         sg += (double)quant->gq[i] * quant->grad_scale;
@@ -1768,6 +1785,7 @@ struct Learner {
       for (int64_t k = 0; k < n; ++k) { sg += (double)g[indices[k]]; sh += (double)h[indices[k]]; }
     }
     smaller = LeafSplits{0, (int)n, sg, sh, is, smaller.weight};
+    root_sums = smaller;
     larger = LeafSplits{};
     trace.clear();
     Tree tree;
@@ -2123,11 +2141,26 @@ struct Bagger {
     bag_cnt = (int64_t)in_bag.size();
     return true;
   }
+  // bagging_by_query: the query boundaries (sample_strategy.rs:239-276)
+  bool by_query = false;
+  std::vector<int> qb;
   bool bagging(int iter) {
     if (!on || !((bag_cnt < n && iter % freq == 0) || need_re_bagging)) return false;
     need_re_bagging = false;
     in_bag.clear();
     out_of_bag.clear();
+    if (by_query) {
+      // query q is drawn by the generator of its 1024-query block; the bag is
+      // the rows of the drawn queries in order
+      const int nq = (int)qb.size() - 1;
+      std::vector<char> in((size_t)n, 0);
+      for (int q = 0; q < nq; ++q)
+        if ((double)rands[q / 1024].next_float() < fraction)
+          for (int r = qb[q]; r < qb[q + 1]; ++r) in[r] = 1;
+      for (int64_t i = 0; i < n; ++i) (in[i] ? in_bag : out_of_bag).push_back((uint32_t)i);
+      bag_cnt = (int64_t)in_bag.size();
+      return true;
+    }
     for (int64_t i = 0; i < n; ++i) {
       const double r = (double)rands[i / 1024].next_float();
       const double frac = balanced ? (label[i] > 0.0f ? pos_fraction : neg_fraction) : fraction;
@@ -2274,12 +2307,10 @@ void* orc_booster_create(void* d, const char* params) {
   if (p.objective == 0)
     for (float l : b->ds->label) num_pos += l > 0.0f ? 1 : 0;
   b->bagger.setup(p, b->ds->n, (int)b->ds->used.size(), num_pos, b->ds->label.data());
-  if (b->bagger.on && p.use_quantized_grad && !b->bagger.subset) {
-    // the full-data bag of the quantized learner is not restated
-    delete b->learner;
-    delete b->rank;
-    delete b;
-    return nullptr;
+  if (p.bagging_by_query && b->bagger.on && !b->bagger.goss) {
+    // sample_strategy.rs:239-276: whole queries are drawn (no query data: no queries)
+    b->bagger.by_query = true;
+    b->bagger.qb = b->ds->query_boundaries.empty() ? std::vector<int>{0} : b->ds->query_boundaries;
   }
   return b;
 }
@@ -2449,6 +2480,13 @@ void orc_learner_leaf_of_row(void* p, int* out) {
   Learner* l = b->learner;
   for (int leaf = 0; leaf < (int)l->leaf_count.size(); ++leaf)
     for (int64_t k = 0; k < l->leaf_count[leaf]; ++k) out[l->indices[l->leaf_begin[leaf] + k]] = leaf;
+}
+// The last tree's root sums (learner/mod.rs:611-628): gradient, hessian, packed integer sum.
+void orc_learner_root_sums(void* p, double* sg, double* sh, int64_t* is) {
+  const LeafSplits& r = ((OrcBooster*)p)->learner->root_sums;
+  *sg = r.sum_gradients;
+  *sh = r.sum_hessians;
+  *is = r.int_sum;
 }
 void orc_learner_quant_scales(void* p, double* gs, double* hs) {
   OrcBooster* b = (OrcBooster*)p;

@@ -85,6 +85,7 @@ def lib():
             "orc_metric_ndcg": (None, [PD, PF, PI, I, PI, I, I, PD]),
             "orc_model_load": (P, [C.c_char_p]),
             "orc_set_num_threads": (None, [I]),
+            "orc_learner_root_sums": (None, [P, C.POINTER(D), C.POINTER(D), C.POINTER(I64)]),
             "orc_goss": (I64, [PF, PF, I64, I, I, D, D, I, PU]),
             "orc_block_info_force_size": (None, [I, I64, I64, C.POINTER(I64), C.POINTER(I64)]),
         }
@@ -222,6 +223,12 @@ class Booster:
         a, b = D(), D()
         lib().orc_learner_quant_scales(self.h, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def root_sums(self):
+        """The last tree's root sums: (gradient, hessian, packed integer)."""
+        a, b, c = D(), D(), I64()
+        lib().orc_learner_root_sums(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
 
     def quant_packed(self):
         gq = np.zeros(self.ds.n, np.int8)

@@ -77,12 +77,26 @@ def test_multiclass_bagging_matches_oracle():
     np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
 
 
-def test_quantized_full_data_bag_is_unsupported():
-    X, y = datagen.binary_100k(5000)
-    params = dict(P, use_quantized_grad=True, bagging_fraction=0.9, bagging_freq=1)
+@pytest.mark.parametrize("sampler", ["bagging", "goss"])
+def test_quantized_full_data_bag_matches_oracle(sampler):
+    # an average bag rate above 0.5 (or GOSS rates above 0.5): the learner
+    # keeps every row in view, the quantizer runs over all rows, and the
+    # root's sums are quant.rs:144-158's (f32 scales, packed sum by position)
+    X, y = datagen.binary_100k(60000)
+    params = dict(P, use_quantized_grad=True, num_grad_quant_bins=16, quant_train_renew_leaf=True)
+    if sampler == "bagging":
+        params.update(bagging_fraction=0.8, bagging_freq=1, bagging_seed=7)
+    else:
+        params.update(learning_rate=0.5, data_sample_strategy="goss", top_rate=0.4, other_rate=0.3, num_threads=4)
     b = lgb.Booster(params, lgb.Dataset(X, label=y, params=params))
-    with pytest.raises(lgb.LightGBMError, match="Unsupported"):
+    ob = O.Booster(O.Dataset(X, y, params), params)
+    for _ in range(5):
         b.update()
+        ob.update()
+    bag = ob.bag()
+    assert bag is not None and 0.5 * len(y) < len(bag) < len(y)
+    _compare(b, ob, 5)
+    np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
 
 
 @pytest.mark.parametrize("quant,threads,K", [(False, 4, 1), (True, 1, 1), (False, 3, 3)])
@@ -128,5 +142,29 @@ def test_lambdarank_row_sampling_matches_oracle(goss):
         b.update()
         ob.update()
     assert ob.bag() is not None
+    _compare(b, ob, 4)
+    np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("quant", [False, True])
+def test_bagging_by_query_matches_oracle(quant):
+    # bagging_by_query (sample_strategy.rs:239-276): whole queries in the bag
+    X, rel, qb = datagen.rank_2m(600, 300)
+    params = dict(objective="lambdarank", num_leaves=31, max_bin=63, learning_rate=0.5, bagging_fraction=0.4,
+                  bagging_freq=1, bagging_seed=4, bagging_by_query=True)
+    if quant:
+        params.update(use_quantized_grad=True, num_grad_quant_bins=16, quant_train_renew_leaf=True)
+    b = lgb.Booster(params, lgb.Dataset(X, label=rel, group=np.diff(qb), params=params))
+    ods = O.Dataset(X, rel, params)
+    ods.set_query_boundaries(qb)
+    ob = O.Booster(ods, params)
+    for _ in range(4):
+        b.update()
+        ob.update()
+    bag = ob.bag()
+    q_of_row = np.searchsorted(qb, bag, side="right") - 1
+    sizes = np.diff(qb)
+    assert np.array_equal(np.bincount(q_of_row, minlength=len(sizes))[np.unique(q_of_row)],
+                          sizes[np.unique(q_of_row)])   # every drawn query whole
     _compare(b, ob, 4)
     np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)

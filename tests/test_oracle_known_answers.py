@@ -792,3 +792,71 @@ def test_oracle_threads_do_not_change_trees_or_scores(form):
     for a, b in zip(runs[0][1], runs[1][1]):
         for f in ("split_feature", "threshold_in_bin", "left_child", "right_child", "leaf_value", "leaf_count"):
             np.testing.assert_array_equal(getattr(a, f), getattr(b, f))
+
+
+def test_bagging_by_query_draws_whole_queries_from_the_lcg_streams():
+    """sample_strategy.rs:239-276: query q is drawn by the generator of its
+    1024-query block (Random(seed + q // 1024), running on from bag to bag)
+    and the bag is the rows of the drawn queries, in order."""
+    rng = np.random.default_rng(23)
+    sizes = rng.integers(1, 12, 2600)           # 3 query blocks, the last one ragged
+    qb = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    n = int(qb[-1])
+    X = rng.normal(size=(n, 3)).astype(np.float32)
+    y = rng.integers(0, 4, n).astype(np.float32)
+    p = dict(objective="lambdarank", num_leaves=4, bagging_fraction=0.4, bagging_freq=1, bagging_seed=9,
+             bagging_by_query=True)
+    ods = O.Dataset(X, y, p)
+    ods.set_query_boundaries(qb)
+    ob = O.Booster(ods, p)
+    nq = len(sizes)
+    nqb = (nq + 1023) // 1024
+    streams = _lcg_streams([9 + j for j in range(nqb)], 1024 * 3)
+    for k in range(3):
+        ob.update()
+        drawn = np.empty(nq, np.float32)
+        for j in range(nqb):
+            lo, hi = j * 1024, min(nq, (j + 1) * 1024)
+            drawn[lo:hi] = streams[j, k * (hi - lo):(k + 1) * (hi - lo)]
+        sampled = np.flatnonzero(drawn.astype(np.float64) < 0.4)
+        want = np.concatenate([np.arange(qb[q], qb[q + 1]) for q in sampled])
+        np.testing.assert_array_equal(ob.bag(), want)
+
+
+@pytest.mark.parametrize("fraction", [0.3, 0.8])
+def test_quantized_bag_root_sums(fraction):
+    """learner/mod.rs:611-628 with a bag: in subset mode (average bag rate
+    <= 0.5) the quantizer runs over the bag and the root sums are the bag
+    rows' dequantized values and packed sums; with a full-data bag the
+    quantizer runs over every row and the sums are quant.rs:144-158's
+    bag_root_sums: f32 scales, and the packed sum of positions 0 .. m - 1."""
+    rng = np.random.default_rng(24)
+    n = 30000
+    X = rng.normal(size=(n, 5)).astype(np.float32)
+    y = (X[:, 0] + rng.normal(size=n) > 0).astype(np.float32)
+    p = dict(objective="binary", num_leaves=7, use_quantized_grad=True, num_grad_quant_bins=16,
+             quant_train_renew_leaf=True, bagging_fraction=fraction, bagging_freq=1, bagging_seed=2)
+    ob = O.Booster(O.Dataset(X, y, p), p)
+    ob.update()
+    ob.update()
+    bag = ob.bag()
+    gq, hq = ob.quant_packed()
+    gs, hs = ob.quant_scales()
+    sg, sh, isum = ob.root_sums()
+    packed = (gq.astype(np.int64) << 32) | hq.view(np.uint8).astype(np.int64)
+    m = len(bag)
+    if fraction <= 0.5:
+        want_g = want_h = 0.0
+        for r in bag:
+            want_g += float(gq[r]) * gs
+            want_h += float(hq[r]) * hs
+        want_i = int(packed[bag].sum())
+    else:
+        g32, h32 = np.float32(gs), np.float32(hs)
+        want_g = want_h = 0.0
+        for r in bag:
+            want_g += float(np.float32(gq[r]) * g32)
+            want_h += float(np.float32(hq[r]) * h32)
+        want_i = int(packed[:m].sum())          # by position, not over the bag
+        assert want_i != int(packed[bag].sum())
+    assert (sg, sh, isum) == (want_g, want_h, want_i)
