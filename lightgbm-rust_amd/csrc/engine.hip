@@ -305,6 +305,7 @@ struct Ctx {
   std::unique_ptr<Mt> start_eng;
   int n_calls = 0;
   int hist_tpr = 8, hist_fb = 0, hist_ng = 1, hist_chunks = 256, hist_min_rows = 2048;
+  int hist_grid = 256;   // blocks of a histogram launch (they loop over the units)
   size_t hist_lds = 0;
   int part_grid = 1024;
   int tune_part_grid = 0, tune_hist_chunks = 0, tune_hist_min_rows = 0, tune_hist_lds = 0;
@@ -735,6 +736,14 @@ static void set_shapes(Ctx* c, bool quant) {
   }
   c->hist_min_rows = c->tune_hist_min_rows > 0 ? c->tune_hist_min_rows : rows_q;
   plan_hist(c, quant);
+  // one resident wave of blocks: blocks loop over the units, and blocks
+  // beyond what the CUs hold at once only add dispatch latency to the
+  // small launches late in a tree
+  {
+    const int occ = hist_occupancy(quant, c->hist_tpr, c->hist_lds);
+    const int units = c->hist_chunks * c->hist_ng;
+    c->hist_grid = occ > 0 ? std::min(units, occ * c->cus) : units;
+  }
   // histogram kernels address a chunk's rows with 32-bit byte offsets. A leaf
   // of cnt rows takes min(hist_chunks, ceil(cnt / hist_min_rows)) chunks, so
   // no chunk is longer than max(hist_min_rows, ceil(N / hist_chunks)) rows.
@@ -1136,7 +1145,7 @@ static void enqueue_head(Ctx* c, Lane& ln, int k) {
   }
   hipLaunchKernelGGL(k_state_init, dim3(1), dim3(256), 0, s, ln.st.p, ln.pos.p, ln.leaves.p, ln.spl.p, (long long)R,
                      c->F);
-  const int hgrid = c->hist_chunks * c->hist_ng;
+  const int hgrid = c->hist_grid;
   prof_mark(c, k, 0, 0, 0);
   launch_hist(hist_args(c, ln, quant, k, 2), quant, c->hist_tpr, hgrid, c->hist_lds, s);
   prof_mark(c, k, 0, 0, 1);
@@ -1153,7 +1162,7 @@ static void enqueue_rounds(Ctx* c, Lane& ln, int k, int r0, int n) {
   const HistArgs ha = hist_args(c, ln, quant, k, 3);
   const SplitArgs sa = split_args(c, ln, k, false);
   const PartArgs pa = part_args(c, ln);
-  const int hgrid = c->hist_chunks * c->hist_ng;
+  const int hgrid = c->hist_grid;
   for (int r = r0; r < r0 + n; ++r) {
     const int step = r + 1;
     prof_mark(c, k, step, 2, 0);

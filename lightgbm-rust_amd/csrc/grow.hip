@@ -30,6 +30,10 @@ constexpr int kPartRowsPerThread = LGBMHIP_PART_ROWS;
 constexpr int kPartTile = kPartThreads * kPartRowsPerThread;
 constexpr int kRowThreads = kPartThreads / (2 * kMaxBatch);   // threads reducing one searched row
 static_assert(kRowThreads >= 1, "a searched row needs at least one thread");
+// The plan's selection runs on the replay's wave (step 3a) once the tree has
+// more than kWaveSelMin leaves, choosing at most kWaveSelMax leaves.
+constexpr int kWaveSelMin = 96;
+constexpr int kWaveSelMax = 16;
 
 __device__ __forceinline__ int child_buf(int buf) { return buf == 1 ? 2 : 1; }
 template <typename T>
@@ -526,6 +530,9 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
   }
   __syncthreads();
   const int units = s_u0[nitems];
+  // block 0 applied the plan: in a round it takes the last unit share, so
+  // the other blocks start on the histograms while it works
+  const int vb = a.mode == 3 ? (blockIdx.x == 0 ? (int)gridDim.x - 1 : (int)blockIdx.x - 1) : (int)blockIdx.x;
   int w = 0;
   if (Q) {
     constexpr int T = HistBlock<true>::threads;
@@ -540,7 +547,7 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
       unsigned* lds32 = (unsigned*)lds;
       for (int e = tid; e < min(a.FB, a.F) * a.B; e += T) lds32[e] = 0u;
       __syncthreads();
-      for (int u = blockIdx.x; u < units; u += gridDim.x) {
+      for (int u = vb; u < units; u += gridDim.x) {
         while (u >= s_u0[w + 1]) ++w;
         const int r = u - s_u0[w];
         hist_unit_q32<TPR>(a, lds32, s_begin[w], s_count[w], s_buf[w], r / a.NG, s_nch[w], r % a.NG,
@@ -550,7 +557,7 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
     }
   }
   if (!Q && blockIdx.x == 0 && tid < nitems) a.units[kUnitForm + tid] = 0;
-  for (int u = blockIdx.x; u < units; u += gridDim.x) {
+  for (int u = vb; u < units; u += gridDim.x) {
     while (u >= s_u0[w + 1]) ++w;
     const int r = u - s_u0[w];
     hist_unit<TPR, Q>(a, lds, s_begin[w], s_count[w], s_buf[w], s_sg[w], s_sh[w], r / a.NG, s_nch[w], r % a.NG,
@@ -621,6 +628,21 @@ __device__ __forceinline__ LeafSums leaf_sums(const SplitArgs& a, int slot, bool
   s.sh = sh + 2.0 * KEPS;
   // learner/mod.rs:1303-1309: the root's parent output is its own output
   s.po = root ? leaf_output_t(sg, sh, a.p, true, true, false, s.nd, 0.0) : L.output;
+  return s;
+}
+
+// The sums of a child of a split of the round, as apply_plan records them in
+// the child's leaf (cnt rows).
+__device__ __forceinline__ LeafSums child_sums(const SplitArgs& a, const SplitRec& rec, long long cnt, bool right) {
+  LeafSums s;
+  s.nd = (int)cnt;
+  s.isum = right ? rec.rgh : rec.lgh;
+  s.gs = a.it->grad_scale;
+  s.hs = a.it->hess_scale;
+  s.sg = right ? rec.rsg : rec.lsg;
+  s.sh = (right ? rec.rsh : rec.lsh) + 2.0 * KEPS;
+  const double o = right ? rec.right_output : rec.left_output;
+  s.po = isnan(o) ? 0.0 : o;
   return s;
 }
 
@@ -697,46 +719,50 @@ __global__ void __launch_bounds__(128) k_grow_split(SplitArgs a) {
   const bool root = a.root != 0;
   int sm, lg, parent, row_sm, row_lg;
   long long sc;
+  LeafSums ls_s, ls_l;
+  // the state, the plan item and the feature are loaded together
+  const FeatDev fd = a.feats[f];
   if (root) {
+    if (st->done) return;
     sm = 0;
     lg = -1;
     parent = -1;
     row_sm = 0;
     row_lg = -1;
     sc = a.leaves[0].count;
+    ls_s = ls_l = leaf_sums(a, 0, true);
   } else {
     const PlanDev* P = a.plan;
-    if (w >= P->nwork) return;
-    const PlanItem& it = P->item[w];
-    const int ls = it.lslot, rs = it.lslot + 1;
-    const long long lc = a.leaves[ls].count, rc = a.leaves[rs].count;
-    if (P->complete || children_skip(a.p, it.depth + 1, lc, rc)) {
+    const int sdone = st->done, nwork = P->nwork, complete = P->complete;
+    const PlanItem& it = P->item[w];   // w < kMaxBatch: inside the plan
+    // the left child's rows as the partition counted them (its slot
+    // counters are cleared by the next round's plan)
+    const long long lc = (long long)st->lcur[w], rc = it.count - lc;
+    const int depth = it.depth, ls = it.lslot;
+    if (sdone || w >= nwork) return;
+    if (complete || children_skip(a.p, depth + 1, lc, rc)) {
       // no search: both children keep (-inf, none)
       if (lane == 0 && wave < 2) {
         SplitRec r;
         reset_rec(r);
         a.feat_split[(2 * w + wave) * F + f] = r;
       }
-      if (f == 0 && w == 0 && tid < kMaxBatch) {
-        st->lcur[tid] = 0;
-        st->rcur[tid] = 0;
-      }
       return;
     }
     const bool left_smaller = lc < rc;   // learner/mod.rs:1200-1208
-    sm = left_smaller ? ls : rs;
-    lg = left_smaller ? rs : ls;
+    sm = left_smaller ? ls : ls + 1;
+    lg = left_smaller ? ls + 1 : ls;
     row_sm = 2 * w + (left_smaller ? 0 : 1);
     row_lg = 2 * w + (left_smaller ? 1 : 0);
     parent = it.parent;
     sc = left_smaller ? lc : rc;
+    // the children's sums from the applied split (the leaf records hold the
+    // same values)
+    const LeafSums l = child_sums(a, it.rec, lc, false), r = child_sums(a, it.rec, rc, true);
+    ls_s = left_smaller ? l : r;
+    ls_l = left_smaller ? r : l;
   }
-  // every load this block needs is independent once the item is known:
-  // issue them together, then work from registers and LDS
-  const FeatDev fd = a.feats[f];
   const bool parent_spl = root ? true : a.splittable[(size_t)parent * F + f] != 0;
-  const LeafSums ls_s = leaf_sums(a, sm, root);
-  const LeafSums ls_l = lg >= 0 ? leaf_sums(a, lg, false) : ls_s;
   constexpr int PB = kMaxBins / 128;   // bins per thread
   if (!a.quant) {
     const int ssg = fix_shift(sc, a.it->max_abs_g_bits);
@@ -867,17 +893,10 @@ __global__ void __launch_bounds__(128) k_grow_split(SplitArgs a) {
       a.splittable[(size_t)slot * F + f] = spl ? 1 : 0;
     }
   }
-  if (root) {
-    if (f == 0 && tid == 0) {
-      // learner/mod.rs:647: the root's output before any split
-      const LeafSums r = leaf_sums(a, 0, true);
-      st->root_output = round0(leaf_output_t(r.sg, r.sh - 2.0 * KEPS, a.p, true, true, false, r.nd, 0.0));
-    }
-  } else if (f == 0 && w == 0 && tid < kMaxBatch) {
-    // the next partition pass reserves slots from zero (nothing in this
-    // launch reads the counters: child counts come from the leaf records)
-    st->lcur[tid] = 0;
-    st->rcur[tid] = 0;
+  if (root && f == 0 && tid == 0) {
+    // learner/mod.rs:647: the root's output before any split
+    const LeafSums r = leaf_sums(a, 0, true);
+    st->root_output = round0(leaf_output_t(r.sg, r.sh - 2.0 * KEPS, a.p, true, true, false, r.nd, 0.0));
   }
 }
 
@@ -939,65 +958,6 @@ __device__ __forceinline__ Cand cand_none() {
   return c;
 }
 
-// The same order as packed keys: k1 = the gain as an order-preserving 64-bit
-// integer (larger is better), k2 = (feature, position) with a missing
-// feature last (smaller is better). Positions < 65536, features < 65535.
-struct KeyRec {
-  unsigned long long k1;
-  unsigned k2;
-  int spec;   // >= 0: split ahead of order
-};
-
-__device__ __forceinline__ unsigned long long gain_key(double g) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(g);
-  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-__device__ __forceinline__ double key_gain(unsigned long long k) {
-  const unsigned long long b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
-  return __longlong_as_double((long long)b);
-}
-__device__ __forceinline__ unsigned tie_key(int feat, int pos) {
-  return ((unsigned)(feat < 0 ? 0xFFFF : feat) << 16) | (unsigned)pos;
-}
-__device__ __forceinline__ bool key_better(unsigned long long a1, unsigned a2, unsigned long long b1, unsigned b2) {
-  return a1 > b1 || (a1 == b1 && a2 < b2);
-}
-
-// DPP step of a wave reduction: this lane's value and the one DPP control
-// CTRL brings from another lane (rows outside ROWMASK keep their own).
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ unsigned dpp_from(unsigned v) {
-  return (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROWMASK, 0xf, false);
-}
-
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ void dpp_best_step(unsigned& h, unsigned& l, unsigned& t) {
-  const unsigned oh = dpp_from<CTRL, ROWMASK>(h), ol = dpp_from<CTRL, ROWMASK>(l), ot = dpp_from<CTRL, ROWMASK>(t);
-  const unsigned long long a = ((unsigned long long)h << 32) | l, b = ((unsigned long long)oh << 32) | ol;
-  const bool take = key_better(b, ot, a, t);
-  h = take ? oh : h;
-  l = take ? ol : l;
-  t = take ? ot : t;
-}
-
-// Wave ArgMax of (k1, k2) pairs with DPP row operations (quad swaps, row
-// mirrors, row broadcasts): lane 63 ends with the best pair, read back into
-// every lane.
-__device__ __forceinline__ void wave_best_key(unsigned long long& k1, unsigned& k2) {
-  unsigned h = (unsigned)(k1 >> 32), l = (unsigned)k1, t = k2;
-  dpp_best_step<0xB1, 0xF>(h, l, t);    // quad_perm [1,0,3,2]
-  dpp_best_step<0x4E, 0xF>(h, l, t);    // quad_perm [2,3,0,1]
-  dpp_best_step<0x141, 0xF>(h, l, t);   // row_half_mirror
-  dpp_best_step<0x140, 0xF>(h, l, t);   // row_mirror
-  dpp_best_step<0x142, 0xA>(h, l, t);   // row_bcast:15 into rows 1, 3
-  dpp_best_step<0x143, 0xC>(h, l, t);   // row_bcast:31 into rows 2, 3
-  h = __builtin_amdgcn_readlane(h, 63);
-  l = __builtin_amdgcn_readlane(l, 63);
-  t = __builtin_amdgcn_readlane(t, 63);
-  k1 = ((unsigned long long)h << 32) | l;
-  k2 = t;
-}
-
 // The plan of a round, one block:
 //  1. each leaf searched by the last split pass gets its best split
 //     (learner/mod.rs:1064-1101);
@@ -1020,20 +980,21 @@ template <>
 struct RowVec<16> { using T = u32x4; };
 
 // Copies n records of RW words, record w from src(w) to dst(w) (a null dst
-// skips the record); each thread issues up to 8 word loads before its stores.
+// skips the record), on threads t of nt; each thread issues up to 8 word
+// loads before its stores.
 template <int RW, class Src, class Dst>
-__device__ __forceinline__ void copy_recs(int n, Src src, Dst dst) {
+__device__ __forceinline__ void copy_recs(int n, Src src, Dst dst, int t = threadIdx.x, int nt = kPartThreads) {
   const int total = n * RW;
-  for (int e0 = threadIdx.x; e0 < total; e0 += kPartThreads * 8) {
+  for (int e0 = t; e0 < total; e0 += nt * 8) {
     unsigned v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int e = min(e0 + u * kPartThreads, total - 1);
+      const int e = min(e0 + u * nt, total - 1);
       v[u] = src(e / RW)[e % RW];
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int e = e0 + u * kPartThreads;
+      const int e = e0 + u * nt;
       if (e < total) {
         unsigned* d = dst(e / RW);
         if (d) d[e % RW] = v[u];
@@ -1054,7 +1015,7 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
   __shared__ FeatDev ifd[kMaxBatch];
   __shared__ long long ibegin[kMaxBatch], icount[kMaxBatch];
   __shared__ int ipos[kMaxBatch], iparent[kMaxBatch], ibuf[kMaxBatch], idepth[kMaxBatch], itile0[kMaxBatch + 1];
-  __shared__ int s_k, s_fin, s_nseq, s_nspl, s_nrep, s_npend, s_complete;
+  __shared__ int s_k, s_fin, s_nseq, s_nspl, s_nrep, s_npend, s_complete, s_wk;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (st->done) return;
   const int nseq0 = st->nseq, npend0 = st->npending, nslots0 = st->nslots, nspl0 = st->num_splits;
@@ -1248,6 +1209,57 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
       --npend;
       if (lane == om || lane == on) lane_best(m1, m2);
     }
+    // ---- 3a. on a large tree the selection runs here, on the registers:
+    // the unsplit leaves are taken in ArgMax order, one at a time; each
+    // one's count of leaves split ahead of order that rank above it is a
+    // wave ballot. It stops where the rule of step 3 rejects a leaf or
+    // kWaveSelMax are chosen (a prefix of step 3's choice, which starts with
+    // the replay's stop). Late in a tree few leaves qualify, and this costs a
+    // few wave ArgMax steps instead of a rank for every leaf.
+    int wk = -1;
+    if (!fin && nseq > kWaveSelMin) {
+      const unsigned long long zero = gain_key(0.0);
+      const int room = L - nseq;
+      unsigned long long e1[RP];
+#pragma unroll
+      for (int r = 0; r < RP; ++r) e1[r] = rsp[r] < 0 ? r1[r] : 0ull;
+      auto lane_best_e = [&](unsigned long long& b1, unsigned& b2) {
+        b1 = e1[0];
+        b2 = r2[0];
+#pragma unroll
+        for (int r = 1; r < RP; ++r)
+          if (key_better(e1[r], r2[r], b1, b2)) {
+            b1 = e1[r];
+            b2 = r2[r];
+          }
+      };
+      unsigned long long x1;
+      unsigned x2;
+      lane_best_e(x1, x2);
+      int rank = 0;
+      while (rank < kWaveSelMax) {
+        unsigned long long c1 = x1;
+        unsigned c2 = x2;
+        wave_best_key(c1, c2);
+        if (!(c1 > zero)) break;
+        int above = 0;
+#pragma unroll
+        for (int r = 0; r < RP; ++r)
+          above += __popcll(__ballot(rsp[r] >= 0 && key_better(r1[r], r2[r], c1, c2)));
+        if (rank + above + 1 > room) break;
+        const int m = (int)(c2 & 0xFFFF);
+        const int om = m & 63, rm = m >> 6;
+        if (lane == 0) ipos[rank] = m;
+        ++rank;
+        if (lane == om) {
+#pragma unroll
+          for (int r = 0; r < RP; ++r)
+            if (r == rm) e1[r] = 0;
+          lane_best_e(x1, x2);
+        }
+      }
+      wk = rank;
+    }
     // LDS copies of this wave's final positions
 #pragma unroll
     for (int r = 0; r < RP; ++r) {
@@ -1275,7 +1287,24 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
       s_nspl = nspl;
       s_nrep = nrep;
       s_npend = npend;
+      s_wk = wk;
+      s_k = wk < 0 ? 0 : wk;
     }
+  } else {
+    // meanwhile the other waves store each searched leaf's best split in its
+    // slot (nothing in this launch reads those slots: a round's parents read
+    // their records from the searched rows)
+    copy_recs<RW>(
+        nsearched,
+        [&](int r) { return (const unsigned*)(a.feat_split + (size_t)r * F + max(rowbest[r].l, 0)); },
+        [&](int r) { return rowbest[r].l >= 0 ? (unsigned*)(a.best + sbase + r) : (unsigned*)nullptr; },
+        tid - 64, kPartThreads - 64);
+    for (int r = tid - 64; r < nsearched; r += kPartThreads - 64)
+      if (rowbest[r].l < 0) {
+        SplitRec z;
+        reset_rec(z);
+        a.best[sbase + r] = z;
+      }
   }
   __syncthreads();
   // ---- 3. selection, all threads: the round splits the unsplit leaves in
@@ -1286,9 +1315,7 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
   // order.
   {
     const int nseq = s_nseq;
-    if (tid == 0) s_k = 0;
-    __syncthreads();
-    if (!s_fin) {
+    if (s_wk < 0 && !s_fin) {
       const unsigned long long zero = gain_key(0.0);
       for (int i = tid; i < nseq; i += kPartThreads) {
         const KeyRec qi = Q[i];
@@ -1388,6 +1415,12 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
       it.pad = 0;
       it.fd = ifd[tid];
     }
+    // the partition pass reserves the children's slots from zero (the
+    // counters of the last round were read by its histogram and search passes)
+    if (tid < kMaxBatch) {
+      st->lcur[tid] = 0;
+      st->rcur[tid] = 0;
+    }
     if (tid == 0) {
       pl->done = s_fin;
       pl->nwork = k;
@@ -1402,16 +1435,6 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
     }
     PosRec* nxt = a.pos + (size_t)(par ^ 1) * L;
     for (int i = tid; i < s_nseq; i += kPartThreads) nxt[i] = P[i];
-    copy_recs<RW>(
-        nsearched,
-        [&](int r) { return (const unsigned*)(a.feat_split + (size_t)r * F + max(rowbest[r].l, 0)); },
-        [&](int r) { return rowbest[r].l >= 0 ? (unsigned*)(a.best + sbase + r) : (unsigned*)nullptr; });
-    for (int r = tid; r < nsearched; r += kPartThreads)
-      if (rowbest[r].l < 0) {
-        SplitRec z;
-        reset_rec(z);
-        a.best[sbase + r] = z;
-      }
   }
 }
 
@@ -1443,19 +1466,22 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
   __shared__ PlanItem items[kMaxBatch];
   __shared__ int s_hdr[3];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (st->done) return;   // changes only in the histogram pass: uniform here
-  // one round of loads: the plan's header and every item slot
+  // the tree state and the plan's header together (the state's done flag
+  // changes only in the histogram pass); blocks past the round's tiles stop
+  // there, the others load the round's items
   const PlanDev* pd = a.plan;
-  for (int e = tid; e < kMaxBatch * IW; e += kPartThreads) ((unsigned*)items)[e] = ((const unsigned*)pd->item)[e];
   if (tid == 0) {
-    s_hdr[0] = pd->done;
-    s_hdr[1] = pd->nwork;
-    s_hdr[2] = pd->total_tiles;
+    const int sd = st->done, pdn = pd->done, nw = pd->nwork, tt = pd->total_tiles;
+    s_hdr[0] = sd | pdn;
+    s_hdr[1] = nw;
+    s_hdr[2] = tt;
   }
   __syncthreads();
   if (s_hdr[0]) return;
   const int k = s_hdr[1], total_tiles = s_hdr[2];
-  if (k == 0) return;
+  if (k == 0 || (int)blockIdx.x >= total_tiles) return;
+  for (int e = tid; e < k * IW; e += kPartThreads) ((unsigned*)items)[e] = ((const unsigned*)pd->item)[e];
+  __syncthreads();
   // ---- tiles
   const int FB = a.FB;
   const size_t gstride = (size_t)a.N * FB;   // bytes of one feature group's array
@@ -2048,6 +2074,27 @@ static void launch_hist(const HistArgs& a, bool quant, int tpr, int grid, size_t
     case 8: launch_hist_t<8>(a, quant, grid, lds, s); break;
     case 16: launch_hist_t<16>(a, quant, grid, lds, s); break;
     default: launch_hist_t<32>(a, quant, grid, lds, s); break;
+  }
+}
+
+// Resident blocks per CU of the histogram kernel for this form, row layout
+// and LDS size (0 when the runtime cannot tell).
+template <int TPR>
+static int hist_occupancy_t(bool quant, size_t lds) {
+  int n = 0;
+  const hipError_t e =
+      quant ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_hist_int<TPR>, HistBlock<true>::threads, lds)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_hist_f64<TPR>, HistBlock<false>::threads, lds);
+  return e == hipSuccess ? n : 0;
+}
+static int hist_occupancy(bool quant, int tpr, size_t lds) {
+  switch (tpr) {
+    case 1: return hist_occupancy_t<1>(quant, lds);
+    case 2: return hist_occupancy_t<2>(quant, lds);
+    case 4: return hist_occupancy_t<4>(quant, lds);
+    case 8: return hist_occupancy_t<8>(quant, lds);
+    case 16: return hist_occupancy_t<16>(quant, lds);
+    default: return hist_occupancy_t<32>(quant, lds);
   }
 }
 

@@ -152,6 +152,110 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Wave reductions and scans on DPP row operations (no LDS round trips).
+// Minimum of an int over the wave, in every lane.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_min_step(int v) {
+  return min(v, __builtin_amdgcn_update_dpp(v, v, CTRL, ROWMASK, 0xf, false));
+}
+__device__ __forceinline__ int wave_min_i32(int v) {
+  v = dpp_min_step<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
+  v = dpp_min_step<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
+  v = dpp_min_step<0x141, 0xF>(v);   // row_half_mirror
+  v = dpp_min_step<0x140, 0xF>(v);   // row_mirror
+  v = dpp_min_step<0x142, 0xA>(v);   // row_bcast:15 into rows 1, 3
+  v = dpp_min_step<0x143, 0xC>(v);   // row_bcast:31 into rows 2, 3
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// The value of lane - k of the same 16-lane row (ROWMASK rows), 0 elsewhere.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ unsigned long long dpp_u64_or0(unsigned long long v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)v, CTRL, ROWMASK, 0xf, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), CTRL, ROWMASK, 0xf, false);
+  return ((unsigned long long)hi << 32) | lo;
+}
+// Inclusive prefix sum over the wave's lanes (wrapping 64-bit).
+__device__ __forceinline__ unsigned long long wave_incl_scan_u64(unsigned long long v) {
+  v += dpp_u64_or0<0x111, 0xF>(v);   // row_shr:1
+  v += dpp_u64_or0<0x112, 0xF>(v);   // row_shr:2
+  v += dpp_u64_or0<0x114, 0xF>(v);   // row_shr:4
+  v += dpp_u64_or0<0x118, 0xF>(v);   // row_shr:8
+  v += dpp_u64_or0<0x142, 0xA>(v);   // row_bcast:15 into rows 1, 3
+  v += dpp_u64_or0<0x143, 0xC>(v);   // row_bcast:31 into rows 2, 3
+  return v;
+}
+
+// The same order as packed keys: k1 = the gain as an order-preserving 64-bit
+// integer (larger is better), k2 = (feature, position) with a missing
+// feature last (smaller is better). Positions < 65536, features < 65535.
+struct KeyRec {
+  unsigned long long k1;
+  unsigned k2;
+  int spec;   // >= 0: split ahead of order
+};
+
+__device__ __forceinline__ unsigned long long gain_key(double g) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(g);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double key_gain(unsigned long long k) {
+  const unsigned long long b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+__device__ __forceinline__ unsigned tie_key(int feat, int pos) {
+  return ((unsigned)(feat < 0 ? 0xFFFF : feat) << 16) | (unsigned)pos;
+}
+__device__ __forceinline__ bool key_better(unsigned long long a1, unsigned a2, unsigned long long b1, unsigned b2) {
+  return a1 > b1 || (a1 == b1 && a2 < b2);
+}
+
+// DPP step of a wave reduction: this lane's value and the one DPP control
+// CTRL brings from another lane (rows outside ROWMASK keep their own).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ unsigned dpp_from(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROWMASK, 0xf, false);
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void dpp_best_step(unsigned& h, unsigned& l, unsigned& t) {
+  const unsigned oh = dpp_from<CTRL, ROWMASK>(h), ol = dpp_from<CTRL, ROWMASK>(l), ot = dpp_from<CTRL, ROWMASK>(t);
+  const unsigned long long a = ((unsigned long long)h << 32) | l, b = ((unsigned long long)oh << 32) | ol;
+  const bool take = key_better(b, ot, a, t);
+  h = take ? oh : h;
+  l = take ? ol : l;
+  t = take ? ot : t;
+}
+
+// Wave ArgMax of (k1, k2) pairs with DPP row operations (quad swaps, row
+// mirrors, row broadcasts): lane 63 ends with the best pair, read back into
+// every lane.
+__device__ __forceinline__ void wave_best_key(unsigned long long& k1, unsigned& k2) {
+  unsigned h = (unsigned)(k1 >> 32), l = (unsigned)k1, t = k2;
+  dpp_best_step<0xB1, 0xF>(h, l, t);    // quad_perm [1,0,3,2]
+  dpp_best_step<0x4E, 0xF>(h, l, t);    // quad_perm [2,3,0,1]
+  dpp_best_step<0x141, 0xF>(h, l, t);   // row_half_mirror
+  dpp_best_step<0x140, 0xF>(h, l, t);   // row_mirror
+  dpp_best_step<0x142, 0xA>(h, l, t);   // row_bcast:15 into rows 1, 3
+  dpp_best_step<0x143, 0xC>(h, l, t);   // row_bcast:31 into rows 2, 3
+  h = __builtin_amdgcn_readlane(h, 63);
+  l = __builtin_amdgcn_readlane(l, 63);
+  t = __builtin_amdgcn_readlane(t, 63);
+  k1 = ((unsigned long long)h << 32) | l;
+  k2 = t;
+}
+
+// The first maximum of a gain over the wave (the smallest k among equal
+// gains) in every lane; a lane without a candidate holds (-inf, INT32_MAX).
+__device__ __forceinline__ void wave_first_max(double& g, int& k) {
+  unsigned long long k1 = gain_key(g == 0.0 ? 0.0 : g);
+  unsigned k2 = (unsigned)k;
+  wave_best_key(k1, k2);
+  g = key_gain(k1);
+  k = (int)k2;
+}
+
+
 // Last block of the grid to arrive (release/acquire around a device counter).
 __device__ bool last_block(int* counter) {
   __shared__ int ticket;
@@ -762,7 +866,7 @@ __device__ __forceinline__ void scan_f64(const double* hg, const double* hh, con
     S.val[0][k] = gain;
   }
   // first break
-  for (int o = 32; o > 0; o >>= 1) kb = min(kb, __shfl_xor(kb, o, 64));
+  kb = wave_min_i32(kb);
   for (int k = lane; k < K; k += 64) {
     if (S.skipped[k] == 2 && k < kb) {
       spl = true;
@@ -770,11 +874,7 @@ __device__ __forceinline__ void scan_f64(const double* hg, const double* hh, con
       if (gain > bg || (gain == bg && k < bk)) { bg = gain; bk = k; }
     }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    double og = __shfl_xor(bg, o, 64);
-    int ok = __shfl_xor(bk, o, 64);
-    if (og > bg || (og == bg && ok < bk)) { bg = og; bk = ok; }
-  }
+  wave_first_max(bg, bk);
   int any = __any(spl);
   wave_sync();
   if (!any) return;
@@ -1123,8 +1223,10 @@ __device__ __forceinline__ void scan_int(const long long* hv, const FeatDev& fd,
   if (!reverse && na && off == 1) {
     unsigned long long part = 0;
     for (int i = lane; i < fd.num_bin - off; i += 64) part += (unsigned long long)hv[i + off];
-    for (int o = 32; o > 0; o >>= 1) part += (unsigned long long)__shfl_xor((long long)part, o, 64);
-    base = (unsigned long long)ls.isum - part;
+    part = wave_incl_scan_u64(part);
+    const unsigned long long total = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(part >> 32), 63) << 32) |
+                                     (unsigned)__builtin_amdgcn_readlane((int)(unsigned)part, 63);
+    base = (unsigned long long)ls.isum - total;
   }
   unsigned long long v4[4], run = 0;
 #pragma unroll
@@ -1141,11 +1243,7 @@ __device__ __forceinline__ void scan_int(const long long* hv, const FeatDev& fd,
     run += v;
     v4[j] = run;
   }
-  unsigned long long incl = run;
-  for (int o = 1; o < 64; o <<= 1) {
-    unsigned long long y = (unsigned long long)__shfl_up((long long)incl, o, 64);
-    if (lane >= o) incl += y;
-  }
+  const unsigned long long incl = wave_incl_scan_u64(run);
   const unsigned long long excl = incl - run + base;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -1172,7 +1270,7 @@ __device__ __forceinline__ void scan_int(const long long* hv, const FeatDev& fd,
     S.skipped[k] = 2;
     S.val[0][k] = gain;
   }
-  for (int o = 32; o > 0; o >>= 1) kb = min(kb, __shfl_xor(kb, o, 64));
+  kb = wave_min_i32(kb);
   double bg = -INFINITY;
   int bk = INT32_MAX;
   bool spl = false;
@@ -1183,11 +1281,7 @@ __device__ __forceinline__ void scan_int(const long long* hv, const FeatDev& fd,
       if (gain > bg || (gain == bg && k < bk)) { bg = gain; bk = k; }
     }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    double og = __shfl_xor(bg, o, 64);
-    int ok = __shfl_xor(bk, o, 64);
-    if (og > bg || (og == bg && ok < bk)) { bg = og; bk = ok; }
-  }
+  wave_first_max(bg, bk);
   int any = __any(spl);
   wave_sync();
   if (!any) return;
