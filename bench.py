@@ -221,6 +221,8 @@ def _train_forms(lgb, ds, base, args, quality):
     forms = {}
     for form in args.forms.split(","):
         params = dict(base)
+        if args.batch_trees is not None:
+            params["batch_trees"] = args.batch_trees
         if form == "quant":
             params.update(use_quantized_grad=True, num_grad_quant_bins=16, stochastic_rounding=True,
                           quant_train_renew_leaf=True)
@@ -504,6 +506,8 @@ def main():
     ap.add_argument("--forms", default="quant,plain")
     ap.add_argument("--no-graph", action="store_true", help="stream launches instead of graphs (counter passes)")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
+    ap.add_argument("--batch-trees", type=int, default=None,
+                    help="multiclass: class trees per shared growth launch (0: one tree per launch)")
     ap.add_argument("--prof-all", action="store_true",
                     help="count every iteration's histogram bytes, warm-up included (counter passes)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",

@@ -120,6 +120,8 @@ typedef struct {
   double top_rate;              /* GOSS: share of rows kept by |g h| and share sampled from the rest */
   double other_rate;
   int32_t bagging_by_query;     /* row bagging draws whole queries (query boundaries required) */
+  int32_t batch_trees;          /* num_class > 1: class trees per shared growth launch; 1: chosen by the engine,
+                                   0: one tree per launch */
 } lgbmhip_train_opts;
 
 /* One metric of lgbmhip_eval (metric.rs:17-43 MetricKind, the Config fields it reads). */

@@ -34,6 +34,7 @@ DEFAULT_PARAMS = dict(
     first_metric_only=False, early_stopping_min_delta=0.0, device_binning=False, bagging_fraction=1.0,
     bagging_freq=0, bagging_seed=3, pos_bagging_fraction=1.0, neg_bagging_fraction=1.0,
     data_sample_strategy="bagging", top_rate=0.2, other_rate=0.1, autotune=False, autotune_trees=4, bagging_by_query=False,
+    batch_trees=True,
 )
 
 ALIASES = {
@@ -215,6 +216,16 @@ class Tree:
     """Tree arrays (tree.rs:120-159)."""
 
 
+def _batch_trees(v):
+    """Class trees per shared growth launch: True = the engine's choice,
+    False or 0 = one tree per launch (growth lanes), n >= 2 = batches of n."""
+    if isinstance(v, str):
+        v = {"true": True, "false": False}.get(v.lower(), v)
+    if isinstance(v, bool):
+        return 1 if v else 0
+    return max(0, int(v))
+
+
 def _quant_table_threads(p):
     """The thread count the quantizer's random tables are drawn for
     (learner/quant.rs:45-79: one generator pair per thread block): the given
@@ -274,7 +285,8 @@ class Booster:
                              float(p["neg_bagging_fraction"]),
                              1 if str(p["data_sample_strategy"]) == "goss" or str(p.get("boosting")) == "goss" else 0,
                              int(p["num_threads"]) if int(p["num_threads"]) > 0 else 4096,
-                             float(p["top_rate"]), float(p["other_rate"]), int(_b(p["bagging_by_query"])))
+                             float(p["top_rate"]), float(p["other_rate"]), int(_b(p["bagging_by_query"])),
+                             _batch_trees(p["batch_trees"]))
         check(lib().lgbmhip_set_train_opts(h, C.byref(to)), h)
         self.tuning = None
         if _b(p["autotune"]):

@@ -54,7 +54,7 @@ class TrainOpts(C.Structure):
                 ("bagging_freq", C.c_int32), ("bagging_seed", C.c_int32), ("pos_bagging_fraction", C.c_double),
                 ("neg_bagging_fraction", C.c_double), ("data_sample_strategy", C.c_int32),
                 ("sample_threads", C.c_int32), ("top_rate", C.c_double), ("other_rate", C.c_double),
-                ("bagging_by_query", C.c_int32)]
+                ("bagging_by_query", C.c_int32), ("batch_trees", C.c_int32)]
 
 
 class MetricOpts(C.Structure):

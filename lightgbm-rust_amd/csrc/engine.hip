@@ -325,6 +325,21 @@ struct Ctx {
     }
   };
   std::vector<std::vector<std::pair<GraphKey, hipGraphExec_t>>> graphs, graphs_cont;
+  // batched trees: the class trees of an iteration share growth launches,
+  // `batch` trees per launch (lane k holds class k's tree; batch g, classes
+  // g * batch .. , runs on its first lane's stream), each kernel finding its
+  // tree's arguments in these tables (one entry per class)
+  int batch = 0;
+  int nbatch = 0;
+  DevBuf<PartArgs> tab_part;
+  DevBuf<HistArgs> tab_hist_root, tab_hist_round;
+  DevBuf<SplitArgs> tab_split_root, tab_split_round;
+  DevBuf<SlabTab> tab_slab;
+  std::vector<PartArgs> h_tab_part;
+  std::vector<HistArgs> h_tab_hist_root, h_tab_hist_round;
+  std::vector<SplitArgs> h_tab_split_root, h_tab_split_round;
+  std::vector<SlabTab> h_tab_slab;
+  std::vector<std::vector<std::pair<GraphKey, hipGraphExec_t>>> graphs_batch;   // [2 g] main, [2 g + 1] continuation
   std::vector<int> want_rounds;   // rounds the class's previous tree ran to completion
   int rounds_main = 24, rounds_cont = 8;
   std::vector<int> last_rounds;   // rounds launched for each class's last tree
@@ -385,7 +400,7 @@ struct Ctx {
     if (mws) metric_ws_free(mws);
     if (grad_ev) (void)hipEventDestroy(grad_ev);
     for (auto e : pev) if (e) (void)hipEventDestroy(e);
-    for (auto* gs : {&graphs, &graphs_cont})
+    for (auto* gs : {&graphs, &graphs_cont, &graphs_batch})
       for (auto& v : *gs)
         for (auto& g : v) (void)hipGraphExecDestroy(g.second);
     for (auto p : h_log) if (p) (void)hipHostFree(p);
@@ -707,7 +722,7 @@ static void reset_valid_scores(Ctx* c, ValidData& v) {
 // Destroys every captured tree launch (their kernels' arguments or the
 // validation sets they update changed).
 static void drop_graphs(Ctx* c) {
-  for (auto* gs : {&c->graphs, &c->graphs_cont})
+  for (auto* gs : {&c->graphs, &c->graphs_cont, &c->graphs_batch})
     for (auto& v : *gs) {
       for (auto& g : v) (void)hipGraphExecDestroy(g.second);
       v.clear();
@@ -800,14 +815,23 @@ static void setup_training(Ctx* c) {
   c->MS = 4 * L;
   const int MS = c->MS;
   // lanes: lane 0 runs on the context's stream; more lanes get streams of
-  // their own (default priority)
-  c->NL = std::min(K, hw_queue_lanes());
+  // their own (default priority). Batched trees: one lane per class, all on
+  // the context's stream.
+  // batch_trees 1 picks the size: three batches (on the multiclass
+  // benchmark, 10 classes, batches of 4 measured 52 it/s against 48 for 5,
+  // 48 for 3, 39 for one batch of 10 and 44 for one tree per launch)
+  const int bt = o.batch_trees == 1 ? (K + 2) / 3 : o.batch_trees;
+  c->batch = K > 1 && bt > 1 ? std::min(K, bt) : 0;
+  c->nbatch = c->batch ? (K + c->batch - 1) / c->batch : 0;
+  c->NL = c->batch ? K : std::min(K, hw_queue_lanes());
   c->lanes.clear();
   for (int l = 0; l < c->NL; ++l) {
     c->lanes.emplace_back(new Lane());
     Lane& ln = *c->lanes.back();
     if (l == 0) {
       ln.stream = c->stream;
+    } else if (c->batch && l % c->batch != 0) {
+      ln.stream = c->lanes[(size_t)(l / c->batch * c->batch)]->stream;   // the batch's stream
     } else {
       HIP_CHECK(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
       ln.own_stream = true;
@@ -990,6 +1014,15 @@ static void setup_training(Ctx* c) {
   drop_graphs(c);
   c->graphs.assign(K, {});
   c->graphs_cont.assign(K, {});
+  c->graphs_batch.assign(2 * std::max(1, c->nbatch), {});
+  if (c->batch) {
+    c->tab_part.alloc(K);
+    c->tab_hist_root.alloc(K);
+    c->tab_hist_round.alloc(K);
+    c->tab_split_root.alloc(K);
+    c->tab_split_round.alloc(K);
+    c->tab_slab.alloc(K);
+  }
   c->last_rounds.assign(K, 0);
   c->want_rounds.assign(K, 0);
   c->trees.clear();
@@ -1100,10 +1133,10 @@ static void slab_total(Ctx* c, Lane& ln, bool quant, hipStream_t s) {
                      ln.acc.p);
 }
 
-// The first launches of a tree for class k (gradients already in g/h and
+// The launches that open a tree for class k (gradients already in g/h and
 // max |g|, |h| in it[k]): identity partition or quantization with the root
-// sums, the growth state, the root's histogram and search.
-static void enqueue_head(Ctx* c, Lane& ln, int k) {
+// sums, and the growth state.
+static void enqueue_head_pre(Ctx* c, Lane& ln, int k) {
   hipStream_t s = ln.stream;
   const int64_t N = c->N;
   const bool quant = c->opts.use_quantized_grad != 0;
@@ -1145,6 +1178,14 @@ static void enqueue_head(Ctx* c, Lane& ln, int k) {
   }
   hipLaunchKernelGGL(k_state_init, dim3(1), dim3(256), 0, s, ln.st.p, ln.pos.p, ln.leaves.p, ln.spl.p, (long long)R,
                      c->F);
+}
+
+// The first launches of a tree for class k: enqueue_head_pre, then the
+// root's histogram and search.
+static void enqueue_head(Ctx* c, Lane& ln, int k) {
+  hipStream_t s = ln.stream;
+  const bool quant = c->opts.use_quantized_grad != 0;
+  enqueue_head_pre(c, ln, k);
   const int hgrid = c->hist_grid;
   prof_mark(c, k, 0, 0, 0);
   launch_hist(hist_args(c, ln, quant, k, 2), quant, c->hist_tpr, hgrid, c->hist_lds, s);
@@ -1239,6 +1280,7 @@ static void enqueue_tail(Ctx* c, Lane& ln, int k, bool score_update, double lr) 
       launch_walk(wv, grid_for(v->n, 256, c->cus * 4), s);
     }
   }
+  hipLaunchKernelGGL(k_tail_done, dim3(1), dim3(64), 0, s, ln.st.p);
 }
 
 static hipGraphExec_t capture(hipStream_t s, const std::function<void()>& body) {
@@ -1315,6 +1357,184 @@ static void launch_cont(Ctx* c, int k, int launched, bool score_update, double l
   HIP_CHECK(hipMemcpyAsync(c->h_st[k], ln.st.p, sizeof(StepDev), hipMemcpyDeviceToHost, ln.stream));
 }
 
+// ---------------------------------------------------------------- batched trees
+// The class trees of an iteration grow in batches: every growth launch
+// (plans, partitions, histograms, slab sums, searches) carries all trees of
+// a batch, each kernel taking its tree's arguments from a table in device
+// memory (grid y, or z for the search, is the tree). Each tree's blocks
+// loop over its pending leaves of uneven row counts as in a single tree's
+// launches; per-tree grids are the tuned ones divided among the batch's
+// trees. Batches run on their own streams, like growth lanes.
+static void fill_tabs(Ctx* c) {
+  const int K = c->K;
+  const bool quant = c->opts.use_quantized_grad != 0;
+  c->h_tab_part.resize(K);
+  c->h_tab_hist_root.resize(K);
+  c->h_tab_hist_round.resize(K);
+  c->h_tab_split_root.resize(K);
+  c->h_tab_split_round.resize(K);
+  c->h_tab_slab.resize(K);
+  for (int k = 0; k < K; ++k) {
+    Lane& ln = *c->lanes[k];
+    c->h_tab_part[k] = part_args(c, ln);
+    c->h_tab_hist_root[k] = hist_args(c, ln, quant, k, 2);
+    c->h_tab_hist_round[k] = hist_args(c, ln, quant, k, 3);
+    c->h_tab_split_root[k] = split_args(c, ln, k, true);
+    c->h_tab_split_round[k] = split_args(c, ln, k, false);
+    c->h_tab_slab[k] = SlabTab{ln.slab.p, ln.units.p, ln.acc.p};
+  }
+  hipStream_t s = c->stream;
+  HIP_CHECK(hipMemcpyAsync(c->tab_part.p, c->h_tab_part.data(), sizeof(PartArgs) * K, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(c->tab_hist_root.p, c->h_tab_hist_root.data(), sizeof(HistArgs) * K,
+                           hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(c->tab_hist_round.p, c->h_tab_hist_round.data(), sizeof(HistArgs) * K,
+                           hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(c->tab_split_root.p, c->h_tab_split_root.data(), sizeof(SplitArgs) * K,
+                           hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(c->tab_split_round.p, c->h_tab_split_round.data(), sizeof(SplitArgs) * K,
+                           hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(c->tab_slab.p, c->h_tab_slab.data(), sizeof(SlabTab) * K, hipMemcpyHostToDevice, s));
+}
+
+// Batch g: its first class, its tree count and its stream.
+struct BatchRef {
+  int k0, T;
+  hipStream_t s;
+};
+static BatchRef batch_ref(Ctx* c, int g) {
+  const int k0 = g * c->batch;
+  return BatchRef{k0, std::min(c->batch, c->K - k0), c->lanes[(size_t)k0]->stream};
+}
+
+// Blocks per tree of a batched launch: the single tree's grid shared among
+// the batch's T trees (at least `floor`).
+static int batch_grid(int grid, int T, int floor) { return std::max(floor, grid / T); }
+
+static void slab_total_batch(Ctx* c, const BatchRef& b, bool quant) {
+  const int per_unit = c->hist_fb * c->B * (quant ? 1 : 2);
+  const int64_t eblocks = ((int64_t)c->hist_ng * per_unit + 31) / 32;
+  const int grid = (int)std::min<int64_t>(eblocks * kMaxBatch, 4LL * c->cus);
+  hipLaunchKernelGGL(k_slab_reduce_tab, dim3(batch_grid(grid, b.T, 32), b.T), dim3(256), 0, b.s,
+                     c->tab_slab.p + b.k0, c->hist_ng, per_unit);
+}
+
+// Every tree's opening launches, then the batch's roots' histograms and searches.
+static void enqueue_head_batch(Ctx* c, const BatchRef& b) {
+  const bool quant = c->opts.use_quantized_grad != 0;
+  for (int k = b.k0; k < b.k0 + b.T; ++k) enqueue_head_pre(c, *c->lanes[k], k);
+  prof_mark(c, b.k0, 0, 0, 0);
+  launch_hist_tab(c->tab_hist_root.p + b.k0, b.T, quant, c->hist_tpr, batch_grid(c->hist_grid, b.T, 32),
+                  c->hist_lds, b.s);
+  prof_mark(c, b.k0, 0, 0, 1);
+  slab_total_batch(c, b, quant);
+  prof_mark(c, b.k0, 0, 3, 1);
+  launch_split_tab(c->tab_split_root.p + b.k0, b.T, c->F, true, b.s);
+  prof_mark(c, b.k0, 0, 1, 1);
+}
+
+static void enqueue_rounds_batch(Ctx* c, const BatchRef& b, int r0, int n) {
+  const bool quant = c->opts.use_quantized_grad != 0;
+  const PartArgs& pa = c->h_tab_part[(size_t)b.k0];
+  const int pgrid = batch_grid(c->part_grid, b.T, 32), hgrid = batch_grid(c->hist_grid, b.T, 32);
+  for (int r = r0; r < r0 + n; ++r) {
+    const int step = r + 1;
+    prof_mark(c, b.k0, step, 2, 0);
+    launch_part_tab(c->tab_part.p + b.k0, b.T, pa, pgrid, b.s);
+    prof_mark(c, b.k0, step, 2, 1);
+    launch_hist_tab(c->tab_hist_round.p + b.k0, b.T, quant, c->hist_tpr, hgrid, c->hist_lds, b.s);
+    prof_mark(c, b.k0, step, 0, 1);
+    slab_total_batch(c, b, quant);
+    prof_mark(c, b.k0, step, 3, 1);
+    launch_split_tab(c->tab_split_round.p + b.k0, b.T, c->F, false, b.s);
+    prof_mark(c, b.k0, step, 1, 1);
+  }
+}
+
+// One launch (a captured graph when graphs are on) of batch g: its trees'
+// opening launches when `head`, `rounds` batched rounds, every tree's tail;
+// then copies of the batch's growth states for the host.
+static void launch_batch(Ctx* c, int g, bool head, int rounds, int r0, bool score_update, double lr) {
+  const BatchRef b = batch_ref(c, g);
+  auto body = [&](int rr0) {
+    if (head) enqueue_head_batch(c, b);
+    enqueue_rounds_batch(c, b, rr0, rounds);
+    for (int k = b.k0; k < b.k0 + b.T; ++k) enqueue_tail(c, *c->lanes[k], k, score_update, lr);
+  };
+  const bool graph = c->opts.use_graph && !c->profile && !c->bag_on && !c->goss_on;
+  if (graph) {
+    const Ctx::GraphKey key{rounds, score_update, lr};
+    auto& gs = c->graphs_batch[(size_t)(2 * g + (head ? 0 : 1))];
+    hipGraphExec_t e = nullptr;
+    for (auto& x : gs)
+      if (x.first == key) e = x.second;
+    if (!e) {
+      e = capture(b.s, [&] { body(0); });
+      gs.emplace_back(key, e);
+    }
+    HIP_CHECK(hipGraphLaunch(e, b.s));
+  } else {
+    body(r0);
+  }
+  HIP_CHECK(hipGetLastError());
+  for (int k = b.k0; k < b.k0 + b.T; ++k)
+    HIP_CHECK(hipMemcpyAsync(c->h_st[k], c->lanes[k]->st.p, sizeof(StepDev), hipMemcpyDeviceToHost, b.s));
+}
+
+// The class trees of an iteration in batches: each batch's main launch of as
+// many rounds as its classes' previous trees needed, then continuation
+// launches while any of its trees still grows (a grown tree's launches
+// return at once and its tail is applied once). The host polls the batches
+// in turn; the context's stream ends ordered after every batch.
+static void grow_trees_batch(Ctx* c, bool score_update, double lr) {
+  const int K = c->K, NB = c->nbatch;
+  fill_tabs(c);
+  HIP_CHECK(hipEventRecord(c->grad_ev, c->stream));
+  std::vector<int> launched(NB, 0);
+  std::vector<bool> active(NB, true);
+  for (int g = 0; g < NB; ++g) {
+    const BatchRef b = batch_ref(c, g);
+    if (b.s != c->stream) HIP_CHECK(hipStreamWaitEvent(b.s, c->grad_ev, 0));
+    int rounds = 0;
+    for (int k = b.k0; k < b.k0 + b.T; ++k) rounds = std::max(rounds, main_rounds(c, k));
+    launch_batch(c, g, true, rounds, 0, score_update, lr);
+    launched[g] = rounds;
+  }
+  int left = NB;
+  while (left > 0) {
+    for (int g = 0; g < NB; ++g) {
+      if (!active[g]) continue;
+      const BatchRef b = batch_ref(c, g);
+      HIP_CHECK(hipStreamSynchronize(b.s));
+      bool all = true;
+      for (int k = b.k0; k < b.k0 + b.T; ++k) all = all && c->h_st[k]->done != 0;
+      if (!all) {
+        // every round replays or splits at least one leaf of each growing tree
+        if (launched[g] > c->L + 1) throw Error(LGBMHIP_INTERNAL, "tree growth did not finish");
+        launch_batch(c, g, false, c->rounds_cont, launched[g], score_update, lr);
+        launched[g] += c->rounds_cont;
+        continue;
+      }
+      for (int k = b.k0; k < b.k0 + b.T; ++k) {
+        Lane& ln = *c->lanes[k];
+        c->last_rounds[k] = launched[g];
+        c->want_rounds[k] = c->h_st[k]->rounds;
+        HIP_CHECK(hipMemcpyAsync(c->h_log[k], ln.log.p, sizeof(SplitRec) * c->L, hipMemcpyDeviceToHost, b.s));
+        HIP_CHECK(hipMemcpyAsync(c->h_raw[k], c->leafraw.p + (size_t)k * c->L, sizeof(double) * c->L,
+                                 hipMemcpyDeviceToHost, b.s));
+      }
+      active[g] = false;
+      left--;
+    }
+  }
+  for (int g = 1; g < NB; ++g) {
+    const BatchRef b = batch_ref(c, g);
+    Lane& ln = *c->lanes[(size_t)b.k0];
+    HIP_CHECK(hipEventRecord(ln.done_ev, b.s));
+    HIP_CHECK(hipStreamWaitEvent(c->stream, ln.done_ev, 0));
+  }
+  (void)K;
+}
+
 // The trees of classes 0 .. ntrees-1 (gradients already enqueued on the
 // context's stream), each to completion: lane l grows classes l, l + NL,
 // ... in order while the other lanes grow theirs; the host tops up a tree
@@ -1322,6 +1542,10 @@ static void launch_cont(Ctx* c, int k, int launched, bool score_update, double l
 // Leaves each tree's log, raw leaf outputs and state in the pinned copies,
 // and the context's stream ordered after every lane's last tree.
 static void grow_trees(Ctx* c, int ntrees, bool score_update, double lr) {
+  if (c->batch && ntrees == c->K) {
+    grow_trees_batch(c, score_update, lr);
+    return;
+  }
   const int NL = std::min(ntrees, c->NL);
   HIP_CHECK(hipEventRecord(c->grad_ev, c->stream));
   for (int l = 1; l < NL; ++l) HIP_CHECK(hipStreamWaitEvent(c->lanes[l]->stream, c->grad_ev, 0));
@@ -1518,6 +1742,20 @@ static void account_profile(Ctx* c, int k) {
   c->prof[10] += (double)st.num_splits;
 }
 
+// The same for a batch of T trees from class k0: one set of launches (events
+// on class k0's slots) for every tree's rows and bytes.
+static void account_profile_batch(Ctx* c, int k0, int T) {
+  account_profile(c, k0);
+  for (int k = k0 + 1; k < k0 + T; ++k) {
+    const StepDev& st = *c->h_st[k];
+    c->prof[2] += (double)st.hist_rows;
+    c->prof[3] += hist_bytes(c, (double)st.hist_rows, (double)st.hist_items);
+    c->prof[8] += (double)st.part_rows;
+    c->prof[9] += (double)st.computed;
+    c->prof[10] += (double)st.num_splits;
+  }
+}
+
 static void compact_bag(Ctx* c);
 
 // SampleStrategy::bagging (sample_strategy.rs:278-318) on the device: every
@@ -1633,7 +1871,10 @@ static int train_iteration(Ctx* c) {
   }
   for (int k = 0; k < K; ++k) {
     HostTree t = build_tree(c, k);
-    if (c->profile) account_profile(c, k);
+    if (c->profile) {
+      if (!c->batch) account_profile(c, k);
+      else if (k % c->batch == 0) account_profile_batch(c, k, std::min(c->batch, K - k));
+    }
     if (t.num_leaves > 1) {
       cont = true;
       t.shrink(c->opts.learning_rate);

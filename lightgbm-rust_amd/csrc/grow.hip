@@ -580,6 +580,21 @@ k_hist_int(HistArgs a) {
   hist_kernel<TPR, true>(a, lds_hist);
 }
 
+// The trees of a batch: grid y picks the tree, grid x is its blocks.
+template <int TPR>
+__global__ void __launch_bounds__(HistBlock<false>::threads) k_hist_f64_tab(const HistArgs* __restrict__ tab) {
+  extern __shared__ unsigned long long lds_hist[];
+  const HistArgs a = tab[blockIdx.y];
+  hist_kernel<TPR, false>(a, lds_hist);
+}
+template <int TPR>
+__global__ void __launch_bounds__(HistBlock<true>::threads) __attribute__((amdgpu_waves_per_eu(6)))
+k_hist_int_tab(const HistArgs* __restrict__ tab) {
+  extern __shared__ unsigned long long lds_hist[];
+  const HistArgs a = tab[blockIdx.y];
+  hist_kernel<TPR, true>(a, lds_hist);
+}
+
 // ---------------------------------------------------------------- split search
 struct SplitArgs {
   StepDev* st;
@@ -705,9 +720,8 @@ __device__ __forceinline__ void slab_sum(const SplitArgs& a, int w, int f, int t
 // histogram.rs:42-78), the larger child's as parent - smaller
 // (histogram.rs:59-85), then wave 0 searches the smaller child and wave 1 the
 // larger (learner/mod.rs:954-1101, split.rs:246-271 / 930-950).
-__global__ void __launch_bounds__(128) k_grow_split(SplitArgs a) {
+__device__ __forceinline__ void grow_split(const SplitArgs& a) {
   StepDev* st = a.st;
-  if (st->done) return;
   __shared__ double hsm[2][kMaxBins];   // smaller: g, h
   __shared__ double hlg[2][kMaxBins];   // larger: g, h
   __shared__ long long ism[kMaxBins], ilg[kMaxBins];
@@ -900,6 +914,13 @@ __global__ void __launch_bounds__(128) k_grow_split(SplitArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(128) k_grow_split(SplitArgs a) { grow_split(a); }
+// The trees of a batch: grid z picks the tree.
+__global__ void __launch_bounds__(128) k_grow_split_tab(const SplitArgs* __restrict__ tab) {
+  const SplitArgs a = tab[blockIdx.z];
+  grow_split(a);
+}
+
 // ---------------------------------------------------------------- partition
 struct PartArgs {
   StepDev* st;
@@ -1004,7 +1025,7 @@ __device__ __forceinline__ void copy_recs(int n, Src src, Dst dst, int t = threa
 }
 
 template <int RP>   // positions per lane: num_leaves <= 64 RP
-__global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
+__device__ __forceinline__ void grow_plan(const PartArgs& a) {
   StepDev* st = a.st;
   extern __shared__ PosRec P[];   // this block's copy of the position array, then its keys
   KeyRec* Q = (KeyRec*)(P + a.L);
@@ -1438,6 +1459,15 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) {
   }
 }
 
+template <int RP>
+__global__ void __launch_bounds__(kPartThreads) k_grow_plan(PartArgs a) { grow_plan<RP>(a); }
+// The trees of a batch: block y plans tree y.
+template <int RP>
+__global__ void __launch_bounds__(kPartThreads) k_grow_plan_tab(const PartArgs* __restrict__ tab) {
+  const PartArgs a = tab[blockIdx.y];
+  grow_plan<RP>(a);
+}
+
 // One partition pass per round: every block moves its tiles' rows -
 // gradients and group-major bins - into the children's segments of the other
 // buffer: left rows from the front, right rows from the back, each tile
@@ -1456,7 +1486,7 @@ struct GradVec<8> { using T = uint2; };
 // last row) and issued before any of the loaded values is used, so a
 // thread's loads are in flight together; stores follow, masked.
 template <int VW, int GB>
-__global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
+__device__ __forceinline__ void grow_part(const PartArgs& a) {
   using VT = typename RowVec<VW>::T;
   using GT = typename GradVec<GB>::T;
   constexpr int VWW = VW / 4;   // words per vector
@@ -1608,6 +1638,15 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) {
   }
 }
 
+template <int VW, int GB>
+__global__ void __launch_bounds__(kPartThreads) k_grow_part(PartArgs a) { grow_part<VW, GB>(a); }
+// The trees of a batch: grid y picks the tree, grid x is its blocks.
+template <int VW, int GB>
+__global__ void __launch_bounds__(kPartThreads) k_grow_part_tab(const PartArgs* __restrict__ tab) {
+  const PartArgs a = tab[blockIdx.y];
+  grow_part<VW, GB>(a);
+}
+
 // Rows of the row-major bins (stride S) into the group-major layout of the
 // ordered buffers: out[g][i][0..FB) = bins of features g*FB.. of row
 // rows[i] (or row i when rows is null), zero past the last feature.
@@ -1680,7 +1719,7 @@ __global__ void __launch_bounds__(256) k_tree_nodes(const StepDev* st, const Spl
   __shared__ int lparent[1024];   // node whose child is leaf l (-1: the root)
   __shared__ int lside[1024];
   const int ns = st->num_splits;
-  if (!st->done) return;
+  if (st->done != 1) return;
   const int tid = threadIdx.x;
   for (int i = tid; i < ns; i += blockDim.x) {
     const SplitRec& r = log[i];
@@ -1803,7 +1842,7 @@ __global__ void __launch_bounds__(256) k_walk(WalkArgs a) {
   unsigned* ncat = (unsigned*)(lval + a.L);                        // [L][kCatWords]
   int* lsh = (int*)(ncat + (size_t)a.L * kCatWords);               // [2 L]
   const StepDev* st = a.st;
-  if (MODE != 2 && !st->done) return;
+  if (MODE != 2 && st->done != 1) return;
   const int ns = st->num_splits, nl = ns + 1;
   const int tid = threadIdx.x;
   for (int i = tid; i < ns; i += blockDim.x) {
@@ -1917,7 +1956,7 @@ __global__ void __launch_bounds__(256) k_score_leaf(const StepDev* st, const uin
                                                     const double* __restrict__ val, double* __restrict__ score,
                                                     long long n) {
   __shared__ double lv[1024];
-  if (!st->done) return;
+  if (st->done != 1) return;
   const int nl = st->num_splits + 1;
   for (int l = threadIdx.x; l < nl; l += blockDim.x) lv[l] = val[l];
   __syncthreads();
@@ -1940,6 +1979,13 @@ __global__ void __launch_bounds__(256) k_score_leaf(const StepDev* st, const uin
   }
 }
 
+// The tree's end has been applied: its state moves from grown (1) to done
+// (2), so a later launch of the tail kernels (a continuation launch that
+// grows other trees of a batch) leaves this tree's scores alone.
+__global__ void k_tail_done(StepDev* st) {
+  if (threadIdx.x == 0 && st->done == 1) st->done = 2;
+}
+
 // Shrunk leaf values (tree.rs:299-319): out[l] = round0(output * lr); with
 // `renew`, output = round0(refit_leaf_output(sum g, sum h)). raw[l] keeps the
 // unshrunk output for the host's tree arrays.
@@ -1947,7 +1993,7 @@ __global__ void k_leaf_values(const StepDev* st, const PosRec* pos, const LeafDe
                               long long* renew_sums, int renew, SplitParamsDev p, double lr, int L, double* out,
                               double* raw) {
   const int l = threadIdx.x + blockIdx.x * blockDim.x;
-  if (l >= L || !st->done) return;
+  if (l >= L || st->done != 1) return;
   if (l >= st->nseq) {
     out[l] = 0.0;
     raw[l] = 0.0;
@@ -1974,8 +2020,8 @@ __global__ void k_leaf_values(const StepDev* st, const PosRec* pos, const LeafDe
 // consecutive entries of one item, each over the chunks c = s, s + 8, ... on
 // slice s of its 8 thread slices; the slices meet in LDS. Wrapping int64
 // sums: exact in any order.
-__global__ void __launch_bounds__(256) k_slab_reduce(const long long* __restrict__ slab, const int* units, int NG,
-                                                     int per_unit, long long* __restrict__ out) {
+__device__ __forceinline__ void slab_reduce(const long long* __restrict__ slab, const int* units, int NG,
+                                            int per_unit, long long* __restrict__ out) {
   __shared__ unsigned long long part[8][32];
   const int t = threadIdx.x, bo = t & 31, sl = t >> 5;
   const int nitems = units[kMaxBatch + 1];
@@ -2028,6 +2074,21 @@ __global__ void __launch_bounds__(256) k_slab_reduce(const long long* __restrict
     }
     __syncthreads();
   }
+}
+
+__global__ void __launch_bounds__(256) k_slab_reduce(const long long* __restrict__ slab, const int* units, int NG,
+                                                     int per_unit, long long* __restrict__ out) {
+  slab_reduce(slab, units, NG, per_unit, out);
+}
+// The trees of a batch: grid y picks the tree.
+struct SlabTab {
+  const long long* slab;
+  const int* units;
+  long long* out;
+};
+__global__ void __launch_bounds__(256) k_slab_reduce_tab(const SlabTab* __restrict__ tab, int NG, int per_unit) {
+  const SlabTab t = tab[blockIdx.y];
+  slab_reduce(t.slab, t.units, NG, per_unit, t.out);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -2098,19 +2159,42 @@ static int hist_occupancy(bool quant, int tpr, size_t lds) {
   }
 }
 
+template <int TPR>
+static void launch_hist_tab_t(const HistArgs* tab, int T, bool quant, int grid, size_t lds, hipStream_t s) {
+  if (quant) hipLaunchKernelGGL(k_hist_int_tab<TPR>, dim3(grid, T), dim3(HistBlock<true>::threads), lds, s, tab);
+  else hipLaunchKernelGGL(k_hist_f64_tab<TPR>, dim3(grid, T), dim3(HistBlock<false>::threads), lds, s, tab);
+}
+
+// The histogram pass of T trees (tab: their arguments in device memory).
+static void launch_hist_tab(const HistArgs* tab, int T, bool quant, int tpr, int grid, size_t lds, hipStream_t s) {
+  switch (tpr) {
+    case 1: launch_hist_tab_t<1>(tab, T, quant, grid, lds, s); break;
+    case 2: launch_hist_tab_t<2>(tab, T, quant, grid, lds, s); break;
+    case 4: launch_hist_tab_t<4>(tab, T, quant, grid, lds, s); break;
+    case 8: launch_hist_tab_t<8>(tab, T, quant, grid, lds, s); break;
+    case 16: launch_hist_tab_t<16>(tab, T, quant, grid, lds, s); break;
+    default: launch_hist_tab_t<32>(tab, T, quant, grid, lds, s); break;
+  }
+}
+
 static void set_hist_lds_limits(size_t bytes) {
   const void* ks[] = {(const void*)k_hist_f64<1>, (const void*)k_hist_f64<2>, (const void*)k_hist_f64<4>,
                       (const void*)k_hist_f64<8>, (const void*)k_hist_f64<16>, (const void*)k_hist_f64<32>,
                       (const void*)k_hist_int<1>, (const void*)k_hist_int<2>, (const void*)k_hist_int<4>,
-                      (const void*)k_hist_int<8>, (const void*)k_hist_int<16>, (const void*)k_hist_int<32>};
+                      (const void*)k_hist_int<8>, (const void*)k_hist_int<16>, (const void*)k_hist_int<32>,
+                      (const void*)k_hist_f64_tab<1>, (const void*)k_hist_f64_tab<2>, (const void*)k_hist_f64_tab<4>,
+                      (const void*)k_hist_f64_tab<8>, (const void*)k_hist_f64_tab<16>, (const void*)k_hist_f64_tab<32>,
+                      (const void*)k_hist_int_tab<1>, (const void*)k_hist_int_tab<2>, (const void*)k_hist_int_tab<4>,
+                      (const void*)k_hist_int_tab<8>, (const void*)k_hist_int_tab<16>, (const void*)k_hist_int_tab<32>};
   for (const void* k : ks) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 static size_t part_lds(int L) { return (size_t)L * (sizeof(PosRec) + sizeof(KeyRec)); }
 static void set_part_lds_limit(size_t bytes) {
-  (void)hipFuncSetAttribute((const void*)k_grow_plan<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  (void)hipFuncSetAttribute((const void*)k_grow_plan<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  (void)hipFuncSetAttribute((const void*)k_grow_plan<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  const void* ks[] = {(const void*)k_grow_plan<4>, (const void*)k_grow_plan<8>, (const void*)k_grow_plan<16>,
+                      (const void*)k_grow_plan_tab<4>, (const void*)k_grow_plan_tab<8>,
+                      (const void*)k_grow_plan_tab<16>};
+  for (const void* k : ks) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 static void launch_split(const SplitArgs& a, hipStream_t s) {
@@ -2135,5 +2219,28 @@ static void launch_part(const PartArgs& a, int grid, hipStream_t s) {
 }
 
 static int part_tile() { return kPartTile; }
+
+// The split pass of T trees (grid z picks the tree).
+static void launch_split_tab(const SplitArgs* tab, int T, int F, bool root, hipStream_t s) {
+  hipLaunchKernelGGL(k_grow_split_tab, dim3(F, root ? 1 : kMaxBatch, T), dim3(128), 0, s, tab);
+}
+
+// The plans and partition passes of T trees; `a` is any of them (the shapes
+// they share: leaves, group width, gradient width).
+static void launch_part_tab(const PartArgs* tab, int T, const PartArgs& a, int grid, hipStream_t s) {
+  const dim3 pg(1, T), g(grid, T);
+  if (a.L <= 256) hipLaunchKernelGGL(k_grow_plan_tab<4>, pg, dim3(kPartThreads), part_lds(a.L), s, tab);
+  else if (a.L <= 512) hipLaunchKernelGGL(k_grow_plan_tab<8>, pg, dim3(kPartThreads), part_lds(a.L), s, tab);
+  else hipLaunchKernelGGL(k_grow_plan_tab<16>, pg, dim3(kPartThreads), part_lds(a.L), s, tab);
+  if (a.gh_bytes == 2) {
+    if (a.FB % 16 == 0) hipLaunchKernelGGL((k_grow_part_tab<16, 2>), g, dim3(kPartThreads), 0, s, tab);
+    else if (a.FB % 8 == 0) hipLaunchKernelGGL((k_grow_part_tab<8, 2>), g, dim3(kPartThreads), 0, s, tab);
+    else hipLaunchKernelGGL((k_grow_part_tab<4, 2>), g, dim3(kPartThreads), 0, s, tab);
+  } else {
+    if (a.FB % 16 == 0) hipLaunchKernelGGL((k_grow_part_tab<16, 8>), g, dim3(kPartThreads), 0, s, tab);
+    else if (a.FB % 8 == 0) hipLaunchKernelGGL((k_grow_part_tab<8, 8>), g, dim3(kPartThreads), 0, s, tab);
+    else hipLaunchKernelGGL((k_grow_part_tab<4, 8>), g, dim3(kPartThreads), 0, s, tab);
+  }
+}
 
 }  // namespace lgbmhip
