@@ -21,15 +21,19 @@ Prints one JSON line (the driver's contract) and exits 0.
 
 Kernel timing: the timed iterations run as captured HIP graphs with no
 instrumentation; --prof-iters further iterations then run with HIP events
-around every histogram / slab-reduction / split / partition launch on the
-stream the kernels run on, which gives the histogram kernel's average launch
-duration (its own launch only, plus the dispatch gap before it) and GB/s. HBM
-traffic per histogram launch comes from rocprofv3 FETCH_SIZE / WRITE_SIZE
-passes over this same command with --prof-all (every iteration's algorithmic
-histogram bytes counted), scaled by the counters' reading on a copy of known
-size (--copy-calibrate): their ratio of HBM bytes to algorithmic bytes over
-the same dispatches is recorded in profiles/hist_traffic.json, and `traffic`
-is that ratio times this run's algorithmic bytes per launch.
+around every partition / histogram / slab-reduction / split launch on the
+stream the kernels run on, which gives each kernel's average launch duration
+(its own launch plus the dispatch gap before it). `roofline` is the dominant
+kernel, the partition pass, with the algorithmic bytes of an index partition
+(9 per moved row: the split feature's bin, a row index read and written);
+`roofline_hist` is the histogram kernel (the metric's hist GB/s). HBM traffic
+per launch comes from rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this same
+command with --prof-all (every iteration's algorithmic bytes counted), scaled
+by the counters' reading on a copy of known size (--copy-calibrate): the
+ratios of HBM bytes to algorithmic bytes over the same dispatches are
+recorded in profiles/hist_traffic.json, and `traffic` is that ratio times this
+run's algorithmic bytes per launch. `iteration` is the whole iteration's
+algorithmic bytes over its time.
 """
 import argparse
 import importlib
@@ -195,9 +199,10 @@ def copy_calibrate(mib):
                       "copy_gbs_read_plus_write": 2 * mib * (1 << 20) / (ms * 1e-3) / 1e9}), flush=True)
 
 
-def recorded_traffic(form, rows, max_bin, num_leaves):
-    """HBM bytes per algorithmic byte of the histogram launches, from the
-    committed counter passes, when they were taken on this exact workload."""
+def recorded_traffic(form, rows, max_bin, num_leaves, kernel="hist"):
+    """HBM bytes per algorithmic byte of the histogram (or partition)
+    launches, from the committed counter passes, when they were taken on
+    this exact workload."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
@@ -206,7 +211,28 @@ def recorded_traffic(form, rows, max_bin, num_leaves):
     w = t.get("workload", {})
     if (w.get("form"), w.get("rows"), w.get("max_bin"), w.get("num_leaves")) != (form, rows, max_bin, num_leaves):
         return None
+    if kernel == "partition":
+        return t.get("partition", {}).get("hbm_per_algorithmic_byte")
     return t.get("hbm_per_algorithmic_byte")
+
+
+# Algorithmic bytes of a partition pass per moved row (the split
+# feature's bin, a row index read and written); the engine moves each row's
+# group-major bins and gradients instead, so every later pass reads its
+# leaves contiguously.
+PART_ALG_BYTES_PER_ROW = 1 + 4 + 4
+
+
+def iteration_bytes(p, n, F, B, quant, K=1):
+    """Algorithmic bytes of the profiled iterations: histograms,
+    gradients, quantization, partitions, score updates and split searches."""
+    c_out = 8 if quant else 16
+    its = max(p["iters"], 1)
+    per_iter = (n * K * (8 + 1 + 8)                       # gradients
+                + (n * K * (8 + 2 + 16) if quant else 0)  # quantizer (random tables read)
+                + n * K * (8 + 8 + 2))                    # score update
+    searches = p["splits_computed"] + its * K             # every computed split, and each root
+    return p["hist_bytes"] + p["part_rows"] * PART_ALG_BYTES_PER_ROW + per_iter * its + searches * 2 * F * B * c_out
 
 
 def _line(metric, value, unit, args, ms, dtype, config, **extra):
@@ -575,11 +601,26 @@ def main():
     p = m["prof"]
     ratio = recorded_traffic(main_form, args.rows, args.max_bin, args.num_leaves)
     traffic = ratio * m["hist_bytes_per_launch"] if ratio else None
-    roof = dict(bound="hbm", achieved=m["hist_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=(m["hist_gbs"] / HBM_PEAK_GBS) if m["hist_gbs"] else None,
-                traffic=traffic, traffic_per_algorithmic_byte=ratio,
-                kernel="k_hist_int" if main_form == "quant" else "k_hist_f64",
-                bytes_per_launch=m["hist_bytes_per_launch"], avg_launch_us=m["hist_avg_launch_us"])
+    roof_hist = dict(bound="hbm", achieved=m["hist_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                     frac=(m["hist_gbs"] / HBM_PEAK_GBS) if m["hist_gbs"] else None,
+                     traffic=traffic, traffic_per_algorithmic_byte=ratio,
+                     kernel="k_hist_int" if main_form == "quant" else "k_hist_f64",
+                     bytes_per_launch=m["hist_bytes_per_launch"], avg_launch_us=m["hist_avg_launch_us"])
+    # the dominant kernel by time: the partition pass (one launch per growth
+    # round; every launch but the roots' histograms is a round's)
+    part_launches = max(p["hist_launches"] - p["iters"], 1)
+    part_bytes = p["part_rows"] * PART_ALG_BYTES_PER_ROW / part_launches
+    part_us = 1000.0 * p["part_ms"] / part_launches
+    pratio = recorded_traffic(main_form, args.rows, args.max_bin, args.num_leaves, "partition")
+    moved = 2 * (m["hist_fb"] * m["hist_ng"] + (2 if main_form == "quant" else 8))   # bins and gradients, read and written
+    roof = dict(bound="hbm", achieved=part_bytes / part_us / 1e3 if part_us > 0 else None, peak=HBM_PEAK_GBS,
+                unit="GB/s", frac=(part_bytes / part_us / 1e3 / HBM_PEAK_GBS) if part_us > 0 else None,
+                traffic=pratio * part_bytes if pratio else None, traffic_per_algorithmic_byte=pratio,
+                kernel="k_grow_part", bytes_per_launch=part_bytes, avg_launch_us=part_us,
+                moved_bytes_per_row=moved,
+                moved_gbs=p["part_rows"] * moved / part_launches / part_us / 1e3 if part_us > 0 else None)
+    it_bytes = iteration_bytes(p, args.rows, ds.num_features, args.max_bin, main_form == "quant")
+    it_gbs = it_bytes / max(p["iters"], 1) / (m["ms_per_iter"] * 1e-3) / 1e9
     # LDS-atomic throughput of the histogram kernel against the device's
     # conflict-free ds_add_u32 rate: one update per (row, feature) is the
     # algorithmic count; the kernel issues one per (row, feature slot of its
@@ -589,7 +630,7 @@ def main():
     if hist_s > 0:
         alg = p["hist_rows"] * ds.num_features / hist_s
         lds = dict(achieved=alg, issued=p["hist_rows"] * m["hist_fb"] * m["hist_ng"] / hist_s, peak=m["lds_peak"],
-                   unit="lane atomics/s", frac=alg / m["lds_peak"], kernel=roof["kernel"])
+                   unit="lane atomics/s", frac=alg / m["lds_peak"], kernel=roof_hist["kernel"])
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
@@ -634,7 +675,11 @@ def main():
                           hist_ms_per_iter=v["prof"]["hist_ms"] / max(v["prof"]["iters"], 1))
                   for k, v in forms.items()},
         "roofline": roof,
+        "roofline_hist": roof_hist,
         "lds_atomics": lds,
+        "iteration": dict(algorithmic_bytes_per_iter=it_bytes / max(p["iters"], 1), gbs=it_gbs,
+                          frac=it_gbs / HBM_PEAK_GBS,
+                          note="SURVEY 8(d) bytes of the profiled iterations over the timed ms per iteration"),
         "cpu_baseline": cpu,
     }
     # quantized against plain at each max_bin: the timed steps, the full runs

@@ -334,7 +334,7 @@ lgbmhip_status lgbmhip_bins_feature_infos(const lgbmhip_bins* b, char* out, int6
  * -1 only reads. out12 (may be NULL) receives the totals since enabling:
  * {histogram-kernel ms, histogram launches (one per growth round plus the
  * root's), rows summed into histograms, algorithmic histogram bytes, split ms,
- * partition ms (with the round's plan), iteration ms, iterations, rows moved
+ * partition-kernel ms (the round's plan excluded), iteration ms, iterations, rows moved
  * by partitions, splits computed, splits in the trees, slab-reduction ms}.
  * Each span runs from the previous event on the same stream, so it includes
  * that launch's dispatch gap. No counterpart in the reference (it has no

@@ -1206,8 +1206,9 @@ static void enqueue_rounds(Ctx* c, Lane& ln, int k, int r0, int n) {
   const int hgrid = c->hist_grid;
   for (int r = r0; r < r0 + n; ++r) {
     const int step = r + 1;
+    launch_plan(pa, s);
     prof_mark(c, k, step, 2, 0);
-    launch_part(pa, c->part_grid, s);
+    launch_partition(pa, c->part_grid, s);
     prof_mark(c, k, step, 2, 1);
     launch_hist(ha, quant, c->hist_tpr, hgrid, c->hist_lds, s);
     prof_mark(c, k, step, 0, 1);
@@ -1438,8 +1439,9 @@ static void enqueue_rounds_batch(Ctx* c, const BatchRef& b, int r0, int n) {
   const int pgrid = batch_grid(c->part_grid, b.T, 32), hgrid = batch_grid(c->hist_grid, b.T, 32);
   for (int r = r0; r < r0 + n; ++r) {
     const int step = r + 1;
+    launch_plan_tab(c->tab_part.p + b.k0, b.T, pa, b.s);
     prof_mark(c, b.k0, step, 2, 0);
-    launch_part_tab(c->tab_part.p + b.k0, b.T, pa, pgrid, b.s);
+    launch_partition_tab(c->tab_part.p + b.k0, b.T, pa, pgrid, b.s);
     prof_mark(c, b.k0, step, 2, 1);
     launch_hist_tab(c->tab_hist_round.p + b.k0, b.T, quant, c->hist_tpr, hgrid, c->hist_lds, b.s);
     prof_mark(c, b.k0, step, 0, 1);

@@ -2201,12 +2201,15 @@ static void launch_split(const SplitArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_grow_split, dim3(a.F, a.root ? 1 : kMaxBatch), dim3(128), 0, s, a);
 }
 
-// The round's plan (one block), then the partition pass; rows move in the
-// widest vectors that divide a group's bytes.
-static void launch_part(const PartArgs& a, int grid, hipStream_t s) {
+// The round's plan (one block).
+static void launch_plan(const PartArgs& a, hipStream_t s) {
   if (a.L <= 256) hipLaunchKernelGGL(k_grow_plan<4>, dim3(1), dim3(kPartThreads), part_lds(a.L), s, a);
   else if (a.L <= 512) hipLaunchKernelGGL(k_grow_plan<8>, dim3(1), dim3(kPartThreads), part_lds(a.L), s, a);
   else hipLaunchKernelGGL(k_grow_plan<16>, dim3(1), dim3(kPartThreads), part_lds(a.L), s, a);
+}
+
+// The partition pass; rows move in the widest vectors that divide a group's bytes.
+static void launch_partition(const PartArgs& a, int grid, hipStream_t s) {
   if (a.gh_bytes == 2) {
     if (a.FB % 16 == 0) hipLaunchKernelGGL((k_grow_part<16, 2>), dim3(grid), dim3(kPartThreads), 0, s, a);
     else if (a.FB % 8 == 0) hipLaunchKernelGGL((k_grow_part<8, 2>), dim3(grid), dim3(kPartThreads), 0, s, a);
@@ -2227,11 +2230,14 @@ static void launch_split_tab(const SplitArgs* tab, int T, int F, bool root, hipS
 
 // The plans and partition passes of T trees; `a` is any of them (the shapes
 // they share: leaves, group width, gradient width).
-static void launch_part_tab(const PartArgs* tab, int T, const PartArgs& a, int grid, hipStream_t s) {
-  const dim3 pg(1, T), g(grid, T);
+static void launch_plan_tab(const PartArgs* tab, int T, const PartArgs& a, hipStream_t s) {
+  const dim3 pg(1, T);
   if (a.L <= 256) hipLaunchKernelGGL(k_grow_plan_tab<4>, pg, dim3(kPartThreads), part_lds(a.L), s, tab);
   else if (a.L <= 512) hipLaunchKernelGGL(k_grow_plan_tab<8>, pg, dim3(kPartThreads), part_lds(a.L), s, tab);
   else hipLaunchKernelGGL(k_grow_plan_tab<16>, pg, dim3(kPartThreads), part_lds(a.L), s, tab);
+}
+static void launch_partition_tab(const PartArgs* tab, int T, const PartArgs& a, int grid, hipStream_t s) {
+  const dim3 g(grid, T);
   if (a.gh_bytes == 2) {
     if (a.FB % 16 == 0) hipLaunchKernelGGL((k_grow_part_tab<16, 2>), g, dim3(kPartThreads), 0, s, tab);
     else if (a.FB % 8 == 0) hipLaunchKernelGGL((k_grow_part_tab<8, 2>), g, dim3(kPartThreads), 0, s, tab);
