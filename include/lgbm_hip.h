@@ -167,6 +167,17 @@ lgbmhip_status lgbmhip_find_mappers(const float* X, int64_t n, int32_t ncol, con
 lgbmhip_status lgbmhip_create(int32_t device, lgbmhip_ctx** out);
 void lgbmhip_destroy(lgbmhip_ctx* ctx);
 
+/* How the next upload stores its rows in HBM: u4 = 1 (the default) keeps
+ * the features of at most 16 bins as 4-bit fields, two per byte after the
+ * byte-wide features; u4 = 0 keeps every feature a byte. Replaces the
+ * reference's storage choice BinKind::dense (bin.rs:314-324), whose Dense4
+ * (bin.rs:92-121) packs the same features. Trees do not depend on it. */
+lgbmhip_status lgbmhip_set_bin_packing(lgbmhip_ctx* ctx, int32_t u4);
+/* The uploaded rows: out3 = {bytes per packed row, row stride in bytes,
+ * features stored as 4-bit fields}. No counterpart in the reference
+ * (Dataset's storage sizes): for reports and tests. */
+lgbmhip_status lgbmhip_bin_layout(lgbmhip_ctx* ctx, int32_t* out3);
+
 /* Uploads the binned matrix once (row-major and feature-major copies in HBM). */
 lgbmhip_status lgbmhip_upload_bins(lgbmhip_ctx* ctx, const lgbmhip_feature_meta* metas, int32_t num_features,
                                    const double* upper, const uint8_t* bins, int64_t num_rows, int32_t num_total_features);

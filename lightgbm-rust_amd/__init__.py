@@ -34,7 +34,7 @@ DEFAULT_PARAMS = dict(
     first_metric_only=False, early_stopping_min_delta=0.0, device_binning=False, bagging_fraction=1.0,
     bagging_freq=0, bagging_seed=3, pos_bagging_fraction=1.0, neg_bagging_fraction=1.0,
     data_sample_strategy="bagging", top_rate=0.2, other_rate=0.1, autotune=False, autotune_trees=4, bagging_by_query=False,
-    batch_trees=True,
+    batch_trees=True, pack_u4=True,
 )
 
 ALIASES = {
@@ -257,6 +257,8 @@ class Booster:
         h = C.c_void_p()
         check(lib().lgbmhip_create(int(p["gpu_device_id"]), C.byref(h)))
         self._ctx = h
+        # features of at most 16 bins stored as 4-bit fields in HBM (bin.rs:92-121 Dense4)
+        check(lib().lgbmhip_set_bin_packing(h, int(_b(p["pack_u4"]))), h)
         ds = train_set
         if ds._raw is not None:
             check(lib().lgbmhip_upload_raw(h, ds._handle, ptr(ds._raw), ds.num_data, ds.num_total_features, 0), h)
@@ -572,6 +574,13 @@ class Booster:
         0 keeps a value). Trees do not depend on them."""
         check(lib().lgbmhip_set_tuning(self._ctx, int(part_grid), int(hist_chunks), int(hist_min_rows),
                                        int(hist_lds_budget), int(rounds_main), int(rounds_cont)), self._ctx)
+
+    def bin_layout(self):
+        """The training rows in HBM: bytes per packed row, its padded stride,
+        and how many features are stored as 4-bit fields."""
+        out = np.zeros(3, np.int32)
+        check(lib().lgbmhip_bin_layout(self._ctx, ptr(out)), self._ctx)
+        return dict(zip(("row_bytes", "row_stride", "u4_features"), out.tolist()))
 
     def get_tuning(self):
         out = np.zeros(9, np.int32)

@@ -129,6 +129,8 @@ SIGNATURES = {
     "lgbmhip_set_tuning": (I32, [P, I32, I32, I32, I32, I32, I32]),
     "lgbmhip_autotune": (I32, [P, I32, P]),
     "lgbmhip_get_tuning": (I32, [P, P]),
+    "lgbmhip_set_bin_packing": (I32, [P, I32]),
+    "lgbmhip_bin_layout": (I32, [P, P]),
 }
 
 _lib = None

@@ -66,9 +66,29 @@ struct BinFeat {
   int col, num_bin, missing_type, bin_type, off, ncat;
 };
 
-// Per used feature, as the split search and partition read it (split.rs:18-29).
+// Per used feature, as the split search and partition read it (split.rs:18-29),
+// and where its bins live: byte `pos` of a packed row (nib -1: the whole
+// byte; 0 / 1: the low / high 4 bits of a feature with at most 16 bins), and
+// its histogram's first entry `hoff` in the histogram unit of group `hg`.
 struct FeatDev {
   int num_bin, missing_type, default_bin, mfb, bin_type, offset;
+  int pos, nib, hg, hoff;
+};
+
+// Byte p of a packed row as the histogram kernels read it: the bin of its
+// low field is (byte & mask_lo), added at entry off_lo + bin of its group's
+// histogram unless it equals mfb_lo (the feature's most frequent bin, never
+// accumulated); a byte holding two 4-bit fields adds its high field the same
+// way. Fields without a feature have mask 0 and mfb 0, so they never add.
+struct ByteDev {
+  int off_lo, off_hi;
+  unsigned mask_lo, mask_hi, mfb_lo, mfb_hi;
+};
+
+// Byte p of a packed row as the binning pass writes it: the features binned
+// into its low and high bits (-1: none; hi >= 0 only for 4-bit fields).
+struct BinByte {
+  int lo, hi;
 };
 
 // Split parameters in device form (split.rs:33-46).

@@ -261,11 +261,20 @@ struct Ctx {
   // data
   int64_t N = 0;
   int F = 0, S = 0, B = 0, ncol = 0;
+  // packed rows: W bytes (the u8 features in order, then the features of at
+  // most 16 bins two per byte; pack_u4 = 0 keeps every feature a byte)
+  int W = 0;
+  int pack_u4 = 1;
+  bool has_u4 = false;
+  std::vector<BinByte> byte_feats;   // [W] the features of each byte
   std::vector<lgbmhip_feature_meta> metas;
   std::vector<std::vector<double>> upper;
   std::vector<FeatDev> feats;
   DevBuf<uint8_t> bins_rm, bins_cm;
   DevBuf<FeatDev> d_feats;
+  DevBuf<ByteDev> d_bytes;        // [NG * FB] the histogram kernels' view of each byte (padding included)
+  DevBuf<int> d_ginfo;            // [NG][2] each histogram group's entries and whether it holds 4-bit fields
+  DevBuf<BinByte> d_binbyte;      // [W] byte_feats on the device (binning into packed rows)
   // value_to_bin tables of the used features (device binning)
   DevBuf<BinFeat> d_binfeat;
   DevBuf<double> d_bounds;
@@ -305,6 +314,7 @@ struct Ctx {
   std::unique_ptr<Mt> start_eng;
   int n_calls = 0;
   int hist_tpr = 8, hist_fb = 0, hist_ng = 1, hist_chunks = 256, hist_min_rows = 2048;
+  int hist_pu = 0;       // entries of a histogram unit (the largest group's)
   int hist_grid = 256;   // blocks of a histogram launch (they loop over the units)
   size_t hist_lds = 0;
   int part_grid = 1024;
@@ -446,7 +456,7 @@ static void launch_group_rows(Ctx* c, const unsigned* rows, int64_t m, uint8_t* 
     }
   } else {
     hipLaunchKernelGGL(k_group_rows, dim3(grid_for(m * c->hist_ng * (c->hist_fb / 4), 256, c->cus * 16)), dim3(256), 0,
-                       s, rows, (long long)m, c->bins_rm.p, c->S, c->F, c->hist_fb, c->hist_ng, (long long)c->N, out);
+                       s, rows, (long long)m, c->bins_rm.p, c->S, c->W, c->hist_fb, c->hist_ng, (long long)c->N, out);
   }
   HIP_CHECK(hipGetLastError());
 }
@@ -484,11 +494,37 @@ static void set_features(Ctx* c, const lgbmhip_feature_meta* metas, int F, const
       throw Error(LGBMHIP_INVALID_DATA, "bin index out of range in feature meta");
     c->upper.emplace_back(upper + off, upper + off + m.num_bin);
     off += (size_t)m.num_bin;
-    FeatDev d{m.num_bin, m.missing_type, m.default_bin, m.most_freq_bin, m.bin_type, m.most_freq_bin == 0 ? 1 : 0};
+    FeatDev d{m.num_bin, m.missing_type, m.default_bin, m.most_freq_bin, m.bin_type, m.most_freq_bin == 0 ? 1 : 0,
+              0, -1, 0, 0};
     c->feats.push_back(d);
     c->B = std::max(c->B, m.num_bin);
   }
-  c->S = row_stride(F);
+  // packed rows: bytes of the u8 features in feature order, then the
+  // features of at most 16 bins as 4-bit fields, two per byte (Dense4 of
+  // bin.rs:92-121 and BinKind::dense :314-324 pick 4 bits the same way)
+  {
+    int n8 = 0, n4 = 0;
+    auto u4 = [&](int f) { return c->pack_u4 && c->feats[(size_t)f].num_bin <= 16; };
+    for (int f = 0; f < F; ++f) (u4(f) ? n4 : n8)++;
+    c->W = n8 + (n4 + 1) / 2;
+    c->has_u4 = n4 > 0;
+    c->byte_feats.assign((size_t)c->W, BinByte{-1, -1});
+    int p8 = 0, j4 = 0;
+    for (int f = 0; f < F; ++f) {
+      FeatDev& d = c->feats[(size_t)f];
+      if (u4(f)) {
+        d.pos = n8 + j4 / 2;
+        d.nib = j4 % 2;
+        (d.nib ? c->byte_feats[(size_t)d.pos].hi : c->byte_feats[(size_t)d.pos].lo) = f;
+        ++j4;
+      } else {
+        d.pos = p8++;
+        d.nib = -1;
+        c->byte_feats[(size_t)d.pos].lo = f;
+      }
+    }
+  }
+  c->S = row_stride(c->W);
   std::vector<BinFeat> bfs;
   std::vector<double> bounds;
   std::vector<int> ck, cb;
@@ -525,10 +561,22 @@ static void set_features(Ctx* c, const lgbmhip_feature_meta* metas, int F, const
   }
   c->d_feats.alloc(F);
   HIP_CHECK(hipMemcpy(c->d_feats.p, c->feats.data(), F * sizeof(FeatDev), hipMemcpyHostToDevice));
+  c->d_binbyte.alloc((size_t)c->W);
+  HIP_CHECK(hipMemcpy(c->d_binbyte.p, c->byte_feats.data(), c->W * sizeof(BinByte), hipMemcpyHostToDevice));
   c->state_ready = false;
   c->trees.clear();
   c->pred_dirty = true;
   c->valid.clear();
+}
+
+// One row of u8 bins (F of them, feature order) into its packed form (out
+// holds zeros: the row's S bytes).
+static void pack_row(const Ctx* c, const uint8_t* in, uint8_t* out) {
+  for (int f = 0; f < c->F; ++f) {
+    const FeatDev& d = c->feats[(size_t)f];
+    if (d.nib < 0) out[d.pos] = in[f];
+    else out[d.pos] = (uint8_t)(out[d.pos] | ((in[f] & 0xF) << (4 * d.nib)));
+  }
 }
 
 static void upload_bins(Ctx* c, const lgbmhip_feature_meta* metas, int F, const double* upper, const uint8_t* bins,
@@ -538,37 +586,39 @@ static void upload_bins(Ctx* c, const lgbmhip_feature_meta* metas, int F, const 
     if (bins[i] >= (uint8_t)c->metas[i % F].num_bin && c->metas[i % F].num_bin < 256)
       throw Error(LGBMHIP_INVALID_DATA, "bin value >= num_bin");
   std::vector<uint8_t> rm((size_t)N * c->S, 0), cm((size_t)N * F);
-  for (int64_t r = 0; r < N; ++r)
-    for (int f = 0; f < F; ++f) {
-      uint8_t v = bins[(size_t)r * F + f];
-      rm[(size_t)r * c->S + f] = v;
-      cm[(size_t)f * N + r] = v;
-    }
+  for (int64_t r = 0; r < N; ++r) {
+    pack_row(c, bins + (size_t)r * F, &rm[(size_t)r * c->S]);
+    for (int f = 0; f < F; ++f) cm[(size_t)f * N + r] = bins[(size_t)r * F + f];
+  }
   c->bins_rm.alloc(rm.size());
   c->bins_cm.alloc(cm.size());
   HIP_CHECK(hipMemcpy(c->bins_rm.p, rm.data(), rm.size(), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->bins_cm.p, cm.data(), cm.size(), hipMemcpyHostToDevice));
 }
 
-// Bins device rows (rows x ncol f32) with the training mappers into u8 rows
-// of `stride` bytes on the context's stream.
-static void launch_bin_rows(Ctx* c, const float* dX, int64_t rows, int ncol, uint8_t* dout, int stride) {
+// Bins device rows (rows x ncol f32) with the training mappers into rows of
+// `stride` bytes on the context's stream: packed rows (the training layout)
+// or, with packed = false, one u8 bin per feature.
+static void launch_bin_rows(Ctx* c, const float* dX, int64_t rows, int ncol, uint8_t* dout, int stride,
+                            bool packed) {
   if (ncol != c->ncol) throw Error(LGBMHIP_INVALID_DATA, "column count differs from the training data");
   if (rows <= 0) return;
   const int64_t work = rows * ((stride + 3) / 4);
   const int nbounds = (int)c->d_bounds.n;
   const size_t lds = (size_t)nbounds * 8;
+  const BinByte* map = c->d_binbyte.p;
   if (lds <= 64 * 1024) {
     // every feature's bounds in LDS: 1024-thread blocks, two per CU at 64 KiB
-    HIP_CHECK(hipFuncSetAttribute((const void*)k_bin_rows<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds));
-    hipLaunchKernelGGL(k_bin_rows<true>, dim3(grid_for(work, 1024, c->cus * 4)), dim3(1024), lds, c->stream, dX,
-                       (long long)rows, ncol, c->d_binfeat.p, c->F, c->d_bounds.p, nbounds, c->d_ckeys.p,
-                       c->d_cbins.p, dout, stride);
+    auto k = packed ? k_bin_rows<true, true> : k_bin_rows<true, false>;
+    HIP_CHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k, dim3(grid_for(work, 1024, c->cus * 4)), dim3(1024), lds, c->stream, dX, (long long)rows,
+                       ncol, c->d_binfeat.p, c->F, c->d_bounds.p, nbounds, c->d_ckeys.p, c->d_cbins.p, dout, stride,
+                       map, c->W);
   } else {
-    hipLaunchKernelGGL(k_bin_rows<false>, dim3(grid_for(work, 256, c->cus * 16)), dim3(256), 0, c->stream, dX,
-                       (long long)rows, ncol, c->d_binfeat.p, c->F, c->d_bounds.p, nbounds, c->d_ckeys.p,
-                       c->d_cbins.p, dout, stride);
+    auto k = packed ? k_bin_rows<false, true> : k_bin_rows<false, false>;
+    hipLaunchKernelGGL(k, dim3(grid_for(work, 256, c->cus * 16)), dim3(256), 0, c->stream, dX, (long long)rows, ncol,
+                       c->d_binfeat.p, c->F, c->d_bounds.p, nbounds, c->d_ckeys.p, c->d_cbins.p, dout, stride, map,
+                       c->W);
   }
   HIP_CHECK(hipGetLastError());
 }
@@ -592,7 +642,7 @@ static void upload_raw(Ctx* c, const Binned& b, const float* X, int64_t N, int n
   c->bins_rm.alloc((size_t)N * c->S);
   c->bins_cm.alloc((size_t)N * F);
   if (on_device) {
-    launch_bin_rows(c, X, N, ncol, c->bins_rm.p, c->S);
+    launch_bin_rows(c, X, N, ncol, c->bins_rm.p, c->S, true);
   } else {
     const int64_t chunk = std::max<int64_t>(1, ((int64_t)64 << 20) / ncol);   // 256 MiB of f32 per copy
     DevBuf<float> stage;
@@ -600,49 +650,95 @@ static void upload_raw(Ctx* c, const Binned& b, const float* X, int64_t N, int n
     for (int64_t r0 = 0; r0 < N; r0 += chunk) {
       const int64_t m = std::min(chunk, N - r0);
       HIP_CHECK(hipMemcpyAsync(stage.p, X + (size_t)r0 * ncol, (size_t)m * ncol * 4, hipMemcpyHostToDevice, c->stream));
-      launch_bin_rows(c, stage.p, m, ncol, c->bins_rm.p + (size_t)r0 * c->S, c->S);
+      launch_bin_rows(c, stage.p, m, ncol, c->bins_rm.p + (size_t)r0 * c->S, c->S, true);
       HIP_CHECK(hipStreamSynchronize(c->stream));
     }
   }
   hipLaunchKernelGGL(k_rm_to_cm, dim3(grid_for(N * F, 256, c->cus * 16)), dim3(256), 0, c->stream, c->bins_rm.p,
-                     (long long)N, c->S, F, c->bins_cm.p);
+                     (long long)N, c->S, F, c->d_feats.p, c->bins_cm.p);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
-// Histogram launch shape: features per block so the block's LDS histogram
-// fits the budget; threads per row = 4-byte chunks of the group, to a power of 2.
+// Histogram launch shape over the packed row's bytes: bytes per group so the
+// block's LDS histogram fits the budget (a u8 feature's byte takes B entries,
+// a byte of two 4-bit fields 32); threads per row = 4-byte chunks of the
+// group, to a power of 2. Then each feature's group and first entry in its
+// group's histogram unit, each byte's view for the histogram kernels, and
+// each group's entry count (a unit holds hist_pu entries, the largest).
 static void plan_hist(Ctx* c, bool quant) {
-  const size_t per_feat = (size_t)c->B * 2 * (quant ? 4 : 8);
+  const int W = c->W, B = c->B;
+  const size_t per_feat = (size_t)B * 2 * (quant ? 4 : 8);
   // 32 KiB by default: five blocks share a CU's LDS and each block zeroes and
   // flushes a small histogram, which wins over wide feature groups at every
   // leaf size measured on 10.5M x 28 (quantized and FP64 forms)
   size_t budget = std::min<size_t>(c->lds_max, 160 * 1024);
   budget = std::min<size_t>(budget, c->tune_hist_lds > 0 ? (size_t)c->tune_hist_lds : (size_t)32 * 1024);
   // at most 128 bytes (32 threads of 4 bins) of a row per block
-  int fb = (int)std::min<size_t>(std::min<size_t>((size_t)c->F, 128), budget / per_feat);
+  int fb = (int)std::min<size_t>(std::min<size_t>((size_t)W, 128), budget / per_feat);
   fb = std::max(4, fb / 4 * 4);
   // the quantized form also holds 32-bit bins and trash words beside its
   // int64 sums (about 12 bytes per bin): narrow the group until that fits
   if (quant)
-    while (fb > 4 && q32_lds_bytes(std::min(fb, c->F) * c->B) > std::min<size_t>(c->lds_max, 160 * 1024)) fb -= 4;
-  if (fb >= c->F) fb = (c->F + 3) / 4 * 4;
-  int ng = (c->F + fb - 1) / fb;
+    while (fb > 4 && q32_lds_bytes(std::min(fb, W) * B) > std::min<size_t>(c->lds_max, 160 * 1024)) fb -= 4;
+  if (fb >= W) fb = (W + 3) / 4 * 4;
+  int ng = (W + fb - 1) / fb;
   // balance the groups
-  fb = ((c->F + ng - 1) / ng + 3) / 4 * 4;
-  ng = (c->F + fb - 1) / fb;
+  fb = ((W + ng - 1) / ng + 3) / 4 * 4;
+  ng = (W + fb - 1) / fb;
   int chunks4 = fb / 4;
   int tpr = 1;
   while (tpr < chunks4) tpr <<= 1;
   if (tpr > 32) throw Error(LGBMHIP_UNSUPPORTED, "feature group wider than 128 bytes");
+  // entries of each group's histogram, in byte order (a 4-bit pair: the low
+  // field's 16 bins, then the high field's)
+  std::vector<ByteDev> bytes((size_t)ng * fb, ByteDev{0, 0, 0u, 0u, 0u, 0u});
+  std::vector<int> ginfo((size_t)2 * ng, 0);   // per group: entries, holds 4-bit fields
+  int pu = 0;
+  for (int g = 0; g < ng; ++g) {
+    int off = 0;
+    for (int p = g * fb; p < std::min(W, (g + 1) * fb); ++p) {
+      const BinByte bf = c->byte_feats[(size_t)p];
+      ByteDev& bd = bytes[(size_t)p];
+      FeatDev& lo = c->feats[(size_t)bf.lo];
+      lo.hg = g;
+      lo.hoff = off;
+      bd.off_lo = off;
+      bd.mfb_lo = (unsigned)lo.mfb;
+      if (lo.nib < 0) {
+        bd.mask_lo = 0xFFu;
+        off += B;
+      } else {
+        bd.mask_lo = 0xFu;
+        if (bf.hi >= 0) {
+          FeatDev& hi = c->feats[(size_t)bf.hi];
+          hi.hg = g;
+          hi.hoff = off + 16;
+          bd.off_hi = off + 16;
+          bd.mask_hi = 0xFu;
+          bd.mfb_hi = (unsigned)hi.mfb;
+        }
+        off += 32;
+        ginfo[(size_t)2 * g + 1] = 1;
+      }
+    }
+    ginfo[(size_t)2 * g] = off;
+    pu = std::max(pu, off);
+  }
   c->hist_fb = fb;
   c->hist_ng = ng;
   c->hist_tpr = tpr;
-  c->hist_lds = (size_t)std::min(fb, c->F) * per_feat;
+  c->hist_pu = pu;
+  c->hist_lds = (size_t)pu * (quant ? 8 : 16);
   // the quantized form's 32-bit bins, trash words and int64 sums
-  if (quant) c->hist_lds = std::max(c->hist_lds, q32_lds_bytes(std::min(fb, c->F) * c->B));
+  if (quant) c->hist_lds = std::max(c->hist_lds, q32_lds_bytes(pu));
   if (c->hist_lds > c->lds_max) throw Error(LGBMHIP_UNSUPPORTED, "histogram does not fit in LDS");
   set_hist_lds_limits(c->hist_lds);
+  c->d_bytes.alloc(bytes.size());
+  HIP_CHECK(hipMemcpy(c->d_bytes.p, bytes.data(), bytes.size() * sizeof(ByteDev), hipMemcpyHostToDevice));
+  c->d_ginfo.alloc(ginfo.size());
+  HIP_CHECK(hipMemcpy(c->d_ginfo.p, ginfo.data(), ginfo.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->d_feats.p, c->feats.data(), c->F * sizeof(FeatDev), hipMemcpyHostToDevice));
 }
 
 // Host-side tables of the lambdarank objective, computed as the reference
@@ -755,7 +851,7 @@ static void set_shapes(Ctx* c, bool quant) {
   // beyond what the CUs hold at once only add dispatch latency to the
   // small launches late in a tree
   {
-    const int occ = hist_occupancy(quant, c->hist_tpr, c->hist_lds);
+    const int occ = hist_occupancy(quant, c->has_u4, c->hist_tpr, c->hist_lds);
     const int units = c->hist_chunks * c->hist_ng;
     c->hist_grid = occ > 0 ? std::min(units, occ * c->cus) : units;
   }
@@ -773,9 +869,9 @@ static void set_shapes(Ctx* c, bool quant) {
   const size_t gbytes = (size_t)c->hist_ng * N * c->hist_fb;
   for (auto& lp : c->lanes) {
     Lane& ln = *lp;
-    ln.acc.alloc((size_t)kMaxBatch * c->hist_ng * c->hist_fb * B * (quant ? 1 : 2));
+    ln.acc.alloc((size_t)kMaxBatch * c->hist_ng * c->hist_pu * (quant ? 1 : 2));
     HIP_CHECK(hipMemcpy(ln.acc_units.p, au.data(), au.size() * sizeof(int), hipMemcpyHostToDevice));
-    ln.slab.alloc((size_t)max_units * c->hist_fb * B * (quant ? 1 : 2));
+    ln.slab.alloc((size_t)max_units * c->hist_pu * (quant ? 1 : 2));
     for (int b = 1; b < 3; ++b) ln.grp[b].alloc(gbytes);
   }
   // buffer 0 (the root's rows, gathered by the caller) and a bag's buffer 0
@@ -1046,10 +1142,14 @@ static HistArgs hist_args(Ctx* c, Lane& ln, bool quant, int k, int mode) {
   a.log = ln.log.p;
   a.best = ln.best.p;
   a.feats = c->d_feats.p;
+  a.bytes = c->d_bytes.p;
+  a.ginfo = c->d_ginfo.p;
   a.F = c->F;
   a.B = c->B;
   a.FB = c->hist_fb;
   a.NG = c->hist_ng;
+  a.pu = c->hist_pu;
+  a.u4 = c->has_u4 ? 1 : 0;
   a.L = c->L;
   a.N = c->N;
   a.max_chunks = c->hist_chunks;
@@ -1078,6 +1178,7 @@ static SplitArgs split_args(Ctx* c, Lane& ln, int k, bool root) {
   a.units = ln.acc_units.p;
   a.FB = c->hist_fb;
   a.NG = c->hist_ng;
+  a.pu = c->hist_pu;
   a.pool = ln.pool.p;
   a.ipool = ln.ipool.p;
   a.feat_split = ln.feat_split.p;
@@ -1126,7 +1227,7 @@ static void prof_mark(Ctx* c, int k, int step, int kind, int end) {
 // Every item's histogram units of the last histogram pass summed into acc
 // (one unit per item and feature group).
 static void slab_total(Ctx* c, Lane& ln, bool quant, hipStream_t s) {
-  const int per_unit = c->hist_fb * c->B * (quant ? 1 : 2);
+  const int per_unit = c->hist_pu * (quant ? 1 : 2);
   const int64_t eblocks = ((int64_t)c->hist_ng * per_unit + 31) / 32;
   const int grid = (int)std::min<int64_t>(eblocks * kMaxBatch, 4LL * c->cus);
   hipLaunchKernelGGL(k_slab_reduce, dim3(grid), dim3(256), 0, s, ln.slab.p, ln.units.p, c->hist_ng, per_unit,
@@ -1412,7 +1513,7 @@ static BatchRef batch_ref(Ctx* c, int g) {
 static int batch_grid(int grid, int T, int floor) { return std::max(floor, grid / T); }
 
 static void slab_total_batch(Ctx* c, const BatchRef& b, bool quant) {
-  const int per_unit = c->hist_fb * c->B * (quant ? 1 : 2);
+  const int per_unit = c->hist_pu * (quant ? 1 : 2);
   const int64_t eblocks = ((int64_t)c->hist_ng * per_unit + 31) / 32;
   const int grid = (int)std::min<int64_t>(eblocks * kMaxBatch, 4LL * c->cus);
   hipLaunchKernelGGL(k_slab_reduce_tab, dim3(batch_grid(grid, b.T, 32), b.T), dim3(256), 0, b.s,
@@ -1424,7 +1525,7 @@ static void enqueue_head_batch(Ctx* c, const BatchRef& b) {
   const bool quant = c->opts.use_quantized_grad != 0;
   for (int k = b.k0; k < b.k0 + b.T; ++k) enqueue_head_pre(c, *c->lanes[k], k);
   prof_mark(c, b.k0, 0, 0, 0);
-  launch_hist_tab(c->tab_hist_root.p + b.k0, b.T, quant, c->hist_tpr, batch_grid(c->hist_grid, b.T, 32),
+  launch_hist_tab(c->tab_hist_root.p + b.k0, b.T, quant, c->has_u4, c->hist_tpr, batch_grid(c->hist_grid, b.T, 32),
                   c->hist_lds, b.s);
   prof_mark(c, b.k0, 0, 0, 1);
   slab_total_batch(c, b, quant);
@@ -1443,7 +1544,7 @@ static void enqueue_rounds_batch(Ctx* c, const BatchRef& b, int r0, int n) {
     prof_mark(c, b.k0, step, 2, 0);
     launch_partition_tab(c->tab_part.p + b.k0, b.T, pa, pgrid, b.s);
     prof_mark(c, b.k0, step, 2, 1);
-    launch_hist_tab(c->tab_hist_round.p + b.k0, b.T, quant, c->hist_tpr, hgrid, c->hist_lds, b.s);
+    launch_hist_tab(c->tab_hist_round.p + b.k0, b.T, quant, c->has_u4, c->hist_tpr, hgrid, c->hist_lds, b.s);
     prof_mark(c, b.k0, step, 0, 1);
     slab_total_batch(c, b, quant);
     prof_mark(c, b.k0, step, 3, 1);
@@ -2231,7 +2332,7 @@ static int add_valid(Ctx* c, const uint8_t* bins, int64_t n, const float* label,
   auto v = std::make_unique<ValidData>();
   v->n = n;
   std::vector<uint8_t> rm((size_t)n * c->S, 0);
-  for (int64_t r = 0; r < n; ++r) std::memcpy(&rm[(size_t)r * c->S], bins + (size_t)r * F, F);
+  for (int64_t r = 0; r < n; ++r) pack_row(c, bins + (size_t)r * F, &rm[(size_t)r * c->S]);
   v->bins.alloc(rm.size());
   HIP_CHECK(hipMemcpy(v->bins.p, rm.data(), rm.size(), hipMemcpyHostToDevice));
   v->label.assign(label, label + n);
@@ -2820,9 +2921,27 @@ static void rollback_one_iter(Ctx* c) {
                        lds, c->stream, a);
     HIP_CHECK(hipGetLastError());
   };
+  // packed rows with 4-bit fields: prediction reads one u8 bin per feature,
+  // so each set's rows are unpacked first (one u8 bin per feature)
+  DevBuf<uint8_t> unpacked;
+  auto rows_of = [&](const uint8_t* bins, int64_t n, int& cols) -> const uint8_t* {
+    cols = c->S;
+    if (!c->has_u4) return bins;
+    unpacked.alloc((size_t)n * c->F);
+    hipLaunchKernelGGL(k_unpack_rows, dim3(grid_for(n * c->F, 256, c->cus * 16)), dim3(256), 0, c->stream, bins,
+                       (long long)n, c->S, c->F, c->d_feats.p, unpacked.p);
+    HIP_CHECK(hipGetLastError());
+    cols = c->F;
+    return unpacked.p;
+  };
   for (int k = 0; k < K; ++k) {
-    add_tree(c->bins_rm.p, c->N, c->S, c->score.p + (size_t)k * c->N, first + k);
-    for (auto& v : c->valid) add_tree(v->bins.p, v->n, c->S, v->score.p + (size_t)k * v->n, first + k);
+    int cols = 0;
+    const uint8_t* rb = rows_of(c->bins_rm.p, c->N, cols);
+    add_tree(rb, c->N, cols, c->score.p + (size_t)k * c->N, first + k);
+    for (auto& v : c->valid) {
+      const uint8_t* vb = rows_of(v->bins.p, v->n, cols);
+      add_tree(vb, v->n, cols, v->score.p + (size_t)k * v->n, first + k);
+    }
   }
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->trees.resize(c->trees.size() - K);
@@ -3268,9 +3387,11 @@ static void stage_hist(Ctx* c, const uint32_t* rows, int64_t m, const void* gsrc
   // acc is [group][feature in group][bin][C]: out is [feature][bin][C]
   for (size_t i = 0; i < cnt; ++i) {
     const size_t f = i / ((size_t)c->B * C), r = i % ((size_t)c->B * C);
-    const size_t src = (f / c->hist_fb) * ((size_t)c->hist_fb * c->B * C) + (f % c->hist_fb) * ((size_t)c->B * C) + r;
-    if (quant) ((long long*)out)[i] = acc[src];
-    else ((double*)out)[i] = std::ldexp((double)acc[src], -((i & 1) ? a.d_shift_h : a.d_shift_g));
+    const FeatDev& fd = c->feats[f];
+    const bool in = r / C < (size_t)fd.num_bin;   // a 4-bit feature's unit holds its 16 bins only
+    const long long v = in ? acc[(size_t)fd.hg * c->hist_pu * C + (size_t)fd.hoff * C + r] : 0;
+    if (quant) ((long long*)out)[i] = v;
+    else ((double*)out)[i] = std::ldexp((double)v, -((i & 1) ? a.d_shift_h : a.d_shift_g));
   }
   *out_B = c->B;
 }
@@ -3450,7 +3571,7 @@ lgbmhip_status lgbmhip_bin_rows_device(lgbmhip_ctx* ctx, const float* d_X, int64
     if (!c->N) throw Error(LGBMHIP_INVALID_DATA, "upload the training data first");
     if (!d_X || !d_out || rows < 0) throw Error(LGBMHIP_INVALID_PARAMETER, "bad device buffers");
     set_device(c);
-    launch_bin_rows(c, d_X, rows, ncol, d_out, c->F);
+    launch_bin_rows(c, d_X, rows, ncol, d_out, c->F, false);
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
 }
@@ -3620,6 +3741,22 @@ lgbmhip_status lgbmhip_autotune(lgbmhip_ctx* ctx, int32_t trees_per_candidate, i
     const auto r = autotune(&ctx->c, trees_per_candidate);
     if (out5)
       for (int i = 0; i < 5; ++i) out5[i] = r[(size_t)i];
+  });
+}
+lgbmhip_status lgbmhip_set_bin_packing(lgbmhip_ctx* ctx, int32_t u4) {
+  return guarded(ctx, [&] {
+    if (u4 != 0 && u4 != 1) throw Error(LGBMHIP_INVALID_PARAMETER, "bin packing must be 0 or 1");
+    ctx->c.pack_u4 = u4;
+  });
+}
+lgbmhip_status lgbmhip_bin_layout(lgbmhip_ctx* ctx, int32_t* out3) {
+  return guarded(ctx, [&] {
+    const Ctx* c = &ctx->c;
+    int n4 = 0;
+    for (const FeatDev& d : c->feats) n4 += d.nib >= 0 ? 1 : 0;
+    out3[0] = c->W;
+    out3[1] = c->S;
+    out3[2] = n4;
   });
 }
 lgbmhip_status lgbmhip_get_tuning(lgbmhip_ctx* ctx, int32_t* out9) {

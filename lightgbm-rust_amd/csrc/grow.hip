@@ -118,12 +118,16 @@ struct HistArgs {
   SplitRec* log;
   const SplitRec* best;
   const FeatDev* feats;
-  int F, B, FB, NG;        // features, max bins, features per group, groups
+  const ByteDev* bytes;    // [NG * FB] each byte of a packed row (its fields' entries, masks, most frequent bins)
+  const int* ginfo;        // [NG][2] each group's histogram entries, whether it holds 4-bit fields
+  int F, B, FB, NG;        // features, max bins, bytes per group, groups
+  int pu;                  // entries of a unit (the largest group's)
+  int u4;                  // 1: some byte holds two 4-bit fields (the kernels' U4 form)
   int L;
   long long N;
   int max_chunks;
   int min_rows_per_chunk;
-  long long* slab;         // per unit: its group's [FB][B][C] sums, C = 2 (plain) or 1 (quantized)
+  long long* slab;         // per unit: its group's [pu][C] sums, C = 2 (plain) or 1 (quantized)
   int* units;              // [kMaxBatch + 2] first unit of each item, the total, the item count
   const IterDev* it;
   SplitParamsDev p;
@@ -145,33 +149,40 @@ template <bool Q>
 struct HistBlock { static constexpr int threads = Q ? 512 : 256; };
 
 // One unit: rows [r0, r1) of the segment's chunk, feature group fg, into LDS,
-// then stored to the unit's slab (an empty chunk stores zeros).
-template <int TPR, bool Q>
+// then stored to the unit's slab (an empty chunk stores zeros). Lane j of a
+// row takes the group's bytes 4j .. 4j + 3; U4: the group may hold pairs of
+// 4-bit fields (a byte then adds its high field too).
+template <int TPR, bool Q, bool U4>
 __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long long begin, long long count, int buf,
                                           int sg, int sh, int chunk, int nchunks, int fg, long long* out_slab) {
   const long long per = (count + nchunks - 1) / nchunks;
   const long long r0 = (long long)chunk * per, r1 = min(count, r0 + per);
-  const int f0 = fg * a.FB;
-  const int FBg = min(a.FB, a.F - f0);
-  const int B = a.B;
   // quantized: one packed (g << 32) + h int64 per bin; FP64: g and h int64
-  const int n_lds = FBg * B * (Q ? 1 : 2);
+  const int n_lds = a.ginfo[2 * fg] * (Q ? 1 : 2);
+  const bool u4g = U4 && a.ginfo[2 * fg + 1] != 0;
   unsigned long long* lds64 = (unsigned long long*)ldsv;
   for (int e = threadIdx.x; e < n_lds; e += blockDim.x) lds64[e] = 0ull;
-  // most-frequent bin of each of this thread's 4 features (never accumulated)
+  // this lane's 4 bytes: entries, masks and most frequent bins of their fields
   const int sub = threadIdx.x / TPR, j = threadIdx.x % TPR;
-  int mfb[4];
+  const bool active = j * 4 < a.FB;
+  int off_lo[4], off_hi[4];
+  unsigned mask_lo[4], mask_hi[4], mfb_lo[4], mfb_hi[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const int fl = j * 4 + k;
-    mfb[k] = fl < FBg ? a.feats[f0 + fl].mfb : 256;
+    ByteDev bd{0, 0, 0u, 0u, 0u, 0u};
+    if (active) bd = a.bytes[fg * a.FB + j * 4 + k];
+    off_lo[k] = bd.off_lo;
+    off_hi[k] = bd.off_hi;
+    mask_lo[k] = bd.mask_lo;
+    mask_hi[k] = bd.mask_hi;
+    mfb_lo[k] = bd.mfb_lo;
+    mfb_hi[k] = bd.mfb_hi;
   }
   __syncthreads();
   const int rpp = HistBlock<Q>::threads / TPR;
   // select, not index: a runtime index into a kernel-argument array makes
   // the compiler copy the arguments to scratch memory
-  // lanes past the group's last feature read the row's first word (unused)
-  const bool active = j * 4 < FBg;
+  // lanes past the group's last byte read the row's first word (unused)
   const uint8_t* __restrict__ gb =
       sel3(a.grp[0], a.grp[1], a.grp[2], buf) + ((size_t)fg * a.N + begin) * a.FB + (active ? j * 4 : 0);
   using GT = typename std::conditional<Q, char2, float2>::type;
@@ -204,21 +215,25 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
         hv[u] = to_fixed(v.y, sh);
       }
     }
+    auto add = [&](int e, long long g, long long h) {
+      if (Q) {
+        atomicAdd(lds64 + e, (unsigned long long)g);   // wrapping, as the reference
+      } else {
+        atomicAdd(lds64 + 2 * e, (unsigned long long)g);
+        atomicAdd(lds64 + 2 * e + 1, (unsigned long long)h);
+      }
+    };
 #pragma unroll
     for (int u = 0; u < kHistUnroll; ++u) {
       if (!ok[u]) continue;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const int b = (w[u] >> (8 * k)) & 0xFF;
-        const int fl = j * 4 + k;
-        if (fl < FBg && b != mfb[k]) {
-          if (Q) {
-            atomicAdd(lds64 + (size_t)fl * B + b, (unsigned long long)gv[u]);   // wrapping, as the reference
-          } else {
-            const size_t c = ((size_t)fl * B + b) * 2;
-            atomicAdd(lds64 + c, (unsigned long long)gv[u]);
-            atomicAdd(lds64 + c + 1, (unsigned long long)hv[u]);
-          }
+        const unsigned by = (w[u] >> (8 * k)) & 0xFFu;
+        const unsigned b = by & mask_lo[k];
+        if (mask_lo[k] && b != mfb_lo[k]) add(off_lo[k] + (int)b, gv[u], hv[u]);
+        if (u4g) {
+          const unsigned bh = (by >> 4) & mask_hi[k];
+          if (mask_hi[k] && bh != mfb_hi[k]) add(off_hi[k] + (int)bh, gv[u], hv[u]);
         }
       }
     }
@@ -240,7 +255,7 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
 // slab, the same entries as the 64-bit form.
 //
 // LDS of a block: [n32 32-bit bins][64 trash words][n32 int64 sums], n32 =
-// min(FB, F) * B, the int64 part 8-byte aligned.
+// the entries of a unit, the int64 part 8-byte aligned.
 __host__ __device__ inline size_t q32_acc_offset(int n32) { return ((size_t)(n32 + 64) * 4 + 7) / 8 * 8; }
 __host__ __device__ inline size_t q32_lds_bytes(int n32) { return q32_acc_offset(n32) + (size_t)n32 * 8; }
 
@@ -263,10 +278,11 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) 
 }
 
 // The row loop has no branches per bin: a bin equal to the feature's most
-// frequent bin (never accumulated), and every byte of a feature the lane
-// does not hold, adds to a per-lane trash entry past the histogram; rows
-// past the chunk's end (the last pass) add 0.
-template <int TPR>
+// frequent bin (never accumulated), and every field the lane does not hold,
+// adds to a per-lane trash entry past the histogram; rows past the chunk's
+// end (the last pass) add 0. U4: the group may hold pairs of 4-bit fields
+// (a byte then adds its high field too).
+template <int TPR, bool U4>
 __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32, long long begin, long long count,
                                               int buf, int chunk, int nchunks, int fg, long long* out_slab,
                                               int epoch_rows) {
@@ -276,11 +292,9 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
   const long long per = (count + nchunks - 1) / nchunks;
   const long long r0 = (long long)chunk * per, r1 = min(count, r0 + per);
   const int rows = (int)max(0LL, r1 - r0);
-  const int f0 = fg * a.FB;
-  const int FBg = min(a.FB, a.F - f0);
-  const int B = a.B;
-  const int n_lds = FBg * B;
-  const int n32 = min(a.FB, a.F) * B;
+  const int n_lds = a.ginfo[2 * fg];
+  const bool u4g = U4 && a.ginfo[2 * fg + 1] != 0;
+  const int n32 = a.pu;
   const int sub = threadIdx.x / TPR, j = threadIdx.x % TPR;
   const unsigned trash = (unsigned)(n32 + (threadIdx.x & 63)) * 4u;   // byte offset
   // this thread's entries e = threadIdx.x + i * T: only it reads and writes
@@ -291,14 +305,22 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
   // length so that every chunk of the item has the same form
   const bool one_epoch = per <= epoch_rows;
   const int epoch_passes = epoch_rows / PR;
+  // this lane's 4 bytes: the entries and most frequent bins of their fields;
+  // a field's mask keeps its bits of the row's word (a masked-off field
+  // reads 0 against most frequent bin 0: the trash)
+  const bool active = j * 4 < a.FB;
   unsigned fbase[4], mfb[4], bytemask = 0u;
+  unsigned fbase_h[4], mfb_h[4], himask = 0u;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const int fl = j * 4 + k;
-    const bool on = fl < FBg;
-    fbase[k] = on ? (unsigned)(fl * B) * 4u : 0u;
-    mfb[k] = on ? (unsigned)a.feats[f0 + fl].mfb : 0u;   // a masked byte reads 0: the trash
-    bytemask |= on ? (0xFFu << (8 * k)) : 0u;
+    ByteDev bd{0, 0, 0u, 0u, 0u, 0u};
+    if (active) bd = a.bytes[fg * a.FB + j * 4 + k];
+    fbase[k] = (unsigned)bd.off_lo * 4u;
+    mfb[k] = bd.mfb_lo;
+    bytemask |= bd.mask_lo << (8 * k);
+    fbase_h[k] = (unsigned)bd.off_hi * 4u;
+    mfb_h[k] = bd.mfb_hi;
+    himask |= (bd.mask_hi << 4) << (8 * k);
   }
   if (!one_epoch)
     for (int e = threadIdx.x; e < n_lds; e += T) acc[e] = 0ull;
@@ -313,7 +335,7 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
     }
   };
   const int FB = a.FB;
-  const unsigned jofs = bytemask ? (unsigned)(j * 4) : 0u;
+  const unsigned jofs = active ? (unsigned)(j * 4) : 0u;
   typedef const __attribute__((address_space(1))) uint8_t* GBytes;
   typedef const __attribute__((address_space(1))) unsigned short* GHalf;
   typedef __attribute__((address_space(3))) unsigned* LWord;
@@ -324,7 +346,10 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
   // LDS byte addresses, the histogram's base folded in
   const unsigned lbase = (unsigned)(size_t)(LWord)lds32;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) fbase[k] += lbase;
+  for (int k = 0; k < 4; ++k) {
+    fbase[k] += lbase;
+    fbase_h[k] += lbase;
+  }
   const unsigned trash_a = trash + lbase;
   // one pass's loads (a row past the chunk reloads its last row)
   auto load_pass = [&](int q0, unsigned* w, unsigned* hr) {
@@ -355,6 +380,15 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
         const unsigned b = (ww >> (8 * k)) & 0xFFu;
         const unsigned addr = b == mfb[k] ? trash_a : fbase[k] + b * 4u;
         __hip_atomic_fetch_add((LWord)(size_t)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);   // wrapping
+      }
+      if (u4g) {
+        const unsigned wh = w[u] & himask;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const unsigned b = (wh >> (8 * k + 4)) & 0xFu;
+          const unsigned addr = b == mfb_h[k] ? trash_a : fbase_h[k] + b * 4u;
+          __hip_atomic_fetch_add((LWord)(size_t)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
       }
     }
     // the loop bounds are the block's own, so every thread takes this branch alike
@@ -451,12 +485,12 @@ __device__ void apply_plan(const HistArgs& a) {
   }
 }
 
-template <int TPR, bool Q>
+template <int TPR, bool Q, bool U4>
 __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
   __shared__ long long s_begin[kMaxBatch], s_count[kMaxBatch];
   __shared__ int s_buf[kMaxBatch], s_sg[kMaxBatch], s_sh[kMaxBatch], s_nch[kMaxBatch], s_u0[kMaxBatch + 1];
   const int tid = threadIdx.x;
-  const size_t ustride = (size_t)a.FB * a.B * (Q ? 1 : 2);   // one unit's slab
+  const size_t ustride = (size_t)a.pu * (Q ? 1 : 2);   // one unit's slab
   int nitems;
   if (a.mode != 3) {
     if (tid == 0) {
@@ -545,12 +579,12 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
     }
     if (q32) {
       unsigned* lds32 = (unsigned*)lds;
-      for (int e = tid; e < min(a.FB, a.F) * a.B; e += T) lds32[e] = 0u;
+      for (int e = tid; e < a.pu; e += T) lds32[e] = 0u;
       __syncthreads();
       for (int u = vb; u < units; u += gridDim.x) {
         while (u >= s_u0[w + 1]) ++w;
         const int r = u - s_u0[w];
-        hist_unit_q32<TPR>(a, lds32, s_begin[w], s_count[w], s_buf[w], r / a.NG, s_nch[w], r % a.NG,
+        hist_unit_q32<TPR, U4>(a, lds32, s_begin[w], s_count[w], s_buf[w], r / a.NG, s_nch[w], r % a.NG,
                            a.slab + (size_t)u * ustride, erows);
       }
       return;
@@ -560,39 +594,39 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
   for (int u = vb; u < units; u += gridDim.x) {
     while (u >= s_u0[w + 1]) ++w;
     const int r = u - s_u0[w];
-    hist_unit<TPR, Q>(a, lds, s_begin[w], s_count[w], s_buf[w], s_sg[w], s_sh[w], r / a.NG, s_nch[w], r % a.NG,
+    hist_unit<TPR, Q, U4>(a, lds, s_begin[w], s_count[w], s_buf[w], s_sg[w], s_sh[w], r / a.NG, s_nch[w], r % a.NG,
                       a.slab + (size_t)u * ustride);
   }
 }
 
-template <int TPR>
+template <int TPR, bool U4>
 __global__ void __launch_bounds__(HistBlock<false>::threads) k_hist_f64(HistArgs a) {
   extern __shared__ unsigned long long lds_hist[];
-  hist_kernel<TPR, false>(a, lds_hist);
+  hist_kernel<TPR, false, U4>(a, lds_hist);
 }
 
 // three 512-thread blocks per CU (6 waves per SIMD: at most 80 VGPRs), the
 // LDS of three 48 KiB histograms
-template <int TPR>
+template <int TPR, bool U4>
 __global__ void __launch_bounds__(HistBlock<true>::threads) __attribute__((amdgpu_waves_per_eu(6)))
 k_hist_int(HistArgs a) {
   extern __shared__ unsigned long long lds_hist[];
-  hist_kernel<TPR, true>(a, lds_hist);
+  hist_kernel<TPR, true, U4>(a, lds_hist);
 }
 
 // The trees of a batch: grid y picks the tree, grid x is its blocks.
-template <int TPR>
+template <int TPR, bool U4>
 __global__ void __launch_bounds__(HistBlock<false>::threads) k_hist_f64_tab(const HistArgs* __restrict__ tab) {
   extern __shared__ unsigned long long lds_hist[];
   const HistArgs a = tab[blockIdx.y];
-  hist_kernel<TPR, false>(a, lds_hist);
+  hist_kernel<TPR, false, U4>(a, lds_hist);
 }
-template <int TPR>
+template <int TPR, bool U4>
 __global__ void __launch_bounds__(HistBlock<true>::threads) __attribute__((amdgpu_waves_per_eu(6)))
 k_hist_int_tab(const HistArgs* __restrict__ tab) {
   extern __shared__ unsigned long long lds_hist[];
   const HistArgs a = tab[blockIdx.y];
-  hist_kernel<TPR, true>(a, lds_hist);
+  hist_kernel<TPR, true, U4>(a, lds_hist);
 }
 
 // ---------------------------------------------------------------- split search
@@ -605,7 +639,8 @@ struct SplitArgs {
   int F, B, L;
   const long long* slab;  // histogram units of the last histogram pass
   const int* units;       // [kMaxBatch + 1] first unit of each item
-  int FB, NG;             // features per histogram group, groups
+  int FB, NG;             // bytes per histogram group, groups
+  int pu;                 // entries of a histogram unit (a feature's bins start at its hoff in unit hg)
   double* pool;          // [slots][F][B][2] FP64 histograms
   long long* ipool;      // [slots][F][B] packed histograms (quantized)
   SplitRec* feat_split;  // [2 kMaxBatch][F] searches of this pass, row = slot - first child slot
@@ -665,12 +700,13 @@ __device__ __forceinline__ LeafSums child_sums(const SplitArgs& a, const SplitRe
 // f's group): out0 (and out1 for C = 2, the hessian sums) per bin b = tid +
 // 128 k. Wrapping int64 sums: exact in any order.
 template <int C>
-__device__ __forceinline__ void slab_sum(const SplitArgs& a, int w, int f, int tid, long long* out0, long long* out1) {
+__device__ __forceinline__ void slab_sum(const SplitArgs& a, int w, const FeatDev& fd, int tid, long long* out0,
+                                         long long* out1) {
   constexpr int PB = kMaxBins / 128;
-  const int g = f / a.FB, fl = f % a.FB;
   const int u0 = a.units[w], nch = (a.units[w + 1] - u0) / a.NG;
-  const size_t ustride = (size_t)a.FB * a.B * C;
-  const long long* base = a.slab + (size_t)(u0 + g) * ustride + (size_t)fl * a.B * C;
+  const size_t ustride = (size_t)a.pu * C;
+  const long long* base = a.slab + (size_t)(u0 + fd.hg) * ustride + (size_t)fd.hoff * C;
+  const int nb = fd.num_bin;   // a 4-bit feature's unit holds its 16 bins only
   const size_t cstride = (size_t)a.NG * ustride;   // the next chunk of the same group
   unsigned long long s0[PB], s1[PB];
 #pragma unroll
@@ -684,7 +720,7 @@ __device__ __forceinline__ void slab_sum(const SplitArgs& a, int w, int f, int t
       for (int k = 0; k < PB; ++k) {
         const int b = tid + 128 * k;
         x0[j][k] = x1[j][k] = 0;
-        if (b < a.B) {
+        if (b < nb) {
           const long long* q = base + (size_t)(c + j) * cstride + (size_t)b * C;
           x0[j][k] = q[0];
           if (C == 2) x1[j][k] = q[1];
@@ -702,7 +738,7 @@ __device__ __forceinline__ void slab_sum(const SplitArgs& a, int w, int f, int t
 #pragma unroll
     for (int k = 0; k < PB; ++k) {
       const int b = tid + 128 * k;
-      if (b < a.B) {
+      if (b < nb) {
         const long long* q = base + (size_t)c * cstride + (size_t)b * C;
         s0[k] += (unsigned long long)q[0];
         if (C == 2) s1[k] += (unsigned long long)q[1];
@@ -793,7 +829,7 @@ __device__ __forceinline__ void grow_split(const SplitArgs& a) {
         ph[k] = pp[2 * b + 1];
       }
     }
-    slab_sum<2>(a, w, f, tid, gi, hi);
+    slab_sum<2>(a, w, fd, tid, gi, hi);
 #pragma unroll
     for (int k = 0; k < PB; ++k) {
       const int b = tid + 128 * k;
@@ -848,7 +884,7 @@ __device__ __forceinline__ void grow_split(const SplitArgs& a) {
       pv[k] = 0;
       if (b < B && pp) pv[k] = pp[b];
     }
-    slab_sum<1>(a, w, f, tid, v, unused);
+    slab_sum<1>(a, w, fd, tid, v, unused);
 #pragma unroll
     for (int k = 0; k < PB; ++k) {
       const int b = tid + 128 * k;
@@ -1532,7 +1568,10 @@ __device__ __forceinline__ void grow_part(const PartArgs& a) {
     GT* ogh = (GT*)sel3(a.gh[0], a.gh[1], a.gh[2], dst) + begin;
     const uint8_t* gsrc = sel3(a.grp[0], a.grp[1], a.grp[2], src) + (size_t)begin * FB;
     uint8_t* gdst = sel3(a.grp[0], a.grp[1], a.grp[2], dst) + (size_t)begin * FB;
-    const uint8_t* scol = gsrc + (size_t)(s.feature / FB) * gstride + (s.feature % FB);
+    // the split feature's byte (its bin, or a pair of 4-bit fields)
+    const uint8_t* scol = gsrc + (size_t)(fd.pos / FB) * gstride + (fd.pos % FB);
+    const int bshift = fd.nib > 0 ? 4 : 0;
+    const unsigned bmask = fd.nib >= 0 ? 0xFu : 0xFFu;
     const long long t0 = (long long)(t - items[w].tile0) * kPartTile;
     // one round of independent loads: gradients, the split feature's bin and
     // the first words of every row (lanes take consecutive rows); a slot past
@@ -1560,7 +1599,7 @@ __device__ __forceinline__ void grow_part(const PartArgs& a) {
       }
     }
 #pragma unroll
-    for (int u = 0; u < R; ++u) left[u] = valid[u] && goes_left(bin[u], fd, s);
+    for (int u = 0; u < R; ++u) left[u] = valid[u] && goes_left((int)((bin[u] >> bshift) & bmask), fd, s);
     // ranks inside each 64-row slice, slice counts to LDS
     int rk[R];
 #pragma unroll
@@ -1647,10 +1686,10 @@ __global__ void __launch_bounds__(kPartThreads) k_grow_part_tab(const PartArgs* 
   grow_part<VW, GB>(a);
 }
 
-// Rows of the row-major bins (stride S) into the group-major layout of the
-// ordered buffers: out[g][i][0..FB) = bins of features g*FB.. of row
-// rows[i] (or row i when rows is null), zero past the last feature.
-__global__ void k_group_rows(const unsigned* rows, long long m, const uint8_t* __restrict__ rm, int S, int F, int FB,
+// Rows of the row-major packed bins (stride S) into the group-major layout
+// of the ordered buffers: out[g][i][0..FB) = bytes g*FB.. of row rows[i] (or
+// row i when rows is null), zero past the row's W bytes.
+__global__ void k_group_rows(const unsigned* rows, long long m, const uint8_t* __restrict__ rm, int S, int W, int FB,
                              int NG, long long N, uint8_t* __restrict__ out) {
   const int words = FB / 4;
   const long long total = m * NG * words;
@@ -1662,8 +1701,8 @@ __global__ void k_group_rows(const unsigned* rows, long long m, const uint8_t* _
     unsigned v = 0;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int f = g * FB + w * 4 + b;
-      if (f < F) v |= (unsigned)rm[r * S + f] << (8 * b);
+      const int p = g * FB + w * 4 + b;
+      if (p < W) v |= (unsigned)rm[r * S + p] << (8 * b);
     }
     *(unsigned*)(out + (size_t)g * N * FB + (size_t)i * FB + 4 * w) = v;
   }
@@ -1763,13 +1802,15 @@ __global__ void __launch_bounds__(256) k_tree_nodes(const StepDev* st, const Spl
   }
 }
 
-// A node packed for walking: x = feature | threshold << 16 | missing bin
-// << 24, y = flags (1 default left, 2 missing applies, 4 categorical),
+// A node packed for walking: x = the feature's byte in a packed row |
+// threshold << 16 | missing bin << 24, y = flags (1 default left, 2 missing
+// applies, 4 categorical, 8 a 4-bit field, 16 the byte's high field),
 // z = left child, w = right child (>= 0 node, < 0 ~leaf).
 __device__ __forceinline__ uint4 pack_node(const NodeDev& n, const FeatDev& fd) {
   uint4 q;
   const bool cat = fd.bin_type == 1;
   int mbin = 0, flags = n.default_left ? 1 : 0;
+  if (fd.nib >= 0) flags |= 8 | (fd.nib ? 16 : 0);
   if (cat) {
     flags |= 4;
     mbin = fd.mfb;
@@ -1780,7 +1821,7 @@ __device__ __forceinline__ uint4 pack_node(const NodeDev& n, const FeatDev& fd) 
     flags |= 2;
     mbin = fd.num_bin - 1;
   }
-  q.x = (unsigned)n.feature | ((unsigned)n.threshold & 0xFF) << 16 | ((unsigned)mbin & 0xFF) << 24;
+  q.x = (unsigned)fd.pos | ((unsigned)n.threshold & 0xFF) << 16 | ((unsigned)mbin & 0xFF) << 24;
   q.y = (unsigned)flags;
   q.z = (unsigned)n.left;
   q.w = (unsigned)n.right;
@@ -1906,7 +1947,7 @@ __global__ void __launch_bounds__(256) k_walk(WalkArgs a) {
       for (int u = 0; u < U; ++u) {
         if (node[u] >= 0) {
           const uint4 q = nq[node[u]];
-          const int f = (int)(q.x & 0xFFFF);
+          const int f = (int)(q.x & 0xFFFF);   // the feature's byte
           int b;
           if (RW > 0) {
             const int wi = f >> 2;
@@ -1917,6 +1958,7 @@ __global__ void __launch_bounds__(256) k_walk(WalkArgs a) {
           } else {
             b = a.bins[(size_t)rc[u] * a.S + f];
           }
+          if (q.y & 8u) b = (b >> ((q.y & 16u) ? 4 : 0)) & 0xF;
           node[u] = walk_step(q, b, ncat + (size_t)node[u] * kCatWords);
           any = true;
         }
@@ -2121,10 +2163,15 @@ static void set_walk_lds_limit(int L) {
   set_walk_lds_limit_m<2>(L);
 }
 
+template <int TPR, bool U4>
+static void launch_hist_tu(const HistArgs& a, bool quant, int grid, size_t lds, hipStream_t s) {
+  if (quant) hipLaunchKernelGGL((k_hist_int<TPR, U4>), dim3(grid), dim3(HistBlock<true>::threads), lds, s, a);
+  else hipLaunchKernelGGL((k_hist_f64<TPR, U4>), dim3(grid), dim3(HistBlock<false>::threads), lds, s, a);
+}
 template <int TPR>
 static void launch_hist_t(const HistArgs& a, bool quant, int grid, size_t lds, hipStream_t s) {
-  if (quant) hipLaunchKernelGGL(k_hist_int<TPR>, dim3(grid), dim3(HistBlock<true>::threads), lds, s, a);
-  else hipLaunchKernelGGL(k_hist_f64<TPR>, dim3(grid), dim3(HistBlock<false>::threads), lds, s, a);
+  if (a.u4) launch_hist_tu<TPR, true>(a, quant, grid, lds, s);
+  else launch_hist_tu<TPR, false>(a, quant, grid, lds, s);
 }
 
 static void launch_hist(const HistArgs& a, bool quant, int tpr, int grid, size_t lds, hipStream_t s) {
@@ -2141,52 +2188,67 @@ static void launch_hist(const HistArgs& a, bool quant, int tpr, int grid, size_t
 // Resident blocks per CU of the histogram kernel for this form, row layout
 // and LDS size (0 when the runtime cannot tell).
 template <int TPR>
-static int hist_occupancy_t(bool quant, size_t lds) {
+static int hist_occupancy_t(bool quant, bool u4, size_t lds) {
   int n = 0;
   const hipError_t e =
-      quant ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_hist_int<TPR>, HistBlock<true>::threads, lds)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_hist_f64<TPR>, HistBlock<false>::threads, lds);
+      quant ? (u4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_hist_int<TPR, true>, HistBlock<true>::threads, lds)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_hist_int<TPR, false>, HistBlock<true>::threads, lds))
+            : (u4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_hist_f64<TPR, true>, HistBlock<false>::threads, lds)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_hist_f64<TPR, false>, HistBlock<false>::threads, lds));
   return e == hipSuccess ? n : 0;
 }
-static int hist_occupancy(bool quant, int tpr, size_t lds) {
+static int hist_occupancy(bool quant, bool u4, int tpr, size_t lds) {
   switch (tpr) {
-    case 1: return hist_occupancy_t<1>(quant, lds);
-    case 2: return hist_occupancy_t<2>(quant, lds);
-    case 4: return hist_occupancy_t<4>(quant, lds);
-    case 8: return hist_occupancy_t<8>(quant, lds);
-    case 16: return hist_occupancy_t<16>(quant, lds);
-    default: return hist_occupancy_t<32>(quant, lds);
+    case 1: return hist_occupancy_t<1>(quant, u4, lds);
+    case 2: return hist_occupancy_t<2>(quant, u4, lds);
+    case 4: return hist_occupancy_t<4>(quant, u4, lds);
+    case 8: return hist_occupancy_t<8>(quant, u4, lds);
+    case 16: return hist_occupancy_t<16>(quant, u4, lds);
+    default: return hist_occupancy_t<32>(quant, u4, lds);
   }
 }
 
+template <int TPR, bool U4>
+static void launch_hist_tab_tu(const HistArgs* tab, int T, bool quant, int grid, size_t lds, hipStream_t s) {
+  if (quant) hipLaunchKernelGGL((k_hist_int_tab<TPR, U4>), dim3(grid, T), dim3(HistBlock<true>::threads), lds, s, tab);
+  else hipLaunchKernelGGL((k_hist_f64_tab<TPR, U4>), dim3(grid, T), dim3(HistBlock<false>::threads), lds, s, tab);
+}
 template <int TPR>
-static void launch_hist_tab_t(const HistArgs* tab, int T, bool quant, int grid, size_t lds, hipStream_t s) {
-  if (quant) hipLaunchKernelGGL(k_hist_int_tab<TPR>, dim3(grid, T), dim3(HistBlock<true>::threads), lds, s, tab);
-  else hipLaunchKernelGGL(k_hist_f64_tab<TPR>, dim3(grid, T), dim3(HistBlock<false>::threads), lds, s, tab);
+static void launch_hist_tab_t(const HistArgs* tab, int T, bool quant, bool u4, int grid, size_t lds, hipStream_t s) {
+  if (u4) launch_hist_tab_tu<TPR, true>(tab, T, quant, grid, lds, s);
+  else launch_hist_tab_tu<TPR, false>(tab, T, quant, grid, lds, s);
 }
 
 // The histogram pass of T trees (tab: their arguments in device memory).
-static void launch_hist_tab(const HistArgs* tab, int T, bool quant, int tpr, int grid, size_t lds, hipStream_t s) {
+static void launch_hist_tab(const HistArgs* tab, int T, bool quant, bool u4, int tpr, int grid, size_t lds,
+                            hipStream_t s) {
   switch (tpr) {
-    case 1: launch_hist_tab_t<1>(tab, T, quant, grid, lds, s); break;
-    case 2: launch_hist_tab_t<2>(tab, T, quant, grid, lds, s); break;
-    case 4: launch_hist_tab_t<4>(tab, T, quant, grid, lds, s); break;
-    case 8: launch_hist_tab_t<8>(tab, T, quant, grid, lds, s); break;
-    case 16: launch_hist_tab_t<16>(tab, T, quant, grid, lds, s); break;
-    default: launch_hist_tab_t<32>(tab, T, quant, grid, lds, s); break;
+    case 1: launch_hist_tab_t<1>(tab, T, quant, u4, grid, lds, s); break;
+    case 2: launch_hist_tab_t<2>(tab, T, quant, u4, grid, lds, s); break;
+    case 4: launch_hist_tab_t<4>(tab, T, quant, u4, grid, lds, s); break;
+    case 8: launch_hist_tab_t<8>(tab, T, quant, u4, grid, lds, s); break;
+    case 16: launch_hist_tab_t<16>(tab, T, quant, u4, grid, lds, s); break;
+    default: launch_hist_tab_t<32>(tab, T, quant, u4, grid, lds, s); break;
   }
 }
 
-static void set_hist_lds_limits(size_t bytes) {
-  const void* ks[] = {(const void*)k_hist_f64<1>, (const void*)k_hist_f64<2>, (const void*)k_hist_f64<4>,
-                      (const void*)k_hist_f64<8>, (const void*)k_hist_f64<16>, (const void*)k_hist_f64<32>,
-                      (const void*)k_hist_int<1>, (const void*)k_hist_int<2>, (const void*)k_hist_int<4>,
-                      (const void*)k_hist_int<8>, (const void*)k_hist_int<16>, (const void*)k_hist_int<32>,
-                      (const void*)k_hist_f64_tab<1>, (const void*)k_hist_f64_tab<2>, (const void*)k_hist_f64_tab<4>,
-                      (const void*)k_hist_f64_tab<8>, (const void*)k_hist_f64_tab<16>, (const void*)k_hist_f64_tab<32>,
-                      (const void*)k_hist_int_tab<1>, (const void*)k_hist_int_tab<2>, (const void*)k_hist_int_tab<4>,
-                      (const void*)k_hist_int_tab<8>, (const void*)k_hist_int_tab<16>, (const void*)k_hist_int_tab<32>};
+template <bool U4>
+static void set_hist_lds_limits_u(size_t bytes) {
+  const void* ks[] = {(const void*)k_hist_f64<1, U4>, (const void*)k_hist_f64<2, U4>, (const void*)k_hist_f64<4, U4>,
+                      (const void*)k_hist_f64<8, U4>, (const void*)k_hist_f64<16, U4>, (const void*)k_hist_f64<32, U4>,
+                      (const void*)k_hist_int<1, U4>, (const void*)k_hist_int<2, U4>, (const void*)k_hist_int<4, U4>,
+                      (const void*)k_hist_int<8, U4>, (const void*)k_hist_int<16, U4>, (const void*)k_hist_int<32, U4>,
+                      (const void*)k_hist_f64_tab<1, U4>, (const void*)k_hist_f64_tab<2, U4>,
+                      (const void*)k_hist_f64_tab<4, U4>, (const void*)k_hist_f64_tab<8, U4>,
+                      (const void*)k_hist_f64_tab<16, U4>, (const void*)k_hist_f64_tab<32, U4>,
+                      (const void*)k_hist_int_tab<1, U4>, (const void*)k_hist_int_tab<2, U4>,
+                      (const void*)k_hist_int_tab<4, U4>, (const void*)k_hist_int_tab<8, U4>,
+                      (const void*)k_hist_int_tab<16, U4>, (const void*)k_hist_int_tab<32, U4>};
   for (const void* k : ks) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+static void set_hist_lds_limits(size_t bytes) {
+  set_hist_lds_limits_u<false>(bytes);
+  set_hist_lds_limits_u<true>(bytes);
 }
 
 static size_t part_lds(int L) { return (size_t)L * (sizeof(PosRec) + sizeof(KeyRec)); }

@@ -2125,14 +2125,17 @@ __device__ __forceinline__ unsigned value_to_bin_dev(const BinFeat& bf, const do
   return (unsigned)l;
 }
 
-// Bins rows x ncol f32 values into u8 rows of `stride` bytes (features F..stride-1
-// zero): one thread per 4 bytes of an output row, values loaded before use.
-// STAGE: the bin bounds of every feature (nbounds doubles) are copied to LDS
-// first, so the binary searches read LDS.
-template <bool STAGE>
+// Bins rows x ncol f32 values into rows of `stride` bytes: one thread per 4
+// bytes of an output row, values loaded before use. Without `map`, byte j is
+// feature j (j < F; the rest zero); with it, byte p < W holds map[p].lo in its
+// low bits and, for a pair of 4-bit fields, map[p].hi in its high 4 bits
+// (bytes W.. zero). STAGE: the bin bounds of every feature (nbounds doubles)
+// are copied to LDS first, so the binary searches read LDS.
+template <bool STAGE, bool PACK>
 __global__ void __launch_bounds__(1024) k_bin_rows(const float* X, long long n, int ncol, const BinFeat* feats, int F,
                                                    const double* gbounds, int nbounds, const int* ckeys,
-                                                   const int* cbins, uint8_t* out, int stride) {
+                                                   const int* cbins, uint8_t* out, int stride, const BinByte* map,
+                                                   int W) {
   extern __shared__ double bin_lds[];
   const double* bounds = gbounds;
   if (STAGE) {
@@ -2140,24 +2143,46 @@ __global__ void __launch_bounds__(1024) k_bin_rows(const float* X, long long n, 
     __syncthreads();
     bounds = bin_lds;
   }
-  const int W = (stride + 3) / 4;
-  const long long total = n * W;
+  const int WW = (stride + 3) / 4;
+  const long long total = n * WW;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
-    const long long r = t / W;
-    const int w = (int)(t - r * W);
-    float x[4];
-    BinFeat bf[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int f = min(4 * w + j, F - 1);
-      bf[j] = feats[f];
-      x[j] = X[r * ncol + bf[j].col];
-    }
+    const long long r = t / WW;
+    const int w = (int)(t - r * WW);
     unsigned word = 0;
+    if (PACK) {
+      BinByte bb[4];
+      float xl[4], xh[4];
+      BinFeat fl[4], fh[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (4 * w + j < F) word |= (value_to_bin_dev(bf[j], bounds, ckeys, cbins, (double)x[j]) & 0xFFu) << (8 * j);
+      for (int j = 0; j < 4; ++j) {
+        const int p = 4 * w + j;
+        bb[j] = p < W ? map[p] : BinByte{-1, -1};
+        fl[j] = feats[max(bb[j].lo, 0)];
+        fh[j] = feats[max(bb[j].hi, 0)];
+        xl[j] = X[r * ncol + fl[j].col];
+        xh[j] = X[r * ncol + fh[j].col];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        unsigned v = 0u;
+        if (bb[j].lo >= 0) v = value_to_bin_dev(fl[j], bounds, ckeys, cbins, (double)xl[j]);
+        if (bb[j].hi >= 0) v |= value_to_bin_dev(fh[j], bounds, ckeys, cbins, (double)xh[j]) << 4;
+        word |= (v & 0xFFu) << (8 * j);
+      }
+    } else {
+      float x[4];
+      BinFeat bf[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int f = min(4 * w + j, F - 1);
+        bf[j] = feats[f];
+        x[j] = X[r * ncol + bf[j].col];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (4 * w + j < F) word |= (value_to_bin_dev(bf[j], bounds, ckeys, cbins, (double)x[j]) & 0xFFu) << (8 * j);
+    }
     uint8_t* o = out + r * stride + 4 * w;
     if ((stride & 3) == 0) {
       *(unsigned*)o = word;
@@ -2169,14 +2194,29 @@ __global__ void __launch_bounds__(1024) k_bin_rows(const float* X, long long n, 
   }
 }
 
-// Column-major copy [F][n] of row-major bins [n][S].
-__global__ void k_rm_to_cm(const uint8_t* rm, long long n, int S, int F, uint8_t* cm) {
+// Column-major u8 copy [F][n] of packed row-major bins [n][S].
+__global__ void k_rm_to_cm(const uint8_t* rm, long long n, int S, int F, const FeatDev* feats, uint8_t* cm) {
   const long long total = n * F;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     const int f = (int)(t / n);
     const long long r = t - (long long)f * n;
-    cm[t] = rm[r * S + f];
+    const FeatDev& d = feats[f];
+    const unsigned v = rm[r * S + d.pos];
+    cm[t] = (uint8_t)(d.nib < 0 ? v : (v >> (4 * d.nib)) & 0xFu);
+  }
+}
+
+// Packed rows [n][S] into one u8 bin per feature [n][F].
+__global__ void k_unpack_rows(const uint8_t* rm, long long n, int S, int F, const FeatDev* feats, uint8_t* out) {
+  const long long total = n * F;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long r = t / F;
+    const int f = (int)(t - r * F);
+    const FeatDev& d = feats[f];
+    const unsigned v = rm[r * S + d.pos];
+    out[t] = (uint8_t)(d.nib < 0 ? v : (v >> (4 * d.nib)) & 0xFu);
   }
 }
 
