@@ -249,6 +249,7 @@ def _train_forms(lgb, ds, base, args, quality):
         params = dict(base)
         if args.batch_trees is not None:
             params["batch_trees"] = args.batch_trees
+        params["pack_u4"] = not args.no_pack_u4
         if form == "quant":
             params.update(use_quantized_grad=True, num_grad_quant_bins=16, stochastic_rounding=True,
                           quant_train_renew_leaf=True)
@@ -532,6 +533,8 @@ def main():
     ap.add_argument("--forms", default="quant,plain")
     ap.add_argument("--no-graph", action="store_true", help="stream launches instead of graphs (counter passes)")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
+    ap.add_argument("--no-pack-u4", action="store_true",
+                    help="features of at most 16 bins stored as bytes instead of 4-bit fields")
     ap.add_argument("--batch-trees", type=int, default=None,
                     help="multiclass: class trees per shared growth launch (0: one tree per launch)")
     ap.add_argument("--prof-all", action="store_true",

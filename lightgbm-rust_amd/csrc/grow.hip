@@ -150,33 +150,34 @@ struct HistBlock { static constexpr int threads = Q ? 512 : 256; };
 
 // One unit: rows [r0, r1) of the segment's chunk, feature group fg, into LDS,
 // then stored to the unit's slab (an empty chunk stores zeros). Lane j of a
-// row takes the group's bytes 4j .. 4j + 3; U4: the group may hold pairs of
+// row takes the group's bytes 4j .. 4j + 3; HI: the group holds pairs of
 // 4-bit fields (a byte then adds its high field too).
-template <int TPR, bool Q, bool U4>
+template <int TPR, bool Q, bool HI>
 __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long long begin, long long count, int buf,
                                           int sg, int sh, int chunk, int nchunks, int fg, long long* out_slab) {
   const long long per = (count + nchunks - 1) / nchunks;
   const long long r0 = (long long)chunk * per, r1 = min(count, r0 + per);
   // quantized: one packed (g << 32) + h int64 per bin; FP64: g and h int64
   const int n_lds = a.ginfo[2 * fg] * (Q ? 1 : 2);
-  const bool u4g = U4 && a.ginfo[2 * fg + 1] != 0;
+  constexpr bool u4g = HI;
   unsigned long long* lds64 = (unsigned long long*)ldsv;
   for (int e = threadIdx.x; e < n_lds; e += blockDim.x) lds64[e] = 0ull;
   // this lane's 4 bytes: entries, masks and most frequent bins of their fields
   const int sub = threadIdx.x / TPR, j = threadIdx.x % TPR;
+  // (a byte's high field's entries follow its low field's 16)
   const bool active = j * 4 < a.FB;
-  int off_lo[4], off_hi[4];
-  unsigned mask_lo[4], mask_hi[4], mfb_lo[4], mfb_hi[4];
+  int off_lo[4];
+  unsigned mask_lo[4], mfb_lo[4];
+  unsigned himask = 0u, mfb_h = 0u;   // the high fields: present (bit k), 4-bit most frequent bins
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     ByteDev bd{0, 0, 0u, 0u, 0u, 0u};
     if (active) bd = a.bytes[fg * a.FB + j * 4 + k];
     off_lo[k] = bd.off_lo;
-    off_hi[k] = bd.off_hi;
     mask_lo[k] = bd.mask_lo;
-    mask_hi[k] = bd.mask_hi;
     mfb_lo[k] = bd.mfb_lo;
-    mfb_hi[k] = bd.mfb_hi;
+    himask |= (bd.mask_hi ? 1u : 0u) << k;
+    mfb_h |= (bd.mfb_hi & 0xFu) << (4 * k);
   }
   __syncthreads();
   const int rpp = HistBlock<Q>::threads / TPR;
@@ -232,8 +233,8 @@ __device__ __forceinline__ void hist_unit(const HistArgs& a, void* ldsv, long lo
         const unsigned b = by & mask_lo[k];
         if (mask_lo[k] && b != mfb_lo[k]) add(off_lo[k] + (int)b, gv[u], hv[u]);
         if (u4g) {
-          const unsigned bh = (by >> 4) & mask_hi[k];
-          if (mask_hi[k] && bh != mfb_hi[k]) add(off_hi[k] + (int)bh, gv[u], hv[u]);
+          const unsigned bh = by >> 4;
+          if (((himask >> k) & 1u) && bh != ((mfb_h >> (4 * k)) & 0xFu)) add(off_lo[k] + 16 + (int)bh, gv[u], hv[u]);
         }
       }
     }
@@ -280,9 +281,9 @@ __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) 
 // The row loop has no branches per bin: a bin equal to the feature's most
 // frequent bin (never accumulated), and every field the lane does not hold,
 // adds to a per-lane trash entry past the histogram; rows past the chunk's
-// end (the last pass) add 0. U4: the group may hold pairs of 4-bit fields
-// (a byte then adds its high field too).
-template <int TPR, bool U4>
+// end (the last pass) add 0. HI: the group holds pairs of 4-bit fields (a
+// byte then adds its high field too).
+template <int TPR, bool HI>
 __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32, long long begin, long long count,
                                               int buf, int chunk, int nchunks, int fg, long long* out_slab,
                                               int epoch_rows) {
@@ -293,7 +294,7 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
   const long long r0 = (long long)chunk * per, r1 = min(count, r0 + per);
   const int rows = (int)max(0LL, r1 - r0);
   const int n_lds = a.ginfo[2 * fg];
-  const bool u4g = U4 && a.ginfo[2 * fg + 1] != 0;
+  constexpr bool u4g = HI;
   const int n32 = a.pu;
   const int sub = threadIdx.x / TPR, j = threadIdx.x % TPR;
   const unsigned trash = (unsigned)(n32 + (threadIdx.x & 63)) * 4u;   // byte offset
@@ -308,9 +309,10 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
   // this lane's 4 bytes: the entries and most frequent bins of their fields;
   // a field's mask keeps its bits of the row's word (a masked-off field
   // reads 0 against most frequent bin 0: the trash)
+  // (a byte's high field's entries follow its low field's 16)
   const bool active = j * 4 < a.FB;
   unsigned fbase[4], mfb[4], bytemask = 0u;
-  unsigned fbase_h[4], mfb_h[4], himask = 0u;
+  unsigned himask = 0u, mfb_h = 0u;   // the high fields: bits of the row's word, 4-bit most frequent bins
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     ByteDev bd{0, 0, 0u, 0u, 0u, 0u};
@@ -318,9 +320,8 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
     fbase[k] = (unsigned)bd.off_lo * 4u;
     mfb[k] = bd.mfb_lo;
     bytemask |= bd.mask_lo << (8 * k);
-    fbase_h[k] = (unsigned)bd.off_hi * 4u;
-    mfb_h[k] = bd.mfb_hi;
     himask |= (bd.mask_hi << 4) << (8 * k);
+    mfb_h |= (bd.mfb_hi & 0xFu) << (4 * k);
   }
   if (!one_epoch)
     for (int e = threadIdx.x; e < n_lds; e += T) acc[e] = 0ull;
@@ -346,10 +347,7 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
   // LDS byte addresses, the histogram's base folded in
   const unsigned lbase = (unsigned)(size_t)(LWord)lds32;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    fbase[k] += lbase;
-    fbase_h[k] += lbase;
-  }
+  for (int k = 0; k < 4; ++k) fbase[k] += lbase;
   const unsigned trash_a = trash + lbase;
   // one pass's loads (a row past the chunk reloads its last row)
   auto load_pass = [&](int q0, unsigned* w, unsigned* hr) {
@@ -386,7 +384,7 @@ __device__ __forceinline__ void hist_unit_q32(const HistArgs& a, unsigned* lds32
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const unsigned b = (wh >> (8 * k + 4)) & 0xFu;
-          const unsigned addr = b == mfb_h[k] ? trash_a : fbase_h[k] + b * 4u;
+          const unsigned addr = b == ((mfb_h >> (4 * k)) & 0xFu) ? trash_a : fbase[k] + (16u + b) * 4u;
           __hip_atomic_fetch_add((LWord)(size_t)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       }
@@ -583,9 +581,14 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
       __syncthreads();
       for (int u = vb; u < units; u += gridDim.x) {
         while (u >= s_u0[w + 1]) ++w;
-        const int r = u - s_u0[w];
-        hist_unit_q32<TPR, U4>(a, lds32, s_begin[w], s_count[w], s_buf[w], r / a.NG, s_nch[w], r % a.NG,
-                           a.slab + (size_t)u * ustride, erows);
+        const int r = u - s_u0[w], fg = r % a.NG;
+        // a group holding 4-bit fields takes the loop that adds both fields of a byte
+        if (U4 && a.ginfo[2 * fg + 1])
+          hist_unit_q32<TPR, true>(a, lds32, s_begin[w], s_count[w], s_buf[w], r / a.NG, s_nch[w], fg,
+                                   a.slab + (size_t)u * ustride, erows);
+        else
+          hist_unit_q32<TPR, false>(a, lds32, s_begin[w], s_count[w], s_buf[w], r / a.NG, s_nch[w], fg,
+                                    a.slab + (size_t)u * ustride, erows);
       }
       return;
     }
@@ -593,14 +596,21 @@ __device__ __forceinline__ void hist_kernel(const HistArgs& a, void* lds) {
   if (!Q && blockIdx.x == 0 && tid < nitems) a.units[kUnitForm + tid] = 0;
   for (int u = vb; u < units; u += gridDim.x) {
     while (u >= s_u0[w + 1]) ++w;
-    const int r = u - s_u0[w];
-    hist_unit<TPR, Q, U4>(a, lds, s_begin[w], s_count[w], s_buf[w], s_sg[w], s_sh[w], r / a.NG, s_nch[w], r % a.NG,
-                      a.slab + (size_t)u * ustride);
+    const int r = u - s_u0[w], fg = r % a.NG;
+    if (U4 && a.ginfo[2 * fg + 1])
+      hist_unit<TPR, Q, true>(a, lds, s_begin[w], s_count[w], s_buf[w], s_sg[w], s_sh[w], r / a.NG, s_nch[w], fg,
+                              a.slab + (size_t)u * ustride);
+    else
+      hist_unit<TPR, Q, false>(a, lds, s_begin[w], s_count[w], s_buf[w], s_sg[w], s_sh[w], r / a.NG, s_nch[w], fg,
+                               a.slab + (size_t)u * ustride);
   }
 }
 
+// five 256-thread blocks per CU's SIMDs (at most 96 VGPRs; the 4-bit form's
+// second field per byte would otherwise take it to four)
 template <int TPR, bool U4>
-__global__ void __launch_bounds__(HistBlock<false>::threads) k_hist_f64(HistArgs a) {
+__global__ void __launch_bounds__(HistBlock<false>::threads) __attribute__((amdgpu_waves_per_eu(5)))
+k_hist_f64(HistArgs a) {
   extern __shared__ unsigned long long lds_hist[];
   hist_kernel<TPR, false, U4>(a, lds_hist);
 }
@@ -616,7 +626,8 @@ k_hist_int(HistArgs a) {
 
 // The trees of a batch: grid y picks the tree, grid x is its blocks.
 template <int TPR, bool U4>
-__global__ void __launch_bounds__(HistBlock<false>::threads) k_hist_f64_tab(const HistArgs* __restrict__ tab) {
+__global__ void __launch_bounds__(HistBlock<false>::threads) __attribute__((amdgpu_waves_per_eu(5)))
+k_hist_f64_tab(const HistArgs* __restrict__ tab) {
   extern __shared__ unsigned long long lds_hist[];
   const HistArgs a = tab[blockIdx.y];
   hist_kernel<TPR, false, U4>(a, lds_hist);
