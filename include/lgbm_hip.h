@@ -362,15 +362,20 @@ lgbmhip_status lgbmhip_lds_atomic_peak(lgbmhip_ctx* ctx, double* lane_atomics_pe
 /* Launch shapes of the tree-growth kernels (0 keeps the current value):
  * partition blocks, histogram row chunks, minimum rows per histogram chunk,
  * LDS budget in bytes of one histogram block (sets features per block),
- * growth rounds in a tree's first launch and in each continuation launch.
- * Defaults: part_grid = min(tiles, 4 x CUs), hist_chunks = 2 x CUs
- * (quantized) or CUs (FP64), 4096 rows (quantized: at most one epoch of the
- * 32-bit histogram bins, 4095 for 16 quantization bins), 32 KiB,
- * 24 and 8 rounds. Trees do not depend on any of them.
+ * growth rounds in a tree's first launch and in each continuation launch,
+ * and late_round (-1 keeps the current value, 0 never): from that growth
+ * round on, the split search sums the histogram pass's row-chunk units
+ * itself and the round has no slab-reduction launch. Defaults: part_grid =
+ * min(tiles, 4 x CUs), hist_chunks = 2 x CUs (quantized) or CUs (FP64),
+ * 4096 rows (quantized: at most one epoch of the 32-bit histogram bins,
+ * 4095 for 16 quantization bins), 32 KiB, 24 and 8 rounds, late_round 12
+ * (two more per doubling of rows past 16M). Trees do not depend on any of
+ * them.
  * No counterpart in the reference (its thread count is its only knob); for
  * launch-shape sweeps on other row/feature shapes (Booster.set_tuning). */
 lgbmhip_status lgbmhip_set_tuning(lgbmhip_ctx* ctx, int32_t part_grid, int32_t hist_chunks, int32_t hist_min_rows,
-                                  int32_t hist_lds_budget, int32_t rounds_main, int32_t rounds_cont);
+                                  int32_t hist_lds_budget, int32_t rounds_main, int32_t rounds_cont,
+                                  int32_t late_round);
 /* The autotuner: before the first iteration, times boosting iterations of
  * the workload itself (trees do not depend on launch shapes) at candidate
  * launch shapes, by coordinate descent over the histogram LDS budget
@@ -388,8 +393,8 @@ lgbmhip_status lgbmhip_set_tuning(lgbmhip_ctx* ctx, int32_t part_grid, int32_t h
  * Unsupported with bagging or GOSS. */
 lgbmhip_status lgbmhip_autotune(lgbmhip_ctx* ctx, int32_t trees_per_candidate, int32_t* out5);
 /* Current launch shapes: {part_grid, hist_chunks, hist_min_rows, hist_lds_budget, features per hist block,
- * hist feature groups, threads per row, rounds_main, rounds_cont}. */
-lgbmhip_status lgbmhip_get_tuning(lgbmhip_ctx* ctx, int32_t* out9);
+ * hist feature groups, threads per row, rounds_main, rounds_cont, late_round}. */
+lgbmhip_status lgbmhip_get_tuning(lgbmhip_ctx* ctx, int32_t* out10);
 
 /* Device-time of the last training iteration's kernels, by kind (ms): histogram, split, partition, other. */
 lgbmhip_status lgbmhip_last_timings(lgbmhip_ctx* ctx, double* out4);

@@ -569,11 +569,13 @@ class Booster:
         return float(out[0])
 
     def set_tuning(self, part_grid=0, hist_chunks=0, hist_min_rows=0, hist_lds_budget=0, rounds_main=0,
-                   rounds_cont=0):
+                   rounds_cont=0, late_round=-1):
         """Launch shapes of the tree-growth kernels (before the first update;
-        0 keeps a value). Trees do not depend on them."""
+        0 keeps a value, and -1 for late_round, where 0 means never). Trees
+        do not depend on them."""
         check(lib().lgbmhip_set_tuning(self._ctx, int(part_grid), int(hist_chunks), int(hist_min_rows),
-                                       int(hist_lds_budget), int(rounds_main), int(rounds_cont)), self._ctx)
+                                       int(hist_lds_budget), int(rounds_main), int(rounds_cont), int(late_round)),
+              self._ctx)
 
     def bin_layout(self):
         """The training rows in HBM: bytes per packed row, its padded stride,
@@ -583,10 +585,10 @@ class Booster:
         return dict(zip(("row_bytes", "row_stride", "u4_features"), out.tolist()))
 
     def get_tuning(self):
-        out = np.zeros(9, np.int32)
+        out = np.zeros(10, np.int32)
         check(lib().lgbmhip_get_tuning(self._ctx, ptr(out)), self._ctx)
         keys = ("part_grid", "hist_chunks", "hist_min_rows", "hist_lds", "hist_features_per_block",
-                "hist_groups", "hist_threads_per_row", "rounds_main", "rounds_cont")
+                "hist_groups", "hist_threads_per_row", "rounds_main", "rounds_cont", "late_round")
         return dict(zip(keys, out.tolist()))
 
     def last_iteration_ms(self):

@@ -126,7 +126,7 @@ SIGNATURES = {
     "lgbmhip_bins_feature_infos": (I32, [P, P, I64, C.POINTER(I64)]),
     "lgbmhip_profile": (I32, [P, I32, P]),
     "lgbmhip_lds_atomic_peak": (I32, [P, P]),
-    "lgbmhip_set_tuning": (I32, [P, I32, I32, I32, I32, I32, I32]),
+    "lgbmhip_set_tuning": (I32, [P, I32, I32, I32, I32, I32, I32, I32]),
     "lgbmhip_autotune": (I32, [P, I32, P]),
     "lgbmhip_get_tuning": (I32, [P, P]),
     "lgbmhip_set_bin_packing": (I32, [P, I32]),

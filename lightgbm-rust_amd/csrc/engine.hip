@@ -315,6 +315,10 @@ struct Ctx {
   int n_calls = 0;
   int hist_tpr = 8, hist_fb = 0, hist_ng = 1, hist_chunks = 256, hist_min_rows = 2048;
   int hist_pu = 0;       // entries of a histogram unit (the largest group's)
+  // growth rounds from this index on search straight from the histogram
+  // units (no slab reduction launch); 0: never; -1 in tune_late_round: the
+  // default
+  int late_round = 12, tune_late_round = -1;
   int hist_grid = 256;   // blocks of a histogram launch (they loop over the units)
   size_t hist_lds = 0;
   int part_grid = 1024;
@@ -846,6 +850,14 @@ static void set_shapes(Ctx* c, bool quant) {
     if (epoch >= 2048) rows_q = std::min(rows_q, epoch);
   }
   c->hist_min_rows = c->tune_hist_min_rows > 0 ? c->tune_hist_min_rows : rows_q;
+  // round 12 on 10.5M rows (its leaves take one or two row chunks; measured
+  // against rounds 1-32), two rounds later per doubling of rows past 16M
+  if (c->tune_late_round >= 0) {
+    c->late_round = c->tune_late_round;
+  } else {
+    c->late_round = 12;
+    for (int64_t n = N; n > ((int64_t)16 << 20); n /= 2) c->late_round += 2;
+  }
   plan_hist(c, quant);
   // one resident wave of blocks: blocks loop over the units, and blocks
   // beyond what the CUs hold at once only add dispatch latency to the
@@ -1303,19 +1315,26 @@ static void enqueue_rounds(Ctx* c, Lane& ln, int k, int r0, int n) {
   const bool quant = c->opts.use_quantized_grad != 0;
   const HistArgs ha = hist_args(c, ln, quant, k, 3);
   const SplitArgs sa = split_args(c, ln, k, false);
+  // rounds from late_round on: few, small leaves of one or two row chunks,
+  // whose units the search sums itself, one launch fewer per round
+  SplitArgs sa_raw = sa;
+  sa_raw.slab = ln.slab.p;
+  sa_raw.units = ln.units.p;
+  sa_raw.raw = 1;
   const PartArgs pa = part_args(c, ln);
   const int hgrid = c->hist_grid;
   for (int r = r0; r < r0 + n; ++r) {
     const int step = r + 1;
+    const bool late = c->late_round > 0 && r >= c->late_round;
     launch_plan(pa, s);
     prof_mark(c, k, step, 2, 0);
     launch_partition(pa, c->part_grid, s);
     prof_mark(c, k, step, 2, 1);
     launch_hist(ha, quant, c->hist_tpr, hgrid, c->hist_lds, s);
     prof_mark(c, k, step, 0, 1);
-    slab_total(c, ln, quant, s);
+    if (!late) slab_total(c, ln, quant, s);
     prof_mark(c, k, step, 3, 1);
-    launch_split(sa, s);
+    launch_split(late ? sa_raw : sa, s);
     prof_mark(c, k, step, 1, 1);
   }
 }
@@ -3718,12 +3737,14 @@ lgbmhip_status lgbmhip_lds_atomic_peak(lgbmhip_ctx* ctx, double* lane_atomics_pe
 }
 
 lgbmhip_status lgbmhip_set_tuning(lgbmhip_ctx* ctx, int32_t part_grid, int32_t hist_chunks, int32_t hist_min_rows,
-                                  int32_t hist_lds_budget, int32_t rounds_main, int32_t rounds_cont) {
+                                  int32_t hist_lds_budget, int32_t rounds_main, int32_t rounds_cont,
+                                  int32_t late_round) {
   return guarded(ctx, [&] {
     Ctx* c = &ctx->c;
     if (part_grid < 0 || hist_chunks < 0 || hist_min_rows < 0 || hist_lds_budget < 0 || rounds_main < 0 ||
-        rounds_cont < 0)
-      throw Error(LGBMHIP_INVALID_PARAMETER, "tuning values must be >= 0");
+        rounds_cont < 0 || late_round < -1)
+      throw Error(LGBMHIP_INVALID_PARAMETER, "tuning values must be >= 0 (late_round >= -1)");
+    if (late_round >= 0) c->tune_late_round = late_round;
     if (hist_lds_budget > 0 && hist_lds_budget < 4096) throw Error(LGBMHIP_INVALID_PARAMETER, "LDS budget below 4 KiB");
     if (rounds_main > 4096 || rounds_cont > 4096) throw Error(LGBMHIP_INVALID_PARAMETER, "growth rounds above 4096");
     if (part_grid) c->tune_part_grid = part_grid;
@@ -3759,19 +3780,20 @@ lgbmhip_status lgbmhip_bin_layout(lgbmhip_ctx* ctx, int32_t* out3) {
     out3[2] = n4;
   });
 }
-lgbmhip_status lgbmhip_get_tuning(lgbmhip_ctx* ctx, int32_t* out9) {
+lgbmhip_status lgbmhip_get_tuning(lgbmhip_ctx* ctx, int32_t* out10) {
   return guarded(ctx, [&] {
     Ctx* c = &ctx->c;
     if (!c->state_ready) setup_training(c);
-    out9[0] = c->part_grid;
-    out9[1] = c->hist_chunks;
-    out9[2] = c->hist_min_rows;
-    out9[3] = (int32_t)c->hist_lds;
-    out9[4] = c->hist_fb;
-    out9[5] = c->hist_ng;
-    out9[6] = c->hist_tpr;
-    out9[7] = c->rounds_main;
-    out9[8] = c->rounds_cont;
+    out10[0] = c->part_grid;
+    out10[1] = c->hist_chunks;
+    out10[2] = c->hist_min_rows;
+    out10[3] = (int32_t)c->hist_lds;
+    out10[4] = c->hist_fb;
+    out10[5] = c->hist_ng;
+    out10[6] = c->hist_tpr;
+    out10[7] = c->rounds_main;
+    out10[8] = c->rounds_cont;
+    out10[9] = c->late_round;
   });
 }
 

@@ -659,6 +659,7 @@ struct SplitArgs {
   IterDev* it;
   int quant;
   int root;              // 1: search the root (slot 0, row 0)
+  int raw;               // 1: slab and units are the histogram pass's own (chunk units, a 32-bit form per item)
 };
 
 // The leaf sums a search needs (leaf_splits.hpp; the root's come from the
@@ -722,6 +723,32 @@ __device__ __forceinline__ void slab_sum(const SplitArgs& a, int w, const FeatDe
   unsigned long long s0[PB], s1[PB];
 #pragma unroll
   for (int k = 0; k < PB; ++k) s0[k] = s1[k] = 0;
+  if (C == 1 && a.raw && a.units[kUnitForm + w]) {
+    // the item's chunks hold packed 32-bit bins, sum_gq * 65536 + sum_hq each
+    const unsigned* b32 = (const unsigned*)(a.slab + (size_t)(u0 + fd.hg) * ustride) + fd.hoff;
+    const size_t cs32 = cstride * 2;
+    for (int c = 0; c < nch; c += 4) {
+      unsigned x[4][PB];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = 0; k < PB; ++k) {
+          const int b = tid + 128 * k;
+          x[j][k] = (c + j < nch && b < nb) ? b32[(size_t)(c + j) * cs32 + b] : 0u;
+        }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = 0; k < PB; ++k)
+          s0[k] += ((unsigned long long)(long long)((int)x[j][k] >> 16) << 32) + (unsigned long long)(x[j][k] & 0xFFFFu);
+    }
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+      out0[k] = (long long)s0[k];
+      out1[k] = 0;
+    }
+    return;
+  }
   int c = 0;
   for (; c + 8 <= nch; c += 8) {
     long long x0[8][PB], x1[8][PB];

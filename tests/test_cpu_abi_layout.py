@@ -114,3 +114,23 @@ def test_rust_stub_layout_equals_the_header(c_layout, name):
         off += sz
         align = max(align, sz)
     assert (off + align - 1) // align * align == lay[name]["."]
+
+
+def header_prototypes():
+    """lgbmhip_* function prototypes of the header: {name: parameter count}."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for name, params in re.findall(r"\b\w[\w\s\*]*?\b(lgbmhip_\w+)\s*\(([^)]*)\)\s*;", text):
+        params = params.strip()
+        out[name] = 0 if params in ("", "void") else len(params.split(","))
+    return out
+
+
+def test_ctypes_signatures_match_the_header():
+    """Every function the binding declares takes as many arguments as its
+    header prototype, and every header function is bound."""
+    protos = header_prototypes()
+    table = capi.SIGNATURES
+    assert set(table) == set(protos), (set(table) ^ set(protos))
+    for name, (_, args) in table.items():
+        assert len(args) == protos[name], f"{name}: ctypes {len(args)} arguments, header {protos[name]}"

@@ -459,11 +459,46 @@ def test_growth_rounds_per_launch_do_not_change_trees():
 # per row (a 64 KiB LDS budget), leaves cut into few long chunks that fold
 # their 32-bit bins into int64 sums several times (8 chunks per leaf), and
 # short chunks (1024 rows). Trees must not depend on any of them.
-@pytest.mark.parametrize("tuning", [dict(hist_lds_budget=65536), dict(hist_chunks=8), dict(hist_min_rows=1024)])
+@pytest.mark.parametrize("tuning", [dict(hist_lds_budget=65536), dict(hist_chunks=8), dict(hist_min_rows=1024),
+                                    dict(late_round=1), dict(late_round=0)])
 def test_quantized_histogram_launch_shapes_do_not_change_trees(tuning):
     params = dict(P_BIN, num_leaves=63, max_bin=255, min_data_in_leaf=20, use_quantized_grad=True,
                   num_grad_quant_bins=16, quant_train_renew_leaf=True)
-    _same_trees_as_oracle(params, trees=2, tuning=tuning)
+    b = _same_trees_as_oracle(params, trees=2, tuning=tuning)
+    if "late_round" in tuning:
+        assert b.get_tuning()["late_round"] == tuning["late_round"]
+
+
+# late_round 1: every growth round after the first searches straight from
+# the histogram pass's row-chunk units (FP64 pairs here, many chunks per
+# leaf early in the tree), with no slab reduction
+def test_fp64_search_from_histogram_units_matches_oracle():
+    params = dict(P_BIN, num_leaves=127, max_bin=63, min_data_in_leaf=20)
+    _same_trees_as_oracle(params, trees=2, tuning=dict(late_round=1, hist_chunks=16))
+
+
+@pytest.mark.parametrize("quant", [False, True])
+def test_search_from_histogram_units_same_boosted_model(quant):
+    """Whole runs with every round, no round and the default rounds searching
+    from the histogram units give one model bit for bit."""
+    X, y = datagen.binary_100k(120000)
+    params = dict(P_BIN, num_leaves=255, max_bin=255, min_data_in_leaf=10)
+    if quant:
+        params.update(use_quantized_grad=True, num_grad_quant_bins=16, stochastic_rounding=True,
+                      quant_train_renew_leaf=True)
+    models = []
+    for late in (0, 1, -1):
+        ds, b = _booster(X, y, params)
+        b.set_tuning(late_round=late)
+        for _ in range(12):
+            b.update()
+        models.append(b)
+    assert models[2].get_tuning()["late_round"] == 12
+    for m in models[1:]:
+        assert m.model_to_string() == models[0].model_to_string()
+        np.testing.assert_array_equal(m.train_scores(), models[0].train_scores())
+    for m in models:
+        m.free()
 
 
 def test_batched_growth_with_max_depth_and_min_data():
