@@ -347,11 +347,11 @@ struct Ctx {
   int nbatch = 0;
   DevBuf<PartArgs> tab_part;
   DevBuf<HistArgs> tab_hist_root, tab_hist_round;
-  DevBuf<SplitArgs> tab_split_root, tab_split_round;
+  DevBuf<SplitArgs> tab_split_root, tab_split_round, tab_split_raw;
   DevBuf<SlabTab> tab_slab;
   std::vector<PartArgs> h_tab_part;
   std::vector<HistArgs> h_tab_hist_root, h_tab_hist_round;
-  std::vector<SplitArgs> h_tab_split_root, h_tab_split_round;
+  std::vector<SplitArgs> h_tab_split_root, h_tab_split_round, h_tab_split_raw;
   std::vector<SlabTab> h_tab_slab;
   std::vector<std::vector<std::pair<GraphKey, hipGraphExec_t>>> graphs_batch;   // [2 g] main, [2 g + 1] continuation
   std::vector<int> want_rounds;   // rounds the class's previous tree ran to completion
@@ -1129,6 +1129,7 @@ static void setup_training(Ctx* c) {
     c->tab_hist_round.alloc(K);
     c->tab_split_root.alloc(K);
     c->tab_split_round.alloc(K);
+    c->tab_split_raw.alloc(K);
     c->tab_slab.alloc(K);
   }
   c->last_rounds.assign(K, 0);
@@ -1494,6 +1495,7 @@ static void fill_tabs(Ctx* c) {
   c->h_tab_hist_round.resize(K);
   c->h_tab_split_root.resize(K);
   c->h_tab_split_round.resize(K);
+  c->h_tab_split_raw.resize(K);
   c->h_tab_slab.resize(K);
   for (int k = 0; k < K; ++k) {
     Lane& ln = *c->lanes[k];
@@ -1502,6 +1504,11 @@ static void fill_tabs(Ctx* c) {
     c->h_tab_hist_round[k] = hist_args(c, ln, quant, k, 3);
     c->h_tab_split_root[k] = split_args(c, ln, k, true);
     c->h_tab_split_round[k] = split_args(c, ln, k, false);
+    // rounds from late_round on: the search reads the histogram units
+    c->h_tab_split_raw[k] = c->h_tab_split_round[k];
+    c->h_tab_split_raw[k].slab = ln.slab.p;
+    c->h_tab_split_raw[k].units = ln.units.p;
+    c->h_tab_split_raw[k].raw = 1;
     c->h_tab_slab[k] = SlabTab{ln.slab.p, ln.units.p, ln.acc.p};
   }
   hipStream_t s = c->stream;
@@ -1513,6 +1520,8 @@ static void fill_tabs(Ctx* c) {
   HIP_CHECK(hipMemcpyAsync(c->tab_split_root.p, c->h_tab_split_root.data(), sizeof(SplitArgs) * K,
                            hipMemcpyHostToDevice, s));
   HIP_CHECK(hipMemcpyAsync(c->tab_split_round.p, c->h_tab_split_round.data(), sizeof(SplitArgs) * K,
+                           hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(c->tab_split_raw.p, c->h_tab_split_raw.data(), sizeof(SplitArgs) * K,
                            hipMemcpyHostToDevice, s));
   HIP_CHECK(hipMemcpyAsync(c->tab_slab.p, c->h_tab_slab.data(), sizeof(SlabTab) * K, hipMemcpyHostToDevice, s));
 }
@@ -1565,9 +1574,10 @@ static void enqueue_rounds_batch(Ctx* c, const BatchRef& b, int r0, int n) {
     prof_mark(c, b.k0, step, 2, 1);
     launch_hist_tab(c->tab_hist_round.p + b.k0, b.T, quant, c->has_u4, c->hist_tpr, hgrid, c->hist_lds, b.s);
     prof_mark(c, b.k0, step, 0, 1);
-    slab_total_batch(c, b, quant);
+    const bool late = c->late_round > 0 && r >= c->late_round;
+    if (!late) slab_total_batch(c, b, quant);
     prof_mark(c, b.k0, step, 3, 1);
-    launch_split_tab(c->tab_split_round.p + b.k0, b.T, c->F, false, b.s);
+    launch_split_tab((late ? c->tab_split_raw.p : c->tab_split_round.p) + b.k0, b.T, c->F, false, b.s);
     prof_mark(c, b.k0, step, 1, 1);
   }
 }

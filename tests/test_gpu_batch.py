@@ -94,3 +94,27 @@ def test_batched_trees_match_oracle_on_the_same_gradients():
             p = max(1e-15, float(np.count_nonzero(y == t)) / n)
             bias = np.log(p)
         assert compare_or_near_tie(b.tree(t), ob.tree(t), 0.1, bias)
+
+
+@pytest.mark.parametrize("quant", [False, True])
+def test_batched_trees_search_from_histogram_units(quant):
+    """Batched rounds whose searches sum the histogram units themselves
+    (late_round 1: every round after the first) give the trees of lanes whose
+    rounds all go through the slab reduction (late_round 0)."""
+    X, y = _data(n=30000, seed=23)
+    params = dict(objective="multiclass", num_class=5, num_leaves=63, max_bin=63, min_data_in_leaf=10,
+                  categorical_feature=[9])
+    if quant:
+        params.update(use_quantized_grad=True, num_grad_quant_bins=16, stochastic_rounding=True,
+                      quant_train_renew_leaf=True)
+    boosters = []
+    for batch, late in ((True, 1), (False, 0)):
+        ds = lgb.Dataset(X, label=y, params=params)
+        b = lgb.Booster(dict(params, batch_trees=batch), ds)
+        b.set_tuning(late_round=late)
+        for _ in range(5):
+            b.update()
+        boosters.append(b)
+    _same_model(*boosters)
+    for b in boosters:
+        b.free()
