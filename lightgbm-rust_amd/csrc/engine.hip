@@ -362,7 +362,7 @@ struct Ctx {
   bool models_empty_init = true;
   int iter = 0;
   // lambdarank state (rank.rs:129-341)
-  DevBuf<int> d_qb, r_rank, r_sorted;
+  DevBuf<int> d_qb, r_rank, r_order;
   DevBuf<double> r_imd, r_table, r_gain, r_disc;
   DevBuf<float> r_lam, r_hes, r_bl;
   DevBuf<double> r_bs;
@@ -796,7 +796,7 @@ static void setup_rank(Ctx* c) {
   c->r_gain.alloc(31);
   c->r_disc.alloc(10000);
   c->r_rank.alloc(c->N);
-  c->r_sorted.alloc(c->N);
+  c->r_order.alloc(Q);
   c->r_lam.alloc(c->N);
   c->r_hes.alloc(c->N);
   c->r_bs.alloc(c->N);
@@ -806,6 +806,12 @@ static void setup_rank(Ctx* c) {
   HIP_CHECK(hipMemcpy(c->r_table.p, table.data(), bins * 8, hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->r_gain.p, gain.data(), 31 * 8, hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->r_disc.p, disc.data(), 10000 * 8, hipMemcpyHostToDevice));
+  // blocks take the longest queries first, so they do not trail the launch
+  std::vector<int> order(Q);
+  for (int q = 0; q < Q; ++q) order[q] = q;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return qb[x + 1] - qb[x] > qb[y + 1] - qb[y]; });
+  HIP_CHECK(hipMemcpy(c->r_order.p, order.data(), Q * 4, hipMemcpyHostToDevice));
 }
 
 // A validation set's scores back to their start: its init scores or zeros
@@ -1818,7 +1824,7 @@ static void enqueue_gradients(Ctx* c) {
     a.label_gain = c->r_gain.p;
     a.discount = c->r_disc.p;
     a.rank = c->r_rank.p;
-    a.sorted = c->r_sorted.p;
+    a.order = c->r_order.p;
     a.lam = c->r_lam.p;
     a.hes = c->r_hes.p;
     a.by_rank_s = c->r_bs.p;

@@ -413,9 +413,7 @@ __global__ void __launch_bounds__(256) k_multiclass_grad(const double* __restric
 // objective/rank.rs:192-257 LambdarankNdcg::one_query, one block per query.
 // The reference accumulates each document's f32 lambda/hessian in (i, j)
 // pair-loop order and the normalizer sum_lambdas as one f64 chain in the
-// same order. Outer iterations i run in sequence; within one, the partners
-// j > i each get their single contribution in parallel, while the anchor's
-// contributions and the f64 chain run in j order on one lane from LDS.
+// same order; k_lambdarank keeps every one of those orders (below).
 struct RankArgs {
   const double* score;
   const float* label;
@@ -427,8 +425,8 @@ struct RankArgs {
   int norm, trunc;
   const double* label_gain;  // 31 entries
   const double* discount;    // 10000 entries
+  const int* order;          // [Q]: the queries, longest first (block b runs query order[b])
   int* rank;                 // scratch [N]: rank of each row within its query
-  int* sorted;               // scratch [N]: row (query-local) at each rank
   double* by_rank_s;         // scratch [N]: score at each rank
   float* by_rank_l;          // scratch [N]: label at each rank
   float* lam;                // scratch [N]: accumulators by rank
@@ -438,7 +436,6 @@ struct RankArgs {
 };
 
 constexpr int kRankThreads = 256;
-constexpr int kRankChunk = 1024;
 
 __device__ __forceinline__ double rank_sigmoid(const RankArgs& a, double s) {
   if (s <= a.min_in) return a.table[0];
@@ -446,49 +443,124 @@ __device__ __forceinline__ double rank_sigmoid(const RankArgs& a, double s) {
   return a.table[(size_t)((s - a.min_in) * a.idx_factor)];
 }
 
+// One pair's contribution (rank.rs:213-247): anchor at rank i (score sa,
+// label la), partner at rank j (sb, lb), labels differ. fl / fh: the f32
+// lambda and hessian terms, pl the f64 lambda before its cast (the
+// normalizer's chain subtracts 2 pl).
+// The score gap of a pair, higher label's score first.
+__device__ __forceinline__ double pair_ds(double sa, float la, double sb, float lb) {
+  const bool anchor_high = la > lb;
+  return (anchor_high ? sa : sb) - (anchor_high ? sb : sa);
+}
+// sig: rank_sigmoid of the pair's gap; disc: the discount table by rank.
+__device__ __forceinline__ void rank_pair(const RankArgs& a, const double* sgain, const double* disc, double imd,
+                                          bool scale_by_delta, int i, int j, double sa, float la, double sb, float lb,
+                                          double sig, float& fl, float& fh, double& plo) {
+  const bool anchor_high = la > lb;
+  const int hr = anchor_high ? i : j, lr = anchor_high ? j : i;
+  const double hg = sgain[(int)(anchor_high ? la : lb)], lg = sgain[(int)(anchor_high ? lb : la)];
+  const double ds = pair_ds(sa, la, sb, lb);
+  const double gap = hg - lg;
+  const double pd = fabs(disc[hr] - disc[lr]);
+  double dp = gap * pd * imd;
+  if (scale_by_delta) dp /= (double)0.01f + fabs(ds);
+  double pl = sig;
+  double ph = pl * (1.0 - pl);
+  pl *= -a.sigmoid * dp;
+  ph *= a.sigmoid * a.sigmoid * dp;
+  fl = (float)pl;
+  fh = (float)ph;
+  plo = pl;
+}
+
+// The ordered sums of one query. Every pair term is a function of scores,
+// labels and ranks only; the reference's order binds only the sums: each
+// document's f32 lambda and hessian (its contributions as a partner of the
+// earlier anchors, in anchor order, then, for an anchor, its own terms in
+// partner order) and the f64 normalizer (every pair, anchor by anchor, in
+// partner order). So anchors go in steps of G: the block computes the
+// step's pair terms (a thread owns a partner and applies its contributions
+// in anchor order) and stages them in LDS; then wave 0's lanes run the G
+// anchors' own chains side by side while lane 0 of wave 1 runs the
+// normalizer chain over the step's terms. A query of at most kRankLds
+// documents keeps its arrays by rank in LDS and takes as many anchors per
+// step as its staged terms allow; a longer one keeps them in the global
+// scratch and takes one anchor per step, its partners in chunks.
+constexpr int kRankLds = 512;
+constexpr int kRankStage = 1024;   // pair terms staged per step
+constexpr int kRankGroup = 8;      // anchors per step, at most
+struct RankLds {
+  double st_p[kRankStage];
+  float st_l[kRankStage], st_h[kRankStage];
+  double rs[kRankLds], disc[kRankLds];
+  float rl[kRankLds], lam[kRankLds], hes[kRankLds];
+  double sgain[31];
+  double sfac, S;
+  int sdo;
+};
+
 __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
-  const int q = blockIdx.x;
-  if (q >= a.Q) return;
+  if ((int)blockIdx.x >= a.Q) return;
+  const int q = a.order[blockIdx.x];   // longest queries first
   const int start = a.qb[q], cnt = a.qb[q + 1] - start;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (cnt <= 0) return;
+  __shared__ RankLds R;
+  const bool small = cnt <= kRankLds;
   const double* s = a.score + start;
   const float* lab = a.label + start;
+  // arrays by rank: LDS or the global scratch
+  double* rs = small ? R.rs : a.by_rank_s + start;
+  float* rl = small ? R.rl : a.by_rank_l + start;
+  float* lam = small ? R.lam : a.lam + start;
+  float* hes = small ? R.hes : a.hes + start;
+  const double* disc = small ? R.disc : a.discount;
   int* rank = a.rank + start;
-  int* sorted = a.sorted + start;
-  double* rs = a.by_rank_s + start;   // by rank: contiguous, no index chase in the pair loop
-  float* rl = a.by_rank_l + start;
-  float* lam = a.lam + start;
-  float* hes = a.hes + start;
-  if (cnt <= 0) return;
-  __shared__ double sgain[31];
-  __shared__ float st_l[kRankChunk], st_h[kRankChunk];
-  __shared__ double st_p[kRankChunk];
-  if (tid < 31) sgain[tid] = a.label_gain[tid];
-  // stable descending order: rank = #{j : s_j > s_i or (s_j == s_i and j < i)},
-  // the scores compared from LDS, kRankChunk at a time
-  for (int i0 = 0; i0 < cnt; i0 += kRankThreads) {
-    const int i = i0 + tid;
+  if (tid < 31) R.sgain[tid] = a.label_gain[tid];
+  if (small)
+    for (int i = tid; i < cnt; i += kRankThreads) R.disc[i] = a.discount[i];
+  // ---- stable descending order: rank = #{j : s_j > s_i or (s_j == s_i and
+  // j < i)}, i.e. j < i counts when !(s_j < s_i), j > i when s_j > s_i. The
+  // scores are compared from LDS, kRankStage at a time; tpd threads (a power
+  // of two, adjacent lanes) share a document, each counting every tpd-th j.
+  int tpd = 1;
+  while (tpd < 64 && cnt * tpd * 2 <= kRankThreads) tpd *= 2;
+  const int part = tid & (tpd - 1), per = kRankThreads / tpd;
+  for (int i0 = 0; i0 < cnt; i0 += per) {
+    const int i = i0 + tid / tpd;
     const double si = s[i < cnt ? i : cnt - 1];
     int r = 0;
-    for (int j0 = 0; j0 < cnt; j0 += kRankChunk) {
-      const int jn = min(cnt, j0 + kRankChunk);
+    for (int j0 = 0; j0 < cnt; j0 += kRankStage) {
+      const int jn = min(cnt, j0 + kRankStage);
       __syncthreads();
-      for (int j = j0 + tid; j < jn; j += kRankThreads) st_p[j - j0] = s[j];
+      for (int j = j0 + tid; j < jn; j += kRankThreads) R.st_p[j - j0] = s[j];
       __syncthreads();
-      for (int j = j0; j < jn; ++j) {
-        const double sj = st_p[j - j0];
-        r += (sj > si) || (!(sj < si) && !(sj > si) && j < i);
+      int j = j0 + part;
+      for (; j + 7 * tpd < jn; j += 8 * tpd) {
+        double s8[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) s8[v] = R.st_p[j - j0 + v * tpd];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+          const int jj = j + v * tpd;
+          r += jj < i ? !(s8[v] < si) : (s8[v] > si);
+        }
+      }
+      for (; j < jn; j += tpd) {
+        const double sj = R.st_p[j - j0];
+        r += j < i ? !(sj < si) : (sj > si);
       }
     }
-    if (i < cnt) {
+    for (int o = 1; o < tpd; o <<= 1) r += __shfl_xor(r, o, 64);
+    if (part == 0 && i < cnt) {
       rank[i] = r;
-      sorted[r] = i;
       rs[r] = si;
       rl[r] = lab[i];
-      lam[i] = 0.0f;
-      hes[i] = 0.0f;
+      lam[r] = 0.0f;
+      hes[r] = 0.0f;
     }
   }
+  if (tid == 0) R.S = 0.0;
   __syncthreads();
   const double best = rs[0];
   int worst_idx = cnt - 1;
@@ -496,92 +568,138 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
   const double worst = rs[worst_idx];
   const double imd = a.inv_max_dcg[q];
   const bool scale_by_delta = a.norm && best != worst;
-  double S = 0.0;
   const int imax = min(cnt - 1, max(a.trunc, 0));
-  for (int i = 0; i < imax; ++i) {
-    const double sa = rs[i];
-    if (sa == -INFINITY) continue;
-    const float la = rl[i];
-    float anc_l = lam[i], anc_h = hes[i];   // lane 0 only
-    for (int j0 = i + 1; j0 < cnt; j0 += kRankChunk) {
-      const int jn = min(cnt, j0 + kRankChunk);
-      for (int j = j0 + tid; j < jn; j += kRankThreads) {
+  float anc_l = 0.0f, anc_h = 0.0f;   // wave 0 lane u: anchor g + u's own chains
+  double S = 0.0;                     // wave 1 lane 0: the normalizer
+  for (int g = 0; g < imax;) {
+    // the step: anchors g .. g + G - 1, partners j0 .. j1 - 1 (W of them)
+    const int G = small ? min(min(kRankGroup, imax - g), max(1, kRankStage / cnt)) : 1;
+    for (int j0 = g + 1; j0 < cnt; j0 += kRankStage) {
+      const int j1 = small ? cnt : min(cnt, j0 + kRankStage);
+      const int W = j1 - j0;
+      // ---- pair terms: thread t owns partners j0 + t, j0 + t + 256, ...
+      for (int j = j0 + tid; j < j1; j += kRankThreads) {
         const double sb = rs[j];
         const float lb = rl[j];
-        const bool ok = sb != -INFINITY && la != lb;
-        if (ok) {
-          const bool anchor_high = la > lb;
-          const int hr = anchor_high ? i : j, lr = anchor_high ? j : i;
-          const double hs = anchor_high ? sa : sb, ls = anchor_high ? sb : sa;
-          const double hg = sgain[(int)(anchor_high ? la : lb)], lg = sgain[(int)(anchor_high ? lb : la)];
-          const double ds = hs - ls;
-          const double gap = hg - lg;
-          const double pd = fabs(a.discount[hr] - a.discount[lr]);
-          double dp = gap * pd * imd;
-          if (scale_by_delta) dp /= (double)0.01f + fabs(ds);
-          double pl = rank_sigmoid(a, ds);
-          double ph = pl * (1.0 - pl);
-          pl *= -a.sigmoid * dp;
-          ph *= a.sigmoid * a.sigmoid * dp;
-          const float fl = (float)pl, fh = (float)ph;
-          // the partner (rank j) gets its one contribution of this iteration
-          if (anchor_high) lam[j] -= fl;   // partner is low
-          else lam[j] += fl;
-          hes[j] += fh;
-          st_l[j - j0] = anchor_high ? fl : -fl;
-          st_h[j - j0] = fh;
-          st_p[j - j0] = 2.0 * pl;   // exact: the chain subtracts 2 p
-        } else {
-          // a skipped pair adds nothing: x + (-0.0f) == x and S - 0.0 == S
-          // bit for bit, so the anchor's chains need no branch
-          st_l[j - j0] = -0.0f;
-          st_h[j - j0] = -0.0f;
-          st_p[j - j0] = 0.0;
+        // the sigmoid lookups first (a global table), all in flight together
+        double sig[kRankGroup];
+#pragma unroll
+        for (int u = 0; u < kRankGroup; ++u) {
+          sig[u] = 0.0;
+          const int i = g + u;
+          if (u < G && j > i) {
+            const double sa = rs[i];
+            const float la = rl[i];
+            if (sa != -INFINITY && sb != -INFINITY && la != lb) sig[u] = rank_sigmoid(a, pair_ds(sa, la, sb, lb));
+          }
         }
+        float pl_ = lam[j], ph_ = hes[j];
+#pragma unroll
+        for (int u = 0; u < kRankGroup; ++u) {
+          const int i = g + u;
+          if (u >= G || j <= i) continue;
+          const double sa = rs[i];
+          if (sa == -INFINITY) continue;   // this anchor is skipped
+          const float la = rl[i];
+          const int e = small ? u * cnt + j : j - j0;
+          if (sb != -INFINITY && la != lb) {
+            const bool anchor_high = la > lb;
+            float fl, fh;
+            double p;
+            rank_pair(a, R.sgain, disc, imd, scale_by_delta, i, j, sa, la, sb, lb, sig[u], fl, fh, p);
+            // the partner gets its one contribution of anchor i
+            if (anchor_high) pl_ -= fl;
+            else pl_ += fl;
+            ph_ += fh;
+            R.st_l[e] = anchor_high ? fl : -fl;
+            R.st_h[e] = fh;
+            R.st_p[e] = 2.0 * p;   // exact: the normalizer subtracts 2 p
+          } else {
+            // a skipped pair adds nothing: x + (-0.0f) == x and S - 0.0 == S
+            R.st_l[e] = -0.0f;
+            R.st_h[e] = -0.0f;
+            R.st_p[e] = 0.0;
+          }
+        }
+        lam[j] = pl_;
+        hes[j] = ph_;
       }
       __syncthreads();
-      if (tid == 0) {
-        // anchor high: lambdas[high] += p; anchor low: lambdas[low] -= p.
-        // Three independent chains in j order; loads batched ahead of the adds.
-        const int m = jn - j0;
-        int k = 0;
-        for (; k + 8 <= m; k += 8) {
-          float l8[8], h8[8];
-          double p8[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            l8[u] = st_l[k + u];
-            h8[u] = st_h[k + u];
-            p8[u] = st_p[k + u];
+      // ---- the chains: the anchors' own on wave 0 (lane u: anchor g + u),
+      // the normalizer on wave 1
+      if (wave == 0 && lane < G) {
+        const int i = g + lane;
+        if (rs[i] != -INFINITY) {
+          // the anchor's sums so far: its contributions as a partner
+          if (j0 == g + 1) {
+            anc_l = lam[i];
+            anc_h = hes[i];
           }
+          const int k0 = max(j0, i + 1);
+          const float* l = R.st_l + (small ? lane * cnt : -j0);
+          const float* h = R.st_h + (small ? lane * cnt : -j0);
+          int k = k0;
+          for (; k + 8 <= j1; k += 8) {
+            float l8[8], h8[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            anc_l += l8[u];
-            anc_h += h8[u];
-            S -= p8[u];
+            for (int v = 0; v < 8; ++v) {
+              l8[v] = l[k + v];
+              h8[v] = h[k + v];
+            }
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+              anc_l += l8[v];
+              anc_h += h8[v];
+            }
+          }
+          for (; k < j1; ++k) {
+            anc_l += l[k];
+            anc_h += h[k];
+          }
+          if (j1 == cnt) {
+            lam[i] = anc_l;
+            hes[i] = anc_h;
           }
         }
-        for (; k < m; ++k) {
-          anc_l += st_l[k];
-          anc_h += st_h[k];
-          S -= st_p[k];
+      }
+      if (wave == 1 && lane == 0) {
+        for (int u = 0; u < G; ++u) {
+          const int i = g + u;
+          if (rs[i] == -INFINITY) continue;
+          const double* p = R.st_p + (small ? u * cnt : -j0);
+          int k = max(j0, i + 1);
+          // batches of 8, the next batch's loads in flight under this one's adds
+          if (k + 8 <= j1) {
+            double cur[8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) cur[v] = p[k + v];
+            for (; k + 16 <= j1; k += 8) {
+              double nxt[8];
+#pragma unroll
+              for (int v = 0; v < 8; ++v) nxt[v] = p[k + 8 + v];
+#pragma unroll
+              for (int v = 0; v < 8; ++v) S -= cur[v];
+#pragma unroll
+              for (int v = 0; v < 8; ++v) cur[v] = nxt[v];
+            }
+#pragma unroll
+            for (int v = 0; v < 8; ++v) S -= cur[v];
+            k += 8;
+          }
+          for (; k < j1; ++k) S -= p[k];
         }
       }
       __syncthreads();
     }
-    if (tid == 0) {
-      lam[i] = anc_l;
-      hes[i] = anc_h;
-    }
-    __syncthreads();
+    g += G;
   }
-  __shared__ double sfac;
-  __shared__ int sdo;
-  if (tid == 0) {
-    sdo = a.norm && S > 0.0;
-    sfac = sdo ? log2(1.0 + S) / S : 1.0;
+  if (wave == 1 && lane == 0) {
+    R.sdo = a.norm && S > 0.0;
+    R.sfac = R.sdo ? log2(1.0 + S) / S : 1.0;
   }
   __syncthreads();
+  const double sfac = R.sfac;
+  const bool sdo = R.sdo != 0;
   for (int d = tid; d < cnt; d += kRankThreads) {
     const int r = rank[d];
     float l = lam[r], hh = hes[r];
