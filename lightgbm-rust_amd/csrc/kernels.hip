@@ -499,14 +499,9 @@ struct RankLds {
   int sdo;
 };
 
-__global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
-  if ((int)blockIdx.x >= a.Q) return;
-  const int q = a.order[blockIdx.x];   // longest queries first
-  const int start = a.qb[q], cnt = a.qb[q + 1] - start;
+template <bool small>
+__device__ __forceinline__ void lambdarank_query(const RankArgs& a, RankLds& R, int q, int start, int cnt) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (cnt <= 0) return;
-  __shared__ RankLds R;
-  const bool small = cnt <= kRankLds;
   const double* s = a.score + start;
   const float* lab = a.label + start;
   // arrays by rank: LDS or the global scratch
@@ -517,6 +512,7 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
   const double* disc = small ? R.disc : a.discount;
   int* rank = a.rank + start;
   if (tid < 31) R.sgain[tid] = a.label_gain[tid];
+  const double imd = a.inv_max_dcg[q];
   if (small)
     for (int i = tid; i < cnt; i += kRankThreads) R.disc[i] = a.discount[i];
   // ---- stable descending order: rank = #{j : s_j > s_i or (s_j == s_i and
@@ -566,7 +562,6 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
   int worst_idx = cnt - 1;
   if (worst_idx > 0 && rs[worst_idx] == -INFINITY) worst_idx--;
   const double worst = rs[worst_idx];
-  const double imd = a.inv_max_dcg[q];
   const bool scale_by_delta = a.norm && best != worst;
   const int imax = min(cnt - 1, max(a.trunc, 0));
   float anc_l = 0.0f, anc_h = 0.0f;   // wave 0 lane u: anchor g + u's own chains
@@ -577,7 +572,16 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
     for (int j0 = g + 1; j0 < cnt; j0 += kRankStage) {
       const int j1 = small ? cnt : min(cnt, j0 + kRankStage);
       const int W = j1 - j0;
-      // ---- pair terms: thread t owns partners j0 + t, j0 + t + 256, ...
+      // ---- pair terms: thread t owns partners j0 + t, j0 + t + 256, ...; the
+      // step's anchors in registers. An anchor scored -inf is skipped (no
+      // pair of it counts); its staged terms are the identities below.
+      double sa[kRankGroup];
+      float la[kRankGroup];
+#pragma unroll
+      for (int u = 0; u < kRankGroup; ++u) {
+        sa[u] = u < G ? rs[g + u] : -INFINITY;
+        la[u] = u < G ? rl[g + u] : 0.0f;
+      }
       for (int j = j0 + tid; j < j1; j += kRankThreads) {
         const double sb = rs[j];
         const float lb = rl[j];
@@ -586,27 +590,20 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
 #pragma unroll
         for (int u = 0; u < kRankGroup; ++u) {
           sig[u] = 0.0;
-          const int i = g + u;
-          if (u < G && j > i) {
-            const double sa = rs[i];
-            const float la = rl[i];
-            if (sa != -INFINITY && sb != -INFINITY && la != lb) sig[u] = rank_sigmoid(a, pair_ds(sa, la, sb, lb));
-          }
+          if (j > g + u && sa[u] != -INFINITY && sb != -INFINITY && la[u] != lb)
+            sig[u] = rank_sigmoid(a, pair_ds(sa[u], la[u], sb, lb));
         }
         float pl_ = lam[j], ph_ = hes[j];
 #pragma unroll
         for (int u = 0; u < kRankGroup; ++u) {
           const int i = g + u;
           if (u >= G || j <= i) continue;
-          const double sa = rs[i];
-          if (sa == -INFINITY) continue;   // this anchor is skipped
-          const float la = rl[i];
           const int e = small ? u * cnt + j : j - j0;
-          if (sb != -INFINITY && la != lb) {
-            const bool anchor_high = la > lb;
+          if (sa[u] != -INFINITY && sb != -INFINITY && la[u] != lb) {
+            const bool anchor_high = la[u] > lb;
             float fl, fh;
             double p;
-            rank_pair(a, R.sgain, disc, imd, scale_by_delta, i, j, sa, la, sb, lb, sig[u], fl, fh, p);
+            rank_pair(a, R.sgain, disc, imd, scale_by_delta, i, j, sa[u], la[u], sb, lb, sig[u], fl, fh, p);
             // the partner gets its one contribution of anchor i
             if (anchor_high) pl_ -= fl;
             else pl_ += fl;
@@ -615,7 +612,8 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
             R.st_h[e] = fh;
             R.st_p[e] = 2.0 * p;   // exact: the normalizer subtracts 2 p
           } else {
-            // a skipped pair adds nothing: x + (-0.0f) == x and S - 0.0 == S
+            // a skipped pair adds nothing: x + (-0.0f) == x, and S - 0.0 == S
+            // as S never holds -0.0
             R.st_l[e] = -0.0f;
             R.st_h[e] = -0.0f;
             R.st_p[e] = 0.0;
@@ -629,43 +627,39 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
       // the normalizer on wave 1
       if (wave == 0 && lane < G) {
         const int i = g + lane;
-        if (rs[i] != -INFINITY) {
-          // the anchor's sums so far: its contributions as a partner
-          if (j0 == g + 1) {
-            anc_l = lam[i];
-            anc_h = hes[i];
-          }
-          const int k0 = max(j0, i + 1);
-          const float* l = R.st_l + (small ? lane * cnt : -j0);
-          const float* h = R.st_h + (small ? lane * cnt : -j0);
-          int k = k0;
-          for (; k + 8 <= j1; k += 8) {
-            float l8[8], h8[8];
+        // the anchor's sums so far: its contributions as a partner
+        if (j0 == g + 1) {
+          anc_l = lam[i];
+          anc_h = hes[i];
+        }
+        const float* l = R.st_l + (small ? lane * cnt : -j0);
+        const float* h = R.st_h + (small ? lane * cnt : -j0);
+        int k = max(j0, i + 1);
+        for (; k + 8 <= j1; k += 8) {
+          float l8[8], h8[8];
 #pragma unroll
-            for (int v = 0; v < 8; ++v) {
-              l8[v] = l[k + v];
-              h8[v] = h[k + v];
-            }
+          for (int v = 0; v < 8; ++v) {
+            l8[v] = l[k + v];
+            h8[v] = h[k + v];
+          }
 #pragma unroll
-            for (int v = 0; v < 8; ++v) {
-              anc_l += l8[v];
-              anc_h += h8[v];
-            }
+          for (int v = 0; v < 8; ++v) {
+            anc_l += l8[v];
+            anc_h += h8[v];
           }
-          for (; k < j1; ++k) {
-            anc_l += l[k];
-            anc_h += h[k];
-          }
-          if (j1 == cnt) {
-            lam[i] = anc_l;
-            hes[i] = anc_h;
-          }
+        }
+        for (; k < j1; ++k) {
+          anc_l += l[k];
+          anc_h += h[k];
+        }
+        if (j1 == cnt) {
+          lam[i] = anc_l;
+          hes[i] = anc_h;
         }
       }
       if (wave == 1 && lane == 0) {
         for (int u = 0; u < G; ++u) {
           const int i = g + u;
-          if (rs[i] == -INFINITY) continue;
           const double* p = R.st_p + (small ? u * cnt : -j0);
           int k = max(j0, i + 1);
           // batches of 8, the next batch's loads in flight under this one's adds
@@ -710,6 +704,16 @@ __global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
     a.g[start + d] = l;
     a.h[start + d] = hh;
   }
+}
+
+__global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
+  if ((int)blockIdx.x >= a.Q) return;
+  const int q = a.order[blockIdx.x];   // longest queries first
+  const int start = a.qb[q], cnt = a.qb[q + 1] - start;
+  if (cnt <= 0) return;
+  __shared__ RankLds R;
+  if (cnt <= kRankLds) lambdarank_query<true>(a, R, q, start, cnt);
+  else lambdarank_query<false>(a, R, q, start, cnt);
 }
 
 // max |g|, |h| after the ranking gradients, or over the rows of a bag
