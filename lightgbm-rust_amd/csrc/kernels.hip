@@ -440,7 +440,10 @@ constexpr int kRankThreads = 256;
 __device__ __forceinline__ double rank_sigmoid(const RankArgs& a, double s) {
   if (s <= a.min_in) return a.table[0];
   if (s >= a.max_in) return a.table[1024 * 1024 - 1];
-  return a.table[(size_t)((s - a.min_in) * a.idx_factor)];
+  // in range the product lies in (0, 2^20]: a 32-bit truncation is the
+  // reference's `as usize`; the clamp only bounds the rounding case 2^20
+  const unsigned ix = (unsigned)((s - a.min_in) * a.idx_factor);
+  return a.table[min(ix, 1024u * 1024u - 1u)];
 }
 
 // One pair's contribution (rank.rs:213-247): anchor at rank i (score sa,
@@ -490,7 +493,7 @@ constexpr int kRankLds = 512;
 constexpr int kRankStage = 1024;   // pair terms staged per step
 constexpr int kRankGroup = 8;      // anchors per step, at most
 struct RankLds {
-  double st_p[kRankStage];
+  double st_p[kRankStage + 32];   // + the zero padding the normalizer chain reads
   float st_l[kRankStage], st_h[kRankStage];
   double rs[kRankLds], disc[kRankLds];
   float rl[kRankLds], lam[kRankLds], hes[kRankLds];
@@ -572,6 +575,11 @@ __device__ __forceinline__ void lambdarank_query(const RankArgs& a, RankLds& R, 
     for (int j0 = g + 1; j0 < cnt; j0 += kRankStage) {
       const int j1 = small ? cnt : min(cnt, j0 + kRankStage);
       const int W = j1 - j0;
+      // the step's terms, compact in the normalizer's order: anchor u's
+      // partners k0_u = max(j0, g + u + 1) .. j1 - 1 from off_u on (with
+      // j0 = g + 1 unless G = 1: off_u = u W - u (u - 1) / 2), then zeros
+      const int len = G * W - G * (G - 1) / 2;
+      if (tid < 32) R.st_p[len + tid] = 0.0;
       // ---- pair terms: thread t owns partners j0 + t, j0 + t + 256, ...; the
       // step's anchors in registers. An anchor scored -inf is skipped (no
       // pair of it counts); its staged terms are the identities below.
@@ -598,7 +606,7 @@ __device__ __forceinline__ void lambdarank_query(const RankArgs& a, RankLds& R, 
         for (int u = 0; u < kRankGroup; ++u) {
           const int i = g + u;
           if (u >= G || j <= i) continue;
-          const int e = small ? u * cnt + j : j - j0;
+          const int e = u * W - u * (u - 1) / 2 + j - max(j0, i + 1);
           if (sa[u] != -INFINITY && sb != -INFINITY && la[u] != lb) {
             const bool anchor_high = la[u] > lb;
             float fl, fh;
@@ -632,10 +640,11 @@ __device__ __forceinline__ void lambdarank_query(const RankArgs& a, RankLds& R, 
           anc_l = lam[i];
           anc_h = hes[i];
         }
-        const float* l = R.st_l + (small ? lane * cnt : -j0);
-        const float* h = R.st_h + (small ? lane * cnt : -j0);
-        int k = max(j0, i + 1);
-        for (; k + 8 <= j1; k += 8) {
+        const int off = lane * W - lane * (lane - 1) / 2, n = j1 - max(j0, i + 1);
+        const float* l = R.st_l + off;
+        const float* h = R.st_h + off;
+        int k = 0;
+        for (; k + 8 <= n; k += 8) {
           float l8[8], h8[8];
 #pragma unroll
           for (int v = 0; v < 8; ++v) {
@@ -648,7 +657,7 @@ __device__ __forceinline__ void lambdarank_query(const RankArgs& a, RankLds& R, 
             anc_h += h8[v];
           }
         }
-        for (; k < j1; ++k) {
+        for (; k < n; ++k) {
           anc_l += l[k];
           anc_h += h[k];
         }
@@ -658,29 +667,23 @@ __device__ __forceinline__ void lambdarank_query(const RankArgs& a, RankLds& R, 
         }
       }
       if (wave == 1 && lane == 0) {
-        for (int u = 0; u < G; ++u) {
-          const int i = g + u;
-          const double* p = R.st_p + (small ? u * cnt : -j0);
-          int k = max(j0, i + 1);
-          // batches of 8, the next batch's loads in flight under this one's adds
-          if (k + 8 <= j1) {
-            double cur[8];
+        // one run of len terms, zero-padded to 16: two batches of 8 in
+        // turn, each reloaded while the other is added
+        double x[8], y[8];
 #pragma unroll
-            for (int v = 0; v < 8; ++v) cur[v] = p[k + v];
-            for (; k + 16 <= j1; k += 8) {
-              double nxt[8];
+        for (int v = 0; v < 8; ++v) {
+          x[v] = R.st_p[v];
+          y[v] = R.st_p[8 + v];
+        }
+        for (int k = 0; k < len; k += 16) {
 #pragma unroll
-              for (int v = 0; v < 8; ++v) nxt[v] = p[k + 8 + v];
+          for (int v = 0; v < 8; ++v) S -= x[v];
 #pragma unroll
-              for (int v = 0; v < 8; ++v) S -= cur[v];
+          for (int v = 0; v < 8; ++v) x[v] = R.st_p[k + 16 + v];
 #pragma unroll
-              for (int v = 0; v < 8; ++v) cur[v] = nxt[v];
-            }
+          for (int v = 0; v < 8; ++v) S -= y[v];
 #pragma unroll
-            for (int v = 0; v < 8; ++v) S -= cur[v];
-            k += 8;
-          }
-          for (; k < j1; ++k) S -= p[k];
+          for (int v = 0; v < 8; ++v) y[v] = R.st_p[k + 24 + v];
         }
       }
       __syncthreads();
