@@ -709,7 +709,9 @@ __device__ __forceinline__ void lambdarank_query(const RankArgs& a, RankLds& R, 
   }
 }
 
-__global__ void __launch_bounds__(kRankThreads) k_lambdarank(RankArgs a) {
+// Five blocks per compute unit (96 registers, 31 KB of LDS each): the serial
+// chains of more queries interleave on each SIMD; six measured slower (spills).
+__global__ void __launch_bounds__(kRankThreads, 5) k_lambdarank(RankArgs a) {
   if ((int)blockIdx.x >= a.Q) return;
   const int q = a.order[blockIdx.x];   // longest queries first
   const int start = a.qb[q], cnt = a.qb[q + 1] - start;
