@@ -216,16 +216,17 @@ def recorded_traffic(form, rows, max_bin, num_leaves, kernel="hist"):
     return t.get("hbm_per_algorithmic_byte")
 
 
-# Algorithmic bytes of a partition pass per moved row (the split
-# feature's bin, a row index read and written); the engine moves each row's
+# Algorithmic bytes of a partition pass per moved row (the reference's index
+# partition: the split feature's bin, a row index read and written); the engine moves each row's
 # group-major bins and gradients instead, so every later pass reads its
 # leaves contiguously.
 PART_ALG_BYTES_PER_ROW = 1 + 4 + 4
 
 
 def iteration_bytes(p, n, F, B, quant, K=1):
-    """Algorithmic bytes of the profiled iterations: histograms,
-    gradients, quantization, partitions, score updates and split searches."""
+    """Algorithmic bytes of the profiled iterations: histograms, gradients,
+    quantization, partitions, score updates and split searches, each at the
+    bytes per row or per bin the reference's algorithm must touch."""
     c_out = 8 if quant else 16
     its = max(p["iters"], 1)
     per_iter = (n * K * (8 + 1 + 8)                       # gradients
@@ -682,7 +683,7 @@ def main():
         "lds_atomics": lds,
         "iteration": dict(algorithmic_bytes_per_iter=it_bytes / max(p["iters"], 1), gbs=it_gbs,
                           frac=it_gbs / HBM_PEAK_GBS,
-                          note="SURVEY 8(d) bytes of the profiled iterations over the timed ms per iteration"),
+                          note="algorithmic bytes of the profiled iterations over the timed ms per iteration"),
         "cpu_baseline": cpu,
     }
     # quantized against plain at each max_bin: the timed steps, the full runs
