@@ -520,32 +520,51 @@ __device__ __forceinline__ void lambdarank_query(const RankArgs& a, RankLds& R, 
     for (int i = tid; i < cnt; i += kRankThreads) R.disc[i] = a.discount[i];
   // ---- stable descending order: rank = #{j : s_j > s_i or (s_j == s_i and
   // j < i)}, i.e. j < i counts when !(s_j < s_i), j > i when s_j > s_i. The
-  // scores are compared from LDS, kRankStage at a time; tpd threads (a power
-  // of two, adjacent lanes) share a document, each counting every tpd-th j.
-  int tpd = 1;
-  while (tpd < 64 && cnt * tpd * 2 <= kRankThreads) tpd *= 2;
-  const int part = tid & (tpd - 1), per = kRankThreads / tpd;
-  for (int i0 = 0; i0 < cnt; i0 += per) {
+  // scores are compared from LDS, kRankStage at a time. Per pass, tpd threads
+  // (a power of two, adjacent lanes) share a document, each counting every
+  // tpd-th j; batches of j wholly before or after all of a wave's documents
+  // take the one comparison that applies there.
+  for (int i0 = 0; i0 < cnt;) {
+    int tpd = 1;
+    while (tpd < 64 && (cnt - i0) * tpd * 2 <= kRankThreads) tpd *= 2;
+    const int part = tid & (tpd - 1), step = 8 * tpd;
     const int i = i0 + tid / tpd;
     const double si = s[i < cnt ? i : cnt - 1];
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int iw_lo = i0 + wv * 64 / tpd, iw_hi = i0 + (wv * 64 + 63) / tpd;
     int r = 0;
     for (int j0 = 0; j0 < cnt; j0 += kRankStage) {
       const int jn = min(cnt, j0 + kRankStage);
       __syncthreads();
       for (int j = j0 + tid; j < jn; j += kRankThreads) R.st_p[j - j0] = s[j];
       __syncthreads();
-      int j = j0 + part;
-      for (; j + 7 * tpd < jn; j += 8 * tpd) {
+      const double* st = R.st_p - j0 + part;
+      int jb = j0;
+      for (; jb + step <= min(jn, iw_lo); jb += step) {   // every j < i
         double s8[8];
 #pragma unroll
-        for (int v = 0; v < 8; ++v) s8[v] = R.st_p[j - j0 + v * tpd];
+        for (int v = 0; v < 8; ++v) s8[v] = st[jb + v * tpd];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) r += !(s8[v] < si);
+      }
+      for (; jb + step <= jn && jb <= iw_hi; jb += step) {
+        double s8[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) s8[v] = st[jb + v * tpd];
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
-          const int jj = j + v * tpd;
+          const int jj = jb + part + v * tpd;
           r += jj < i ? !(s8[v] < si) : (s8[v] > si);
         }
       }
-      for (; j < jn; j += tpd) {
+      for (; jb + step <= jn; jb += step) {   // every j > i
+        double s8[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) s8[v] = st[jb + v * tpd];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) r += s8[v] > si;
+      }
+      for (int j = jb + part; j < jn; j += tpd) {
         const double sj = R.st_p[j - j0];
         r += j < i ? !(sj < si) : (sj > si);
       }
@@ -558,6 +577,7 @@ __device__ __forceinline__ void lambdarank_query(const RankArgs& a, RankLds& R, 
       lam[r] = 0.0f;
       hes[r] = 0.0f;
     }
+    i0 += kRankThreads / tpd;
   }
   if (tid == 0) R.S = 0.0;
   __syncthreads();
