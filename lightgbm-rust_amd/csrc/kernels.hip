@@ -446,27 +446,22 @@ __device__ __forceinline__ double rank_sigmoid(const RankArgs& a, double s) {
   return a.table[min(ix, 1024u * 1024u - 1u)];
 }
 
-// One pair's contribution (rank.rs:213-247): anchor at rank i (score sa,
-// label la), partner at rank j (sb, lb), labels differ. fl / fh: the f32
-// lambda and hessian terms, pl the f64 lambda before its cast (the
-// normalizer's chain subtracts 2 pl).
-// The score gap of a pair, higher label's score first.
-__device__ __forceinline__ double pair_ds(double sa, float la, double sb, float lb) {
-  const bool anchor_high = la > lb;
-  return (anchor_high ? sa : sb) - (anchor_high ? sb : sa);
-}
-// sig: rank_sigmoid of the pair's gap; disc: the discount table by rank.
-__device__ __forceinline__ void rank_pair(const RankArgs& a, const double* sgain, const double* disc, double imd,
-                                          bool scale_by_delta, int i, int j, double sa, float la, double sb, float lb,
-                                          double sig, float& fl, float& fh, double& plo) {
-  const bool anchor_high = la > lb;
-  const int hr = anchor_high ? i : j, lr = anchor_high ? j : i;
-  const double hg = sgain[(int)(anchor_high ? la : lb)], lg = sgain[(int)(anchor_high ? lb : la)];
-  const double ds = pair_ds(sa, la, sb, lb);
-  const double gap = hg - lg;
-  const double pd = fabs(disc[hr] - disc[lr]);
+// One pair's contribution (rank.rs:213-247), labels differ, from the
+// anchor's and the partner's values: d = s_anchor - s_partner, their label
+// gains ga, gb and rank discounts da, db. The reference takes high minus low
+// throughout; with the anchor low that negates d and ga - gb, exactly (IEEE
+// subtraction is sign-symmetric), and leaves |da - db| unchanged, so these
+// terms are the reference's bit for bit. sig: the sigmoid of the gap. fl / fh:
+// the f32 lambda and hessian terms, plo the f64 lambda before its cast (the
+// normalizer's chain subtracts 2 plo).
+__device__ __forceinline__ void rank_pair(const RankArgs& a, double imd, bool scale_by_delta, bool anchor_high,
+                                          double d, double ga, double gb, double da, double db, double sig,
+                                          float& fl, float& fh, double& plo) {
+  const double gd = ga - gb;
+  const double gap = anchor_high ? gd : -gd;
+  const double pd = fabs(da - db);
   double dp = gap * pd * imd;
-  if (scale_by_delta) dp /= (double)0.01f + fabs(ds);
+  if (scale_by_delta) dp /= (double)0.01f + fabs(d);
   double pl = sig;
   double ph = pl * (1.0 - pl);
   pl *= -a.sigmoid * dp;
@@ -491,7 +486,7 @@ __device__ __forceinline__ void rank_pair(const RankArgs& a, const double* sgain
 // scratch and takes one anchor per step, its partners in chunks.
 constexpr int kRankLds = 512;
 constexpr int kRankStage = 1024;   // pair terms staged per step
-constexpr int kRankGroup = 8;      // anchors per step, at most
+constexpr int kRankGroup = 6;      // anchors per step, at most
 struct RankLds {
   double st_p[kRankStage + 32];   // + the zero padding the normalizer chain reads
   float st_l[kRankStage], st_h[kRankStage];
@@ -613,13 +608,17 @@ __device__ __forceinline__ void lambdarank_query(const RankArgs& a, RankLds& R, 
       for (int j = j0 + tid; j < j1; j += kRankThreads) {
         const double sb = rs[j];
         const float lb = rl[j];
-        // the sigmoid lookups first (a global table), all in flight together
+        const double gb = R.sgain[(int)lb], db = disc[j];
+        // the sigmoid lookups first (a global table), all in flight together;
+        // the gap high minus low is d with the anchor high, else -d
         double sig[kRankGroup];
 #pragma unroll
         for (int u = 0; u < kRankGroup; ++u) {
           sig[u] = 0.0;
-          if (j > g + u && sa[u] != -INFINITY && sb != -INFINITY && la[u] != lb)
-            sig[u] = rank_sigmoid(a, pair_ds(sa[u], la[u], sb, lb));
+          if (j > g + u && sa[u] != -INFINITY && sb != -INFINITY && la[u] != lb) {
+            const double d = sa[u] - sb;
+            sig[u] = rank_sigmoid(a, la[u] > lb ? d : -d);
+          }
         }
         float pl_ = lam[j], ph_ = hes[j];
 #pragma unroll
@@ -631,7 +630,8 @@ __device__ __forceinline__ void lambdarank_query(const RankArgs& a, RankLds& R, 
             const bool anchor_high = la[u] > lb;
             float fl, fh;
             double p;
-            rank_pair(a, R.sgain, disc, imd, scale_by_delta, i, j, sa[u], la[u], sb, lb, sig[u], fl, fh, p);
+            rank_pair(a, imd, scale_by_delta, anchor_high, sa[u] - sb, R.sgain[(int)la[u]], gb, disc[i], db, sig[u], fl,
+                      fh, p);
             // the partner gets its one contribution of anchor i
             if (anchor_high) pl_ -= fl;
             else pl_ += fl;
