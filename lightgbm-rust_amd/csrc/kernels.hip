@@ -485,12 +485,12 @@ __device__ __forceinline__ void rank_pair(const RankArgs& a, double imd, bool sc
 // step as its staged terms allow; a longer one keeps them in the global
 // scratch and takes one anchor per step, its partners in chunks.
 constexpr int kRankLds = 512;
-constexpr int kRankStage = 1024;   // pair terms staged per step
-constexpr int kRankGroup = 6;      // anchors per step, at most
+constexpr int kRankStage = 896;    // pair terms staged per step
+constexpr int kRankGroup = 4;      // anchors per step, at most (more spill)
 struct RankLds {
   double st_p[kRankStage + 32];   // + the zero padding the normalizer chain reads
   float st_l[kRankStage], st_h[kRankStage];
-  double rs[kRankLds], disc[kRankLds];
+  double rs[kRankLds];
   float rl[kRankLds], lam[kRankLds], hes[kRankLds];
   double sgain[31];
   double sfac, S;
@@ -507,12 +507,10 @@ __device__ __forceinline__ void lambdarank_query(const RankArgs& a, RankLds& R, 
   float* rl = small ? R.rl : a.by_rank_l + start;
   float* lam = small ? R.lam : a.lam + start;
   float* hes = small ? R.hes : a.hes + start;
-  const double* disc = small ? R.disc : a.discount;
+  const double* disc = a.discount;   // 80 KB, cache-resident
   int* rank = a.rank + start;
   if (tid < 31) R.sgain[tid] = a.label_gain[tid];
   const double imd = a.inv_max_dcg[q];
-  if (small)
-    for (int i = tid; i < cnt; i += kRankThreads) R.disc[i] = a.discount[i];
   // ---- stable descending order: rank = #{j : s_j > s_i or (s_j == s_i and
   // j < i)}, i.e. j < i counts when !(s_j < s_i), j > i when s_j > s_i. The
   // scores are compared from LDS, kRankStage at a time. Per pass, tpd threads
@@ -729,9 +727,9 @@ __device__ __forceinline__ void lambdarank_query(const RankArgs& a, RankLds& R, 
   }
 }
 
-// Five blocks per compute unit (96 registers, 31 KB of LDS each): the serial
-// chains of more queries interleave on each SIMD; six measured slower (spills).
-__global__ void __launch_bounds__(kRankThreads, 5) k_lambdarank(RankArgs a) {
+// Six blocks per compute unit (at most 80 registers, 25 KB of LDS each): the
+// serial chains of more queries interleave on each SIMD.
+__global__ void __launch_bounds__(kRankThreads, 6) k_lambdarank(RankArgs a) {
   if ((int)blockIdx.x >= a.Q) return;
   const int q = a.order[blockIdx.x];   // longest queries first
   const int start = a.qb[q], cnt = a.qb[q + 1] - start;
