@@ -726,10 +726,14 @@ def test_quantized_categorical_trees_and_prediction_match_oracle():
 
 @pytest.mark.parametrize("trunc", [30, 100])
 def test_lambdarank_gradients_every_query_size_class(trunc):
-    # queries of 1 to 2600 documents (one and several 1024-partner chunks,
-    # sizes at and around powers of two), about 1% of scores -inf, and anchors
-    # past the first 64 ranks (trunc 100)
-    sizes = np.array([1, 2, 3, 63, 64, 65, 130, 255, 256, 257, 700, 2047, 2048, 2049, 2600])
+    # queries of 1 to 2600 documents: sizes at and around powers of two, at
+    # the kernel's shape boundaries (224 / 225 and 448 / 449: four, three and
+    # two anchors per step in 896 staged terms; 512 / 513: arrays in LDS or in
+    # the global scratch; 896 / 897: one or two partner chunks); ties in the
+    # scores (two decimals), about 1% of scores -inf, and anchors past the
+    # first 64 ranks (trunc 100)
+    sizes = np.array([1, 2, 3, 63, 64, 65, 130, 224, 225, 255, 256, 257, 448, 449, 511, 512, 513, 700, 896, 897,
+                      2047, 2048, 2049, 2600])
     qb = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
     n = int(qb[-1])
     rng = np.random.default_rng(31 + trunc)
