@@ -149,6 +149,12 @@ lgbmhip_status lgbmhip_bin_dense(const float* X, int64_t n, int32_t ncol, const 
                                  const lgbmhip_bin_params* params, lgbmhip_bins** out);
 void lgbmhip_bins_free(lgbmhip_bins* b);
 int32_t lgbmhip_bins_num_features(const lgbmhip_bins* b);
+/* The number of feature groups the reference's exclusive feature bundling
+ * forms over the used features (replaces Dataset::num_feature_groups,
+ * crates/lgbm-core/src/dataset.rs:721-723, with the groups of
+ * feature_groups.rs:169-211). The engine stores every feature on its own;
+ * the count feeds the bagging subset-mode rule (sample_strategy.rs:198-201). */
+int32_t lgbmhip_bins_num_groups(const lgbmhip_bins* b);
 /* metas: num_features entries; upper: per feature num_bin values concatenated (bin upper bounds, or the category of each bin). */
 lgbmhip_status lgbmhip_bins_metas(const lgbmhip_bins* b, lgbmhip_feature_meta* metas, double* upper);
 int64_t lgbmhip_bins_num_upper(const lgbmhip_bins* b);
@@ -173,6 +179,13 @@ void lgbmhip_destroy(lgbmhip_ctx* ctx);
  * reference's storage choice BinKind::dense (bin.rs:314-324), whose Dense4
  * (bin.rs:92-121) packs the same features. Trees do not depend on it. */
 lgbmhip_status lgbmhip_set_bin_packing(lgbmhip_ctx* ctx, int32_t u4);
+/* The reference's feature-group count of the uploaded data, for callers of
+ * lgbmhip_upload_bins (lgbmhip_upload_raw takes it from its mappers): the
+ * value of lgbmhip_bins_num_groups, or the number of used features when the
+ * reference would not bundle (enable_bundle false); 0 resets to the latter.
+ * Replaces the count Dataset::num_feature_groups (dataset.rs:721-723) hands
+ * the bagging subset-mode rule (sample_strategy.rs:198-201). */
+lgbmhip_status lgbmhip_set_feature_groups(lgbmhip_ctx* ctx, int32_t num_groups);
 /* The uploaded rows: out3 = {bytes per packed row, row stride in bytes,
  * features stored as 4-bit fields}. No counterpart in the reference
  * (Dataset's storage sizes): for reports and tests. */

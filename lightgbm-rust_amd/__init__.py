@@ -34,7 +34,7 @@ DEFAULT_PARAMS = dict(
     first_metric_only=False, early_stopping_min_delta=0.0, device_binning=False, bagging_fraction=1.0,
     bagging_freq=0, bagging_seed=3, pos_bagging_fraction=1.0, neg_bagging_fraction=1.0,
     data_sample_strategy="bagging", top_rate=0.2, other_rate=0.1, autotune=False, autotune_trees=4, bagging_by_query=False,
-    batch_trees=True, pack_u4=True,
+    batch_trees=True, pack_u4=True, enable_bundle=True,
 )
 
 ALIASES = {
@@ -153,6 +153,9 @@ class Dataset:
                                            ptr(self._bins)))
             self.metas, self.upper = reference.metas, reference.upper
             self.num_features = reference.num_features
+            # a dataset binned with another's mappers keeps one group per
+            # feature (dataset.rs:628-635, Layout::Valid)
+            self.num_feature_groups = self.num_features
             return
         p = self.params
         bp = _capi.BinParams(int(p["max_bin"]), int(p["min_data_in_bin"]), int(p["min_data_in_leaf"]),
@@ -170,6 +173,10 @@ class Dataset:
         self._handle = h
         self._owns = True
         self.num_features = lib().lgbmhip_bins_num_features(h)
+        # the reference's feature-group count (exclusive feature bundling,
+        # feature_groups.rs:169-211); one group per feature without bundling
+        self.num_feature_groups = (lib().lgbmhip_bins_num_groups(h) if _b(p["enable_bundle"])
+                                   else self.num_features)
         self.metas = (_capi.FeatureMeta * max(self.num_features, 1))()
         self.upper = np.zeros(max(lib().lgbmhip_bins_num_upper(h), 1), np.float64)
         check(lib().lgbmhip_bins_metas(h, C.cast(self.metas, C.c_void_p), ptr(self.upper)))
@@ -265,6 +272,8 @@ class Booster:
         else:
             check(lib().lgbmhip_upload_bins(h, C.cast(ds.metas, C.c_void_p), ds.num_features, ptr(ds.upper),
                                             ptr(ds.bins), ds.num_data, ds.num_total_features), h)
+        # the bagging subset-mode rule reads the reference's feature-group count
+        check(lib().lgbmhip_set_feature_groups(h, int(ds.num_feature_groups)), h)
         qb = None
         nq = 0
         if ds.group is not None:

@@ -79,6 +79,7 @@ SIGNATURES = {
     "lgbmhip_find_mappers": (I32, [P, I64, I32, P, I32, C.POINTER(BinParams), PP]),
     "lgbmhip_bins_free": (None, [P]),
     "lgbmhip_bins_num_features": (I32, [P]),
+    "lgbmhip_bins_num_groups": (I32, [P]),
     "lgbmhip_bins_metas": (I32, [P, P, P]),
     "lgbmhip_bins_num_upper": (I64, [P]),
     "lgbmhip_bins_matrix": (I32, [P, P]),
@@ -130,6 +131,7 @@ SIGNATURES = {
     "lgbmhip_autotune": (I32, [P, I32, P]),
     "lgbmhip_get_tuning": (I32, [P, P]),
     "lgbmhip_set_bin_packing": (I32, [P, I32]),
+    "lgbmhip_set_feature_groups": (I32, [P, I32]),
     "lgbmhip_bin_layout": (I32, [P, P]),
 }
 

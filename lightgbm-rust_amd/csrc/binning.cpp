@@ -319,6 +319,103 @@ uint32_t Mapper::to_bin(double value) const {
   return (uint32_t)lo;
 }
 
+// The number of feature groups the reference's exclusive feature bundling
+// forms over the used features (feature_groups.rs:20-211, upstream_groups
+// with enable_bundle and is_enable_sparse on). This engine stores every
+// feature on its own; the reference's bagging reads this count to choose its
+// subset mode (sample_strategy.rs:198-201). nz[f]: the sample positions at
+// which used feature f holds a kept value (|v| > kZeroThreshold or NaN),
+// ascending, with the values in vals[f].
+namespace {
+
+// feature_groups.rs:21-42: a feature whose zero bin is not its most frequent
+// bin occupies every sample position outside its most frequent bin
+std::vector<int> occupied(const Mapper& m, const std::vector<int>& nz, const std::vector<double>& vals, int total) {
+  if (m.default_bin == m.most_freq_bin) return nz;
+  std::vector<int> out;
+  size_t j = 0;
+  for (int i = 0; i < total; ++i) {
+    const bool kept = j < nz.size() && nz[j] == i;
+    if (!kept || m.to_bin(vals[j]) != m.most_freq_bin) out.push_back(i);
+    if (kept) ++j;
+  }
+  return out.empty() ? nz : out;
+}
+
+// feature_groups.rs:62-158 find_groups: groups after the greedy pass and the
+// merge of the sparse ones into one trailing group
+int greedy_group_count(const std::vector<int>& order, const std::vector<std::vector<int>>& occ, int64_t total,
+                       int64_t n) {
+  const int64_t max_conflict = total / 10000;
+  Lcg rng((int)n);
+  std::vector<std::vector<char>> mark;
+  std::vector<int64_t> data_cnt, used_cnt;
+  for (int f : order) {
+    const std::vector<int>& ix = occ[f];
+    const int64_t cur = (int64_t)ix.size();
+    std::vector<int> avail;
+    for (int g = 0; g < (int)mark.size(); ++g)
+      if (data_cnt[g] + cur <= total + max_conflict) avail.push_back(g);
+    std::vector<int> search;
+    if (!avail.empty()) {
+      const int last = (int)avail.size() - 1;
+      search.push_back(avail.back());
+      for (int k : rng.sample(last, std::min(last, 99))) search.push_back(avail[k]);
+    }
+    int pick = -1;
+    int64_t pick_cnt = 0;
+    for (int g : search) {
+      const int64_t room = max_conflict - data_cnt[g] + used_cnt[g];
+      int64_t cnt = 0;
+      bool over = false;
+      for (int i : ix) {
+        cnt += mark[g][i];
+        if (cnt > room) {
+          over = true;
+          break;
+        }
+      }
+      if (!over && cnt <= room && cnt <= cur / 2) {
+        pick = g;
+        pick_cnt = cnt;
+        break;
+      }
+    }
+    if (pick < 0) {
+      pick = (int)mark.size();
+      mark.emplace_back((size_t)total, 0);
+      data_cnt.push_back(0);
+      used_cnt.push_back(0);
+      pick_cnt = 0;
+    }
+    data_cnt[pick] += cur;
+    used_cnt[pick] += cur - pick_cnt;
+    for (int i : ix) mark[pick][i] = 1;
+  }
+  int dense = 0;
+  bool sparse = false;
+  for (size_t g = 0; g < mark.size(); ++g) {
+    if ((double)used_cnt[g] / (double)total >= 0.4) ++dense;
+    else sparse = true;
+  }
+  return dense + (sparse ? 1 : 0);
+}
+
+int reference_group_count(const Binned& b, const std::vector<std::vector<int>>& nz,
+                          const std::vector<std::vector<double>>& vals, int64_t total) {
+  const int F = (int)b.used.size();
+  if (F == 0 || total == 0) return F;
+  std::vector<std::vector<int>> occ(F);
+  for (int f = 0; f < F; ++f) occ[f] = occupied(b.mappers[b.used[f]], nz[f], vals[f], (int)total);
+  std::vector<int> by_col(F), by_cnt(F);
+  for (int f = 0; f < F; ++f) by_col[f] = by_cnt[f] = f;
+  // feature_groups.rs:181-184: the densest columns first (by kept values), stable
+  std::stable_sort(by_cnt.begin(), by_cnt.end(), [&](int x, int y) { return nz[x].size() > nz[y].size(); });
+  return std::min(greedy_group_count(by_col, occ, total, b.n), greedy_group_count(by_cnt, occ, total, b.n));
+}
+
+}  // namespace
+
 Binned* find_mappers(const float* X, int64_t n, int ncol, const int32_t* cats, int n_cat, const lgbmhip_bin_params& p) {
   if (n <= 0 || ncol <= 0) throw Error(LGBMHIP_INVALID_DATA, "dataset has no rows or no feature columns");
   if (n > INT32_MAX) throw Error(LGBMHIP_INVALID_DATA, "more than 2^31-1 rows");
@@ -347,6 +444,21 @@ Binned* find_mappers(const float* X, int64_t n, int ncol, const int32_t* cats, i
       throw Error(LGBMHIP_UNSUPPORTED, "a feature has more than 256 bins (u8 bin storage)");
     out->used.push_back(c);
   }
+  // the sampled kept values of the used features (matrix.rs:198-214)
+  const int F = (int)out->used.size();
+  std::vector<std::vector<int>> nz(F);
+  std::vector<std::vector<double>> vals(F);
+  parallel_for(F, p.num_threads, [&](int64_t f) {
+    const int c = out->used[f];
+    for (int64_t i = 0; i < total; ++i) {
+      const double v = (double)X[(size_t)sample[i] * ncol + c];
+      if (std::fabs(v) > kZero || std::isnan(v)) {
+        nz[f].push_back((int)i);
+        vals[f].push_back(v);
+      }
+    }
+  });
+  out->num_groups = reference_group_count(*out, nz, vals, total);
   return out.release();
 }
 

@@ -41,6 +41,7 @@ struct Binned {
   std::vector<Mapper> mappers;
   std::vector<int> used;
   std::vector<uint8_t> bins;
+  int num_groups = 0;   // the reference's feature groups (exclusive feature bundling) over `used`
 };
 
 Binned* bin_dense(const float* X, int64_t n, int ncol, const int32_t* cats, int n_cat, const lgbmhip_bin_params& p);

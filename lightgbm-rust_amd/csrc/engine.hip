@@ -319,6 +319,7 @@ struct Ctx {
   // units (no slab reduction launch); 0: never; -1 in tune_late_round: the
   // default
   int late_round = 12, tune_late_round = -1;
+  int num_groups = 0;   // the reference's feature-group count (0: one group per used feature)
   int hist_grid = 256;   // blocks of a histogram launch (they loop over the units)
   size_t hist_lds = 0;
   int part_grid = 1024;
@@ -1075,7 +1076,9 @@ static void setup_training(Ctx* c) {
       c->bag_cnt = c->bag_balanced ? (int64_t)((double)num_pos * o.pos_bagging_fraction) +
                                          (int64_t)((double)(N - num_pos) * o.neg_bagging_fraction)
                                    : (int64_t)(o.bagging_fraction * (double)N);
-      c->bag_subset = ((double)c->bag_cnt / (double)N) / (double)o.bagging_freq <= 0.5 && F < 100;
+      // upstream's subset mode counts feature groups, not features (sample_strategy.rs:198-201)
+      const int groups = c->num_groups > 0 ? c->num_groups : F;
+      c->bag_subset = ((double)c->bag_cnt / (double)N) / (double)o.bagging_freq <= 0.5 && groups < 100;
 
       c->need_re_bagging = true;
       c->bag_by_query = o.bagging_by_query != 0;
@@ -3091,6 +3094,7 @@ lgbmhip_status lgbmhip_find_mappers(const float* X, int64_t n, int32_t ncol, con
 }
 void lgbmhip_bins_free(lgbmhip_bins* b) { delete b; }
 int32_t lgbmhip_bins_num_features(const lgbmhip_bins* b) { return b ? (int32_t)b->b->used.size() : 0; }
+int32_t lgbmhip_bins_num_groups(const lgbmhip_bins* b) { return b ? (int32_t)b->b->num_groups : 0; }
 int64_t lgbmhip_bins_num_upper(const lgbmhip_bins* b) {
   int64_t s = 0;
   for (int c : b->b->used) s += b->b->mappers[c].num_bin;
@@ -3597,6 +3601,7 @@ lgbmhip_status lgbmhip_upload_raw(lgbmhip_ctx* ctx, const lgbmhip_bins* mappers,
   return guarded(ctx, [&] {
     if (!mappers || !X) throw Error(LGBMHIP_INVALID_PARAMETER, "mappers and rows are required");
     upload_raw(&ctx->c, *mappers->b, X, rows, ncol, x_on_device != 0);
+    ctx->c.num_groups = mappers->b->num_groups;
   });
 }
 lgbmhip_status lgbmhip_bin_rows_device(lgbmhip_ctx* ctx, const float* d_X, int64_t rows, int32_t ncol,
@@ -3778,6 +3783,12 @@ lgbmhip_status lgbmhip_autotune(lgbmhip_ctx* ctx, int32_t trees_per_candidate, i
     const auto r = autotune(&ctx->c, trees_per_candidate);
     if (out5)
       for (int i = 0; i < 5; ++i) out5[i] = r[(size_t)i];
+  });
+}
+lgbmhip_status lgbmhip_set_feature_groups(lgbmhip_ctx* ctx, int32_t num_groups) {
+  return guarded(ctx, [&] {
+    if (num_groups < 0) throw Error(LGBMHIP_INVALID_PARAMETER, "the feature-group count must be non-negative");
+    ctx->c.num_groups = num_groups;
   });
 }
 lgbmhip_status lgbmhip_set_bin_packing(lgbmhip_ctx* ctx, int32_t u4) {

@@ -23,6 +23,7 @@
 //   metric_auc (weighted)      : sklearn roc_auc_score(sample_weight) + brute force over pairs
 //   metric_point               : sklearn log_loss / accuracy (binary and multiclass, weighted)
 //   metric_ndcg                : dcg.rs:195-209 known answers + sklearn ndcg_score (gains 2^rel-1)
+//   feature-group count        : cases the greedy bundling fixes (dense, exclusive one-hot, sparse merge)
 //   full-pipeline model text   : parity unpinned (no committed artifact in the reference)
 
 #include <algorithm>
@@ -430,6 +431,7 @@ struct Dataset {
   bool wide = false;
   std::vector<float> label;
   std::vector<int> query_boundaries;
+  int num_groups = 0;               // upstream's feature groups (bundling), for the bagging subset rule
   uint32_t bin(int64_t row, int f) const {
     size_t F = used.size();
     return wide ? bins16[(size_t)row * F + f] : bins[(size_t)row * F + f];
@@ -437,6 +439,7 @@ struct Dataset {
 };
 
 struct Params {
+  bool enable_bundle = true;   // dataset.rs:835-853 (explicit enable_bundle, default true)
   int max_bin = 255, min_data_in_bin = 3, bin_construct_sample_cnt = 200000, data_random_seed = 1;
   int min_data_in_leaf = 20;
   double min_sum_hessian_in_leaf = 1e-3;
@@ -487,6 +490,7 @@ static bool parse_params(const char* s, Params& p) {
       else if (k == "min_data_in_leaf") p.min_data_in_leaf = atoi(v.c_str());
       else if (k == "min_sum_hessian_in_leaf") p.min_sum_hessian_in_leaf = atof(v.c_str());
       else if (k == "feature_pre_filter") p.feature_pre_filter = v == "true" || v == "1";
+      else if (k == "enable_bundle") p.enable_bundle = v == "true" || v == "1";
       else if (k == "use_missing") p.use_missing = v == "true" || v == "1";
       else if (k == "zero_as_missing") p.zero_as_missing = v == "true" || v == "1";
       else if (k == "lambda_l1") p.lambda_l1 = atof(v.c_str());
@@ -551,10 +555,105 @@ static bool parse_params(const char* s, Params& p) {
   return true;
 }
 
+// Upstream's feature-group count (feature_groups.rs:169-211 upstream_groups,
+// is_enable_sparse on): FindGroups in two feature orders, the fewer groups
+// kept. Only the count is used (the bagging subset rule); the bins stay per
+// feature. cols[f]: the sample positions of used feature f's kept values
+// (matrix.rs:198-214), ascending; vals[f] those values.
+static int bundle_group_count(const Dataset* ds, const std::vector<std::vector<int>>& cols,
+                              const std::vector<std::vector<double>>& vals, int64_t total) {
+  const int F = (int)ds->used.size();
+  if (F == 0 || total == 0) return F;
+  // FixSampleIndices (feature_groups.rs:21-42)
+  std::vector<std::vector<int>> si(F);
+  for (int f = 0; f < F; ++f) {
+    const BinMapper& m = ds->mappers[ds->used[f]];
+    std::vector<int> fixed;
+    if (m.default_bin != m.most_freq_bin) {
+      int j = 0;
+      const int nv = (int)cols[f].size();
+      for (int i = 0; i < (int)total;) {
+        if (j < nv && cols[f][j] < i) {
+          ++j;
+        } else if (j < nv && cols[f][j] == i) {
+          if (m.value_to_bin(vals[f][j]) != m.most_freq_bin) fixed.push_back(i);
+          ++i;
+        } else {
+          fixed.push_back(i);
+          ++i;
+        }
+      }
+    }
+    si[f] = fixed.empty() ? cols[f] : fixed;
+  }
+  // FindGroups (feature_groups.rs:62-158), counting the groups
+  auto find_groups = [&](const std::vector<int>& order) {
+    const int64_t max_cnt = total / 10000;
+    Random rand((int)ds->n);
+    std::vector<std::vector<bool>> marks;
+    std::vector<int64_t> total_cnt, used_cnt;
+    for (int fidx : order) {
+      const std::vector<int>& idx = si[fidx];
+      const int64_t cur = (int64_t)idx.size();
+      std::vector<int> available;
+      for (int g = 0; g < (int)marks.size(); ++g)
+        if (total_cnt[g] + cur <= total + max_cnt) available.push_back(g);
+      std::vector<int> search;
+      if (!available.empty()) {
+        const int last = (int)available.size() - 1;
+        std::vector<int> picked = rand.sample(last, std::min(last, 100 - 1));
+        search.push_back(available.back());
+        for (int k : picked) search.push_back(available[k]);
+      }
+      int best = -1;
+      int64_t best_cnt = 0;
+      for (int g : search) {
+        const int64_t rest = max_cnt - total_cnt[g] + used_cnt[g];
+        int64_t cnt = 0;   // GetConflictCount: -1 once past rest
+        for (int i : idx) {
+          if (marks[g][i]) ++cnt;
+          if (cnt > rest) {
+            cnt = -1;
+            break;
+          }
+        }
+        if (cnt >= 0 && cnt <= rest && cnt <= cur / 2) {
+          best = g;
+          best_cnt = cnt;
+          break;
+        }
+      }
+      if (best >= 0) {
+        total_cnt[best] += cur;
+        used_cnt[best] += cur - best_cnt;
+        for (int i : idx) marks[best][i] = true;
+      } else {
+        marks.emplace_back((size_t)total, false);
+        for (int i : idx) marks.back()[i] = true;
+        total_cnt.push_back(cur);
+        used_cnt.push_back(cur);
+      }
+    }
+    // the second round: groups under the dense threshold merge into one
+    int kept = 0, sparse = 0;
+    for (size_t g = 0; g < marks.size(); ++g) {
+      if ((double)used_cnt[g] / (double)total >= 0.4) ++kept;
+      else ++sparse;
+    }
+    return kept + (sparse > 0 ? 1 : 0);
+  };
+  std::vector<int> in_order(F), by_cnt(F);
+  for (int f = 0; f < F; ++f) in_order[f] = by_cnt[f] = f;
+  std::stable_sort(by_cnt.begin(), by_cnt.end(),
+                   [&](int a, int b) { return cols[b].size() < cols[a].size(); });
+  const int g1 = find_groups(in_order), g2 = find_groups(by_cnt);
+  return g1 > g2 ? g2 : g1;
+}
+
 // dataset.rs:364-422 with the dense-matrix sampling of matrix.rs:198-214.
-// Exclusive feature bundling is not restated: every used feature keeps its
-// own bins (value_to_bin of its value), which equals the reference's decoded
-// bins whenever no two bundled features conflict in a row.
+// Exclusive feature bundling changes only the storage of the bins: every
+// used feature keeps its own bins (value_to_bin of its value), and the
+// number of groups it would form is restated above for the bagging rule.
 static Dataset* build_dataset(const float* X, int64_t n, int ncol, const Params& p) {
   Dataset* ds = new Dataset();
   ds->n = n;
@@ -584,6 +683,22 @@ static Dataset* build_dataset(const float* X, int64_t n, int ncol, const Params&
     }
   }
   size_t F = ds->used.size();
+  if (p.enable_bundle) {
+    std::vector<std::vector<int>> cols(F);
+    std::vector<std::vector<double>> cvals(F);
+    for (size_t f = 0; f < F; ++f) {
+      for (size_t i = 0; i < total; ++i) {
+        double v = (double)X[(size_t)si[i] * ncol + ds->used[f]];
+        if (std::fabs(v) > K_ZERO_THRESHOLD || std::isnan(v)) {
+          cols[f].push_back((int)i);
+          cvals[f].push_back(v);
+        }
+      }
+    }
+    ds->num_groups = bundle_group_count(ds, cols, cvals, (int64_t)total);
+  } else {
+    ds->num_groups = (int)F;
+  }
   if (ds->wide) ds->bins16.resize((size_t)n * F); else ds->bins.resize((size_t)n * F);
   for (int64_t r = 0; r < n; ++r) {
     for (size_t f = 0; f < F; ++f) {
@@ -2264,6 +2379,7 @@ void* orc_dataset_create(const float* X, int64_t n, int ncol, const float* label
 }
 void orc_dataset_free(void* d) { delete (Dataset*)d; }
 int orc_dataset_num_features(void* d) { return (int)((Dataset*)d)->used.size(); }
+int orc_dataset_num_groups(void* d) { return ((Dataset*)d)->num_groups; }
 // Bin mapper of inner feature f: returns num_bin; fills the scalar fields and,
 // when `upper` is given, num_bin upper bounds (categories for categorical).
 int orc_dataset_feature(void* d, int f, int* real, int* missing_type, int* default_bin, int* mfb, int* bin_type,
@@ -2306,7 +2422,7 @@ void* orc_booster_create(void* d, const char* params) {
   int64_t num_pos = 0;
   if (p.objective == 0)
     for (float l : b->ds->label) num_pos += l > 0.0f ? 1 : 0;
-  b->bagger.setup(p, b->ds->n, (int)b->ds->used.size(), num_pos, b->ds->label.data());
+  b->bagger.setup(p, b->ds->n, b->ds->num_groups, num_pos, b->ds->label.data());
   if (p.bagging_by_query && b->bagger.on && !b->bagger.goss) {
     // sample_strategy.rs:239-276: whole queries are drawn (no query data: no queries)
     b->bagger.by_query = true;

@@ -41,6 +41,7 @@ def lib():
             "orc_dataset_create": (P, [PF, I64, I, PF, C.c_char_p]),
             "orc_dataset_free": (None, [P]),
             "orc_dataset_num_features": (I, [P]),
+            "orc_dataset_num_groups": (I, [P]),
             "orc_dataset_feature": (I, [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I), PD]),
             "orc_dataset_bins": (None, [P, PU16]),
             "orc_booster_create": (P, [P, C.c_char_p]),
@@ -123,6 +124,7 @@ class Dataset:
         if not self.h:
             raise ValueError("oracle rejected the parameters")
         self.num_features = lib().orc_dataset_num_features(self.h)
+        self.num_groups = lib().orc_dataset_num_groups(self.h)
         self.features = []
         for f in range(self.num_features):
             r, mt, db, mfb, bt = I(), I(), I(), I(), I()

@@ -124,3 +124,44 @@ def test_feature_infos_are_sample_min_max_and_categories():
     # the categories of the bins in bin order, bin 0 holding -1 (binning.rs:704-779 order [-1, 3, 1, 0, 2])
     cats = [int(c) for c in infos[3].split(":")]
     assert cats[0] == -1 and sorted(cats[1:]) == [0, 1, 2, 3]
+
+
+def _groups(X, **params):
+    params = dict(dict(objective="binary"), **params)
+    ds = lgb.Dataset(X, label=np.zeros(len(X), np.float32), params=params)
+    od = O.Dataset(X, np.zeros(len(X), np.float32), params)
+    return ds.num_features, ds.num_feature_groups, od.num_groups
+
+
+def test_feature_group_count_cases_the_greedy_bundling_fixes():
+    # feature_groups.rs:62-158: a feature joins a group only when its sampled
+    # non-zero positions conflict in at most total/10000 of them; groups used
+    # by under 40% of the sample merge into one trailing group.
+    rng = np.random.default_rng(5)
+    n = 30000
+    dense = rng.normal(size=(n, 12)).astype(np.float32)
+    # dense columns conflict everywhere: one group each
+    assert _groups(dense) == (12, 12, 12)
+    # mutually exclusive one-hot columns share one group (it covers every row)
+    onehot = np.zeros((n, 9), np.float32)
+    onehot[np.arange(n), rng.integers(0, 9, n)] = 1.0
+    assert _groups(onehot) == (9, 1, 1)
+    assert _groups(np.hstack([dense, onehot])) == (21, 13, 13)
+    # independent 30%-non-zero columns conflict in ~9% of rows: each its own
+    # group in the first pass, all under 40% used, merged into one
+    sparse = (rng.normal(size=(n, 10)) * (rng.random((n, 10)) < 0.3)).astype(np.float32)
+    assert _groups(np.hstack([dense, sparse])) == (22, 13, 13)
+    # without bundling every feature is a group
+    assert _groups(np.hstack([dense, sparse]), enable_bundle=False) == (22, 22, 22)
+
+
+def test_feature_group_count_decides_the_bagging_subset_rule():
+    # 60 dense + 60 sparse features: 120 features but 61 groups, so a bag of
+    # at most half the rows takes the reference's subset mode (num_feature_groups
+    # < 100, sample_strategy.rs:198-201); the count reaches the engine unchanged
+    rng = np.random.default_rng(6)
+    n = 20000
+    X = np.hstack([rng.normal(size=(n, 60)),
+                   rng.normal(size=(n, 60)) * (rng.random((n, 60)) < 0.3)]).astype(np.float32)
+    nf, ng, og = _groups(X)
+    assert (nf, ng, og) == (120, 61, 61)

@@ -168,3 +168,32 @@ def test_bagging_by_query_matches_oracle(quant):
                           sizes[np.unique(q_of_row)])   # every drawn query whole
     _compare(b, ob, 4)
     np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("bundle", [True, False])
+def test_bagging_subset_rule_counts_feature_groups(bundle):
+    # 60 dense + 60 30%-non-zero features: 120 features in 61 of the
+    # reference's feature groups. With bundling the bag of 40% of the rows
+    # takes the subset mode (num_feature_groups < 100, sample_strategy.rs:
+    # 198-201): the quantizer draws over the bag's rows; without it (120
+    # groups) the full-data mode quantizes every row. Either way the trees are
+    # the oracle's, which follows the same rule.
+    rng = np.random.default_rng(8)
+    n = 30000
+    X = np.hstack([rng.normal(size=(n, 60)),
+                   rng.normal(size=(n, 60)) * (rng.random((n, 60)) < 0.3)]).astype(np.float32)
+    y = (X[:, 0] + X[:, 61] + 0.5 * rng.normal(size=n) > 0).astype(np.float32)
+    params = dict(P, bagging_fraction=0.4, bagging_freq=1, bagging_seed=5, use_quantized_grad=True,
+                  num_grad_quant_bins=16, quant_train_renew_leaf=True, enable_bundle=bundle)
+    print(f"bagging subset rule bundle={bundle}", flush=True)
+    ds = lgb.Dataset(X, label=y, params=params)
+    od = O.Dataset(X, y, params)
+    assert (ds.num_feature_groups, od.num_groups) == ((61, 61) if bundle else (120, 120))
+    b = lgb.Booster(params, ds)
+    ob = O.Booster(od, params)
+    iters = 4
+    for _ in range(iters):
+        b.update()
+        ob.update()
+    _compare(b, ob, iters)
+    np.testing.assert_allclose(b.train_scores(), ob.scores(), rtol=1e-9, atol=1e-12)
